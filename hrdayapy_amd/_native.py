@@ -1,0 +1,133 @@
+"""ctypes binding of libhrdaya_hip.so (C ABI declared in include/hrdaya_hip.h).
+
+The library is loaded lazily on first use.  There is no fallback: if the
+shared library is missing or cannot be loaded, every native entry point
+raises ``RuntimeError`` so that the GPU path can never silently run on some
+other implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
+_lib = None
+
+ABI_VERSION = 1
+
+c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
+
+
+class HpBricks(C.Structure):
+    _fields_ = [("n_bricks", c_i64), ("bx", c_i32), ("by", c_i32), ("bz", c_i32), ("pad_", c_i32),
+                ("nbr", c_vp), ("label", c_vp), ("D6", c_vp), ("inst", c_vp)]
+
+
+class HpMonoParams(C.Structure):
+    _fields_ = [("dt", c_f64), ("inv_cm", c_f64), ("dscale", c_f64), ("ki_base", c_f64),
+                ("nai_base", c_f64), ("vmin", c_f64), ("vmax", c_f64), ("at_thr", c_f64),
+                ("do_reaction", c_i32), ("n_inst", c_i32), ("inst_params", c_vp),
+                ("step_ctr", c_vp), ("group_amp", c_vp), ("at", c_vp), ("rate_ext", c_vp)]
+
+
+class HpCable(C.Structure):
+    _fields_ = [("n", c_i64), ("indptr", c_vp), ("indices", c_vp), ("label", c_vp), ("w", c_f64)]
+
+
+class HpPmj(C.Structure):
+    _fields_ = [("n_junctions", c_i64), ("n_entries", c_i64), ("pk", c_vp), ("ptr", c_vp),
+                ("patch", c_vp), ("n_targets", c_i64), ("tgt_slot", c_vp), ("tgt_ptr", c_vp),
+                ("tgt_entry", c_vp), ("c_pmj", c_f64), ("ai_p", c_f64), ("ai_m", c_f64),
+                ("cm", c_f64)]
+
+
+P = C.POINTER
+_SIGS = {
+    "hp_abi_version": (C.c_int, []),
+    "hp_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "hp_ttp06_step_f32": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
+                                    c_i64, c_f32, c_f32, c_f32, c_vp]),
+    "hp_ttp06_step_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
+                                    c_i64, c_f64, c_f64, c_f64, c_vp]),
+    "hp_myo_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_myo_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_stencil19_apply_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
+    "hp_stencil19_apply_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
+    "hp_cable_step_f32": (C.c_int, [P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_cable_step_f64": (C.c_int, [P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_pmj_couple_f32": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_vp]),
+    "hp_pmj_couple_f64": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_vp]),
+    "hp_stim_eval_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_stim_eval_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_step_advance_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_step_advance_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+}
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load():
+    """Load the shared library (once) and declare every signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise RuntimeError(
+            f"libhrdaya_hip.so not found at {_LIB_PATH}; build it with "
+            "`python -m hrdayapy_amd.build` (there is no non-HIP fallback)")
+    # torch's HIP runtime must be the one the library binds to
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(str(_LIB_PATH), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    v = lib.hp_abi_version()
+    if v != ABI_VERSION:
+        raise RuntimeError(f"libhrdaya_hip ABI {v} != expected {ABI_VERSION}; rebuild")
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(1024)
+    load().hp_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def call(name: str, *args) -> None:
+    """Call an entry point and raise RuntimeError on a non-zero return code."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name} failed ({rc}): {last_error()}")
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def ptr(t) -> int:
+    """Device pointer of a torch tensor, or None (-> NULL) for None."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def sfx(dtype) -> str:
+    import torch
+    if dtype == torch.float32:
+        return "f32"
+    if dtype == torch.float64:
+        return "f64"
+    raise ValueError(f"unsupported dtype {dtype}; the HIP kernels take float32 or float64")

@@ -1,0 +1,57 @@
+// Shared helpers for the hrdaya HIP library: error reporting, math traits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+namespace hp {
+
+// Last error message of the calling thread; read through hp_last_error().
+void set_error(const char* fmt, ...);
+
+// Checks the launch status of the most recent kernel on this thread.
+int check_launch(const char* what);
+
+inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Arithmetic traits.  The FP32 path uses the hardware transcendental and
+// reciprocal instructions (v_exp_f32 / v_log_f32 / v_rcp_f32 / v_sqrt_f32,
+// ~1 ulp); the FP64 path uses the correctly rounded library routines so that
+// it can be compared with the FP64 oracle at round-off level.
+// ---------------------------------------------------------------------------
+template <typename T> struct M;
+
+template <> struct M<float> {
+  static __device__ __forceinline__ float ex(float x) { return __expf(x); }
+  static __device__ __forceinline__ float lg(float x) { return __logf(x); }
+  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float mx(float a, float b) { return fmaxf(a, b); }
+  static __device__ __forceinline__ float mn(float a, float b) { return fminf(a, b); }
+  static __device__ __forceinline__ float ab(float a) { return fabsf(a); }
+};
+
+template <> struct M<double> {
+  static __device__ __forceinline__ double ex(double x) { return exp(x); }
+  static __device__ __forceinline__ double lg(double x) { return log(x); }
+  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+  static __device__ __forceinline__ double dv(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double mx(double a, double b) { return fmax(a, b); }
+  static __device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
+  static __device__ __forceinline__ double ab(double a) { return fabs(a); }
+};
+
+}  // namespace hp
+
+#define HP_CHECK(expr)                                                              \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      hp::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                    __LINE__);                                                      \
+      return -(int)_e;                                                              \
+    }                                                                               \
+  } while (0)

@@ -1,0 +1,467 @@
+// Monodomain time step on the brick layout: fused TTP06 reaction + 19-point
+// anisotropic diffusion + stimulus + clamp + activation detection, the
+// Purkinje cable step, and the PMJ exchange.
+//
+// Scheme (one call per 0.02 ms step, all terms evaluated at V^n):
+//   V^{n+1} = clamp(V^n - dt*I_ion(V^n, s^n) + (dt/Cm)*(I_stim + (L V^n)), -150, 80)
+//   s^{n+1} = TTP06 Rush-Larsen / forward-Euler update from (V^n, s^n)
+// followed by the exact PMJ exchange on V^{n+1}.  The reference instead
+// solves the diffusion part with Crank-Nicolson + BiCGSTAB
+// (purkinje_myocardium_anisotropic_solver.py:1787-1791); at dt = 0.02 ms the
+// explicit operator is stable (dt*|diag L|/Cm < 0.2 for the shipped
+// conductivities) and the two schemes agree to within the activation-time
+// and Vm tolerances that the tests check.
+//
+// Memory plan per workgroup (one brick of BX*BY*BZ voxels, one lane each):
+// the (BX+2)(BY+2)(BZ+2) halo tile of V, label and the six D components is
+// staged in LDS once; each lane then reads its 18 neighbours from LDS and
+// streams its own 18 state variables from HBM (coalesced SoA).
+#include "common.h"
+#include "hrdaya_hip.h"
+#include "ttp06_cell.h"
+
+namespace hp {
+
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of bricks so that halo re-reads stay in its own L2.
+  const int xcd = bid & 7, k = bid >> 3;
+  const int base = nblocks >> 3, rem = nblocks & 7;
+  return xcd * base + (xcd < rem ? xcd : rem) + k;
+}
+
+template <typename T, int BX, int BY, int BZ>
+struct Tile {
+  static constexpr int TX = BX + 2, TY = BY + 2, TZ = BZ + 2;
+  static constexpr int N = TX * TY * TZ;
+  static constexpr int SX = TY * TZ, SY = TZ;
+  static constexpr int BVOL = BX * BY * BZ;
+  T V[N];
+  T D[6][N];
+  uint8_t L[N];
+  int32_t nbr[27];
+};
+
+// Stage the halo tile of brick `b` into LDS.
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void stage_tile(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
+                                           const T* __restrict__ V, int64_t b, int64_t ld,
+                                           bool need_D) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  const int tid = threadIdx.x;
+  if (tid < 27) t.nbr[tid] = g.nbr[b * 27 + tid];
+  __syncthreads();
+  const T* __restrict__ D6 = (const T*)g.D6;
+  for (int i = tid; i < TT::N; i += TT::BVOL) {
+    const int tz = i % TT::TZ, ty = (i / TT::TZ) % TT::TY, tx = i / TT::SX;
+    const int gx = tx - 1, gy = ty - 1, gz = tz - 1;
+    const int dbx = gx < 0 ? -1 : (gx >= BX ? 1 : 0);
+    const int dby = gy < 0 ? -1 : (gy >= BY ? 1 : 0);
+    const int dbz = gz < 0 ? -1 : (gz >= BZ ? 1 : 0);
+    const int nout = (dbx != 0) + (dby != 0) + (dbz != 0);
+    uint8_t lab = 0xFF;
+    if (nout < 3) {  // the 19-point stencil never reads tile corners
+      const int nb = t.nbr[(dbx + 1) * 9 + (dby + 1) * 3 + (dbz + 1)];
+      if (nb >= 0) {
+        const int lx = gx - dbx * BX, ly = gy - dby * BY, lz = gz - dbz * BZ;
+        const int64_t slot = (int64_t)nb * TT::BVOL + (lx * BY + ly) * BZ + lz;
+        lab = g.label[slot];
+        if (lab != 0xFF) {
+          t.V[i] = V[slot];
+          if (need_D) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t.D[k][i] = D6[k * ld + slot];
+          }
+        }
+      }
+    }
+    t.L[i] = lab;
+  }
+  __syncthreads();
+}
+
+// Sum over the 18 neighbours of w_ab*(V_b - V_a)/h, in units of D (mS/mm)*mV.
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  const T Vc = t.V[c];
+  T acc = T(0);
+  // face neighbours: harmonic mean of the axis component (aniso.py:996-1004)
+  const int fo[3] = {TT::SX, TT::SY, 1};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T Da = t.D[a][c];
+#pragma unroll
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      const int nb = c + sgn * fo[a];
+      if (t.L[nb] != 0xFF) {
+        const T Db = t.D[a][nb];
+        const T s = Da + Db;
+        const T dh = T(2) * Da * Db / (s > T(1e-30) ? s : T(1e-30));
+        acc += dh * (t.V[nb] - Vc);
+      }
+    }
+  }
+  // edge-diagonal neighbours: +-0.5*mean(D_ij) (aniso.py:1006-1029); in
+  // units of D*mV/h this is +-0.25*(D_ij,a + D_ij,b)*(V_b - V_a)
+  const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T Da = t.D[3 + q][c];
+    const int o1 = po[q][0], o2 = po[q][1];
+    const int co[2] = {o1 + o2, -(o1 + o2)};
+    const int ct[2] = {o1 - o2, -(o1 - o2)};
+    T sum = T(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int nb = c + co[k];
+      if (t.L[nb] != 0xFF) sum += (Da + t.D[3 + q][nb]) * (t.V[nb] - Vc);
+      const int nm = c + ct[k];
+      if (t.L[nm] != 0xFF) sum -= (Da + t.D[3 + q][nm]) * (t.V[nm] - Vc);
+    }
+    acc += T(0.25) * sum;
+  }
+  return acc;
+}
+
+template <typename T, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
+    int64_t ld, hp_mono_params p) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  __shared__ TT t;
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  stage_tile<T, BX, BY, BZ>(t, g, V_in, b, ld, true);
+  const int tid = threadIdx.x;
+  const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
+  const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
+  const uint8_t lab = t.L[c];
+  if (lab == 0xFF) return;
+  const int64_t slot = b * TT::BVOL + tid;
+  const T Vc = t.V[c];
+  const T lv = T(p.dscale) * stencil19<T, BX, BY, BZ>(t, c);
+  const T dt = T(p.dt);
+  T dvdt = T(0);  // reaction part of dV/dt
+  if (p.do_reaction) {
+    const int ct = lab & 3;
+    T gks = T(kGKs[ct]), gto = T(kGto[ct]);
+    if (p.inst_params && g.inst) {
+      const T* ip = (const T*)p.inst_params + 4 * (int)g.inst[slot];
+      gks *= ip[0];
+      gto *= ip[1];
+    }
+    T s[N_STATE];
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+    constexpr bool exact = sizeof(T) == 8;
+    dvdt = -ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
+  }
+  if (p.rate_ext) dvdt += ((const T*)p.rate_ext)[slot];
+  const int grp = (lab >> 2) & 31;
+  const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
+  T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
+  Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
+  V_out[slot] = Vn;
+  if (p.at) {
+    const T thr = T(p.at_thr);
+    if (Vc < thr && Vn >= thr) {
+      T* at = (T*)p.at;
+      const T a = at[slot];
+      if (a != a) {
+        const int64_t step = *p.step_ctr;
+        at[slot] = (T)((double)(step - 1) * p.dt) + dt * (thr - Vc) / (Vn - Vc);
+      }
+    }
+  }
+}
+
+template <typename T, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
+                                                                 const T* __restrict__ V,
+                                                                 T* __restrict__ out, int64_t ld,
+                                                                 double dscale) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  __shared__ TT t;
+  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  stage_tile<T, BX, BY, BZ>(t, g, V, b, ld, true);
+  const int tid = threadIdx.x;
+  const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
+  const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
+  const int64_t slot = b * TT::BVOL + tid;
+  out[slot] = t.L[c] == 0xFF ? T(0) : T(dscale) * stencil19<T, BX, BY, BZ>(t, c);
+}
+
+template <typename T, template <typename, int, int, int> class K, typename... A>
+static int launch_bricks(const hp_bricks* g, const char* what, hipStream_t st, A... args) {
+  if (!g || g->n_bricks < 0 || g->n_bricks > 0x7fffffff) {
+    set_error("%s: bad brick grid", what);
+    return -1;
+  }
+  if (g->n_bricks == 0) return 0;
+  const dim3 grid((unsigned)g->n_bricks);
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  switch (key) {
+    case 444: hipLaunchKernelGGL((K<T, 4, 4, 4>::fn), grid, dim3(64), 0, st, *g, args...); break;
+    case 844: hipLaunchKernelGGL((K<T, 8, 4, 4>::fn), grid, dim3(128), 0, st, *g, args...); break;
+    case 884: hipLaunchKernelGGL((K<T, 8, 8, 4>::fn), grid, dim3(256), 0, st, *g, args...); break;
+    case 888: hipLaunchKernelGGL((K<T, 8, 8, 8>::fn), grid, dim3(512), 0, st, *g, args...); break;
+    default:
+      set_error("%s: unsupported brick shape %dx%dx%d", what, g->bx, g->by, g->bz);
+      return -1;
+  }
+  return check_launch(what);
+}
+
+template <typename T, int BX, int BY, int BZ>
+struct MyoK {
+  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ>;
+};
+template <typename T, int BX, int BY, int BZ>
+struct StencilK {
+  static constexpr auto fn = stencil19_kernel<T, BX, BY, BZ>;
+};
+
+// ---------------------------------------------------------------------------
+// Purkinje cable
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __restrict__ V_in,
+                                                         T* __restrict__ V_out,
+                                                         T* __restrict__ state, int64_t ld,
+                                                         hp_mono_params p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  const T Vc = V_in[i];
+  T acc = T(0);
+  for (int e = c.indptr[i]; e < c.indptr[i + 1]; ++e) acc += V_in[c.indices[e]] - Vc;
+  const T lv = T(p.dscale) * T(c.w) * acc;
+  const T dt = T(p.dt);
+  const uint8_t lab = c.label[i];
+  T dvdt = T(0);
+  if (p.do_reaction) {
+    const int ct = lab & 3;
+    T s[N_STATE];
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + i];
+    constexpr bool exact = sizeof(T) == 8;
+    dvdt = -ttp06_cell<T, exact>(Vc, s, dt, T(kGKs[ct]), T(kGto[ct]), ct != 2, T(p.ki_base),
+                                 T(p.nai_base));
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) state[k * ld + i] = s[k];
+  }
+  const int grp = (lab >> 2) & 31;
+  const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
+  T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
+  V_out[i] = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
+}
+
+// ---------------------------------------------------------------------------
+// PMJ exchange: exact integration of the junction conductances over dt.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __restrict__ V_myo,
+                                    T* __restrict__ scratch, double dt, double vmin,
+                                    double vmax) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= j.n_junctions) return;
+  const int e0 = j.ptr[q], e1 = j.ptr[q + 1];
+  const int n = e1 - e0;
+  const int p = j.pk[q];
+  const double Vp = (double)V_pk[p];
+  double mean = 0.0;
+  for (int e = e0; e < e1; ++e) mean += (double)V_myo[j.patch[e]];
+  mean /= n;
+  const double Cp = j.ai_p * j.cm, Cq = n * j.ai_m * j.cm;
+  const double vbar = (Cp * Vp + Cq * mean) / (Cp + Cq);
+  const double r1 = exp(-j.c_pmj * dt * (1.0 / Cp + 1.0 / Cq));
+  const double r2 = exp(-(j.c_pmj / n) * dt / (j.ai_m * j.cm));
+  const double vp_new = vbar + (Vp - vbar) * r1;
+  const double mean_new = vbar + (mean - vbar) * r1;
+  for (int e = e0; e < e1; ++e) {
+    const double vm = (double)V_myo[j.patch[e]];
+    scratch[e] = (T)(mean_new + (vm - mean) * r2 - vm);
+  }
+  V_pk[p] = (T)fmin(fmax(vp_new, vmin), vmax);
+}
+
+template <typename T>
+__global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __restrict__ scratch,
+                                  double vmin, double vmax) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= j.n_targets) return;
+  T acc = T(0);
+  for (int k = j.tgt_ptr[q]; k < j.tgt_ptr[q + 1]; ++k) acc += scratch[j.tgt_entry[k]];
+  const int s = j.tgt_slot[q];
+  const T v = V_myo[s] + acc;
+  V_myo[s] = M<T>::mn(M<T>::mx(v, T(vmin)), T(vmax));
+}
+
+// ---------------------------------------------------------------------------
+// Stimulus schedule and step counter
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool site_active(const double* sp, double t) {
+  const double onset = sp[0], dur = sp[1], period = sp[2], npulse = sp[3];
+  if (t < onset) return false;
+  if (period <= 0.0) return t < onset + dur;
+  const double k = floor((t - onset) / period);
+  if (npulse >= 0.0 && k >= npulse) return false;
+  return (t - onset) - k * period < dur;
+}
+
+template <typename T>
+__global__ void stim_kernel(const double* __restrict__ sp, int n_sites,
+                            const uint32_t* __restrict__ gs, int n_groups, int64_t* step_ctr,
+                            double dt, T* __restrict__ amp, int advance) {
+  const int g = threadIdx.x;  // one wave, 32 groups
+  const int64_t step = *step_ctr + advance;
+  __syncthreads();
+  if (g < 32) {
+    double a = 0.0;
+    if (g > 0 && g < n_groups) {
+      const double t = (double)step * dt;
+      const uint32_t mask = gs[g];
+      for (int s = 0; s < n_sites; ++s)
+        if ((mask >> s) & 1u)
+          if (site_active(sp + 5 * s, t)) a += sp[5 * s + 4];
+    }
+    amp[g] = (T)a;
+  }
+  if (advance && g == 0) *step_ctr = step;
+}
+
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ src, const int32_t* __restrict__ idx,
+                              T* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = src[idx[i]];
+}
+
+template <typename T>
+static int myo_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_t ld,
+                    const hp_mono_params* p, void* stream) {
+  if (!p || !p->step_ctr || !p->group_amp) {
+    set_error("hp_myo_step: params, step_ctr and group_amp are required");
+    return -1;
+  }
+  return launch_bricks<T, MyoK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state, ld,
+                                *p);
+}
+
+template <typename T>
+static int cable_step(const hp_cable* c, const T* V_in, T* V_out, T* state, int64_t ld,
+                      const hp_mono_params* p, void* stream) {
+  if (!c || !p || c->n < 0 || !p->group_amp) {
+    set_error("hp_cable_step: bad arguments");
+    return -1;
+  }
+  if (c->n == 0) return 0;
+  hipLaunchKernelGGL(cable_step_kernel<T>, dim3(ceil_div(c->n, 256)), dim3(256), 0,
+                     (hipStream_t)stream, *c, V_in, V_out, state, ld, *p);
+  return check_launch("hp_cable_step");
+}
+
+template <typename T>
+static int pmj_couple(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, double vmin,
+                      double vmax, void* stream) {
+  if (!j || j->n_junctions < 0) {
+    set_error("hp_pmj_couple: bad arguments");
+    return -1;
+  }
+  if (j->n_junctions == 0 || j->c_pmj == 0.0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(pmj_junction_kernel<T>, dim3(ceil_div(j->n_junctions, 128)), dim3(128), 0,
+                     st, *j, V_pk, V_myo, scratch, dt, vmin, vmax);
+  int rc = check_launch("hp_pmj_couple/junction");
+  if (rc) return rc;
+  if (j->n_targets > 0)
+    hipLaunchKernelGGL(pmj_target_kernel<T>, dim3(ceil_div(j->n_targets, 128)), dim3(128), 0, st,
+                       *j, V_myo, scratch, vmin, vmax);
+  return check_launch("hp_pmj_couple/target");
+}
+
+template <typename T>
+static int stim(const double* sp, int n_sites, const uint32_t* gs, int n_groups,
+                const int64_t* ctr, double dt, T* amp, int advance, void* stream) {
+  if (n_sites < 0 || n_sites > 32 || n_groups < 0 || n_groups > 32 || !ctr || !amp) {
+    set_error("hp_stim: at most 32 sites and 32 groups (got %d, %d)", n_sites, n_groups);
+    return -1;
+  }
+  hipLaunchKernelGGL(stim_kernel<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, sp, n_sites, gs,
+                     n_groups, (int64_t*)ctr, dt, amp, advance);
+  return check_launch("hp_stim");
+}
+
+template <typename T>
+static int gather(const T* src, const int32_t* idx, T* out, int64_t n, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(gather_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     src, idx, out, n);
+  return check_launch("hp_gather");
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+int hp_myo_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state, int64_t ld,
+                    const hp_mono_params* p, void* stream) {
+  return myo_step<float>(g, V_in, V_out, state, ld, p, stream);
+}
+int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                    int64_t ld, const hp_mono_params* p, void* stream) {
+  return myo_step<double>(g, V_in, V_out, state, ld, p, stream);
+}
+int hp_stencil19_apply_f32(const hp_bricks* g, const float* V, float* out, double dscale,
+                           void* stream) {
+  const int64_t ld = g ? g->n_bricks * g->bx * g->by * g->bz : 0;
+  return launch_bricks<float, StencilK>(g, "hp_stencil19_apply", (hipStream_t)stream, V, out, ld,
+                                        dscale);
+}
+int hp_stencil19_apply_f64(const hp_bricks* g, const double* V, double* out, double dscale,
+                           void* stream) {
+  const int64_t ld = g ? g->n_bricks * g->bx * g->by * g->bz : 0;
+  return launch_bricks<double, StencilK>(g, "hp_stencil19_apply", (hipStream_t)stream, V, out,
+                                         ld, dscale);
+}
+int hp_cable_step_f32(const hp_cable* c, const float* V_in, float* V_out, float* state,
+                      int64_t ld, const hp_mono_params* p, void* stream) {
+  return cable_step<float>(c, V_in, V_out, state, ld, p, stream);
+}
+int hp_cable_step_f64(const hp_cable* c, const double* V_in, double* V_out, double* state,
+                      int64_t ld, const hp_mono_params* p, void* stream) {
+  return cable_step<double>(c, V_in, V_out, state, ld, p, stream);
+}
+int hp_pmj_couple_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
+                      double vmin, double vmax, void* stream) {
+  return pmj_couple<float>(j, V_pk, V_myo, scratch, dt, vmin, vmax, stream);
+}
+int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
+                      double vmin, double vmax, void* stream) {
+  return pmj_couple<double>(j, V_pk, V_myo, scratch, dt, vmin, vmax, stream);
+}
+int hp_stim_eval_f32(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
+                     const int64_t* ctr, double dt, float* amp, void* stream) {
+  return stim<float>(sp, ns, gs, ng, ctr, dt, amp, 0, stream);
+}
+int hp_stim_eval_f64(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
+                     const int64_t* ctr, double dt, double* amp, void* stream) {
+  return stim<double>(sp, ns, gs, ng, ctr, dt, amp, 0, stream);
+}
+int hp_step_advance_f32(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
+                        int64_t* ctr, double dt, float* amp, void* stream) {
+  return stim<float>(sp, ns, gs, ng, ctr, dt, amp, 1, stream);
+}
+int hp_step_advance_f64(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
+                        int64_t* ctr, double dt, double* amp, void* stream) {
+  return stim<double>(sp, ns, gs, ng, ctr, dt, amp, 1, stream);
+}
+int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream) {
+  return gather<float>(src, idx, out, n, stream);
+}
+int hp_gather_f64(const double* src, const int32_t* idx, double* out, int64_t n, void* stream) {
+  return gather<double>(src, idx, out, n, stream);
+}
+}

@@ -1,0 +1,258 @@
+"""Device-resident monodomain engine: owns the HBM buffers and launches the
+HIP kernels of one time step (myocardium fused step, Purkinje cable step,
+PMJ exchange, stimulus/step-counter advance), optionally captured in a HIP
+graph of K steps.
+
+Global node numbering follows the reference: Purkinje computational nodes
+0..Np-1, then myocardial nodes Np..Np+N_myo-1 in ``np.argwhere(S)`` order
+(purkinje_myocardium_anisotropic_solver.py:611-613).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .layout import BrickLayout
+from .ttp06 import STATE_NAMES, INIT, V_REST
+
+KI_BASE, NAI_BASE = INIT["Ki"], INIT["Nai"]
+
+
+def stim_groups(site_node_ids, n_nodes):
+    """Assign each node the set of sites it belongs to, encoded as a group id.
+
+    Returns (group (n_nodes,) int, group_sites (32,) uint32 bitmasks).
+    """
+    if len(site_node_ids) > 32:
+        raise ValueError("at most 32 stimulus sites")
+    mask = np.zeros(n_nodes, dtype=np.uint64)
+    for s, ids in enumerate(site_node_ids):
+        ids = np.asarray(ids, dtype=np.int64)
+        if ids.size:
+            mask[ids] |= np.uint64(1 << s)
+    uniq, inv = np.unique(mask, return_inverse=True)
+    nz = uniq != 0
+    gid = np.zeros(len(uniq), dtype=np.int64)
+    gid[nz] = np.arange(1, int(nz.sum()) + 1)
+    if nz.sum() > 31:
+        raise ValueError("more than 31 distinct stimulus-site combinations")
+    group_sites = np.zeros(32, dtype=np.uint32)
+    group_sites[gid[nz]] = uniq[nz].astype(np.uint32)
+    return gid[inv], group_sites
+
+
+class MonodomainEngine:
+    """One coupled Purkinje + myocardium monodomain system on one GPU.
+
+    Parameters are in the reference's units (mm, ms, mV, mS, uF, uA).
+    ``D6`` is (6, N_myo) D00,D11,D22,D01,D02,D12 in mS/mm; ``celltype_myo``
+    phenotype codes 0/1/2; ``cable`` dict(n, edges, w_pp, Ai_P, celltype);
+    ``pmj`` dict(ids, patches, c_pmj) with patches in myocardial node ids;
+    ``stim_sites`` list of dict(node_ids (global), amp, dur, onset, period, n_pulses).
+    """
+
+    def __init__(self, layout: BrickLayout, D6, celltype_myo, *, h, A_M, Cm, dt,
+                 dtype=torch.float32, device="cuda", cable=None, pmj=None, stim_sites=None,
+                 at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None):
+        nat.load()
+        self.lay = layout
+        self.dtype = dtype
+        self.dev = torch.device(device)
+        self.sfx = nat.sfx(dtype)
+        self.dt, self.Cm = float(dt), float(Cm)
+        self.vmin, self.vmax = float(vmin), float(vmax)
+        self.N_myo = layout.n_nodes
+        self.Np = int(cable["n"]) if cable else 0
+        dev, dt_ = self.dev, dtype
+        f64 = dtype == torch.float64
+        self.ki_base, self.nai_base = (0.0, 0.0) if f64 else (KI_BASE, NAI_BASE)
+
+        # stimulus groups over the global node set
+        sites = stim_sites or []
+        gid, group_sites = stim_groups([s["node_ids"] for s in sites], self.Np + self.N_myo)
+        sp = np.zeros((max(1, len(sites)), 5))
+        for k, s in enumerate(sites):
+            per = s.get("period") or 0.0
+            npl = s.get("n_pulses")
+            sp[k] = (s.get("onset", 0.0), s["dur"], per, -1.0 if npl is None else npl, s["amp"])
+        self.n_sites = len(sites)
+        self.n_groups = int(gid.max(initial=0)) + 1
+        self.site_params = torch.tensor(sp.ravel(), dtype=torch.float64, device=dev)
+        self.group_sites = torch.tensor(group_sites.view(np.int32), device=dev)
+        self.group_amp = torch.zeros(32, dtype=dt_, device=dev)
+        self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
+
+        # myocardium
+        L = layout
+        self.label = torch.from_numpy(L.labels(np.asarray(celltype_myo), gid[self.Np:])).to(dev)
+        self.D6 = torch.from_numpy(L.to_slots(np.asarray(D6, dtype=np.float64))).to(dev, dt_)
+        self.nbr = torch.from_numpy(L.nbr).to(dev)
+        self.slot_of_node = torch.from_numpy(L.slot_of_node).to(dev)
+        self.V = [torch.full((L.n_slots,), V0, dtype=dt_, device=dev) for _ in range(2)]
+        self.state = self._init_state(L.n_slots)
+        self.at = torch.full((L.n_slots,), float("nan"), dtype=dt_, device=dev)
+        self.inst = None if inst is None else torch.from_numpy(
+            L.to_slots(np.asarray(inst, dtype=np.uint16)).view(np.int16)).to(dev)
+        self.inst_params = None if inst_params is None else torch.as_tensor(
+            np.asarray(inst_params, dtype=np.float64).reshape(-1, 4)).to(dev, dt_).contiguous()
+        self.bricks = nat.HpBricks(L.n_bricks, *L.brick, 0, nat.ptr(self.nbr), nat.ptr(self.label),
+                                   nat.ptr(self.D6), nat.ptr(self.inst))
+        self.p_myo = self._params(1.0 / (A_M * h * h), at_thr, self.at)
+
+        # Purkinje cable
+        self.cable = None
+        if self.Np:
+            n = self.Np
+            e = np.asarray(cable["edges"], dtype=np.int64).reshape(-1, 2)
+            a = np.concatenate([e[:, 0], e[:, 1]])
+            b = np.concatenate([e[:, 1], e[:, 0]])
+            order = np.lexsort((b, a))
+            a, b = a[order], b[order]
+            indptr = np.zeros(n + 1, dtype=np.int32)
+            np.add.at(indptr, a + 1, 1)
+            indptr = np.cumsum(indptr).astype(np.int32)
+            ctp = np.asarray(cable.get("celltype", np.zeros(n)), dtype=np.uint8) & 3
+            lab_pk = ctp | (gid[:n].astype(np.uint8) << 2)
+            self.pk_indptr = torch.from_numpy(indptr).to(dev)
+            self.pk_indices = torch.from_numpy(b.astype(np.int32)).to(dev)
+            self.pk_label = torch.from_numpy(lab_pk.astype(np.uint8)).to(dev)
+            self.Vpk = [torch.full((n,), V0, dtype=dt_, device=dev) for _ in range(2)]
+            self.state_pk = self._init_state(n)
+            self.cable = nat.HpCable(n, nat.ptr(self.pk_indptr), nat.ptr(self.pk_indices),
+                                     nat.ptr(self.pk_label), float(cable["w_pp"]))
+            self.p_pk = self._params(1.0 / float(cable["Ai_P"]), at_thr, None)
+
+        # PMJ
+        self.pmj = None
+        if pmj and self.Np and pmj["c_pmj"] != 0.0 and len(pmj["ids"]):
+            patches = [np.asarray(p, dtype=np.int64) for p in pmj["patches"]]
+            ptrs = np.zeros(len(patches) + 1, dtype=np.int32)
+            ptrs[1:] = np.cumsum([len(p) for p in patches])
+            entries = L.slot_of_node[np.concatenate(patches)].astype(np.int32)
+            order = np.argsort(entries, kind="stable")
+            tgt, first = np.unique(entries[order], return_index=True)
+            tptr = np.append(first, len(entries)).astype(np.int32)
+            self.pmj_t = [torch.from_numpy(x).to(dev) for x in (
+                np.asarray(pmj["ids"], dtype=np.int32), ptrs, entries, tgt.astype(np.int32), tptr,
+                order.astype(np.int32))]
+            self.pmj_scratch = torch.zeros(len(entries), dtype=dt_, device=dev)
+            pk_, ptr_, ent_, tgt_, tptr_, tent_ = self.pmj_t
+            self.pmj = nat.HpPmj(len(patches), len(entries), nat.ptr(pk_), nat.ptr(ptr_),
+                                 nat.ptr(ent_), len(tgt), nat.ptr(tgt_), nat.ptr(tptr_),
+                                 nat.ptr(tent_), float(pmj["c_pmj"]), float(pmj["Ai_P"]),
+                                 float(pmj["Ai_M"]), self.Cm)
+        self.p = 0
+        self._eval_stim()
+
+    # ------------------------------------------------------------------ setup
+    def _init_state(self, n):
+        st = torch.empty((18, n), dtype=self.dtype, device=self.dev)
+        for k, name in enumerate(STATE_NAMES):
+            v = INIT[name]
+            if name == "Ki":
+                v -= self.ki_base
+            elif name == "Nai":
+                v -= self.nai_base
+            st[k].fill_(v)
+        return st
+
+    def _params(self, dscale, at_thr, at):
+        return nat.HpMonoParams(
+            self.dt, 1.0 / self.Cm, dscale, self.ki_base, self.nai_base, self.vmin, self.vmax,
+            float(at_thr), 1, 0 if self.inst_params is None else self.inst_params.shape[0],
+            nat.ptr(self.inst_params), nat.ptr(self.step_ctr), nat.ptr(self.group_amp),
+            nat.ptr(at), None)
+
+    def _eval_stim(self):
+        nat.call(f"hp_stim_eval_{self.sfx}", nat.ptr(self.site_params), self.n_sites,
+                 nat.ptr(self.group_sites), self.n_groups, nat.ptr(self.step_ctr), self.dt,
+                 nat.ptr(self.group_amp), nat.stream_handle())
+
+    # ------------------------------------------------------------------- step
+    def _launch_step(self):
+        s = nat.stream_handle()
+        a, b = self.p, 1 - self.p
+        nat.call(f"hp_myo_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
+                 nat.ptr(self.V[b]), nat.ptr(self.state), self.lay.n_slots, C.byref(self.p_myo), s)
+        if self.cable is not None:
+            nat.call(f"hp_cable_step_{self.sfx}", C.byref(self.cable), nat.ptr(self.Vpk[a]),
+                     nat.ptr(self.Vpk[b]), nat.ptr(self.state_pk), self.Np, C.byref(self.p_pk), s)
+        if self.pmj is not None:
+            nat.call(f"hp_pmj_couple_{self.sfx}", C.byref(self.pmj), nat.ptr(self.Vpk[b]),
+                     nat.ptr(self.V[b]), nat.ptr(self.pmj_scratch), self.dt, self.vmin, self.vmax, s)
+        nat.call(f"hp_step_advance_{self.sfx}", nat.ptr(self.site_params), self.n_sites,
+                 nat.ptr(self.group_sites), self.n_groups, nat.ptr(self.step_ctr), self.dt,
+                 nat.ptr(self.group_amp), s)
+        self.p = b
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._launch_step()
+
+    def capture(self, k: int):
+        """Capture k (even) steps into a HIP graph; returns the graph."""
+        if k <= 0 or k % 2:
+            raise ValueError("graph chunk must be a positive even number of steps")
+        p0 = self.p
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.dev)
+        side.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(g, stream=side):
+                for _ in range(k):
+                    self._launch_step()
+        torch.cuda.current_stream(self.dev).wait_stream(side)
+        self.p = p0
+        return g
+
+    def run(self, n_steps: int, graph_chunk: int = 0):
+        """Advance n_steps; with graph_chunk > 0, replay a captured graph of that many steps."""
+        if graph_chunk and n_steps >= graph_chunk:
+            key = ("graph", graph_chunk)
+            if not hasattr(self, "_graphs"):
+                self._graphs = {}
+            if key not in self._graphs:
+                self._graphs[key] = self.capture(graph_chunk)
+            g = self._graphs[key]
+            for _ in range(n_steps // graph_chunk):
+                g.replay()
+            n_steps %= graph_chunk
+        self.step(n_steps)
+
+    # --------------------------------------------------------------- readback
+    @property
+    def step_index(self) -> int:
+        return int(self.step_ctr.item())
+
+    def V_myo_nodes(self, out=None):
+        """Myocardial V in reference node order (N_myo,)."""
+        out = torch.empty(self.N_myo, dtype=self.dtype, device=self.dev) if out is None else out
+        nat.call(f"hp_gather_{self.sfx}", nat.ptr(self.V[self.p]), nat.ptr(self.slot_of_node),
+                 nat.ptr(out), self.N_myo, nat.stream_handle())
+        return out
+
+    def V_global(self):
+        """[V_purkinje | V_myo] in the reference's global order."""
+        vm = self.V_myo_nodes()
+        if self.Np:
+            return torch.cat([self.Vpk[self.p], vm])
+        return vm
+
+    def at_nodes(self):
+        out = torch.empty(self.N_myo, dtype=self.dtype, device=self.dev)
+        nat.call(f"hp_gather_{self.sfx}", nat.ptr(self.at), nat.ptr(self.slot_of_node),
+                 nat.ptr(out), self.N_myo, nat.stream_handle())
+        return out
+
+    def state_nodes(self):
+        """(N_global, 18) state in the reference's column order, absolute Ki/Nai."""
+        sm = self.state[:, self.slot_of_node.long()]
+        if self.Np:
+            sm = torch.cat([self.state_pk, sm], dim=1)
+        sm = sm.t().contiguous()
+        sm[:, 0] += self.ki_base
+        sm[:, 1] += self.nai_base
+        return sm

@@ -1,0 +1,144 @@
+"""Seeded synthetic inputs for the benchmark workloads (no patient data needed).
+
+* ``slab``       -- rectangular all-myocardium slab (the reference's CPU case).
+* ``biv_shell``  -- biventricular shell from two truncated confocal ellipsoids,
+                    transmural coordinate phi, rule-based helix fibres
+                    (+60 deg endo -> -60 deg epi), and a seeded fractal
+                    Purkinje tree on the endocardium, returned in the
+                    reference's input conventions: S/phi/fibre_dir in raw
+                    array-axis order, tree nodes in voxel units with the
+                    (col,row,depth) convention (purkinje_myocardium_anisotropic_solver.py:1057-1068).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy import ndimage as ndi
+
+
+def slab(nx=64, ny=64, nz=16):
+    S = np.ones((nx, ny, nz), dtype=np.uint8)
+    phi = np.zeros(S.shape, dtype=np.float32)
+    f = np.zeros(S.shape + (3,), dtype=np.float32)
+    f[..., 0] = 1.0
+    return S, phi, f
+
+
+def stub_tree(shape):
+    """Inert 2-node Purkinje stub outside the grid (used with c_pmj = 0)."""
+    Nx, Ny, Nz = shape
+    nodes = np.array([[Ny + 5.0, Nx + 5.0, Nz + 5.0], [Ny + 5.0, Nx + 5.0, Nz + 10.0]])
+    return nodes, np.array([[0, 1]]), np.array([0.0, 1.0])
+
+
+def _ellipsoid(X, Y, Z, c, ax):
+    return ((X - c[0]) / ax[0]) ** 2 + ((Y - c[1]) / ax[1]) ** 2 + ((Z - c[2]) / ax[2]) ** 2 <= 1.0
+
+
+def biv_shell(n=256, h=0.4, seed=0, wall_mm=(11.5, 5.0)):
+    """Biventricular shell on an n^3 grid of spacing h (mm).
+
+    LV: confocal prolate shell (cavity semi-axes a_i, c_i; epicardium with the
+    same focal length, thickness wall_mm[0] at the equator); RV: a second,
+    wider confocal shell shifted along x, wrapping the LV, free wall
+    wall_mm[1]; both truncated by a base plane.  Returns dict(S, phi,
+    fibre_dir, cavity_lv, cavity_rv, h).
+    """
+    L = n * h
+    ax = (np.arange(n) + 0.5) * h - L / 2
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    scale = L / 102.4
+    a_i, c_i = 22.0 * scale, 42.0 * scale
+    f2 = c_i ** 2 - a_i ** 2
+    a_o = a_i + wall_mm[0] * scale
+    c_o = np.sqrt(a_o ** 2 + f2)
+    zc = 8.0 * scale                      # long axis along z, base at the top
+    lv_out = _ellipsoid(X, Y, Z, (0, 0, zc), (a_o, a_o, c_o))
+    lv_cav = _ellipsoid(X, Y, Z, (0, 0, zc), (a_i, a_i, c_i))
+    rvc = (18.0 * scale, 0.0, zc + 4.0 * scale)
+    b_i = (30.0 * scale, 24.0 * scale, 38.0 * scale)
+    rv_cav = _ellipsoid(X, Y, Z, rvc, b_i) & ~lv_out
+    b_o = tuple(b + wall_mm[1] * scale for b in b_i)
+    rv_out = _ellipsoid(X, Y, Z, rvc, b_o)
+    base = Z <= zc + 18.0 * scale
+    S = (lv_out | rv_out) & ~lv_cav & ~rv_cav & base
+    # keep the largest connected piece
+    lab, nl = ndi.label(S)
+    if nl > 1:
+        sizes = ndi.sum(S, lab, range(1, nl + 1))
+        S = lab == (1 + int(np.argmax(sizes)))
+    cav = (lv_cav | rv_cav) & base
+    # transmural coordinate: distance to cavity (endo) vs to the outside (epi)
+    outside = ~(S | cav)
+    d_endo = ndi.distance_transform_edt(~cav)
+    d_epi = ndi.distance_transform_edt(~outside)
+    phi = np.where(S, d_endo / np.maximum(d_endo + d_epi, 1e-6), np.nan).astype(np.float32)
+    phi = np.clip(phi, 0.0, 1.0)
+    fibre = rule_based_fibres(S, phi)
+    return dict(S=S.astype(np.uint8), phi=phi, fibre_dir=fibre, cavity_lv=lv_cav & base,
+                cavity_rv=rv_cav & base, h=h)
+
+
+def rule_based_fibres(S, phi, alpha_endo=60.0, alpha_epi=-60.0):
+    """Helix fibres: alpha(phi) rotation of the circumferential direction about
+    the transmural gradient (Bayer-style rule, long axis = array axis 2)."""
+    p = np.nan_to_num(phi, nan=0.0).astype(np.float64)
+    p = ndi.gaussian_filter(p, 1.0)
+    g = np.stack(np.gradient(p), axis=-1)
+    t = g / np.maximum(np.linalg.norm(g, axis=-1, keepdims=True), 1e-9)
+    ez = np.array([0.0, 0.0, 1.0])
+    l = ez - (t @ ez)[..., None] * t
+    l /= np.maximum(np.linalg.norm(l, axis=-1, keepdims=True), 1e-9)
+    c = np.cross(l, t)
+    a = np.deg2rad(alpha_endo + (alpha_epi - alpha_endo) * np.nan_to_num(phi, nan=0.5))[..., None]
+    f = np.cos(a) * c + np.sin(a) * l
+    f /= np.maximum(np.linalg.norm(f, axis=-1, keepdims=True), 1e-9)
+    f[~S.astype(bool)] = 0.0
+    return f.astype(np.float32)
+
+
+def purkinje_tree(geo, n_pmj=800, seed=0, step_mm=0.55, root_frac=0.85):
+    """Seeded fractal Purkinje tree on the endocardial surfaces.
+
+    Terminals are n_pmj seeded endocardial voxels (LV and RV in proportion to
+    their endocardial area); they are attached, nearest-first, to the growing
+    tree through straight paths with nodes every ``step_mm``.  Returns
+    (nodes [voxel units, (col,row,depth)], elements (E,2), activation_times).
+    """
+    rng = np.random.default_rng(seed)
+    S = geo["S"].astype(bool)
+    h = geo["h"]
+    cav = geo["cavity_lv"] | geo["cavity_rv"]
+    endo = cav & ndi.binary_dilation(S)  # cavity voxels touching the wall
+    pts = np.argwhere(endo).astype(np.float64)
+    if len(pts) == 0:
+        raise ValueError("no endocardial surface")
+    # root: high on the septum (top of the LV cavity, toward the RV)
+    lv = np.argwhere(geo["cavity_lv"] & endo)
+    top = lv[:, 2] >= np.quantile(lv[:, 2], root_frac)
+    root = lv[top][np.argmax(lv[top][:, 0])].astype(np.float64)
+    pick = rng.choice(len(pts), size=min(n_pmj, len(pts)), replace=False)
+    targets = pts[pick]
+    order = np.argsort(np.linalg.norm(targets - root, axis=1))
+    nodes = [root]
+    elements = []
+    dist = [0.0]
+    open_ = [True]  # terminals are not attachment points, so each stays a PMJ
+    step = step_mm / h
+    for k, ti in enumerate(order):
+        tgt = targets[ti]
+        arr = np.asarray(nodes)
+        d = np.linalg.norm(arr - tgt, axis=1)
+        d[~np.asarray(open_)] = np.inf
+        j = int(np.argmin(d))
+        nseg = max(1, int(np.ceil(d[j] / step)))
+        prev = j
+        for s in range(1, nseg + 1):
+            p = arr[j] + (tgt - arr[j]) * (s / nseg)
+            nodes.append(p)
+            open_.append(s < nseg)
+            elements.append([prev, len(nodes) - 1])
+            dist.append(dist[prev] + d[j] / nseg)
+            prev = len(nodes) - 1
+    nodes = np.asarray(nodes)[:, [1, 0, 2]]  # (row,col,depth) -> (col,row,depth)
+    act = np.asarray(dist) * h / 2.5          # ~2.5 mm/ms conduction
+    return nodes, np.asarray(elements, dtype=np.int64), act

@@ -1,0 +1,99 @@
+"""Brick layout of the myocardium in HBM.
+
+The voxel mask S (Nx,Ny,Nz) is cut into bricks of (bx,by,bz) voxels; only
+bricks holding at least one myocardial voxel are stored, in raster order of
+the brick grid.  Every per-voxel array (V, the 18 TTP06 state variables, the
+six conductivity-tensor components, the label byte) is a flat array of
+n_bricks*bvol slots, slot = brick*bvol + (lx*by + ly)*bz + lz, so a wave of
+64 lanes reads 64 consecutive slots of one brick (coalesced SoA).  A
+27-entry neighbour table per brick lets a workgroup stage its halo tile.
+
+Node order: node i is the i-th voxel of ``np.argwhere(S)`` -- the reference's
+myocardial node numbering (purkinje_myocardium_anisotropic_solver.py:960) --
+and ``slot_of_node[i]`` maps it to its slot.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BRICK_SHAPES = ((4, 4, 4), (8, 4, 4), (8, 8, 4), (8, 8, 8))
+INACTIVE = 0xFF
+
+
+class BrickLayout:
+    def __init__(self, S: np.ndarray, brick=(8, 8, 8)):
+        brick = tuple(int(b) for b in brick)
+        if brick not in BRICK_SHAPES:
+            raise ValueError(f"brick shape {brick} not supported; choose one of {BRICK_SHAPES}")
+        S = np.asarray(S).astype(bool)
+        if S.ndim != 3:
+            raise ValueError("S must be a 3-D voxel mask")
+        self.shape = S.shape
+        self.brick = brick
+        bx, by, bz = brick
+        self.bvol = bx * by * bz
+        Nx, Ny, Nz = S.shape
+        self.grid = (-(-Nx // bx), -(-Ny // by), -(-Nz // bz))
+        gx, gy, gz = self.grid
+        vox = np.argwhere(S).astype(np.int64)
+        self.vox_idx = vox.astype(np.int32)
+        self.n_nodes = len(vox)
+        bc = vox // np.array(brick)
+        bid = (bc[:, 0] * gy + bc[:, 1]) * gz + bc[:, 2]
+        active = np.unique(bid)
+        self.n_bricks = len(active)
+        index_of = -np.ones(gx * gy * gz, dtype=np.int64)
+        index_of[active] = np.arange(self.n_bricks)
+        lc = vox % np.array(brick)
+        local = (lc[:, 0] * by + lc[:, 1]) * bz + lc[:, 2]
+        self.slot_of_node = (index_of[bid] * self.bvol + local).astype(np.int32)
+        self.n_slots = self.n_bricks * self.bvol
+        ab = np.stack([active // (gy * gz), (active // gz) % gy, active % gz], axis=1)
+        self.brick_coords = ab.astype(np.int32)
+        nbr = np.empty((self.n_bricks, 27), dtype=np.int32)
+        k = 0
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    c = ab + np.array([dx, dy, dz])
+                    ok = ((c[:, 0] >= 0) & (c[:, 0] < gx) & (c[:, 1] >= 0) & (c[:, 1] < gy)
+                          & (c[:, 2] >= 0) & (c[:, 2] < gz))
+                    lin = np.where(ok, (c[:, 0] * gy + c[:, 1]) * gz + c[:, 2], 0)
+                    nbr[:, k] = np.where(ok, index_of[lin], -1)
+                    k += 1
+        self.nbr = nbr
+
+    @property
+    def occupancy(self) -> float:
+        return self.n_nodes / max(1, self.n_slots)
+
+    def to_slots(self, values: np.ndarray, fill=0):
+        """Scatter per-node values (N,) or (k,N) into slot arrays."""
+        values = np.asarray(values)
+        if values.ndim == 1:
+            out = np.full(self.n_slots, fill, dtype=values.dtype)
+            out[self.slot_of_node] = values
+        else:
+            out = np.full((values.shape[0], self.n_slots), fill, dtype=values.dtype)
+            out[:, self.slot_of_node] = values
+        return out
+
+    def labels(self, celltype: np.ndarray, group: np.ndarray | None = None) -> np.ndarray:
+        """Label byte per slot: phenotype in bits 0-1, stimulus group in bits 2-6."""
+        lab = celltype.astype(np.uint8) & 3
+        if group is not None:
+            if group.max(initial=0) > 31:
+                raise ValueError("at most 31 stimulus groups")
+            lab = lab | (group.astype(np.uint8) << 2)
+        return self.to_slots(lab, fill=INACTIVE).astype(np.uint8)
+
+
+def choose_brick(S: np.ndarray, target_blocks: int = 2048) -> tuple:
+    """Default brick shape: the largest brick that still gives enough workgroups
+    to fill the chip (>= target_blocks); otherwise the smallest brick."""
+    n = int(np.count_nonzero(S))
+    for b in reversed(BRICK_SHAPES):
+        lay_blocks = n / (b[0] * b[1] * b[2])
+        if lay_blocks >= target_blocks:
+            return b
+    return BRICK_SHAPES[0]

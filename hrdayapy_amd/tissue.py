@@ -1,0 +1,157 @@
+"""Host-side set-up of the coupled tissue model (one-time, numpy/scipy).
+
+Builds what the engine consumes from the reference's inputs:
+  * conductivity tensor per myocardial voxel
+    (purkinje_myocardium_anisotropic_solver.py:856-883);
+  * phenotype per node from phi (hetero_solver.py:92-101; NaN -> 0.5 as in
+    purkinje_myocardium_anisotropic_solver.py:1548-1551);
+  * Purkinje cable discretisation (same file :113-136), root stimulus zone
+    (:1491-1502), terminals and PMJ patches (:1145-1177, :1566-1578);
+  * derived constants w_pp, Ai_P, Ai_M (:1506-1538);
+  * stimulus sites (:1204-1287).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def conductivity_D6(fibre_dir, vox_idx, sigma_l, sigma_t, scale=None):
+    """(6, N) D00,D11,D22,D01,D02,D12 of D = st*I + (sl-st) f f^T."""
+    f = np.asarray(fibre_dir)[vox_idx[:, 0], vox_idx[:, 1], vox_idx[:, 2], :].astype(np.float64)
+    ds = sigma_l - sigma_t
+    D = np.empty((6, len(vox_idx)))
+    D[0] = sigma_t + ds * f[:, 0] * f[:, 0]
+    D[1] = sigma_t + ds * f[:, 1] * f[:, 1]
+    D[2] = sigma_t + ds * f[:, 2] * f[:, 2]
+    D[3] = ds * f[:, 0] * f[:, 1]
+    D[4] = ds * f[:, 0] * f[:, 2]
+    D[5] = ds * f[:, 1] * f[:, 2]
+    if scale is not None:
+        D *= np.asarray(scale)[None, :]
+    return D
+
+
+def isotropic_D6(n, sigma):
+    D = np.zeros((6, n))
+    D[:3] = sigma
+    return D
+
+
+def phenotype_from_phi(phi, vox_idx, phi_endo_max=0.35, phi_epi_min=0.65):
+    ph = np.asarray(phi)[vox_idx[:, 0], vox_idx[:, 1], vox_idx[:, 2]].astype(np.float32)
+    ph = np.where(np.isfinite(ph), ph, np.float32(0.5))
+    ct = np.ones(len(ph), dtype=np.uint8)
+    ct[ph < phi_endo_max] = 0
+    ct[ph >= phi_epi_min] = 2
+    return ct
+
+
+def discretise_tree(nodes_mm, elements, dx):
+    """Split every element into max(1, round(len/dx)) segments; original nodes keep their
+    index, interior points are appended.  Returns (comp_nodes, comp_edges, branch_map)."""
+    nodes_mm = np.asarray(nodes_mm, dtype=np.float32)
+    elements = np.asarray(elements, dtype=np.int64).reshape(-1, 2)
+    n0 = len(nodes_mm)
+    p0 = nodes_mm[elements[:, 0]]
+    p1 = nodes_mm[elements[:, 1]]
+    nseg = np.maximum(1, np.round(np.linalg.norm(p1 - p0, axis=1) / dx).astype(np.int64))
+    n_extra = int((nseg - 1).sum())
+    extra = np.empty((n_extra, 3), dtype=np.float32)
+    edges = np.empty((int(nseg.sum()), 2), dtype=np.int32)
+    bmaps = []
+    nxt, k, ke = n0, 0, 0
+    for e in range(len(elements)):
+        m = int(nseg[e])
+        ids = [int(elements[e, 0])] + list(range(nxt, nxt + m - 1)) + [int(elements[e, 1])]
+        if m > 1:
+            t = (np.arange(1, m) / m)[:, None]
+            extra[k:k + m - 1] = (1 - t) * p0[e] + t * p1[e]
+            k += m - 1
+            nxt += m - 1
+        edges[ke:ke + m, 0] = ids[:-1]
+        edges[ke:ke + m, 1] = ids[1:]
+        ke += m
+        bmaps.append(ids)
+    return np.concatenate([nodes_mm, extra]), edges, bmaps
+
+
+def root_zone(nodes_mm, act_times, comp_nodes, T, stim_len_mm):
+    act = np.array(act_times, dtype=np.float32)
+    fin = np.isfinite(act)
+    act[~fin] = T
+    lo = float(act[fin].min())
+    root = np.nonzero(np.isclose(act, lo, atol=1e-6))[0]
+    centre = np.asarray(nodes_mm)[root].mean(axis=0)
+    d = np.linalg.norm(comp_nodes - centre[None, :], axis=1)
+    mask = d <= stim_len_mm
+    if not mask.any():
+        mask[int(d.argmin())] = True
+    return mask
+
+
+def pmj_sites(elements, branch_map, n_nodes, Np):
+    """Cable node of every terminal branch end (terminal = degree-1 node other than 0)."""
+    elements = np.asarray(elements, dtype=np.int64).reshape(-1, 2)
+    deg = np.bincount(elements.ravel(), minlength=n_nodes)
+    term = np.nonzero(deg == 1)[0]
+    term = term[term != 0]
+    if term.size == 0:
+        term = np.array([n_nodes - 1])
+    is_term = np.zeros(n_nodes, dtype=bool)
+    is_term[term] = True
+    ids = sorted({bm[-1] for i, bm in enumerate(branch_map) if is_term[elements[i, 1]]})
+    return ids if ids else [Np - 1]
+
+
+def pmj_patches(comp_nodes, ids, myo_nodes_mm, n_pmj=1):
+    from scipy.spatial import cKDTree
+    k = min(n_pmj, len(myo_nodes_mm))
+    _, nn = cKDTree(myo_nodes_mm).query(np.asarray(comp_nodes)[ids], k=k)
+    nn = np.asarray(nn).reshape(len(ids), k)
+    return [row.tolist() for row in nn]
+
+
+def purkinje_constants(R_P, sigma_P, dx_p, A_M, voxel_size, A_P=None):
+    S_P = math.pi * R_P ** 2
+    A_P = 2.0 / R_P if A_P is None else A_P
+    return dict(w_pp=sigma_P * S_P / dx_p, Ai_P=A_P * S_P * dx_p, Ai_M=A_M * voxel_size ** 3)
+
+
+def stim_sites(stim_protocol, root_mask, ectopic_region, vox_idx, Np, stim_regions=None):
+    """Per-site global node ids and timing, the schema of the reference's stimulus protocol."""
+    root_ids = np.nonzero(root_mask)[0]
+
+    def in_mask(mask):
+        m = np.asarray(mask)[vox_idx[:, 0], vox_idx[:, 1], vox_idx[:, 2]].astype(bool)
+        return Np + np.nonzero(m)[0]
+
+    ect = in_mask(ectopic_region) if ectopic_region is not None else np.zeros(0, np.int64)
+    regions = {k: in_mask(v) for k, v in (stim_regions or {}).items()}
+    out = []
+    for ev in stim_protocol:
+        target = ev.get("target", "root")
+        if target == "root":
+            ids = root_ids
+        elif target == "ectopic":
+            ids = ect
+            if ids.size == 0:
+                print(f"  WARNING: stimulus '{ev.get('name', target)}' targets 'ectopic' but no "
+                      "myocardial nodes fall inside ectopic_region.")
+        elif target == "region":
+            rname = ev.get("region")
+            if rname not in regions:
+                raise ValueError(f"Stimulus '{ev.get('name', target)}' has target='region' with "
+                                 f"region={rname!r}, but stim_regions only defines {sorted(regions)}.")
+            ids = regions[rname]
+        elif target == "custom":
+            ids = np.asarray(ev["node_ids"], dtype=np.int64)
+        else:
+            raise ValueError(f"Unknown stimulus target '{target}' (expected 'root', 'ectopic', "
+                             "'region', or 'custom').")
+        out.append(dict(name=ev.get("name", target), target=target, node_ids=np.asarray(ids),
+                        n_nodes=int(np.asarray(ids).size), amp=float(ev["amp"]),
+                        dur=float(ev["dur"]), onset=float(ev.get("onset", 0.0)),
+                        period=ev.get("period", None) or None, n_pulses=ev.get("n_pulses", None)))
+    return out

@@ -1,0 +1,212 @@
+/* hrdaya_hip.h -- C ABI of libhrdaya_hip.so, the MI355X (gfx950) kernels behind
+ * the hrdayapy simulation and ECG hot path.
+ *
+ * Conventions
+ *   - Every pointer is a device pointer owned by the caller (torch tensors on
+ *     the Python side); the library allocates nothing per call.
+ *   - `stream` is a hipStream_t passed as void* (torch's current stream).
+ *   - Every function returns 0 on success or a negative hipError_t; the
+ *     message of the last failure on the calling thread is available from
+ *     hp_last_error().
+ *   - Suffix _f32 / _f64 selects the arithmetic type of the state and of the
+ *     voltages.  FP64 kernels exist so that the GPU path can be compared with
+ *     the FP64 oracle at round-off level; FP32 is the production form.
+ *
+ * Reference interfaces replaced (paths relative to the HrdayaPy tree):
+ *   ionic step      hrdayapy/simulation/functions/ionic_models/gpu_ttp06.py:198-412
+ *                   (RegionIonicModel.step, protocol.py:72-121;
+ *                    HeteroIonicSolver.step, hetero_solver.py:141-194)
+ *   diffusion       torch.sparse.mm(L, V*) + _bicgstab,
+ *                   hrdayapy/anisotropic_simulation/functions/
+ *                   purkinje_myocardium_anisotropic_solver.py:1787-1791, 786-849
+ *   stimulus        _stim_active / I_stim_buf, same file :1290-1301, :1754-1759
+ *   ECG source      make_gpu_solver.solve_frame, hrdayapy/ecg/functions/compute_bspm.py:381-419
+ *   torso PCG       _pcg_gpu, compute_bspm.py:164-263
+ */
+#ifndef HRDAYA_HIP_H
+#define HRDAYA_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP_ABI_VERSION 1
+
+/* ---------------------------------------------------------------------------
+ * Library
+ * ------------------------------------------------------------------------- */
+int hp_abi_version(void);
+/* Copies the last error message of the calling thread into buf (NUL-terminated). */
+int hp_last_error(char* buf, size_t size);
+
+/* ---------------------------------------------------------------------------
+ * TTP06 reaction step (ionic plugin seam).
+ * Replaces _currents_and_gates (gpu_ttp06.py:198-412) as called through
+ * EndoTTP06/MidTTP06/EpiTTP06.step (gpu_ttp06.py:419-513).
+ *
+ * state element (node i, variable k) lives at state[k*stride_var + i*stride_node]:
+ * (N,18) row-major tensors use stride_var=1, stride_node=18; the engine's SoA
+ * buffers use stride_var=ld, stride_node=1.  Variables are in the reference's
+ * column order (Ki, Nai, Cai, Xr1, Xr2, Xs, m, h, j, d, f, f2, fCass, s, r,
+ * CaSR, Rprime, Cass).  Ki/Nai are stored as (value - ki_base/nai_base).
+ * celltype: per-node phenotype code (0 endo, 1 mid, 2 epi) or NULL, in which
+ * case fixed_celltype applies to every node.  gks_scale/gto_scale: optional
+ * per-node multipliers of g_Ks/g_to (NULL = 1).
+ * ------------------------------------------------------------------------- */
+int hp_ttp06_step_f32(const float* V, float* state, float* I_ion, const uint8_t* celltype,
+                      int fixed_celltype, const float* gks_scale, const float* gto_scale,
+                      int64_t n, int64_t stride_var, int64_t stride_node, float dt,
+                      float ki_base, float nai_base, void* stream);
+int hp_ttp06_step_f64(const double* V, double* state, double* I_ion, const uint8_t* celltype,
+                      int fixed_celltype, const double* gks_scale, const double* gto_scale,
+                      int64_t n, int64_t stride_var, int64_t stride_node, double dt,
+                      double ki_base, double nai_base, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Myocardium on a brick layout.
+ *
+ * The myocardial voxels of the (Nx,Ny,Nz) mask are stored in bricks of
+ * bx*by*bz voxels (bx,by,bz in {4,8}); only bricks that hold at least one
+ * myocardial voxel are stored.  Voxel (lx,ly,lz) of brick b has slot
+ * b*bvol + (lx*by + ly)*bz + lz.  Per slot:
+ *   label  uint8: 0xFF = not myocardium; else bits 0-1 phenotype code,
+ *          bits 2-6 stimulus group (0 = none)
+ *   D6     conductivity tensor D00,D11,D22,D01,D02,D12 [mS/mm], SoA [6][n_slots]
+ *   inst   uint16 ensemble instance (or NULL: instance 0)
+ * nbr[b*27 + (dx+1)*9 + (dy+1)*3 + (dz+1)] is the brick at offset (dx,dy,dz),
+ * or -1.
+ *
+ * The 19-point operator is the reference's build_myo_mesh_anisotropic
+ * (purkinje_myocardium_anisotropic_solver.py:886-1074) with L = D^-1 K
+ * (:606-746): face weight harm(D_kk,a, D_kk,b)*h, edge-diagonal weight
+ * +-0.5*(D_ij,a + D_ij,b)*h/2, rows divided by A_M*h^3.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_bricks {
+  int64_t n_bricks;
+  int32_t bx, by, bz;
+  int32_t pad_;
+  const int32_t* nbr;    /* [n_bricks][27] */
+  const uint8_t* label;  /* [n_slots] */
+  const void* D6;        /* T[6][n_slots] */
+  const uint16_t* inst;  /* [n_slots] or NULL */
+} hp_bricks;
+
+/* Per-step scalars of the monodomain update.
+ *   V_new = clamp(V + dt*(-I_ion) + (dt/Cm)*(I_stim + L V), vmin, vmax)
+ * with L V computed on V^n (explicit, fused with the reaction step). */
+typedef struct hp_mono_params {
+  double dt;          /* ms */
+  double inv_cm;      /* 1/Cm, mm^2/uF */
+  double dscale;      /* 1/(A_M*h^2) for the myocardium, 1/Ai_P for the cable */
+  double ki_base, nai_base;
+  double vmin, vmax;  /* clamp, reference: [-150, 80] */
+  double at_thr;      /* activation threshold, reference: -40 mV */
+  int32_t do_reaction;   /* 1: TTP06 step; 0: diffusion only */
+  int32_t n_inst;        /* number of rows of inst_params */
+  const void* inst_params;  /* T[n_inst][4]: g_Ks scale, g_to scale, sigma scale, spare */
+  const int64_t* step_ctr;  /* device step counter; time label t = step*dt */
+  const void* group_amp;    /* T[32] stimulus amplitude of each group this step [uA/mm^2] */
+  void* at;                 /* T[n] first activation time (NaN until crossed) or NULL */
+  const void* rate_ext;     /* T[n] extra dV/dt [mV/ms] (surrogate reaction) or NULL */
+} hp_mono_params;
+
+/* One fused explicit step of the myocardium: TTP06 + 19-point diffusion +
+ * stimulus + clamp + first-activation detection.  Reads V_in, writes V_out,
+ * updates state (SoA, ld = n_slots) in place. */
+int hp_myo_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state,
+                    int64_t ld, const hp_mono_params* p, void* stream);
+int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                    int64_t ld, const hp_mono_params* p, void* stream);
+
+/* out = L V on the brick layout (uA/mm^2); out is 0 on non-myocardial slots.
+ * Replaces torch.sparse.mm(L_global, V) restricted to the myocardium
+ * (purkinje_myocardium_anisotropic_solver.py:1787). */
+int hp_stencil19_apply_f32(const hp_bricks* g, const float* V, float* out, double dscale,
+                           void* stream);
+int hp_stencil19_apply_f64(const hp_bricks* g, const double* V, double* out, double dscale,
+                           void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Purkinje cable (replaces the Purkinje rows of L_global,
+ * purkinje_myocardium_anisotropic_solver.py:678-681, 721-724, and their part in
+ * the ionic step, hetero_solver.py:163-168).  Nodes are stored compactly;
+ * neighbours in CSR (indptr/indices), all edges weight w_pp.
+ * label: bits 0-1 phenotype (always endo in the reference), bits 2-6 stim group.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_cable {
+  int64_t n;
+  const int32_t* indptr;   /* [n+1] */
+  const int32_t* indices;  /* [nnz] */
+  const uint8_t* label;    /* [n] */
+  double w;                /* w_pp = sigma_P*pi*R_P^2/dx_p [mS] */
+} hp_cable;
+
+int hp_cable_step_f32(const hp_cable* c, const float* V_in, float* V_out, float* state,
+                      int64_t ld, const hp_mono_params* p, void* stream);
+int hp_cable_step_f64(const hp_cable* c, const double* V_in, double* V_out, double* state,
+                      int64_t ld, const hp_mono_params* p, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Purkinje-myocardium junctions (replaces the PMJ entries of L_global,
+ * purkinje_myocardium_anisotropic_solver.py:684-692).  Junction j couples
+ * cable node pk[j] to the myocardial slots patch[ptr[j]..ptr[j+1]) with total
+ * conductance c_pmj (c_pmj/|patch| per slot).  Over dt the exchange is
+ * integrated exactly (two-compartment exponential relaxation of the cable
+ * node against the patch mean, plus relaxation of the patch deviations),
+ * conserving charge Ai_P*Vp + Ai_M*sum(Vm).  Increments on slots shared by
+ * several junctions are summed in a fixed order (slot-major CSR
+ * tgt_ptr/tgt_entry over the patch entries), so results are deterministic.
+ * scratch: T[n_entries + n_junctions].
+ * ------------------------------------------------------------------------- */
+typedef struct hp_pmj {
+  int64_t n_junctions;
+  int64_t n_entries;
+  const int32_t* pk;        /* [n_junctions] cable node */
+  const int32_t* ptr;       /* [n_junctions+1] */
+  const int32_t* patch;     /* [n_entries] myocardial slot */
+  int64_t n_targets;
+  const int32_t* tgt_slot;  /* [n_targets] distinct slots */
+  const int32_t* tgt_ptr;   /* [n_targets+1] */
+  const int32_t* tgt_entry; /* entries of each target slot */
+  double c_pmj;             /* mS */
+  double ai_p, ai_m, cm;    /* mm^2, mm^2, uF/mm^2 */
+} hp_pmj;
+
+int hp_pmj_couple_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
+                      double vmin, double vmax, void* stream);
+int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
+                      double vmin, double vmax, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Stimulus schedule on the device (replaces _stim_active, :1290-1301, and the
+ * host loop :1754-1759).  site_params[s*5 + 0..4] = onset, dur, period (0 =
+ * single pulse), n_pulses (< 0 = unlimited), amp [uA/mm^2].  group_sites[g]
+ * is the bitmask of sites of stimulus group g (group 0 = none).
+ * hp_step_advance increments *step_ctr and writes group_amp for the new time
+ * step*dt; call hp_stim_eval once after setting the counter.
+ * ------------------------------------------------------------------------- */
+int hp_stim_eval_f32(const double* site_params, int32_t n_sites, const uint32_t* group_sites,
+                     int32_t n_groups, const int64_t* step_ctr, double dt, float* group_amp,
+                     void* stream);
+int hp_stim_eval_f64(const double* site_params, int32_t n_sites, const uint32_t* group_sites,
+                     int32_t n_groups, const int64_t* step_ctr, double dt, double* group_amp,
+                     void* stream);
+int hp_step_advance_f32(const double* site_params, int32_t n_sites, const uint32_t* group_sites,
+                        int32_t n_groups, int64_t* step_ctr, double dt, float* group_amp,
+                        void* stream);
+int hp_step_advance_f64(const double* site_params, int32_t n_sites, const uint32_t* group_sites,
+                        int32_t n_groups, int64_t* step_ctr, double dt, double* group_amp,
+                        void* stream);
+
+/* Gather slot values into reference node order: out[i] = src[slot_of_node[i]]
+ * (Vm snapshots, vm_snapshots.npz schema, :1975-1984). */
+int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream);
+int hp_gather_f64(const double* src, const int32_t* idx, double* out, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HRDAYA_HIP_H */

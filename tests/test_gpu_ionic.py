@@ -1,0 +1,98 @@
+"""GPU parity of the TTP06 reaction kernel (hp_ttp06_step_*) against the oracle
+and the reference's golden vectors."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from hrdayapy_amd import _native as nat
+from oracle import ttp06 as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _gold():
+    return np.load(os.path.join(HERE, "golden", "ttp06_golden.npz"))
+
+
+def _step(V, st, dt, ct, dtype, soa=True, ki_base=0.0, nai_base=0.0):
+    dev = V.device
+    n = V.numel()
+    I = torch.empty(n, dtype=dtype, device=dev)
+    if soa:
+        buf = st.t().contiguous()  # (18, n)
+        sv, sn = n, 1
+    else:
+        buf = st.contiguous()      # (n, 18)
+        sv, sn = 1, 18
+    ctt = torch.as_tensor(ct, dtype=torch.uint8, device=dev)
+    nat.call(f"hp_ttp06_step_{nat.sfx(dtype)}", V.data_ptr(), buf.data_ptr(), I.data_ptr(),
+             ctt.data_ptr(), 0, None, None, n, sv, sn, dt, ki_base, nai_base, nat.stream_handle())
+    out = buf.t() if soa else buf
+    return I, out
+
+
+@pytest.mark.parametrize("layout", ["soa", "aos"])
+def test_ttp06_f64_matches_golden(cuda, layout):
+    print(f"[case] ttp06 f64 golden {layout}", flush=True)
+    g = _gold()
+    n = len(g["V"])
+    for code, ph in enumerate(["endo", "mid", "epi"]):
+        V = torch.from_numpy(g["V"]).to(cuda)
+        st = torch.from_numpy(g["state"]).to(cuda)
+        I, S = _step(V, st, 0.02, np.full(n, code), torch.float64, soa=layout == "soa")
+        np.testing.assert_allclose(I.cpu().numpy(), g[f"I_{ph}"], rtol=1e-10, atol=1e-11)
+        np.testing.assert_allclose(S.cpu().numpy(), g[f"S_{ph}"], rtol=1e-10, atol=1e-14)
+
+
+def test_ttp06_f32_matches_oracle(cuda):
+    print("[case] ttp06 f32 vs oracle", flush=True)
+    g = _gold()
+    n = len(g["V"])
+    ct = np.arange(n) % 3
+    Io, So = O.step_celltype(torch.from_numpy(g["V"]), torch.from_numpy(g["state"]), 0.02,
+                             torch.from_numpy(ct))
+    V = torch.from_numpy(g["V"]).float().to(cuda)
+    st = torch.from_numpy(g["state"]).float().to(cuda)
+    st[:, 0] -= 138.4
+    st[:, 1] -= 10.355
+    I, S = _step(V, st, 0.02, ct, torch.float32, ki_base=138.4, nai_base=10.355)
+    S = S.double().cpu()
+    S[:, 0] += 138.4
+    S[:, 1] += 10.355
+    # FP32 tolerance: ~1e-5 relative on currents (hardware exp/log/rcp, 1-2 ulp)
+    scale = np.maximum(np.abs(Io.numpy()), 1.0)
+    assert np.max(np.abs(I.double().cpu().numpy() - Io.numpy()) / scale) < 2e-4
+    np.testing.assert_allclose(S.numpy(), So.numpy(), rtol=2e-4, atol=2e-7)
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-6), (torch.float32, 0.5)])
+def test_single_cell_ap_trace(cuda, dtype, tol):
+    """Experiment-1 protocol on the GPU kernel vs the reference's own epi trace."""
+    print(f"[case] single-cell AP {dtype}", flush=True)
+    g = _gold()
+    ref = g["trace_epi"]
+    f32 = dtype == torch.float32
+    kb, nb = (138.4, 10.355) if f32 else (0.0, 0.0)
+    st = torch.tensor([[O.INIT[k] for k in O.STATE_NAMES]], dtype=torch.float64)
+    st[0, 0] -= kb
+    st[0, 1] -= nb
+    buf = st.t().contiguous().to(cuda, dtype)
+    V = torch.full((1,), O.V_REST, dtype=dtype, device=cuda)
+    I = torch.empty(1, dtype=dtype, device=cuda)
+    ct = torch.full((1,), 2, dtype=torch.uint8, device=cuda)
+    out = torch.empty(len(ref), dtype=dtype, device=cuda)
+    fn = f"hp_ttp06_step_{nat.sfx(dtype)}"
+    s = nat.stream_handle()
+    for k in range(len(ref) * 10):
+        nat.call(fn, V.data_ptr(), buf.data_ptr(), I.data_ptr(), ct.data_ptr(), 0, None, None, 1,
+                 1, 1, 0.02, kb, nb, s)
+        V.add_(0.02 * ((52.0 if k < 50 else 0.0) - I))
+        if k % 10 == 0:
+            out[k // 10] = V[0]
+    tr = out.double().cpu().numpy()
+    rms = float(np.sqrt(np.mean((tr - ref) ** 2)))
+    print(f"  Vm RMS vs reference trace = {rms:.3e} mV")
+    assert rms < tol

@@ -1,0 +1,102 @@
+"""Pins the CPU oracle of the stencil, Laplacian, CN/BiCGSTAB and stimulus (oracle/mesh.py)."""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import mesh as M
+from tests._problems import slab
+
+
+def _K(myo, N):
+    e = myo["edges"]
+    w = myo["edge_w"]
+    K = sp.coo_matrix((np.r_[w, w], (np.r_[e[:, 0], e[:, 1]], np.r_[e[:, 1], e[:, 0]])),
+                      shape=(N, N)).tocsr()
+    return K - sp.diags(np.asarray(K.sum(axis=1)).ravel())
+
+
+def test_row_sums_zero_and_symmetric():
+    S, phi, f = slab(10, 8, 6)
+    myo = M.build_myo_mesh(S, 0.25, f, 0.24, 0.08)
+    K = _K(myo, myo["N_myo"])
+    assert abs(K.sum(axis=1)).max() < 1e-15
+    assert abs(K - K.T).max() < 1e-18
+    L, dL = M.global_laplacian(0, myo["N_myo"], np.zeros((0, 2)), 0, 1, myo["edges"],
+                               myo["edge_w"], 140 * 0.25 ** 3, [], [], 0.0)
+    assert abs(L.sum(axis=1)).max() < 1e-12
+    np.testing.assert_allclose(L.diagonal(), dL, rtol=1e-14)
+
+
+def test_isotropic_reduces_to_seven_point():
+    S, phi, f = slab(8, 7, 5)
+    a = M.build_myo_mesh(S, 0.4, f, 0.13, 0.13)
+    b = M.build_myo_mesh_iso(S, 0.4, 0.13)
+    N = a["N_myo"]
+    Ka, Kb = _K(a, N), _K(b, N)
+    assert abs(Ka - Kb).max() < 1e-15
+
+
+def test_uniform_tensor_exact_on_quadratics():
+    # interior rows of L applied to a quadratic u = x^T Q x / 2 give tr(D Q)/A_M
+    h, A_M = 0.3, 140.0
+    S = np.ones((9, 9, 9), dtype=np.uint8)
+    fdir = np.array([0.6, 0.48, 0.64])
+    f = np.broadcast_to(fdir / np.linalg.norm(fdir), (9, 9, 9, 3)).astype(np.float64)
+    sl, st = 0.24, 0.08
+    myo = M.build_myo_mesh(S, h, f, sl, st)
+    N = myo["N_myo"]
+    L, _ = M.global_laplacian(0, N, np.zeros((0, 2)), 0, 1, myo["edges"], myo["edge_w"],
+                              A_M * h ** 3, [], [], 0.0)
+    Q = np.array([[1.0, 0.3, -0.2], [0.3, -0.5, 0.7], [-0.2, 0.7, 0.9]])
+    x = myo["vox_idx"] * h
+    u = 0.5 * np.einsum("ni,ij,nj->n", x, Q, x) + x @ np.array([0.1, -0.2, 0.3])
+    ff = fdir / np.linalg.norm(fdir)
+    D = st * np.eye(3) + (sl - st) * np.outer(ff, ff)
+    interior = np.all((myo["vox_idx"] > 0) & (myo["vox_idx"] < 8), axis=1)
+    np.testing.assert_allclose((L @ u)[interior], np.trace(D @ Q) / A_M, rtol=1e-10)
+
+
+def test_harmonic_face_mean_gives_series_flux():
+    # two voxels along x with different axial conductivity (fibre along x in the
+    # first, along y in the second): the face weight must be the conductance of
+    # two half-cells in series, h^2 / ((h/2)/s_a + (h/2)/s_b)
+    S = np.ones((2, 1, 1), dtype=np.uint8)
+    f = np.zeros((2, 1, 1, 3))
+    f[0, 0, 0, 0] = 1.0
+    f[1, 0, 0, 1] = 1.0
+    h, sl, st = 0.5, 0.24, 0.03
+    myo = M.build_myo_mesh(S, h, f, sl, st)
+    face = [k for k, (a, b) in enumerate(myo["edges"]) if {a, b} == {0, 1}]
+    assert len(face) == 1
+    np.testing.assert_allclose(myo["edge_w"][face[0]], h * h / ((h / 2) / sl + (h / 2) / st),
+                               rtol=1e-14)
+
+
+def test_bicgstab_matches_direct_solve():
+    S, phi, f = slab(10, 8, 6)
+    myo = M.build_myo_mesh(S, 0.25, f, 0.24, 0.08)
+    N = myo["N_myo"]
+    L, dL = M.global_laplacian(0, N, np.zeros((0, 2)), 0, 1, myo["edges"], myo["edge_w"],
+                               140 * 0.25 ** 3, [], [], 0.0)
+    A, Mi = M.cn_matrix(L, dL, 0.01 / 0.02, 0.5)
+    b = np.random.default_rng(0).standard_normal(N)
+    x = M.bicgstab(A, Mi, b, np.zeros(N), tol=1e-12, maxiter=500)
+    np.testing.assert_allclose(x, spla.spsolve(A.tocsc(), b), rtol=1e-9, atol=1e-12)
+
+
+def test_stim_active_schedule():
+    assert not M.stim_active(0.5, 1.0, 2.0, None, None)
+    assert M.stim_active(1.0, 1.0, 2.0, None, None)
+    assert not M.stim_active(3.0, 1.0, 2.0, None, None)
+    assert M.stim_active(101.5, 1.0, 2.0, 100.0, None)
+    assert not M.stim_active(101.5, 1.0, 2.0, 100.0, 1)
+    assert not M.stim_active(104.0, 1.0, 2.0, 100.0, None)
+
+
+def test_discretise_tree_segments():
+    nodes = np.array([[0, 0, 0], [1.0, 0, 0], [1.0, 0.45, 0]], dtype=np.float32)
+    cn, ce, bm = M.discretise_tree(nodes, np.array([[0, 1], [1, 2]]), 0.2)
+    assert [len(b) for b in bm] == [6, 3]
+    assert len(cn) == 3 + 4 + 1
+    seg = np.linalg.norm(cn[ce[:, 0]] - cn[ce[:, 1]], axis=1)
+    np.testing.assert_allclose(seg[:5], 0.2, rtol=1e-6)
