@@ -42,6 +42,16 @@ class HpPmj(C.Structure):
                 ("cm", c_f64)]
 
 
+class HpTorso(C.Structure):
+    _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
+                ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
+                ("gz", c_vp), ("diag", c_vp), ("pin", c_i64)]
+
+
+class HpPcgWs(C.Structure):
+    _fields_ = [("part", c_vp), ("counter", c_vp), ("out", c_vp), ("scalars", c_vp)]
+
+
 P = C.POINTER
 _SIGS = {
     "hp_abi_version": (C.c_int, []),
@@ -62,6 +72,18 @@ _SIGS = {
     "hp_stim_eval_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_step_advance_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_step_advance_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_poisson_spmv_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp]),
+    "hp_poisson_residual_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_vp]),
+    "hp_pcg_begin_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64, C.c_int,
+                                   C.c_int, c_vp]),
+    "hp_pcg_iterate_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, c_vp, P(HpPcgWs), C.c_int,
+                                     c_vp]),
+    "hp_refine_axpy_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_f64, c_vp]),
+    "hp_cast_scale_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_f64, c_vp]),
+    "hp_heart_lap6_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
+    "hp_bspm_source_f64": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f64, c_f64,
+                                     c_vp, c_i64, P(HpPcgWs), c_vp]),
+    "hp_leads_accumulate_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
 }

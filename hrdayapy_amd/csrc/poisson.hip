@@ -1,0 +1,511 @@
+// Torso Poisson forward solve: structured 7-point operator, Jacobi-PCG with
+// FP32 SpMV and FP64 inner products, FP64 residual for iterative refinement,
+// the heart source term and the FP64 electrode/lead accumulation.
+//
+// Operator (reference: _assemble_stiffness, hrdayapy/ecg/functions/
+// build_torso_grid.py:276-397, pinned as in compute_bspm.py:327-331):
+//   (A x)_c = diag_c x_c - sum_faces g_f x_nb,  g_f = harm(sigma_c, sigma_nb)/h^2,
+//   diag_c = sum of g_f over the six faces (Neumann at the box edge and air),
+//   row `pin` is the identity (reference potential).
+// Layout: every (i,j) column of the box stores the contiguous z-run
+// [k0, k0+len) that covers its active cells; cell (i,j,k) of the run has index
+// off[i*nj+j] + k - k0.  Cells inside a run that are not active (holes) have
+// zero conductances and diag 1, so they stay at zero.  g_x/g_y/g_z hold the
+// conductance of the +x/+y/+z face of each stored cell.
+//
+// The PCG loop runs without host round trips: alpha/beta, the stopping test
+// and an iteration counter live in device memory, every kernel returns at
+// once when the `done` flag is set, and reductions are finished by the last
+// block to arrive (fixed summation order: deterministic results).
+#include "common.h"
+#include "hrdaya_hip.h"
+
+namespace hp {
+
+constexpr int PWAVES = 4;          // waves per block
+constexpr int PBLOCK = 64 * PWAVES;
+
+struct ColMeta {
+  int k0, len, off;
+};
+
+__device__ __forceinline__ ColMeta col_meta(const hp_torso& t, int i, int j) {
+  ColMeta m;
+  if (i < 0 || i >= t.ni || j < 0 || j >= t.nj) {
+    m.k0 = 0;
+    m.len = 0;
+    m.off = 0;
+    return m;
+  }
+  const int c = i * t.nj + j;
+  m.k0 = t.col_k0[c];
+  m.len = t.col_len[c];
+  m.off = t.col_off[c];
+  return m;
+}
+
+__device__ __forceinline__ int cell_at(const ColMeta& m, int k) {
+  const int dk = k - m.k0;
+  return (dk >= 0 && dk < m.len) ? m.off + dk : -1;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block reduction of NV doubles followed by the last-block finish: the block
+// partials go to part[NV][gridDim.x]; the last block to arrive sums them in
+// block order into out[0..NV) and resets the arrival counter.
+template <int NV>
+__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double* part, int* counter,
+                                                  double* out) {
+  __shared__ double sh[NV][PWAVES];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double s = wave_sum(v[q]);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
+      part[q * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    const int old = atomicAdd(counter, 1);
+    is_last = (old == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += PBLOCK) s += part[q * gridDim.x + b];
+    s = wave_sum(s);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
+      out[q] = s;
+    }
+    *counter = 0;
+  }
+  return true;
+}
+
+// Iterate over (column, 64-wide k chunk) tasks with a grid-stride loop; one
+// task per wave.  F(cell, i, j, k, meta) is called for lanes inside the run.
+template <typename F>
+__device__ __forceinline__ void for_cells(const hp_torso& t, F&& f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kch = (t.nk + 63) >> 6;
+  const int64_t ntask = (int64_t)t.ni * t.nj * kch;
+  for (int64_t w = (int64_t)blockIdx.x * PWAVES + wave; w < ntask; w += (int64_t)gridDim.x * PWAVES) {
+    const int col = (int)(w / kch), kc = (int)(w % kch);
+    const int i = col / t.nj, j = col % t.nj;
+    const ColMeta m = col_meta(t, i, j);
+    const int dk = kc * 64 + lane;
+    if (dk < m.len) f(m.off + dk, i, j, m.k0 + dk, m);
+  }
+}
+
+// A x at one cell in FP32 with the stored diagonal (the inner PCG operator).
+__device__ __forceinline__ float apply_row_f32(const hp_torso& t, const float* __restrict__ x, int c,
+                                               int i, int j, int k, const ColMeta& m) {
+  if (c == t.pin) return x[c];
+  float acc = t.diag[c] * x[c];
+  if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * x[c + 1];
+  if (k - 1 >= m.k0) acc -= t.gz[c - 1] * x[c - 1];
+  int n;
+  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * x[n];
+  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * x[n];
+  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * x[n];
+  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * x[n];
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// PCG state in device memory (FP64 scalars)
+// ---------------------------------------------------------------------------
+enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR0, S_RR, S_TOL2, S_DONE, S_ITER, S_MAXIT, S_ZZ,
+       S_MODE, S_NCELL, S_NSC };
+
+__global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const float* __restrict__ x,
+                                                          float* __restrict__ y) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+    y[c] = apply_row_f32(t, x, c, i, j, k, m);
+  });
+}
+
+// FP64 row of the operator in zero-row-sum form: sum_f g_f (x_c - x_nb); a
+// cell without any conducting face (hole, isolated voxel) and the pinned cell
+// are identity rows.  The diagonal is the exact FP64 sum of the FP32 face
+// conductances, so every interior row sums to zero exactly.
+__device__ __forceinline__ double apply_row_f64(const hp_torso& t, const double* __restrict__ x,
+                                                int c, int i, int j, int k, const ColMeta& m) {
+  const double xc = x[c];
+  if (c == t.pin) return xc;
+  double acc = 0.0, gs = 0.0;
+  double g;
+  if (k + 1 < m.k0 + m.len && (g = t.gz[c]) != 0.0) { acc += g * (xc - x[c + 1]); gs += g; }
+  if (k - 1 >= m.k0 && (g = t.gz[c - 1]) != 0.0) { acc += g * (xc - x[c - 1]); gs += g; }
+  int n;
+  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0 && (g = t.gx[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0 && (g = t.gx[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0 && (g = t.gy[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0 && (g = t.gy[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  return gs == 0.0 ? xc : acc;
+}
+
+// r = b - A x in FP64 and ||r||^2, ||b||^2 -> out[0], out[1]
+__global__ void __launch_bounds__(PBLOCK) residual_f64_kernel(hp_torso t, const double* __restrict__ b,
+                                                              const double* __restrict__ x,
+                                                              double* __restrict__ r,
+                                                              double* part, int* counter,
+                                                              double* out) {
+  double v[2] = {0.0, 0.0};
+  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+    const double rc = b[c] - apply_row_f64(t, x, c, i, j, k, m);
+    r[c] = rc;
+    v[0] += rc * rc;
+    v[1] += b[c] * b[c];
+  });
+  block_reduce_last<2>(v, part, counter, out);
+}
+
+// PCG init for A d = r (FP32), d = 0: z = r/diag, p = z, rz, rr0 = ||r||^2.
+__global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, const float* __restrict__ r,
+                                                          float* __restrict__ d,
+                                                          float* __restrict__ p, double* part,
+                                                          int* counter, double* sc, double tol,
+                                                          int maxit, int mode) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+    const float rc = r[c];
+    const float z = rc / t.diag[c];
+    d[c] = 0.0f;
+    p[c] = z;
+    v[0] += (double)rc * (double)z;
+    v[1] += (double)rc * (double)rc;
+    v[2] += (double)z * (double)z;
+  });
+  double red[3];
+  if (block_reduce_last<3>(v, part, counter, red) && threadIdx.x == 0) {
+    sc[S_RZ] = red[0];
+    sc[S_RR0] = red[1];
+    sc[S_RR] = red[1];
+    sc[S_ZZ] = red[2];
+    sc[S_MODE] = (double)mode;
+    sc[S_NCELL] = (double)t.n_cells;
+    sc[S_TOL2] = tol * tol;
+    sc[S_ITER] = 0.0;
+    sc[S_MAXIT] = (double)maxit;
+    sc[S_BETA] = 0.0;
+    const bool conv = mode == 0 ? red[1] == 0.0 : red[2] <= tol * tol * (double)t.n_cells;
+    sc[S_DONE] = (conv || maxit <= 0) ? 1.0 : 0.0;
+  }
+}
+
+// p = z + beta p (skipped on the first iteration: beta = 0 and p = z already),
+// Ap = A p, pAp.  z of a neighbour is recomputed as r/diag on the fly.
+__global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const float* __restrict__ r,
+                                                          float* __restrict__ p,
+                                                          float* __restrict__ Ap, double* part,
+                                                          int* counter, double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const float beta = (float)sc[S_BETA];
+  const bool first = sc[S_ITER] == 0.0;
+  double v[1] = {0.0};
+  auto pnew = [&](int n) -> float { return first ? p[n] : r[n] / t.diag[n] + beta * p[n]; };
+  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+    const float pc = pnew(c);
+    float acc;
+    if (c == t.pin) {
+      acc = pc;
+    } else {
+      acc = t.diag[c] * pc;
+      if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * pnew(c + 1);
+      if (k - 1 >= m.k0) acc -= t.gz[c - 1] * pnew(c - 1);
+      int n;
+      if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * pnew(n);
+      if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * pnew(n);
+      if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * pnew(n);
+      if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * pnew(n);
+    }
+    Ap[c] = acc;
+    v[0] += (double)pc * (double)acc;
+  });
+  // p itself is not written here (other blocks still read the old p of their
+  // neighbours); pcg_update materialises the same z + beta*p next.
+  double red[1];
+  if (block_reduce_last<1>(v, part, counter, red) && threadIdx.x == 0) {
+    sc[S_PAP] = red[0];
+    if (!(fabs(red[0]) > 1e-300)) sc[S_DONE] = 2.0;
+    else sc[S_ALPHA] = sc[S_RZ] / red[0];
+  }
+}
+
+// p <- z + beta p (materialised), d += alpha p, r -= alpha Ap, rz, rr; finish:
+// beta, stopping test, iteration count.
+__global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* __restrict__ r,
+                                                            float* __restrict__ d,
+                                                            float* __restrict__ p,
+                                                            const float* __restrict__ Ap,
+                                                            double* part, int* counter,
+                                                            double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const float alpha = (float)sc[S_ALPHA];
+  const float beta = (float)sc[S_BETA];
+  const bool first = sc[S_ITER] == 0.0;
+  double v[3] = {0.0, 0.0, 0.0};
+  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+    const float inv = 1.0f / t.diag[c];
+    const float pc = first ? p[c] : r[c] * inv + beta * p[c];
+    p[c] = pc;
+    d[c] += alpha * pc;
+    const float rc = r[c] - alpha * Ap[c];
+    r[c] = rc;
+    const double z = (double)(rc * inv);
+    v[0] += (double)rc * z;
+    v[1] += (double)rc * (double)rc;
+    v[2] += z * z;
+  });
+  double red[3];
+  if (block_reduce_last<3>(v, part, counter, red) && threadIdx.x == 0) {
+    sc[S_BETA] = red[0] / sc[S_RZ];
+    sc[S_RZ] = red[0];
+    sc[S_RR] = red[1];
+    sc[S_ZZ] = red[2];
+    sc[S_ITER] += 1.0;
+    // mode 0: relative residual; mode 1: rms of z = M^-1 r (compute_bspm.py:222-256)
+    const bool conv = sc[S_MODE] == 0.0 ? red[1] <= sc[S_TOL2] * sc[S_RR0]
+                                        : red[2] <= sc[S_TOL2] * sc[S_NCELL];
+    if (conv || sc[S_ITER] >= sc[S_MAXIT]) sc[S_DONE] = 1.0;
+  }
+}
+
+// x64 += scale * d
+__global__ void axpy_f64_kernel(hp_torso t, double* __restrict__ x, const float* __restrict__ d,
+                                double scale) {
+  for_cells(t, [&](int c, int, int, int, const ColMeta&) { x[c] += scale * (double)d[c]; });
+}
+
+// r32 = r64 * scale
+__global__ void cast_scale_kernel(hp_torso t, const double* __restrict__ r64, float* __restrict__ r32,
+                                  double scale) {
+  for_cells(t, [&](int c, int, int, int, const ColMeta&) { r32[c] = (float)(r64[c] * scale); });
+}
+
+// ---------------------------------------------------------------------------
+// Heart source: 6-neighbour unit-weight graph Laplacian of Vm on the brick
+// layout (build_heart_laplacian, compute_bspm.py:88-119), then coarse rows
+// gather their fine voxels (fixed order) minus the mean leak
+// (compute_bspm.py:386-419).
+// ---------------------------------------------------------------------------
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ) heart_lap6_kernel(hp_bricks g, const float* __restrict__ V,
+                                                                  double* __restrict__ lap,
+                                                                  double vscale) {
+  constexpr int BV = BX * BY * BZ;
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t slot = b * BV + tid;
+  if (g.label[slot] == 0xFF) {
+    lap[slot] = 0.0;
+    return;
+  }
+  const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
+  const double vc = (double)V[slot];
+  double acc = 0.0;
+  const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int x = lx + off[q][0], y = ly + off[q][1], z = lz + off[q][2];
+    const int dbx = x < 0 ? -1 : (x >= BX ? 1 : 0);
+    const int dby = y < 0 ? -1 : (y >= BY ? 1 : 0);
+    const int dbz = z < 0 ? -1 : (z >= BZ ? 1 : 0);
+    int64_t nb = b;
+    if (dbx | dby | dbz) nb = g.nbr[b * 27 + (dbx + 1) * 9 + (dby + 1) * 3 + (dbz + 1)];
+    if (nb < 0) continue;
+    x -= dbx * BX;
+    y -= dby * BY;
+    z -= dbz * BZ;
+    const int64_t s = nb * BV + (x * BY + y) * BZ + z;
+    if (g.label[s] != 0xFF) acc += (double)V[s] - vc;
+  }
+  lap[slot] = acc * vscale;
+}
+
+// mean over the valid fine voxels of lap -> out[0]
+__global__ void __launch_bounds__(PBLOCK) masked_sum_kernel(const double* __restrict__ lap,
+                                                            const int32_t* __restrict__ slots,
+                                                            int64_t n, double* part, int* counter,
+                                                            double* out) {
+  double v[1] = {0.0};
+  for (int64_t q = (int64_t)blockIdx.x * PBLOCK + threadIdx.x; q < n; q += (int64_t)gridDim.x * PBLOCK)
+    v[0] += lap[slots[q]];
+  block_reduce_last<1>(v, part, counter, out);
+}
+
+// b[row] = coeff*(sum_{fine in row} lap - cnt*mean)/n_expected; b[pin] = 0
+__global__ void source_gather_kernel(const double* __restrict__ lap, const int32_t* __restrict__ rows,
+                                     const int32_t* __restrict__ ptr, const int32_t* __restrict__ fine,
+                                     int64_t n_rows_hit, const double* __restrict__ sum_n,
+                                     double inv_nvalid, double coeff, double inv_nexp,
+                                     double* __restrict__ b, int64_t pin) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_rows_hit) return;
+  const double mean = sum_n[0] * inv_nvalid;
+  double s = 0.0;
+  for (int e = ptr[q]; e < ptr[q + 1]; ++e) s += lap[fine[e]] - mean;
+  const int row = rows[q];
+  b[row] = row == pin ? 0.0 : coeff * s * inv_nexp;
+}
+
+// out[e] = sum_k w[k] * phi[idx[k]] over k in [ptr[e], ptr[e+1])  (FP64)
+__global__ void leads_kernel(const double* __restrict__ phi, const int32_t* __restrict__ ptr,
+                             const int32_t* __restrict__ idx, const double* __restrict__ w,
+                             double* __restrict__ out, int64_t n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0.0;
+  for (int k = ptr[e]; k < ptr[e + 1]; ++k) s += w[k] * phi[idx[k]];
+  out[e] = s;
+}
+
+static int grid_blocks(const hp_torso* t) {
+  const int64_t ntask = (int64_t)t->ni * t->nj * ((t->nk + 63) / 64);
+  int64_t nb = (ntask + PWAVES - 1) / PWAVES;
+  if (nb > 4096) nb = 4096;
+  return (int)(nb < 1 ? 1 : nb);
+}
+
+static bool torso_ok(const hp_torso* t, const char* what) {
+  if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > (1 << 20)) {
+    set_error("%s: bad torso grid", what);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+
+int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* stream) {
+  if (!torso_ok(t, "hp_poisson_spmv_f32")) return -1;
+  hipLaunchKernelGGL(spmv_f32_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,
+                     *t, x, y);
+  return check_launch("hp_poisson_spmv_f32");
+}
+
+int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x, double* r,
+                            hp_pcg_ws* ws, void* stream) {
+  if (!torso_ok(t, "hp_poisson_residual_f64") || !ws) return -1;
+  hipLaunchKernelGGL(residual_f64_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0,
+                     (hipStream_t)stream, *t, b, x, r, ws->part, ws->counter, ws->out);
+  return check_launch("hp_poisson_residual_f64");
+}
+
+int hp_pcg_begin_f32(const hp_torso* t, const float* r, float* d, float* p, hp_pcg_ws* ws,
+                     double tol, int max_iter, int mode, void* stream) {
+  if (!torso_ok(t, "hp_pcg_begin_f32") || !ws) return -1;
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,
+                     *t, r, d, p, ws->part, ws->counter, ws->scalars, tol, max_iter, mode);
+  return check_launch("hp_pcg_begin_f32");
+}
+
+int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap, hp_pcg_ws* ws,
+                       int n_iter, void* stream) {
+  if (!torso_ok(t, "hp_pcg_iterate_f32") || !ws || n_iter < 0) return -1;
+  const int nb = grid_blocks(t);
+  hipStream_t st = (hipStream_t)stream;
+  for (int it = 0; it < n_iter; ++it) {
+    hipLaunchKernelGGL(pcg_spmv_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, p, Ap, ws->part,
+                       ws->counter, ws->scalars);
+    hipLaunchKernelGGL(pcg_update_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, d, p, Ap,
+                       ws->part, ws->counter, ws->scalars);
+  }
+  return check_launch("hp_pcg_iterate_f32");
+}
+
+int hp_refine_axpy_f64(const hp_torso* t, double* x, const float* d, double scale, void* stream) {
+  if (!torso_ok(t, "hp_refine_axpy_f64")) return -1;
+  hipLaunchKernelGGL(axpy_f64_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,
+                     *t, x, d, scale);
+  return check_launch("hp_refine_axpy_f64");
+}
+
+int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double scale,
+                      void* stream) {
+  if (!torso_ok(t, "hp_cast_scale_f32")) return -1;
+  hipLaunchKernelGGL(cast_scale_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0,
+                     (hipStream_t)stream, *t, r64, r32, scale);
+  return check_launch("hp_cast_scale_f32");
+}
+
+int hp_heart_lap6_f32(const hp_bricks* g, const float* V, double* lap, double vscale,
+                      void* stream) {
+  if (!g || g->n_bricks <= 0) {
+    set_error("hp_heart_lap6_f32: empty brick grid");
+    return -1;
+  }
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  const dim3 grid((unsigned)g->n_bricks);
+  hipStream_t st = (hipStream_t)stream;
+  switch (key) {
+    case 444: hipLaunchKernelGGL((heart_lap6_kernel<4, 4, 4>), grid, dim3(64), 0, st, *g, V, lap, vscale); break;
+    case 844: hipLaunchKernelGGL((heart_lap6_kernel<8, 4, 4>), grid, dim3(128), 0, st, *g, V, lap, vscale); break;
+    case 884: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 4>), grid, dim3(256), 0, st, *g, V, lap, vscale); break;
+    case 888: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 8>), grid, dim3(512), 0, st, *g, V, lap, vscale); break;
+    default:
+      set_error("hp_heart_lap6_f32: unsupported brick shape");
+      return -1;
+  }
+  return check_launch("hp_heart_lap6_f32");
+}
+
+int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,
+                       const int32_t* rows, const int32_t* ptr, const int32_t* fine,
+                       int64_t n_rows_hit, double coeff, double n_expected, double* b,
+                       int64_t pin, hp_pcg_ws* ws, void* stream) {
+  if (!ws || n_valid <= 0 || n_expected <= 0.0) {
+    set_error("hp_bspm_source_f64: bad arguments");
+    return -1;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int nb = (int)((n_valid + PBLOCK - 1) / PBLOCK);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(masked_sum_kernel, dim3(nb), dim3(PBLOCK), 0, st, lap, valid_slots, n_valid,
+                     ws->part, ws->counter, ws->out);
+  int rc = check_launch("hp_bspm_source_f64/sum");
+  if (rc) return rc;
+  if (n_rows_hit > 0)
+    hipLaunchKernelGGL(source_gather_kernel, dim3(ceil_div(n_rows_hit, 256)), dim3(256), 0, st,
+                       lap, rows, ptr, fine, n_rows_hit, ws->out, 1.0 / (double)n_valid, coeff,
+                       1.0 / n_expected, b, pin);
+  return check_launch("hp_bspm_source_f64/gather");
+}
+
+int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
+                            const double* w, double* out, int64_t n_electrodes, void* stream) {
+  if (n_electrodes <= 0) return 0;
+  hipLaunchKernelGGL(leads_kernel, dim3(ceil_div(n_electrodes, 256)), dim3(256), 0,
+                     (hipStream_t)stream, phi, ptr, idx, w, out, n_electrodes);
+  return check_launch("hp_leads_accumulate_f64");
+}
+
+}  // extern "C"

@@ -1,0 +1,256 @@
+"""ECG forward solve on the GPU: heart source, torso Poisson PCG with FP64
+iterative refinement, electrodes and the 12-lead ECG.
+
+Reference: hrdayapy/ecg/functions/compute_bspm.py
+  * build_projection_map   :126-157  (heart voxel -> coarse torso row)
+  * solve_frame source     :381-419  (sigma_i/h^2 K Vm, masked, mean-removed,
+                                      scattered, / n_expected, pinned)
+  * _pcg_gpu               :164-263  (Jacobi PCG, warm start, rms metric)
+The refinement mode (FP32 PCG inside, FP64 residual outside, relative
+residual target) has no counterpart in the reference; the 'rms' mode follows
+its stopping rule.  Electrodes and limb/precordial leads are not in the
+reference either (it outputs raw surface potentials); they use the textbook
+Einthoven/Goldberger/Wilson definitions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ... import _native as nat
+from ...layout import BrickLayout
+
+
+class PoissonSolver:
+    """The torso operator of a TorsoGrid resident in HBM, plus PCG work vectors."""
+
+    def __init__(self, grid, device="cuda"):
+        nat.load()
+        self.grid = grid
+        self.dev = dev = torch.device(device)
+        L = grid.layout
+        self.n = n = int(L["n_cells"])
+        self._t = {k: torch.from_numpy(np.ascontiguousarray(L[k])).to(dev)
+                   for k in ("col_k0", "col_len", "col_off", "gx", "gy", "gz", "diag")}
+        ni, nj, nk = grid["shape"]
+        t = self._t
+        self.torso = nat.HpTorso(ni, nj, nk, 0, n, t["col_k0"].data_ptr(), t["col_len"].data_ptr(),
+                                 t["col_off"].data_ptr(), t["gx"].data_ptr(), t["gy"].data_ptr(),
+                                 t["gz"].data_ptr(), t["diag"].data_ptr(), int(L["pin"]))
+        self.part = torch.zeros(4 * 4096, dtype=torch.float64, device=dev)
+        self.counter = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.out = torch.zeros(4, dtype=torch.float64, device=dev)
+        self.sc = torch.zeros(16, dtype=torch.float64, device=dev)
+        self.ws = nat.HpPcgWs(self.part.data_ptr(), self.counter.data_ptr(), self.out.data_ptr(),
+                              self.sc.data_ptr())
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.r32, self.d32, self.p32, self.Ap32 = (torch.zeros(n, **f32) for _ in range(4))
+        self.r64 = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.last_stats = {}
+
+    # ------------------------------------------------------------ primitives
+    def spmv32(self, x, y):
+        nat.call("hp_poisson_spmv_f32", C.byref(self.torso), x.data_ptr(), y.data_ptr(),
+                 nat.stream_handle())
+
+    def residual(self, b, x, r=None):
+        """r = b - A x (FP64); returns (||r||, ||b||) as floats (one host sync)."""
+        r = self.r64 if r is None else r
+        nat.call("hp_poisson_residual_f64", C.byref(self.torso), b.data_ptr(), x.data_ptr(),
+                 r.data_ptr(), C.byref(self.ws), nat.stream_handle())
+        o = self.out[:2].cpu().numpy()
+        return float(np.sqrt(o[0])), float(np.sqrt(o[1]))
+
+    def _pcg(self, tol, max_iter, mode, check_every):
+        s = nat.stream_handle()
+        nat.call("hp_pcg_begin_f32", C.byref(self.torso), self.r32.data_ptr(), self.d32.data_ptr(),
+                 self.p32.data_ptr(), C.byref(self.ws), float(tol), int(max_iter), int(mode), s)
+        done_iters = 0
+        while True:
+            n = min(check_every, max_iter - done_iters) if max_iter > done_iters else 0
+            if n > 0:
+                nat.call("hp_pcg_iterate_f32", C.byref(self.torso), self.r32.data_ptr(),
+                         self.d32.data_ptr(), self.p32.data_ptr(), self.Ap32.data_ptr(),
+                         C.byref(self.ws), n, s)
+                done_iters += n
+            sc = self.sc[:11].cpu().numpy()
+            if sc[7] != 0.0 or n == 0:
+                return int(sc[8]), sc
+
+    # ---------------------------------------------------------------- solves
+    def solve_refined(self, b, x=None, tol=1e-8, inner_tol=1e-4, max_outer=30, max_inner=20000,
+                      check_every=32):
+        """Solve A x = b to ||b - A x|| <= tol ||b|| (FP64 residual) by FP32 PCG
+        corrections.  b, x: FP64 device vectors in cell order; x is updated in place."""
+        x = torch.zeros(self.n, dtype=torch.float64, device=self.dev) if x is None else x
+        inner_total, outer = 0, 0
+        rn, bn = self.residual(b, x)
+        while bn > 0 and rn > tol * bn and outer < max_outer:
+            nat.call("hp_cast_scale_f32", C.byref(self.torso), self.r64.data_ptr(),
+                     self.r32.data_ptr(), 1.0 / rn, nat.stream_handle())
+            it, _ = self._pcg(inner_tol, max_inner, 0, check_every)
+            nat.call("hp_refine_axpy_f64", C.byref(self.torso), x.data_ptr(), self.d32.data_ptr(),
+                     rn, nat.stream_handle())
+            inner_total += it
+            outer += 1
+            rn, bn = self.residual(b, x)
+        self.last_stats = dict(outer=outer, inner=inner_total, rel_res=rn / bn if bn else 0.0)
+        return x
+
+    def solve_rms(self, b, x=None, tol_V=1e-5, max_iter=500, check_every=16):
+        """The reference's stopping rule: FP32 Jacobi PCG from the warm start x until
+        rms(M^-1 r) < tol_V (compute_bspm.py:164-263).  Returns x (FP64 storage)."""
+        x = torch.zeros(self.n, dtype=torch.float64, device=self.dev) if x is None else x
+        self.residual(b, x)
+        nat.call("hp_cast_scale_f32", C.byref(self.torso), self.r64.data_ptr(),
+                 self.r32.data_ptr(), 1.0, nat.stream_handle())
+        it, sc = self._pcg(tol_V, max_iter, 1, check_every)
+        nat.call("hp_refine_axpy_f64", C.byref(self.torso), x.data_ptr(), self.d32.data_ptr(), 1.0,
+                 nat.stream_handle())
+        self.last_stats = dict(inner=it, rms_z=float(np.sqrt(sc[10] / self.n)))
+        return x
+
+
+def projection_rows(coords_mm, R, t, scale, origin_mm, dx_mm, id_map, shape):
+    """Heart node -> torso row (-1 if outside), as build_projection_map (:126-157)."""
+    torso = (np.asarray(R) @ (np.asarray(coords_mm, dtype=np.float64) * scale).T).T + np.asarray(t)
+    ijk = np.floor((torso - np.asarray(origin_mm)) / dx_mm).astype(np.int64)
+    ni, nj, nk = shape
+    inb = ((ijk[:, 0] >= 0) & (ijk[:, 0] < ni) & (ijk[:, 1] >= 0) & (ijk[:, 1] < nj)
+           & (ijk[:, 2] >= 0) & (ijk[:, 2] < nk))
+    proj = np.full(len(torso), -1, dtype=np.int64)
+    flat = (ijk[inb, 0] * nj + ijk[inb, 1]) * nk + ijk[inb, 2]
+    proj[np.nonzero(inb)[0]] = id_map[flat]
+    return proj
+
+
+class HeartSource:
+    """Per-frame torso right-hand side from the heart's Vm on the brick layout."""
+
+    def __init__(self, layout: BrickLayout, coords_mm, grid, registration, sigma_i, h_fine_mm,
+                 device="cuda", bricks=None):
+        self.dev = dev = torch.device(device)
+        self.lay = layout
+        R, t, sc = registration["R"], registration["t"], float(registration["scale"])
+        proj = projection_rows(coords_mm, R, t, sc, grid["origin_mm"], grid["dx_mm"],
+                               grid["active_id_map"], grid["shape"])
+        valid = proj >= 0
+        self.n_valid = int(valid.sum())
+        if self.n_valid == 0:
+            raise ValueError("no heart voxel projects into the torso's active cells")
+        slots = layout.slot_of_node[valid].astype(np.int32)
+        cells = grid.cell_of_row[proj[valid]]
+        order = np.argsort(cells, kind="stable")
+        rows_c, first = np.unique(cells[order], return_index=True)
+        self.valid_slots = torch.from_numpy(slots).to(dev)
+        self.rows = torch.from_numpy(rows_c.astype(np.int32)).to(dev)
+        self.ptr = torch.from_numpy(np.append(first, len(order)).astype(np.int32)).to(dev)
+        self.fine = torch.from_numpy(slots[order]).to(dev)
+        h_m = h_fine_mm * 1e-3
+        self.coeff = sigma_i / h_m ** 2
+        self.n_expected = (grid["dx_mm"] / h_fine_mm) ** 3
+        self.pin = int(grid.layout["pin"])
+        self.lap = torch.zeros(layout.n_slots, dtype=torch.float64, device=dev)
+        if bricks is None:
+            self._label = torch.from_numpy(layout.labels(np.zeros(layout.n_nodes, np.uint8))).to(dev)
+            self._nbr = torch.from_numpy(layout.nbr).to(dev)
+            bricks = nat.HpBricks(layout.n_bricks, *layout.brick, 0, self._nbr.data_ptr(),
+                                  self._label.data_ptr(), None, None)
+        self.bricks = bricks
+        self.n_miss = int((~valid).sum())
+
+    def __call__(self, V_slots_f32, b64, ws):
+        """b64[hit cells] <- source of V (mV, FP32, slot order); other cells untouched."""
+        s = nat.stream_handle()
+        nat.call("hp_heart_lap6_f32", C.byref(self.bricks), V_slots_f32.data_ptr(),
+                 self.lap.data_ptr(), 1e-3, s)
+        nat.call("hp_bspm_source_f64", self.lap.data_ptr(), self.valid_slots.data_ptr(),
+                 self.n_valid, self.rows.data_ptr(), self.ptr.data_ptr(), self.fine.data_ptr(),
+                 int(self.rows.numel()), self.coeff, self.n_expected, b64.data_ptr(), self.pin,
+                 C.byref(ws), s)
+
+
+LEAD_NAMES = ["I", "II", "III", "aVR", "aVL", "aVF", "V1", "V2", "V3", "V4", "V5", "V6"]
+LIMB = ["RA", "LA", "LL"]
+PRECORDIAL = ["V1", "V2", "V3", "V4", "V5", "V6"]
+
+
+def leads_from_electrodes(phi):
+    """12 leads from electrode potentials (dict name -> array), textbook definitions."""
+    RA, LA, LL = phi["RA"], phi["LA"], phi["LL"]
+    wct = (RA + LA + LL) / 3.0
+    out = {"I": LA - RA, "II": LL - RA, "III": LL - LA, "aVR": RA - 0.5 * (LA + LL),
+           "aVL": LA - 0.5 * (RA + LL), "aVF": LL - 0.5 * (RA + LA)}
+    for v in PRECORDIAL:
+        out[v] = phi[v] - wct
+    return out
+
+
+class Electrodes:
+    """Electrodes on the body surface: each is the mean of the k surface cells
+    nearest to its position (FP64 accumulation on the device)."""
+
+    def __init__(self, grid, positions_mm: dict, k=4, device="cuda"):
+        from scipy.spatial import cKDTree
+        self.names = list(positions_mm)
+        xyz = grid["surface_xyz"].astype(np.float64)
+        cells = grid.cell_of_row[grid["surface_rows"]]
+        kk = min(k, len(xyz))
+        _, nn = cKDTree(xyz).query(np.array([positions_mm[n] for n in self.names]), k=kk)
+        nn = np.asarray(nn).reshape(len(self.names), kk)
+        self.cells = cells[nn]
+        dev = torch.device(device)
+        self.ptr = torch.arange(0, kk * len(self.names) + 1, kk, dtype=torch.int32, device=dev)
+        self.idx = torch.from_numpy(self.cells.ravel().astype(np.int32)).to(dev)
+        self.w = torch.full((kk * len(self.names),), 1.0 / kk, dtype=torch.float64, device=dev)
+        self.out = torch.zeros(len(self.names), dtype=torch.float64, device=dev)
+
+    def __call__(self, phi64):
+        nat.call("hp_leads_accumulate_f64", phi64.data_ptr(), self.ptr.data_ptr(),
+                 self.idx.data_ptr(), self.w.data_ptr(), self.out.data_ptr(), len(self.names),
+                 nat.stream_handle())
+        return self.out
+
+
+def standard_electrodes(grid, heart_centre_mm, n_bspm=256, seed=0):
+    """Electrode positions on a torso: limb electrodes RA/LA/LL, precordial
+    V1-V6 around the heart on the anterior surface, and n_bspm seeded sites on
+    a regular (azimuth, height) grid over the outer surface."""
+    xyz = grid["surface_xyz"].astype(np.float64)
+    c = xyz.mean(axis=0)
+    ext = xyz.max(axis=0) - xyz.min(axis=0)
+    lo, hi = xyz.min(axis=0), xyz.max(axis=0)
+
+    def ray(direction, origin):
+        d = np.asarray(direction, float)
+        d /= np.linalg.norm(d)
+        v = xyz - origin
+        along = v @ d
+        perp = np.linalg.norm(v - along[:, None] * d, axis=1)
+        score = perp - 0.05 * along
+        score[along <= 0] = np.inf
+        return xyz[int(np.argmin(score))]
+
+    hc = np.asarray(heart_centre_mm, float)
+    pos = {}
+    # frame: x = left-right, y = anterior(-)/posterior(+), z = foot(-)/head(+)
+    top = c + np.array([0, 0, 0.35 * ext[2]])
+    bottom = c - np.array([0, 0, 0.35 * ext[2]])
+    pos["RA"] = ray([-1, 0, 0.3], top)
+    pos["LA"] = ray([1, 0, 0.3], top)
+    pos["LL"] = ray([0.3, 0, -1], bottom)
+    for i, ang in enumerate(np.deg2rad([-20, 0, 20, 40, 60, 80])):
+        pos[f"V{i + 1}"] = ray([np.sin(ang), -np.cos(ang), 0.0], hc)
+    rng = np.random.default_rng(seed)
+    nz = int(np.sqrt(n_bspm))
+    na = n_bspm // nz
+    k = 0
+    for iz in range(nz):
+        z = lo[2] + (hi[2] - lo[2]) * (0.1 + 0.8 * (iz + 0.5) / nz)
+        for ia in range(na):
+            a = 2 * np.pi * (ia + 0.5 + 0.2 * rng.uniform(-1, 1)) / na
+            pos[f"E{k:03d}"] = ray([np.cos(a), np.sin(a), 0.0], np.array([c[0], c[1], z]))
+            k += 1
+    return pos

@@ -142,3 +142,33 @@ def purkinje_tree(geo, n_pmj=800, seed=0, step_mm=0.55, root_frac=0.85):
     nodes = np.asarray(nodes)[:, [1, 0, 2]]  # (row,col,depth) -> (col,row,depth)
     act = np.asarray(dist) * h / 2.5          # ~2.5 mm/ms conduction
     return nodes, np.asarray(elements, dtype=np.int64), act
+
+
+TORSO_SIGMA = {1: 0.2, 2: 0.2, 3: 0.05, 4: 0.7}  # bulk, skeletal muscle, lungs, blood [S/m]
+
+
+def torso(shape=(384, 384, 256), dx_mm=1.5, heart_centre_frac=(0.58, 0.42, 0.55)):
+    """Synthetic torso label volume: nested ellipsoids of bulk tissue (1),
+    a skeletal-muscle rim (2), two lungs (3) and a blood pool (4) at the heart
+    position.  Returns (labels, origin_mm, heart_centre_mm); the outer
+    ellipsoid is inscribed in the box (about 52 % of the cells are tissue)."""
+    ni, nj, nk = shape
+    ext = np.array(shape, float) * dx_mm
+    c = ext / 2.0
+    ax = [(np.arange(n) + 0.5) * dx_mm for n in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij", sparse=True)
+    semi = 0.49 * ext
+
+    def ell(cen, s):
+        return (((X - cen[0]) / s[0]) ** 2 + ((Y - cen[1]) / s[1]) ** 2
+                + ((Z - cen[2]) / s[2]) ** 2) <= 1.0
+
+    lab = np.zeros(shape, dtype=np.uint8)
+    lab[ell(c, semi)] = 2
+    lab[ell(c, semi * 0.93)] = 1
+    for sgn in (-1, 1):
+        lab[ell(c + np.array([sgn * 0.22 * ext[0], 0.05 * ext[1], 0.08 * ext[2]]),
+                np.array([0.14, 0.2, 0.26]) * ext)] = 3
+    hc = np.asarray(heart_centre_frac) * ext
+    lab[ell(hc, np.array([0.05, 0.05, 0.07]) * ext)] = 4
+    return lab, np.zeros(3), hc

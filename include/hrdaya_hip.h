@@ -206,6 +206,78 @@ int hp_step_advance_f64(const double* site_params, int32_t n_sites, const uint32
 int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream);
 int hp_gather_f64(const double* src, const int32_t* idx, double* out, int64_t n, void* stream);
 
+
+/* ---------------------------------------------------------------------------
+ * Torso Poisson forward solve (replaces _assemble_stiffness's operator,
+ * build_torso_grid.py:276-397 with the pin of compute_bspm.py:327-331, and
+ * the _pcg_gpu solve, compute_bspm.py:164-263).
+ *
+ * Column-run layout of the (ni,nj,nk) box: column (i,j) stores the z-run
+ * [col_k0, col_k0+col_len) at cells col_off.. (holes inside a run have zero
+ * conductance and diag 1).  gx/gy/gz: conductance of the +x/+y/+z face of
+ * each stored cell, harm(sigma_a, sigma_b)/h^2 [S/m^3]; diag: sum of the six
+ * face conductances; row `pin` is the identity.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_torso {
+  int32_t ni, nj, nk, pad_;
+  int64_t n_cells;
+  const int32_t* col_k0;   /* [ni*nj] */
+  const int32_t* col_len;  /* [ni*nj] */
+  const int32_t* col_off;  /* [ni*nj] */
+  const float* gx;
+  const float* gy;
+  const float* gz;
+  const float* diag;
+  int64_t pin;             /* stored index of the pinned cell, or -1 */
+} hp_torso;
+
+/* Reduction workspace: part >= 3*4096 doubles, counter = one int (zero at
+ * creation; every kernel leaves it at zero), out >= 2 doubles, scalars >= 16
+ * doubles (PCG state: rz, pAp, alpha, beta, rr0, rr, tol^2, done, iter, maxit). */
+typedef struct hp_pcg_ws {
+  double* part;
+  int* counter;
+  double* out;
+  double* scalars;
+} hp_pcg_ws;
+
+/* y = A x in FP32. */
+int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* stream);
+/* r = b - A x in FP64; ws->out[0] = ||r||^2, ws->out[1] = ||b||^2. */
+int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x, double* r,
+                            hp_pcg_ws* ws, void* stream);
+/* Jacobi-PCG for A d = r (FP32 vectors, FP64 inner products), d = 0 initially.
+ * begin: sets p = M^-1 r and the device-side state; iterate: n_iter iterations,
+ * each a no-op once converged or max_iter was reached.  mode 0 stops at
+ * ||r|| <= tol*||r0||; mode 1 at rms(M^-1 r) <= tol (the reference's 'rms'
+ * metric, compute_bspm.py:222-256).  ws->scalars then holds rz, pAp, alpha,
+ * beta, ||r0||^2, ||r||^2, tol^2, done, iterations, max_iter, ||M^-1 r||^2.
+ * r is overwritten. */
+int hp_pcg_begin_f32(const hp_torso* t, const float* r, float* d, float* p, hp_pcg_ws* ws,
+                     double tol, int max_iter, int mode, void* stream);
+int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap,
+                       hp_pcg_ws* ws, int n_iter, void* stream);
+/* Iterative refinement helpers: x += scale*d (FP64), r32 = scale*r64. */
+int hp_refine_axpy_f64(const hp_torso* t, double* x, const float* d, double scale, void* stream);
+int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double scale,
+                      void* stream);
+
+/* Heart source (replaces build_heart_laplacian + solve_frame's source,
+ * compute_bspm.py:88-119, 381-419): lap = vscale * (6-neighbour unit-weight
+ * graph Laplacian of V) on the brick layout (FP64 out); then
+ * b[rows[q]] = coeff * sum_{e in ptr[q]..} (lap[fine[e]] - mean) / n_expected
+ * with mean over valid_slots, and b[pin] = 0.  Rows not listed are untouched. */
+int hp_heart_lap6_f32(const hp_bricks* g, const float* V, double* lap, double vscale,
+                      void* stream);
+int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,
+                       const int32_t* rows, const int32_t* ptr, const int32_t* fine,
+                       int64_t n_rows_hit, double coeff, double n_expected, double* b,
+                       int64_t pin, hp_pcg_ws* ws, void* stream);
+/* Electrode potentials, FP64: out[e] = sum_k w[k]*phi[idx[k]], k in [ptr[e], ptr[e+1]).
+ * No counterpart in the reference (it outputs raw surface potentials only). */
+int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
+                            const double* w, double* out, int64_t n_electrodes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
