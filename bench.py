@@ -5,7 +5,8 @@ Workload (default): the 2M-voxel synthetic biventricular shell -- two
 truncated confocal ellipsoids voxelised at 0.4 mm in a 256^3 grid, rule-based
 helix fibres (+60 -> -60 deg), sigma_l = 0.24, sigma_t = 0.08 S/m, endo/mid/epi
 TTP06 by transmural thirds, a seeded Purkinje tree with 800 PMJs coupled at
-c_pmj = 2 mS (500 Ohm), FP32, dt = 0.02 ms, plain Rush-Larsen.  One "step" is
+c_pmj = 2 mS (500 Ohm), FP32, dt = 0.02 ms, plain Rush-Larsen; stimulus at the
+Purkinje root and at an LV endocardial apical site (2 ms, 52 pA/pF).  One "step" is
 one 0.02 ms time step of the whole coupled system (myocardium + Purkinje +
 PMJ + stimulus), replayed from captured HIP graphs.
 
@@ -45,7 +46,9 @@ def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None):
     kw = dict(voxel_size=h, dx_p=0.2, sigma_P=0.45, sigma_l=0.24, sigma_t=0.08, Cm=0.01,
               A_M=140.0, R_P=0.015, c_pmj=2.0, n_pmj=1, stim_len_mm=2.0, T=600.0,
               phi_endo_max=1.0 / 3.0, phi_epi_min=2.0 / 3.0,
-              stim_protocol=[dict(name="av_node", target="root", amp=12.0, dur=2.0, onset=0.0)])
+              ectopic_region=G.apical_stimulus_region(geo, 3.0),
+              stim_protocol=[dict(name="av_node", target="root", amp=12.0, dur=2.0, onset=0.0),
+                             dict(name="apex", target="ectopic", amp=0.52, dur=2.0, onset=0.0)])
     eng, meta = build_coupled_system(nodes, el, act, geo["S"], geo["phi"], geo["fibre_dir"],
                                      dt=0.02, dtype=dtype or torch.float32, device="cuda",
                                      brick=brick, **kw)

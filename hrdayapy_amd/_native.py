@@ -42,6 +42,11 @@ class HpPmj(C.Structure):
                 ("cm", c_f64)]
 
 
+class HpPmjAt(C.Structure):
+    _fields_ = [("at", c_vp), ("V_prev", c_vp), ("step_ctr", c_vp), ("dt", c_f64),
+                ("at_thr", c_f64)]
+
+
 class HpTorso(C.Structure):
     _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
                 ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
@@ -66,8 +71,10 @@ _SIGS = {
     "hp_stencil19_apply_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
     "hp_cable_step_f32": (C.c_int, [P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
     "hp_cable_step_f64": (C.c_int, [P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
-    "hp_pmj_couple_f32": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_vp]),
-    "hp_pmj_couple_f64": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, c_vp]),
+    "hp_pmj_couple_f32": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, P(HpPmjAt),
+                                    c_vp]),
+    "hp_pmj_couple_f64": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, P(HpPmjAt),
+                                    c_vp]),
     "hp_stim_eval_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_stim_eval_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_step_advance_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),

@@ -164,7 +164,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
   Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
   V_out[slot] = Vn;
-  if (p.at) {
+  // PMJ target voxels (label bit 7) are tested after the PMJ exchange instead
+  if (p.at && !(lab & 0x80)) {
     const T thr = T(p.at_thr);
     if (Vc < thr && Vn >= thr) {
       T* at = (T*)p.at;
@@ -288,14 +289,24 @@ __global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __r
 
 template <typename T>
 __global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __restrict__ scratch,
-                                  double vmin, double vmax) {
+                                  double vmin, double vmax, hp_pmj_at at) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= j.n_targets) return;
   T acc = T(0);
   for (int k = j.tgt_ptr[q]; k < j.tgt_ptr[q + 1]; ++k) acc += scratch[j.tgt_entry[k]];
   const int s = j.tgt_slot[q];
-  const T v = V_myo[s] + acc;
-  V_myo[s] = M<T>::mn(M<T>::mx(v, T(vmin)), T(vmax));
+  T v = V_myo[s] + acc;
+  v = M<T>::mn(M<T>::mx(v, T(vmin)), T(vmax));
+  V_myo[s] = v;
+  if (at.at) {  // first activation between V^n (start of step) and the final V^{n+1}
+    const T v0 = ((const T*)at.V_prev)[s];
+    const T thr = T(at.at_thr);
+    T* a = (T*)at.at;
+    if (v0 < thr && v >= thr && a[s] != a[s]) {
+      const int64_t step = *at.step_ctr;
+      a[s] = (T)((double)(step - 1) * at.dt) + T(at.dt) * (thr - v0) / (v - v0);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +375,7 @@ static int cable_step(const hp_cable* c, const T* V_in, T* V_out, T* state, int6
 
 template <typename T>
 static int pmj_couple(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, double vmin,
-                      double vmax, void* stream) {
+                      double vmax, const hp_pmj_at* at, void* stream) {
   if (!j || j->n_junctions < 0) {
     set_error("hp_pmj_couple: bad arguments");
     return -1;
@@ -377,7 +388,7 @@ static int pmj_couple(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt,
   if (rc) return rc;
   if (j->n_targets > 0)
     hipLaunchKernelGGL(pmj_target_kernel<T>, dim3(ceil_div(j->n_targets, 128)), dim3(128), 0, st,
-                       *j, V_myo, scratch, vmin, vmax);
+                       *j, V_myo, scratch, vmin, vmax, at ? *at : hp_pmj_at{});
   return check_launch("hp_pmj_couple/target");
 }
 
@@ -435,12 +446,12 @@ int hp_cable_step_f64(const hp_cable* c, const double* V_in, double* V_out, doub
   return cable_step<double>(c, V_in, V_out, state, ld, p, stream);
 }
 int hp_pmj_couple_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
-                      double vmin, double vmax, void* stream) {
-  return pmj_couple<float>(j, V_pk, V_myo, scratch, dt, vmin, vmax, stream);
+                      double vmin, double vmax, const hp_pmj_at* at, void* stream) {
+  return pmj_couple<float>(j, V_pk, V_myo, scratch, dt, vmin, vmax, at, stream);
 }
 int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
-                      double vmin, double vmax, void* stream) {
-  return pmj_couple<double>(j, V_pk, V_myo, scratch, dt, vmin, vmax, stream);
+                      double vmin, double vmax, const hp_pmj_at* at, void* stream) {
+  return pmj_couple<double>(j, V_pk, V_myo, scratch, dt, vmin, vmax, at, stream);
 }
 int hp_stim_eval_f32(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
                      const int64_t* ctr, double dt, float* amp, void* stream) {

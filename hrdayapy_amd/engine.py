@@ -87,7 +87,11 @@ class MonodomainEngine:
 
         # myocardium
         L = layout
-        self.label = torch.from_numpy(L.labels(np.asarray(celltype_myo), gid[self.Np:])).to(dev)
+        lab = L.labels(np.asarray(celltype_myo), gid[self.Np:])
+        if pmj and self.Np and pmj["c_pmj"] != 0.0 and len(pmj["ids"]):
+            tgt_nodes = np.unique(np.concatenate([np.asarray(p, np.int64) for p in pmj["patches"]]))
+            lab[L.slot_of_node[tgt_nodes]] |= np.uint8(0x80)
+        self.label = torch.from_numpy(lab).to(dev)
         self.D6 = torch.from_numpy(L.to_slots(np.asarray(D6, dtype=np.float64))).to(dev, dt_)
         self.nbr = torch.from_numpy(L.nbr).to(dev)
         self.slot_of_node = torch.from_numpy(L.slot_of_node).to(dev)
@@ -144,6 +148,9 @@ class MonodomainEngine:
                                  nat.ptr(ent_), len(tgt), nat.ptr(tgt_), nat.ptr(tptr_),
                                  nat.ptr(tent_), float(pmj["c_pmj"]), float(pmj["Ai_P"]),
                                  float(pmj["Ai_M"]), self.Cm)
+            self.pmj_at = [nat.HpPmjAt(nat.ptr(self.at), nat.ptr(self.V[k]),
+                                       nat.ptr(self.step_ctr), self.dt, float(at_thr))
+                           for k in range(2)]
         self.p = 0
         self._eval_stim()
 
@@ -182,7 +189,8 @@ class MonodomainEngine:
                      nat.ptr(self.Vpk[b]), nat.ptr(self.state_pk), self.Np, C.byref(self.p_pk), s)
         if self.pmj is not None:
             nat.call(f"hp_pmj_couple_{self.sfx}", C.byref(self.pmj), nat.ptr(self.Vpk[b]),
-                     nat.ptr(self.V[b]), nat.ptr(self.pmj_scratch), self.dt, self.vmin, self.vmax, s)
+                     nat.ptr(self.V[b]), nat.ptr(self.pmj_scratch), self.dt, self.vmin, self.vmax,
+                     C.byref(self.pmj_at[a]), s)
         nat.call(f"hp_step_advance_{self.sfx}", nat.ptr(self.site_params), self.n_sites,
                  nat.ptr(self.group_sites), self.n_groups, nat.ptr(self.step_ctr), self.dt,
                  nat.ptr(self.group_amp), s)

@@ -172,3 +172,18 @@ def torso(shape=(384, 384, 256), dx_mm=1.5, heart_centre_frac=(0.58, 0.42, 0.55)
     hc = np.asarray(heart_centre_frac) * ext
     lab[ell(hc, np.array([0.05, 0.05, 0.07]) * ext)] = 4
     return lab, np.zeros(3), hc
+
+
+def apical_stimulus_region(geo, radius_mm=3.0):
+    """Myocardial voxels within radius_mm of the LV endocardial apex: an
+    ectopic site that starts the ventricular wave (the thin Purkinje cable of
+    the reference's parameters cannot drive the tissue load through the PMJs)."""
+    S = geo["S"].astype(bool)
+    lv = np.argwhere(geo["cavity_lv"])
+    apex = lv[np.argmin(lv[:, 2])].astype(float)
+    vox = np.argwhere(S)
+    d = np.linalg.norm(vox - apex, axis=1) * geo["h"]
+    region = np.zeros(S.shape, dtype=bool)
+    sel = vox[d <= radius_mm]
+    region[sel[:, 0], sel[:, 1], sel[:, 2]] = True
+    return region

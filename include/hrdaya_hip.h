@@ -73,7 +73,7 @@ int hp_ttp06_step_f64(const double* V, double* state, double* I_ion, const uint8
  * myocardial voxel are stored.  Voxel (lx,ly,lz) of brick b has slot
  * b*bvol + (lx*by + ly)*bz + lz.  Per slot:
  *   label  uint8: 0xFF = not myocardium; else bits 0-1 phenotype code,
- *          bits 2-6 stimulus group (0 = none)
+ *          bits 2-6 stimulus group (0 = none), bit 7 = PMJ target voxel
  *   D6     conductivity tensor D00,D11,D22,D01,D02,D12 [mS/mm], SoA [6][n_slots]
  *   inst   uint16 ensemble instance (or NULL: instance 0)
  * nbr[b*27 + (dx+1)*9 + (dy+1)*3 + (dz+1)] is the brick at offset (dx,dy,dz),
@@ -175,10 +175,20 @@ typedef struct hp_pmj {
   double ai_p, ai_m, cm;    /* mm^2, mm^2, uF/mm^2 */
 } hp_pmj;
 
+/* Optional first-activation bookkeeping for the junction voxels: these carry
+ * label bit 7, the fused myocardium step skips them, and the exchange tests
+ * V_prev (V^n) against the final V^{n+1} instead (NULL at = off). */
+typedef struct hp_pmj_at {
+  void* at;                 /* T[n_slots] */
+  const void* V_prev;       /* T[n_slots], V^n */
+  const int64_t* step_ctr;
+  double dt, at_thr;
+} hp_pmj_at;
+
 int hp_pmj_couple_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
-                      double vmin, double vmax, void* stream);
+                      double vmin, double vmax, const hp_pmj_at* at, void* stream);
 int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
-                      double vmin, double vmax, void* stream);
+                      double vmin, double vmax, const hp_pmj_at* at, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Stimulus schedule on the device (replaces _stim_active, :1290-1301, and the
