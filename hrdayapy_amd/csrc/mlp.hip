@@ -1,0 +1,459 @@
+// Learned ionic surrogate: BF16 MFMA kernels (gfx950 v_mfma_f32_16x16x32_bf16,
+// FP32 accumulation) for training and for the fused in-tissue inference.
+//
+// Network: 22 inputs (V, the 18 TTP06 state variables, phenotype one-hot) ->
+// 128 -> 128 -> 128 (SiLU) -> 19 outputs (increments of V and of the 18
+// state variables over one macro-step of 10 reaction substeps).  Inputs and
+// outputs are normalised per feature; gates/concentrations with a wide
+// dynamic range travel as logarithms (feature flags in hp_mlp_norm).
+//
+// MFMA operand layout (16x16x32): lane l holds 8 bf16 of A row (l%16) at
+// k = 8*(l/16)..+8, 8 bf16 of B column (l%16) at the same k, and 4 floats of
+// C column (l%16) at rows 4*(l/16)..+4.
+//
+// Inference computes H^T = W^T X^T with neurons as rows and voxels as
+// columns, so the C registers of one layer become the B operand of the next
+// without leaving registers: the 8 bf16 of a B fragment at K-step j are the 4
+// rows of C tile 2j followed by the 4 rows of C tile 2j+1.  The weight
+// matrices of layers 2-4 are therefore stored in LDS with their input index
+// permuted the same way (perm_k below); the host keeps them in natural order.
+#include "common.h"
+#include "hrdaya_hip.h"
+#include "ttp06_cell.h"
+
+namespace hp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma16(const bf16x8& a, const bf16x8& b, const floatx4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ __bf16 f2bf(float x) { return (__bf16)x; }
+
+// ---------------------------------------------------------------------------
+// Generic GEMM for training: C[m,n] = alpha * sum_k A(m,k) B(k,n) + beta * C[m,n]
+// A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bf16), C fp32 row-major (ldc).
+// Optional bias[n] added (bias_mode 1) -- used for the forward layers.
+// Block tile 64x64, 4 waves of 32x32, K step 32, operands staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 32;
+
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, const __bf16* __restrict__ A,
+                                                        int64_t sam, int64_t sak,
+                                                        const __bf16* __restrict__ B, int64_t sbk,
+                                                        int64_t sbn, float* __restrict__ C,
+                                                        int64_t ldc, float alpha, float beta,
+                                                        const float* __restrict__ bias) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[GT][GK + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 Bt[GT][GK + 8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const __bf16 zero = f2bf(0.f);
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    for (int e = tid; e < GT * GK; e += 256) {
+      const int r = e / GK, kk = e % GK;
+      const int m = m0 + r, k = k0 + kk;
+      As[r][kk] = (m < M && k < K) ? A[(int64_t)m * sam + (int64_t)k * sak] : zero;
+      const int n = n0 + r;
+      Bt[r][kk] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : zero;
+    }
+    __syncthreads();
+    const int row = lane & 15, kg = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const bf16x8 a = *(const bf16x8*)&As[wm + i * 16 + row][kg];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const bf16x8 b = *(const bf16x8*)&Bt[wn + j * 16 + row][kg];
+        acc[i][j] = mfma16(a, b, acc[i][j]);
+      }
+    }
+    __syncthreads();
+  }
+  const int col = lane & 15, rg = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm + i * 16 + rg + q, n = n0 + wn + j * 16 + col;
+        if (m < M && n < N) {
+          float v = alpha * acc[i][j][q];
+          if (beta != 0.f) v += beta * C[(int64_t)m * ldc + n];
+          if (bias) v += bias[n];
+          C[(int64_t)m * ldc + n] = v;
+        }
+      }
+}
+
+// y = silu(x) as bf16 (and keep x), or the backward dx = dy * silu'(x)
+__global__ void silu_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n,
+                            int ld_in, int ld_out, int cols) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t r = i / cols, c = i % cols;
+  const float v = x[r * ld_in + c];
+  y[r * ld_out + c] = f2bf(v / (1.f + __expf(-v)));
+}
+
+__global__ void silu_bwd_kernel(const float* __restrict__ x, const float* __restrict__ dy,
+                                __bf16* __restrict__ dx, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = x[i];
+  const float s = 1.f / (1.f + __expf(-v));
+  dx[i] = f2bf(dy[i] * s * (1.f + v * (1.f - s)));
+}
+
+__global__ void cast_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// masked MSE: loss partial and dY = 2 (Y - T) / (rows * used) in bf16, over `used` of `cols`
+__global__ void mse_grad_kernel(const float* __restrict__ Y, const float* __restrict__ T,
+                                __bf16* __restrict__ dY, float* __restrict__ dYf, int rows,
+                                int cols, int used, float* __restrict__ loss) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < (int64_t)rows * cols) {
+    const int c = (int)(i % cols);
+    float g = 0.f;
+    if (c < used) {
+      const float d = Y[i] - T[(i / cols) * used + c];
+      l = d * d;
+      g = 2.f * d / ((float)rows * (float)used);
+    }
+    dY[i] = f2bf(g);
+    if (dYf) dYf[i] = g;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o, 64);
+  if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l / ((float)rows * (float)used));
+}
+
+// column sums (bias gradient): out[c] = sum_r X[r*ld + c]
+__global__ void colsum_kernel(const float* __restrict__ X, int rows, int cols, int ld,
+                              float* __restrict__ out) {
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += X[(int64_t)r * ld + c];
+  __shared__ float sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t n, float lr, float b1, float b2,
+                            float eps, float bc1, float bc2, float wd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] + wd * w[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Fused in-tissue inference.
+// ---------------------------------------------------------------------------
+constexpr int NIN = 22, NHID = 128, NOUT = 19;
+
+// position p (0..31) of a 32-wide K block -> neuron index inside the block
+__device__ __forceinline__ int perm_k(int p) {
+  const int g = p >> 3, q = p & 7;
+  return q < 4 ? 4 * g + q : 16 + 4 * g + (q - 4);
+}
+
+struct MlpSmem {
+  __bf16 w1[NHID][32];          // [out][in], natural input order (padded 22 -> 32)
+  __bf16 w2[NHID][NHID];        // [out][perm(in)]
+  __bf16 w3[NHID][NHID];
+  __bf16 w4[32][NHID];          // 19 -> 32 rows
+  float b1[NHID], b2[NHID], b3[NHID], b4[32];
+};
+
+__device__ __forceinline__ float feat_fwd(float x, int lg) { return lg ? __logf(fmaxf(x, 1e-30f)) : x; }
+
+// Load one input feature (0..21) of the slot, raw (not normalised).
+__device__ __forceinline__ float raw_feature(int f, const float* __restrict__ V,
+                                             const float* __restrict__ st, int64_t ld,
+                                             int64_t slot, int ct, float ki_base, float nai_base) {
+  if (f == 0) return V[slot];
+  if (f <= 18) {
+    float x = st[(int64_t)(f - 1) * ld + slot];
+    if (f == 1) x += ki_base;
+    if (f == 2) x += nai_base;
+    return x;
+  }
+  if (f <= 21) return (f - 19) == ct ? 1.f : 0.f;
+  return 0.f;
+}
+
+__global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
+                                                        float* __restrict__ V_prev,
+                                                        float* __restrict__ state, int64_t ld,
+                                                        float* __restrict__ rate,
+                                                        hp_mlp_weights wts, hp_mlp_norm nrm,
+                                                        hp_surrogate_params sp) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  MlpSmem& S = *reinterpret_cast<MlpSmem*>(smem_raw);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const __bf16* __restrict__ W1 = (const __bf16*)wts.w1;
+  const __bf16* __restrict__ W2 = (const __bf16*)wts.w2;
+  const __bf16* __restrict__ W3 = (const __bf16*)wts.w3;
+  const __bf16* __restrict__ W4 = (const __bf16*)wts.w4;
+  // stage weights (bf16 from global, natural order) into LDS
+  for (int e = tid; e < NHID * 32; e += 256) {
+    const int o = e / 32, i = e % 32;
+    S.w1[o][i] = i < NIN ? W1[o * NIN + i] : f2bf(0.f);
+  }
+  for (int e = tid; e < NHID * NHID; e += 256) {
+    const int o = e / NHID, p = e % NHID;
+    const int kin = (p & ~31) + perm_k(p & 31);
+    S.w2[o][p] = W2[o * NHID + kin];
+    S.w3[o][p] = W3[o * NHID + kin];
+  }
+  for (int e = tid; e < 32 * NHID; e += 256) {
+    const int o = e / NHID, p = e % NHID;
+    const int kin = (p & ~31) + perm_k(p & 31);
+    S.w4[o][p] = o < NOUT ? W4[o * NHID + kin] : f2bf(0.f);
+  }
+  for (int e = tid; e < NHID; e += 256) {
+    S.b1[e] = wts.b1[e];
+    S.b2[e] = wts.b2[e];
+    S.b3[e] = wts.b3[e];
+  }
+  if (tid < 32) S.b4[tid] = tid < NOUT ? wts.b4[tid] : 0.f;
+  __syncthreads();
+
+  const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
+  const int col = lane & 15, grp = lane >> 4;
+  const float dtm = sp.t_macro;
+  // persistent loop: one 32-voxel tile per wave per iteration
+  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile * 32 < n_slots; tile += (int64_t)gridDim.x * 4) {
+    const int64_t base = tile * 32;
+    // ---- inputs: B fragments X^T (32 features x 16 voxels) for the 2 voxel tiles
+    bf16x8 xb[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t slot = base + t * 16 + col;
+      const uint8_t lab = g.label[slot];
+      const int ct = lab & 3;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int f = grp * 8 + q;
+        float x = 0.f;
+        if (lab != 0xFF && f < NIN) {
+          x = raw_feature(f, V, state, ld, slot, ct, sp.ki_base, sp.nai_base);
+          x = (feat_fwd(x, nrm.in_log[f]) - nrm.in_mean[f]) * nrm.in_rstd[f];
+        }
+        xb[t][q] = f2bf(x);
+      }
+    }
+    // ---- layer 1: 8 output tiles x 2 voxel tiles, one K step
+    floatx4 acc[8][2];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const bf16x8 a = *(const bf16x8*)&S.w1[m * 16 + col][grp * 8];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) acc[m][t] = mfma16(a, xb[t], floatx4{0.f, 0.f, 0.f, 0.f});
+    }
+    // ---- hidden layers 2, 3: SiLU(acc + b) -> B fragments -> MFMA
+    bf16x8 hb[4][2];
+    auto act_to_b = [&](const float* bias) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int m = 2 * j + (q >> 2), r = grp * 4 + (q & 3);
+            const float z = acc[m][t][q & 3] + bias[m * 16 + r];
+            hb[j][t][q] = f2bf(z / (1.f + __expf(-z)));
+          }
+    };
+    act_to_b(S.b1);
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {
+      const __bf16(*W)[NHID] = layer == 0 ? S.w2 : S.w3;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bf16x8 a = *(const bf16x8*)&W[m * 16 + col][j * 32 + grp * 8];
+          c0 = mfma16(a, hb[j][0], c0);
+          c1 = mfma16(a, hb[j][1], c1);
+        }
+        acc[m][0] = c0;
+        acc[m][1] = c1;
+      }
+      act_to_b(layer == 0 ? S.b2 : S.b3);
+    }
+    // ---- output layer: 2 output tiles (19 used)
+    floatx4 out[2][2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bf16x8 a = *(const bf16x8*)&S.w4[m * 16 + col][j * 32 + grp * 8];
+        c0 = mfma16(a, hb[j][0], c0);
+        c1 = mfma16(a, hb[j][1], c1);
+      }
+      out[m][0] = c0;
+      out[m][1] = c1;
+    }
+    // ---- mode decision and update, per voxel (this lane's 4+4 outputs)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t slot = base + t * 16 + col;
+      const uint8_t lab = g.label[slot];
+      if (lab == 0xFF) continue;
+      const float v = V[slot], vp = V_prev[slot];
+      const float dvdt = (v - vp) / dtm;
+      const bool fallback = sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
+                            (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int o = m * 16 + grp * 4 + q;
+          if (o >= NOUT) continue;
+          const float d = (out[m][t][q] + S.b4[o]) * nrm.out_std[o] + nrm.out_mean[o];
+          if (o == 0) {
+            rate[slot] = fallback ? __builtin_nanf("") : d / dtm;
+          } else if (!fallback) {
+            float* p = state + (int64_t)(o - 1) * ld + slot;
+            const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
+            const float x0 = *p + base_v;
+            float x1;
+            if (nrm.in_log[o]) x1 = __expf(__logf(fmaxf(x0, 1e-30f)) + d);
+            else x1 = x0 + d;
+            *p = x1 - base_v;
+          }
+        }
+    }
+    // V_prev of this tile was consumed above (the mode decision depends on it),
+    // so the loads have completed before these stores are issued
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int64_t slot = base + t * 16 + col;
+      if (grp == 0 && g.label[slot] != 0xFF) V_prev[slot] = V[slot];
+    }
+  }
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+
+int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
+                 int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
+                 const float* bias, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0) {
+    set_error("hp_gemm_bf16: bad shape %d x %d x %d", M, N, K);
+    return -1;
+  }
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+  hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
+                     (const __bf16*)A, sam, sak, (const __bf16*)B, sbk, sbn, C, ldc, alpha, beta,
+                     bias);
+  return check_launch("hp_gemm_bf16");
+}
+
+int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int ld_out,
+                 void* stream) {
+  const int64_t n = rows * cols;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(silu_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, x,
+                     (__bf16*)y, n, ld_in, ld_out, cols);
+  return check_launch("hp_silu_bf16");
+}
+
+int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, int64_t n, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(silu_bwd_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     x, dy, (__bf16*)dx, n);
+  return check_launch("hp_silu_bwd_bf16");
+}
+
+int hp_cast_bf16(const float* x, void* y, int64_t n, void* stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(cast_bf16_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
+                     x, (__bf16*)y, n);
+  return check_launch("hp_cast_bf16");
+}
+
+int hp_mse_grad(const float* Y, const float* T, void* dY, float* dYf, int rows, int cols, int used,
+                float* loss, void* stream) {
+  const int64_t n = (int64_t)rows * cols;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(mse_grad_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, Y,
+                     T, (__bf16*)dY, dYf, rows, cols, used, loss);
+  return check_launch("hp_mse_grad");
+}
+
+int hp_colsum(const float* X, int rows, int cols, int ld, float* out, void* stream) {
+  if (rows <= 0 || cols <= 0) return 0;
+  hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, (hipStream_t)stream, X, rows, cols,
+                     ld, out);
+  return check_launch("hp_colsum");
+}
+
+int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+            float eps, int step, float wd, void* stream) {
+  if (n <= 0 || step <= 0) return 0;
+  const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m,
+                     v, n, lr, b1, b2, eps, bc1, bc2, wd);
+  return check_launch("hp_adam");
+}
+
+int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* state, int64_t ld,
+                      float* rate, const hp_mlp_weights* w, const hp_mlp_norm* n,
+                      const hp_surrogate_params* sp, int n_blocks, void* stream) {
+  if (!g || !w || !n || !sp || g->n_bricks <= 0) {
+    set_error("hp_surrogate_step: bad arguments");
+    return -1;
+  }
+  const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
+  if (n_slots % 32) {
+    set_error("hp_surrogate_step: slot count must be a multiple of 32");
+    return -1;
+  }
+  const size_t smem = sizeof(MlpSmem);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    attr_set = true;
+  }
+  if (n_blocks <= 0) n_blocks = 512;
+  const int64_t tiles = n_slots / 32;
+  if ((int64_t)n_blocks * 4 > tiles) n_blocks = (int)((tiles + 3) / 4);
+  hipLaunchKernelGGL(surrogate_kernel, dim3(n_blocks), dim3(256), smem, (hipStream_t)stream, *g, V,
+                     V_prev, state, ld, rate, *w, *n, *sp);
+  return check_launch("hp_surrogate_step");
+}
+
+}  // extern "C"

@@ -142,7 +142,13 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
   const T lv = T(p.dscale) * stencil19<T, BX, BY, BZ>(t, c);
   const T dt = T(p.dt);
   T dvdt = T(0);  // reaction part of dV/dt
-  if (p.do_reaction) {
+  bool full = p.do_reaction == 1;
+  if (p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
+    const T r = ((const T*)p.rate_ext)[slot];
+    full = r != r;
+    if (!full) dvdt = r;
+  }
+  if (full) {
     const int ct = lab & 3;
     T gks = T(kGKs[ct]), gto = T(kGto[ct]);
     if (p.inst_params && g.inst) {
@@ -158,7 +164,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
   }
-  if (p.rate_ext) dvdt += ((const T*)p.rate_ext)[slot];
+  if (p.rate_ext && p.do_reaction == 0) dvdt += ((const T*)p.rate_ext)[slot];
   const int grp = (lab >> 2) & 31;
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);

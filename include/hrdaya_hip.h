@@ -104,7 +104,8 @@ typedef struct hp_mono_params {
   double ki_base, nai_base;
   double vmin, vmax;  /* clamp, reference: [-150, 80] */
   double at_thr;      /* activation threshold, reference: -40 mV */
-  int32_t do_reaction;   /* 1: TTP06 step; 0: diffusion only */
+  int32_t do_reaction;   /* 1: TTP06 step; 0: diffusion only; 2: TTP06 only where
+                            rate_ext is NaN, else dV/dt = rate_ext (surrogate form) */
   int32_t n_inst;        /* number of rows of inst_params */
   const void* inst_params;  /* T[n_inst][4]: g_Ks scale, g_to scale, sigma scale, spare */
   const int64_t* step_ctr;  /* device step counter; time label t = step*dt */
@@ -287,6 +288,63 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
  * No counterpart in the reference (it outputs raw surface potentials only). */
 int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
                             const double* w, double* out, int64_t n_electrodes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned ionic surrogate (no counterpart in the reference).  An MLP
+ * 22 -> 128 -> 128 -> 128 -> 19 (SiLU), BF16 weights, FP32 accumulation on
+ * MFMA, predicts the increments of V and of the 18 TTP06 variables over one
+ * macro-step of 10 reaction substeps.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_mlp_weights {
+  const void* w1;  /* bf16 [128][22]  (out, in) */
+  const void* w2;  /* bf16 [128][128] */
+  const void* w3;  /* bf16 [128][128] */
+  const void* w4;  /* bf16 [19][128]  */
+  const float* b1; const float* b2; const float* b3; const float* b4;
+} hp_mlp_weights;
+
+/* Feature normalisation: x_in = (f(x) - in_mean)/in_std * ... with f = log for
+ * in_log features; outputs y = out_mean + out_std * net (increments of f(x)). */
+typedef struct hp_mlp_norm {
+  float in_mean[22], in_rstd[22];
+  int32_t in_log[22];
+  float out_mean[19], out_std[19];
+} hp_mlp_norm;
+
+typedef struct hp_surrogate_params {
+  float t_macro;        /* macro-step length, ms (10 substeps) */
+  float dvdt_max;       /* fallback if |dV/dt| above, mV/ms (reference choice: 1) */
+  float v_lo, v_hi;     /* fallback if v_lo < V < v_hi (-70, -40) */
+  float ki_base, nai_base;
+  int32_t force_fallback;
+  int32_t pad_;
+} hp_surrogate_params;
+
+/* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], bf16 operands, FP32 C. */
+int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
+                 int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
+                 const float* bias, void* stream);
+int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int ld_out,
+                 void* stream);
+int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, int64_t n, void* stream);
+int hp_cast_bf16(const float* x, void* y, int64_t n, void* stream);
+/* masked MSE over the first `used` of `cols` columns; loss accumulated into *loss. */
+int hp_mse_grad(const float* Y, const float* T, void* dY, float* dYf, int rows, int cols,
+                int used, float* loss, void* stream);
+int hp_colsum(const float* X, int rows, int cols, int ld, float* out, void* stream);
+int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+            float b2, float eps, int step, float wd, void* stream);
+
+/* One surrogate macro-step decision + update over the brick layout (FP32
+ * engine state): per voxel, fallback if |V - V_prev|/t_macro > dvdt_max, or
+ * v_lo < V < v_hi, or V_prev is NaN (first call); fallback voxels get
+ * rate = NaN (the fused step then runs the full TTP06 for them), the others
+ * get their 18 state variables advanced by the network and rate = dV/t_macro
+ * (applied by hp_myo_step with do_reaction = 2).  V_prev <- V. */
+int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* state,
+                      int64_t ld, float* rate, const hp_mlp_weights* w, const hp_mlp_norm* n,
+                      const hp_surrogate_params* sp, int n_blocks, void* stream);
 
 #ifdef __cplusplus
 }
