@@ -47,6 +47,21 @@ class HpPmjAt(C.Structure):
                 ("at_thr", c_f64)]
 
 
+class HpMlpWeights(C.Structure):
+    _fields_ = [("w1", c_vp), ("w2", c_vp), ("w3", c_vp), ("w4", c_vp), ("b1", c_vp), ("b2", c_vp),
+                ("b3", c_vp), ("b4", c_vp)]
+
+
+class HpMlpNorm(C.Structure):
+    _fields_ = [("in_mean", c_f32 * 22), ("in_rstd", c_f32 * 22), ("in_log", c_i32 * 22),
+                ("out_mean", c_f32 * 19), ("out_std", c_f32 * 19)]
+
+
+class HpSurrogateParams(C.Structure):
+    _fields_ = [("t_macro", c_f32), ("dvdt_max", c_f32), ("v_lo", c_f32), ("v_hi", c_f32),
+                ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("pad_", c_i32)]
+
+
 class HpTorso(C.Structure):
     _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
                 ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
@@ -91,6 +106,17 @@ _SIGS = {
     "hp_bspm_source_f64": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f64, c_f64,
                                      c_vp, c_i64, P(HpPcgWs), c_vp]),
     "hp_leads_accumulate_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "hp_gemm_bf16": (C.c_int, [C.c_int, C.c_int, C.c_int, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64,
+                               c_vp, c_i64, c_f32, c_f32, c_vp, C.c_int, c_vp]),
+    "hp_silu_bf16": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, C.c_int, C.c_int, c_vp]),
+    "hp_silu_bwd_bf16": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "hp_cast_bf16": (C.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "hp_mse_grad": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp]),
+    "hp_colsum": (C.c_int, [c_vp, C.c_int, C.c_int, C.c_int, c_vp, c_vp]),
+    "hp_adam": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_f32, c_f32, c_f32, c_f32, C.c_int,
+                          c_f32, c_vp]),
+    "hp_surrogate_step": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, c_vp, P(HpMlpWeights),
+                                    P(HpMlpNorm), P(HpSurrogateParams), C.c_int, c_vp]),
     "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
 }

@@ -51,19 +51,23 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  // split-K: slice z of gridDim.z handles K range [kb, ke) and adds atomically
+  const int ksl = (K + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int kb = (int)blockIdx.z * ((ksl + GK - 1) / GK * GK);
+  const int ke = min(K, kb + (ksl + GK - 1) / GK * GK);
   floatx4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
   const __bf16 zero = f2bf(0.f);
-  for (int k0 = 0; k0 < K; k0 += GK) {
+  for (int k0 = kb; k0 < ke; k0 += GK) {
     for (int e = tid; e < GT * GK; e += 256) {
       const int r = e / GK, kk = e % GK;
       const int m = m0 + r, k = k0 + kk;
-      As[r][kk] = (m < M && k < K) ? A[(int64_t)m * sam + (int64_t)k * sak] : zero;
+      As[r][kk] = (m < M && k < ke) ? A[(int64_t)m * sam + (int64_t)k * sak] : zero;
       const int n = n0 + r;
-      Bt[r][kk] = (n < N && k < K) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : zero;
+      Bt[r][kk] = (n < N && k < ke) ? B[(int64_t)k * sbk + (int64_t)n * sbn] : zero;
     }
     __syncthreads();
     const int row = lane & 15, kg = (lane >> 4) * 8;
@@ -88,9 +92,13 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
         const int m = m0 + wm + i * 16 + rg + q, n = n0 + wn + j * 16 + col;
         if (m < M && n < N) {
           float v = alpha * acc[i][j][q];
-          if (beta != 0.f) v += beta * C[(int64_t)m * ldc + n];
-          if (bias) v += bias[n];
-          C[(int64_t)m * ldc + n] = v;
+          if (gridDim.z > 1) {
+            atomicAdd(&C[(int64_t)m * ldc + n], v);  // caller zeroes C; no beta/bias
+          } else {
+            if (beta != 0.f) v += beta * C[(int64_t)m * ldc + n];
+            if (bias) v += bias[n];
+            C[(int64_t)m * ldc + n] = v;
+          }
         }
       }
 }
@@ -106,12 +114,14 @@ __global__ void silu_kernel(const float* __restrict__ x, __bf16* __restrict__ y,
 }
 
 __global__ void silu_bwd_kernel(const float* __restrict__ x, const float* __restrict__ dy,
-                                __bf16* __restrict__ dx, int64_t n) {
+                                __bf16* __restrict__ dx, float* __restrict__ dxf, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float v = x[i];
   const float s = 1.f / (1.f + __expf(-v));
-  dx[i] = f2bf(dy[i] * s * (1.f + v * (1.f - s)));
+  const float g = dy[i] * s * (1.f + v * (1.f - s));
+  dx[i] = f2bf(g);
+  if (dxf) dxf[i] = g;
 }
 
 __global__ void cast_bf16_kernel(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
@@ -369,12 +379,16 @@ extern "C" {
 
 int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
                  int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
-                 const float* bias, void* stream) {
-  if (M <= 0 || N <= 0 || K <= 0) {
-    set_error("hp_gemm_bf16: bad shape %d x %d x %d", M, N, K);
+                 const float* bias, int ksplit, void* stream) {
+  if (M <= 0 || N <= 0 || K <= 0 || ksplit < 1 || ksplit > 1024) {
+    set_error("hp_gemm_bf16: bad shape %d x %d x %d (ksplit %d)", M, N, K, ksplit);
     return -1;
   }
-  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+  if (ksplit > 1 && (beta != 0.f || bias)) {
+    set_error("hp_gemm_bf16: split-K accumulates atomically; beta and bias must be 0");
+    return -1;
+  }
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, ksplit);
   hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream, M, N, K,
                      (const __bf16*)A, sam, sak, (const __bf16*)B, sbk, sbn, C, ldc, alpha, beta,
                      bias);
@@ -390,10 +404,11 @@ int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int
   return check_launch("hp_silu_bf16");
 }
 
-int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, int64_t n, void* stream) {
+int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, float* dxf, int64_t n,
+                     void* stream) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(silu_bwd_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
-                     x, dy, (__bf16*)dx, n);
+                     x, dy, (__bf16*)dx, dxf, n);
   return check_launch("hp_silu_bwd_bf16");
 }
 

@@ -1,0 +1,366 @@
+"""Learned TTP06 surrogate: data generation with the plain integrator, training
+with the project's own HIP kernels (BF16 MFMA GEMMs, SiLU, MSE, Adam), and
+the surrogate form of the monodomain time loop.
+
+The network maps the cell state at the start of a 0.2 ms macro-step
+(V, the 18 TTP06 variables, phenotype one-hot) to the increments that ten
+0.02 ms reaction substeps (Rush-Larsen gates, forward-Euler concentrations,
+V += -dt*I_ion) produce.  In tissue, voxels at rest or on the plateau take the
+network's increment once per macro-step (V through a constant rate while
+diffusion still runs every substep); voxels whose |dV/dt| exceeds 1 mV/ms or
+whose V lies in (-70, -40) mV fall back to the plain integrator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .ttp06 import INIT, STATE_NAMES
+
+N_IN, N_HID, N_OUT = 22, 128, 19
+# features carried as logarithms: Cai, m, d, r, Cass, Xs, Xr1 (span orders of magnitude)
+LOG_FEATURES = ("Cai", "m", "d", "r", "Cass", "Xs", "Xr1")
+FEATURE_NAMES = ["V"] + STATE_NAMES + ["is_endo", "is_mid", "is_epi"]
+IN_LOG = np.array([1 if n in LOG_FEATURES else 0 for n in FEATURE_NAMES], dtype=np.int32)
+KI_BASE, NAI_BASE = INIT["Ki"], INIT["Nai"]
+T_MACRO, N_SUB = 0.2, 10
+
+
+def _bf16(t):
+    return t.to(torch.bfloat16).contiguous()
+
+
+class SurrogateNet:
+    """FP32 master weights + BF16 copies on the GPU, and the normalisation."""
+
+    def __init__(self, device="cuda", seed=0):
+        g = torch.Generator().manual_seed(seed)
+        dev = torch.device(device)
+        self.dev = dev
+
+        def init(o, i):
+            return (torch.randn(o, i, generator=g) * math.sqrt(2.0 / i)).to(dev)
+
+        self.W = [init(N_HID, N_IN), init(N_HID, N_HID), init(N_HID, N_HID),
+                  init(N_OUT, N_HID) * 0.1]
+        self.b = [torch.zeros(N_HID, device=dev), torch.zeros(N_HID, device=dev),
+                  torch.zeros(N_HID, device=dev), torch.zeros(N_OUT, device=dev)]
+        self.in_mean = np.zeros(N_IN, np.float32)
+        self.in_std = np.ones(N_IN, np.float32)
+        self.out_mean = np.zeros(N_OUT, np.float32)
+        self.out_std = np.ones(N_OUT, np.float32)
+        self.refresh()
+
+    # -- parameters as C structs
+    def refresh(self):
+        self.Wb = [_bf16(w) for w in self.W]
+        self.hp_w = nat.HpMlpWeights(*[w.data_ptr() for w in self.Wb],
+                                     *[b.data_ptr() for b in self.b])
+        n = nat.HpMlpNorm()
+        for k in range(N_IN):
+            n.in_mean[k] = float(self.in_mean[k])
+            n.in_rstd[k] = float(1.0 / self.in_std[k])
+            n.in_log[k] = int(IN_LOG[k])
+        for k in range(N_OUT):
+            n.out_mean[k] = float(self.out_mean[k])
+            n.out_std[k] = float(self.out_std[k])
+        self.hp_norm = n
+
+    def state_dict(self):
+        return {"W": [w.cpu() for w in self.W], "b": [b.cpu() for b in self.b],
+                "in_mean": self.in_mean, "in_std": self.in_std, "out_mean": self.out_mean,
+                "out_std": self.out_std}
+
+    def load_state_dict(self, d):
+        self.W = [torch.as_tensor(w).to(self.dev) for w in d["W"]]
+        self.b = [torch.as_tensor(b).to(self.dev) for b in d["b"]]
+        for k in ("in_mean", "in_std", "out_mean", "out_std"):
+            setattr(self, k, np.asarray(d[k], np.float32))
+        self.refresh()
+
+    # -- feature transforms (host side of data preparation)
+    @staticmethod
+    def features(V, S, ct):
+        """(n,22) raw features: V, 18 absolute state variables, one-hot phenotype."""
+        n = V.shape[0]
+        X = torch.empty((n, N_IN), dtype=torch.float32, device=V.device)
+        X[:, 0] = V
+        X[:, 1:19] = S
+        X[:, 19:] = torch.nn.functional.one_hot(ct.long(), 3).float()
+        return X
+
+    @staticmethod
+    def transform(X):
+        X = X.clone()
+        lg = torch.from_numpy(IN_LOG.astype(bool)).to(X.device)
+        X[:, lg] = torch.log(X[:, lg].clamp_min(1e-30))
+        return X
+
+    def fit_normalisation(self, Xt, Dt):
+        self.in_mean = Xt.mean(0).cpu().numpy().astype(np.float32)
+        sd = Xt.std(0).cpu().numpy().astype(np.float32)
+        sd[19:] = 1.0
+        self.in_mean[19:] = 0.0
+        self.in_std = np.where(sd > 1e-12, sd, 1.0).astype(np.float32)
+        self.out_mean = Dt.mean(0).cpu().numpy().astype(np.float32)
+        so = Dt.std(0).cpu().numpy().astype(np.float32)
+        self.out_std = np.where(so > 1e-20, so, 1e-20).astype(np.float32)
+        self.refresh()
+
+
+# ---------------------------------------------------------------------------
+# training data from the plain integrator
+# ---------------------------------------------------------------------------
+def reaction_targets(V, S, ct, dtype=torch.float64):
+    """Increments of (V, 18 states) over N_SUB reaction-only substeps (V += -dt*I_ion),
+    computed with the FP64 HIP ionic kernel."""
+    n = V.shape[0]
+    Vd = V.to(dtype).clone()
+    st = S.to(dtype).t().contiguous()  # SoA (18, n), absolute values
+    I = torch.empty(n, dtype=dtype, device=V.device)
+    ctt = ct.to(torch.uint8).contiguous()
+    s = nat.stream_handle(V.device)
+    dt = T_MACRO / N_SUB
+    for _ in range(N_SUB):
+        nat.call("hp_ttp06_step_f64", Vd.data_ptr(), st.data_ptr(), I.data_ptr(), ctt.data_ptr(),
+                 0, None, None, n, n, 1, dt, 0.0, 0.0, s)
+        Vd -= dt * I
+    return Vd, st.t()
+
+
+def output_increments(V0, S0, V1, S1):
+    """(n,19) increments in the network's output space (log features as log-ratios)."""
+    D = torch.empty((V0.shape[0], N_OUT), dtype=torch.float64, device=V0.device)
+    D[:, 0] = V1 - V0
+    a, b = S0.double(), S1.double()
+    lg = torch.from_numpy(IN_LOG[1:19].astype(bool)).to(V0.device)
+    d = b - a
+    d[:, lg] = torch.log(b[:, lg].clamp_min(1e-30)) - torch.log(a[:, lg].clamp_min(1e-30))
+    D[:, 1:] = d
+    return D.float()
+
+
+def surrogate_mask(V, V_prev, dvdt_max=1.0, v_lo=-70.0, v_hi=-40.0):
+    """True where the network (not the fallback) handles the voxel."""
+    dvdt = (V - V_prev).abs() / T_MACRO
+    return (dvdt <= dvdt_max) & ~((V > v_lo) & (V < v_hi))
+
+
+def sample_from_engine(eng, n_macro, per_snapshot, seed=0, only_surrogate=True):
+    """Run the plain engine and sample (V, state, phenotype) at macro-step
+    boundaries; returns device tensors (V, S_abs, ct)."""
+    g = torch.Generator(device=eng.dev).manual_seed(seed)
+    sl = eng.slot_of_node.long()
+    ct_all = (eng.label[sl] & 3)
+    Vs, Ss, Cs = [], [], []
+    V_prev = None
+    for k in range(n_macro):
+        eng.run(N_SUB, graph_chunk=N_SUB)
+        V = eng.V[eng.p][sl].float()
+        if V_prev is not None:
+            cand = torch.arange(len(sl), device=eng.dev)
+            if only_surrogate:
+                cand = cand[surrogate_mask(V, V_prev)]
+            if cand.numel():
+                pick = cand[torch.randint(cand.numel(), (min(per_snapshot, cand.numel()),),
+                                          generator=g, device=eng.dev)]
+                st = eng.state[:, sl[pick]].t().float()
+                st[:, 0] += eng.ki_base
+                st[:, 1] += eng.nai_base
+                Vs.append(V[pick])
+                Ss.append(st)
+                Cs.append(ct_all[pick])
+        V_prev = V
+    return torch.cat(Vs), torch.cat(Ss), torch.cat(Cs)
+
+
+# ---------------------------------------------------------------------------
+# training with the HIP kernels
+# ---------------------------------------------------------------------------
+class Trainer:
+    """Mini-batch Adam on the MSE of normalised increments; every FLOP of the
+    forward and backward pass runs in hrdaya HIP kernels."""
+
+    def __init__(self, net: SurrogateNet, batch=8192, lr=2e-3, wd=0.0):
+        self.net, self.B, self.lr, self.wd = net, batch, lr, wd
+        dev = net.dev
+        B = batch
+        f32 = dict(dtype=torch.float32, device=dev)
+        bf = dict(dtype=torch.bfloat16, device=dev)
+        self.X0 = torch.zeros((B, 32), **bf)
+        self.Z = [torch.empty((B, N_HID), **f32) for _ in range(3)]
+        self.A = [torch.empty((B, N_HID), **bf) for _ in range(3)]
+        self.Y = torch.empty((B, N_OUT), **f32)
+        self.dY = torch.empty((B, N_OUT), **bf)
+        self.dYf = torch.empty((B, N_OUT), **f32)
+        self.dA = torch.empty((B, N_HID), **f32)
+        self.dZ = torch.empty((B, N_HID), **bf)
+        self.dZf = torch.empty((B, N_HID), **f32)
+        self.gW = [torch.zeros_like(w) for w in net.W]
+        self.gb = [torch.zeros_like(b) for b in net.b]
+        self.mW = [torch.zeros_like(w) for w in net.W]
+        self.vW = [torch.zeros_like(w) for w in net.W]
+        self.mb = [torch.zeros_like(b) for b in net.b]
+        self.vb = [torch.zeros_like(b) for b in net.b]
+        self.loss = torch.zeros(1, **f32)
+        self.step_no = 0
+
+    def _gemm(self, M, N, K, A, sam, sak, B, sbk, sbn, Cout, ldc, bias=None, ksplit=1):
+        nat.call("hp_gemm_bf16", M, N, K, A.data_ptr(), sam, sak, B.data_ptr(), sbk, sbn,
+                 Cout.data_ptr(), ldc, 1.0, 0.0, nat.ptr(bias), ksplit, nat.stream_handle())
+
+    def forward(self, Xn_bf16, B):
+        """Xn_bf16: (B, 32) normalised inputs (first 22 columns used)."""
+        net, s = self.net, nat.stream_handle()
+        W = net.Wb
+        self._gemm(B, N_HID, N_IN, Xn_bf16, 32, 1, W[0], 1, N_IN, self.Z[0], N_HID, net.b[0])
+        for l in range(3):
+            nat.call("hp_silu_bf16", self.Z[l].data_ptr(), self.A[l].data_ptr(), B, N_HID, N_HID,
+                     N_HID, s)
+            if l < 2:
+                self._gemm(B, N_HID, N_HID, self.A[l], N_HID, 1, W[l + 1], 1, N_HID, self.Z[l + 1],
+                           N_HID, net.b[l + 1])
+        self._gemm(B, N_OUT, N_HID, self.A[2], N_HID, 1, W[3], 1, N_HID, self.Y, N_OUT, net.b[3])
+        return self.Y[:B]
+
+    def backward(self, Xn_bf16, Tn, B):
+        """Gradients of the masked MSE (over the 19 outputs) into self.gW/self.gb."""
+        net, s = self.net, nat.stream_handle()
+        W = net.Wb
+        self.loss.zero_()
+        nat.call("hp_mse_grad", self.Y.data_ptr(), Tn.data_ptr(), self.dY.data_ptr(),
+                 self.dYf.data_ptr(), B, N_OUT, N_OUT, self.loss.data_ptr(), s)
+        ks = max(1, min(64, B // 1024))
+        for g in self.gW:
+            g.zero_()
+        # output layer
+        self._gemm(N_OUT, N_HID, B, self.dY, 1, N_OUT, self.A[2], N_HID, 1, self.gW[3], N_HID,
+                   ksplit=ks)
+        nat.call("hp_colsum", self.dYf.data_ptr(), B, N_OUT, N_OUT, self.gb[3].data_ptr(), s)
+        self._gemm(B, N_HID, N_OUT, self.dY, N_OUT, 1, W[3], N_HID, 1, self.dA, N_HID)
+        for l in (2, 1, 0):
+            nat.call("hp_silu_bwd_bf16", self.Z[l].data_ptr(), self.dA.data_ptr(),
+                     self.dZ.data_ptr(), self.dZf.data_ptr(), B * N_HID, s)
+            nat.call("hp_colsum", self.dZf.data_ptr(), B, N_HID, N_HID, self.gb[l].data_ptr(), s)
+            if l > 0:
+                self._gemm(N_HID, N_HID, B, self.dZ, 1, N_HID, self.A[l - 1], N_HID, 1, self.gW[l],
+                           N_HID, ksplit=ks)
+                self._gemm(B, N_HID, N_HID, self.dZ, N_HID, 1, W[l], N_HID, 1, self.dA, N_HID)
+            else:
+                self._gemm(N_HID, N_IN, B, self.dZ, 1, N_HID, Xn_bf16, 32, 1, self.gW[0], N_IN,
+                           ksplit=ks)
+
+    def adam(self):
+        self.step_no += 1
+        s = nat.stream_handle()
+        for p, g, m, v in list(zip(self.net.W, self.gW, self.mW, self.vW)) + list(
+                zip(self.net.b, self.gb, self.mb, self.vb)):
+            nat.call("hp_adam", p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), p.numel(),
+                     self.lr, 0.9, 0.999, 1e-8, self.step_no, self.wd, s)
+        for w, wb in zip(self.net.W, self.net.Wb):
+            nat.call("hp_cast_bf16", w.data_ptr(), wb.data_ptr(), w.numel(), s)
+
+    def prepare(self, X, D):
+        """Normalised bf16 inputs (n,32) and fp32 targets (n,19) from raw features/increments."""
+        net = self.net
+        Xt = SurrogateNet.transform(X)
+        Xn = torch.zeros((X.shape[0], 32), dtype=torch.bfloat16, device=X.device)
+        Xn[:, :N_IN] = ((Xt - torch.from_numpy(net.in_mean).to(X.device))
+                        / torch.from_numpy(net.in_std).to(X.device)).to(torch.bfloat16)
+        Tn = ((D - torch.from_numpy(net.out_mean).to(X.device))
+              / torch.from_numpy(net.out_std).to(X.device)).float().contiguous()
+        return Xn, Tn
+
+    def fit(self, Xn, Tn, epochs=1, lr_decay=0.5, log=None, seed=0):
+        n = Xn.shape[0]
+        B = self.B
+        g = torch.Generator(device=Xn.device).manual_seed(seed)
+        hist = []
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=g, device=Xn.device)
+            tot, cnt = 0.0, 0
+            for i in range(0, n - B + 1, B):
+                idx = perm[i:i + B]
+                xb = Xn[idx].contiguous()
+                tb = Tn[idx].contiguous()
+                self.forward(xb, B)
+                self.backward(xb, tb, B)
+                self.adam()
+                if cnt % 50 == 0:
+                    tot += float(self.loss)
+                cnt += 1
+            hist.append(tot / max(1, (cnt + 49) // 50))
+            if log:
+                log(f"epoch {ep}: loss {hist[-1]:.3e} ({time.perf_counter() - t0:.1f} s)")
+            self.lr *= lr_decay
+        return hist
+
+
+# ---------------------------------------------------------------------------
+# surrogate form of the time loop
+# ---------------------------------------------------------------------------
+class SurrogateRunner:
+    """Drives a MonodomainEngine (FP32) in the surrogate form: per macro-step
+    one hp_surrogate_step, then N_SUB fused steps with do_reaction = 2."""
+
+    def __init__(self, eng, net: SurrogateNet, dvdt_max=1.0, v_lo=-70.0, v_hi=-40.0,
+                 n_blocks=0):
+        if eng.dtype != torch.float32:
+            raise ValueError("the surrogate form runs on the FP32 engine")
+        self.eng, self.net = eng, net
+        self.rate = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
+                               device=eng.dev)
+        self.V_prev = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
+                                 device=eng.dev)
+        self.sp = nat.HpSurrogateParams(T_MACRO, dvdt_max, v_lo, v_hi, eng.ki_base, eng.nai_base,
+                                        0, 0)
+        self.n_blocks = n_blocks
+        self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
+        self._graph = None
+
+    def _launch_macro(self):
+        eng = self.eng
+        nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
+                 nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots, nat.ptr(self.rate),
+                 C.byref(self.net.hp_w), C.byref(self.net.hp_norm), C.byref(self.sp),
+                 self.n_blocks, nat.stream_handle())
+        eng.p_myo.do_reaction = 2
+        eng.p_myo.rate_ext = nat.ptr(self.rate)
+        try:
+            for _ in range(N_SUB):
+                eng._launch_step()
+        finally:
+            eng.p_myo.do_reaction, eng.p_myo.rate_ext = self._orig
+
+    def run(self, n_macro, graph=True):
+        if not graph:
+            for _ in range(n_macro):
+                self._launch_macro()
+            return
+        if self._graph is None:
+            eng = self.eng
+            p0 = eng.p
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(device=eng.dev)
+            side.wait_stream(torch.cuda.current_stream(eng.dev))
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    self._launch_macro()
+                    self._launch_macro()
+            torch.cuda.current_stream(eng.dev).wait_stream(side)
+            eng.p = p0
+            self._graph = g
+        for _ in range(n_macro // 2):
+            self._graph.replay()
+        if n_macro % 2:
+            self._launch_macro()
+
+    def fallback_fraction(self):
+        live = self.eng.label != 0xFF
+        return float(torch.isnan(self.rate[live]).float().mean())

@@ -321,13 +321,16 @@ typedef struct hp_surrogate_params {
 } hp_surrogate_params;
 
 /* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),
- * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], bf16 operands, FP32 C. */
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], bf16 operands, FP32 C.
+ * ksplit > 1 splits K over that many workgroup slices that add into C
+ * atomically (C zeroed by the caller, beta = 0, no bias). */
 int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
                  int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
-                 const float* bias, void* stream);
+                 const float* bias, int ksplit, void* stream);
 int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int ld_out,
                  void* stream);
-int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, int64_t n, void* stream);
+int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, float* dxf, int64_t n,
+                     void* stream);
 int hp_cast_bf16(const float* x, void* y, int64_t n, void* stream);
 /* masked MSE over the first `used` of `cols` columns; loss accumulated into *loss. */
 int hp_mse_grad(const float* Y, const float* T, void* dY, float* dYf, int rows, int cols,

@@ -1,0 +1,204 @@
+"""GPU checks of the surrogate's MFMA kernels: GEMM against numpy, gradients
+against CPU autograd, fused inference against a CPU forward, and the learned
+model against the same-cost alternative without it (one 0.2 ms Rush-Larsen
+step in place of ten 0.02 ms substeps)."""
+import numpy as np
+import pytest
+import torch
+
+from hrdayapy_amd import _native as nat
+from hrdayapy_amd import surrogate as SG
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(x):
+    return torch.as_tensor(x).to(torch.bfloat16)
+
+
+@pytest.mark.parametrize("M,N,K,trans_a,trans_b,ksplit", [
+    (64, 64, 32, False, False, 1), (100, 37, 70, False, True, 1), (19, 128, 3000, True, False, 8),
+    (1000, 128, 22, False, True, 1)])
+def test_gemm_matches_numpy(cuda, M, N, K, trans_a, trans_b, ksplit):
+    print(f"[case] gemm {M}x{N}x{K} ta={trans_a} tb={trans_b} ks={ksplit}", flush=True)
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if trans_a else (M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K) if trans_b else (K, N)).astype(np.float32)
+    Ab, Bb = _bf(A), _bf(B)
+    Ar, Br = Ab.float().numpy(), Bb.float().numpy()
+    ref = (Ar.T if trans_a else Ar) @ (Br.T if trans_b else Br)
+    Ad, Bd = Ab.to(cuda), Bb.to(cuda)
+    C = torch.zeros((M, N), dtype=torch.float32, device=cuda)
+    sam, sak = (1, M) if trans_a else (K, 1)
+    sbk, sbn = (1, K) if trans_b else (N, 1)
+    nat.call("hp_gemm_bf16", M, N, K, Ad.data_ptr(), sam, sak, Bd.data_ptr(), sbk, sbn,
+             C.data_ptr(), N, 1.0, 0.0, None, ksplit, nat.stream_handle())
+    got = C.cpu().numpy()
+    assert np.max(np.abs(got - ref)) < 1e-3 * max(1.0, np.abs(ref).max())
+
+
+def _cpu_forward(net, Xn, bf16_acts=True):
+    """FP32 CPU forward with the same bf16 rounding points as the kernels."""
+    W = [w.to(torch.bfloat16).float().cpu() for w in net.W]
+    b = [x.cpu() for x in net.b]
+    a = Xn.float().cpu()[:, :SG.N_IN]
+    zs = []
+    for l in range(3):
+        z = a @ W[l].t() + b[l]
+        zs.append(z)
+        a = torch.nn.functional.silu(z)
+        if bf16_acts:
+            a = a.to(torch.bfloat16).float()
+    return a @ W[3].t() + b[3], zs
+
+
+def test_gradients_match_cpu_autograd(cuda):
+    print("[case] surrogate gradients", flush=True)
+    torch.manual_seed(0)
+    net = SG.SurrogateNet(cuda, seed=3)
+    for b in net.b:
+        b.normal_(0, 0.1)
+    net.refresh()
+    B = 512
+    tr = SG.Trainer(net, batch=B)
+    Xn = torch.zeros((B, 32), dtype=torch.bfloat16, device=cuda)
+    Xn[:, :SG.N_IN] = torch.randn(B, SG.N_IN, device=cuda).to(torch.bfloat16)
+    Tn = torch.randn(B, SG.N_OUT, device=cuda)
+    Y = tr.forward(Xn, B).clone()
+    tr.backward(Xn, Tn, B)
+    # CPU autograd on the same bf16 weights (parameters as leaves)
+    W = [w.to(torch.bfloat16).float().cpu().requires_grad_() for w in net.W]
+    bb = [x.cpu().clone().requires_grad_() for x in net.b]
+    a = Xn.float().cpu()[:, :SG.N_IN]
+    for l in range(3):
+        a = torch.nn.functional.silu(a @ W[l].t() + bb[l])
+    Yc = a @ W[3].t() + bb[3]
+    loss = ((Yc - Tn.cpu()) ** 2).mean()
+    loss.backward()
+    assert torch.allclose(Y.cpu(), Yc.detach(), rtol=3e-2, atol=3e-2)
+    for l in range(4):
+        g_gpu = tr.gW[l].cpu()
+        g_cpu = W[l].grad
+        rel = float((g_gpu - g_cpu).norm() / g_cpu.norm())
+        print(f"  layer {l}: relative gradient error {rel:.2e}")
+        assert rel < 5e-2
+        relb = float((tr.gb[l].cpu() - bb[l].grad).norm() / bb[l].grad.norm())
+        assert relb < 5e-2
+
+
+def test_inference_kernel_matches_cpu_forward(cuda):
+    print("[case] fused surrogate inference", flush=True)
+    from hrdayapy_amd.layout import BrickLayout
+    S = np.zeros((12, 10, 9), bool)
+    S[1:11, 1:9, 1:8] = True
+    lay = BrickLayout(S, (4, 4, 4))
+    n = lay.n_nodes
+    rng = np.random.default_rng(7)
+    net = SG.SurrogateNet(cuda, seed=5)
+    for b in net.b:
+        b.normal_(0, 0.05)
+    # normalisation from a random state cloud around rest/plateau
+    V = torch.from_numpy(rng.uniform(-88, 20, n)).float()
+    st = torch.tensor([[SG.INIT[k] for k in SG.STATE_NAMES]]).repeat(n, 1).float()
+    st *= torch.from_numpy(rng.uniform(0.8, 1.2, (n, 18))).float()
+    ct = torch.from_numpy(rng.integers(0, 3, n))
+    X = SG.SurrogateNet.features(V, st, ct)
+    Xt = SG.SurrogateNet.transform(X)
+    D = torch.randn(n, SG.N_OUT) * 0.01
+    net.fit_normalisation(Xt, D)
+    # engine-like buffers
+    Vd = torch.from_numpy(lay.to_slots(V.numpy())).to(cuda)
+    Vprev = Vd.clone()
+    Vprev[torch.from_numpy(lay.slot_of_node[:10].astype(np.int64)).to(cuda)] -= 5.0  # fallback
+    stc = st.clone()
+    stc[:, 0] -= SG.KI_BASE
+    stc[:, 1] -= SG.NAI_BASE
+    state = torch.from_numpy(lay.to_slots(stc.t().numpy())).to(cuda).contiguous()
+    state0 = state.clone()
+    lab = torch.from_numpy(lay.labels(ct.numpy().astype(np.uint8))).to(cuda)
+    nbr = torch.from_numpy(lay.nbr).to(cuda)
+    br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None)
+    rate = torch.zeros(lay.n_slots, device=cuda)
+    sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, 0)
+    import ctypes
+    nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
+             state.data_ptr(), lay.n_slots, rate.data_ptr(), ctypes.byref(net.hp_w),
+             ctypes.byref(net.hp_norm), ctypes.byref(sp), 4, nat.stream_handle())
+    sl = torch.from_numpy(lay.slot_of_node.astype(np.int64))
+    r = rate.cpu()[sl]
+    Vn = V.numpy()
+    fb = np.zeros(n, bool)
+    fb[:10] = True
+    fb |= (Vn > -70) & (Vn < -40)
+    assert np.array_equal(np.isnan(r.numpy()), fb)
+    # CPU forward (bf16 inputs, as the kernel rounds them)
+    Xn = ((Xt - torch.from_numpy(net.in_mean)) / torch.from_numpy(net.in_std)).to(torch.bfloat16)
+    Yc, _ = _cpu_forward(net, Xn)
+    Dc = Yc * torch.from_numpy(net.out_std) + torch.from_numpy(net.out_mean)
+    ok = ~fb
+    np.testing.assert_allclose(r.numpy()[ok], (Dc[:, 0] / 0.2).numpy()[ok], rtol=5e-2, atol=2e-3)
+    new = state.cpu()[:, sl].t()
+    old = state0.cpu()[:, sl].t()
+    # untouched on fallback voxels
+    assert torch.equal(new[torch.from_numpy(fb)], old[torch.from_numpy(fb)])
+    # updated as f^-1(f(x) + d) elsewhere
+    lg = torch.from_numpy(SG.IN_LOG[1:19].astype(bool))
+    oa = old.clone()
+    oa[:, 0] += SG.KI_BASE
+    oa[:, 1] += SG.NAI_BASE
+    exp_new = oa + Dc[:, 1:]
+    exp_new[:, lg] = torch.exp(torch.log(oa[:, lg]) + Dc[:, 1:][:, lg])
+    na = new.clone()
+    na[:, 0] += SG.KI_BASE
+    na[:, 1] += SG.NAI_BASE
+    okt = torch.from_numpy(ok)
+    rel = ((na[okt] - exp_new[okt]).abs() / (exp_new[okt].abs() + 1e-9)).max()
+    assert float(rel) < 2e-2
+
+
+def test_learned_model_beats_same_cost_step(cuda):
+    """Trains on trajectories of the plain integrator (small slab) and checks the
+    network against one 0.2 ms Rush-Larsen step on held-out states."""
+    print("[case] surrogate training + ablation", flush=True)
+    from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
+        build_coupled_system)
+    from tests._problems import corner_region, slab, stub_tree
+    S, phi, f = slab(24, 20, 9)
+    nodes, el, act = stub_tree(S, 0.25)
+    kw = dict(voxel_size=0.25, dx_p=0.25, sigma_P=0.0, sigma_l=0.24, sigma_t=0.08, c_pmj=0.0,
+              T=400.0, ectopic_region=corner_region(S),
+              stim_protocol=[dict(name="c", target="ectopic", amp=0.52, dur=2.0, onset=0.0)])
+    eng, meta = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, device=cuda, **kw)
+    V, St, ct = SG.sample_from_engine(eng, 1500, 600, seed=1)
+    V1, S1 = SG.reaction_targets(V, St, ct)
+    D = SG.output_increments(V.double(), St.double(), V1, S1)
+    X = SG.SurrogateNet.features(V, St, ct)
+    n = X.shape[0]
+    ntr = int(0.9 * n)
+    net = SG.SurrogateNet(cuda, seed=0)
+    net.fit_normalisation(SG.SurrogateNet.transform(X[:ntr]), D[:ntr])
+    tr = SG.Trainer(net, batch=4096, lr=2e-3)
+    Xn, Tn = tr.prepare(X, D)
+    hist = tr.fit(Xn[:ntr], Tn[:ntr], epochs=8, lr_decay=0.7, log=print)
+    assert hist[-1] < 0.5 * hist[0]
+    # held-out: network dV over the macro-step vs the true 10-substep dV
+    te = slice(ntr, n)
+    B = 4096
+    pred = []
+    for i in range(ntr, n - B + 1, B):
+        pred.append(tr.forward(Xn[i:i + B].contiguous(), B).clone())
+    m = len(pred) * B
+    Yp = torch.cat(pred) * torch.from_numpy(net.out_std).to(cuda) + torch.from_numpy(
+        net.out_mean).to(cuda)
+    dv_true = D[ntr:ntr + m, 0]
+    err_net = (Yp[:, 0] - dv_true).abs().mean()
+    # same cost without the network: one Rush-Larsen step of 0.2 ms
+    Vb, Sb = V[ntr:ntr + m].double().clone(), St[ntr:ntr + m].double().t().contiguous()
+    I = torch.empty(m, dtype=torch.float64, device=cuda)
+    ctt = ct[ntr:ntr + m].to(torch.uint8).contiguous()
+    nat.call("hp_ttp06_step_f64", Vb.data_ptr(), Sb.data_ptr(), I.data_ptr(), ctt.data_ptr(), 0,
+             None, None, m, m, 1, 0.2, 0.0, 0.0, nat.stream_handle())
+    err_big = (-0.2 * I - dv_true.double()).abs().mean()
+    print(f"  held-out mean |dV error|: network {float(err_net):.3e} mV, "
+          f"one 0.2 ms step {float(err_big):.3e} mV")
+    assert float(err_net) < float(err_big)
