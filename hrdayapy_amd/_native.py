@@ -117,6 +117,12 @@ _SIGS = {
                           c_f32, c_vp]),
     "hp_surrogate_step": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, c_vp, P(HpMlpWeights),
                                     P(HpMlpNorm), P(HpSurrogateParams), C.c_int, c_vp]),
+    "hp_step_tail_f32": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, P(HpPmjAt),
+                                   c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "hp_step_tail_f64": (C.c_int, [P(HpPmj), c_vp, c_vp, c_vp, c_f64, c_f64, c_f64, P(HpPmjAt),
+                                   c_vp, c_i32, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "hp_bigstep_reaction": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, c_vp,
+                                      P(HpSurrogateParams), c_vp]),
     "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
 }

@@ -28,6 +28,7 @@ template <> struct M<float> {
   static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
   static __device__ __forceinline__ float mx(float a, float b) { return fmaxf(a, b); }
   static __device__ __forceinline__ float mn(float a, float b) { return fminf(a, b); }
   static __device__ __forceinline__ float ab(float a) { return fabsf(a); }
@@ -39,6 +40,7 @@ template <> struct M<double> {
   static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
   static __device__ __forceinline__ double dv(double a, double b) { return a / b; }
   static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
+  static __device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
   static __device__ __forceinline__ double mx(double a, double b) { return fmax(a, b); }
   static __device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
   static __device__ __forceinline__ double ab(double a) { return fabs(a); }

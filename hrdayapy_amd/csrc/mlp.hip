@@ -197,26 +197,37 @@ struct MlpSmem {
   __bf16 w3[NHID][NHID];
   __bf16 w4[32][NHID];          // 19 -> 32 rows
   float b1[NHID], b2[NHID], b3[NHID], b4[32];
+  float in_mean[32], in_rstd[32], out_mean[32], out_std[32];
+  int in_log[32];
 };
 
-__device__ __forceinline__ float feat_fwd(float x, int lg) { return lg ? __logf(fmaxf(x, 1e-30f)) : x; }
-
-// Load one input feature (0..21) of the slot, raw (not normalised).
+// Raw input feature f (0..21) of a slot: V, the 18 state variables (absolute),
+// phenotype one-hot.
 __device__ __forceinline__ float raw_feature(int f, const float* __restrict__ V,
                                              const float* __restrict__ st, int64_t ld,
                                              int64_t slot, int ct, float ki_base, float nai_base) {
   if (f == 0) return V[slot];
   if (f <= 18) {
-    float x = st[(int64_t)(f - 1) * ld + slot];
-    if (f == 1) x += ki_base;
-    if (f == 2) x += nai_base;
-    return x;
+    const float x = st[(int64_t)(f - 1) * ld + slot];
+    return f == 1 ? x + ki_base : (f == 2 ? x + nai_base : x);
   }
-  if (f <= 21) return (f - 19) == ct ? 1.f : 0.f;
-  return 0.f;
+  return (f - 19) == ct ? 1.f : 0.f;
 }
 
-__global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
+// bias + SiLU of two C tiles (rows 4*grp..+4 of output tile m) into the B
+// fragment positions (m&1)*4..+4 of K-step m/2
+__device__ __forceinline__ void silu_pack(const floatx4& c, const float* bias, int m, int grp,
+                                          bf16x8& dst) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float z = c[q] + bias[m * 16 + grp * 4 + q];
+    dst[(m & 1) * 4 + q] = f2bf(z / (1.f + __expf(-z)));
+  }
+}
+
+constexpr int SUR_THREADS = 512, SUR_WAVES = SUR_THREADS / 64;
+
+__global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
                                                         float* __restrict__ V_prev,
                                                         float* __restrict__ state, int64_t ld,
                                                         float* __restrict__ rate,
@@ -229,95 +240,96 @@ __global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float
   const __bf16* __restrict__ W2 = (const __bf16*)wts.w2;
   const __bf16* __restrict__ W3 = (const __bf16*)wts.w3;
   const __bf16* __restrict__ W4 = (const __bf16*)wts.w4;
-  // stage weights (bf16 from global, natural order) into LDS
-  for (int e = tid; e < NHID * 32; e += 256) {
+  // stage weights (natural order in global) and the normalisation into LDS
+  for (int e = tid; e < NHID * 32; e += SUR_THREADS) {
     const int o = e / 32, i = e % 32;
     S.w1[o][i] = i < NIN ? W1[o * NIN + i] : f2bf(0.f);
   }
-  for (int e = tid; e < NHID * NHID; e += 256) {
+  for (int e = tid; e < NHID * NHID; e += SUR_THREADS) {
     const int o = e / NHID, p = e % NHID;
     const int kin = (p & ~31) + perm_k(p & 31);
     S.w2[o][p] = W2[o * NHID + kin];
     S.w3[o][p] = W3[o * NHID + kin];
   }
-  for (int e = tid; e < 32 * NHID; e += 256) {
+  for (int e = tid; e < 32 * NHID; e += SUR_THREADS) {
     const int o = e / NHID, p = e % NHID;
     const int kin = (p & ~31) + perm_k(p & 31);
     S.w4[o][p] = o < NOUT ? W4[o * NHID + kin] : f2bf(0.f);
   }
-  for (int e = tid; e < NHID; e += 256) {
+  for (int e = tid; e < NHID; e += SUR_THREADS) {
     S.b1[e] = wts.b1[e];
     S.b2[e] = wts.b2[e];
     S.b3[e] = wts.b3[e];
   }
-  if (tid < 32) S.b4[tid] = tid < NOUT ? wts.b4[tid] : 0.f;
+  if (tid < 32) {
+    S.b4[tid] = tid < NOUT ? wts.b4[tid] : 0.f;
+    S.in_mean[tid] = tid < NIN ? nrm.in_mean[tid] : 0.f;
+    S.in_rstd[tid] = tid < NIN ? nrm.in_rstd[tid] : 0.f;
+    S.in_log[tid] = tid < NIN ? nrm.in_log[tid] : 0;
+    S.out_mean[tid] = tid < NOUT ? nrm.out_mean[tid] : 0.f;
+    S.out_std[tid] = tid < NOUT ? nrm.out_std[tid] : 0.f;
+  }
   __syncthreads();
 
   const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
   const int col = lane & 15, grp = lane >> 4;
   const float dtm = sp.t_macro;
-  // persistent loop: one 32-voxel tile per wave per iteration
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + wave; tile * 32 < n_slots; tile += (int64_t)gridDim.x * 4) {
+  // persistent loop: one 32-voxel tile (two 16-voxel MFMA columns) per wave
+  for (int64_t tile = (int64_t)blockIdx.x * SUR_WAVES + wave; tile * 32 < n_slots;
+       tile += (int64_t)gridDim.x * SUR_WAVES) {
     const int64_t base = tile * 32;
-    // ---- inputs: B fragments X^T (32 features x 16 voxels) for the 2 voxel tiles
-    bf16x8 xb[2];
+    const int64_t slot0 = base + col, slot1 = base + 16 + col;
+    const uint8_t lab0 = g.label[slot0], lab1 = g.label[slot1];
+    // skip tiles without myocardium (wave-uniform)
+    if (!__any((lab0 != 0xFF) || (lab1 != 0xFF))) continue;
+    // ---- inputs: B fragments (32 features x 16 voxels) of the two columns
+    bf16x8 xin[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const int64_t slot = base + t * 16 + col;
-      const uint8_t lab = g.label[slot];
-      const int ct = lab & 3;
+      const int64_t slot = t ? slot1 : slot0;
+      const uint8_t lab = t ? lab1 : lab0;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int f = grp * 8 + q;
         float x = 0.f;
         if (lab != 0xFF && f < NIN) {
-          x = raw_feature(f, V, state, ld, slot, ct, sp.ki_base, sp.nai_base);
-          x = (feat_fwd(x, nrm.in_log[f]) - nrm.in_mean[f]) * nrm.in_rstd[f];
+          x = raw_feature(f, V, state, ld, slot, lab & 3, sp.ki_base, sp.nai_base);
+          if (S.in_log[f]) x = __logf(fmaxf(x, 1e-30f));
+          x = (x - S.in_mean[f]) * S.in_rstd[f];
         }
-        xb[t][q] = f2bf(x);
+        xin[t][q] = f2bf(x);
       }
     }
-    // ---- layer 1: 8 output tiles x 2 voxel tiles, one K step
-    floatx4 acc[8][2];
+    // ---- layer 1 (one K step) -> hidden activations in B-fragment form
+    bf16x8 ha[4][2], hb[4][2];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const bf16x8 a = *(const bf16x8*)&S.w1[m * 16 + col][grp * 8];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) acc[m][t] = mfma16(a, xb[t], floatx4{0.f, 0.f, 0.f, 0.f});
+      const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+      silu_pack(mfma16(a, xin[0], z), S.b1, m, grp, ha[m >> 1][0]);
+      silu_pack(mfma16(a, xin[1], z), S.b1, m, grp, ha[m >> 1][1]);
     }
-    // ---- hidden layers 2, 3: SiLU(acc + b) -> B fragments -> MFMA
-    bf16x8 hb[4][2];
-    auto act_to_b = [&](const float* bias) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int m = 2 * j + (q >> 2), r = grp * 4 + (q & 3);
-            const float z = acc[m][t][q & 3] + bias[m * 16 + r];
-            hb[j][t][q] = f2bf(z / (1.f + __expf(-z)));
-          }
-    };
-    act_to_b(S.b1);
+    // ---- layers 2 and 3
 #pragma unroll
     for (int layer = 0; layer < 2; ++layer) {
       const __bf16(*W)[NHID] = layer == 0 ? S.w2 : S.w3;
+      const float* bias = layer == 0 ? S.b2 : S.b3;
+      bf16x8(&hin)[4][2] = layer == 0 ? ha : hb;
+      bf16x8(&hout)[4][2] = layer == 0 ? hb : ha;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const bf16x8 a = *(const bf16x8*)&W[m * 16 + col][j * 32 + grp * 8];
-          c0 = mfma16(a, hb[j][0], c0);
-          c1 = mfma16(a, hb[j][1], c1);
+          c0 = mfma16(a, hin[j][0], c0);
+          c1 = mfma16(a, hin[j][1], c1);
         }
-        acc[m][0] = c0;
-        acc[m][1] = c1;
+        silu_pack(c0, bias, m, grp, hout[m >> 1][0]);
+        silu_pack(c1, bias, m, grp, hout[m >> 1][1]);
       }
-      act_to_b(layer == 0 ? S.b2 : S.b3);
     }
-    // ---- output layer: 2 output tiles (19 used)
+    // ---- output layer (19 of 32 rows used); input is `ha` after two swaps
     floatx4 out[2][2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
@@ -325,8 +337,8 @@ __global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bf16x8 a = *(const bf16x8*)&S.w4[m * 16 + col][j * 32 + grp * 8];
-        c0 = mfma16(a, hb[j][0], c0);
-        c1 = mfma16(a, hb[j][1], c1);
+        c0 = mfma16(a, ha[j][0], c0);
+        c1 = mfma16(a, ha[j][1], c1);
       }
       out[m][0] = c0;
       out[m][1] = c1;
@@ -334,8 +346,8 @@ __global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float
     // ---- mode decision and update, per voxel (this lane's 4+4 outputs)
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const int64_t slot = base + t * 16 + col;
-      const uint8_t lab = g.label[slot];
+      const int64_t slot = t ? slot1 : slot0;
+      const uint8_t lab = t ? lab1 : lab0;
       if (lab == 0xFF) continue;
       const float v = V[slot], vp = V_prev[slot];
       const float dvdt = (v - vp) / dtm;
@@ -347,28 +359,57 @@ __global__ void __launch_bounds__(256) surrogate_kernel(hp_bricks g, const float
         for (int q = 0; q < 4; ++q) {
           const int o = m * 16 + grp * 4 + q;
           if (o >= NOUT) continue;
-          const float d = (out[m][t][q] + S.b4[o]) * nrm.out_std[o] + nrm.out_mean[o];
+          const float d = (out[m][t][q] + S.b4[o]) * S.out_std[o] + S.out_mean[o];
           if (o == 0) {
             rate[slot] = fallback ? __builtin_nanf("") : d / dtm;
           } else if (!fallback) {
             float* p = state + (int64_t)(o - 1) * ld + slot;
             const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
             const float x0 = *p + base_v;
-            float x1;
-            if (nrm.in_log[o]) x1 = __expf(__logf(fmaxf(x0, 1e-30f)) + d);
-            else x1 = x0 + d;
+            const float x1 = S.in_log[o] ? __expf(__logf(fmaxf(x0, 1e-30f)) + d) : x0 + d;
             *p = x1 - base_v;
           }
         }
     }
     // V_prev of this tile was consumed above (the mode decision depends on it),
     // so the loads have completed before these stores are issued
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int64_t slot = base + t * 16 + col;
-      if (grp == 0 && g.label[slot] != 0xFF) V_prev[slot] = V[slot];
+    if (grp == 0) {
+      if (lab0 != 0xFF) V_prev[slot0] = V[slot0];
+      if (lab1 != 0xFF) V_prev[slot1] = V[slot1];
     }
   }
+}
+
+// Multirate alternative without the network, same decision rule: one
+// Rush-Larsen/forward-Euler step of t_macro for the non-fallback voxels.
+__global__ void __launch_bounds__(256) bigstep_kernel(hp_bricks g, const float* __restrict__ V,
+                                                      float* __restrict__ V_prev,
+                                                      float* __restrict__ state, int64_t ld,
+                                                      float* __restrict__ rate,
+                                                      hp_surrogate_params sp) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
+  if (slot >= n_slots) return;
+  const uint8_t lab = g.label[slot];
+  if (lab == 0xFF) return;
+  const float v = V[slot], vp = V_prev[slot];
+  const float dtm = sp.t_macro;
+  const bool fallback = sp.force_fallback || fabsf((v - vp) / dtm) > sp.dvdt_max ||
+                        (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
+  V_prev[slot] = v;
+  if (fallback) {
+    rate[slot] = __builtin_nanf("");
+    return;
+  }
+  const int ct = lab & 3;
+  float s[N_STATE];
+#pragma unroll
+  for (int k = 0; k < N_STATE; ++k) s[k] = state[(int64_t)k * ld + slot];
+  const float I = ttp06_cell<float, false>(v, s, dtm, (float)kGKs[ct], (float)kGto[ct], ct != 2,
+                                           sp.ki_base, sp.nai_base);
+#pragma unroll
+  for (int k = 0; k < N_STATE; ++k) state[(int64_t)k * ld + slot] = s[k];
+  rate[slot] = -I;
 }
 
 }  // namespace hp
@@ -463,12 +504,24 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     attr_set = true;
   }
-  if (n_blocks <= 0) n_blocks = 512;
+  if (n_blocks <= 0) n_blocks = 256;
   const int64_t tiles = n_slots / 32;
-  if ((int64_t)n_blocks * 4 > tiles) n_blocks = (int)((tiles + 3) / 4);
-  hipLaunchKernelGGL(surrogate_kernel, dim3(n_blocks), dim3(256), smem, (hipStream_t)stream, *g, V,
+  if ((int64_t)n_blocks * SUR_WAVES > tiles) n_blocks = (int)((tiles + SUR_WAVES - 1) / SUR_WAVES);
+  hipLaunchKernelGGL(surrogate_kernel, dim3(n_blocks), dim3(SUR_THREADS), smem, (hipStream_t)stream, *g, V,
                      V_prev, state, ld, rate, *w, *n, *sp);
   return check_launch("hp_surrogate_step");
+}
+
+int hp_bigstep_reaction(const hp_bricks* g, const float* V, float* V_prev, float* state,
+                        int64_t ld, float* rate, const hp_surrogate_params* sp, void* stream) {
+  if (!g || !sp || g->n_bricks <= 0) {
+    set_error("hp_bigstep_reaction: bad arguments");
+    return -1;
+  }
+  const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
+  hipLaunchKernelGGL(bigstep_kernel, dim3(ceil_div(n_slots, 256)), dim3(256), 0,
+                     (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *sp);
+  return check_launch("hp_bigstep_reaction");
 }
 
 }  // extern "C"

@@ -97,7 +97,7 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
       if (t.L[nb] != 0xFF) {
         const T Db = t.D[a][nb];
         const T s = Da + Db;
-        const T dh = T(2) * Da * Db / (s > T(1e-30) ? s : T(1e-30));
+        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
         acc += dh * (t.V[nb] - Vc);
       }
     }
@@ -268,11 +268,9 @@ __global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __
 // PMJ exchange: exact integration of the junction conductances over dt.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __restrict__ V_myo,
-                                    T* __restrict__ scratch, double dt, double vmin,
-                                    double vmax) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= j.n_junctions) return;
+__device__ __forceinline__ void pmj_junction(const hp_pmj& j, int64_t q, T* __restrict__ V_pk,
+                                             const T* __restrict__ V_myo, T* __restrict__ scratch,
+                                             double dt, double vmin, double vmax) {
   const int e0 = j.ptr[q], e1 = j.ptr[q + 1];
   const int n = e1 - e0;
   const int p = j.pk[q];
@@ -294,10 +292,9 @@ __global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __r
 }
 
 template <typename T>
-__global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __restrict__ scratch,
-                                  double vmin, double vmax, hp_pmj_at at) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= j.n_targets) return;
+__device__ __forceinline__ void pmj_target(const hp_pmj& j, int64_t q, T* __restrict__ V_myo,
+                                           const T* __restrict__ scratch, double vmin, double vmax,
+                                           const hp_pmj_at& at) {
   T acc = T(0);
   for (int k = j.tgt_ptr[q]; k < j.tgt_ptr[q + 1]; ++k) acc += scratch[j.tgt_entry[k]];
   const int s = j.tgt_slot[q];
@@ -313,6 +310,21 @@ __global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __re
       a[s] = (T)((double)(step - 1) * at.dt) + T(at.dt) * (thr - v0) / (v - v0);
     }
   }
+}
+
+template <typename T>
+__global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __restrict__ V_myo,
+                                    T* __restrict__ scratch, double dt, double vmin,
+                                    double vmax) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < j.n_junctions) pmj_junction<T>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax);
+}
+
+template <typename T>
+__global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __restrict__ scratch,
+                                  double vmin, double vmax, hp_pmj_at at) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < j.n_targets) pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,6 +358,42 @@ __global__ void stim_kernel(const double* __restrict__ sp, int n_sites,
     amp[g] = (T)a;
   }
   if (advance && g == 0) *step_ctr = step;
+}
+
+// One workgroup finishes the step: PMJ junctions, PMJ targets, then the
+// step counter and the stimulus amplitudes of the next step (replaces three
+// latency-bound launches; used while the junction count fits one block well).
+constexpr int TAIL_THREADS = 1024;
+constexpr int64_t TAIL_MAX_JUNCTIONS = 65536;
+
+template <typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
+    hp_pmj j, int has_pmj, T* __restrict__ V_pk, T* __restrict__ V_myo, T* __restrict__ scratch,
+    double dt, double vmin, double vmax, hp_pmj_at at, const double* __restrict__ sp, int n_sites,
+    const uint32_t* __restrict__ gs, int n_groups, int64_t* step_ctr, T* __restrict__ amp) {
+  const int tid = threadIdx.x;
+  if (has_pmj) {
+    for (int64_t q = tid; q < j.n_junctions; q += TAIL_THREADS)
+      pmj_junction<T>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax);
+    __syncthreads();
+    for (int64_t q = tid; q < j.n_targets; q += TAIL_THREADS)
+      pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at);
+    __syncthreads();
+  }
+  const int64_t step = *step_ctr + 1;
+  __syncthreads();
+  if (tid < 32) {
+    double a = 0.0;
+    if (tid > 0 && tid < n_groups) {
+      const double t = (double)step * dt;
+      const uint32_t mask = gs[tid];
+      for (int s = 0; s < n_sites; ++s)
+        if ((mask >> s) & 1u)
+          if (site_active(sp + 5 * s, t)) a += sp[5 * s + 4];
+    }
+    amp[tid] = (T)a;
+  }
+  if (tid == 0) *step_ctr = step;
 }
 
 template <typename T>
@@ -408,6 +456,26 @@ static int stim(const double* sp, int n_sites, const uint32_t* gs, int n_groups,
   hipLaunchKernelGGL(stim_kernel<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, sp, n_sites, gs,
                      n_groups, (int64_t*)ctr, dt, amp, advance);
   return check_launch("hp_stim");
+}
+
+template <typename T>
+static int step_tail(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, double vmin,
+                     double vmax, const hp_pmj_at* at, const double* sp, int n_sites,
+                     const uint32_t* gs, int n_groups, int64_t* ctr, T* amp, void* stream) {
+  if (n_sites < 0 || n_sites > 32 || n_groups < 0 || n_groups > 32 || !ctr || !amp) {
+    set_error("hp_step_tail: at most 32 sites and 32 groups");
+    return -1;
+  }
+  const bool has = j && j->n_junctions > 0 && j->c_pmj != 0.0;
+  if (has && j->n_junctions > TAIL_MAX_JUNCTIONS) {  // large trees: multi-block path
+    int rc = pmj_couple<T>(j, V_pk, V_myo, scratch, dt, vmin, vmax, at, stream);
+    if (rc) return rc;
+    return stim<T>(sp, n_sites, gs, n_groups, ctr, dt, amp, 1, stream);
+  }
+  hipLaunchKernelGGL(step_tail_kernel<T>, dim3(1), dim3(TAIL_THREADS), 0, (hipStream_t)stream,
+                     has ? *j : hp_pmj{}, has ? 1 : 0, V_pk, V_myo, scratch, dt, vmin, vmax,
+                     at ? *at : hp_pmj_at{}, sp, n_sites, gs, n_groups, ctr, amp);
+  return check_launch("hp_step_tail");
 }
 
 template <typename T>
@@ -474,6 +542,20 @@ int hp_step_advance_f32(const double* sp, int32_t ns, const uint32_t* gs, int32_
 int hp_step_advance_f64(const double* sp, int32_t ns, const uint32_t* gs, int32_t ng,
                         int64_t* ctr, double dt, double* amp, void* stream) {
   return stim<double>(sp, ns, gs, ng, ctr, dt, amp, 1, stream);
+}
+int hp_step_tail_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
+                     double vmin, double vmax, const hp_pmj_at* at, const double* sp,
+                     int32_t ns, const uint32_t* gs, int32_t ng, int64_t* ctr, float* amp,
+                     void* stream) {
+  return step_tail<float>(j, V_pk, V_myo, scratch, dt, vmin, vmax, at, sp, ns, gs, ng, ctr, amp,
+                          stream);
+}
+int hp_step_tail_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
+                     double vmin, double vmax, const hp_pmj_at* at, const double* sp,
+                     int32_t ns, const uint32_t* gs, int32_t ng, int64_t* ctr, double* amp,
+                     void* stream) {
+  return step_tail<double>(j, V_pk, V_myo, scratch, dt, vmin, vmax, at, sp, ns, gs, ng, ctr, amp,
+                           stream);
 }
 int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream) {
   return gather<float>(src, idx, out, n, stream);

@@ -9,6 +9,18 @@
 // The rate expressions are the published model equations and are therefore
 // the same as the reference's.
 //
+// Two algebraic forms of the same equations:
+//   kRef = true  (FP64 kernels): the reference's own arrangement, so that the
+//                FP64 GPU path meets the FP64 oracle at round-off level;
+//   kRef = false (FP32 kernels): reciprocals merged where the model allows it
+//                -- Nernst potentials as differences of logs, Rush-Larsen
+//                exponents -dt/tau computed directly from the rate
+//                (h, j: -dt*(alpha+beta); Xr1, Xr2, m: products of the
+//                sigmoid denominators), buffer factors q^2/(q^2+c), one
+//                reciprocal for x_K1,inf, R' opening, I_up, I_NaK -- which
+//                removes about a third of the v_rcp_f32 instructions.  The
+//                FP32 path uses the hardware exp/log/rcp (1-2 ulp).
+//
 // Storage of Ki and Nai: the state holds Ki - ki_base and Nai - nai_base.
 // With base 0 this is the reference's state.  The FP32 engine passes the
 // resting values (138.4, 10.355) so that the per-step increments (~1e-5 mM,
@@ -31,7 +43,7 @@ enum StateCol {
 __device__ __constant__ static const double kGKs[3] = {0.392, 0.098, 0.392};
 __device__ __constant__ static const double kGto[3] = {0.073, 0.294, 0.294};
 
-template <typename T, bool kExactCaL>
+template <typename T, bool kRef>
 __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
                                          bool endo_s, T ki_base, T nai_base) {
   typedef M<T> m_;
@@ -49,20 +61,39 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
   const T fCass = s[S_FCASS], sg = s[S_S], rg = s[S_R], CaSR = s[S_CASR], Rp = s[S_RPRIME];
   const T Cass = s[S_CASS];
   const T one = C(1.0);
+  // Rush-Larsen with the exponent supplied: x_inf - (x_inf - x) exp(-dt/tau)
+#define RLE(x, xi, ndt_over_tau) ((xi) - ((xi) - (x)) * m_::ex(ndt_over_tau))
 
   // Nernst potentials (gpu_ttp06.py:242-247)
-  const T E_Na = RTF * m_::lg(m_::dv(Nao, Nai));
-  const T E_K = RTF * m_::lg(m_::dv(Ko, Ki));
-  const T E_Ks = RTF * m_::lg(m_::dv(Ko + C(0.03) * Nao, Ki + C(0.03) * Nai));
-  const T E_Ca = C(0.5) * RTF * m_::lg(m_::dv(Cao, Cai));
+  T E_Na, E_K, E_Ks, E_Ca;
+  if (kRef) {
+    E_Na = RTF * m_::lg(m_::dv(Nao, Nai));
+    E_K = RTF * m_::lg(m_::dv(Ko, Ki));
+    E_Ks = RTF * m_::lg(m_::dv(Ko + C(0.03) * Nao, Ki + C(0.03) * Nai));
+    E_Ca = C(0.5) * RTF * m_::lg(m_::dv(Cao, Cai));
+  } else {
+    E_Na = RTF * (C(4.941642422609304) - m_::lg(Nai));          // log(140)
+    E_K = RTF * (C(1.6863989535702288) - m_::lg(Ki));           // log(5.4)
+    E_Ks = RTF * (C(2.2082744135228043) - m_::lg(Ki + C(0.03) * Nai));  // log(5.4+0.03*140)
+    E_Ca = C(0.5) * RTF * (C(0.6931471805599453) - m_::lg(Cai));  // log(2)
+  }
 
   // I_Na (:250-276)
+  const T e_m1 = m_::ex((C(-60.0) - V) * C(0.2));
+  const T e_m2 = m_::ex((V + C(35.0)) * C(0.2));
+  const T e_m3 = m_::ex((V - C(50.0)) * C(1.0 / 200.0));
   T t1 = m_::rcp(one + m_::ex((C(-56.86) - V) * C(1.0 / 9.03)));
   const T m_inf = t1 * t1;
-  const T a_m = m_::rcp(one + m_::ex((C(-60.0) - V) * C(0.2)));
-  const T b_m = C(0.1) * m_::rcp(one + m_::ex((V + C(35.0)) * C(0.2))) +
-                C(0.1) * m_::rcp(one + m_::ex((V - C(50.0)) * C(1.0 / 200.0)));
-  const T tau_m = a_m * b_m;
+  T ndt_m;  // -dt / tau_m, tau_m = a_m * b_m
+  if (kRef) {
+    const T a_m = m_::rcp(one + e_m1);
+    const T b_m = C(0.1) * m_::rcp(one + e_m2) + C(0.1) * m_::rcp(one + e_m3);
+    ndt_m = -dt * m_::rcp(a_m * b_m);
+  } else {
+    // 1/tau_m = (1+e1)(1+e2)(1+e3) / (0.1 (2 + e2 + e3))
+    ndt_m = -dt * (one + e_m1) * (one + e_m2) * (one + e_m3) *
+            m_::rcp(C(0.1) * (C(2.0) + e_m2 + e_m3));
+  }
   t1 = m_::rcp(one + m_::ex((V + C(71.55)) * C(1.0 / 7.43)));
   const T h_inf = t1 * t1;
   T a_h, b_h, a_j, b_j;
@@ -79,32 +110,59 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
     a_j = C(0.0);
     b_j = m_::dv(C(0.6) * m_::ex(C(0.057) * V), one + m_::ex(C(-0.1) * (V + C(32.0))));
   }
-  const T tau_h = m_::rcp(a_h + b_h + C(1e-30));
-  const T tau_j = m_::rcp(a_j + b_j + C(1e-30));
+  T ndt_h, ndt_j;
+  if (kRef) {
+    ndt_h = -dt * m_::rcp(m_::rcp(a_h + b_h + C(1e-30)));
+    ndt_j = -dt * m_::rcp(m_::rcp(a_j + b_j + C(1e-30)));
+  } else {
+    ndt_h = -dt * (a_h + b_h + C(1e-30));
+    ndt_j = -dt * (a_j + b_j + C(1e-30));
+  }
   const T i_Na = C(14.838) * mg * mg * mg * hg * jg * (V - E_Na);
 
   // I_Kr (:279-287); sqrt(Ko/5.4) = 1
   const T xr1_inf = m_::rcp(one + m_::ex((C(-26.0) - V) * C(1.0 / 7.0)));
-  const T tau_xr1 = C(450.0) * m_::rcp(one + m_::ex((C(-45.0) - V) * C(0.1))) * C(6.0) *
-                    m_::rcp(one + m_::ex((V + C(30.0)) * C(1.0 / 11.5)));
+  const T e_r1 = m_::ex((C(-45.0) - V) * C(0.1)), e_r2 = m_::ex((V + C(30.0)) * C(1.0 / 11.5));
   const T xr2_inf = m_::rcp(one + m_::ex((V + C(88.0)) * C(1.0 / 24.0)));
-  const T tau_xr2 = C(3.0) * m_::rcp(one + m_::ex((C(-60.0) - V) * C(0.05))) * C(1.12) *
-                    m_::rcp(one + m_::ex((V - C(60.0)) * C(0.05)));
+  const T e_r3 = m_::ex((C(-60.0) - V) * C(0.05)), e_r4 = m_::ex((V - C(60.0)) * C(0.05));
+  T ndt_xr1, ndt_xr2;
+  if (kRef) {
+    const T tau_xr1 = C(450.0) * m_::rcp(one + e_r1) * C(6.0) * m_::rcp(one + e_r2);
+    const T tau_xr2 = C(3.0) * m_::rcp(one + e_r3) * C(1.12) * m_::rcp(one + e_r4);
+    ndt_xr1 = -dt * m_::rcp(tau_xr1);
+    ndt_xr2 = -dt * m_::rcp(tau_xr2);
+  } else {
+    ndt_xr1 = -dt * (one + e_r1) * (one + e_r2) * C(1.0 / 2700.0);
+    ndt_xr2 = -dt * (one + e_r3) * (one + e_r4) * C(1.0 / 3.36);
+  }
   const T vek = V - E_K;
   const T i_Kr = g_Kr * Xr1 * Xr2 * vek;
 
   // I_Ks (:290-294)
   const T xs_inf = m_::rcp(one + m_::ex((C(-5.0) - V) * C(1.0 / 14.0)));
-  const T a_xs = m_::dv(C(1400.0), m_::sq(one + m_::ex((C(5.0) - V) * C(1.0 / 6.0))));
+  const T e_xs = m_::ex((C(5.0) - V) * C(1.0 / 6.0));
+  T a_xs;
+  if (kRef) a_xs = m_::dv(C(1400.0), m_::sq(one + e_xs));
+  else a_xs = C(1400.0) * m_::rsq(one + e_xs);
   const T b_xs = m_::rcp(one + m_::ex((V - C(35.0)) * C(1.0 / 15.0)));
   const T tau_xs = a_xs * b_xs + C(80.0);
   const T i_Ks = g_Ks * Xs * Xs * (V - E_Ks);
 
   // I_K1 (:297-302)
-  const T a_k1 = C(0.1) * m_::rcp(one + m_::ex(C(0.06) * (vek - C(200.0))));
-  const T b_k1 = m_::dv(C(3.0) * m_::ex(C(2e-4) * (vek + C(100.0))) + m_::ex(C(0.1) * (vek - C(10.0))),
-                        one + m_::ex(C(-0.5) * vek));
-  const T i_K1 = C(5.405) * m_::dv(a_k1, a_k1 + b_k1) * vek;
+  const T e_k1 = m_::ex(C(0.06) * (vek - C(200.0)));
+  const T nb_k1 = C(3.0) * m_::ex(C(2e-4) * (vek + C(100.0))) + m_::ex(C(0.1) * (vek - C(10.0)));
+  const T e_k2 = m_::ex(C(-0.5) * vek);
+  T xk1_inf;
+  if (kRef) {
+    const T a_k1 = C(0.1) * m_::rcp(one + e_k1);
+    const T b_k1 = m_::dv(nb_k1, one + e_k2);
+    xk1_inf = m_::dv(a_k1, a_k1 + b_k1);
+  } else {
+    // a/(a+b) = 0.1(1+e2) / (0.1(1+e2) + nb (1+e1))
+    const T num = C(0.1) * (one + e_k2);
+    xk1_inf = num * m_::rcp(num + nb_k1 * (one + e_k1));
+  }
+  const T i_K1 = C(5.405) * xk1_inf * vek;
 
   // I_to and the phenotype s-gate (:305-317)
   T s_inf, tau_s;
@@ -130,21 +188,22 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
                   m_::rcp(one + m_::ex((C(50.0) - V) * C(0.05)));
   const T f_inf = m_::rcp(one + m_::ex((V + C(20.0)) * C(1.0 / 7.0)));
   const T u27 = V + C(27.0);
+  const T r30 = m_::rcp(one + m_::ex((V + C(30.0)) * C(0.1)));
   const T tau_f = C(1102.5) * m_::ex(-(u27 * u27) * C(1.0 / 225.0)) +
-                  C(200.0) * m_::rcp(one + m_::ex((C(13.0) - V) * C(0.1))) +
-                  C(180.0) * m_::rcp(one + m_::ex((V + C(30.0)) * C(0.1))) + C(20.0);
+                  C(200.0) * m_::rcp(one + m_::ex((C(13.0) - V) * C(0.1))) + C(180.0) * r30 +
+                  C(20.0);
   const T f2_inf = C(0.67) * m_::rcp(one + m_::ex((V + C(35.0)) * C(1.0 / 7.0))) + C(0.33);
   const T tau_f2 = C(562.0) * m_::ex(-(u27 * u27) * C(1.0 / 240.0)) +
-                   C(31.0) * m_::rcp(one + m_::ex((C(25.0) - V) * C(0.1))) +
-                   C(80.0) * m_::rcp(one + m_::ex((V + C(30.0)) * C(0.1)));
+                   C(31.0) * m_::rcp(one + m_::ex((C(25.0) - V) * C(0.1))) + C(80.0) * r30;
   const T cs = Cass * C(20.0);  // Cass / 0.05
   const T cs2 = cs * cs;
-  const T fCass_inf = C(0.6) * m_::rcp(one + cs2) + C(0.4);
-  const T tau_fCass = C(80.0) * m_::rcp(one + cs2) + C(2.0);
+  const T rcs = m_::rcp(one + cs2);
+  const T fCass_inf = C(0.6) * rcs + C(0.4);
+  const T tau_fCass = C(80.0) * rcs + C(2.0);
   const T xcal = C(2.0) * (V - C(15.0)) * FRT;
   const T e2v = m_::ex(xcal);
   T i_CaL;
-  if (kExactCaL) {
+  if (kRef) {
     T den = e2v - one;
     if (m_::ab(den) < C(1e-10)) den = C(1e-10);
     i_CaL = g_CaL * d * f * f2 * fCass * C(4.0) * (V - C(15.0)) * F * F / (R * TK) *
@@ -161,8 +220,10 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
   const T i_bCa = C(0.000592) * (V - E_Ca);
   const T i_pCa = m_::dv(C(0.1238) * Cai, Cai + C(0.0005));
   const T i_pK = m_::dv(C(0.0146) * vek, one + m_::ex((C(25.0) - V) * C(1.0 / 5.98)));
-  const T i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * m_::dv(Nai, Nai + C(40.0)) *
-                  m_::rcp(one + C(0.1245) * m_::ex(C(-0.1) * V * FRT) + C(0.0353) * m_::ex(-V * FRT));
+  const T nak_den = one + C(0.1245) * m_::ex(C(-0.1) * V * FRT) + C(0.0353) * m_::ex(-V * FRT);
+  T i_NaK;
+  if (kRef) i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * m_::dv(Nai, Nai + C(40.0)) * m_::rcp(nak_den);
+  else i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * Nai * m_::rcp((Nai + C(40.0)) * nak_den);
   const T e_g = m_::ex(C(0.35) * V * FRT);
   const T e_gm1 = m_::ex(C(0.35 - 1.0) * V * FRT);
   const T i_NaCa = m_::dv(C(1000.0) * (e_g * Nai * Nai * Nai * Cao - e_gm1 * C(140.0 * 140.0 * 140.0) * Cai * C(2.5)),
@@ -173,22 +234,37 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
   const T I_ion = i_K1 + i_to + i_Kr + i_Ks + i_CaL + i_NaK + i_Na + i_bNa + i_NaCa + i_bCa + i_pCa + i_pK;
 
   // SR / subspace Ca (:364-375)
-  const T ec = m_::dv(C(1.5), CaSR);
-  const T kcasr = C(2.5) - C(1.5) * m_::rcp(one + ec * ec);
-  const T k1 = m_::dv(C(0.15), kcasr);
-  const T k2 = C(0.045) * kcasr;
   const T c2 = Cass * Cass;
-  const T O = m_::dv(k1 * c2 * Rp, C(0.06) + k1 * c2 + C(1e-30));
+  T kcasr, O;
+  if (kRef) {
+    const T ec = m_::dv(C(1.5), CaSR);
+    kcasr = C(2.5) - C(1.5) * m_::rcp(one + ec * ec);
+    const T k1 = m_::dv(C(0.15), kcasr);
+    O = m_::dv(k1 * c2 * Rp, C(0.06) + k1 * c2 + C(1e-30));
+  } else {
+    const T sr2 = CaSR * CaSR;
+    kcasr = C(2.5) - C(1.5) * sr2 * m_::rcp(sr2 + C(2.25));
+    // k1 = 0.15/kcasr folded in: O = 0.15 c2 Rp / (0.06 kcasr + 0.15 c2)
+    O = C(0.15) * c2 * Rp * m_::rcp(C(0.06) * kcasr + C(0.15) * c2 + C(1e-30));
+  }
+  const T k2 = C(0.045) * kcasr;
   const T i_rel = C(0.102) * O * (CaSR - Cass);
-  const T i_up = m_::dv(C(0.006375), one + m_::dv(C(0.00025 * 0.00025), Cai * Cai + C(1e-30)));
+  T i_up, bufc, bufsr, bufss;
+  const T qc = Cai + C(0.001), qsr = CaSR + C(0.3), qss = Cass + C(0.00025);
+  if (kRef) {
+    i_up = m_::dv(C(0.006375), one + m_::dv(C(0.00025 * 0.00025), Cai * Cai + C(1e-30)));
+    bufc = m_::rcp(one + m_::dv(C(0.2 * 0.001), qc * qc));
+    bufsr = m_::rcp(one + m_::dv(C(10.0 * 0.3), qsr * qsr));
+    bufss = m_::rcp(one + m_::dv(C(0.4 * 0.00025), qss * qss));
+  } else {
+    const T ca2 = Cai * Cai + C(1e-30);
+    i_up = C(0.006375) * ca2 * m_::rcp(ca2 + C(0.00025 * 0.00025));
+    bufc = qc * qc * m_::rcp(qc * qc + C(0.2 * 0.001));
+    bufsr = qsr * qsr * m_::rcp(qsr * qsr + C(10.0 * 0.3));
+    bufss = qss * qss * m_::rcp(qss * qss + C(0.4 * 0.00025));
+  }
   const T i_leak = C(0.00036) * (CaSR - Cai);
   const T i_xfer = C(0.0038) * (Cass - Cai);
-  T q = Cai + C(0.001);
-  const T bufc = m_::rcp(one + m_::dv(C(0.2 * 0.001), q * q));
-  q = CaSR + C(0.3);
-  const T bufsr = m_::rcp(one + m_::dv(C(10.0 * 0.3), q * q));
-  q = Cass + C(0.00025);
-  const T bufss = m_::rcp(one + m_::dv(C(0.4 * 0.00025), q * q));
 
   // concentration derivatives (:378-384)
   const T inv_vcf = C(1.0 / (0.016404 * 96485.3415));
@@ -201,20 +277,21 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
                            i_rel * C(0.001094 / 0.00005468) - i_xfer * C(0.016404 / 0.00005468));
 
   // Rush-Larsen gates (:387-399)
-#define RL(x, xi, tau) ((xi) - ((xi) - (x)) * m_::ex(-dt * m_::rcp(tau)))
-  s[S_M] = RL(mg, m_inf, tau_m);
-  s[S_H] = RL(hg, h_inf, tau_h);
-  s[S_J] = RL(jg, h_inf, tau_j);
-  s[S_XR1] = RL(Xr1, xr1_inf, tau_xr1);
-  s[S_XR2] = RL(Xr2, xr2_inf, tau_xr2);
-  s[S_XS] = RL(Xs, xs_inf, tau_xs);
-  s[S_D] = RL(d, d_inf, tau_d);
-  s[S_F] = RL(f, f_inf, tau_f);
-  s[S_F2] = RL(f2, f2_inf, tau_f2);
-  s[S_FCASS] = RL(fCass, fCass_inf, tau_fCass);
-  s[S_S] = RL(sg, s_inf, tau_s);
-  s[S_R] = RL(rg, r_inf, tau_r);
-#undef RL
+#define NDT(tau) (-dt * m_::rcp(tau))
+  s[S_M] = RLE(mg, m_inf, ndt_m);
+  s[S_H] = RLE(hg, h_inf, ndt_h);
+  s[S_J] = RLE(jg, h_inf, ndt_j);
+  s[S_XR1] = RLE(Xr1, xr1_inf, ndt_xr1);
+  s[S_XR2] = RLE(Xr2, xr2_inf, ndt_xr2);
+  s[S_XS] = RLE(Xs, xs_inf, NDT(tau_xs));
+  s[S_D] = RLE(d, d_inf, NDT(tau_d));
+  s[S_F] = RLE(f, f_inf, NDT(tau_f));
+  s[S_F2] = RLE(f2, f2_inf, NDT(tau_f2));
+  s[S_FCASS] = RLE(fCass, fCass_inf, NDT(tau_fCass));
+  s[S_S] = RLE(sg, s_inf, NDT(tau_s));
+  s[S_R] = RLE(rg, r_inf, NDT(tau_r));
+#undef NDT
+#undef RLE
   // R' forward Euler (:402-403)
   s[S_RPRIME] = Rp + dt * (-k2 * Cass * Rp + C(0.005) * (one - Rp));
   // concentrations, forward Euler with clamps (:406-410)

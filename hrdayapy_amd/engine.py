@@ -152,6 +152,8 @@ class MonodomainEngine:
                                        nat.ptr(self.step_ctr), self.dt, float(at_thr))
                            for k in range(2)]
         self.p = 0
+        self.overlap_cable = True
+        self._side = torch.cuda.Stream(device=dev) if self.Np else None
         self._eval_stim()
 
     # ------------------------------------------------------------------ setup
@@ -180,21 +182,34 @@ class MonodomainEngine:
 
     # ------------------------------------------------------------------- step
     def _launch_step(self):
-        s = nat.stream_handle()
+        """One time step: the myocardium kernel and (on a forked stream) the
+        cable kernel run concurrently; one tail kernel then does the PMJ
+        exchange and advances the step counter / stimulus amplitudes."""
+        cur = torch.cuda.current_stream(self.dev)
+        s = cur.cuda_stream
         a, b = self.p, 1 - self.p
+        fork = self.cable is not None and self.overlap_cable
+        if fork:
+            self._side.wait_stream(cur)
+            self._launch_cable(a, b, self._side.cuda_stream)
         nat.call(f"hp_myo_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
                  nat.ptr(self.V[b]), nat.ptr(self.state), self.lay.n_slots, C.byref(self.p_myo), s)
         if self.cable is not None:
-            nat.call(f"hp_cable_step_{self.sfx}", C.byref(self.cable), nat.ptr(self.Vpk[a]),
-                     nat.ptr(self.Vpk[b]), nat.ptr(self.state_pk), self.Np, C.byref(self.p_pk), s)
-        if self.pmj is not None:
-            nat.call(f"hp_pmj_couple_{self.sfx}", C.byref(self.pmj), nat.ptr(self.Vpk[b]),
-                     nat.ptr(self.V[b]), nat.ptr(self.pmj_scratch), self.dt, self.vmin, self.vmax,
-                     C.byref(self.pmj_at[a]), s)
-        nat.call(f"hp_step_advance_{self.sfx}", nat.ptr(self.site_params), self.n_sites,
-                 nat.ptr(self.group_sites), self.n_groups, nat.ptr(self.step_ctr), self.dt,
-                 nat.ptr(self.group_amp), s)
+            if fork:
+                cur.wait_stream(self._side)
+            else:
+                self._launch_cable(a, b, s)
+        nat.call(f"hp_step_tail_{self.sfx}", None if self.pmj is None else C.byref(self.pmj),
+                 nat.ptr(self.Vpk[b]) if self.Np else None, nat.ptr(self.V[b]),
+                 nat.ptr(self.pmj_scratch) if self.pmj is not None else None, self.dt, self.vmin,
+                 self.vmax, C.byref(self.pmj_at[a]) if self.pmj is not None else None,
+                 nat.ptr(self.site_params), self.n_sites, nat.ptr(self.group_sites),
+                 self.n_groups, nat.ptr(self.step_ctr), nat.ptr(self.group_amp), s)
         self.p = b
+
+    def _launch_cable(self, a, b, s):
+        nat.call(f"hp_cable_step_{self.sfx}", C.byref(self.cable), nat.ptr(self.Vpk[a]),
+                 nat.ptr(self.Vpk[b]), nat.ptr(self.state_pk), self.Np, C.byref(self.p_pk), s)
 
     def step(self, n=1):
         for _ in range(n):

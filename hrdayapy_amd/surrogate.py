@@ -306,14 +306,24 @@ class Trainer:
 # surrogate form of the time loop
 # ---------------------------------------------------------------------------
 class SurrogateRunner:
-    """Drives a MonodomainEngine (FP32) in the surrogate form: per macro-step
-    one hp_surrogate_step, then N_SUB fused steps with do_reaction = 2."""
+    """Drives a MonodomainEngine (FP32) in a multirate form: per macro-step one
+    reaction update for the non-fallback voxels, then N_SUB fused steps with
+    do_reaction = 2 (fallback voxels run the full TTP06 every substep).
 
-    def __init__(self, eng, net: SurrogateNet, dvdt_max=1.0, v_lo=-70.0, v_hi=-40.0,
-                 n_blocks=0):
+    mode "mlp": the learned network (hp_surrogate_step);
+    mode "bigstep": one 0.2 ms Rush-Larsen step of the exact model
+    (hp_bigstep_reaction) -- the same rule without the network.
+    """
+
+    def __init__(self, eng, net: SurrogateNet = None, mode="mlp", dvdt_max=1.0, v_lo=-70.0,
+                 v_hi=-40.0, n_blocks=0):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
-        self.eng, self.net = eng, net
+        if mode not in ("mlp", "bigstep"):
+            raise ValueError(mode)
+        if mode == "mlp" and net is None:
+            raise ValueError("mode 'mlp' needs a trained SurrogateNet")
+        self.eng, self.net, self.mode = eng, net, mode
         self.rate = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
                                device=eng.dev)
         self.V_prev = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
@@ -324,12 +334,21 @@ class SurrogateRunner:
         self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
         self._graph = None
 
+    def launch_reaction(self):
+        eng = self.eng
+        if self.mode == "mlp":
+            nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
+                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
+                     nat.ptr(self.rate), C.byref(self.net.hp_w), C.byref(self.net.hp_norm),
+                     C.byref(self.sp), self.n_blocks, nat.stream_handle())
+        else:
+            nat.call("hp_bigstep_reaction", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
+                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
+                     nat.ptr(self.rate), C.byref(self.sp), nat.stream_handle())
+
     def _launch_macro(self):
         eng = self.eng
-        nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
-                 nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots, nat.ptr(self.rate),
-                 C.byref(self.net.hp_w), C.byref(self.net.hp_norm), C.byref(self.sp),
-                 self.n_blocks, nat.stream_handle())
+        self.launch_reaction()
         eng.p_myo.do_reaction = 2
         eng.p_myo.rate_ext = nat.ptr(self.rate)
         try:
@@ -364,3 +383,57 @@ class SurrogateRunner:
     def fallback_fraction(self):
         live = self.eng.label != 0xFF
         return float(torch.isnan(self.rate[live]).float().mean())
+
+
+def training_slab(device="cuda", nx=64, ny=64, nz=16, h=0.25, seed=0):
+    """The config-1 slab (64x64x16 @ 0.25 mm, sigma 0.13 isotropic, corner
+    stimulus 52 pA/pF for 2 ms) with a transmural phi across z, so that endo,
+    mid and epi cells are all sampled.  Returns an FP32 MonodomainEngine."""
+    from .anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
+        build_coupled_system)
+    from .geometry import slab, stub_tree
+    S, phi, f = slab(nx, ny, nz)
+    phi = (np.arange(nz)[None, None, :] / max(1, nz - 1) * np.ones((nx, ny, 1))).astype(np.float32)
+    region = np.zeros(S.shape, bool)
+    region[:4, :4, :4] = True
+    nodes, el, act = stub_tree(S.shape)
+    eng, _ = build_coupled_system(
+        nodes, el, act, S, phi, f, voxel_size=h, dx_p=h, sigma_P=0.0, sigma_l=0.13, sigma_t=0.13,
+        Cm=0.01, A_M=140.0, c_pmj=0.0, T=500.0, phi_endo_max=1 / 3, phi_epi_min=2 / 3,
+        ectopic_region=region, dt=0.02, dtype=torch.float32, device=device,
+        stim_protocol=[dict(name="corner", target="ectopic", amp=0.52, dur=2.0, onset=0.0)])
+    return eng
+
+
+def build_training_set(eng, n_macro, per_snapshot, seed=0):
+    """Samples of the plain integrator's trajectories and their 10-substep
+    reaction increments: returns (X raw features, D increments)."""
+    V, St, ct = sample_from_engine(eng, n_macro, per_snapshot, seed=seed)
+    V1, S1 = reaction_targets(V, St, ct)
+    D = output_increments(V.double(), St.double(), V1, S1)
+    return SurrogateNet.features(V, St, ct), D
+
+
+def train_surrogate(X, D, device="cuda", epochs=6, batch=8192, lr=2e-3, lr_decay=0.6,
+                    holdout=0.05, log=None, seed=0):
+    """Fit a SurrogateNet with the HIP training kernels; returns (net, trainer, stats)."""
+    n = X.shape[0]
+    nte = max(batch, int(holdout * n))
+    ntr = n - nte
+    net = SurrogateNet(device, seed=seed)
+    net.fit_normalisation(SurrogateNet.transform(X[:ntr]), D[:ntr])
+    tr = Trainer(net, batch=batch, lr=lr)
+    Xn, Tn = tr.prepare(X, D)
+    t0 = time.perf_counter()
+    hist = tr.fit(Xn[:ntr], Tn[:ntr], epochs=epochs, lr_decay=lr_decay, log=log, seed=seed)
+    train_s = time.perf_counter() - t0
+    # held-out error of dV over the macro-step (mV)
+    preds = []
+    for i in range(ntr, n - batch + 1, batch):
+        preds.append(tr.forward(Xn[i:i + batch].contiguous(), batch).clone())
+    m = len(preds) * batch
+    Yp = torch.cat(preds) * torch.from_numpy(net.out_std).to(X.device) + torch.from_numpy(
+        net.out_mean).to(X.device)
+    err = (Yp[:, 0] - D[ntr:ntr + m, 0]).abs()
+    return net, tr, dict(loss=hist, train_s=train_s, n_train=ntr, heldout_dv_mae=float(err.mean()),
+                         heldout_dv_max=float(err.max()))

@@ -212,6 +212,19 @@ int hp_step_advance_f64(const double* site_params, int32_t n_sites, const uint32
                         int32_t n_groups, int64_t* step_ctr, double dt, double* group_amp,
                         void* stream);
 
+/* End of a time step in one launch: the PMJ exchange (if j is non-NULL and
+ * c_pmj != 0) followed by the step-counter advance and the stimulus
+ * amplitudes of the next step.  Same results as hp_pmj_couple_* followed by
+ * hp_step_advance_*; trees with more than 65536 junctions take that path. */
+int hp_step_tail_f32(const hp_pmj* j, float* V_pk, float* V_myo, float* scratch, double dt,
+                     double vmin, double vmax, const hp_pmj_at* at, const double* site_params,
+                     int32_t n_sites, const uint32_t* group_sites, int32_t n_groups,
+                     int64_t* step_ctr, float* group_amp, void* stream);
+int hp_step_tail_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scratch, double dt,
+                     double vmin, double vmax, const hp_pmj_at* at, const double* site_params,
+                     int32_t n_sites, const uint32_t* group_sites, int32_t n_groups,
+                     int64_t* step_ctr, double* group_amp, void* stream);
+
 /* Gather slot values into reference node order: out[i] = src[slot_of_node[i]]
  * (Vm snapshots, vm_snapshots.npz schema, :1975-1984). */
 int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream);
@@ -348,6 +361,12 @@ int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, f
 int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* state,
                       int64_t ld, float* rate, const hp_mlp_weights* w, const hp_mlp_norm* n,
                       const hp_surrogate_params* sp, int n_blocks, void* stream);
+
+/* The same decision rule without the network (the multirate comparison form):
+ * non-fallback voxels take one t_macro Rush-Larsen/forward-Euler TTP06 step
+ * and rate = -I_ion; fallback voxels get rate = NaN.  V_prev <- V. */
+int hp_bigstep_reaction(const hp_bricks* g, const float* V, float* V_prev, float* state,
+                        int64_t ld, float* rate, const hp_surrogate_params* sp, void* stream);
 
 #ifdef __cplusplus
 }
