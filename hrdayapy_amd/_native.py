@@ -80,6 +80,10 @@ _SIGS = {
                                     c_i64, c_f32, c_f32, c_f32, c_vp]),
     "hp_ttp06_step_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
                                     c_i64, c_f64, c_f64, c_f64, c_vp]),
+    "hp_ttp06_step_host_f32": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64,
+                                         c_i64, c_i64, c_f32, c_f32, c_f32]),
+    "hp_ttp06_step_host_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64,
+                                         c_i64, c_i64, c_f64, c_f64, c_f64]),
     "hp_myo_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
     "hp_myo_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
     "hp_stencil19_apply_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),

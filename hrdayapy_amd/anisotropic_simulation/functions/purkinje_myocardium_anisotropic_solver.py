@@ -70,7 +70,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
     meta = dict(Np=Np, N_myo=lay.n_nodes, comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,
                 pmj_ids=pmj_ids, pmj_patches=patches, sites=sites, layout=lay,
-                constants=pc, setup_s=time.perf_counter() - t0)
+                constants=pc, voxel_size=voxel_size, setup_s=time.perf_counter() - t0)
     if verbose:
         print(f"[hrdaya] Purkinje {Np:,} nodes, myocardium {lay.n_nodes:,} voxels in "
               f"{lay.n_bricks:,} bricks of {lay.brick} (occupancy {lay.occupancy:.2f}), "

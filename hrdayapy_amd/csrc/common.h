@@ -18,32 +18,44 @@ inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 // Arithmetic traits.  The FP32 path uses the hardware transcendental and
 // reciprocal instructions (v_exp_f32 / v_log_f32 / v_rcp_f32 / v_sqrt_f32,
 // ~1 ulp); the FP64 path uses the correctly rounded library routines so that
-// it can be compared with the FP64 oracle at round-off level.
+// it can be compared with the FP64 oracle at round-off level.  The host pass
+// (used by the host entry points that check the cell algebra on a CPU) takes
+// the C library routines in place of the FP32 hardware instructions.
 // ---------------------------------------------------------------------------
+#define HP_HD __host__ __device__ __forceinline__
 template <typename T> struct M;
 
 template <> struct M<float> {
-  static __device__ __forceinline__ float ex(float x) { return __expf(x); }
-  static __device__ __forceinline__ float lg(float x) { return __logf(x); }
-  static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-  static __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-  static __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
-  static __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-  static __device__ __forceinline__ float mx(float a, float b) { return fmaxf(a, b); }
-  static __device__ __forceinline__ float mn(float a, float b) { return fminf(a, b); }
-  static __device__ __forceinline__ float ab(float a) { return fabsf(a); }
+#if defined(__HIP_DEVICE_COMPILE__)
+  static HP_HD float ex(float x) { return __expf(x); }
+  static HP_HD float lg(float x) { return __logf(x); }
+  static HP_HD float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  static HP_HD float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static HP_HD float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static HP_HD float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+#else
+  static HP_HD float ex(float x) { return expf(x); }
+  static HP_HD float lg(float x) { return logf(x); }
+  static HP_HD float rcp(float x) { return 1.0f / x; }
+  static HP_HD float dv(float a, float b) { return a / b; }
+  static HP_HD float sq(float x) { return sqrtf(x); }
+  static HP_HD float rsq(float x) { return 1.0f / sqrtf(x); }
+#endif
+  static HP_HD float mx(float a, float b) { return fmaxf(a, b); }
+  static HP_HD float mn(float a, float b) { return fminf(a, b); }
+  static HP_HD float ab(float a) { return fabsf(a); }
 };
 
 template <> struct M<double> {
-  static __device__ __forceinline__ double ex(double x) { return exp(x); }
-  static __device__ __forceinline__ double lg(double x) { return log(x); }
-  static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
-  static __device__ __forceinline__ double dv(double a, double b) { return a / b; }
-  static __device__ __forceinline__ double sq(double x) { return sqrt(x); }
-  static __device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
-  static __device__ __forceinline__ double mx(double a, double b) { return fmax(a, b); }
-  static __device__ __forceinline__ double mn(double a, double b) { return fmin(a, b); }
-  static __device__ __forceinline__ double ab(double a) { return fabs(a); }
+  static HP_HD double ex(double x) { return exp(x); }
+  static HP_HD double lg(double x) { return log(x); }
+  static HP_HD double rcp(double x) { return 1.0 / x; }
+  static HP_HD double dv(double a, double b) { return a / b; }
+  static HP_HD double sq(double x) { return sqrt(x); }
+  static HP_HD double rsq(double x) { return 1.0 / sqrt(x); }
+  static HP_HD double mx(double a, double b) { return fmax(a, b); }
+  static HP_HD double mn(double a, double b) { return fmin(a, b); }
+  static HP_HD double ab(double a) { return fabs(a); }
 };
 
 }  // namespace hp

@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
     for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
   }
   if (p.rate_ext && p.do_reaction == 0) dvdt += ((const T*)p.rate_ext)[slot];
-  const int grp = (lab >> 2) & 31;
+  int grp = (lab >> 2) & 31;
+  if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
   Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
@@ -339,25 +340,35 @@ __device__ __forceinline__ bool site_active(const double* sp, double t) {
   return (t - onset) - k * period < dur;
 }
 
+// Amplitude of stimulus group g at time t: sum over the sites named by the
+// group's bitmask, relative to its site base (gs[2g] = base, gs[2g+1] = mask).
+__device__ __forceinline__ double group_amplitude(int g, const double* __restrict__ sp,
+                                                  const uint32_t* __restrict__ gs, double t) {
+  const uint32_t base = gs[2 * g], mask = gs[2 * g + 1];
+  double a = 0.0;
+  for (int b = 0; b < 32; ++b)
+    if ((mask >> b) & 1u) {
+      const double* q = sp + 5 * (int64_t)(base + b);
+      if (site_active(q, t)) a += q[4];
+    }
+  return a;
+}
+
+constexpr int STIM_THREADS = 1024;
+constexpr int MAX_GROUPS = 8192;
+
 template <typename T>
-__global__ void stim_kernel(const double* __restrict__ sp, int n_sites,
-                            const uint32_t* __restrict__ gs, int n_groups, int64_t* step_ctr,
-                            double dt, T* __restrict__ amp, int advance) {
-  const int g = threadIdx.x;  // one wave, 32 groups
+__global__ void __launch_bounds__(STIM_THREADS) stim_kernel(const double* __restrict__ sp,
+                                                            const uint32_t* __restrict__ gs,
+                                                            int n_groups, int64_t* step_ctr,
+                                                            double dt, T* __restrict__ amp,
+                                                            int advance) {
   const int64_t step = *step_ctr + advance;
   __syncthreads();
-  if (g < 32) {
-    double a = 0.0;
-    if (g > 0 && g < n_groups) {
-      const double t = (double)step * dt;
-      const uint32_t mask = gs[g];
-      for (int s = 0; s < n_sites; ++s)
-        if ((mask >> s) & 1u)
-          if (site_active(sp + 5 * s, t)) a += sp[5 * s + 4];
-    }
-    amp[g] = (T)a;
-  }
-  if (advance && g == 0) *step_ctr = step;
+  const double t = (double)step * dt;
+  for (int g = threadIdx.x; g < n_groups; g += STIM_THREADS)
+    amp[g] = g == 0 ? T(0) : (T)group_amplitude(g, sp, gs, t);
+  if (advance && threadIdx.x == 0) *step_ctr = step;
 }
 
 // One workgroup finishes the step: PMJ junctions, PMJ targets, then the
@@ -369,7 +380,7 @@ constexpr int64_t TAIL_MAX_JUNCTIONS = 65536;
 template <typename T>
 __global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
     hp_pmj j, int has_pmj, T* __restrict__ V_pk, T* __restrict__ V_myo, T* __restrict__ scratch,
-    double dt, double vmin, double vmax, hp_pmj_at at, const double* __restrict__ sp, int n_sites,
+    double dt, double vmin, double vmax, hp_pmj_at at, const double* __restrict__ sp,
     const uint32_t* __restrict__ gs, int n_groups, int64_t* step_ctr, T* __restrict__ amp) {
   const int tid = threadIdx.x;
   if (has_pmj) {
@@ -382,17 +393,9 @@ __global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
   }
   const int64_t step = *step_ctr + 1;
   __syncthreads();
-  if (tid < 32) {
-    double a = 0.0;
-    if (tid > 0 && tid < n_groups) {
-      const double t = (double)step * dt;
-      const uint32_t mask = gs[tid];
-      for (int s = 0; s < n_sites; ++s)
-        if ((mask >> s) & 1u)
-          if (site_active(sp + 5 * s, t)) a += sp[5 * s + 4];
-    }
-    amp[tid] = (T)a;
-  }
+  const double t = (double)step * dt;
+  for (int g = tid; g < n_groups; g += TAIL_THREADS)
+    amp[g] = g == 0 ? T(0) : (T)group_amplitude(g, sp, gs, t);
   if (tid == 0) *step_ctr = step;
 }
 
@@ -449,11 +452,11 @@ static int pmj_couple(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt,
 template <typename T>
 static int stim(const double* sp, int n_sites, const uint32_t* gs, int n_groups,
                 const int64_t* ctr, double dt, T* amp, int advance, void* stream) {
-  if (n_sites < 0 || n_sites > 32 || n_groups < 0 || n_groups > 32 || !ctr || !amp) {
-    set_error("hp_stim: at most 32 sites and 32 groups (got %d, %d)", n_sites, n_groups);
+  if (n_sites < 0 || n_groups < 1 || n_groups > MAX_GROUPS || !ctr || !amp || !gs) {
+    set_error("hp_stim: 1..%d groups required (got %d)", MAX_GROUPS, n_groups);
     return -1;
   }
-  hipLaunchKernelGGL(stim_kernel<T>, dim3(1), dim3(64), 0, (hipStream_t)stream, sp, n_sites, gs,
+  hipLaunchKernelGGL(stim_kernel<T>, dim3(1), dim3(STIM_THREADS), 0, (hipStream_t)stream, sp, gs,
                      n_groups, (int64_t*)ctr, dt, amp, advance);
   return check_launch("hp_stim");
 }
@@ -462,8 +465,8 @@ template <typename T>
 static int step_tail(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, double vmin,
                      double vmax, const hp_pmj_at* at, const double* sp, int n_sites,
                      const uint32_t* gs, int n_groups, int64_t* ctr, T* amp, void* stream) {
-  if (n_sites < 0 || n_sites > 32 || n_groups < 0 || n_groups > 32 || !ctr || !amp) {
-    set_error("hp_step_tail: at most 32 sites and 32 groups");
+  if (n_sites < 0 || n_groups < 1 || n_groups > MAX_GROUPS || !ctr || !amp || !gs) {
+    set_error("hp_step_tail: 1..%d groups required (got %d)", MAX_GROUPS, n_groups);
     return -1;
   }
   const bool has = j && j->n_junctions > 0 && j->c_pmj != 0.0;
@@ -474,7 +477,7 @@ static int step_tail(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, 
   }
   hipLaunchKernelGGL(step_tail_kernel<T>, dim3(1), dim3(TAIL_THREADS), 0, (hipStream_t)stream,
                      has ? *j : hp_pmj{}, has ? 1 : 0, V_pk, V_myo, scratch, dt, vmin, vmax,
-                     at ? *at : hp_pmj_at{}, sp, n_sites, gs, n_groups, ctr, amp);
+                     at ? *at : hp_pmj_at{}, sp, gs, n_groups, ctr, amp);
   return check_launch("hp_step_tail");
 }
 

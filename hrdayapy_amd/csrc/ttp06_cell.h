@@ -43,8 +43,12 @@ enum StateCol {
 __device__ __constant__ static const double kGKs[3] = {0.392, 0.098, 0.392};
 __device__ __constant__ static const double kGto[3] = {0.073, 0.294, 0.294};
 
+// Host copies of the two tables, for the host entry points.
+static const double kGKsHost[3] = {0.392, 0.098, 0.392};
+static const double kGtoHost[3] = {0.073, 0.294, 0.294};
+
 template <typename T, bool kRef>
-__device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
+HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
                                          bool endo_s, T ki_base, T nai_base) {
   typedef M<T> m_;
 #define C(x) T(x)
@@ -74,7 +78,7 @@ __device__ __forceinline__ T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_
   } else {
     E_Na = RTF * (C(4.941642422609304) - m_::lg(Nai));          // log(140)
     E_K = RTF * (C(1.6863989535702288) - m_::lg(Ki));           // log(5.4)
-    E_Ks = RTF * (C(2.2082744135228043) - m_::lg(Ki + C(0.03) * Nai));  // log(5.4+0.03*140)
+    E_Ks = RTF * (C(2.2617630984737906) - m_::lg(Ki + C(0.03) * Nai));  // log(5.4+0.03*140)
     E_Ca = C(0.5) * RTF * (C(0.6931471805599453) - m_::lg(Cai));  // log(2)
   }
 

@@ -24,10 +24,11 @@ KI_BASE, NAI_BASE = INIT["Ki"], INIT["Nai"]
 def stim_groups(site_node_ids, n_nodes):
     """Assign each node the set of sites it belongs to, encoded as a group id.
 
-    Returns (group (n_nodes,) int, group_sites (32,) uint32 bitmasks).
+    Returns (group (n_nodes,) int, group_sites (n_groups, 2) uint32 pairs of
+    (site base, site bitmask)); at most 32 sites and 31 site combinations.
     """
     if len(site_node_ids) > 32:
-        raise ValueError("at most 32 stimulus sites")
+        raise ValueError("at most 32 stimulus sites (use ensemble instances for more)")
     mask = np.zeros(n_nodes, dtype=np.uint64)
     for s, ids in enumerate(site_node_ids):
         ids = np.asarray(ids, dtype=np.int64)
@@ -39,9 +40,31 @@ def stim_groups(site_node_ids, n_nodes):
     gid[nz] = np.arange(1, int(nz.sum()) + 1)
     if nz.sum() > 31:
         raise ValueError("more than 31 distinct stimulus-site combinations")
-    group_sites = np.zeros(32, dtype=np.uint32)
-    group_sites[gid[nz]] = uniq[nz].astype(np.uint32)
+    group_sites = np.zeros((32, 2), dtype=np.uint32)
+    group_sites[gid[nz], 1] = uniq[nz].astype(np.uint32)
     return gid[inv], group_sites
+
+
+def ensemble_stim_groups(inst_of_node, site_inst, site_node_ids, n_inst):
+    """Per-instance stimulus groups: each instance has at most 3 sites; the group
+    of a node is (inst << 3) | local combination (1..7).  Returns (local group
+    per node, group_sites (8*n_inst, 2))."""
+    n_nodes = len(inst_of_node)
+    local = np.zeros(n_nodes, dtype=np.int64)
+    group_sites = np.zeros((8 * n_inst, 2), dtype=np.uint32)
+    order = np.argsort(site_inst, kind="stable")
+    base = np.zeros(n_inst + 1, dtype=np.int64)
+    np.add.at(base, np.asarray(site_inst)[order] + 1, 1)
+    base = np.cumsum(base)
+    for i in range(n_inst):
+        sites = order[base[i]:base[i + 1]]
+        if len(sites) > 3:
+            raise ValueError("at most 3 stimulus sites per ensemble instance")
+        for b, s in enumerate(sites):
+            local[np.asarray(site_node_ids[s], dtype=np.int64)] |= 1 << b
+        for c in range(1, 8):
+            group_sites[8 * i + c] = (base[i], c)
+    return local, group_sites, order
 
 
 class MonodomainEngine:
@@ -72,17 +95,26 @@ class MonodomainEngine:
 
         # stimulus groups over the global node set
         sites = stim_sites or []
-        gid, group_sites = stim_groups([s["node_ids"] for s in sites], self.Np + self.N_myo)
+        if inst is not None:
+            if self.Np:
+                raise ValueError("ensemble instances run without a Purkinje cable")
+            n_inst = int(np.asarray(inst).max()) + 1
+            gid, group_sites, order = ensemble_stim_groups(
+                np.asarray(inst), [s["inst"] for s in sites], [s["node_ids"] for s in sites],
+                n_inst)
+            sites = [sites[k] for k in order]
+        else:
+            gid, group_sites = stim_groups([s["node_ids"] for s in sites], self.Np + self.N_myo)
         sp = np.zeros((max(1, len(sites)), 5))
         for k, s in enumerate(sites):
             per = s.get("period") or 0.0
             npl = s.get("n_pulses")
             sp[k] = (s.get("onset", 0.0), s["dur"], per, -1.0 if npl is None else npl, s["amp"])
         self.n_sites = len(sites)
-        self.n_groups = int(gid.max(initial=0)) + 1
+        self.n_groups = len(group_sites)
         self.site_params = torch.tensor(sp.ravel(), dtype=torch.float64, device=dev)
-        self.group_sites = torch.tensor(group_sites.view(np.int32), device=dev)
-        self.group_amp = torch.zeros(32, dtype=dt_, device=dev)
+        self.group_sites = torch.from_numpy(group_sites.ravel().view(np.int32).copy()).to(dev)
+        self.group_amp = torch.zeros(self.n_groups, dtype=dt_, device=dev)
         self.step_ctr = torch.zeros(1, dtype=torch.int64, device=dev)
 
         # myocardium

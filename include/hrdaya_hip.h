@@ -64,6 +64,18 @@ int hp_ttp06_step_f64(const double* V, double* state, double* I_ion, const uint8
                       int fixed_celltype, const double* gks_scale, const double* gto_scale,
                       int64_t n, int64_t stride_var, int64_t stride_node, double dt,
                       double ki_base, double nai_base, void* stream);
+/* The same step on HOST memory, computed on the CPU with the kernels' own
+ * arithmetic form (C library exp/log/division standing in for the FP32
+ * hardware instructions).  For checking that form without a GPU; same
+ * arguments as above, no stream. */
+int hp_ttp06_step_host_f32(const float* V, float* state, float* I_ion, const uint8_t* celltype,
+                           int fixed_celltype, const float* gks_scale, const float* gto_scale,
+                           int64_t n, int64_t stride_var, int64_t stride_node, float dt,
+                           float ki_base, float nai_base);
+int hp_ttp06_step_host_f64(const double* V, double* state, double* I_ion,
+                           const uint8_t* celltype, int fixed_celltype, const double* gks_scale,
+                           const double* gto_scale, int64_t n, int64_t stride_var,
+                           int64_t stride_node, double dt, double ki_base, double nai_base);
 
 /* ---------------------------------------------------------------------------
  * Myocardium on a brick layout.
@@ -107,9 +119,10 @@ typedef struct hp_mono_params {
   int32_t do_reaction;   /* 1: TTP06 step; 0: diffusion only; 2: TTP06 only where
                             rate_ext is NaN, else dV/dt = rate_ext (surrogate form) */
   int32_t n_inst;        /* number of rows of inst_params */
-  const void* inst_params;  /* T[n_inst][4]: g_Ks scale, g_to scale, sigma scale, spare */
+  const void* inst_params;  /* T[n_inst][4]: g_Ks scale, g_to scale, 2 spare (conductivity
+                               scaling of an instance is folded into its D6 at set-up) */
   const int64_t* step_ctr;  /* device step counter; time label t = step*dt */
-  const void* group_amp;    /* T[32] stimulus amplitude of each group this step [uA/mm^2] */
+  const void* group_amp;    /* T[n_groups] stimulus amplitude of each group this step [uA/mm^2] */
   void* at;                 /* T[n] first activation time (NaN until crossed) or NULL */
   const void* rate_ext;     /* T[n] extra dV/dt [mV/ms] (surrogate reaction) or NULL */
 } hp_mono_params;
@@ -194,8 +207,12 @@ int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scra
 /* ---------------------------------------------------------------------------
  * Stimulus schedule on the device (replaces _stim_active, :1290-1301, and the
  * host loop :1754-1759).  site_params[s*5 + 0..4] = onset, dur, period (0 =
- * single pulse), n_pulses (< 0 = unlimited), amp [uA/mm^2].  group_sites[g]
- * is the bitmask of sites of stimulus group g (group 0 = none).
+ * single pulse), n_pulses (< 0 = unlimited), amp [uA/mm^2].  Stimulus group
+ * g (1 <= g < n_groups <= 8192; group 0 = none) is the set of sites
+ * group_sites[2g] + b for every bit b of the mask group_sites[2g+1];
+ * group_amp[g] receives the summed amplitude of its active sites.  A voxel's
+ * group is its label bits 2-6, or (inst << 3) | (bits 2-4) when the brick grid
+ * carries instance ids (ensembles: up to 7 site combinations per instance).
  * hp_step_advance increments *step_ctr and writes group_amp for the new time
  * step*dt; call hp_stim_eval once after setting the counter.
  * ------------------------------------------------------------------------- */
