@@ -101,7 +101,11 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
     wall = time.perf_counter() - t0
     steps = n_macro * SG.N_SUB
     out = dict(value=meta["N_myo"] * steps / wall, ms_per_step=wall / steps * 1e3,
-               fallback_fraction=round(run.fallback_fraction(), 4))
+               fallback_fraction=round(run.fallback_fraction(), 4),
+               fallback_brick_fraction=round(run.fallback_brick_fraction(), 4))
+    # per-kernel breakdown (HIP events on the launching stream)
+    out["reaction_us"] = _event_us(run.launch_reaction, 20)
+    out["substep_us"] = _event_us(run.launch_substep, 20)
     if ref_at is not None:
         at = eng.at_nodes().double().cpu().numpy()
         both = np.isfinite(at) & np.isfinite(ref_at)
@@ -109,6 +113,35 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
         out["at_err_vs_plain_mean_ms"] = float(np.mean(np.abs(at[both] - ref_at[both]))) if both.any() else None
         out["activated_mismatch"] = int(np.isfinite(at).sum() - np.isfinite(ref_at).sum())
     return out
+
+
+def _event_us(fn, reps):
+    """Average device time of fn() over reps calls, HIP events on the current stream."""
+    import torch
+    st = torch.cuda.current_stream()
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    e1.synchronize()
+    return round(e0.elapsed_time(e1) / reps * 1e3, 2)
+
+
+def stream_copy_bandwidth(mib, reps=20):
+    """Achieved read+write bandwidth of the plain streaming-copy kernel."""
+    import torch
+    from hrdayapy_amd import _native as nat
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    s = torch.cuda.current_stream()
+    us = _event_us(lambda: nat.call("hp_stream_copy_f32", nat.ptr(x), nat.ptr(y), n,
+                                    s.cuda_stream), reps)
+    del x, y
+    torch.cuda.empty_cache()
+    return dict(mib=mib, us=us, GBs=2 * n * 4 / (us * 1e-6) / 1e9)
 
 
 def time_myo_kernel(eng, reps=50):
@@ -168,6 +201,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--calib-copy-mb", type=int, default=1024,
+                    help="size of the streaming-copy calibration (0 = off)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -212,6 +247,10 @@ def main():
     achieved = BYTES_PER_VOXEL_STEP * n_myo / k_s / 1e9
     print(f"[bench] fused myo kernel {k_s * 1e6:.1f} us/launch -> {achieved:.0f} GB/s algorithmic; "
           f"activated {activated}/{n_myo} at t={(a.warmup + a.steps) * 0.02:.1f} ms", flush=True)
+    copy = stream_copy_bandwidth(a.calib_copy_mb) if a.calib_copy_mb else None
+    if copy:
+        print(f"[bench] streaming copy {a.calib_copy_mb} MiB: {copy['us']:.1f} us -> "
+              f"{copy['GBs']:.0f} GB/s (read+write)", flush=True)
     results = {"plain": dict(value=vs, ms_per_step=wall / a.steps * 1e3)}
     del eng
     torch.cuda.empty_cache()
@@ -258,7 +297,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": results[head]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "synthetic BiV shell 256^3 @0.4 mm (config 2), FP32, dt 0.02 ms, "
+        "config": {"workload": "synthetic BiV shell 256^3 @0.4 mm, FP32, dt 0.02 ms, "
                                "Purkinje + PMJ, no ECG",
                    "form": head, "n_myo_voxels": n_myo, "n_purkinje_nodes": meta["Np"],
                    "n_pmj": len(meta["pmj_ids"]), "brick": list(brick), "brick_autotune_us": tune,
@@ -269,6 +308,7 @@ def main():
                      "kernel": "myo_step_kernel<float> (plain form)", "kernel_us": k_s * 1e6,
                      "bytes_per_voxel": BYTES_PER_VOXEL_STEP},
         "cpu_baseline": cpu,
+        "stream_copy": copy,
     }
     print(json.dumps(out))
 

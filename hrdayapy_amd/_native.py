@@ -59,7 +59,8 @@ class HpMlpNorm(C.Structure):
 
 class HpSurrogateParams(C.Structure):
     _fields_ = [("t_macro", c_f32), ("dvdt_max", c_f32), ("v_lo", c_f32), ("v_hi", c_f32),
-                ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("pad_", c_i32)]
+                ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("pad_", c_i32),
+                ("fb_list", c_vp), ("fb_count", c_vp)]
 
 
 class HpTorso(C.Structure):
@@ -86,6 +87,8 @@ _SIGS = {
                                          c_i64, c_i64, c_f64, c_f64, c_f64]),
     "hp_myo_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
     "hp_myo_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_fallback_react_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
+                                        P(HpMonoParams), c_vp]),
     "hp_stencil19_apply_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
     "hp_stencil19_apply_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
     "hp_cable_step_f32": (C.c_int, [P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
@@ -129,6 +132,7 @@ _SIGS = {
                                       P(HpSurrogateParams), c_vp]),
     "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "hp_stream_copy_f32": (C.c_int, [c_vp, c_vp, c_i64, c_vp]),
 }
 
 

@@ -221,11 +221,28 @@ __device__ __forceinline__ void silu_pack(const floatx4& c, const float* bias, i
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const float z = c[q] + bias[m * 16 + grp * 4 + q];
-    dst[(m & 1) * 4 + q] = f2bf(z / (1.f + __expf(-z)));
+    // z * sigmoid(z) with the hardware exp2 and reciprocal (~1 ulp each, far
+    // below the bf16 rounding of the result); exp2 -> inf gives 0 for z << 0
+    dst[(m & 1) * 4 + q] =
+        f2bf(z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f)));
   }
 }
 
 constexpr int SUR_THREADS = 512, SUR_WAVES = SUR_THREADS / 64;
+
+// Append the slots of the lanes with fb set to the fallback list: one atomic
+// per wave.  Called by every lane of the wave.
+__device__ __forceinline__ void append_fallback(bool fb, int64_t slot, int32_t* __restrict__ list,
+                                                int32_t* __restrict__ count) {
+  const uint64_t m = __ballot(fb);
+  if (m == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __shfl(base, leader);
+  if (fb) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)slot;
+}
 
 __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
                                                         float* __restrict__ V_prev,
@@ -348,11 +365,13 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
     for (int t = 0; t < 2; ++t) {
       const int64_t slot = t ? slot1 : slot0;
       const uint8_t lab = t ? lab1 : lab0;
-      if (lab == 0xFF) continue;
       const float v = V[slot], vp = V_prev[slot];
       const float dvdt = (v - vp) / dtm;
       const bool fallback = sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
                             (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
+      if (sp.fb_list) append_fallback(lab != 0xFF && grp == 0 && fallback, slot, sp.fb_list,
+                                      sp.fb_count);
+      if (lab == 0xFF) continue;
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
@@ -389,13 +408,13 @@ __global__ void __launch_bounds__(256) bigstep_kernel(hp_bricks g, const float* 
                                                       hp_surrogate_params sp) {
   const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
-  if (slot >= n_slots) return;
-  const uint8_t lab = g.label[slot];
-  if (lab == 0xFF) return;
-  const float v = V[slot], vp = V_prev[slot];
+  const uint8_t lab = slot < n_slots ? g.label[slot] : (uint8_t)0xFF;
+  const float v = lab != 0xFF ? V[slot] : 0.f, vp = lab != 0xFF ? V_prev[slot] : 0.f;
   const float dtm = sp.t_macro;
   const bool fallback = sp.force_fallback || fabsf((v - vp) / dtm) > sp.dvdt_max ||
                         (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
+  if (sp.fb_list) append_fallback(lab != 0xFF && fallback, slot, sp.fb_list, sp.fb_count);
+  if (lab == 0xFF) return;
   V_prev[slot] = v;
   if (fallback) {
     rate[slot] = __builtin_nanf("");

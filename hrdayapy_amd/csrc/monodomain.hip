@@ -42,40 +42,69 @@ struct Tile {
   int32_t nbr[27];
 };
 
-// Stage the halo tile of brick `b` into LDS.
+// Stage the halo tile of brick `b` into LDS.  Each lane handles NIT tile
+// cells; all their global loads (label, V and the six D components, read
+// whether or not the slot is tissue -- padding slots hold V0 and D = 0) are
+// issued back to back before any LDS write, so a workgroup waits for one
+// memory round trip instead of one per cell.
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ void stage_tile(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
                                            const T* __restrict__ V, int64_t b, int64_t ld,
                                            bool need_D) {
   typedef Tile<T, BX, BY, BZ> TT;
+  constexpr int NIT = (TT::N + TT::BVOL - 1) / TT::BVOL;
   const int tid = threadIdx.x;
   if (tid < 27) t.nbr[tid] = g.nbr[b * 27 + tid];
   __syncthreads();
   const T* __restrict__ D6 = (const T*)g.D6;
-  for (int i = tid; i < TT::N; i += TT::BVOL) {
-    const int tz = i % TT::TZ, ty = (i / TT::TZ) % TT::TY, tx = i / TT::SX;
-    const int gx = tx - 1, gy = ty - 1, gz = tz - 1;
-    const int dbx = gx < 0 ? -1 : (gx >= BX ? 1 : 0);
-    const int dby = gy < 0 ? -1 : (gy >= BY ? 1 : 0);
-    const int dbz = gz < 0 ? -1 : (gz >= BZ ? 1 : 0);
-    const int nout = (dbx != 0) + (dby != 0) + (dbz != 0);
-    uint8_t lab = 0xFF;
-    if (nout < 3) {  // the 19-point stencil never reads tile corners
-      const int nb = t.nbr[(dbx + 1) * 9 + (dby + 1) * 3 + (dbz + 1)];
-      if (nb >= 0) {
-        const int lx = gx - dbx * BX, ly = gy - dby * BY, lz = gz - dbz * BZ;
-        const int64_t slot = (int64_t)nb * TT::BVOL + (lx * BY + ly) * BZ + lz;
-        lab = g.label[slot];
-        if (lab != 0xFF) {
-          t.V[i] = V[slot];
-          if (need_D) {
+  int64_t sl[NIT];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) t.D[k][i] = D6[k * ld + slot];
-          }
+  for (int it = 0; it < NIT; ++it) {
+    const int i = tid + it * TT::BVOL;
+    sl[it] = -1;
+    if (i < TT::N) {
+      const int tz = i % TT::TZ, ty = (i / TT::TZ) % TT::TY, tx = i / TT::SX;
+      const int gx = tx - 1, gy = ty - 1, gz = tz - 1;
+      const int dbx = gx < 0 ? -1 : (gx >= BX ? 1 : 0);
+      const int dby = gy < 0 ? -1 : (gy >= BY ? 1 : 0);
+      const int dbz = gz < 0 ? -1 : (gz >= BZ ? 1 : 0);
+      const int nout = (dbx != 0) + (dby != 0) + (dbz != 0);
+      if (nout < 3) {  // the 19-point stencil never reads tile corners
+        const int nb = t.nbr[(dbx + 1) * 9 + (dby + 1) * 3 + (dbz + 1)];
+        if (nb >= 0) {
+          const int lx = gx - dbx * BX, ly = gy - dby * BY, lz = gz - dbz * BZ;
+          sl[it] = (int64_t)nb * TT::BVOL + (lx * BY + ly) * BZ + lz;
         }
       }
     }
-    t.L[i] = lab;
+  }
+  uint8_t lab[NIT];
+  T v[NIT], d[NIT][6];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    lab[it] = 0xFF;
+    if (sl[it] >= 0) {
+      lab[it] = g.label[sl[it]];
+      v[it] = V[sl[it]];
+      if (need_D) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) d[it][k] = D6[k * ld + sl[it]];
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int i = tid + it * TT::BVOL;
+    if (i < TT::N) {
+      t.L[i] = lab[it];
+      if (lab[it] != 0xFF) {
+        t.V[i] = v[it];
+        if (need_D) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) t.D[k][i] = d[it][k];
+        }
+      }
+    }
   }
   __syncthreads();
 }
@@ -124,31 +153,43 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
   return acc;
 }
 
-template <typename T, int BX, int BY, int BZ>
+// kReact = true: do_reaction 1 (TTP06 every voxel) and 2 (TTP06 where rate_ext
+// is NaN); kReact = false: do_reaction 0 (no TTP06; dV/dt = rate_ext if given),
+// a lighter specialisation for the multirate substeps.
+template <typename T, int BX, int BY, int BZ, bool kReact>
 __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
     hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
     int64_t ld, hp_mono_params p) {
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
   const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  stage_tile<T, BX, BY, BZ>(t, g, V_in, b, ld, true);
   const int tid = threadIdx.x;
+  const int64_t slot = b * TT::BVOL + tid;
+  // every-voxel reaction: issue the 18 state loads first, so that they are in
+  // flight together with the halo staging (slots of padding voxels are valid
+  // memory; their values are not used)
+  const bool pre = kReact && p.do_reaction == 1;
+  T s[N_STATE];
+  if (pre) {
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+  }
+  stage_tile<T, BX, BY, BZ>(t, g, V_in, b, ld, true);
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
   const uint8_t lab = t.L[c];
   if (lab == 0xFF) return;
-  const int64_t slot = b * TT::BVOL + tid;
   const T Vc = t.V[c];
   const T lv = T(p.dscale) * stencil19<T, BX, BY, BZ>(t, c);
   const T dt = T(p.dt);
   T dvdt = T(0);  // reaction part of dV/dt
-  bool full = p.do_reaction == 1;
-  if (p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
+  bool full = kReact && p.do_reaction == 1;
+  if (kReact && p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
     const T r = ((const T*)p.rate_ext)[slot];
     full = r != r;
     if (!full) dvdt = r;
   }
-  if (full) {
+  if (kReact && full) {
     const int ct = lab & 3;
     T gks = T(kGKs[ct]), gto = T(kGto[ct]);
     if (p.inst_params && g.inst) {
@@ -156,15 +197,16 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
       gks *= ip[0];
       gto *= ip[1];
     }
-    T s[N_STATE];
+    if (!pre) {
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+      for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+    }
     constexpr bool exact = sizeof(T) == 8;
     dvdt = -ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
   }
-  if (p.rate_ext && p.do_reaction == 0) dvdt += ((const T*)p.rate_ext)[slot];
+  if (!kReact && p.rate_ext) dvdt = ((const T*)p.rate_ext)[slot];
   int grp = (lab >> 2) & 31;
   if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
@@ -224,7 +266,11 @@ static int launch_bricks(const hp_bricks* g, const char* what, hipStream_t st, A
 
 template <typename T, int BX, int BY, int BZ>
 struct MyoK {
-  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ>;
+  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ, true>;
+};
+template <typename T, int BX, int BY, int BZ>
+struct MyoDiffK {
+  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ, false>;
 };
 template <typename T, int BX, int BY, int BZ>
 struct StencilK {
@@ -413,8 +459,45 @@ static int myo_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64
     set_error("hp_myo_step: params, step_ctr and group_amp are required");
     return -1;
   }
+  if (p->do_reaction < 0 || p->do_reaction > 2 || (p->do_reaction == 2 && !p->rate_ext)) {
+    set_error("hp_myo_step: do_reaction must be 0, 1 or 2 (2 needs rate_ext)");
+    return -1;
+  }
+  if (p->do_reaction == 0)
+    return launch_bricks<T, MyoDiffK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state,
+                                      ld, *p);
   return launch_bricks<T, MyoK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state, ld,
                                 *p);
+}
+
+// ---------------------------------------------------------------------------
+// Multirate substep: the full TTP06 step for the compacted list of fallback
+// voxels, writing rate[slot] = -I_ion(V^n) for the diffusion kernel
+// (do_reaction = 0) that follows.  Grid-stride over *count entries.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) fallback_react_kernel(
+    hp_bricks g, const float* __restrict__ V, float* __restrict__ state, int64_t ld,
+    float* __restrict__ rate, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
+    hp_mono_params p) {
+  const int n = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int64_t slot = list[i];
+    const int ct = g.label[slot] & 3;
+    float gks = (float)kGKs[ct], gto = (float)kGto[ct];
+    if (p.inst_params && g.inst) {
+      const float* ip = (const float*)p.inst_params + 4 * (int)g.inst[slot];
+      gks *= ip[0];
+      gto *= ip[1];
+    }
+    float s[N_STATE];
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+    const float I = ttp06_cell<float, false>(V[slot], s, (float)p.dt, gks, gto, ct != 2,
+                                             (float)p.ki_base, (float)p.nai_base);
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
+    rate[slot] = -I;
+  }
 }
 
 template <typename T>
@@ -481,6 +564,15 @@ static int step_tail(const hp_pmj* j, T* V_pk, T* V_myo, T* scratch, double dt, 
   return check_launch("hp_step_tail");
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the reference point for
+// the HBM bandwidth and traffic counters of the other kernels.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restrict__ src,
+                                                          float4* __restrict__ dst, int64_t n4) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 template <typename T>
 static int gather(const T* src, const int32_t* idx, T* out, int64_t n, void* stream) {
   if (n <= 0) return 0;
@@ -501,6 +593,19 @@ int hp_myo_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* 
 int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
                     int64_t ld, const hp_mono_params* p, void* stream) {
   return myo_step<double>(g, V_in, V_out, state, ld, p, stream);
+}
+int hp_fallback_react_f32(const hp_bricks* g, const float* V, float* state, int64_t ld,
+                          float* rate, const int32_t* list, const int32_t* count,
+                          int64_t max_count, const hp_mono_params* p, void* stream) {
+  if (!g || !p || !list || !count || !rate || max_count < 0) {
+    set_error("hp_fallback_react: bad arguments");
+    return -1;
+  }
+  if (max_count == 0) return 0;
+  const int blocks = (int)(max_count < 1024 * 256 ? ceil_div(max_count, 256) : 1024);
+  hipLaunchKernelGGL(fallback_react_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, *g,
+                     V, state, ld, rate, list, count, *p);
+  return check_launch("hp_fallback_react");
 }
 int hp_stencil19_apply_f32(const hp_bricks* g, const float* V, float* out, double dscale,
                            void* stream) {
@@ -562,6 +667,18 @@ int hp_step_tail_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scrat
 }
 int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream) {
   return gather<float>(src, idx, out, n, stream);
+}
+int hp_stream_copy_f32(const float* src, float* dst, int64_t n, void* stream) {
+  if (n < 0 || (n & 3) || (((uintptr_t)src | (uintptr_t)dst) & 15)) {
+    set_error("hp_stream_copy_f32: n must be a multiple of 4 and pointers 16-byte aligned");
+    return -1;
+  }
+  if (n == 0) return 0;
+  const int64_t n4 = n / 4;
+  const int blocks = (int)(n4 / 256 < 8192 ? ceil_div(n4, 256) : 8192);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const float4*)src, (float4*)dst, n4);
+  return check_launch("hp_stream_copy_f32");
 }
 int hp_gather_f64(const double* src, const int32_t* idx, double* out, int64_t n, void* stream) {
   return gather<double>(src, idx, out, n, stream);

@@ -63,6 +63,43 @@ class BrickLayout:
                     k += 1
         self.nbr = nbr
 
+    @classmethod
+    def concat(cls, layouts):
+        """Layouts of independent instances (same brick shape) back to back: bricks,
+        slots and nodes of instance i follow those of instance i-1; neighbour
+        entries keep pointing inside their own instance.  ``node_offset``,
+        ``slot_offset`` and ``brick_offset`` give each instance's range."""
+        layouts = list(layouts)
+        if not layouts:
+            raise ValueError("no layouts to concatenate")
+        brick = layouts[0].brick
+        if any(l.brick != brick for l in layouts):
+            raise ValueError("all layouts of a concatenation need the same brick shape")
+        out = cls.__new__(cls)
+        out.shape = None
+        out.brick = brick
+        out.bvol = layouts[0].bvol
+        out.grid = None
+        nb = np.cumsum([0] + [l.n_bricks for l in layouts])
+        nn = np.cumsum([0] + [l.n_nodes for l in layouts])
+        out.brick_offset = nb
+        out.slot_offset = nb * out.bvol
+        out.node_offset = nn
+        out.n_bricks = int(nb[-1])
+        out.n_nodes = int(nn[-1])
+        out.n_slots = out.n_bricks * out.bvol
+        if out.n_slots > np.iinfo(np.int32).max:
+            raise ValueError("more than 2^31 slots")
+        out.vox_idx = np.concatenate([l.vox_idx for l in layouts])
+        out.brick_coords = np.concatenate([l.brick_coords for l in layouts])
+        out.slot_of_node = np.concatenate(
+            [l.slot_of_node + np.int32(o) for l, o in zip(layouts, out.slot_offset[:-1])]
+        ).astype(np.int32)
+        out.nbr = np.concatenate(
+            [np.where(l.nbr >= 0, l.nbr + np.int32(o), -1) for l, o in zip(layouts, nb[:-1])]
+        ).astype(np.int32)
+        return out
+
     @property
     def occupancy(self) -> float:
         return self.n_nodes / max(1, self.n_slots)
@@ -88,10 +125,11 @@ class BrickLayout:
         return self.to_slots(lab, fill=INACTIVE).astype(np.uint8)
 
 
-def choose_brick(S: np.ndarray, target_blocks: int = 2048) -> tuple:
+def choose_brick(S, target_blocks: int = 2048) -> tuple:
     """Default brick shape: the largest brick that still gives enough workgroups
-    to fill the chip (>= target_blocks); otherwise the smallest brick."""
-    n = int(np.count_nonzero(S))
+    to fill the chip (>= target_blocks); otherwise the smallest brick.  S is the
+    voxel mask or the voxel count."""
+    n = int(S) if np.ndim(S) == 0 else int(np.count_nonzero(S))
     for b in reversed(BRICK_SHAPES):
         lay_blocks = n / (b[0] * b[1] * b[2])
         if lay_blocks >= target_blocks:

@@ -307,16 +307,22 @@ class Trainer:
 # ---------------------------------------------------------------------------
 class SurrogateRunner:
     """Drives a MonodomainEngine (FP32) in a multirate form: per macro-step one
-    reaction update for the non-fallback voxels, then N_SUB fused steps with
-    do_reaction = 2 (fallback voxels run the full TTP06 every substep).
+    reaction update for the non-fallback voxels, then N_SUB substeps in which
+    the fallback voxels run the full TTP06 step.
 
     mode "mlp": the learned network (hp_surrogate_step);
     mode "bigstep": one 0.2 ms Rush-Larsen step of the exact model
     (hp_bigstep_reaction) -- the same rule without the network.
+
+    compact=True: the reaction call compacts the fallback voxels into a list;
+    each substep runs hp_fallback_react over that list, then the diffusion-only
+    fused kernel (do_reaction = 0, dV/dt from the rate array).  compact=False:
+    each substep runs the fused kernel with do_reaction = 2 (fallback voxels
+    found by their NaN rate, in divergent waves).  Both give identical results.
     """
 
     def __init__(self, eng, net: SurrogateNet = None, mode="mlp", dvdt_max=1.0, v_lo=-70.0,
-                 v_hi=-40.0, n_blocks=0):
+                 v_hi=-40.0, n_blocks=0, compact=True):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
         if mode not in ("mlp", "bigstep"):
@@ -328,14 +334,21 @@ class SurrogateRunner:
                                device=eng.dev)
         self.V_prev = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
                                  device=eng.dev)
+        self.compact = bool(compact)
+        self.fb_list = torch.zeros(eng.lay.n_slots if compact else 1, dtype=torch.int32,
+                                   device=eng.dev)
+        self.fb_count = torch.zeros(1, dtype=torch.int32, device=eng.dev)
         self.sp = nat.HpSurrogateParams(T_MACRO, dvdt_max, v_lo, v_hi, eng.ki_base, eng.nai_base,
-                                        0, 0)
+                                        0, 0, nat.ptr(self.fb_list) if compact else None,
+                                        nat.ptr(self.fb_count) if compact else None)
         self.n_blocks = n_blocks
         self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
         self._graph = None
 
     def launch_reaction(self):
         eng = self.eng
+        if self.compact:
+            self.fb_count.zero_()  # a fill kernel (safe inside a captured graph)
         if self.mode == "mlp":
             nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
                      nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
@@ -346,16 +359,25 @@ class SurrogateRunner:
                      nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
                      nat.ptr(self.rate), C.byref(self.sp), nat.stream_handle())
 
-    def _launch_macro(self):
+    def launch_substep(self):
+        """One substep: fallback voxels' TTP06 step, then diffusion + rate."""
         eng = self.eng
-        self.launch_reaction()
-        eng.p_myo.do_reaction = 2
+        if self.compact:
+            nat.call("hp_fallback_react_f32", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
+                     nat.ptr(eng.state), eng.lay.n_slots, nat.ptr(self.rate),
+                     nat.ptr(self.fb_list), nat.ptr(self.fb_count), eng.lay.n_slots,
+                     C.byref(eng.p_myo), nat.stream_handle())
+        eng.p_myo.do_reaction = 0 if self.compact else 2
         eng.p_myo.rate_ext = nat.ptr(self.rate)
         try:
-            for _ in range(N_SUB):
-                eng._launch_step()
+            eng._launch_step()
         finally:
             eng.p_myo.do_reaction, eng.p_myo.rate_ext = self._orig
+
+    def _launch_macro(self):
+        self.launch_reaction()
+        for _ in range(N_SUB):
+            self.launch_substep()
 
     def run(self, n_macro, graph=True):
         if not graph:
@@ -380,9 +402,24 @@ class SurrogateRunner:
         if n_macro % 2:
             self._launch_macro()
 
+    def _fallback_mask(self):
+        """(n_slots,) bool: fallback voxels of the current macro-step."""
+        if not self.compact:
+            return torch.isnan(self.rate) & (self.eng.label != 0xFF)
+        m = torch.zeros(self.eng.lay.n_slots, dtype=torch.bool, device=self.eng.dev)
+        n = int(self.fb_count.item())
+        m[self.fb_list[:n].long()] = True
+        return m
+
     def fallback_fraction(self):
         live = self.eng.label != 0xFF
-        return float(torch.isnan(self.rate[live]).float().mean())
+        return float(self._fallback_mask()[live].float().mean())
+
+    def fallback_brick_fraction(self):
+        """Fraction of bricks holding a fallback voxel (the waves that would
+        diverge without compaction)."""
+        fb = self._fallback_mask().view(-1, self.eng.lay.bvol)
+        return float(fb.any(dim=1).float().mean())
 
 
 def training_slab(device="cuda", nx=64, ny=64, nz=16, h=0.25, seed=0):

@@ -116,7 +116,7 @@ typedef struct hp_mono_params {
   double ki_base, nai_base;
   double vmin, vmax;  /* clamp, reference: [-150, 80] */
   double at_thr;      /* activation threshold, reference: -40 mV */
-  int32_t do_reaction;   /* 1: TTP06 step; 0: diffusion only; 2: TTP06 only where
+  int32_t do_reaction;   /* 1: TTP06 step; 0: no TTP06, dV/dt = rate_ext (if given); 2: TTP06 only where
                             rate_ext is NaN, else dV/dt = rate_ext (surrogate form) */
   int32_t n_inst;        /* number of rows of inst_params */
   const void* inst_params;  /* T[n_inst][4]: g_Ks scale, g_to scale, 2 spare (conductivity
@@ -134,6 +134,16 @@ int hp_myo_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* 
                     int64_t ld, const hp_mono_params* p, void* stream);
 int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
                     int64_t ld, const hp_mono_params* p, void* stream);
+
+/* Multirate substep, fallback part: one full TTP06 step of dt = p->dt for
+ * each of the *count slots in list (the fallback voxels compacted by the
+ * reaction call), at V^n = V, writing rate[slot] = -I_ion so that the
+ * diffusion step that follows (hp_myo_step with do_reaction = 0 and
+ * rate_ext = rate) applies it.  max_count bounds *count (sizes the grid).
+ * Uses p->dt, ki_base, nai_base and inst_params. */
+int hp_fallback_react_f32(const hp_bricks* g, const float* V, float* state, int64_t ld,
+                          float* rate, const int32_t* list, const int32_t* count,
+                          int64_t max_count, const hp_mono_params* p, void* stream);
 
 /* out = L V on the brick layout (uA/mm^2); out is 0 on non-myocardial slots.
  * Replaces torch.sparse.mm(L_global, V) restricted to the myocardium
@@ -247,6 +257,11 @@ int hp_step_tail_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scrat
 int hp_gather_f32(const float* src, const int32_t* idx, float* out, int64_t n, void* stream);
 int hp_gather_f64(const double* src, const int32_t* idx, double* out, int64_t n, void* stream);
 
+/* dst[0:n] = src[0:n], a plain streaming copy (n a multiple of 4, 16-byte
+ * aligned pointers).  No counterpart in the reference: the calibration point
+ * for the HBM bandwidth and the traffic counters of the kernels above. */
+int hp_stream_copy_f32(const float* src, float* dst, int64_t n, void* stream);
+
 
 /* ---------------------------------------------------------------------------
  * Torso Poisson forward solve (replaces _assemble_stiffness's operator,
@@ -348,6 +363,11 @@ typedef struct hp_surrogate_params {
   float ki_base, nai_base;
   int32_t force_fallback;
   int32_t pad_;
+  /* Optional compaction of the fallback voxels (both NULL = off): their slots
+   * are appended to fb_list (capacity n_slots) and counted in *fb_count, which
+   * the caller zeroes before the reaction call.  List order is unspecified. */
+  int32_t* fb_list;
+  int32_t* fb_count;
 } hp_surrogate_params;
 
 /* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),

@@ -119,12 +119,14 @@ class Problem:
 
 def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtype=torch.float64,
         record_every=0, record_nodes=None, at_thr=-40.0, max_steps=None, state_init=None,
-        V_init=None):
+        V_init=None, g_ks_scale=None, g_to_scale=None):
     """Integrate the problem; returns dict(V, state, at, trace_t, trace_V, steps, wall_s).
 
     ``at`` is (N,) first activation time (NaN if none), on every node.
     ``trace_V`` holds V of ``record_nodes`` (all nodes if None) every
     ``record_every`` steps (0 = off), stamped like the reference's frames.
+    ``g_ks_scale``/``g_to_scale``: optional per-node multipliers of the
+    phenotype conductances (the ensemble parameter block).
     """
     N, Cm = prob.N, prob.Cm
     n_steps = int(T / dt)
@@ -133,6 +135,8 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
     V = np.full(N, O.V_REST) if V_init is None else np.array(V_init, dtype=np.float64)
     st = O.init_state(N, dtype) if state_init is None else state_init.clone()
     ct = torch.from_numpy(prob.celltype)
+    gks_s = None if g_ks_scale is None else torch.as_tensor(np.asarray(g_ks_scale), dtype=dtype)
+    gto_s = None if g_to_scale is None else torch.as_tensor(np.asarray(g_to_scale), dtype=dtype)
     if scheme == "cn":
         A, M_inv = M.cn_matrix(prob.L, prob.diag_L, Cm / dt, prob.theta)
     at = np.full(N, np.nan)
@@ -141,7 +145,7 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
     for i in range(n_steps):
         t = i * dt
         I_stim = prob.stim_vector(t)
-        Iion, st = O.step_celltype(torch.from_numpy(V).to(dtype), st, dt, ct)
+        Iion, st = O.step_celltype(torch.from_numpy(V).to(dtype), st, dt, ct, gks_s, gto_s)
         Iion = Iion.double().numpy()
         if scheme == "cn":
             Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)

@@ -175,7 +175,7 @@ def test_training_converges(cuda, trained):
     assert st["heldout_dv_mae"] < 0.05  # mV over a 0.2 ms macro-step
 
 
-def _slab_run(cuda, form, net=None, T=120.0):
+def _slab_run(cuda, form, net=None, T=120.0, compact=True, graph=False):
     from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
         build_coupled_system)
     from tests._problems import corner_region, slab, stub_tree
@@ -190,16 +190,19 @@ def _slab_run(cuda, form, net=None, T=120.0):
     if form == "plain":
         for k in range(n_macro):
             eng.run(SG.N_SUB, graph_chunk=SG.N_SUB)
-            if k % 5 == 4:
+            if (k + 1) % 10 == 0:
                 trace.append(eng.V_myo_nodes().double().cpu().numpy())
         fb = 1.0
     else:
-        run = SG.SurrogateRunner(eng, net, mode=form)
+        run = SG.SurrogateRunner(eng, net, mode=form, compact=compact)
         fbs = []
-        for k in range(n_macro):
-            run.run(1, graph=False)
+        done = 0
+        while done < n_macro:
+            k = 2 if graph else 1  # the graph replays two macro-steps at once
+            run.run(k, graph=graph)
             fbs.append(run.fallback_fraction())
-            if k % 5 == 4:
+            done += k
+            if done % 10 == 0:
                 trace.append(eng.V_myo_nodes().double().cpu().numpy())
         fb = float(np.mean(fbs))
     return eng.at_nodes().double().cpu().numpy(), np.array(trace), fb
@@ -224,3 +227,21 @@ def test_surrogate_forms_vs_plain(cuda, trained):
         assert both.sum() == np.isfinite(at0).sum()
         assert err <= 1.0
     assert res["mlp"][2] < 0.5
+
+
+@pytest.mark.parametrize("form", ["bigstep", "mlp"])
+def test_compacted_fallback_matches_divergent(cuda, trained, form):
+    """The compacted fallback path (list + hp_fallback_react + diffusion-only
+    kernel, replayed as a HIP graph) against the divergent path (fused kernel
+    with do_reaction = 2, launched eagerly): the fallback voxels run the same
+    cell step in both, so Vm and activation times agree to round-off."""
+    print(f"[case] compacted fallback {form}", flush=True)
+    net, _ = trained
+    at0, tr0, fb0 = _slab_run(cuda, form, net, T=40.0, compact=False)
+    at1, tr1, fb1 = _slab_run(cuda, form, net, T=40.0, compact=True, graph=True)
+    dv = float(np.max(np.abs(tr1 - tr0)))
+    both = np.isfinite(at0) & np.isfinite(at1)
+    dat = float(np.max(np.abs(at1[both] - at0[both])))
+    print(f"  max |dV| {dv:.2e} mV, max |dAT| {dat:.2e} ms, fallback {fb0:.3f} / {fb1:.3f}")
+    assert np.array_equal(np.isfinite(at0), np.isfinite(at1))
+    assert dv < 1e-2 and dat < 1e-3

@@ -50,3 +50,43 @@ def test_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
     assert names <= set(nat.exported_symbols())
     assert lib.hp_abi_version() == nat.ABI_VERSION
+
+
+def test_concat_layout_keeps_instances_apart():
+    """Concatenated instances: slots/nodes offset per instance, neighbour table
+    entries stay inside their own instance, and each instance's part equals its
+    own layout shifted by the offsets."""
+    parts = [BrickLayout(slab(20, 16, 8)[0], (4, 4, 4)),
+             BrickLayout(slab(9, 13, 5, seed=2)[0], (4, 4, 4)),
+             BrickLayout(slab(4, 4, 4)[0], (4, 4, 4))]
+    L = BrickLayout.concat(parts)
+    assert L.n_nodes == sum(p.n_nodes for p in parts)
+    assert L.n_slots == sum(p.n_slots for p in parts)
+    for i, p in enumerate(parts):
+        b0, b1 = L.brick_offset[i], L.brick_offset[i + 1]
+        nb = L.nbr[b0:b1]
+        assert np.array_equal(nb, np.where(p.nbr >= 0, p.nbr + b0, -1))
+        n0, n1 = L.node_offset[i], L.node_offset[i + 1]
+        assert np.array_equal(L.slot_of_node[n0:n1], p.slot_of_node + L.slot_offset[i])
+    with pytest.raises(ValueError):
+        BrickLayout.concat([parts[0], BrickLayout(slab(8, 8, 8)[0], (8, 8, 8))])
+
+
+def test_ensemble_stimulus_groups():
+    """Per-instance groups: (inst << 3) | local site combination, with the group
+    table naming each instance's sites by (base, mask)."""
+    from hrdayapy_amd.engine import ensemble_stim_groups
+    inst = np.array([0] * 6 + [1] * 4 + [2] * 3)
+    # instance 0: sites a (nodes 0,1,2), b (2,3); instance 1: none; instance 2: c (10,11)
+    site_inst = [2, 0, 0]
+    site_nodes = [np.array([10, 11]), np.array([0, 1, 2]), np.array([2, 3])]
+    local, gs, order = ensemble_stim_groups(inst, site_inst, site_nodes, 3)
+    assert list(order) == [1, 2, 0]  # sites sorted by instance, stable
+    assert list(local) == [1, 1, 3, 2, 0, 0, 0, 0, 0, 0, 1, 1, 0]
+    assert gs.shape == (24, 2)
+    # instance 0 combination 3 = both of its sites, based at site 0
+    assert tuple(gs[3]) == (0, 3)
+    # instance 2's group 1 names global site base 2 (the third site after sorting)
+    assert tuple(gs[8 * 2 + 1]) == (2, 1)
+    with pytest.raises(ValueError):
+        ensemble_stim_groups(inst, [0, 0, 0, 0], [np.array([0])] * 4, 3)
