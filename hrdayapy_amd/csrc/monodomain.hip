@@ -42,66 +42,170 @@ struct Tile {
   int32_t nbr[27];
 };
 
-// Stage the halo tile of brick `b` into LDS.  Each lane handles NIT tile
-// cells; all their global loads (label, V and the six D components, read
-// whether or not the slot is tissue -- padding slots hold V0 and D = 0) are
-// issued back to back before any LDS write, so a workgroup waits for one
-// memory round trip instead of one per cell.
+// Cells of the halo tile that the 19-point stencil reads, enumerated so that
+// consecutive lanes read consecutive slots of one neighbour brick:
+//   [0, NI)           the brick itself (lane = own slot), all six D components;
+//   then the -x/+x face layers (BY*BZ cells each, contiguous slots), the -y/+y
+//   layers (runs of BZ), the -z/+z layers (stride BZ): three D components each
+//   (the face-axis diagonal and the two off-diagonals of the planes through the
+//   face); then the four z-, y- and x-parallel edge lines of the tile: one
+//   off-diagonal each.  Tile corners are never read.
+template <int BX, int BY, int BZ>
+struct HaloPlan {
+  static constexpr int NI = BX * BY * BZ;
+  static constexpr int E0 = NI, E1 = E0 + BY * BZ, E2 = E1 + BY * BZ, E3 = E2 + BX * BZ,
+                       E4 = E3 + BX * BZ, E5 = E4 + BX * BY, E6 = E5 + BX * BY,
+                       E7 = E6 + 4 * BZ, E8 = E7 + 4 * BY, NT = E8 + 4 * BX;
+  static constexpr int NIT = (NT + NI - 1) / NI;
+};
+
+struct HaloCell {
+  int tile;        // index into the LDS tile, -1 = no cell for this lane
+  int dir;         // neighbour-table entry of the brick holding the cell
+  int local;       // slot of the cell inside that brick
+  int ncomp;       // D components to load: 6 (own brick), 3 (face layer), 1 (edge line)
+  int c0, c1, c2;  // the components (face layers and edges)
+};
+
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ HaloCell halo_cell(int e) {
+  typedef HaloPlan<BX, BY, BZ> H;
+  constexpr int TY = BY + 2, TZ = BZ + 2;
+  HaloCell h;
+  int gx, gy, gz, dx = 0, dy = 0, dz = 0, lx, ly, lz;
+  h.ncomp = 3;
+  h.c0 = h.c1 = h.c2 = 0;
+  if (e < H::E0) {  // own brick
+    lx = e / (BY * BZ); ly = (e / BZ) % BY; lz = e % BZ;
+    gx = lx; gy = ly; gz = lz;
+    h.ncomp = 6;
+  } else if (e < H::E2) {  // x layers: D00, D01, D02
+    const int q = e - H::E0, sd = q / (BY * BZ), r = q % (BY * BZ);
+    ly = r / BZ; lz = r % BZ; dx = sd ? 1 : -1; lx = sd ? 0 : BX - 1;
+    gx = sd ? BX : -1; gy = ly; gz = lz;
+    h.c0 = 0; h.c1 = 3; h.c2 = 4;
+  } else if (e < H::E4) {  // y layers: D11, D01, D12
+    const int q = e - H::E2, sd = q / (BX * BZ), r = q % (BX * BZ);
+    lx = r / BZ; lz = r % BZ; dy = sd ? 1 : -1; ly = sd ? 0 : BY - 1;
+    gx = lx; gy = sd ? BY : -1; gz = lz;
+    h.c0 = 1; h.c1 = 3; h.c2 = 5;
+  } else if (e < H::E6) {  // z layers: D22, D02, D12
+    const int q = e - H::E4, sd = q / (BX * BY), r = q % (BX * BY);
+    lx = r / BY; ly = r % BY; dz = sd ? 1 : -1; lz = sd ? 0 : BZ - 1;
+    gx = lx; gy = ly; gz = sd ? BZ : -1;
+    h.c0 = 2; h.c1 = 4; h.c2 = 5;
+  } else if (e < H::E7) {  // z-parallel edges: D01
+    const int q = e - H::E6, ed = q / BZ;
+    const int sx = ed >> 1, sy = ed & 1;
+    lz = q % BZ; dx = sx ? 1 : -1; dy = sy ? 1 : -1;
+    lx = sx ? 0 : BX - 1; ly = sy ? 0 : BY - 1;
+    gx = sx ? BX : -1; gy = sy ? BY : -1; gz = lz;
+    h.ncomp = 1; h.c0 = 3;
+  } else if (e < H::E8) {  // y-parallel edges: D02
+    const int q = e - H::E7, ed = q / BY;
+    const int sx = ed >> 1, sz = ed & 1;
+    ly = q % BY; dx = sx ? 1 : -1; dz = sz ? 1 : -1;
+    lx = sx ? 0 : BX - 1; lz = sz ? 0 : BZ - 1;
+    gx = sx ? BX : -1; gy = ly; gz = sz ? BZ : -1;
+    h.ncomp = 1; h.c0 = 4;
+  } else if (e < H::NT) {  // x-parallel edges: D12
+    const int q = e - H::E8, ed = q / BX;
+    const int sy = ed >> 1, sz = ed & 1;
+    lx = q % BX; dy = sy ? 1 : -1; dz = sz ? 1 : -1;
+    ly = sy ? 0 : BY - 1; lz = sz ? 0 : BZ - 1;
+    gx = lx; gy = sy ? BY : -1; gz = sz ? BZ : -1;
+    h.ncomp = 1; h.c0 = 5;
+  } else {
+    h.tile = -1;
+    h.dir = 13;
+    h.local = 0;
+    h.ncomp = 0;
+    return h;
+  }
+  h.tile = ((gx + 1) * TY + (gy + 1)) * TZ + (gz + 1);
+  h.dir = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+  h.local = (lx * BY + ly) * BZ + lz;
+  return h;
+}
+
+// Stage the halo tile of brick `b` into LDS, in two phases: stage_nbr puts
+// the brick's neighbour table in LDS; stage_cells then loads NIT cells per
+// lane, all global loads (label, V, D components) issued before the first LDS
+// write, so a workgroup waits for one memory round trip.  `mylab` is the
+// lane's own label (loaded by the caller before stage_nbr): padding voxels of
+// the brick itself load nothing.  Padding slots of neighbour bricks are read
+// (they hold V0 and D = 0) and masked by their label.
 template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void stage_tile(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
-                                           const T* __restrict__ V, int64_t b, int64_t ld,
-                                           bool need_D) {
-  typedef Tile<T, BX, BY, BZ> TT;
-  constexpr int NIT = (TT::N + TT::BVOL - 1) / TT::BVOL;
-  const int tid = threadIdx.x;
-  if (tid < 27) t.nbr[tid] = g.nbr[b * 27 + tid];
+__device__ __forceinline__ void stage_nbr(Tile<T, BX, BY, BZ>& t, const hp_bricks& g, int64_t b) {
+  if (threadIdx.x < 27) t.nbr[threadIdx.x] = g.nbr[b * 27 + threadIdx.x];
   __syncthreads();
+}
+
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void stage_cells(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
+                                            const T* __restrict__ V, int64_t b, int64_t ld,
+                                            bool need_D, uint8_t mylab) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  typedef HaloPlan<BX, BY, BZ> H;
+  constexpr int NIT = H::NIT;
+  const int tid = threadIdx.x;
   const T* __restrict__ D6 = (const T*)g.D6;
+  HaloCell hc[NIT];
   int64_t sl[NIT];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = tid + it * TT::BVOL;
+    hc[it] = halo_cell<BX, BY, BZ>(it * TT::BVOL + tid);
     sl[it] = -1;
-    if (i < TT::N) {
-      const int tz = i % TT::TZ, ty = (i / TT::TZ) % TT::TY, tx = i / TT::SX;
-      const int gx = tx - 1, gy = ty - 1, gz = tz - 1;
-      const int dbx = gx < 0 ? -1 : (gx >= BX ? 1 : 0);
-      const int dby = gy < 0 ? -1 : (gy >= BY ? 1 : 0);
-      const int dbz = gz < 0 ? -1 : (gz >= BZ ? 1 : 0);
-      const int nout = (dbx != 0) + (dby != 0) + (dbz != 0);
-      if (nout < 3) {  // the 19-point stencil never reads tile corners
-        const int nb = t.nbr[(dbx + 1) * 9 + (dby + 1) * 3 + (dbz + 1)];
-        if (nb >= 0) {
-          const int lx = gx - dbx * BX, ly = gy - dby * BY, lz = gz - dbz * BZ;
-          sl[it] = (int64_t)nb * TT::BVOL + (lx * BY + ly) * BZ + lz;
-        }
-      }
+    if (hc[it].tile >= 0) {
+      const int nb = it == 0 ? (mylab != 0xFF ? (int)b : -1) : t.nbr[hc[it].dir];
+      if (nb >= 0) sl[it] = (int64_t)nb * TT::BVOL + hc[it].local;
     }
   }
   uint8_t lab[NIT];
-  T v[NIT], d[NIT][6];
+  T v[NIT], d[NIT][3], dself[3];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     lab[it] = 0xFF;
     if (sl[it] >= 0) {
-      lab[it] = g.label[sl[it]];
-      v[it] = V[sl[it]];
+      const int64_t q = sl[it];
+      lab[it] = g.label[q];
+      v[it] = V[q];
       if (need_D) {
+        if (it == 0) {  // own brick: all six components
 #pragma unroll
-        for (int k = 0; k < 6; ++k) d[it][k] = D6[k * ld + sl[it]];
+          for (int k = 0; k < 3; ++k) d[0][k] = D6[k * ld + q];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dself[k] = D6[(3 + k) * ld + q];
+        } else {
+          d[it][0] = D6[hc[it].c0 * ld + q];
+          if (hc[it].ncomp == 3) {
+            d[it][1] = D6[hc[it].c1 * ld + q];
+            d[it][2] = D6[hc[it].c2 * ld + q];
+          }
+        }
       }
     }
   }
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = tid + it * TT::BVOL;
-    if (i < TT::N) {
+    const int i = hc[it].tile;
+    if (i >= 0) {
       t.L[i] = lab[it];
       if (lab[it] != 0xFF) {
         t.V[i] = v[it];
         if (need_D) {
+          if (it == 0) {
 #pragma unroll
-          for (int k = 0; k < 6; ++k) t.D[k][i] = d[it][k];
+            for (int k = 0; k < 3; ++k) t.D[k][i] = d[0][k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) t.D[3 + k][i] = dself[k];
+          } else {
+            t.D[hc[it].c0][i] = d[it][0];
+            if (hc[it].ncomp == 3) {
+              t.D[hc[it].c1][i] = d[it][1];
+              t.D[hc[it].c2][i] = d[it][2];
+            }
+          }
         }
       }
     }
@@ -165,16 +269,17 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
   const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int64_t slot = b * TT::BVOL + tid;
-  // every-voxel reaction: issue the 18 state loads first, so that they are in
-  // flight together with the halo staging (slots of padding voxels are valid
-  // memory; their values are not used)
+  // every-voxel reaction: the 18 state loads of a tissue voxel are issued
+  // before the halo staging loads, so that both are in flight together
   const bool pre = kReact && p.do_reaction == 1;
+  const uint8_t mylab = g.label[slot];
+  stage_nbr<T, BX, BY, BZ>(t, g, b);
   T s[N_STATE];
-  if (pre) {
+  if (pre && mylab != 0xFF) {
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
   }
-  stage_tile<T, BX, BY, BZ>(t, g, V_in, b, ld, true);
+  stage_cells<T, BX, BY, BZ>(t, g, V_in, b, ld, true, mylab);
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
   const uint8_t lab = t.L[c];
@@ -235,7 +340,9 @@ __global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
   const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  stage_tile<T, BX, BY, BZ>(t, g, V, b, ld, true);
+  const uint8_t mylab = g.label[b * TT::BVOL + threadIdx.x];
+  stage_nbr<T, BX, BY, BZ>(t, g, b);
+  stage_cells<T, BX, BY, BZ>(t, g, V, b, ld, true, mylab);
   const int tid = threadIdx.x;
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);

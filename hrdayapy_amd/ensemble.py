@@ -97,6 +97,17 @@ def instance_geometry(spec: InstanceSpec):
     raise ValueError(f"unknown instance kind {spec.kind!r}")
 
 
+def instance_geometries(specs, workers=8):
+    """instance_geometry for every spec, on a pool of forked worker processes
+    (host numpy/scipy work only; the workers never touch the GPU)."""
+    specs = list(specs)
+    if workers <= 1 or len(specs) <= 1:
+        return [instance_geometry(s) for s in specs]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(min(workers, len(specs))) as pool:
+        return pool.map(instance_geometry, specs, chunksize=1)
+
+
 def _site_nodes(spec_site, vox, h, rng_pick):
     """Myocardial node ids within radius of a tissue voxel chosen by rng_pick in [0,1)."""
     centre = vox[min(len(vox) - 1, int(rng_pick * len(vox)))].astype(np.float64)
@@ -116,7 +127,7 @@ class Ensemble:
                  brick=None, A_M=140.0, Cm=0.01, phi_endo_max=1 / 3, phi_epi_min=2 / 3):
         t0 = time.perf_counter()
         self.specs = list(specs)
-        geoms = geoms if geoms is not None else [instance_geometry(s) for s in self.specs]
+        geoms = geoms if geoms is not None else instance_geometries(self.specs)
         self.t_geometry = time.perf_counter() - t0
         if brick is None:
             brick = choose_brick(sum(int(np.count_nonzero(g[0])) for g in geoms))
