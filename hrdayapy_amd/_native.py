@@ -63,6 +63,21 @@ class HpSurrogateParams(C.Structure):
                 ("fb_list", c_vp), ("fb_count", c_vp)]
 
 
+class HpMgLevel(C.Structure):
+    _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n", c_i64),
+                ("gx", c_vp), ("gy", c_vp), ("gz", c_vp), ("inv_diag", c_vp), ("x", c_vp),
+                ("tmp", c_vp), ("b", c_vp)]
+
+
+MG_MAX_LEVELS = 12
+
+
+class HpMg(C.Structure):
+    _fields_ = [("n_levels", c_i32), ("nu", c_i32), ("omega", c_f32), ("pad_", c_i32),
+                ("inv_diag0", c_vp), ("tmp0", c_vp), ("coarse_inv", c_vp),
+                ("level", HpMgLevel * MG_MAX_LEVELS)]
+
+
 class HpTorso(C.Structure):
     _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
                 ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
@@ -101,6 +116,11 @@ _SIGS = {
     "hp_stim_eval_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_step_advance_f32": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
     "hp_step_advance_f64": (C.c_int, [c_vp, c_i32, c_vp, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_mg_vcycle_f32": (C.c_int, [P(HpTorso), P(HpMg), c_vp, c_vp, c_vp, c_vp]),
+    "hp_mgpcg_begin_f32": (C.c_int, [P(HpTorso), P(HpMg), c_vp, c_vp, c_vp, c_vp, P(HpPcgWs),
+                                     c_f64, C.c_int, c_vp]),
+    "hp_mgpcg_iterate_f32": (C.c_int, [P(HpTorso), P(HpMg), c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       P(HpPcgWs), C.c_int, c_vp]),
     "hp_poisson_spmv_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp]),
     "hp_poisson_residual_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_vp]),
     "hp_pcg_begin_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64, C.c_int,
@@ -109,6 +129,9 @@ _SIGS = {
                                      c_vp]),
     "hp_refine_axpy_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_f64, c_vp]),
     "hp_cast_scale_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_f64, c_vp]),
+    "hp_heart_lap6_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
+    "hp_leadfield_apply_f32": (C.c_int, [c_vp, c_vp, c_i64, C.c_int, c_vp, c_i64, C.c_int, c_vp,
+                                         c_i64, c_vp, c_vp]),
     "hp_heart_lap6_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
     "hp_bspm_source_f64": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f64, c_f64,
                                      c_vp, c_i64, P(HpPcgWs), c_vp]),

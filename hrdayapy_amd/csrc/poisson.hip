@@ -17,129 +17,9 @@
 // and an iteration counter live in device memory, every kernel returns at
 // once when the `done` flag is set, and reductions are finished by the last
 // block to arrive (fixed summation order: deterministic results).
-#include "common.h"
-#include "hrdaya_hip.h"
+#include "poisson_common.h"
 
 namespace hp {
-
-constexpr int PWAVES = 4;          // waves per block
-constexpr int PBLOCK = 64 * PWAVES;
-
-struct ColMeta {
-  int k0, len, off;
-};
-
-__device__ __forceinline__ ColMeta col_meta(const hp_torso& t, int i, int j) {
-  ColMeta m;
-  if (i < 0 || i >= t.ni || j < 0 || j >= t.nj) {
-    m.k0 = 0;
-    m.len = 0;
-    m.off = 0;
-    return m;
-  }
-  const int c = i * t.nj + j;
-  m.k0 = t.col_k0[c];
-  m.len = t.col_len[c];
-  m.off = t.col_off[c];
-  return m;
-}
-
-__device__ __forceinline__ int cell_at(const ColMeta& m, int k) {
-  const int dk = k - m.k0;
-  return (dk >= 0 && dk < m.len) ? m.off + dk : -1;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block reduction of NV doubles followed by the last-block finish: the block
-// partials go to part[NV][gridDim.x]; the last block to arrive sums them in
-// block order into out[0..NV) and resets the arrival counter.
-template <int NV>
-__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double* part, int* counter,
-                                                  double* out) {
-  __shared__ double sh[NV][PWAVES];
-  __shared__ int is_last;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    const double s = wave_sum(v[q]);
-    if (lane == 0) sh[q][wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
-      part[q * gridDim.x + blockIdx.x] = s;
-    }
-    __threadfence();
-    const int old = atomicAdd(counter, 1);
-    is_last = (old == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += PBLOCK) s += part[q * gridDim.x + b];
-    s = wave_sum(s);
-    if (lane == 0) sh[q][wave] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      double s = 0.0;
-      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
-      out[q] = s;
-    }
-    *counter = 0;
-  }
-  return true;
-}
-
-// Iterate over (column, 64-wide k chunk) tasks with a grid-stride loop; one
-// task per wave.  F(cell, i, j, k, meta) is called for lanes inside the run.
-template <typename F>
-__device__ __forceinline__ void for_cells(const hp_torso& t, F&& f) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kch = (t.nk + 63) >> 6;
-  const int64_t ntask = (int64_t)t.ni * t.nj * kch;
-  for (int64_t w = (int64_t)blockIdx.x * PWAVES + wave; w < ntask; w += (int64_t)gridDim.x * PWAVES) {
-    const int col = (int)(w / kch), kc = (int)(w % kch);
-    const int i = col / t.nj, j = col % t.nj;
-    const ColMeta m = col_meta(t, i, j);
-    const int dk = kc * 64 + lane;
-    if (dk < m.len) f(m.off + dk, i, j, m.k0 + dk, m);
-  }
-}
-
-// A x at one cell in FP32 with the stored diagonal (the inner PCG operator).
-__device__ __forceinline__ float apply_row_f32(const hp_torso& t, const float* __restrict__ x, int c,
-                                               int i, int j, int k, const ColMeta& m) {
-  if (c == t.pin) return x[c];
-  float acc = t.diag[c] * x[c];
-  if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * x[c + 1];
-  if (k - 1 >= m.k0) acc -= t.gz[c - 1] * x[c - 1];
-  int n;
-  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * x[n];
-  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * x[n];
-  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * x[n];
-  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * x[n];
-  return acc;
-}
-
-// ---------------------------------------------------------------------------
-// PCG state in device memory (FP64 scalars)
-// ---------------------------------------------------------------------------
-enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR0, S_RR, S_TOL2, S_DONE, S_ITER, S_MAXIT, S_ZZ,
-       S_MODE, S_NCELL, S_NSC };
 
 __global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const float* __restrict__ x,
                                                           float* __restrict__ y) {
@@ -313,8 +193,8 @@ __global__ void cast_scale_kernel(hp_torso t, const double* __restrict__ r64, fl
 // gather their fine voxels (fixed order) minus the mean leak
 // (compute_bspm.py:386-419).
 // ---------------------------------------------------------------------------
-template <int BX, int BY, int BZ>
-__global__ void __launch_bounds__(BX * BY * BZ) heart_lap6_kernel(hp_bricks g, const float* __restrict__ V,
+template <int BX, int BY, int BZ, typename TV>
+__global__ void __launch_bounds__(BX * BY * BZ) heart_lap6_kernel(hp_bricks g, const TV* __restrict__ V,
                                                                   double* __restrict__ lap,
                                                                   double vscale) {
   constexpr int BV = BX * BY * BZ;
@@ -373,6 +253,54 @@ __global__ void source_gather_kernel(const double* __restrict__ lap, const int32
   b[row] = row == pin ? 0.0 : coeff * s * inv_nexp;
 }
 
+// Lead-field ECG: out[f][e] = sum_r W[r][e] * V[f][slot[r]] over the rows r
+// (heart voxels), FP32 products accumulated in FP64.  Pass 1: block q sums
+// its row range into part[q][f][e]; pass 2 adds the block partials in block
+// order (deterministic).  Threads of a block take consecutive electrodes, so
+// a row of W is read coalesced; the block's V values are staged in LDS.
+constexpr int LF_ROWS = 256, LF_MAXF = 16;
+__global__ void __launch_bounds__(256) leadfield_partial_kernel(
+    const float* __restrict__ W, const int32_t* __restrict__ row_slot, int64_t n_rows, int ne,
+    const float* __restrict__ V, int64_t ldv, int n_frames, int64_t rows_per_block,
+    double* __restrict__ part) {
+  __shared__ float vs[LF_MAXF][LF_ROWS];
+  const int e = blockIdx.y * blockDim.x + threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < n_rows ? r0 + rows_per_block : n_rows;
+  double acc[LF_MAXF];
+#pragma unroll
+  for (int f = 0; f < LF_MAXF; ++f) acc[f] = 0.0;
+  for (int64_t c0 = r0; c0 < r1; c0 += LF_ROWS) {
+    const int nr = (int)(r1 - c0 < LF_ROWS ? r1 - c0 : LF_ROWS);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nr; q += blockDim.x) {
+      const int64_t sl = row_slot[c0 + q];
+      for (int f = 0; f < n_frames; ++f) vs[f][q] = V[(int64_t)f * ldv + sl];
+    }
+    __syncthreads();
+    if (e < ne) {
+      for (int q = 0; q < nr; ++q) {
+        const double w = (double)W[(c0 + q) * ne + e];
+#pragma unroll
+        for (int f = 0; f < LF_MAXF; ++f)
+          if (f < n_frames) acc[f] += w * (double)vs[f][q];
+      }
+    }
+  }
+  if (e < ne)
+    for (int f = 0; f < n_frames; ++f)
+      part[((int64_t)blockIdx.x * n_frames + f) * ne + e] = acc[f];
+}
+
+__global__ void leadfield_reduce_kernel(const double* __restrict__ part, int n_blocks, int ne,
+                                        int n_frames, double* __restrict__ out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;  // f*ne + e
+  if (q >= ne * n_frames) return;
+  double s = 0.0;
+  for (int b = 0; b < n_blocks; ++b) s += part[(int64_t)b * ne * n_frames + q];
+  out[q] = s;
+}
+
 // out[e] = sum_k w[k] * phi[idx[k]] over k in [ptr[e], ptr[e+1])  (FP64)
 __global__ void leads_kernel(const double* __restrict__ phi, const int32_t* __restrict__ ptr,
                              const int32_t* __restrict__ idx, const double* __restrict__ w,
@@ -384,21 +312,30 @@ __global__ void leads_kernel(const double* __restrict__ phi, const int32_t* __re
   out[e] = s;
 }
 
-static int grid_blocks(const hp_torso* t) {
-  const int64_t ntask = (int64_t)t->ni * t->nj * ((t->nk + 63) / 64);
-  int64_t nb = (ntask + PWAVES - 1) / PWAVES;
-  if (nb > 4096) nb = 4096;
-  return (int)(nb < 1 ? 1 : nb);
-}
+}  // namespace hp
 
-static bool torso_ok(const hp_torso* t, const char* what) {
-  if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > (1 << 20)) {
-    set_error("%s: bad torso grid", what);
-    return false;
+namespace hp {
+template <typename TV>
+static int heart_lap6(const hp_bricks* g, const TV* V, double* lap, double vscale,
+                      void* stream) {
+  if (!g || g->n_bricks <= 0) {
+    set_error("hp_heart_lap6: empty brick grid");
+    return -1;
   }
-  return true;
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  const dim3 grid((unsigned)g->n_bricks);
+  hipStream_t st = (hipStream_t)stream;
+  switch (key) {
+    case 444: hipLaunchKernelGGL((heart_lap6_kernel<4, 4, 4, TV>), grid, dim3(64), 0, st, *g, V, lap, vscale); break;
+    case 844: hipLaunchKernelGGL((heart_lap6_kernel<8, 4, 4, TV>), grid, dim3(128), 0, st, *g, V, lap, vscale); break;
+    case 884: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 4, TV>), grid, dim3(256), 0, st, *g, V, lap, vscale); break;
+    case 888: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 8, TV>), grid, dim3(512), 0, st, *g, V, lap, vscale); break;
+    default:
+      set_error("hp_heart_lap6: unsupported brick shape");
+      return -1;
+  }
+  return check_launch("hp_heart_lap6");
 }
-
 }  // namespace hp
 
 using namespace hp;
@@ -459,23 +396,38 @@ int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double s
 
 int hp_heart_lap6_f32(const hp_bricks* g, const float* V, double* lap, double vscale,
                       void* stream) {
-  if (!g || g->n_bricks <= 0) {
-    set_error("hp_heart_lap6_f32: empty brick grid");
+  return heart_lap6<float>(g, V, lap, vscale, stream);
+}
+
+int hp_heart_lap6_f64(const hp_bricks* g, const double* V, double* lap, double vscale,
+                      void* stream) {
+  return heart_lap6<double>(g, V, lap, vscale, stream);
+}
+
+int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_rows, int ne,
+                           const float* V, int64_t ldv, int n_frames, double* part,
+                           int64_t part_len, double* out, void* stream) {
+  if (!W || !row_slot || !V || !part || !out || n_rows <= 0 || ne <= 0 || n_frames < 1 ||
+      n_frames > LF_MAXF) {
+    set_error("hp_leadfield_apply_f32: bad arguments (1..%d frames)", LF_MAXF);
     return -1;
   }
-  const int key = g->bx * 100 + g->by * 10 + g->bz;
-  const dim3 grid((unsigned)g->n_bricks);
-  hipStream_t st = (hipStream_t)stream;
-  switch (key) {
-    case 444: hipLaunchKernelGGL((heart_lap6_kernel<4, 4, 4>), grid, dim3(64), 0, st, *g, V, lap, vscale); break;
-    case 844: hipLaunchKernelGGL((heart_lap6_kernel<8, 4, 4>), grid, dim3(128), 0, st, *g, V, lap, vscale); break;
-    case 884: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 4>), grid, dim3(256), 0, st, *g, V, lap, vscale); break;
-    case 888: hipLaunchKernelGGL((heart_lap6_kernel<8, 8, 8>), grid, dim3(512), 0, st, *g, V, lap, vscale); break;
-    default:
-      set_error("hp_heart_lap6_f32: unsupported brick shape");
-      return -1;
+  int nb = 1024;
+  const int64_t rpb = (n_rows + nb - 1) / nb;
+  nb = (int)((n_rows + rpb - 1) / rpb);
+  if ((int64_t)nb * n_frames * ne > part_len) {
+    set_error("hp_leadfield_apply_f32: partial buffer needs %lld doubles",
+              (long long)nb * n_frames * ne);
+    return -1;
   }
-  return check_launch("hp_heart_lap6_f32");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (ne + 255) / 256), dim3(256), 0, st, W,
+                     row_slot, n_rows, ne, V, ldv, n_frames, rpb, part);
+  const int rc = check_launch("hp_leadfield_apply_f32/partial");
+  if (rc) return rc;
+  hipLaunchKernelGGL(leadfield_reduce_kernel, dim3(ceil_div((int64_t)ne * n_frames, 256)),
+                     dim3(256), 0, st, part, nb, ne, n_frames, out);
+  return check_launch("hp_leadfield_apply_f32/reduce");
 }
 
 int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,

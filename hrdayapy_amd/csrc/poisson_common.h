@@ -1,0 +1,143 @@
+// Shared pieces of the torso Poisson kernels (column-run layout, reductions,
+// PCG scalar slots) used by poisson.hip and multigrid.hip.
+#pragma once
+#include "common.h"
+#include "hrdaya_hip.h"
+
+namespace hp {
+
+constexpr int PWAVES = 4;          // waves per block
+constexpr int PBLOCK = 64 * PWAVES;
+
+struct ColMeta {
+  int k0, len, off;
+};
+
+__device__ __forceinline__ ColMeta col_meta(const hp_torso& t, int i, int j) {
+  ColMeta m;
+  if (i < 0 || i >= t.ni || j < 0 || j >= t.nj) {
+    m.k0 = 0;
+    m.len = 0;
+    m.off = 0;
+    return m;
+  }
+  const int c = i * t.nj + j;
+  m.k0 = t.col_k0[c];
+  m.len = t.col_len[c];
+  m.off = t.col_off[c];
+  return m;
+}
+
+__device__ __forceinline__ int cell_at(const ColMeta& m, int k) {
+  const int dk = k - m.k0;
+  return (dk >= 0 && dk < m.len) ? m.off + dk : -1;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block reduction of NV doubles followed by the last-block finish: the block
+// partials go to part[NV][gridDim.x]; the last block to arrive sums them in
+// block order into out[0..NV) and resets the arrival counter.
+template <int NV>
+__device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double* part, int* counter,
+                                                  double* out) {
+  __shared__ double sh[NV][PWAVES];
+  __shared__ int is_last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double s = wave_sum(v[q]);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
+      part[q * gridDim.x + blockIdx.x] = s;
+    }
+    __threadfence();
+    const int old = atomicAdd(counter, 1);
+    is_last = (old == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += PBLOCK) s += part[q * gridDim.x + b];
+    s = wave_sum(s);
+    if (lane == 0) sh[q][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < PWAVES; ++w) s += sh[q][w];
+      out[q] = s;
+    }
+    *counter = 0;
+  }
+  return true;
+}
+
+// Iterate over (column, 64-wide k chunk) tasks with a grid-stride loop; one
+// task per wave.  F(cell, i, j, k, meta) is called for lanes inside the run.
+template <typename F>
+__device__ __forceinline__ void for_cells(const hp_torso& t, F&& f) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int kch = (t.nk + 63) >> 6;
+  const int64_t ntask = (int64_t)t.ni * t.nj * kch;
+  for (int64_t w = (int64_t)blockIdx.x * PWAVES + wave; w < ntask; w += (int64_t)gridDim.x * PWAVES) {
+    const int col = (int)(w / kch), kc = (int)(w % kch);
+    const int i = col / t.nj, j = col % t.nj;
+    const ColMeta m = col_meta(t, i, j);
+    const int dk = kc * 64 + lane;
+    if (dk < m.len) f(m.off + dk, i, j, m.k0 + dk, m);
+  }
+}
+
+// A x at one cell in FP32 with the stored diagonal (the inner PCG operator).
+__device__ __forceinline__ float apply_row_f32(const hp_torso& t, const float* __restrict__ x, int c,
+                                               int i, int j, int k, const ColMeta& m) {
+  if (c == t.pin) return x[c];
+  float acc = t.diag[c] * x[c];
+  if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * x[c + 1];
+  if (k - 1 >= m.k0) acc -= t.gz[c - 1] * x[c - 1];
+  int n;
+  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * x[n];
+  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * x[n];
+  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * x[n];
+  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * x[n];
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// PCG state in device memory (FP64 scalars)
+// ---------------------------------------------------------------------------
+enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR0, S_RR, S_TOL2, S_DONE, S_ITER, S_MAXIT, S_ZZ,
+       S_MODE, S_NCELL, S_NSC };
+
+inline int grid_blocks(const hp_torso* t) {
+  const int64_t ntask = (int64_t)t->ni * t->nj * ((t->nk + 63) / 64);
+  int64_t nb = (ntask + PWAVES - 1) / PWAVES;
+  if (nb > 4096) nb = 4096;
+  return (int)(nb < 1 ? 1 : nb);
+}
+
+inline bool torso_ok(const hp_torso* t, const char* what) {
+  if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > (1 << 20)) {
+    set_error("%s: bad torso grid", what);
+    return false;
+  }
+  return true;
+}
+
+}  // namespace hp

@@ -15,6 +15,7 @@ Einthoven/Goldberger/Wilson definitions.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 import torch
@@ -23,10 +24,117 @@ from ... import _native as nat
 from ...layout import BrickLayout
 
 
-class PoissonSolver:
-    """The torso operator of a TorsoGrid resident in HBM, plus PCG work vectors."""
+def _face_sum(Gx, Gy, Gz):
+    """Sum of the six face conductances of every cell of a dense box (faces stored
+    on the + side of each cell)."""
+    S = Gx + Gy + Gz
+    S[1:] += Gx[:-1]
+    S[:, 1:] += Gy[:, :-1]
+    S[:, :, 1:] += Gz[:, :, :-1]
+    return S
 
-    def __init__(self, grid, device="cuda"):
+
+def coarsen_faces(Gx, Gy, Gz):
+    """2x2x2 agglomeration of a dense box of + face conductances: a coarse face is
+    1/2 of the sum of the four fine faces it covers (the operator at 2h)."""
+    pad = [(0, n % 2) for n in Gx.shape]
+    Gx, Gy, Gz = (np.pad(G, pad) for G in (Gx, Gy, Gz))
+    Cx = 0.5 * (Gx[1::2, 0::2, 0::2] + Gx[1::2, 1::2, 0::2] + Gx[1::2, 0::2, 1::2]
+                + Gx[1::2, 1::2, 1::2])
+    Cy = 0.5 * (Gy[0::2, 1::2, 0::2] + Gy[1::2, 1::2, 0::2] + Gy[0::2, 1::2, 1::2]
+                + Gy[1::2, 1::2, 1::2])
+    Cz = 0.5 * (Gz[0::2, 0::2, 1::2] + Gz[1::2, 0::2, 1::2] + Gz[0::2, 1::2, 1::2]
+                + Gz[1::2, 1::2, 1::2])
+    return Cx, Cy, Cz
+
+
+def dense_operator(Gx, Gy, Gz):
+    """The (n x n) operator of a dense box (Neumann), FP64, for small boxes."""
+    shape = Gx.shape
+    n = int(np.prod(shape))
+    idx = np.arange(n).reshape(shape)
+    A = np.zeros((n, n))
+    A[idx.ravel(), idx.ravel()] = _face_sum(Gx, Gy, Gz).ravel()
+    pairs = [(Gx[:-1], idx[:-1], idx[1:]), (Gy[:, :-1], idx[:, :-1], idx[:, 1:]),
+             (Gz[:, :, :-1], idx[:, :, :-1], idx[:, :, 1:])]
+    for g, a, b in pairs:
+        g, a, b = g.ravel(), a.ravel(), b.ravel()
+        A[a, b] -= g
+        A[b, a] -= g
+    return A
+
+
+class Multigrid:
+    """V-cycle preconditioner of a TorsoGrid's operator, resident in HBM
+    (hp_mg): level 0 is the column-run grid, coarser levels are dense boxes
+    down to at most max_coarse cells, solved there with a dense inverse."""
+
+    def __init__(self, grid, device="cuda", nu=2, omega=0.8, max_coarse=1024):
+        dev = torch.device(device)
+        L = grid.layout
+        shape = tuple(int(x) for x in grid["shape"])
+        flat = grid.flat_of_cell
+        G = []
+        for key in ("gx", "gy", "gz"):
+            a = np.zeros(int(np.prod(shape)), dtype=np.float64)
+            a[flat] = L[key]
+            G.append(a.reshape(shape))
+        pin_cell = int(L["pin"])
+        pin = np.array(np.unravel_index(int(flat[pin_cell]), shape))
+        S0 = _face_sum(*G).ravel()[flat]
+        inv0 = np.where(S0 > 0, 1.0 / L["diag"].astype(np.float64), 0.0)
+        inv0[pin_cell] = 0.0
+        self.keep = [torch.from_numpy(inv0.astype(np.float32)).to(dev),
+                     torch.zeros(int(L["n_cells"]), dtype=torch.float32, device=dev)]
+        levels, self.shapes, self.active = [], [], []
+        while True:
+            G = list(coarsen_faces(*G))
+            pin = pin // 2
+            S = _face_sum(*G)
+            inv = np.where(S > 0, 1.0 / np.where(S > 0, S, 1.0), 0.0)
+            inv[tuple(pin)] = 0.0
+            levels.append((G, inv))
+            self.shapes.append(G[0].shape)
+            self.active.append(int((inv > 0).sum()))
+            if G[0].size <= max_coarse:
+                break
+            if len(levels) == nat.MG_MAX_LEVELS:
+                raise ValueError("torso grid needs more multigrid levels than supported")
+        # exact inverse on the coarsest level's active cells
+        Gc, invc = levels[-1]
+        self.coarse_faces = Gc
+        A = dense_operator(*Gc)
+        act = np.nonzero(invc.ravel() > 0)[0]
+        Ainv = np.zeros_like(A)
+        if len(act):
+            Ainv[np.ix_(act, act)] = np.linalg.pinv(A[np.ix_(act, act)])
+        self.coarse_inv = torch.from_numpy(Ainv.astype(np.float32)).to(dev)
+        mg = nat.HpMg()
+        mg.n_levels, mg.nu, mg.omega = len(levels), int(nu), float(omega)
+        mg.inv_diag0, mg.tmp0 = self.keep[0].data_ptr(), self.keep[1].data_ptr()
+        mg.coarse_inv = self.coarse_inv.data_ptr()
+        for l, (Gl, inv) in enumerate(levels):
+            ni, nj, nk = Gl[0].shape
+            n = ni * nj * nk
+            ts = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).ravel()).to(dev)
+                  for a in (*Gl, inv)]
+            ts += [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(3)]
+            self.keep += ts
+            mg.level[l] = nat.HpMgLevel(ni, nj, nk, 0, n, *[t.data_ptr() for t in ts])
+        self.mg = mg
+
+    def vcycle(self, torso, r32, z32):
+        nat.call("hp_mg_vcycle_f32", C.byref(torso), C.byref(self.mg), r32.data_ptr(),
+                 z32.data_ptr(), None, nat.stream_handle())
+
+
+class PoissonSolver:
+    """The torso operator of a TorsoGrid resident in HBM, plus PCG work vectors.
+
+    precond: "jacobi" (the reference's diagonal preconditioner) or "mg" (one
+    V-cycle of the geometric multigrid of this module)."""
+
+    def __init__(self, grid, device="cuda", precond="jacobi"):
         nat.load()
         self.grid = grid
         self.dev = dev = torch.device(device)
@@ -49,6 +157,13 @@ class PoissonSolver:
         self.r32, self.d32, self.p32, self.Ap32 = (torch.zeros(n, **f32) for _ in range(4))
         self.r64 = torch.zeros(n, dtype=torch.float64, device=dev)
         self.last_stats = {}
+        if precond not in ("jacobi", "mg"):
+            raise ValueError(f"unknown preconditioner {precond!r}")
+        self.precond = precond
+        self.mg = None
+        if precond == "mg":
+            self.mg = Multigrid(grid, dev)
+            self.z32 = torch.zeros(n, **f32)
 
     # ------------------------------------------------------------ primitives
     def spmv32(self, x, y):
@@ -65,15 +180,30 @@ class PoissonSolver:
 
     def _pcg(self, tol, max_iter, mode, check_every):
         s = nat.stream_handle()
-        nat.call("hp_pcg_begin_f32", C.byref(self.torso), self.r32.data_ptr(), self.d32.data_ptr(),
-                 self.p32.data_ptr(), C.byref(self.ws), float(tol), int(max_iter), int(mode), s)
+        mg = self.mg is not None
+        if mg and mode != 0:
+            raise ValueError("the multigrid PCG stops on the relative residual only")
+        if mg:
+            check_every = max(1, check_every // 8)
+            nat.call("hp_mgpcg_begin_f32", C.byref(self.torso), C.byref(self.mg.mg),
+                     self.r32.data_ptr(), self.d32.data_ptr(), self.z32.data_ptr(),
+                     self.p32.data_ptr(), C.byref(self.ws), float(tol), int(max_iter), s)
+        else:
+            nat.call("hp_pcg_begin_f32", C.byref(self.torso), self.r32.data_ptr(),
+                     self.d32.data_ptr(), self.p32.data_ptr(), C.byref(self.ws), float(tol),
+                     int(max_iter), int(mode), s)
         done_iters = 0
         while True:
             n = min(check_every, max_iter - done_iters) if max_iter > done_iters else 0
             if n > 0:
-                nat.call("hp_pcg_iterate_f32", C.byref(self.torso), self.r32.data_ptr(),
-                         self.d32.data_ptr(), self.p32.data_ptr(), self.Ap32.data_ptr(),
-                         C.byref(self.ws), n, s)
+                if mg:
+                    nat.call("hp_mgpcg_iterate_f32", C.byref(self.torso), C.byref(self.mg.mg),
+                             self.r32.data_ptr(), self.d32.data_ptr(), self.z32.data_ptr(),
+                             self.p32.data_ptr(), self.Ap32.data_ptr(), C.byref(self.ws), n, s)
+                else:
+                    nat.call("hp_pcg_iterate_f32", C.byref(self.torso), self.r32.data_ptr(),
+                             self.d32.data_ptr(), self.p32.data_ptr(), self.Ap32.data_ptr(),
+                             C.byref(self.ws), n, s)
                 done_iters += n
             sc = self.sc[:11].cpu().numpy()
             if sc[7] != 0.0 or n == 0:
@@ -142,6 +272,7 @@ class HeartSource:
             raise ValueError("no heart voxel projects into the torso's active cells")
         slots = layout.slot_of_node[valid].astype(np.int32)
         cells = grid.cell_of_row[proj[valid]]
+        self.valid_cells = torch.from_numpy(cells.astype(np.int64)).to(dev)
         order = np.argsort(cells, kind="stable")
         rows_c, first = np.unique(cells[order], return_index=True)
         self.valid_slots = torch.from_numpy(slots).to(dev)
@@ -170,6 +301,76 @@ class HeartSource:
                  self.n_valid, self.rows.data_ptr(), self.ptr.data_ptr(), self.fine.data_ptr(),
                  int(self.rows.numel()), self.coeff, self.n_expected, b64.data_ptr(), self.pin,
                  C.byref(ws), s)
+
+
+class LeadField:
+    """Electrode potentials as a fixed linear functional of Vm.
+
+    The torso potential is x = A^-1 b(Vm) with b linear in Vm (the heart
+    source), and an electrode reads E_e^T x, so E_e^T x = (A^-1 E_e)^T b(Vm)
+    (reciprocity; the pinned operator restricted to the unpinned cells is
+    symmetric, and both E_e and b vanish on the pinned cell).  One refined
+    solve per electrode gives y_e = A^-1 E_e; pulling y_e back through the
+    source map (gather at each heart voxel's torso cell, remove the mean over
+    the valid voxels, apply the symmetric heart Laplacian, scale) gives the
+    per-voxel weights W[:, e].  Every ECG frame is then W^T Vm: one pass over
+    W for up to 16 frames, instead of a torso solve per frame.
+    """
+
+    def __init__(self, ps: "PoissonSolver", src: "HeartSource", el: "Electrodes", tol=1e-10,
+                 batch=16, log=None):
+        dev = ps.dev
+        self.dev, self.batch = dev, int(batch)
+        lay = src.lay
+        self.ne = ne = len(el.names)
+        self.names = el.names
+        t0 = time.perf_counter()
+        active = np.nonzero(lay.labels(np.zeros(lay.n_nodes, np.uint8)) != 0xFF)[0]
+        self.row_slot = torch.from_numpy(active.astype(np.int32)).to(dev)
+        n_rows = len(active)
+        W = torch.zeros((n_rows, ne), dtype=torch.float32, device=dev)
+        rhs = torch.zeros(ps.n, dtype=torch.float64, device=dev)
+        y = torch.zeros(ps.n, dtype=torch.float64, device=dev)
+        h = torch.zeros(lay.n_slots, dtype=torch.float64, device=dev)
+        kh = torch.zeros(lay.n_slots, dtype=torch.float64, device=dev)
+        cells = torch.from_numpy(el.cells.astype(np.int64)).to(dev)
+        kk = el.cells.shape[1]
+        vslots = src.valid_slots.long()
+        scale = src.coeff / src.n_expected
+        self.solve_stats = []
+        for e in range(ne):
+            rhs.zero_()
+            rhs.index_add_(0, cells[e], torch.full((kk,), 1.0 / kk, dtype=torch.float64,
+                                                   device=dev))
+            rhs[src.pin] = 0.0
+            y.zero_()
+            ps.solve_refined(rhs, y, tol=tol)
+            self.solve_stats.append(dict(ps.last_stats))
+            g = y[src.valid_cells]
+            h.zero_()
+            h[vslots] = g - g.mean()
+            nat.call("hp_heart_lap6_f64", C.byref(src.bricks), h.data_ptr(), kh.data_ptr(), 1e-3,
+                     nat.stream_handle())
+            W[:, e] = (kh[self.row_slot.long()] * scale).float()
+            if log and (e + 1) % 32 == 0:
+                log(f"[leadfield] {e + 1}/{ne} electrodes, {time.perf_counter() - t0:.1f} s")
+        self.W = W
+        self.setup_s = time.perf_counter() - t0
+        self.part = torch.zeros(1024 * self.batch * ne, dtype=torch.float64, device=dev)
+        self.out = torch.zeros((self.batch, ne), dtype=torch.float64, device=dev)
+
+    def apply(self, V_frames, n_frames=None, ldv=None):
+        """Electrode potentials (n_frames, ne) in V (FP64, on the device) of
+        n_frames Vm frames in slot layout: V_frames (n_frames, n_slots) FP32, mV."""
+        nf = int(V_frames.shape[0] if n_frames is None else n_frames)
+        if nf > self.batch:
+            raise ValueError(f"at most {self.batch} frames per call")
+        ldv = int(V_frames.shape[-1] if ldv is None else ldv)
+        nat.call("hp_leadfield_apply_f32", self.W.data_ptr(), self.row_slot.data_ptr(),
+                 int(self.W.shape[0]), self.ne, V_frames.data_ptr(), ldv, nf,
+                 self.part.data_ptr(), int(self.part.numel()), self.out.data_ptr(),
+                 nat.stream_handle())
+        return self.out[:nf]
 
 
 LEAD_NAMES = ["I", "II", "III", "aVR", "aVL", "aVF", "V1", "V2", "V3", "V4", "V5", "V6"]

@@ -313,6 +313,54 @@ int hp_pcg_begin_f32(const hp_torso* t, const float* r, float* d, float* p, hp_p
                      double tol, int max_iter, int mode, void* stream);
 int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap,
                        hp_pcg_ws* ws, int n_iter, void* stream);
+/* ---------------------------------------------------------------------------
+ * Geometric multigrid preconditioner for the torso operator.  No counterpart
+ * in the reference, whose _pcg_gpu preconditions with the diagonal
+ * (bspm_fn.py:164-263); it solves the same pinned system.  Level 0 is the
+ * column-run grid above; level[l] (l = 0..n_levels-1) is the dense box of grid
+ * level l+1, from 2x2x2 agglomeration of the level below, with face
+ * conductance 1/2 of the sum of the four fine faces it covers.  Cells with
+ * inv_diag = 0 (inactive, or holding the pinned cell) stay zero.  One V-cycle:
+ * nu damped-Jacobi sweeps (nu even) before and after each coarse correction,
+ * restriction = sum of child residuals, prolongation = parent value, exact
+ * solve on the coarsest level (<= 1024 cells) with a dense inverse.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_mg_level {
+  int32_t ni, nj, nk, pad_;
+  int64_t n;              /* ni*nj*nk, cell (i,j,k) at (i*nj + j)*nk + k */
+  const float* gx;        /* conductance of the +x face of each cell (0 = none) */
+  const float* gy;
+  const float* gz;
+  const float* inv_diag;  /* 1/(sum of the six faces); 0 = inactive or pinned */
+  float* x;               /* work vectors, n each */
+  float* tmp;
+  float* b;
+} hp_mg_level;
+
+#define HP_MG_MAX_LEVELS 12
+typedef struct hp_mg {
+  int32_t n_levels;       /* dense levels used, 1..HP_MG_MAX_LEVELS */
+  int32_t nu;             /* Jacobi sweeps before and after each coarse correction (even) */
+  float omega;            /* Jacobi damping */
+  int32_t pad_;
+  const float* inv_diag0; /* level 0: 1/diag per stored cell; 0 = pinned, hole, isolated */
+  float* tmp0;            /* level-0 work vector (n_cells) */
+  const float* coarse_inv;  /* [n][n] inverse on the coarsest level's active cells, 0 elsewhere */
+  hp_mg_level level[HP_MG_MAX_LEVELS];
+} hp_mg;
+
+/* z = B r, one V-cycle (all level work vectors overwritten).  scalars may be
+ * NULL; otherwise every kernel returns at once when the PCG done flag is set. */
+int hp_mg_vcycle_f32(const hp_torso* t, const hp_mg* mg, const float* r, float* z,
+                     const double* scalars, void* stream);
+/* PCG for A d = r with the V-cycle as preconditioner (FP32 vectors, FP64 inner
+ * products), stopping at ||r|| <= tol*||r0||: the same protocol and scalar
+ * slots as hp_pcg_begin/iterate (mode 0); z holds the preconditioned residual. */
+int hp_mgpcg_begin_f32(const hp_torso* t, const hp_mg* mg, const float* r, float* d, float* z,
+                       float* p, hp_pcg_ws* ws, double tol, int max_iter, void* stream);
+int hp_mgpcg_iterate_f32(const hp_torso* t, const hp_mg* mg, float* r, float* d, float* z,
+                         float* p, float* Ap, hp_pcg_ws* ws, int n_iter, void* stream);
+
 /* Iterative refinement helpers: x += scale*d (FP64), r32 = scale*r64. */
 int hp_refine_axpy_f64(const hp_torso* t, double* x, const float* d, double scale, void* stream);
 int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double scale,
@@ -325,6 +373,18 @@ int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double s
  * with mean over valid_slots, and b[pin] = 0.  Rows not listed are untouched. */
 int hp_heart_lap6_f32(const hp_bricks* g, const float* V, double* lap, double vscale,
                       void* stream);
+int hp_heart_lap6_f64(const hp_bricks* g, const double* V, double* lap, double vscale,
+                      void* stream);
+
+/* Lead-field ECG (no counterpart in the reference, which solves the torso for
+ * every frame): the electrode potentials are a fixed linear functional of Vm,
+ * phi[e] = sum_r W[r][e] Vm[slot[r]] (reciprocity with the pinned operator),
+ * so out[f*ne + e] = sum_r W[r*ne + e] * V[f*ldv + row_slot[r]] for n_frames
+ * (<= 16) frames, FP32 products summed in FP64 in a fixed order.  part holds
+ * part_len >= 1024*n_frames*ne doubles. */
+int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_rows, int ne,
+                           const float* V, int64_t ldv, int n_frames, double* part,
+                           int64_t part_len, double* out, void* stream);
 int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,
                        const int32_t* rows, const int32_t* ptr, const int32_t* fine,
                        int64_t n_rows_hit, double coeff, double n_expected, double* b,

@@ -71,3 +71,43 @@ def test_coupled_ecg_matches_oracle(cuda):
         print(f"  lead {i}: r = {r:.6f}, relative L2 error {rel:.2e}")
         assert r >= 0.999
         assert rel < 1e-2
+
+
+def test_leadfield_matches_forward_solve(cuda):
+    """Electrode potentials from the lead field (one solve per electrode, then
+    W^T Vm per frame) against the per-frame forward solve, on Vm frames of a
+    running slab, with the multigrid-preconditioned solver."""
+    print("[case] lead field vs forward solve", flush=True)
+    from hrdayapy_amd.ecg.functions.forward import (Electrodes, HeartSource, LeadField,
+                                                    PoissonSolver)
+    h = 0.25
+    S, phi, f = slab(24, 20, 8)
+    nodes, el, act = stub_tree(S, h)
+    kw = dict(voxel_size=h, dx_p=h, sigma_P=0.0, sigma_l=0.24, sigma_t=0.08, c_pmj=0.0, T=40.0,
+              ectopic_region=corner_region(S), phi_endo_max=1 / 3, phi_epi_min=2 / 3,
+              stim_protocol=[dict(name="c", target="ectopic", amp=0.52, dur=2.0, onset=0.0)])
+    eng, meta = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, device=cuda, **kw)
+    lab, origin, hc = G.torso((40, 36, 28), 1.5, heart_centre_frac=(0.5, 0.45, 0.5))
+    grid = TorsoGrid.from_labels(lab, G.TORSO_SIGMA, 1.5, origin)
+    reg = dict(R=np.eye(3), t=hc - meta["nodes_mm"].max(axis=0) / 2, scale=1.0)
+    pos = standard_electrodes(grid, hc, n_bspm=16)
+    ps = PoissonSolver(grid, cuda, precond="mg")
+    src = HeartSource(eng.lay, meta["nodes_mm"], grid, reg, 0.2625, h, cuda, bricks=eng.bricks)
+    els = Electrodes(grid, pos, device=cuda)
+    lf = LeadField(ps, src, els, tol=1e-11)
+    print(f"  {lf.ne} electrodes, setup {lf.setup_s:.2f} s, mean inner iterations "
+          f"{np.mean([s['inner'] for s in lf.solve_stats]):.0f}")
+    frames, fwd = [], []
+    for k in range(6):
+        eng.run(150)
+        V = eng.V[eng.p].clone()
+        frames.append(V)
+        b = torch.zeros(ps.n, dtype=torch.float64, device=cuda)
+        src(V, b, ps.ws)
+        x = ps.solve_refined(b, tol=1e-11)
+        fwd.append(els(x).clone())
+    fwd = torch.stack(fwd).cpu().numpy()
+    got = lf.apply(torch.stack(frames)).cpu().numpy()
+    err = np.abs(got - fwd).max() / np.abs(fwd).max()
+    print(f"  max |lead field - forward| / max |forward| = {err:.2e}")
+    assert err < 1e-6

@@ -155,3 +155,50 @@ def test_electrodes_and_leads(cuda):
     L = leads_from_electrodes(dict(zip(el.names, out)))
     assert abs(L["I"] + L["III"] - L["II"]) < 1e-12          # Einthoven's law
     assert abs(L["aVR"] + L["aVL"] + L["aVF"]) < 1e-12       # Goldberger sum
+
+
+def test_multigrid_vcycle_is_spd(cuda):
+    """The V-cycle is a symmetric positive definite operator B: u.Bv = v.Bu and
+    v.Bv > 0 for random residual vectors (zero on the pinned cell and holes)."""
+    print("[case] multigrid V-cycle symmetry", flush=True)
+    lab, g, Ap = _setup()
+    ps = PoissonSolver(g, cuda, precond="mg")
+    rng = np.random.default_rng(5)
+    n = Ap.shape[0]
+    vs = []
+    for _ in range(3):
+        v = rng.standard_normal(n)
+        v[g["pin_row"]] = 0.0
+        vs.append(_to_cells(g, v, cuda).float())
+    Bv = []
+    for v in vs:
+        z = torch.zeros_like(v)
+        ps.mg.vcycle(ps.torso, v, z)
+        Bv.append(z.double())
+    for a in range(3):
+        assert float(vs[a].double() @ Bv[a]) > 0
+        for b in range(a + 1, 3):
+            x, y = float(vs[a].double() @ Bv[b]), float(vs[b].double() @ Bv[a])
+            print(f"  u.Bv {x:.6e}  v.Bu {y:.6e}")
+            assert abs(x - y) <= 1e-5 * max(abs(x), abs(y))
+
+
+def test_multigrid_pcg_matches_direct(cuda):
+    print("[case] multigrid PCG vs direct", flush=True)
+    lab, g, Ap = _setup()
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(Ap.shape[0]) * 1e3
+    b -= b.mean()
+    b[g["pin_row"]] = 0.0
+    xd = spla.spsolve(Ap.tocsc(), b)
+    its = {}
+    for pc in ("jacobi", "mg"):
+        ps = PoissonSolver(g, cuda, precond=pc)
+        x = ps.solve_refined(_to_cells(g, b, cuda), tol=1e-10)
+        st = ps.last_stats
+        its[pc] = st["inner"]
+        print(f"  {pc}: outer {st['outer']} inner {st['inner']} rel_res {st['rel_res']:.2e}")
+        assert st["rel_res"] <= 1e-10
+        xs = x.cpu().numpy()[g.cell_of_row]
+        assert np.max(np.abs(xs - xd)) < 1e-7 * np.abs(xd).max()
+    assert its["mg"] * 4 < its["jacobi"]
