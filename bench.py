@@ -32,6 +32,19 @@ sys.path.insert(0, HERE)
 METRIC = "voxel-steps/s, 2M-voxel BiV: surrogate vs plain FP32 at AT err≤1ms, ECG r≥.999"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_STEP = 177  # 19 FP32 state r/w + 6 FP32 tensor + 1 label byte
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "myo_step_biv256_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the streaming-copy
+    calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def build_geometry(n=256, h=0.4, seed=0):
@@ -85,33 +98,45 @@ def autotune_brick(geo, tree, kw, steps=100):
     return best[0], res
 
 
-def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None, ref_at=None):
-    """Time the multirate form (mlp or bigstep) and compare its activation times
-    with the plain run's over the same window."""
+def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None, ref_at=None,
+                       ecg=None, ecg_every_macro=5, phi_ref=None):
+    """Time the multirate form (mlp or bigstep), with the ECG recorded every
+    ecg_every_macro macro-steps when a lead field is given, and compare its
+    activation times and leads with the plain run's over the same window."""
     import torch
     from hrdayapy_amd import surrogate as SG
+    from hrdayapy_amd import workloads as W
     eng, meta = build_engine(geo, tree, kw, brick=brick)
-    run = SG.SurrogateRunner(eng, net, mode=form)
+    run = SG.SurrogateRunner(eng, net, mode=form, graph_macros=ecg_every_macro)
     run.run(warm_macro)
-    run.run(2)
     torch.cuda.synchronize()
+    rec = W.EcgRecorder(ecg.lf, eng) if ecg else None
     t0 = time.perf_counter()
-    run.run(n_macro)
+    for _ in range(n_macro // ecg_every_macro):
+        run.run(ecg_every_macro)
+        if rec:
+            rec.record()
+    phi = rec.potentials() if rec else None
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     steps = n_macro * SG.N_SUB
     out = dict(value=meta["N_myo"] * steps / wall, ms_per_step=wall / steps * 1e3,
                fallback_fraction=round(run.fallback_fraction(), 4),
                fallback_brick_fraction=round(run.fallback_brick_fraction(), 4))
-    # per-kernel breakdown (HIP events on the launching stream)
+    at = eng.at_nodes().double().cpu().numpy()
+    # per-kernel breakdown (HIP events on the launching stream; advances the run)
     out["reaction_us"] = _event_us(run.launch_reaction, 20)
     out["substep_us"] = _event_us(run.launch_substep, 20)
     if ref_at is not None:
-        at = eng.at_nodes().double().cpu().numpy()
         both = np.isfinite(at) & np.isfinite(ref_at)
         out["at_err_vs_plain_max_ms"] = float(np.max(np.abs(at[both] - ref_at[both]))) if both.any() else None
         out["at_err_vs_plain_mean_ms"] = float(np.mean(np.abs(at[both] - ref_at[both]))) if both.any() else None
         out["activated_mismatch"] = int(np.isfinite(at).sum() - np.isfinite(ref_at).sum())
+    if phi is not None and phi_ref is not None and len(phi) == len(phi_ref):
+        r, rel = W.lead_agreement(W.twelve_leads(ecg.lf.names, phi),
+                                  W.twelve_leads(ecg.lf.names, phi_ref))
+        out["ecg_r_min_vs_plain"] = r
+        out["ecg_rel_l2_max_vs_plain"] = rel
     return out
 
 
@@ -179,7 +204,7 @@ def cpu_baseline(args_geo, kw, n_steps, dtype_name="float32"):
     prob = OMD.Problem(nodes, el, act, geo["S"], geo["phi"], geo["fibre_dir"], **kwo)
     dt_ = torch.float32 if dtype_name == "float32" else torch.float64
     OMD.run(prob, 0.02 * 2, 0.02, scheme="cn", dtype=dt_)  # warm caches
-    r = OMD.run(prob, 0.02 * n_steps, 0.02, scheme="cn", dtype=dt_)
+    r = OMD.run(prob, 0.02 * (n_steps + 0.5), 0.02, scheme="cn", dtype=dt_, max_steps=n_steps)
     vs = prob.N_myo * r["steps"] / r["wall_s"]
     return dict(value=vs, unit="voxel-steps/s", cores=cores, kind="port",
                 sample=f"{r['steps']} CN+BiCGSTAB steps of the same {prob.N_myo:,}-voxel BiV "
@@ -227,7 +252,7 @@ def main():
     ap.add_argument("--forms", type=str, default="plain,surrogate,bigstep")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
     ap.add_argument("--train-epochs", type=int, default=4)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
@@ -236,6 +261,9 @@ def main():
                     help="biv: the coupled 2M-voxel BiV step (default); torso: the 384x384x256 "
                          "Poisson solve; ensemble: 64 instances in one brick list")
     ap.add_argument("--ensemble-size", type=int, default=64)
+    ap.add_argument("--ecg", type=str, default="leadfield", choices=["leadfield", "none"])
+    ap.add_argument("--ecg-every-ms", type=float, default=1.0)
+    ap.add_argument("--bspm", type=int, default=256, help="BSPM electrodes besides the 12-lead set")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -248,7 +276,9 @@ def main():
         sys.exit(1)
     if a.workload != "biv":
         return other_workload(a)
+    from hrdayapy_amd import workloads as W
     forms = [f for f in a.forms.split(",") if f]
+    log = lambda m: print(m, flush=True)
     print(f"[bench] building geometry grid={a.grid}", flush=True)
     geo, tree, kw, t_geo = build_geometry(n=a.grid, h=0.4 * 256 / a.grid)
     tune = None
@@ -263,25 +293,49 @@ def main():
           f"{eng.lay.brick} occ={eng.lay.occupancy:.3f} PMJ={len(meta['pmj_ids'])} "
           f"geo {t_geo:.1f}s setup {meta['setup_s']:.1f}s", flush=True)
     chunk = a.graph_chunk
-    if a.steps % chunk:
-        a.steps += chunk - a.steps % chunk
+    every = int(round(a.ecg_every_ms / 0.02))
+    if a.ecg != "none" and every % chunk:
+        raise SystemExit("--ecg-every-ms must be a multiple of the graph chunk")
+    if a.steps % every:
+        a.steps += every - a.steps % every
     if a.warmup % chunk:
         a.warmup += chunk - a.warmup % chunk
+    ecg = None
+    if a.ecg == "leadfield":
+        ecg = W.EcgSetup(eng, meta, n_bspm=a.bspm, log=log)
+    # ---- plain form
     eng.run(a.warmup, graph_chunk=chunk)
     torch.cuda.synchronize()
     print(f"[bench] warmup done, timing {a.steps} steps", flush=True)
+    rec = W.EcgRecorder(ecg.lf, eng) if ecg else None
     t0 = time.perf_counter()
-    eng.run(a.steps, graph_chunk=chunk)
+    for _ in range(a.steps // every):
+        eng.run(every, graph_chunk=chunk)
+        if rec:
+            rec.record()
+    phi_plain = rec.potentials() if rec else None
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     vs = n_myo * a.steps / wall
-    print(f"[bench] plain: {wall:.3f} s, {vs:.4e} voxel-steps/s", flush=True)
+    print(f"[bench] plain: {wall:.3f} s, {vs:.4e} voxel-steps/s"
+          + (f" (ECG every {a.ecg_every_ms} ms included, {len(phi_plain)} frames)" if ecg else ""),
+          flush=True)
     at_plain = eng.at_nodes().double().cpu().numpy()
     activated = int(np.isfinite(at_plain).sum())
     k_s = time_myo_kernel(eng, a.kernel_reps)
     achieved = BYTES_PER_VOXEL_STEP * n_myo / k_s / 1e9
     print(f"[bench] fused myo kernel {k_s * 1e6:.1f} us/launch -> {achieved:.0f} GB/s algorithmic; "
           f"activated {activated}/{n_myo} at t={(a.warmup + a.steps) * 0.02:.1f} ms", flush=True)
+    ecg_info = None
+    if ecg:
+        fw_s, fw_stats = ecg.forward_solve_time(eng)
+        lf_us = _event_us(lambda: ecg.lf.apply(rec.frames, ecg.lf.batch), 5)
+        ecg_info = dict(ecg.info, method="leadfield", every_ms=a.ecg_every_ms,
+                        leadfield_apply_us_per_frame=round(lf_us / ecg.lf.batch, 1),
+                        forward_solve_ms_per_frame=round(fw_s * 1e3, 2),
+                        forward_solve_stats=fw_stats)
+        print(f"[bench] ECG: lead field {lf_us / ecg.lf.batch:.1f} us/frame; per-frame forward "
+              f"solve (MG-PCG, FP64 refinement to 1e-8) {fw_s * 1e3:.1f} ms", flush=True)
     copy = stream_copy_bandwidth(a.calib_copy_mb) if a.calib_copy_mb else None
     if copy:
         print(f"[bench] streaming copy {a.calib_copy_mb} MiB: {copy['us']:.1f} us -> "
@@ -307,14 +361,15 @@ def main():
         del X, D, tr
         torch.cuda.empty_cache()
         print(f"[bench] surrogate trained: {net_info}", flush=True)
-        r = run_surrogate_form(geo, tree, kw, "mlp", n_macro, warm_macro, brick, net, at_plain)
+        r = run_surrogate_form(geo, tree, kw, "mlp", n_macro, warm_macro, brick, net, at_plain,
+                               ecg, every // 10, phi_plain)
         r["speedup_vs_plain"] = r["value"] / vs
         r["network"] = net_info
         results["surrogate"] = r
         print(f"[bench] surrogate: {r}", flush=True)
     if "bigstep" in forms:
         r = run_surrogate_form(geo, tree, kw, "bigstep", n_macro, warm_macro, brick, None,
-                               at_plain)
+                               at_plain, ecg, every // 10, phi_plain)
         r["speedup_vs_plain"] = r["value"] / vs
         results["bigstep"] = r
         print(f"[bench] bigstep: {r}", flush=True)
@@ -322,24 +377,28 @@ def main():
     if not a.no_cpu_baseline:
         print(f"[bench] CPU baseline ({a.cpu_steps} steps)", flush=True)
         cpu = cpu_baseline((geo, tree), kw, a.cpu_steps)
-    # the headline value: the learned form when it meets the activation-time
-    # gate against the plain run, else the plain form
-    s_ok = ("surrogate" in results and results["surrogate"].get("at_err_vs_plain_max_ms") is not None
-            and results["surrogate"]["at_err_vs_plain_max_ms"] <= 1.0)
+    # the headline value: the learned form when it meets the activation-time and
+    # ECG gates against the plain run, else the plain form
+    sr = results.get("surrogate", {})
+    s_ok = (sr.get("at_err_vs_plain_max_ms") is not None and sr["at_err_vs_plain_max_ms"] <= 1.0
+            and (ecg is None or (sr.get("ecg_r_min_vs_plain") or 0.0) >= 0.999))
     head = "surrogate" if s_ok else "plain"
     value = results[head]["value"]
+    wl = ("synthetic BiV shell 256^3 @0.4 mm, FP32, dt 0.02 ms, Purkinje + PMJ"
+          + (f", ECG every {a.ecg_every_ms} ms in a 384x384x256 @1.5 mm torso "
+             f"({ecg.info['electrodes']} electrodes, lead field)" if ecg else ", no ECG"))
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": results[head]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "synthetic BiV shell 256^3 @0.4 mm, FP32, dt 0.02 ms, "
-                               "Purkinje + PMJ, no ECG",
-                   "form": head, "n_myo_voxels": n_myo, "n_purkinje_nodes": meta["Np"],
-                   "n_pmj": len(meta["pmj_ids"]), "brick": list(brick), "brick_autotune_us": tune,
+        "config": {"workload": wl, "form": head, "n_myo_voxels": n_myo,
+                   "n_purkinje_nodes": meta["Np"], "n_pmj": len(meta["pmj_ids"]),
+                   "brick": list(brick), "brick_autotune_us": tune,
                    "brick_occupancy": round(meta["layout"].occupancy, 4), "graph_chunk": chunk},
         "forms": results,
+        "ecg": ecg_info,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "myo_step_kernel<float> (plain form)", "kernel_us": k_s * 1e6,
                      "bytes_per_voxel": BYTES_PER_VOXEL_STEP},
         "cpu_baseline": cpu,

@@ -322,7 +322,7 @@ class SurrogateRunner:
     """
 
     def __init__(self, eng, net: SurrogateNet = None, mode="mlp", dvdt_max=1.0, v_lo=-70.0,
-                 v_hi=-40.0, n_blocks=0, compact=True):
+                 v_hi=-40.0, n_blocks=0, compact=True, graph_macros=2):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
         if mode not in ("mlp", "bigstep"):
@@ -344,6 +344,7 @@ class SurrogateRunner:
         self.n_blocks = n_blocks
         self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
         self._graph = None
+        self.graph_macros = int(graph_macros)
 
     def launch_reaction(self):
         eng = self.eng
@@ -390,16 +391,20 @@ class SurrogateRunner:
             g = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream(device=eng.dev)
             side.wait_stream(torch.cuda.current_stream(eng.dev))
+            k = self.graph_macros
+            if (k * N_SUB) % 2:
+                raise ValueError("a captured graph must hold an even number of substeps")
             with torch.cuda.stream(side):
                 with torch.cuda.graph(g, stream=side):
-                    self._launch_macro()
-                    self._launch_macro()
+                    for _ in range(k):
+                        self._launch_macro()
             torch.cuda.current_stream(eng.dev).wait_stream(side)
             eng.p = p0
             self._graph = g
-        for _ in range(n_macro // 2):
+        k = self.graph_macros
+        for _ in range(n_macro // k):
             self._graph.replay()
-        if n_macro % 2:
+        for _ in range(n_macro % k):
             self._launch_macro()
 
     def _fallback_mask(self):

@@ -42,34 +42,53 @@ def torso_solve(eng, meta, log=print, shape=(384, 384, 256), dx_mm=1.5, warm_ste
     lab, origin, hc = G.torso(shape, dx_mm)
     grid = TorsoGrid.from_labels(lab, G.TORSO_SIGMA, dx_mm, origin)
     t_grid = time.perf_counter() - t0
-    ps = PoissonSolver(grid, eng.dev)
+    t1 = time.perf_counter()
+    ps = PoissonSolver(grid, eng.dev, precond="mg")
+    t_mg = time.perf_counter() - t1
     extent = meta["nodes_mm"].max(axis=0)
     reg = dict(R=np.eye(3), t=hc - extent / 2, scale=1.0)
     src = HeartSource(eng.lay, meta["nodes_mm"], grid, reg, 0.2625, meta["voxel_size"], eng.dev,
                       bricks=eng.bricks)
     el = Electrodes(grid, standard_electrodes(grid, hc, n_bspm), device=eng.dev)
     log(f"[torso] {shape} @ {dx_mm} mm: {ps.n:,} stored cells "
-        f"({int((lab != 0).sum()):,} tissue), grid {t_grid:.1f} s; heart voxels in torso "
-        f"{src.n_valid:,}")
+        f"({int((lab != 0).sum()):,} tissue), grid {t_grid:.1f} s, multigrid {t_mg:.1f} s "
+        f"{ps.mg.shapes}; heart voxels in torso {src.n_valid:,}")
     eng.run(warm_steps, graph_chunk=50)
     b = torch.zeros(ps.n, dtype=torch.float64, device=eng.dev)
-    x = torch.zeros(ps.n, dtype=torch.float64, device=eng.dev)
-    src(eng.V[eng.p], b, ps.ws)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ps.solve_refined(b, x, tol=tol)
-    torch.cuda.synchronize()
-    cold = dict(ps.last_stats, wall_s=time.perf_counter() - t1)
-    log(f"[torso] cold solve: {cold}")
+    V0 = eng.V[eng.p].clone()
     eng.run(50, graph_chunk=50)
-    src(eng.V[eng.p], b, ps.ws)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ps.solve_refined(b, x, tol=tol)
-    torch.cuda.synchronize()
-    warm = dict(ps.last_stats, wall_s=time.perf_counter() - t1)
-    log(f"[torso] warm solve (1 ms later): {warm}")
+    V1 = eng.V[eng.p].clone()
+    solves = {}
+    x = torch.zeros(ps.n, dtype=torch.float64, device=eng.dev)
+    res = {}
+    for tag, V in (("cold", V0), ("warm", V1)):
+        b.zero_()
+        src(V, b, ps.ws)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ps.solve_refined(b, x, tol=tol)
+        torch.cuda.synchronize()
+        res[tag] = dict(ps.last_stats, wall_s=time.perf_counter() - t1)
+        log(f"[torso] multigrid PCG {tag} solve to {tol:g}: {res[tag]}")
+    solves["mg"] = res
     phi = el(x)
+    mg_keep = ps.mg
+    # the diagonal preconditioner (the reference's) for comparison: a bounded
+    # number of iterations from the cold start, reporting the residual reached
+    ps.mg, ps.precond = None, "jacobi"
+    xj = torch.zeros(ps.n, dtype=torch.float64, device=eng.dev)
+    b.zero_()
+    src(V0, b, ps.ws)
+    rn0, _ = ps.residual(b, xj)
+    jac = []
+    for chunk in range(4):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ps.solve_refined(b, xj, tol=tol, max_outer=1, max_inner=1000)
+        torch.cuda.synchronize()
+        jac.append(dict(ps.last_stats, wall_s=time.perf_counter() - t1))
+        log(f"[torso] Jacobi PCG, +1000 iterations: {jac[-1]}")
+    solves["jacobi_4x1000"] = jac
     # kernel times: SpMV and one fused PCG iteration
     s = torch.cuda.current_stream().cuda_stream
     spmv_us = _event_us(lambda: ps.spmv32(ps.p32, ps.Ap32), 50)
@@ -80,8 +99,11 @@ def torso_solve(eng, meta, log=print, shape=(384, 384, 256), dx_mm=1.5, warm_ste
                                        ps.Ap32.data_ptr(), C.byref(ps.ws), 10, s), 5) / 10
     res_us = _event_us(lambda: ps.residual(b, x), 10)
     n = ps.n
+    vc_us = _event_us(lambda: mg_keep.vcycle(ps.torso, ps.r32, ps.p32), 5)
+    log(f"[torso] V-cycle {vc_us:.0f} us")
     out = dict(cells=n, tissue_cells=int((lab != 0).sum()), grid_s=round(t_grid, 1),
-               cold=cold, warm=warm, spmv_us=round(spmv_us, 2),
+               mg_setup_s=round(t_mg, 1), mg_levels=[list(x) for x in mg_keep.shapes],
+               solves=solves, vcycle_us=round(vc_us, 1), spmv_us=round(spmv_us, 2),
                spmv_GBs=SPMV_BYTES_PER_CELL * n / spmv_us / 1e3,
                pcg_iter_us=round(it_us, 2), pcg_iter_GBs=PCG_BYTES_PER_CELL * n / it_us / 1e3,
                residual_f64_us=round(res_us, 2), electrodes=len(el.names),
@@ -119,3 +141,107 @@ def ensemble_run(n_inst=64, seed=0, steps=1000, warmup=100, graph_chunk=50, bric
                 brick=list(ens.layout.brick), steps=steps, wall_s=wall,
                 value=ens.n_nodes * steps / wall, ms_per_step=wall / steps * 1e3,
                 activated_fraction_mean=float(np.mean(act)), graph_chunk=graph_chunk)
+
+
+class EcgSetup:
+    """The ECG of a heart engine inside the 384x384x256 @ 1.5 mm synthetic torso:
+    torso operator, multigrid-preconditioned refined solver, heart source,
+    12-lead + n_bspm electrodes and the lead field."""
+
+    def __init__(self, eng, meta, shape=(384, 384, 256), dx_mm=1.5, n_bspm=256, tol=1e-10,
+                 log=print):
+        from .ecg.functions.forward import (Electrodes, HeartSource, LeadField, PoissonSolver,
+                                            standard_electrodes)
+        from .ecg.functions.torso import TorsoGrid
+        t0 = time.perf_counter()
+        lab, origin, hc = G.torso(shape, dx_mm)
+        self.grid = TorsoGrid.from_labels(lab, G.TORSO_SIGMA, dx_mm, origin)
+        self.t_grid = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        self.ps = PoissonSolver(self.grid, eng.dev, precond="mg")
+        self.t_mg = time.perf_counter() - t1
+        extent = meta["nodes_mm"].max(axis=0)
+        self.reg = dict(R=np.eye(3), t=hc - extent / 2, scale=1.0)
+        self.src = HeartSource(eng.lay, meta["nodes_mm"], self.grid, self.reg, 0.2625,
+                               meta["voxel_size"], eng.dev, bricks=eng.bricks)
+        self.el = Electrodes(self.grid, standard_electrodes(self.grid, hc, n_bspm),
+                             device=eng.dev)
+        log(f"[ecg] torso {shape} @ {dx_mm} mm: {self.ps.n:,} stored cells, grid "
+            f"{self.t_grid:.1f} s, multigrid {self.t_mg:.1f} s ({len(self.ps.mg.shapes)} levels "
+            f"down to {self.ps.mg.shapes[-1]}); {len(self.el.names)} electrodes; heart voxels "
+            f"in torso {self.src.n_valid:,}")
+        self.lf = LeadField(self.ps, self.src, self.el, tol=tol, log=log)
+        inner = [s["inner"] for s in self.lf.solve_stats]
+        self.info = dict(torso=list(shape), dx_mm=dx_mm, cells=self.ps.n, grid_s=round(self.t_grid, 1),
+                         mg_setup_s=round(self.t_mg, 1), mg_levels=len(self.ps.mg.shapes),
+                         electrodes=len(self.el.names), leadfield_setup_s=round(self.lf.setup_s, 1),
+                         leadfield_tol=tol, leadfield_inner_iters_mean=float(np.mean(inner)),
+                         heart_voxels_in_torso=self.src.n_valid)
+        log(f"[ecg] lead field: {self.lf.setup_s:.1f} s for {len(inner)} solves "
+            f"(mean {np.mean(inner):.1f} MG-PCG iterations)")
+
+    def forward_solve_time(self, eng, n=3, tol=1e-8):
+        """Per-frame forward solve (source + MG-PCG in FP64 refinement to tol,
+        warm-started), the reference's per-frame method: mean seconds."""
+        b = torch.zeros(self.ps.n, dtype=torch.float64, device=eng.dev)
+        x = torch.zeros(self.ps.n, dtype=torch.float64, device=eng.dev)
+        self.src(eng.V[eng.p], b, self.ps.ws)
+        self.ps.solve_refined(b, x, tol=tol)
+        ts, st = [], []
+        for _ in range(n):
+            eng.run(50, graph_chunk=50)
+            b.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.src(eng.V[eng.p], b, self.ps.ws)
+            self.ps.solve_refined(b, x, tol=tol)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+            st.append(dict(self.ps.last_stats))
+        return float(np.mean(ts)), st
+
+
+class EcgRecorder:
+    """Copies the engine's Vm every call into a frame batch and applies the lead
+    field when the batch is full; potentials() returns (n_frames, ne) in mV."""
+
+    def __init__(self, lf, eng):
+        self.lf, self.eng = lf, eng
+        self.frames = torch.zeros((lf.batch, eng.lay.n_slots), dtype=torch.float32,
+                                  device=eng.dev)
+        self.nb = 0
+        self.out = []
+
+    def record(self):
+        self.frames[self.nb].copy_(self.eng.V[self.eng.p])
+        self.nb += 1
+        if self.nb == self.lf.batch:
+            self.flush()
+
+    def flush(self):
+        if self.nb:
+            self.out.append((self.lf.apply(self.frames, self.nb) * 1e3).clone())
+            self.nb = 0
+
+    def potentials(self):
+        self.flush()
+        return torch.cat(self.out).cpu().numpy() if self.out else np.zeros((0, self.lf.ne))
+
+
+def twelve_leads(names, phi):
+    from .ecg.functions.forward import LEAD_NAMES, leads_from_electrodes
+    d = {n: phi[:, i] for i, n in enumerate(names)}
+    L = leads_from_electrodes(d)
+    return np.stack([L[k] for k in LEAD_NAMES], axis=1)
+
+
+def lead_agreement(a, b):
+    """min over the 12 leads of the correlation of a with b, and the max relative L2 error."""
+    rs, rel = [], []
+    for i in range(a.shape[1]):
+        x, y = a[:, i], b[:, i]
+        if np.std(x) == 0 or np.std(y) == 0:
+            continue
+        rs.append(float(np.corrcoef(x, y)[0, 1]))
+        rel.append(float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)))
+    return (min(rs) if rs else None), (max(rel) if rel else None)
