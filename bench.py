@@ -348,10 +348,7 @@ def main():
     if "surrogate" in forms:
         from hrdayapy_amd import surrogate as SG
         t1 = time.perf_counter()
-        teng = SG.training_slab("cuda")
-        per = max(1, a.train_samples // 2500)
-        X, D = SG.build_training_set(teng, 2500, per, seed=0)
-        del teng
+        X, D = SG.training_runs("cuda", a.train_samples)
         t_data = time.perf_counter() - t1
         net, tr, st = SG.train_surrogate(X, D, "cuda", epochs=a.train_epochs,
                                          log=lambda m: print("[bench] train " + m, flush=True))

@@ -81,7 +81,8 @@ class HpMg(C.Structure):
 class HpTorso(C.Structure):
     _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
                 ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
-                ("gz", c_vp), ("diag", c_vp), ("pin", c_i64)]
+                ("gz", c_vp), ("diag", c_vp), ("pin", c_i64), ("col_list", c_vp),
+                ("n_col_list", c_i64)]
 
 
 class HpPcgWs(C.Structure):

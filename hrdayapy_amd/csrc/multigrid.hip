@@ -10,8 +10,10 @@
 //                conductance between two coarse cells is half the sum of the
 //                four fine faces it covers (the operator rediscretised at 2h;
 //                the Galerkin product with piecewise-constant transfer would
-//                be twice as stiff).  Inactive cells and the cell holding the
-//                pinned reference have inv_diag = 0 and stay at zero.
+//                be twice as stiff).  Inactive cells have inv_diag = 0 and
+//                stay at zero.  The pinned cell is removed from the unknowns of
+//                level 0; its faces act as leaks to ground on its neighbours,
+//                carried to the coarse levels like faces (in the diagonal).
 //   smoothing    nu damped-Jacobi sweeps before and nu after each coarse
 //                correction (nu even, so the sweeps ping-pong back into place);
 //   transfer     restriction = sum of the children's residuals, prolongation =
@@ -32,18 +34,19 @@ __device__ __forceinline__ void dense_ijk(const hp_mg_level& L, int64_t c, int& 
   i = (int)(q / L.nj);
 }
 
-// sum over the faces of g_f (x_c - x_nb); faces to inactive cells have g = 0
+// (A x)_c = diag_c x_c - sum_f g_f x_nb with diag_c = 1/inv_diag_c (the face
+// sum plus the leak to the pinned cell, near it); faces to inactive cells have
+// g = 0.  Only called for active cells (inv_diag != 0).
 __device__ __forceinline__ float dense_Ax(const hp_mg_level& L, const float* __restrict__ x,
                                           int64_t c, int i, int j, int k) {
   const int64_t sx = (int64_t)L.nj * L.nk, sy = L.nk;
-  const float xc = x[c];
-  float acc = 0.f;
-  if (i + 1 < L.ni) acc += L.gx[c] * (xc - x[c + sx]);
-  if (i > 0) acc += L.gx[c - sx] * (xc - x[c - sx]);
-  if (j + 1 < L.nj) acc += L.gy[c] * (xc - x[c + sy]);
-  if (j > 0) acc += L.gy[c - sy] * (xc - x[c - sy]);
-  if (k + 1 < L.nk) acc += L.gz[c] * (xc - x[c + 1]);
-  if (k > 0) acc += L.gz[c - 1] * (xc - x[c - 1]);
+  float acc = x[c] * __builtin_amdgcn_rcpf(L.inv_diag[c]);
+  if (i + 1 < L.ni) acc -= L.gx[c] * x[c + sx];
+  if (i > 0) acc -= L.gx[c - sx] * x[c - sx];
+  if (j + 1 < L.nj) acc -= L.gy[c] * x[c + sy];
+  if (j > 0) acc -= L.gy[c - sy] * x[c - sy];
+  if (k + 1 < L.nk) acc -= L.gz[c] * x[c + 1];
+  if (k > 0) acc -= L.gz[c - 1] * x[c - 1];
   return acc;
 }
 
@@ -131,7 +134,7 @@ __global__ void __launch_bounds__(PBLOCK) mg_smooth0_kernel(hp_torso t, const fl
                                                            const float* __restrict__ b, float omega,
                                                            int zero, const double* sc) {
   if (mg_done(sc)) return;
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float inv = inv0[c];
     float v = 0.f;
     if (inv != 0.f)
@@ -156,11 +159,11 @@ __global__ void __launch_bounds__(256) mg_restrict0_kernel(hp_torso t, const flo
       for (int q = 0; q < 4; ++q) {
         const int i = 2 * I + (q >> 1), j = 2 * J + (q & 1);
         if (i >= t.ni || j >= t.nj) continue;
-        const ColMeta m = col_meta(t, i, j);
+        const ColNbr m = col_nbr(t, i, j);
 #pragma unroll
         for (int dk = 0; dk < 2; ++dk) {
           const int k = 2 * K + dk;
-          const int c = cell_at(m, k);
+          const int c = cell_at(m.c, k);
           if (c >= 0 && inv0[c] != 0.f) s += r[c] - apply_row_f32(t, x, c, i, j, k, m);
         }
       }
@@ -174,7 +177,7 @@ __global__ void __launch_bounds__(PBLOCK) mg_prolong0_kernel(hp_torso t, const f
                                                             float* __restrict__ z, hp_mg_level Cl,
                                                             const double* sc) {
   if (mg_done(sc)) return;
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta&) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr&) {
     if (inv0[c] != 0.f) z[c] += Cl.x[((int64_t)(i >> 1) * Cl.nj + (j >> 1)) * Cl.nk + (k >> 1)];
   });
 }
@@ -247,7 +250,7 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_init_kernel(hp_torso t, const fl
                                                            int* counter, double* sc, double tol,
                                                            int maxit) {
   double v[2] = {0.0, 0.0};
-  for_cells(t, [&](int c, int, int, int, const ColMeta&) {
+  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
     const float rc = r[c], zc = z[c];
     d[c] = 0.f;
     p[c] = zc;
@@ -280,20 +283,20 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_spmv_kernel(hp_torso t, const fl
   const bool first = sc[S_ITER] == 0.0;
   auto pnew = [&](int n) -> float { return first ? p[n] : z[n] + beta * p[n]; };
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float pc = pnew(c);
     float acc;
     if (c == t.pin) {
       acc = pc;
     } else {
       acc = t.diag[c] * pc;
-      if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * pnew(c + 1);
-      if (k - 1 >= m.k0) acc -= t.gz[c - 1] * pnew(c - 1);
+      if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * pnew(c + 1);
+      if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * pnew(c - 1);
       int n;
-      if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * pnew(n);
-      if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * pnew(n);
-      if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * pnew(n);
-      if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * pnew(n);
+      if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * pnew(n);
+      if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * pnew(n);
+      if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * pnew(n);
+      if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * pnew(n);
     }
     Ap[c] = acc;
     v[0] += (double)pc * (double)acc;
@@ -319,7 +322,7 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_update_kernel(hp_torso t, const 
   const float beta = (float)sc[S_BETA];
   const bool first = sc[S_ITER] == 0.0;
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int, int, int, const ColMeta&) {
+  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
     const float pc = first ? p[c] : z[c] + beta * p[c];
     p[c] = pc;
     d[c] += alpha * pc;
@@ -341,7 +344,7 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_rz_kernel(hp_torso t, const floa
                                                          int* counter, double* sc) {
   if (sc[S_DONE] != 0.0) return;
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int, int, int, const ColMeta&) {
+  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
     v[0] += (double)r[c] * (double)z[c];
   });
   double red[1];

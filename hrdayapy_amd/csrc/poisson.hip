@@ -23,7 +23,7 @@ namespace hp {
 
 __global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const float* __restrict__ x,
                                                           float* __restrict__ y) {
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     y[c] = apply_row_f32(t, x, c, i, j, k, m);
   });
 }
@@ -33,18 +33,18 @@ __global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const floa
 // are identity rows.  The diagonal is the exact FP64 sum of the FP32 face
 // conductances, so every interior row sums to zero exactly.
 __device__ __forceinline__ double apply_row_f64(const hp_torso& t, const double* __restrict__ x,
-                                                int c, int i, int j, int k, const ColMeta& m) {
+                                                int c, int i, int j, int k, const ColNbr& m) {
   const double xc = x[c];
   if (c == t.pin) return xc;
   double acc = 0.0, gs = 0.0;
   double g;
-  if (k + 1 < m.k0 + m.len && (g = t.gz[c]) != 0.0) { acc += g * (xc - x[c + 1]); gs += g; }
-  if (k - 1 >= m.k0 && (g = t.gz[c - 1]) != 0.0) { acc += g * (xc - x[c - 1]); gs += g; }
+  if (k + 1 < m.c.k0 + m.c.len && (g = t.gz[c]) != 0.0) { acc += g * (xc - x[c + 1]); gs += g; }
+  if (k - 1 >= m.c.k0 && (g = t.gz[c - 1]) != 0.0) { acc += g * (xc - x[c - 1]); gs += g; }
   int n;
-  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0 && (g = t.gx[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0 && (g = t.gx[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0 && (g = t.gy[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0 && (g = t.gy[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(m.xp, k)) >= 0 && (g = t.gx[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(m.xm, k)) >= 0 && (g = t.gx[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(m.yp, k)) >= 0 && (g = t.gy[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
+  if ((n = cell_at(m.ym, k)) >= 0 && (g = t.gy[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
   return gs == 0.0 ? xc : acc;
 }
 
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(PBLOCK) residual_f64_kernel(hp_torso t, const 
                                                               double* part, int* counter,
                                                               double* out) {
   double v[2] = {0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const double rc = b[c] - apply_row_f64(t, x, c, i, j, k, m);
     r[c] = rc;
     v[0] += rc * rc;
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, const floa
                                                           int* counter, double* sc, double tol,
                                                           int maxit, int mode) {
   double v[3] = {0.0, 0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float rc = r[c];
     const float z = rc / t.diag[c];
     d[c] = 0.0f;
@@ -108,20 +108,20 @@ __global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const floa
   const bool first = sc[S_ITER] == 0.0;
   double v[1] = {0.0};
   auto pnew = [&](int n) -> float { return first ? p[n] : r[n] / t.diag[n] + beta * p[n]; };
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float pc = pnew(c);
     float acc;
     if (c == t.pin) {
       acc = pc;
     } else {
       acc = t.diag[c] * pc;
-      if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * pnew(c + 1);
-      if (k - 1 >= m.k0) acc -= t.gz[c - 1] * pnew(c - 1);
+      if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * pnew(c + 1);
+      if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * pnew(c - 1);
       int n;
-      if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * pnew(n);
-      if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * pnew(n);
-      if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * pnew(n);
-      if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * pnew(n);
+      if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * pnew(n);
+      if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * pnew(n);
+      if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * pnew(n);
+      if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * pnew(n);
     }
     Ap[c] = acc;
     v[0] += (double)pc * (double)acc;
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
   const float beta = (float)sc[S_BETA];
   const bool first = sc[S_ITER] == 0.0;
   double v[3] = {0.0, 0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColMeta& m) {
+  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float inv = 1.0f / t.diag[c];
     const float pc = first ? p[c] : r[c] * inv + beta * p[c];
     p[c] = pc;
@@ -178,13 +178,13 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
 // x64 += scale * d
 __global__ void axpy_f64_kernel(hp_torso t, double* __restrict__ x, const float* __restrict__ d,
                                 double scale) {
-  for_cells(t, [&](int c, int, int, int, const ColMeta&) { x[c] += scale * (double)d[c]; });
+  for_cells(t, [&](int c, int, int, int, const ColNbr&) { x[c] += scale * (double)d[c]; });
 }
 
 // r32 = r64 * scale
 __global__ void cast_scale_kernel(hp_torso t, const double* __restrict__ r64, float* __restrict__ r32,
                                   double scale) {
-  for_cells(t, [&](int c, int, int, int, const ColMeta&) { r32[c] = (float)(r64[c] * scale); });
+  for_cells(t, [&](int c, int, int, int, const ColNbr&) { r32[c] = (float)(r64[c] * scale); });
 }
 
 // ---------------------------------------------------------------------------

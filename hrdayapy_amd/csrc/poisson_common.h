@@ -88,34 +88,52 @@ __device__ __forceinline__ bool block_reduce_last(double (&v)[NV], double* part,
   return true;
 }
 
-// Iterate over (column, 64-wide k chunk) tasks with a grid-stride loop; one
-// task per wave.  F(cell, i, j, k, meta) is called for lanes inside the run.
+// The metadata of a column and of its four x/y neighbours, loaded once per
+// column (wave-uniform: scalar loads).
+struct ColNbr {
+  ColMeta c, xp, xm, yp, ym;
+};
+
+__device__ __forceinline__ ColNbr col_nbr(const hp_torso& t, int i, int j) {
+  ColNbr m;
+  m.c = col_meta(t, i, j);
+  m.xp = col_meta(t, i + 1, j);
+  m.xm = col_meta(t, i - 1, j);
+  m.yp = col_meta(t, i, j + 1);
+  m.ym = col_meta(t, i, j - 1);
+  return m;
+}
+
+// One wave per non-empty column (t.col_list), grid-stride over the list; the
+// wave walks the column's run in 64-cell chunks.  F(cell, i, j, k, nbr) is
+// called for the lanes inside the run.
 template <typename F>
 __device__ __forceinline__ void for_cells(const hp_torso& t, F&& f) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kch = (t.nk + 63) >> 6;
-  const int64_t ntask = (int64_t)t.ni * t.nj * kch;
-  for (int64_t w = (int64_t)blockIdx.x * PWAVES + wave; w < ntask; w += (int64_t)gridDim.x * PWAVES) {
-    const int col = (int)(w / kch), kc = (int)(w % kch);
+  for (int64_t w = (int64_t)blockIdx.x * PWAVES + wave; w < t.n_col_list;
+       w += (int64_t)gridDim.x * PWAVES) {
+    const int col = t.col_list[w];
     const int i = col / t.nj, j = col % t.nj;
-    const ColMeta m = col_meta(t, i, j);
-    const int dk = kc * 64 + lane;
-    if (dk < m.len) f(m.off + dk, i, j, m.k0 + dk, m);
+    const ColNbr m = col_nbr(t, i, j);
+    for (int dk0 = 0; dk0 < m.c.len; dk0 += 64) {
+      const int dk = dk0 + lane;
+      if (dk < m.c.len) f(m.c.off + dk, i, j, m.c.k0 + dk, m);
+    }
   }
 }
 
 // A x at one cell in FP32 with the stored diagonal (the inner PCG operator).
 __device__ __forceinline__ float apply_row_f32(const hp_torso& t, const float* __restrict__ x, int c,
-                                               int i, int j, int k, const ColMeta& m) {
+                                               int i, int j, int k, const ColNbr& m) {
   if (c == t.pin) return x[c];
   float acc = t.diag[c] * x[c];
-  if (k + 1 < m.k0 + m.len) acc -= t.gz[c] * x[c + 1];
-  if (k - 1 >= m.k0) acc -= t.gz[c - 1] * x[c - 1];
+  if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * x[c + 1];
+  if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * x[c - 1];
   int n;
-  if ((n = cell_at(col_meta(t, i + 1, j), k)) >= 0) acc -= t.gx[c] * x[n];
-  if ((n = cell_at(col_meta(t, i - 1, j), k)) >= 0) acc -= t.gx[n] * x[n];
-  if ((n = cell_at(col_meta(t, i, j + 1), k)) >= 0) acc -= t.gy[c] * x[n];
-  if ((n = cell_at(col_meta(t, i, j - 1), k)) >= 0) acc -= t.gy[n] * x[n];
+  if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * x[n];
+  if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * x[n];
+  if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * x[n];
+  if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * x[n];
   return acc;
 }
 
@@ -126,14 +144,14 @@ enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR0, S_RR, S_TOL2, S_DONE, S_ITER, S_
        S_MODE, S_NCELL, S_NSC };
 
 inline int grid_blocks(const hp_torso* t) {
-  const int64_t ntask = (int64_t)t->ni * t->nj * ((t->nk + 63) / 64);
-  int64_t nb = (ntask + PWAVES - 1) / PWAVES;
+  int64_t nb = (t->n_col_list + PWAVES - 1) / PWAVES;
   if (nb > 4096) nb = 4096;
   return (int)(nb < 1 ? 1 : nb);
 }
 
 inline bool torso_ok(const hp_torso* t, const char* what) {
-  if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > (1 << 20)) {
+  if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > (1 << 20) || !t->col_list ||
+      t->n_col_list < 0 || t->n_col_list > (int64_t)t->ni * t->nj) {
     set_error("%s: bad torso grid", what);
     return false;
   }

@@ -48,6 +48,14 @@ def coarsen_faces(Gx, Gy, Gz):
     return Cx, Cy, Cz
 
 
+def _block_sum(a):
+    """Sum over the 2x2x2 children of each coarse cell (odd extents zero-padded)."""
+    a = np.pad(a, [(0, n % 2) for n in a.shape])
+    return (a[0::2, 0::2, 0::2] + a[1::2, 0::2, 0::2] + a[0::2, 1::2, 0::2] + a[0::2, 0::2, 1::2]
+            + a[1::2, 1::2, 0::2] + a[1::2, 0::2, 1::2] + a[0::2, 1::2, 1::2]
+            + a[1::2, 1::2, 1::2])
+
+
 def dense_operator(Gx, Gy, Gz):
     """The (n x n) operator of a dense box (Neumann), FP64, for small boxes."""
     shape = Gx.shape
@@ -84,16 +92,32 @@ class Multigrid:
         S0 = _face_sum(*G).ravel()[flat]
         inv0 = np.where(S0 > 0, 1.0 / L["diag"].astype(np.float64), 0.0)
         inv0[pin_cell] = 0.0
+        # The pinned cell is no unknown: its faces become leaks to ground on its
+        # neighbours, which the coarse levels inherit (agglomerated like faces).
+        # Eliminating the whole coarse parent of the pin instead would ground a
+        # cell 2^l times wider on level l and starve the near-constant mode.
+        leak = np.zeros(shape)
+        pi, pj, pk = (int(v) for v in pin)
+        for ax, Gd in enumerate(G):
+            lo = list(pin)
+            lo[ax] -= 1
+            if lo[ax] >= 0:                      # face (pin-1 -> pin), stored at pin-1
+                leak[tuple(lo)] += Gd[tuple(lo)]
+                Gd[tuple(lo)] = 0.0
+            hi = list(pin)
+            hi[ax] += 1
+            if hi[ax] < shape[ax]:               # face (pin -> pin+1), stored at pin
+                leak[tuple(hi)] += Gd[pi, pj, pk]
+                Gd[pi, pj, pk] = 0.0
         self.keep = [torch.from_numpy(inv0.astype(np.float32)).to(dev),
                      torch.zeros(int(L["n_cells"]), dtype=torch.float32, device=dev)]
         levels, self.shapes, self.active = [], [], []
         while True:
             G = list(coarsen_faces(*G))
-            pin = pin // 2
-            S = _face_sum(*G)
+            leak = 0.5 * _block_sum(leak)
+            S = _face_sum(*G) + leak
             inv = np.where(S > 0, 1.0 / np.where(S > 0, S, 1.0), 0.0)
-            inv[tuple(pin)] = 0.0
-            levels.append((G, inv))
+            levels.append((G, inv, leak))
             self.shapes.append(G[0].shape)
             self.active.append(int((inv > 0).sum()))
             if G[0].size <= max_coarse:
@@ -101,9 +125,9 @@ class Multigrid:
             if len(levels) == nat.MG_MAX_LEVELS:
                 raise ValueError("torso grid needs more multigrid levels than supported")
         # exact inverse on the coarsest level's active cells
-        Gc, invc = levels[-1]
-        self.coarse_faces = Gc
-        A = dense_operator(*Gc)
+        Gc, invc, leakc = levels[-1]
+        self.coarse_faces, self.coarse_leak = Gc, leakc
+        A = dense_operator(*Gc) + np.diag(leakc.ravel())
         act = np.nonzero(invc.ravel() > 0)[0]
         Ainv = np.zeros_like(A)
         if len(act):
@@ -113,7 +137,7 @@ class Multigrid:
         mg.n_levels, mg.nu, mg.omega = len(levels), int(nu), float(omega)
         mg.inv_diag0, mg.tmp0 = self.keep[0].data_ptr(), self.keep[1].data_ptr()
         mg.coarse_inv = self.coarse_inv.data_ptr()
-        for l, (Gl, inv) in enumerate(levels):
+        for l, (Gl, inv, _) in enumerate(levels):
             ni, nj, nk = Gl[0].shape
             n = ni * nj * nk
             ts = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).ravel()).to(dev)
@@ -142,11 +166,18 @@ class PoissonSolver:
         self.n = n = int(L["n_cells"])
         self._t = {k: torch.from_numpy(np.ascontiguousarray(L[k])).to(dev)
                    for k in ("col_k0", "col_len", "col_off", "gx", "gy", "gz", "diag")}
+        # non-empty columns, longest first (the grid-stride loop then ends on
+        # short columns: less tail imbalance)
+        lens = np.asarray(L["col_len"])
+        cols = np.nonzero(lens > 0)[0]
+        cols = cols[np.argsort(-lens[cols], kind="stable")].astype(np.int32)
+        self._t["col_list"] = torch.from_numpy(cols).to(dev)
         ni, nj, nk = grid["shape"]
         t = self._t
         self.torso = nat.HpTorso(ni, nj, nk, 0, n, t["col_k0"].data_ptr(), t["col_len"].data_ptr(),
                                  t["col_off"].data_ptr(), t["gx"].data_ptr(), t["gy"].data_ptr(),
-                                 t["gz"].data_ptr(), t["diag"].data_ptr(), int(L["pin"]))
+                                 t["gz"].data_ptr(), t["diag"].data_ptr(), int(L["pin"]),
+                                 t["col_list"].data_ptr(), len(cols))
         self.part = torch.zeros(4 * 4096, dtype=torch.float64, device=dev)
         self.counter = torch.zeros(4, dtype=torch.int32, device=dev)
         self.out = torch.zeros(4, dtype=torch.float64, device=dev)
@@ -344,7 +375,7 @@ class LeadField:
                                                    device=dev))
             rhs[src.pin] = 0.0
             y.zero_()
-            ps.solve_refined(rhs, y, tol=tol)
+            ps.solve_refined(rhs, y, tol=tol, max_outer=8, max_inner=2000)
             self.solve_stats.append(dict(ps.last_stats))
             g = y[src.valid_cells]
             h.zero_()
@@ -352,7 +383,7 @@ class LeadField:
             nat.call("hp_heart_lap6_f64", C.byref(src.bricks), h.data_ptr(), kh.data_ptr(), 1e-3,
                      nat.stream_handle())
             W[:, e] = (kh[self.row_slot.long()] * scale).float()
-            if log and (e + 1) % 32 == 0:
+            if log and (e + 1) % 8 == 0:
                 log(f"[leadfield] {e + 1}/{ne} electrodes, {time.perf_counter() - t0:.1f} s")
         self.W = W
         self.setup_s = time.perf_counter() - t0

@@ -427,10 +427,10 @@ class SurrogateRunner:
         return float(fb.any(dim=1).float().mean())
 
 
-def training_slab(device="cuda", nx=64, ny=64, nz=16, h=0.25, seed=0):
-    """The config-1 slab (64x64x16 @ 0.25 mm, sigma 0.13 isotropic, corner
-    stimulus 52 pA/pF for 2 ms) with a transmural phi across z, so that endo,
-    mid and epi cells are all sampled.  Returns an FP32 MonodomainEngine."""
+def training_slab(device="cuda", nx=64, ny=64, nz=16, h=0.25, seed=0, onset=0.0):
+    """The 64x64x16 @ 0.25 mm slab (sigma 0.13 isotropic, corner stimulus
+    52 pA/pF for 2 ms at `onset` ms) with a transmural phi across z, so that
+    endo, mid and epi cells are all sampled.  Returns an FP32 MonodomainEngine."""
     from .anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
         build_coupled_system)
     from .geometry import slab, stub_tree
@@ -443,8 +443,26 @@ def training_slab(device="cuda", nx=64, ny=64, nz=16, h=0.25, seed=0):
         nodes, el, act, S, phi, f, voxel_size=h, dx_p=h, sigma_P=0.0, sigma_l=0.13, sigma_t=0.13,
         Cm=0.01, A_M=140.0, c_pmj=0.0, T=500.0, phi_endo_max=1 / 3, phi_epi_min=2 / 3,
         ectopic_region=region, dt=0.02, dtype=torch.float32, device=device,
-        stim_protocol=[dict(name="corner", target="ectopic", amp=0.52, dur=2.0, onset=0.0)])
+        stim_protocol=[dict(name="corner", target="ectopic", amp=0.52, dur=2.0, onset=onset)])
     return eng
+
+
+def training_runs(device="cuda", samples=20_000_000, runs=((0.0, 500.0), (150.0, 500.0)), seed=0):
+    """Training set from plain-integrator runs of the slab: (stimulus onset, length)
+    per run.  The onset-0 run is the reference's slab protocol; the delayed
+    onset adds cells resting from the initial state for longer than the slab's
+    activation takes, which tissue larger than the slab contains.  Returns
+    (X raw features, D increments) on the device."""
+    Xs, Ds = [], []
+    per_run = samples // len(runs)
+    for k, (onset, T) in enumerate(runs):
+        eng = training_slab(device, onset=onset)
+        n_macro = int(round(T / T_MACRO))
+        X, D = build_training_set(eng, n_macro, max(1, per_run // n_macro), seed=seed + k)
+        Xs.append(X)
+        Ds.append(D)
+        del eng
+    return torch.cat(Xs), torch.cat(Ds)
 
 
 def build_training_set(eng, n_macro, per_snapshot, seed=0):
