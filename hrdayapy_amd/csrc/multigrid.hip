@@ -10,10 +10,8 @@
 //                conductance between two coarse cells is half the sum of the
 //                four fine faces it covers (the operator rediscretised at 2h;
 //                the Galerkin product with piecewise-constant transfer would
-//                be twice as stiff).  Inactive cells have inv_diag = 0 and
-//                stay at zero.  The pinned cell is removed from the unknowns of
-//                level 0; its faces act as leaks to ground on its neighbours,
-//                carried to the coarse levels like faces (in the diagonal).
+//                be twice as stiff).  Inactive cells and the cell holding the
+//                pinned reference have inv_diag = 0 and stay at zero.
 //   smoothing    nu damped-Jacobi sweeps before and nu after each coarse
 //                correction (nu even, so the sweeps ping-pong back into place);
 //   transfer     restriction = sum of the children's residuals, prolongation =
@@ -34,19 +32,19 @@ __device__ __forceinline__ void dense_ijk(const hp_mg_level& L, int64_t c, int& 
   i = (int)(q / L.nj);
 }
 
-// (A x)_c = diag_c x_c - sum_f g_f x_nb with diag_c = 1/inv_diag_c (the face
-// sum plus the leak to the pinned cell, near it); faces to inactive cells have
-// g = 0.  Only called for active cells (inv_diag != 0).
+// sum over the faces of g_f (x_c - x_nb); faces to inactive cells have g = 0,
+// and the grounded cell holds x = 0
 __device__ __forceinline__ float dense_Ax(const hp_mg_level& L, const float* __restrict__ x,
                                           int64_t c, int i, int j, int k) {
   const int64_t sx = (int64_t)L.nj * L.nk, sy = L.nk;
-  float acc = x[c] * __builtin_amdgcn_rcpf(L.inv_diag[c]);
-  if (i + 1 < L.ni) acc -= L.gx[c] * x[c + sx];
-  if (i > 0) acc -= L.gx[c - sx] * x[c - sx];
-  if (j + 1 < L.nj) acc -= L.gy[c] * x[c + sy];
-  if (j > 0) acc -= L.gy[c - sy] * x[c - sy];
-  if (k + 1 < L.nk) acc -= L.gz[c] * x[c + 1];
-  if (k > 0) acc -= L.gz[c - 1] * x[c - 1];
+  const float xc = x[c];
+  float acc = 0.f;
+  if (i + 1 < L.ni) acc += L.gx[c] * (xc - x[c + sx]);
+  if (i > 0) acc += L.gx[c - sx] * (xc - x[c - sx]);
+  if (j + 1 < L.nj) acc += L.gy[c] * (xc - x[c + sy]);
+  if (j > 0) acc += L.gy[c - sy] * (xc - x[c - sy]);
+  if (k + 1 < L.nk) acc += L.gz[c] * (xc - x[c + 1]);
+  if (k > 0) acc += L.gz[c - 1] * (xc - x[c - 1]);
   return acc;
 }
 

@@ -259,11 +259,11 @@ __global__ void source_gather_kernel(const double* __restrict__ lap, const int32
 // order (deterministic).  Threads of a block take consecutive electrodes, so
 // a row of W is read coalesced; the block's V values are staged in LDS.
 constexpr int LF_ROWS = 256, LF_MAXF = 16;
-__global__ void __launch_bounds__(256) leadfield_partial_kernel(
+__global__ void __launch_bounds__(1024) leadfield_partial_kernel(
     const float* __restrict__ W, const int32_t* __restrict__ row_slot, int64_t n_rows, int ne,
     const float* __restrict__ V, int64_t ldv, int n_frames, int64_t rows_per_block,
     double* __restrict__ part) {
-  __shared__ float vs[LF_MAXF][LF_ROWS];
+  __shared__ double vs[LF_MAXF][LF_ROWS];
   const int e = blockIdx.y * blockDim.x + threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < n_rows ? r0 + rows_per_block : n_rows;
@@ -273,9 +273,9 @@ __global__ void __launch_bounds__(256) leadfield_partial_kernel(
   for (int64_t c0 = r0; c0 < r1; c0 += LF_ROWS) {
     const int nr = (int)(r1 - c0 < LF_ROWS ? r1 - c0 : LF_ROWS);
     __syncthreads();
-    for (int q = threadIdx.x; q < nr; q += blockDim.x) {
-      const int64_t sl = row_slot[c0 + q];
-      for (int f = 0; f < n_frames; ++f) vs[f][q] = V[(int64_t)f * ldv + sl];
+    for (int q = threadIdx.x; q < nr * n_frames; q += blockDim.x) {
+      const int f = q / nr, rr = q % nr;
+      vs[f][rr] = (double)V[(int64_t)f * ldv + row_slot[c0 + rr]];
     }
     __syncthreads();
     if (e < ne) {
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256) leadfield_partial_kernel(
         const double w = (double)W[(c0 + q) * ne + e];
 #pragma unroll
         for (int f = 0; f < LF_MAXF; ++f)
-          if (f < n_frames) acc[f] += w * (double)vs[f][q];
+          if (f < n_frames) acc[f] = fma(w, vs[f][q], acc[f]);
       }
     }
   }
@@ -421,7 +421,9 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
     return -1;
   }
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (ne + 255) / 256), dim3(256), 0, st, W,
+  // one thread per electrode (up to 1024 per block), whole waves
+  const int bt = ne <= 1024 ? (ne + 63) / 64 * 64 : 1024;
+  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (ne + bt - 1) / bt), dim3(bt), 0, st, W,
                      row_slot, n_rows, ne, V, ldv, n_frames, rpb, part);
   const int rc = check_launch("hp_leadfield_apply_f32/partial");
   if (rc) return rc;

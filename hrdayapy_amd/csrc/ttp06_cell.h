@@ -18,8 +18,12 @@
 //                (h, j: -dt*(alpha+beta); Xr1, Xr2, m: products of the
 //                sigmoid denominators), buffer factors q^2/(q^2+c), one
 //                reciprocal for x_K1,inf, R' opening, I_up, I_NaK -- which
-//                removes about a third of the v_rcp_f32 instructions.  The
-//                FP32 path uses the hardware exp/log/rcp (1-2 ulp).
+//                removes about a third of the v_rcp_f32 instructions -- and
+//                exponentials of a common slope in V shared: exp(a(V+b)) =
+//                e^{ab} exp(aV) for the slopes 0.1, 0.2 (= 0.1 squared), 0.05,
+//                1/7, -1/6 and F/RT, which replaces 24 exponentials and an
+//                expm1 by 11 hardware exp/rcp.  The FP32 path uses the hardware
+//                exp/log/rcp (1-2 ulp).
 //
 // Storage of Ki and Nai: the state holds Ki - ki_base and Nai - nai_base.
 // With base 0 this is the reference's state.  The FP32 engine passes the
@@ -27,7 +31,7 @@
 // about one FP32 ulp of 138) are accumulated at full relative precision.
 //
 // FP32 I_CaL: the factor (V-15)/(exp(2(V-15)F/RT)-1) is evaluated as
-// x/expm1(x) with a series at |x| < 1e-6, instead of the reference's guard
+// x/(e^x - 1) with its series at |x| < 0.01, instead of the reference's guard
 // |den| < 1e-10 (below FP32 resolution).  FP64 keeps the reference form.
 #pragma once
 #include "common.h"
@@ -67,6 +71,26 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T one = C(1.0);
   // Rush-Larsen with the exponent supplied: x_inf - (x_inf - x) exp(-dt/tau)
 #define RLE(x, xi, ndt_over_tau) ((xi) - ((xi) - (x)) * m_::ex(ndt_over_tau))
+  // exp(ref) in the reference arrangement, `fast` (a shared exponential times
+  // a constant) in the FP32 form
+#define XP(ref, fast) (kRef ? m_::ex(ref) : (fast))
+  // shared exponentials of the FP32 form (V is clamped to [-150, 80], so none
+  // of these leaves the FP32 range)
+  T Q = one, iQ = one, P = one, iP = one, R5 = one, iR5 = one, S7 = one, iS7 = one, W6 = one,
+    E = one, iE = one;
+  if (!kRef) {
+    Q = m_::ex(C(0.1) * V);          // e^{0.1 V}
+    iQ = m_::rcp(Q);
+    P = Q * Q;                       // e^{0.2 V}
+    iP = iQ * iQ;
+    R5 = m_::ex(C(0.05) * V);        // e^{0.05 V}
+    iR5 = m_::rcp(R5);
+    S7 = m_::ex(V * C(1.0 / 7.0));   // e^{V/7}
+    iS7 = m_::rcp(S7);
+    W6 = m_::ex(V * C(-1.0 / 6.0));  // e^{-V/6}
+    E = m_::ex(V * FRT);             // e^{V F/RT}
+    iE = m_::rcp(E);
+  }
 
   // Nernst potentials (gpu_ttp06.py:242-247)
   T E_Na, E_K, E_Ks, E_Ca;
@@ -83,8 +107,8 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   }
 
   // I_Na (:250-276)
-  const T e_m1 = m_::ex((C(-60.0) - V) * C(0.2));
-  const T e_m2 = m_::ex((V + C(35.0)) * C(0.2));
+  const T e_m1 = XP((C(-60.0) - V) * C(0.2), C(6.14421235332821e-06) * iP);   // e^-12
+  const T e_m2 = XP((V + C(35.0)) * C(0.2), C(1096.6331584284585) * P);       // e^7
   const T e_m3 = m_::ex((V - C(50.0)) * C(1.0 / 200.0));
   T t1 = m_::rcp(one + m_::ex((C(-56.86) - V) * C(1.0 / 9.03)));
   const T m_inf = t1 * t1;
@@ -112,7 +136,8 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
     a_h = C(0.0);
     b_h = m_::dv(C(0.77), C(0.13) * (one + m_::ex((V + C(10.66)) * C(-1.0 / 11.1))));
     a_j = C(0.0);
-    b_j = m_::dv(C(0.6) * m_::ex(C(0.057) * V), one + m_::ex(C(-0.1) * (V + C(32.0))));
+    b_j = m_::dv(C(0.6) * m_::ex(C(0.057) * V),
+                 one + XP(C(-0.1) * (V + C(32.0)), C(0.04076220397836621) * iQ));  // e^-3.2
   }
   T ndt_h, ndt_j;
   if (kRef) {
@@ -125,10 +150,13 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T i_Na = C(14.838) * mg * mg * mg * hg * jg * (V - E_Na);
 
   // I_Kr (:279-287); sqrt(Ko/5.4) = 1
-  const T xr1_inf = m_::rcp(one + m_::ex((C(-26.0) - V) * C(1.0 / 7.0)));
-  const T e_r1 = m_::ex((C(-45.0) - V) * C(0.1)), e_r2 = m_::ex((V + C(30.0)) * C(1.0 / 11.5));
+  const T xr1_inf = m_::rcp(one + XP((C(-26.0) - V) * C(1.0 / 7.0),
+                                     C(0.0243728440732796) * iS7));  // e^(-26/7)
+  const T e_r1 = XP((C(-45.0) - V) * C(0.1), C(0.011108996538242306) * iQ);  // e^-4.5
+  const T e_r2 = m_::ex((V + C(30.0)) * C(1.0 / 11.5));
   const T xr2_inf = m_::rcp(one + m_::ex((V + C(88.0)) * C(1.0 / 24.0)));
-  const T e_r3 = m_::ex((C(-60.0) - V) * C(0.05)), e_r4 = m_::ex((V - C(60.0)) * C(0.05));
+  const T e_r3 = XP((C(-60.0) - V) * C(0.05), C(0.049787068367863944) * iR5);  // e^-3
+  const T e_r4 = XP((V - C(60.0)) * C(0.05), C(0.049787068367863944) * R5);
   T ndt_xr1, ndt_xr2;
   if (kRef) {
     const T tau_xr1 = C(450.0) * m_::rcp(one + e_r1) * C(6.0) * m_::rcp(one + e_r2);
@@ -144,7 +172,7 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
 
   // I_Ks (:290-294)
   const T xs_inf = m_::rcp(one + m_::ex((C(-5.0) - V) * C(1.0 / 14.0)));
-  const T e_xs = m_::ex((C(5.0) - V) * C(1.0 / 6.0));
+  const T e_xs = XP((C(5.0) - V) * C(1.0 / 6.0), C(2.300975890892825) * W6);  // e^(5/6)
   T a_xs;
   if (kRef) a_xs = m_::dv(C(1400.0), m_::sq(one + e_xs));
   else a_xs = C(1400.0) * m_::rsq(one + e_xs);
@@ -171,16 +199,18 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   // I_to and the phenotype s-gate (:305-317)
   T s_inf, tau_s;
   if (endo_s) {
-    s_inf = m_::rcp(one + m_::ex((V + C(28.0)) * C(0.2)));
+    s_inf = m_::rcp(one + XP((V + C(28.0)) * C(0.2), C(270.42640742615254) * P));  // e^5.6
     const T u = V + C(67.0);
     tau_s = C(1000.0) * m_::ex(-u * u * C(1.0 / 1000.0)) + C(8.0);
   } else {
-    s_inf = m_::rcp(one + m_::ex((V + C(20.0)) * C(0.2)));
+    s_inf = m_::rcp(one + XP((V + C(20.0)) * C(0.2), C(54.598150033144236) * P));  // e^4
     const T u = V + C(45.0);
     tau_s = C(85.0) * m_::ex(-u * u * C(1.0 / 320.0)) +
-            C(5.0) * m_::rcp(one + m_::ex((V - C(20.0)) * C(0.2))) + C(3.0);
+            C(5.0) * m_::rcp(one + XP((V - C(20.0)) * C(0.2), C(0.01831563888873418) * P)) +
+            C(3.0);  // e^-4
   }
-  const T r_inf = m_::rcp(one + m_::ex((C(20.0) - V) * C(1.0 / 6.0)));
+  const T r_inf = m_::rcp(one + XP((C(20.0) - V) * C(1.0 / 6.0),
+                                  C(28.03162489452614) * W6));  // e^(20/6)
   const T u40 = V + C(40.0);
   const T tau_r = C(9.5) * m_::ex(-u40 * u40 * C(1.0 / 1800.0)) + C(0.8);
   const T i_to = g_to * rg * sg * vek;
@@ -188,24 +218,32 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   // I_CaL (:320-340)
   const T d_inf = m_::rcp(one + m_::ex((C(-8.0) - V) * C(1.0 / 7.5)));
   const T tau_d = (C(1.4) * m_::rcp(one + m_::ex((C(-35.0) - V) * C(1.0 / 13.0))) + C(0.25)) * C(1.4) *
-                      m_::rcp(one + m_::ex((V + C(5.0)) * C(0.2))) +
-                  m_::rcp(one + m_::ex((C(50.0) - V) * C(0.05)));
-  const T f_inf = m_::rcp(one + m_::ex((V + C(20.0)) * C(1.0 / 7.0)));
+                      m_::rcp(one + XP((V + C(5.0)) * C(0.2), C(2.718281828459045) * P)) +
+                  m_::rcp(one + XP((C(50.0) - V) * C(0.05), C(12.182493960703473) * iR5));
+  const T f_inf = m_::rcp(one + XP((V + C(20.0)) * C(1.0 / 7.0),
+                                   C(17.41170806332765) * S7));  // e^(20/7)
   const T u27 = V + C(27.0);
-  const T r30 = m_::rcp(one + m_::ex((V + C(30.0)) * C(0.1)));
+  const T r30 = m_::rcp(one + XP((V + C(30.0)) * C(0.1), C(20.085536923187668) * Q));  // e^3
   const T tau_f = C(1102.5) * m_::ex(-(u27 * u27) * C(1.0 / 225.0)) +
-                  C(200.0) * m_::rcp(one + m_::ex((C(13.0) - V) * C(0.1))) + C(180.0) * r30 +
+                  C(200.0) * m_::rcp(one + XP((C(13.0) - V) * C(0.1),
+                                              C(3.6692966676192444) * iQ)) +  // e^1.3
+                  C(180.0) * r30 +
                   C(20.0);
-  const T f2_inf = C(0.67) * m_::rcp(one + m_::ex((V + C(35.0)) * C(1.0 / 7.0))) + C(0.33);
+  const T f2_inf = C(0.67) * m_::rcp(one + XP((V + C(35.0)) * C(1.0 / 7.0),
+                                             C(148.4131591025766) * S7)) +  // e^5
+                   C(0.33);
   const T tau_f2 = C(562.0) * m_::ex(-(u27 * u27) * C(1.0 / 240.0)) +
-                   C(31.0) * m_::rcp(one + m_::ex((C(25.0) - V) * C(0.1))) + C(80.0) * r30;
+                   C(31.0) * m_::rcp(one + XP((C(25.0) - V) * C(0.1),
+                                              C(12.182493960703473) * iQ)) +  // e^2.5
+                   C(80.0) * r30;
   const T cs = Cass * C(20.0);  // Cass / 0.05
   const T cs2 = cs * cs;
   const T rcs = m_::rcp(one + cs2);
   const T fCass_inf = C(0.6) * rcs + C(0.4);
   const T tau_fCass = C(80.0) * rcs + C(2.0);
   const T xcal = C(2.0) * (V - C(15.0)) * FRT;
-  const T e2v = m_::ex(xcal);
+  // e^{2(V-15)F/RT} = e^{2VF/RT} e^{-30F/RT}
+  const T e2v = XP(xcal, E * E * C(0.3252969814969026));
   T i_CaL;
   if (kRef) {
     T den = e2v - one;
@@ -213,9 +251,11 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
     i_CaL = g_CaL * d * f * f2 * fCass * C(4.0) * (V - C(15.0)) * F * F / (R * TK) *
             (C(0.25) * Cass * e2v - Cao) / den;
   } else {
-    T xe;  // x / (exp(x) - 1)
-    if (m_::ab(xcal) < C(1e-6)) xe = one - C(0.5) * xcal;
-    else xe = m_::dv(xcal, (T)expm1(xcal));
+    // x / (exp(x) - 1): three terms of its series for |x| < 0.01 (truncation
+    // < 2e-11), else the quotient (cancellation error < 1e-5 relative there)
+    T xe;
+    if (m_::ab(xcal) < C(0.01)) xe = one - C(0.5) * xcal + xcal * xcal * C(1.0 / 12.0);
+    else xe = xcal * m_::rcp(e2v - one);
     i_CaL = g_CaL * C(2.0 * 96485.3415) * d * f * f2 * fCass * (C(0.25) * Cass * e2v - Cao) * xe;
   }
 
@@ -224,12 +264,12 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T i_bCa = C(0.000592) * (V - E_Ca);
   const T i_pCa = m_::dv(C(0.1238) * Cai, Cai + C(0.0005));
   const T i_pK = m_::dv(C(0.0146) * vek, one + m_::ex((C(25.0) - V) * C(1.0 / 5.98)));
-  const T nak_den = one + C(0.1245) * m_::ex(C(-0.1) * V * FRT) + C(0.0353) * m_::ex(-V * FRT);
+  const T nak_den = one + C(0.1245) * m_::ex(C(-0.1) * V * FRT) + C(0.0353) * XP(-V * FRT, iE);
   T i_NaK;
   if (kRef) i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * m_::dv(Nai, Nai + C(40.0)) * m_::rcp(nak_den);
   else i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * Nai * m_::rcp((Nai + C(40.0)) * nak_den);
   const T e_g = m_::ex(C(0.35) * V * FRT);
-  const T e_gm1 = m_::ex(C(0.35 - 1.0) * V * FRT);
+  const T e_gm1 = XP(C(0.35 - 1.0) * V * FRT, e_g * iE);
   const T i_NaCa = m_::dv(C(1000.0) * (e_g * Nai * Nai * Nai * Cao - e_gm1 * C(140.0 * 140.0 * 140.0) * Cai * C(2.5)),
                           C((87.5 * 87.5 * 87.5 + 140.0 * 140.0 * 140.0) * (1.38 + 2.0)) *
                               (one + C(0.1) * e_gm1));
@@ -296,6 +336,7 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   s[S_R] = RLE(rg, r_inf, NDT(tau_r));
 #undef NDT
 #undef RLE
+#undef XP
   // R' forward Euler (:402-403)
   s[S_RPRIME] = Rp + dt * (-k2 * Cass * Rp + C(0.005) * (one - Rp));
   // concentrations, forward Euler with clamps (:406-410)

@@ -48,14 +48,6 @@ def coarsen_faces(Gx, Gy, Gz):
     return Cx, Cy, Cz
 
 
-def _block_sum(a):
-    """Sum over the 2x2x2 children of each coarse cell (odd extents zero-padded)."""
-    a = np.pad(a, [(0, n % 2) for n in a.shape])
-    return (a[0::2, 0::2, 0::2] + a[1::2, 0::2, 0::2] + a[0::2, 1::2, 0::2] + a[0::2, 0::2, 1::2]
-            + a[1::2, 1::2, 0::2] + a[1::2, 0::2, 1::2] + a[0::2, 1::2, 1::2]
-            + a[1::2, 1::2, 1::2])
-
-
 def dense_operator(Gx, Gy, Gz):
     """The (n x n) operator of a dense box (Neumann), FP64, for small boxes."""
     shape = Gx.shape
@@ -75,7 +67,12 @@ def dense_operator(Gx, Gy, Gz):
 class Multigrid:
     """V-cycle preconditioner of a TorsoGrid's operator, resident in HBM
     (hp_mg): level 0 is the column-run grid, coarser levels are dense boxes
-    down to at most max_coarse cells, solved there with a dense inverse."""
+    down to at most max_coarse cells, solved there with a dense inverse.  The
+    coarse cell holding the pinned cell is grounded on every level.  That suits
+    right-hand sides of zero sum (the heart source: its mean is removed); a
+    right-hand side with a net charge excites the near-constant mode, which the
+    widened coarse ground corrects poorly -- LeadField moves such a charge onto
+    the pin's neighbours first."""
 
     def __init__(self, grid, device="cuda", nu=2, omega=0.8, max_coarse=1024):
         dev = torch.device(device)
@@ -92,32 +89,16 @@ class Multigrid:
         S0 = _face_sum(*G).ravel()[flat]
         inv0 = np.where(S0 > 0, 1.0 / L["diag"].astype(np.float64), 0.0)
         inv0[pin_cell] = 0.0
-        # The pinned cell is no unknown: its faces become leaks to ground on its
-        # neighbours, which the coarse levels inherit (agglomerated like faces).
-        # Eliminating the whole coarse parent of the pin instead would ground a
-        # cell 2^l times wider on level l and starve the near-constant mode.
-        leak = np.zeros(shape)
-        pi, pj, pk = (int(v) for v in pin)
-        for ax, Gd in enumerate(G):
-            lo = list(pin)
-            lo[ax] -= 1
-            if lo[ax] >= 0:                      # face (pin-1 -> pin), stored at pin-1
-                leak[tuple(lo)] += Gd[tuple(lo)]
-                Gd[tuple(lo)] = 0.0
-            hi = list(pin)
-            hi[ax] += 1
-            if hi[ax] < shape[ax]:               # face (pin -> pin+1), stored at pin
-                leak[tuple(hi)] += Gd[pi, pj, pk]
-                Gd[pi, pj, pk] = 0.0
         self.keep = [torch.from_numpy(inv0.astype(np.float32)).to(dev),
                      torch.zeros(int(L["n_cells"]), dtype=torch.float32, device=dev)]
         levels, self.shapes, self.active = [], [], []
         while True:
             G = list(coarsen_faces(*G))
-            leak = 0.5 * _block_sum(leak)
-            S = _face_sum(*G) + leak
+            pin = pin // 2
+            S = _face_sum(*G)
             inv = np.where(S > 0, 1.0 / np.where(S > 0, S, 1.0), 0.0)
-            levels.append((G, inv, leak))
+            inv[tuple(pin)] = 0.0   # the cell holding the pinned cell is grounded
+            levels.append((G, inv))
             self.shapes.append(G[0].shape)
             self.active.append(int((inv > 0).sum()))
             if G[0].size <= max_coarse:
@@ -125,9 +106,9 @@ class Multigrid:
             if len(levels) == nat.MG_MAX_LEVELS:
                 raise ValueError("torso grid needs more multigrid levels than supported")
         # exact inverse on the coarsest level's active cells
-        Gc, invc, leakc = levels[-1]
-        self.coarse_faces, self.coarse_leak = Gc, leakc
-        A = dense_operator(*Gc) + np.diag(leakc.ravel())
+        Gc, invc = levels[-1]
+        self.coarse_faces = Gc
+        A = dense_operator(*Gc)
         act = np.nonzero(invc.ravel() > 0)[0]
         Ainv = np.zeros_like(A)
         if len(act):
@@ -137,7 +118,7 @@ class Multigrid:
         mg.n_levels, mg.nu, mg.omega = len(levels), int(nu), float(omega)
         mg.inv_diag0, mg.tmp0 = self.keep[0].data_ptr(), self.keep[1].data_ptr()
         mg.coarse_inv = self.coarse_inv.data_ptr()
-        for l, (Gl, inv, _) in enumerate(levels):
+        for l, (Gl, inv) in enumerate(levels):
             ni, nj, nk = Gl[0].shape
             n = ni * nj * nk
             ts = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32).ravel()).to(dev)
@@ -188,6 +169,35 @@ class PoissonSolver:
         self.r32, self.d32, self.p32, self.Ap32 = (torch.zeros(n, **f32) for _ in range(4))
         self.r64 = torch.zeros(n, dtype=torch.float64, device=dev)
         self.last_stats = {}
+        # the pinned cell's neighbours and the share of each in the conductance
+        # to the pin: A_p^-1 of this vector is constant (1/G) on the unpinned cells
+        pc = int(L["pin"])
+        flat = grid.flat_of_cell
+        pi, pj, pk = np.unravel_index(int(flat[pc]), grid["shape"])
+        shape = grid["shape"]
+        nbr, gw = [], []
+        col_k0, col_len, col_off = L["col_k0"], L["col_len"], L["col_off"]
+
+        def cell_at(i, j, k):
+            if not (0 <= i < shape[0] and 0 <= j < shape[1]):
+                return -1
+            c = i * shape[1] + j
+            dk = k - int(col_k0[c])
+            return int(col_off[c]) + dk if 0 <= dk < int(col_len[c]) else -1
+        for (i, j, k, g_of) in ((pi + 1, pj, pk, lambda n: L["gx"][pc]),
+                                (pi - 1, pj, pk, lambda n: L["gx"][n]),
+                                (pi, pj + 1, pk, lambda n: L["gy"][pc]),
+                                (pi, pj - 1, pk, lambda n: L["gy"][n]),
+                                (pi, pj, pk + 1, lambda n: L["gz"][pc]),
+                                (pi, pj, pk - 1, lambda n: L["gz"][n])):
+            n = cell_at(i, j, k)
+            if n >= 0 and float(g_of(n)) > 0:
+                nbr.append(n)
+                gw.append(float(g_of(n)))
+        gw = np.array(gw)
+        self.pin_G = float(gw.sum())
+        self.pin_nbr = torch.tensor(nbr, dtype=torch.int64, device=dev)
+        self.pin_w = torch.from_numpy(gw / max(self.pin_G, 1e-300)).to(dev)
         if precond not in ("jacobi", "mg"):
             raise ValueError(f"unknown preconditioner {precond!r}")
         self.precond = precond
@@ -374,6 +384,11 @@ class LeadField:
             rhs.index_add_(0, cells[e], torch.full((kk,), 1.0 / kk, dtype=torch.float64,
                                                    device=dev))
             rhs[src.pin] = 0.0
+            # move the electrode's net charge onto the pin's neighbours: the
+            # solution changes by a constant, which the source map cancels
+            # (it removes the mean over the heart voxels)
+            q = float(rhs.sum())
+            rhs.index_add_(0, ps.pin_nbr, -q * ps.pin_w)
             y.zero_()
             ps.solve_refined(rhs, y, tol=tol, max_outer=8, max_inner=2000)
             self.solve_stats.append(dict(ps.last_stats))

@@ -321,10 +321,8 @@ int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* A
  * (bspm_fn.py:164-263); it solves the same pinned system.  Level 0 is the
  * column-run grid above; level[l] (l = 0..n_levels-1) is the dense box of grid
  * level l+1, from 2x2x2 agglomeration of the level below, with face
- * conductance 1/2 of the sum of the four fine faces it covers; the faces of the
- * pinned cell become leaks to ground, kept in the coarse diagonals
- * (1/inv_diag = face sum + leak).  Cells with inv_diag = 0 (inactive) stay
- * zero.  One V-cycle:
+ * conductance 1/2 of the sum of the four fine faces it covers.  Cells with
+ * inv_diag = 0 (inactive, or holding the pinned cell) stay zero.  One V-cycle:
  * nu damped-Jacobi sweeps (nu even) before and after each coarse correction,
  * restriction = sum of child residuals, prolongation = parent value, exact
  * solve on the coarsest level (<= 1024 cells) with a dense inverse.
@@ -335,7 +333,7 @@ typedef struct hp_mg_level {
   const float* gx;        /* conductance of the +x face of each cell (0 = none) */
   const float* gy;
   const float* gz;
-  const float* inv_diag;  /* 1/(sum of the six faces + leak to the pinned cell); 0 = inactive */
+  const float* inv_diag;  /* 1/(sum of the six faces); 0 = inactive or holding the pinned cell */
   float* x;               /* work vectors, n each */
   float* tmp;
   float* b;

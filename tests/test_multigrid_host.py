@@ -55,37 +55,15 @@ def test_hierarchy_of_a_torso():
     assert inv.shape == (n, n)
     act = np.nonzero(np.abs(inv).sum(axis=1) > 0)[0]
     assert len(act) == mg.active[-1]
-    A = dense_operator(*mg.coarse_faces) + np.diag(mg.coarse_leak.ravel())
+    A = dense_operator(*mg.coarse_faces)
     prod = inv[np.ix_(act, act)] @ A[np.ix_(act, act)]
     assert np.abs(prod - np.eye(len(act))).max() < 1e-4
-    # the pinned cell's faces reach the coarsest level as a leak to ground: the
-    # coarse operator is nonsingular; its total equals the pin's face sum
-    # halved once per coarsening
-    pin_faces = float(g.layout["diag"][g.layout["pin"]])  # 1.0: identity row
-    assert mg.coarse_leak.sum() > 0 and pin_faces == 1.0
+    # the grounded coarse cell (parent of the pinned cell) is excluded, which
+    # makes the operator on the remaining cells nonsingular
+    assert mg.active[-1] < int((_face_sum(*mg.coarse_faces) > 0).sum())
     assert np.linalg.matrix_rank(A[np.ix_(act, act)]) == len(act)
     # level-0 inverse diagonal: zero exactly on the pinned cell and the holes
     inv0 = mg.keep[0].numpy()
     assert inv0[g.layout["pin"]] == 0.0
     holes = ~g["active_mask"].ravel()[g.flat_of_cell]
     assert np.all(inv0[holes] == 0.0)
-
-
-def test_pin_leak_total_per_level():
-    """The faces of the pinned cell leave the face arrays and enter the leak
-    with the same total; each coarsening halves it like a face."""
-    lab, smap = _torso(n=(30, 26, 22))
-    g = TorsoGrid.from_labels(lab, smap, 1.5)
-    L = g.layout
-    shape = g["shape"]
-    pin_flat = int(g.flat_of_cell[L["pin"]])
-    pi, pj, pk = np.unravel_index(pin_flat, shape)
-    gx = np.zeros(int(np.prod(shape)))
-    gy, gz = gx.copy(), gx.copy()
-    gx[g.flat_of_cell], gy[g.flat_of_cell], gz[g.flat_of_cell] = L["gx"], L["gy"], L["gz"]
-    gx, gy, gz = (a.reshape(shape) for a in (gx, gy, gz))
-    total = (gx[pi, pj, pk] + gx[pi - 1, pj, pk] + gy[pi, pj, pk] + gy[pi, pj - 1, pk]
-             + gz[pi, pj, pk] + gz[pi, pj, pk - 1])
-    mg = Multigrid(g, "cpu")
-    n_levels = len(mg.shapes)
-    assert np.isclose(mg.coarse_leak.sum(), total * 0.5 ** n_levels, rtol=1e-6)
