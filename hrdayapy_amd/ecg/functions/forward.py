@@ -190,10 +190,10 @@ class PoissonSolver:
                                 (pi, pj - 1, pk, lambda n: L["gy"][n]),
                                 (pi, pj, pk + 1, lambda n: L["gz"][pc]),
                                 (pi, pj, pk - 1, lambda n: L["gz"][n])):
-            n = cell_at(i, j, k)
-            if n >= 0 and float(g_of(n)) > 0:
-                nbr.append(n)
-                gw.append(float(g_of(n)))
+            cn = cell_at(i, j, k)
+            if cn >= 0 and float(g_of(cn)) > 0:
+                nbr.append(cn)
+                gw.append(float(g_of(cn)))
         gw = np.array(gw)
         self.pin_G = float(gw.sum())
         self.pin_nbr = torch.tensor(nbr, dtype=torch.int64, device=dev)

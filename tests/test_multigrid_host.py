@@ -67,3 +67,20 @@ def test_hierarchy_of_a_torso():
     assert inv0[g.layout["pin"]] == 0.0
     holes = ~g["active_mask"].ravel()[g.flat_of_cell]
     assert np.all(inv0[holes] == 0.0)
+
+
+def test_solver_setup_on_host_memory():
+    """PoissonSolver with the multigrid builds on host tensors (no kernel is
+    launched at set-up): vectors sized to the grid, and the pin's neighbours
+    carry weights that sum to one over the conductance to the pin."""
+    import torch
+    from hrdayapy_amd.ecg.functions.forward import PoissonSolver
+    lab, smap = _torso(n=(30, 26, 22))
+    g = TorsoGrid.from_labels(lab, smap, 1.5)
+    ps = PoissonSolver(g, "cpu", precond="mg")
+    assert ps.z32.numel() == ps.n == g.layout["n_cells"]
+    assert abs(float(ps.pin_w.sum()) - 1.0) < 1e-12
+    L = g.layout
+    faces = L["gx"][ps.pin_nbr.numpy()]  # neighbours exist and conduct
+    assert len(ps.pin_nbr) >= 1 and ps.pin_G > 0 and np.all(np.isfinite(faces))
+    assert int(ps.torso.n_col_list) == int((L["col_len"] > 0).sum())
