@@ -126,6 +126,10 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
     at = eng.at_nodes().double().cpu().numpy()
     # per-kernel breakdown (HIP events on the launching stream; advances the run)
     out["reaction_us"] = _event_us(run.launch_reaction, 20)
+    if form == "mlp":  # the other inference kernel shape, for comparison
+        run.sp.variant = 1
+        out["reaction_us_32_per_wave"] = _event_us(run.launch_reaction, 20)
+        run.sp.variant = 0
     out["substep_us"] = _event_us(run.launch_substep, 20)
     if ref_at is not None:
         both = np.isfinite(at) & np.isfinite(ref_at)

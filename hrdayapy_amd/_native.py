@@ -39,7 +39,7 @@ class HpPmj(C.Structure):
     _fields_ = [("n_junctions", c_i64), ("n_entries", c_i64), ("pk", c_vp), ("ptr", c_vp),
                 ("patch", c_vp), ("n_targets", c_i64), ("tgt_slot", c_vp), ("tgt_ptr", c_vp),
                 ("tgt_entry", c_vp), ("c_pmj", c_f64), ("ai_p", c_f64), ("ai_m", c_f64),
-                ("cm", c_f64)]
+                ("cm", c_f64), ("coef", c_vp)]
 
 
 class HpPmjAt(C.Structure):
@@ -59,7 +59,7 @@ class HpMlpNorm(C.Structure):
 
 class HpSurrogateParams(C.Structure):
     _fields_ = [("t_macro", c_f32), ("dvdt_max", c_f32), ("v_lo", c_f32), ("v_hi", c_f32),
-                ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("pad_", c_i32),
+                ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("variant", c_i32),
                 ("fb_list", c_vp), ("fb_count", c_vp)]
 
 

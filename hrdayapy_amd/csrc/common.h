@@ -28,6 +28,7 @@ template <typename T> struct M;
 template <> struct M<float> {
 #if defined(__HIP_DEVICE_COMPILE__)
   static HP_HD float ex(float x) { return __expf(x); }
+  static HP_HD float ex2(float x) { return __builtin_amdgcn_exp2f(x); }
   static HP_HD float lg(float x) { return __logf(x); }
   static HP_HD float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   static HP_HD float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -35,6 +36,7 @@ template <> struct M<float> {
   static HP_HD float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 #else
   static HP_HD float ex(float x) { return expf(x); }
+  static HP_HD float ex2(float x) { return exp2f(x); }
   static HP_HD float lg(float x) { return logf(x); }
   static HP_HD float rcp(float x) { return 1.0f / x; }
   static HP_HD float dv(float a, float b) { return a / b; }
@@ -48,6 +50,7 @@ template <> struct M<float> {
 
 template <> struct M<double> {
   static HP_HD double ex(double x) { return exp(x); }
+  static HP_HD double ex2(double x) { return exp2(x); }
   static HP_HD double lg(double x) { return log(x); }
   static HP_HD double rcp(double x) { return 1.0 / x; }
   static HP_HD double dv(double a, double b) { return a / b; }

@@ -228,7 +228,6 @@ __device__ __forceinline__ void silu_pack(const floatx4& c, const float* bias, i
   }
 }
 
-constexpr int SUR_THREADS = 512, SUR_WAVES = SUR_THREADS / 64;
 
 // Append the slots of the lanes with fb set to the fallback list: one atomic
 // per wave.  Called by every lane of the wave.
@@ -244,12 +243,18 @@ __device__ __forceinline__ void append_fallback(bool fb, int64_t slot, int32_t* 
   if (fb) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)slot;
 }
 
-__global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
+// NC MFMA columns of 16 voxels per wave, THREADS per workgroup.  NC = 1 with
+// 768 threads fits 170 registers (three waves per SIMD, the weights in LDS
+// shared by 12 waves); NC = 2 with 512 threads reuses each A fragment for two
+// columns at two waves per SIMD.
+template <int NC, int THREADS>
+__global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
                                                         float* __restrict__ V_prev,
                                                         float* __restrict__ state, int64_t ld,
                                                         float* __restrict__ rate,
                                                         hp_mlp_weights wts, hp_mlp_norm nrm,
                                                         hp_surrogate_params sp) {
+  constexpr int WAVES = THREADS / 64, TV = 16 * NC;  // voxels per wave tile
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   MlpSmem& S = *reinterpret_cast<MlpSmem*>(smem_raw);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -258,22 +263,22 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
   const __bf16* __restrict__ W3 = (const __bf16*)wts.w3;
   const __bf16* __restrict__ W4 = (const __bf16*)wts.w4;
   // stage weights (natural order in global) and the normalisation into LDS
-  for (int e = tid; e < NHID * 32; e += SUR_THREADS) {
+  for (int e = tid; e < NHID * 32; e += THREADS) {
     const int o = e / 32, i = e % 32;
     S.w1[o][i] = i < NIN ? W1[o * NIN + i] : f2bf(0.f);
   }
-  for (int e = tid; e < NHID * NHID; e += SUR_THREADS) {
+  for (int e = tid; e < NHID * NHID; e += THREADS) {
     const int o = e / NHID, p = e % NHID;
     const int kin = (p & ~31) + perm_k(p & 31);
     S.w2[o][p] = W2[o * NHID + kin];
     S.w3[o][p] = W3[o * NHID + kin];
   }
-  for (int e = tid; e < 32 * NHID; e += SUR_THREADS) {
+  for (int e = tid; e < 32 * NHID; e += THREADS) {
     const int o = e / NHID, p = e % NHID;
     const int kin = (p & ~31) + perm_k(p & 31);
     S.w4[o][p] = o < NOUT ? W4[o * NHID + kin] : f2bf(0.f);
   }
-  for (int e = tid; e < NHID; e += SUR_THREADS) {
+  for (int e = tid; e < NHID; e += THREADS) {
     S.b1[e] = wts.b1[e];
     S.b2[e] = wts.b2[e];
     S.b3[e] = wts.b3[e];
@@ -291,26 +296,31 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
   const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
   const int col = lane & 15, grp = lane >> 4;
   const float dtm = sp.t_macro;
-  // persistent loop: one 32-voxel tile (two 16-voxel MFMA columns) per wave
-  for (int64_t tile = (int64_t)blockIdx.x * SUR_WAVES + wave; tile * 32 < n_slots;
-       tile += (int64_t)gridDim.x * SUR_WAVES) {
-    const int64_t base = tile * 32;
-    const int64_t slot0 = base + col, slot1 = base + 16 + col;
-    const uint8_t lab0 = g.label[slot0], lab1 = g.label[slot1];
-    // skip tiles without myocardium (wave-uniform)
-    if (!__any((lab0 != 0xFF) || (lab1 != 0xFF))) continue;
-    // ---- inputs: B fragments (32 features x 16 voxels) of the two columns
-    bf16x8 xin[2];
+  // persistent loop: one TV-voxel tile (NC 16-voxel MFMA columns) per wave
+  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile * TV < n_slots;
+       tile += (int64_t)gridDim.x * WAVES) {
+    const int64_t base = tile * TV;
+    int64_t slot[NC];
+    uint8_t lab[NC];
+    bool any = false;
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int64_t slot = t ? slot1 : slot0;
-      const uint8_t lab = t ? lab1 : lab0;
+    for (int t = 0; t < NC; ++t) {
+      slot[t] = base + 16 * t + col;
+      lab[t] = g.label[slot[t]];
+      any |= lab[t] != 0xFF;
+    }
+    // skip tiles without myocardium (wave-uniform)
+    if (!__any(any)) continue;
+    // ---- inputs: B fragments (32 features x 16 voxels) of the columns
+    bf16x8 xin[NC];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int f = grp * 8 + q;
         float x = 0.f;
-        if (lab != 0xFF && f < NIN) {
-          x = raw_feature(f, V, state, ld, slot, lab & 3, sp.ki_base, sp.nai_base);
+        if (lab[t] != 0xFF && f < NIN) {
+          x = raw_feature(f, V, state, ld, slot[t], lab[t] & 3, sp.ki_base, sp.nai_base);
           if (S.in_log[f]) x = __logf(fmaxf(x, 1e-30f));
           x = (x - S.in_mean[f]) * S.in_rstd[f];
         }
@@ -318,60 +328,63 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
       }
     }
     // ---- layer 1 (one K step) -> hidden activations in B-fragment form
-    bf16x8 ha[4][2], hb[4][2];
+    bf16x8 ha[4][NC], hb[4][NC];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const bf16x8 a = *(const bf16x8*)&S.w1[m * 16 + col][grp * 8];
       const floatx4 z = {0.f, 0.f, 0.f, 0.f};
-      silu_pack(mfma16(a, xin[0], z), S.b1, m, grp, ha[m >> 1][0]);
-      silu_pack(mfma16(a, xin[1], z), S.b1, m, grp, ha[m >> 1][1]);
+#pragma unroll
+      for (int t = 0; t < NC; ++t) silu_pack(mfma16(a, xin[t], z), S.b1, m, grp, ha[m >> 1][t]);
     }
     // ---- layers 2 and 3
 #pragma unroll
     for (int layer = 0; layer < 2; ++layer) {
       const __bf16(*W)[NHID] = layer == 0 ? S.w2 : S.w3;
       const float* bias = layer == 0 ? S.b2 : S.b3;
-      bf16x8(&hin)[4][2] = layer == 0 ? ha : hb;
-      bf16x8(&hout)[4][2] = layer == 0 ? hb : ha;
+      bf16x8(&hin)[4][NC] = layer == 0 ? ha : hb;
+      bf16x8(&hout)[4][NC] = layer == 0 ? hb : ha;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+        floatx4 c[NC];
+#pragma unroll
+        for (int t = 0; t < NC; ++t) c[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const bf16x8 a = *(const bf16x8*)&W[m * 16 + col][j * 32 + grp * 8];
-          c0 = mfma16(a, hin[j][0], c0);
-          c1 = mfma16(a, hin[j][1], c1);
+#pragma unroll
+          for (int t = 0; t < NC; ++t) c[t] = mfma16(a, hin[j][t], c[t]);
         }
-        silu_pack(c0, bias, m, grp, hout[m >> 1][0]);
-        silu_pack(c1, bias, m, grp, hout[m >> 1][1]);
+#pragma unroll
+        for (int t = 0; t < NC; ++t) silu_pack(c[t], bias, m, grp, hout[m >> 1][t]);
       }
     }
     // ---- output layer (19 of 32 rows used); input is `ha` after two swaps
-    floatx4 out[2][2];
+    floatx4 out[2][NC];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      floatx4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+      floatx4 c[NC];
+#pragma unroll
+      for (int t = 0; t < NC; ++t) c[t] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bf16x8 a = *(const bf16x8*)&S.w4[m * 16 + col][j * 32 + grp * 8];
-        c0 = mfma16(a, ha[j][0], c0);
-        c1 = mfma16(a, ha[j][1], c1);
+#pragma unroll
+        for (int t = 0; t < NC; ++t) c[t] = mfma16(a, ha[j][t], c[t]);
       }
-      out[m][0] = c0;
-      out[m][1] = c1;
+#pragma unroll
+      for (int t = 0; t < NC; ++t) out[m][t] = c[t];
     }
     // ---- mode decision and update, per voxel (this lane's 4+4 outputs)
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int64_t slot = t ? slot1 : slot0;
-      const uint8_t lab = t ? lab1 : lab0;
-      const float v = V[slot], vp = V_prev[slot];
+    for (int t = 0; t < NC; ++t) {
+      const int64_t sl = slot[t];
+      const float v = V[sl], vp = V_prev[sl];
       const float dvdt = (v - vp) / dtm;
       const bool fallback = sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
                             (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
-      if (sp.fb_list) append_fallback(lab != 0xFF && grp == 0 && fallback, slot, sp.fb_list,
+      if (sp.fb_list) append_fallback(lab[t] != 0xFF && grp == 0 && fallback, sl, sp.fb_list,
                                       sp.fb_count);
-      if (lab == 0xFF) continue;
+      if (lab[t] == 0xFF) continue;
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
@@ -380,9 +393,9 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
           if (o >= NOUT) continue;
           const float d = (out[m][t][q] + S.b4[o]) * S.out_std[o] + S.out_mean[o];
           if (o == 0) {
-            rate[slot] = fallback ? __builtin_nanf("") : d / dtm;
+            rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
           } else if (!fallback) {
-            float* p = state + (int64_t)(o - 1) * ld + slot;
+            float* p = state + (int64_t)(o - 1) * ld + sl;
             const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
             const float x0 = *p + base_v;
             const float x1 = S.in_log[o] ? __expf(__logf(fmaxf(x0, 1e-30f)) + d) : x0 + d;
@@ -393,8 +406,9 @@ __global__ void __launch_bounds__(SUR_THREADS) surrogate_kernel(hp_bricks g, con
     // V_prev of this tile was consumed above (the mode decision depends on it),
     // so the loads have completed before these stores are issued
     if (grp == 0) {
-      if (lab0 != 0xFF) V_prev[slot0] = V[slot0];
-      if (lab1 != 0xFF) V_prev[slot1] = V[slot1];
+#pragma unroll
+      for (int t = 0; t < NC; ++t)
+        if (lab[t] != 0xFF) V_prev[slot[t]] = V[slot[t]];
     }
   }
 }
@@ -519,15 +533,22 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
   const size_t smem = sizeof(MlpSmem);
   static bool attr_set = false;
   if (!attr_set) {
-    HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel,
+    HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel<1, 768>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel<2, 512>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     attr_set = true;
   }
   if (n_blocks <= 0) n_blocks = 256;
-  const int64_t tiles = n_slots / 32;
-  if ((int64_t)n_blocks * SUR_WAVES > tiles) n_blocks = (int)((tiles + SUR_WAVES - 1) / SUR_WAVES);
-  hipLaunchKernelGGL(surrogate_kernel, dim3(n_blocks), dim3(SUR_THREADS), smem, (hipStream_t)stream, *g, V,
-                     V_prev, state, ld, rate, *w, *n, *sp);
+  const int nc = sp->variant == 1 ? 2 : 1, waves = nc == 2 ? 8 : 12;
+  const int64_t tiles = n_slots / (16 * nc);
+  if ((int64_t)n_blocks * waves > tiles) n_blocks = (int)((tiles + waves - 1) / waves);
+  if (nc == 1)
+    hipLaunchKernelGGL((surrogate_kernel<1, 768>), dim3(n_blocks), dim3(768), smem,
+                       (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *w, *n, *sp);
+  else
+    hipLaunchKernelGGL((surrogate_kernel<2, 512>), dim3(n_blocks), dim3(512), smem,
+                       (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *w, *n, *sp);
   return check_launch("hp_surrogate_step");
 }
 

@@ -431,11 +431,21 @@ __device__ __forceinline__ void pmj_junction(const hp_pmj& j, int64_t q, T* __re
   const double Vp = (double)V_pk[p];
   double mean = 0.0;
   for (int e = e0; e < e1; ++e) mean += (double)V_myo[j.patch[e]];
-  mean /= n;
-  const double Cp = j.ai_p * j.cm, Cq = n * j.ai_m * j.cm;
-  const double vbar = (Cp * Vp + Cq * mean) / (Cp + Cq);
-  const double r1 = exp(-j.c_pmj * dt * (1.0 / Cp + 1.0 / Cq));
-  const double r2 = exp(-(j.c_pmj / n) * dt / (j.ai_m * j.cm));
+  double wp, r1, r2;  // Cp/(Cp+Cq), relaxation factors of the mean and the deviations
+  if (j.coef) {
+    const double* k = j.coef + 4 * q;
+    wp = k[0];
+    r1 = k[1];
+    r2 = k[2];
+    mean *= k[3];
+  } else {
+    mean /= n;
+    const double Cp = j.ai_p * j.cm, Cq = n * j.ai_m * j.cm;
+    wp = Cp / (Cp + Cq);
+    r1 = exp(-j.c_pmj * dt * (1.0 / Cp + 1.0 / Cq));
+    r2 = exp(-(j.c_pmj / n) * dt / (j.ai_m * j.cm));
+  }
+  const double vbar = wp * Vp + (1.0 - wp) * mean;
   const double vp_new = vbar + (Vp - vbar) * r1;
   const double mean_new = vbar + (mean - vbar) * r1;
   for (int e = e0; e < e1; ++e) {

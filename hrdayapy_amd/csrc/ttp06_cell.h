@@ -69,26 +69,35 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T fCass = s[S_FCASS], sg = s[S_S], rg = s[S_R], CaSR = s[S_CASR], Rp = s[S_RPRIME];
   const T Cass = s[S_CASS];
   const T one = C(1.0);
-  // Rush-Larsen with the exponent supplied: x_inf - (x_inf - x) exp(-dt/tau)
-#define RLE(x, xi, ndt_over_tau) ((xi) - ((xi) - (x)) * m_::ex(ndt_over_tau))
+  // The FP32 form evaluates every exponential as exp2 of an argument already
+  // scaled by log2(e): exp(a*V + b) = exp2(V*(a log2 e) + b log2 e), one FMA
+  // and one v_exp_f32.  L2E(c) is the constant c*log2(e).
+#define L2E(c) C((c) * 1.4426950408889634)
+  // Rush-Larsen with the exponent supplied: x_inf - (x_inf - x) exp(-dt/tau);
+  // the exponent is -dt/tau (reference form) or -dt log2(e)/tau (FP32 form)
+#define RLE(x, xi, ndt_over_tau) \
+  ((xi) - ((xi) - (x)) * (kRef ? m_::ex(ndt_over_tau) : m_::ex2(ndt_over_tau)))
+  const T ndt0 = kRef ? -dt : -dt * L2E(1.0);
   // exp(ref) in the reference arrangement, `fast` (a shared exponential times
   // a constant) in the FP32 form
 #define XP(ref, fast) (kRef ? m_::ex(ref) : (fast))
+  // exp(ref) = exp(a*u + b) in the reference arrangement / FP32 form
+#define LX(ref, u, a, b) (kRef ? m_::ex(ref) : m_::ex2((u) * L2E(a) + L2E(b)))
   // shared exponentials of the FP32 form (V is clamped to [-150, 80], so none
   // of these leaves the FP32 range)
   T Q = one, iQ = one, P = one, iP = one, R5 = one, iR5 = one, S7 = one, iS7 = one, W6 = one,
     E = one, iE = one;
   if (!kRef) {
-    Q = m_::ex(C(0.1) * V);          // e^{0.1 V}
+    Q = m_::ex2(V * L2E(0.1));       // e^{0.1 V}
     iQ = m_::rcp(Q);
     P = Q * Q;                       // e^{0.2 V}
     iP = iQ * iQ;
-    R5 = m_::ex(C(0.05) * V);        // e^{0.05 V}
+    R5 = m_::ex2(V * L2E(0.05));     // e^{0.05 V}
     iR5 = m_::rcp(R5);
-    S7 = m_::ex(V * C(1.0 / 7.0));   // e^{V/7}
+    S7 = m_::ex2(V * L2E(1.0 / 7.0));  // e^{V/7}
     iS7 = m_::rcp(S7);
-    W6 = m_::ex(V * C(-1.0 / 6.0));  // e^{-V/6}
-    E = m_::ex(V * FRT);             // e^{V F/RT}
+    W6 = m_::ex2(V * L2E(-1.0 / 6.0));  // e^{-V/6}
+    E = m_::ex2(V * L2E(96485.3415 / (8314.472 * 310.0)));  // e^{V F/RT}
     iE = m_::rcp(E);
   }
 
@@ -109,43 +118,46 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   // I_Na (:250-276)
   const T e_m1 = XP((C(-60.0) - V) * C(0.2), C(6.14421235332821e-06) * iP);   // e^-12
   const T e_m2 = XP((V + C(35.0)) * C(0.2), C(1096.6331584284585) * P);       // e^7
-  const T e_m3 = m_::ex((V - C(50.0)) * C(1.0 / 200.0));
-  T t1 = m_::rcp(one + m_::ex((C(-56.86) - V) * C(1.0 / 9.03)));
+  const T e_m3 = LX((V - C(50.0)) * C(1.0 / 200.0), V, 1.0 / 200.0, -50.0 / 200.0);
+  T t1 = m_::rcp(one + LX((C(-56.86) - V) * C(1.0 / 9.03), V, -1.0 / 9.03, -56.86 / 9.03));
   const T m_inf = t1 * t1;
   T ndt_m;  // -dt / tau_m, tau_m = a_m * b_m
   if (kRef) {
     const T a_m = m_::rcp(one + e_m1);
     const T b_m = C(0.1) * m_::rcp(one + e_m2) + C(0.1) * m_::rcp(one + e_m3);
-    ndt_m = -dt * m_::rcp(a_m * b_m);
+    ndt_m = ndt0 * m_::rcp(a_m * b_m);
   } else {
     // 1/tau_m = (1+e1)(1+e2)(1+e3) / (0.1 (2 + e2 + e3))
-    ndt_m = -dt * (one + e_m1) * (one + e_m2) * (one + e_m3) *
+    ndt_m = ndt0 * (one + e_m1) * (one + e_m2) * (one + e_m3) *
             m_::rcp(C(0.1) * (C(2.0) + e_m2 + e_m3));
   }
-  t1 = m_::rcp(one + m_::ex((V + C(71.55)) * C(1.0 / 7.43)));
+  t1 = m_::rcp(one + LX((V + C(71.55)) * C(1.0 / 7.43), V, 1.0 / 7.43, 71.55 / 7.43));
   const T h_inf = t1 * t1;
   T a_h, b_h, a_j, b_j;
   if (V < C(-40.0)) {
-    a_h = C(0.057) * m_::ex(-(V + C(80.0)) * C(1.0 / 6.8));
-    b_h = C(2.7) * m_::ex(C(0.079) * V) + C(3.1e5) * m_::ex(C(0.3485) * V);
-    a_j = m_::dv((C(-25428.0) * m_::ex(C(0.2444) * V) - C(6.948e-6) * m_::ex(C(-0.04391) * V)) *
+    a_h = C(0.057) * LX(-(V + C(80.0)) * C(1.0 / 6.8), V, -1.0 / 6.8, -80.0 / 6.8);
+    b_h = C(2.7) * LX(C(0.079) * V, V, 0.079, 0.0) + C(3.1e5) * LX(C(0.3485) * V, V, 0.3485, 0.0);
+    a_j = m_::dv((C(-25428.0) * LX(C(0.2444) * V, V, 0.2444, 0.0) -
+                  C(6.948e-6) * LX(C(-0.04391) * V, V, -0.04391, 0.0)) *
                      (V + C(37.78)),
-                 one + m_::ex(C(0.311) * (V + C(79.23))));
-    b_j = m_::dv(C(0.02424) * m_::ex(C(-0.01052) * V), one + m_::ex(C(-0.1378) * (V + C(40.14))));
+                 one + LX(C(0.311) * (V + C(79.23)), V, 0.311, 0.311 * 79.23));
+    b_j = m_::dv(C(0.02424) * LX(C(-0.01052) * V, V, -0.01052, 0.0),
+                 one + LX(C(-0.1378) * (V + C(40.14)), V, -0.1378, -0.1378 * 40.14));
   } else {
     a_h = C(0.0);
-    b_h = m_::dv(C(0.77), C(0.13) * (one + m_::ex((V + C(10.66)) * C(-1.0 / 11.1))));
+    b_h = m_::dv(C(0.77), C(0.13) * (one + LX((V + C(10.66)) * C(-1.0 / 11.1), V, -1.0 / 11.1,
+                                              -10.66 / 11.1)));
     a_j = C(0.0);
-    b_j = m_::dv(C(0.6) * m_::ex(C(0.057) * V),
+    b_j = m_::dv(C(0.6) * LX(C(0.057) * V, V, 0.057, 0.0),
                  one + XP(C(-0.1) * (V + C(32.0)), C(0.04076220397836621) * iQ));  // e^-3.2
   }
   T ndt_h, ndt_j;
   if (kRef) {
-    ndt_h = -dt * m_::rcp(m_::rcp(a_h + b_h + C(1e-30)));
-    ndt_j = -dt * m_::rcp(m_::rcp(a_j + b_j + C(1e-30)));
+    ndt_h = ndt0 * m_::rcp(m_::rcp(a_h + b_h + C(1e-30)));
+    ndt_j = ndt0 * m_::rcp(m_::rcp(a_j + b_j + C(1e-30)));
   } else {
-    ndt_h = -dt * (a_h + b_h + C(1e-30));
-    ndt_j = -dt * (a_j + b_j + C(1e-30));
+    ndt_h = ndt0 * (a_h + b_h + C(1e-30));
+    ndt_j = ndt0 * (a_j + b_j + C(1e-30));
   }
   const T i_Na = C(14.838) * mg * mg * mg * hg * jg * (V - E_Na);
 
@@ -153,37 +165,38 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T xr1_inf = m_::rcp(one + XP((C(-26.0) - V) * C(1.0 / 7.0),
                                      C(0.0243728440732796) * iS7));  // e^(-26/7)
   const T e_r1 = XP((C(-45.0) - V) * C(0.1), C(0.011108996538242306) * iQ);  // e^-4.5
-  const T e_r2 = m_::ex((V + C(30.0)) * C(1.0 / 11.5));
-  const T xr2_inf = m_::rcp(one + m_::ex((V + C(88.0)) * C(1.0 / 24.0)));
+  const T e_r2 = LX((V + C(30.0)) * C(1.0 / 11.5), V, 1.0 / 11.5, 30.0 / 11.5);
+  const T xr2_inf = m_::rcp(one + LX((V + C(88.0)) * C(1.0 / 24.0), V, 1.0 / 24.0, 88.0 / 24.0));
   const T e_r3 = XP((C(-60.0) - V) * C(0.05), C(0.049787068367863944) * iR5);  // e^-3
   const T e_r4 = XP((V - C(60.0)) * C(0.05), C(0.049787068367863944) * R5);
   T ndt_xr1, ndt_xr2;
   if (kRef) {
     const T tau_xr1 = C(450.0) * m_::rcp(one + e_r1) * C(6.0) * m_::rcp(one + e_r2);
     const T tau_xr2 = C(3.0) * m_::rcp(one + e_r3) * C(1.12) * m_::rcp(one + e_r4);
-    ndt_xr1 = -dt * m_::rcp(tau_xr1);
-    ndt_xr2 = -dt * m_::rcp(tau_xr2);
+    ndt_xr1 = ndt0 * m_::rcp(tau_xr1);
+    ndt_xr2 = ndt0 * m_::rcp(tau_xr2);
   } else {
-    ndt_xr1 = -dt * (one + e_r1) * (one + e_r2) * C(1.0 / 2700.0);
-    ndt_xr2 = -dt * (one + e_r3) * (one + e_r4) * C(1.0 / 3.36);
+    ndt_xr1 = ndt0 * (one + e_r1) * (one + e_r2) * C(1.0 / 2700.0);
+    ndt_xr2 = ndt0 * (one + e_r3) * (one + e_r4) * C(1.0 / 3.36);
   }
   const T vek = V - E_K;
   const T i_Kr = g_Kr * Xr1 * Xr2 * vek;
 
   // I_Ks (:290-294)
-  const T xs_inf = m_::rcp(one + m_::ex((C(-5.0) - V) * C(1.0 / 14.0)));
+  const T xs_inf = m_::rcp(one + LX((C(-5.0) - V) * C(1.0 / 14.0), V, -1.0 / 14.0, -5.0 / 14.0));
   const T e_xs = XP((C(5.0) - V) * C(1.0 / 6.0), C(2.300975890892825) * W6);  // e^(5/6)
   T a_xs;
   if (kRef) a_xs = m_::dv(C(1400.0), m_::sq(one + e_xs));
   else a_xs = C(1400.0) * m_::rsq(one + e_xs);
-  const T b_xs = m_::rcp(one + m_::ex((V - C(35.0)) * C(1.0 / 15.0)));
+  const T b_xs = m_::rcp(one + LX((V - C(35.0)) * C(1.0 / 15.0), V, 1.0 / 15.0, -35.0 / 15.0));
   const T tau_xs = a_xs * b_xs + C(80.0);
   const T i_Ks = g_Ks * Xs * Xs * (V - E_Ks);
 
   // I_K1 (:297-302)
-  const T e_k1 = m_::ex(C(0.06) * (vek - C(200.0)));
-  const T nb_k1 = C(3.0) * m_::ex(C(2e-4) * (vek + C(100.0))) + m_::ex(C(0.1) * (vek - C(10.0)));
-  const T e_k2 = m_::ex(C(-0.5) * vek);
+  const T e_k1 = LX(C(0.06) * (vek - C(200.0)), vek, 0.06, -12.0);
+  const T nb_k1 = C(3.0) * LX(C(2e-4) * (vek + C(100.0)), vek, 2e-4, 0.02) +
+                  LX(C(0.1) * (vek - C(10.0)), vek, 0.1, -1.0);
+  const T e_k2 = LX(C(-0.5) * vek, vek, -0.5, 0.0);
   T xk1_inf;
   if (kRef) {
     const T a_k1 = C(0.1) * m_::rcp(one + e_k1);
@@ -201,30 +214,32 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   if (endo_s) {
     s_inf = m_::rcp(one + XP((V + C(28.0)) * C(0.2), C(270.42640742615254) * P));  // e^5.6
     const T u = V + C(67.0);
-    tau_s = C(1000.0) * m_::ex(-u * u * C(1.0 / 1000.0)) + C(8.0);
+    tau_s = C(1000.0) * LX(-u * u * C(1.0 / 1000.0), u * u, -1.0 / 1000.0, 0.0) + C(8.0);
   } else {
     s_inf = m_::rcp(one + XP((V + C(20.0)) * C(0.2), C(54.598150033144236) * P));  // e^4
     const T u = V + C(45.0);
-    tau_s = C(85.0) * m_::ex(-u * u * C(1.0 / 320.0)) +
+    tau_s = C(85.0) * LX(-u * u * C(1.0 / 320.0), u * u, -1.0 / 320.0, 0.0) +
             C(5.0) * m_::rcp(one + XP((V - C(20.0)) * C(0.2), C(0.01831563888873418) * P)) +
             C(3.0);  // e^-4
   }
   const T r_inf = m_::rcp(one + XP((C(20.0) - V) * C(1.0 / 6.0),
                                   C(28.03162489452614) * W6));  // e^(20/6)
   const T u40 = V + C(40.0);
-  const T tau_r = C(9.5) * m_::ex(-u40 * u40 * C(1.0 / 1800.0)) + C(0.8);
+  const T tau_r = C(9.5) * LX(-u40 * u40 * C(1.0 / 1800.0), u40 * u40, -1.0 / 1800.0, 0.0) + C(0.8);
   const T i_to = g_to * rg * sg * vek;
 
   // I_CaL (:320-340)
-  const T d_inf = m_::rcp(one + m_::ex((C(-8.0) - V) * C(1.0 / 7.5)));
-  const T tau_d = (C(1.4) * m_::rcp(one + m_::ex((C(-35.0) - V) * C(1.0 / 13.0))) + C(0.25)) * C(1.4) *
+  const T d_inf = m_::rcp(one + LX((C(-8.0) - V) * C(1.0 / 7.5), V, -1.0 / 7.5, -8.0 / 7.5));
+  const T tau_d = (C(1.4) * m_::rcp(one + LX((C(-35.0) - V) * C(1.0 / 13.0), V, -1.0 / 13.0,
+                                             -35.0 / 13.0)) +
+                   C(0.25)) * C(1.4) *
                       m_::rcp(one + XP((V + C(5.0)) * C(0.2), C(2.718281828459045) * P)) +
                   m_::rcp(one + XP((C(50.0) - V) * C(0.05), C(12.182493960703473) * iR5));
   const T f_inf = m_::rcp(one + XP((V + C(20.0)) * C(1.0 / 7.0),
                                    C(17.41170806332765) * S7));  // e^(20/7)
   const T u27 = V + C(27.0);
   const T r30 = m_::rcp(one + XP((V + C(30.0)) * C(0.1), C(20.085536923187668) * Q));  // e^3
-  const T tau_f = C(1102.5) * m_::ex(-(u27 * u27) * C(1.0 / 225.0)) +
+  const T tau_f = C(1102.5) * LX(-(u27 * u27) * C(1.0 / 225.0), u27 * u27, -1.0 / 225.0, 0.0) +
                   C(200.0) * m_::rcp(one + XP((C(13.0) - V) * C(0.1),
                                               C(3.6692966676192444) * iQ)) +  // e^1.3
                   C(180.0) * r30 +
@@ -232,7 +247,7 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T f2_inf = C(0.67) * m_::rcp(one + XP((V + C(35.0)) * C(1.0 / 7.0),
                                              C(148.4131591025766) * S7)) +  // e^5
                    C(0.33);
-  const T tau_f2 = C(562.0) * m_::ex(-(u27 * u27) * C(1.0 / 240.0)) +
+  const T tau_f2 = C(562.0) * LX(-(u27 * u27) * C(1.0 / 240.0), u27 * u27, -1.0 / 240.0, 0.0) +
                    C(31.0) * m_::rcp(one + XP((C(25.0) - V) * C(0.1),
                                               C(12.182493960703473) * iQ)) +  // e^2.5
                    C(80.0) * r30;
@@ -263,12 +278,15 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
   const T i_bNa = C(0.00029) * (V - E_Na);
   const T i_bCa = C(0.000592) * (V - E_Ca);
   const T i_pCa = m_::dv(C(0.1238) * Cai, Cai + C(0.0005));
-  const T i_pK = m_::dv(C(0.0146) * vek, one + m_::ex((C(25.0) - V) * C(1.0 / 5.98)));
-  const T nak_den = one + C(0.1245) * m_::ex(C(-0.1) * V * FRT) + C(0.0353) * XP(-V * FRT, iE);
+  const T i_pK = m_::dv(C(0.0146) * vek,
+                        one + LX((C(25.0) - V) * C(1.0 / 5.98), V, -1.0 / 5.98, 25.0 / 5.98));
+  const T nak_den = one +
+                    C(0.1245) * LX(C(-0.1) * V * FRT, V, -0.1 * 96485.3415 / (8314.472 * 310.0), 0.0) +
+                    C(0.0353) * XP(-V * FRT, iE);
   T i_NaK;
   if (kRef) i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * m_::dv(Nai, Nai + C(40.0)) * m_::rcp(nak_den);
   else i_NaK = C(2.724 * 5.4 / (5.4 + 1.0)) * Nai * m_::rcp((Nai + C(40.0)) * nak_den);
-  const T e_g = m_::ex(C(0.35) * V * FRT);
+  const T e_g = LX(C(0.35) * V * FRT, V, 0.35 * 96485.3415 / (8314.472 * 310.0), 0.0);
   const T e_gm1 = XP(C(0.35 - 1.0) * V * FRT, e_g * iE);
   const T i_NaCa = m_::dv(C(1000.0) * (e_g * Nai * Nai * Nai * Cao - e_gm1 * C(140.0 * 140.0 * 140.0) * Cai * C(2.5)),
                           C((87.5 * 87.5 * 87.5 + 140.0 * 140.0 * 140.0) * (1.38 + 2.0)) *
@@ -321,7 +339,7 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
                            i_rel * C(0.001094 / 0.00005468) - i_xfer * C(0.016404 / 0.00005468));
 
   // Rush-Larsen gates (:387-399)
-#define NDT(tau) (-dt * m_::rcp(tau))
+#define NDT(tau) (ndt0 * m_::rcp(tau))
   s[S_M] = RLE(mg, m_inf, ndt_m);
   s[S_H] = RLE(hg, h_inf, ndt_h);
   s[S_J] = RLE(jg, h_inf, ndt_j);
@@ -337,6 +355,8 @@ HP_HD T ttp06_cell(T V, T (&s)[N_STATE], T dt, T g_Ks, T g_to,
 #undef NDT
 #undef RLE
 #undef XP
+#undef LX
+#undef L2E
   // R' forward Euler (:402-403)
   s[S_RPRIME] = Rp + dt * (-k2 * Cass * Rp + C(0.005) * (one - Rp));
   // concentrations, forward Euler with clamps (:406-410)

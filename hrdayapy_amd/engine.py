@@ -175,11 +175,19 @@ class MonodomainEngine:
                 np.asarray(pmj["ids"], dtype=np.int32), ptrs, entries, tgt.astype(np.int32), tptr,
                 order.astype(np.int32))]
             self.pmj_scratch = torch.zeros(len(entries), dtype=dt_, device=dev)
+            # per-junction constants of the exact exchange over dt (FP64)
+            npatch = np.diff(ptrs).astype(np.float64)
+            c, Cp = float(pmj["c_pmj"]), float(pmj["Ai_P"]) * self.Cm
+            Cq = npatch * float(pmj["Ai_M"]) * self.Cm
+            coef = np.stack([Cp / (Cp + Cq), np.exp(-c * self.dt * (1.0 / Cp + 1.0 / Cq)),
+                             np.exp(-(c / npatch) * self.dt / (float(pmj["Ai_M"]) * self.Cm)),
+                             1.0 / npatch], axis=1)
+            self.pmj_coef = torch.from_numpy(np.ascontiguousarray(coef)).to(dev)
             pk_, ptr_, ent_, tgt_, tptr_, tent_ = self.pmj_t
             self.pmj = nat.HpPmj(len(patches), len(entries), nat.ptr(pk_), nat.ptr(ptr_),
                                  nat.ptr(ent_), len(tgt), nat.ptr(tgt_), nat.ptr(tptr_),
                                  nat.ptr(tent_), float(pmj["c_pmj"]), float(pmj["Ai_P"]),
-                                 float(pmj["Ai_M"]), self.Cm)
+                                 float(pmj["Ai_M"]), self.Cm, nat.ptr(self.pmj_coef))
             self.pmj_at = [nat.HpPmjAt(nat.ptr(self.at), nat.ptr(self.V[k]),
                                        nat.ptr(self.step_ctr), self.dt, float(at_thr))
                            for k in range(2)]

@@ -197,6 +197,10 @@ typedef struct hp_pmj {
   const int32_t* tgt_entry; /* entries of each target slot */
   double c_pmj;             /* mS */
   double ai_p, ai_m, cm;    /* mm^2, mm^2, uF/mm^2 */
+  /* optional [n_junctions][4] per-junction constants for the step dt:
+   * Cp/(Cp+Cq), exp(-c dt (1/Cp + 1/Cq)), exp(-(c/n) dt/(Ai_M Cm)), 1/n
+   * (Cp = Ai_P Cm, Cq = n Ai_M Cm, n = patch size); NULL = computed per step */
+  const double* coef;
 } hp_pmj;
 
 /* Optional first-activation bookkeeping for the junction voxels: these carry
@@ -424,7 +428,8 @@ typedef struct hp_surrogate_params {
   float v_lo, v_hi;     /* fallback if v_lo < V < v_hi (-70, -40) */
   float ki_base, nai_base;
   int32_t force_fallback;
-  int32_t pad_;
+  int32_t variant;      /* inference kernel: 0 = 16 voxels per wave, 768 threads;
+                           1 = 32 voxels per wave, 512 threads */
   /* Optional compaction of the fallback voxels (both NULL = off): their slots
    * are appended to fb_list (capacity n_slots) and counted in *fb_count, which
    * the caller zeroes before the reaction call.  List order is unspecified. */

@@ -86,8 +86,9 @@ def test_gradients_match_cpu_autograd(cuda):
         assert relb < 5e-2
 
 
-def test_inference_kernel_matches_cpu_forward(cuda):
-    print("[case] fused surrogate inference", flush=True)
+@pytest.mark.parametrize("variant", [0, 1])
+def test_inference_kernel_matches_cpu_forward(cuda, variant):
+    print(f"[case] fused surrogate inference, kernel variant {variant}", flush=True)
     from hrdayapy_amd.layout import BrickLayout
     S = np.zeros((12, 10, 9), bool)
     S[1:11, 1:9, 1:8] = True
@@ -119,7 +120,7 @@ def test_inference_kernel_matches_cpu_forward(cuda):
     nbr = torch.from_numpy(lay.nbr).to(cuda)
     br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None)
     rate = torch.zeros(lay.n_slots, device=cuda)
-    sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, 0)
+    sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, variant)
     import ctypes
     nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
              state.data_ptr(), lay.n_slots, rate.data_ptr(), ctypes.byref(net.hp_w),
