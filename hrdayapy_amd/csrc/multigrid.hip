@@ -283,19 +283,7 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_spmv_kernel(hp_torso t, const fl
   double v[1] = {0.0};
   for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float pc = pnew(c);
-    float acc;
-    if (c == t.pin) {
-      acc = pc;
-    } else {
-      acc = t.diag[c] * pc;
-      if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * pnew(c + 1);
-      if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * pnew(c - 1);
-      int n;
-      if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * pnew(n);
-      if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * pnew(n);
-      if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * pnew(n);
-      if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * pnew(n);
-    }
+    const float acc = row_f32(t, c, k, m, pc, pnew);
     Ap[c] = acc;
     v[0] += (double)pc * (double)acc;
   });

@@ -7,6 +7,10 @@
 //   (A x)_c = diag_c x_c - sum_faces g_f x_nb,  g_f = harm(sigma_c, sigma_nb)/h^2,
 //   diag_c = sum of g_f over the six faces (Neumann at the box edge and air),
 //   row `pin` is the identity (reference potential).
+// Both the FP32 and the FP64 products are evaluated as sum_f g_f (x_c - x_nb),
+// exact for constants: with the reference cell on the body surface the
+// solution carries a large near-constant part, and a rounded FP32 diagonal
+// would turn it into an error that FP64 refinement cannot remove.
 // Layout: every (i,j) column of the box stores the contiguous z-run
 // [k0, k0+len) that covers its active cells; cell (i,j,k) of the run has index
 // off[i*nj+j] + k - k0.  Cells inside a run that are not active (holes) have
@@ -110,19 +114,7 @@ __global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const floa
   auto pnew = [&](int n) -> float { return first ? p[n] : r[n] / t.diag[n] + beta * p[n]; };
   for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float pc = pnew(c);
-    float acc;
-    if (c == t.pin) {
-      acc = pc;
-    } else {
-      acc = t.diag[c] * pc;
-      if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * pnew(c + 1);
-      if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * pnew(c - 1);
-      int n;
-      if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * pnew(n);
-      if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * pnew(n);
-      if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * pnew(n);
-      if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * pnew(n);
-    }
+    const float acc = row_f32(t, c, k, m, pc, pnew);
     Ap[c] = acc;
     v[0] += (double)pc * (double)acc;
   });

@@ -123,18 +123,29 @@ __device__ __forceinline__ void for_cells(const hp_torso& t, F&& f) {
 }
 
 // A x at one cell in FP32 with the stored diagonal (the inner PCG operator).
+// (A x)_c in FP32 in zero-row-sum form, sum_f g_f (x_c - x_nb) with the
+// neighbour values from xat(n): exact for a constant x, so the large
+// near-constant component of a torso potential measured against one pinned
+// cell does not leak into the product through a rounded diagonal.  The pinned
+// cell and cells without any conducting face are identity rows.
+template <typename F>
+__device__ __forceinline__ float row_f32(const hp_torso& t, int c, int k, const ColNbr& m,
+                                         float xc, F&& xat) {
+  if (c == t.pin) return xc;
+  float acc = 0.f, gs = 0.f, g;
+  int n;
+  if (k + 1 < m.c.k0 + m.c.len) { g = t.gz[c]; acc += g * (xc - xat(c + 1)); gs += g; }
+  if (k - 1 >= m.c.k0) { g = t.gz[c - 1]; acc += g * (xc - xat(c - 1)); gs += g; }
+  if ((n = cell_at(m.xp, k)) >= 0) { g = t.gx[c]; acc += g * (xc - xat(n)); gs += g; }
+  if ((n = cell_at(m.xm, k)) >= 0) { g = t.gx[n]; acc += g * (xc - xat(n)); gs += g; }
+  if ((n = cell_at(m.yp, k)) >= 0) { g = t.gy[c]; acc += g * (xc - xat(n)); gs += g; }
+  if ((n = cell_at(m.ym, k)) >= 0) { g = t.gy[n]; acc += g * (xc - xat(n)); gs += g; }
+  return gs == 0.f ? xc : acc;
+}
+
 __device__ __forceinline__ float apply_row_f32(const hp_torso& t, const float* __restrict__ x, int c,
                                                int i, int j, int k, const ColNbr& m) {
-  if (c == t.pin) return x[c];
-  float acc = t.diag[c] * x[c];
-  if (k + 1 < m.c.k0 + m.c.len) acc -= t.gz[c] * x[c + 1];
-  if (k - 1 >= m.c.k0) acc -= t.gz[c - 1] * x[c - 1];
-  int n;
-  if ((n = cell_at(m.xp, k)) >= 0) acc -= t.gx[c] * x[n];
-  if ((n = cell_at(m.xm, k)) >= 0) acc -= t.gx[n] * x[n];
-  if ((n = cell_at(m.yp, k)) >= 0) acc -= t.gy[c] * x[n];
-  if ((n = cell_at(m.ym, k)) >= 0) acc -= t.gy[n] * x[n];
-  return acc;
+  return row_f32(t, c, k, m, x[c], [&](int n) { return x[n]; });
 }
 
 // ---------------------------------------------------------------------------
