@@ -15,8 +15,7 @@ are accepted for signature compatibility and ignored.
 """
 from __future__ import annotations
 
-import os
-import tempfile
+import math
 import time
 from pathlib import Path
 
@@ -35,11 +34,14 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
                          stim_regions=None, stim_len_mm=2.0, c_pmj=0.05, n_pmj=1,
                          phi_endo_max=0.35, phi_epi_min=0.65, dtype=torch.float32,
                          device="cuda", brick=None, verbose=False, sigma_scale=None,
-                         inst=None, inst_params=None):
+                         inst=None, inst_params=None, auto_substep=False):
     """Set up the tissue on the host and upload it into a MonodomainEngine.
 
     ``sigma_M`` selects the isotropic 6-point operator (simulation.compute_coupled);
     otherwise sigma_l/sigma_t and fibre_dir give the 19-point anisotropic one.
+    Diffusion is explicit, so ``dt`` above the forward-Euler limit of the
+    stencil or the cable raises ValueError; with ``auto_substep`` the engine
+    instead steps dt/n_sub, n_sub = meta["n_sub"] engine steps per dt.
     Returns (engine, meta) where meta holds the reference-order bookkeeping.
     """
     t0 = time.perf_counter()
@@ -62,12 +64,20 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
     if stim_protocol is None:
         stim_protocol = [dict(name="av_node", target="root", amp=12.0, dur=2.0, onset=0.0)]
     sites = tissue.stim_sites(stim_protocol, root, ectopic_region, vox, Np, stim_regions)
+    dt_lim = tissue.explicit_dt_limit(D6, voxel_size, A_M, Cm, comp_edges, pc["w_pp"], pc["Ai_P"])
+    n_sub = 1
+    if dt > dt_lim:
+        if not auto_substep:
+            raise ValueError(f"dt={dt} ms exceeds the explicit diffusion limit "
+                             f"{dt_lim:.4g} ms of this tissue (pass a smaller dt)")
+        n_sub = int(math.ceil(dt / dt_lim))
     eng = MonodomainEngine(
-        lay, D6, ct, h=voxel_size, A_M=A_M, Cm=Cm, dt=dt, dtype=dtype, device=device,
+        lay, D6, ct, h=voxel_size, A_M=A_M, Cm=Cm, dt=dt / n_sub, dtype=dtype, device=device,
         cable=dict(n=Np, edges=comp_edges, w_pp=pc["w_pp"], Ai_P=pc["Ai_P"]),
         pmj=dict(ids=pmj_ids, patches=patches, c_pmj=c_pmj, Ai_P=pc["Ai_P"], Ai_M=pc["Ai_M"]),
         stim_sites=sites, inst=inst, inst_params=inst_params)
-    meta = dict(Np=Np, N_myo=lay.n_nodes, comp_nodes=comp_nodes, comp_edges=comp_edges,
+    meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim,
+                comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,
                 pmj_ids=pmj_ids, pmj_patches=patches, sites=sites, layout=lay,
                 constants=pc, voxel_size=voxel_size, setup_s=time.perf_counter() - t0)
@@ -114,8 +124,11 @@ def purkinje_myocardium_anisotropic_solver(
         R_P=R_P, A_P=A_P if _sigma_M is not None else None, dt=dt, T=T,
         stim_protocol=stim_protocol, ectopic_region=ectopic_region, stim_regions=stim_regions,
         stim_len_mm=stim_len_mm, c_pmj=c_pmj, n_pmj=n_pmj, phi_endo_max=phi_endo_max,
-        phi_epi_min=phi_epi_min, dtype=dtype, device=dev, verbose=True)
-    Np, N_myo = meta["Np"], meta["N_myo"]
+        phi_epi_min=phi_epi_min, dtype=dtype, device=dev, verbose=True, auto_substep=True)
+    Np, N_myo, n_sub = meta["Np"], meta["N_myo"], meta["n_sub"]
+    if n_sub > 1:
+        print(f"[hrdaya] dt={dt} ms is above the explicit diffusion limit "
+              f"{meta['dt_limit']:.4g} ms: {n_sub} substeps of {dt / n_sub:.4g} ms per step")
     vox = meta["vox_idx"]
     n_steps = int(T / dt)
     save_every = (max(1, int(round(frame_save_dt / dt))) if frame_save_dt is not None
@@ -134,7 +147,8 @@ def purkinje_myocardium_anisotropic_solver(
         # output after step `step` is due when step % every == 0 (frame stamped step*dt)
         nxt = min([((step // c) + (1 if step % c else 0)) * c for c in cadence] + [n_steps - 1])
         nxt = max(nxt, step)
-        eng.run(nxt - step + 1, graph_chunk=graph_chunk if nxt - step + 1 >= graph_chunk else 0)
+        k = (nxt - step + 1) * n_sub
+        eng.run(k, graph_chunk=graph_chunk if k >= graph_chunk else 0)
         t = nxt * dt
         if nxt % save_every == 0:
             Vg = eng.V_global().float().cpu().numpy()
@@ -148,8 +162,8 @@ def purkinje_myocardium_anisotropic_solver(
         step = nxt + 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    print(f"[hrdaya] {n_steps:,} steps x {N_myo:,} voxels in {wall:.2f} s "
-          f"({n_steps * N_myo / max(wall, 1e-9):.3e} voxel-steps/s)")
+    print(f"[hrdaya] {n_steps * n_sub:,} steps x {N_myo:,} voxels in {wall:.2f} s "
+          f"({n_steps * n_sub * N_myo / max(wall, 1e-9):.3e} voxel-steps/s)")
     if run_vm and vm_frames:
         _save_vm_snapshots(out_vm_npz, np.arange(N_myo, dtype=np.int32) + Np, meta["nodes_mm"],
                            vox, np.stack(vm_frames), np.array(vm_times, np.float32), dt,

@@ -22,8 +22,66 @@ def _harm(a, b):
     return np.where(s > 1e-30, 2.0 * a * b / np.where(s > 1e-30, s, 1.0), 0.0)
 
 
+def _face_conductances(sigma, tissue, flat_of_cell, dx_mm):
+    """FP64 conductance of the +x/+y/+z face of every stored cell:
+    harm(sigma_c, sigma_nb)/h^2 between two active cells, 0 otherwise
+    (build_torso_grid.py:325-390, sharp boundary)."""
+    ni, nj, nk = sigma.shape
+    sflat, act = sigma.ravel(), tissue.ravel()
+    inv_h2 = 1.0 / (dx_mm * 1e-3) ** 2
+    n_cells = len(flat_of_cell)
+    ci, cj, kk = flat_of_cell // (nj * nk), (flat_of_cell // nk) % nj, flat_of_cell % nk
+
+    def face(off_flat, valid):
+        g = np.zeros(n_cells, dtype=np.float64)
+        f = flat_of_cell[valid]
+        nb = f + off_flat
+        both = act[f] & act[nb]
+        gv = np.zeros(len(f))
+        gv[both] = _harm(sflat[f[both]], sflat[nb[both]]) * inv_h2
+        g[valid] = gv
+        return g
+
+    return face(nj * nk, ci + 1 < ni), face(nk, cj + 1 < nj), face(1, kk + 1 < nk)
+
+
 class TorsoGrid(dict):
-    """dict with the reference's torso-grid keys plus the GPU layout arrays."""
+    """dict with the reference's torso-grid keys plus the GPU layout arrays.
+
+    ``grid["A_scipy"]`` (the unpinned FP64 stiffness as CSR, rows in active
+    order) and ``grid["solve_fn"]`` (b -> phi for the pinned system,
+    build_torso_grid.py:404-449) are built on first access: the CSR matrix of
+    a full torso takes gigabytes of host memory that the GPU path never needs.
+    """
+
+    def __missing__(self, key):
+        if key == "A_scipy":
+            val = self.stiffness_csr()
+        elif key == "solve_fn":
+            val = self._solve_fn()
+        else:
+            raise KeyError(key)
+        self[key] = val
+        return val
+
+    def _solve_fn(self, device="cuda", tol=1e-10):
+        """solve(b) -> phi with b, phi in active-row order (numpy FP64).  As in the
+        reference, b[pin_row] is set to 0 in place.  The solve is the multigrid
+        PCG inside FP64 iterative refinement on the device."""
+        import torch
+        from .forward import PoissonSolver
+        ps = PoissonSolver(self, device, precond="mg")
+        cells = torch.from_numpy(self.cell_of_row).to(ps.dev)
+        pin_row = int(self["pin_row"])
+
+        def solve_fn(b):
+            b[pin_row] = 0.0
+            bc = torch.zeros(ps.n, dtype=torch.float64, device=ps.dev)
+            bc[cells] = torch.as_tensor(np.asarray(b, dtype=np.float64), device=ps.dev)
+            x = ps.solve_refined(bc, tol=tol)
+            return x[cells].cpu().numpy()
+
+        return solve_fn
 
     @classmethod
     def from_labels(cls, label_vol, sigma_map, dx_mm, origin_mm=(0.0, 0.0, 0.0), verbose=False):
@@ -53,25 +111,9 @@ class TorsoGrid(dict):
         dk = np.arange(n_cells) - col_off[cols]
         kk = k_first.ravel()[cols] + dk
         flat_of_cell = cols * nk + kk
-        h_m = dx_mm * 1e-3
-        inv_h2 = 1.0 / (h_m * h_m)
-        sflat = sigma.ravel()
+        gx, gy, gz = _face_conductances(sigma, tissue, flat_of_cell, dx_mm)
         act = tissue.ravel()
-
-        def face(off_flat, valid):
-            g = np.zeros(n_cells, dtype=np.float64)
-            f = flat_of_cell[valid]
-            nb = f + off_flat
-            both = act[f] & act[nb]
-            gv = np.zeros(len(f))
-            gv[both] = _harm(sflat[f[both]], sflat[nb[both]]) * inv_h2
-            g[valid] = gv
-            return g
-
         ci, cj = cols // nj, cols % nj
-        gx = face(nj * nk, ci + 1 < ni)
-        gy = face(nk, cj + 1 < nj)
-        gz = face(1, kk + 1 < nk)
         # diag = sum of the six faces: + faces of this cell and + faces of the - neighbours
         cell_of_flat = np.full(lab.size, -1, dtype=np.int64)
         cell_of_flat[flat_of_cell] = np.arange(n_cells)
@@ -127,11 +169,13 @@ class TorsoGrid(dict):
         cell_of_flat = np.full(ni * nj * nk, -1, dtype=np.int64)
         cell_of_flat[flat] = np.arange(L["n_cells"])
         rows, cols, vals = [], [], []
-        for g, off in ((L["gx"], nj * nk), (L["gy"], nk), (L["gz"], 1)):
+        g64 = _face_conductances(np.asarray(self["sigma_vol"], dtype=np.float64),
+                                 np.asarray(self["tissue_mask"]), flat, self["dx_mm"])
+        for g, off in zip(g64, (nj * nk, nk, 1)):
             c = np.nonzero(g > 0)[0]
             nb = cell_of_flat[flat[c] + off]
             a, b = row_of_cell[c], row_of_cell[nb]
-            w = g[c].astype(np.float64)
+            w = g[c]
             rows += [a, b, a, b]
             cols += [b, a, a, b]
             vals += [-w, -w, w, w]

@@ -33,6 +33,29 @@ def conductivity_D6(fibre_dir, vox_idx, sigma_l, sigma_t, scale=None):
     return D
 
 
+def explicit_dt_limit(D6, h, A_M, Cm, cable_edges=None, w_pp=0.0, Ai_P=1.0):
+    """Largest forward-Euler diffusion step (ms) that a spectral bound proves stable.
+
+    Forward Euler V += (dt/Cm) L V is stable while dt * rho(L) <= 2 Cm.
+    Myocardium (Gershgorin): a node's 18 stencil weights sum to at most
+    h*(2*tr D + 2*sum|D_offdiag|) (faces harm(D_kk)*h, edge diagonals
+    |D_ab|*h/2 each, four per plane); divided by Ai_M = A_M*h^3, twice that
+    bounds rho.  Cable (Anderson-Morley, uniform weights): rho <= (w_pp/Ai_P)
+    * max over edges of deg(a) + deg(b).  The PMJ exchange is integrated
+    exactly and sets no limit.
+    """
+    D = np.abs(np.asarray(D6, dtype=np.float64))
+    row = 2.0 * (D[0] + D[1] + D[2]) + 2.0 * (D[3] + D[4] + D[5])
+    lim = np.inf
+    if row.size and row.max() > 0:
+        lim = A_M * h * h * Cm / row.max()
+    e = None if cable_edges is None else np.asarray(cable_edges, dtype=np.int64).reshape(-1, 2)
+    if e is not None and len(e) and w_pp > 0:
+        deg = np.bincount(e.ravel())
+        lim = min(lim, 2.0 * Ai_P * Cm / (w_pp * float((deg[e[:, 0]] + deg[e[:, 1]]).max())))
+    return lim
+
+
 def isotropic_D6(n, sigma):
     D = np.zeros((6, n))
     D[:3] = sigma

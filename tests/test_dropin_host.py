@@ -1,0 +1,186 @@
+"""Host-side behaviour of the drop-in call surface: activation maps and the
+snapshot reader (closed-form crossings), argument errors raised before any
+device work, the explicit-step limit against the oracle operator's spectrum,
+and the torso grid dict against the oracle stiffness."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from hrdayapy_amd import tissue
+from hrdayapy_amd.anisotropic_simulation import compute_anisotropic_coupled
+from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
+    build_coupled_system)
+from hrdayapy_amd.ecg import compute_torso_grid
+from hrdayapy_amd.simulation import compute_coupled
+from hrdayapy_amd.simulation.functions.activation_maps import (
+    VmSnapshots, compute_activation_maps, load_activation_maps, load_vm_snapshots)
+from oracle import ecg as OE
+from oracle import monodomain as OMD
+from tests._problems import corner_region, slab, small_tree, stub_tree
+
+NAN = np.nan
+
+
+def _snapshots():
+    """Four nodes, eight frames 1 ms apart; crossings of -40 mV chosen so the
+    interpolated times are exact decimals (worked out by hand below)."""
+    Vm = np.array([
+        [-80, -80, -60, -20, 10, 10, -60, -80],   # up 2->3 at 2.5, down 5->6 at 5+50/70
+        [-80, 0, -80, -80, 0, -80, -80, -80],     # up 0.5, down 1.5, up 3.5, down 4.5
+        [-80] * 8,                                # never
+        [-80, -39, -41, -39, -80, -80, -80, -80], # up 40/41, down 1.5, up 2.5, down 3+1/41
+    ], dtype=np.float32).T.copy()
+    vox = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 1]], np.int32)
+    return VmSnapshots(np.arange(4, dtype=np.int32) + 7, vox.astype(np.float32) * 0.4, vox, Vm,
+                       np.arange(8, dtype=np.float32), 0.02, 1.0)
+
+
+def _rows(a):
+    return [list(r) for r in a]
+
+
+def test_activation_maps_closed_form(tmp_path):
+    m = compute_activation_maps(_snapshots())
+    exp_a = np.array([[2.5, NAN], [0.5, 3.5], [NAN, NAN], [40 / 41, 2.5]])
+    exp_d = np.array([[5 + 50 / 70, NAN], [1.5, 4.5], [NAN, NAN], [1.5, 3 + 1 / 41]])
+    np.testing.assert_allclose(m.activation_times, exp_a, rtol=1e-6)
+    np.testing.assert_allclose(m.deactivation_times, exp_d, rtol=1e-6)
+    assert m.n_activations.tolist() == [1, 2, 0, 2]
+    assert m.n_deactivations.tolist() == [1, 2, 0, 2]
+    np.testing.assert_allclose(m.apd_map(1), exp_d[:, 0] - exp_a[:, 0], rtol=1e-6)
+    assert np.isnan(m.activation_map(3)).all() and m.n_events == 2
+    g = m.activation_grid(1)
+    assert g.shape == (2, 2, 2) and g[1, 1, 1] == pytest.approx(40 / 41, rel=1e-6)
+    # save / load keeps every table
+    m.node_ids = m.node_ids.copy()
+    from hrdayapy_amd.simulation.functions.activation_maps import save_activation_maps
+    save_activation_maps(m, tmp_path / "am")
+    r = load_activation_maps(tmp_path / "am")
+    np.testing.assert_array_equal(r.activation_times, m.activation_times)
+    np.testing.assert_array_equal(r.deactivation_times, m.deactivation_times)
+    assert r.node_ids.tolist() == [7, 8, 9, 10] and r.min_upstroke_vel is None
+
+
+def test_activation_debounce_and_velocity_gate():
+    s = _snapshots()
+    # gap 2 ms: node 3's second upstroke (1.52 ms after the first) is dropped
+    m = compute_activation_maps(s, min_event_gap_ms=2.0)
+    assert m.n_activations.tolist() == [1, 2, 0, 1]
+    # gap 3.5 ms drops node 1's second beat as well: the table narrows to one column
+    m = compute_activation_maps(s, min_event_gap_ms=3.5)
+    assert m.activation_times.shape == (4, 1)
+    # velocity gate: node 3's second upstroke rises 2 mV/ms and is dropped
+    m = compute_activation_maps(s, min_upstroke_vel=30.0)
+    assert m.n_activations.tolist() == [1, 2, 0, 1]
+    assert m.activation_times[3, 0] == pytest.approx(40 / 41, rel=1e-6)
+    # falling gate: node 3's first downstroke falls 2 mV/ms and is dropped
+    m = compute_activation_maps(s, min_downstroke_vel=30.0)
+    assert m.n_deactivations.tolist() == [1, 2, 0, 1]
+    assert m.deactivation_times[3, 0] == pytest.approx(3 + 1 / 41, rel=1e-6)
+
+
+def test_snapshot_reader(tmp_path):
+    s = _snapshots()
+    np.savez(tmp_path / "vm.npz", node_ids=s.node_ids, coords_mm=s.coords_mm, vox_idx=s.vox_idx,
+             Vm=s.Vm, time_vm=s.time_vm, dt_solver=np.float32(0.02), vm_save_dt=np.float32(1.0))
+    r = load_vm_snapshots(tmp_path / "vm")
+    assert r.n_frames == 8 and r.N_myo == 4
+    np.testing.assert_array_equal(r["Vm"], s.Vm)
+    assert r.nearest_node([0.4, 0.4, 0.4]) == 3
+    f, t = r.field_at_time(2.6)
+    assert t == 3.0 and f[0] == -20
+    assert r.nodes_in_sphere([0, 0, 0], 0.41).tolist() == [0, 1, 2]
+    # a dict with the same keys is accepted by compute_activation_maps
+    d = {k: r[k] for k in ("Vm", "time_vm", "coords_mm", "vox_idx")}
+    np.testing.assert_array_equal(compute_activation_maps(d).activation_times,
+                                  compute_activation_maps(r).activation_times)
+    with pytest.raises(FileNotFoundError):
+        load_vm_snapshots(tmp_path / "missing")
+    np.savez(tmp_path / "bad.npz", Vm=s.Vm)
+    with pytest.raises(KeyError):
+        load_vm_snapshots(tmp_path / "bad.npz")
+    one = VmSnapshots(s.node_ids, s.coords_mm, s.vox_idx, s.Vm[:1], s.time_vm[:1], 0.02, 1.0)
+    with pytest.raises(ValueError):
+        compute_activation_maps(one)
+
+
+def _slab_args():
+    S, phi, f = slab(10, 8, 6)
+    nodes, el, act = stub_tree(S, 0.25)
+    return nodes, el, act, S, S, phi, f
+
+
+def test_argument_errors_before_device_work():
+    nodes, el, act, S, Z, phi, f = _slab_args()
+    with pytest.raises(TypeError, match="A_P"):
+        compute_anisotropic_coupled(nodes, el, act, S, Z, phi, f, A_P=100.0)
+    with pytest.raises(RuntimeError, match="GPU"):
+        compute_anisotropic_coupled(nodes, el, act, S, Z, phi, f, device="cpu")
+    with pytest.raises(RuntimeError, match="GPU"):
+        compute_coupled(nodes, el, act, S, Z, phi, device="cpu")
+    bad = [dict(name="x", target="apex", amp=1.0, dur=1.0)]
+    with pytest.raises(ValueError, match="Unknown stimulus target"):
+        build_coupled_system(nodes, el, act, S, phi, f, stim_protocol=bad, device="cpu")
+    reg = [dict(name="x", target="region", region="lv", amp=1.0, dur=1.0)]
+    with pytest.raises(ValueError, match="stim_regions"):
+        build_coupled_system(nodes, el, act, S, phi, f, stim_protocol=reg, device="cpu",
+                             stim_regions={"rv": corner_region(S)})
+    # an explicit step above the diffusion limit is refused unless substepping is asked for
+    with pytest.raises(ValueError, match="explicit diffusion limit"):
+        build_coupled_system(nodes, el, act, S, phi, f, voxel_size=0.25, sigma_l=0.24,
+                             sigma_t=0.08, dt=0.2, device="cpu")
+
+
+@pytest.mark.parametrize("kind", ["stub", "tree"])
+def test_explicit_limit_is_below_the_true_stability_bound(kind):
+    """Forward Euler V += (dt/Cm) L V is stable iff dt * rho(L) / Cm <= 2; the
+    Gershgorin limit must stay below that bound (and not far below it)."""
+    h = 0.25
+    S, phi, f = slab(10, 8, 6)
+    if kind == "stub":
+        nodes, el, act = stub_tree(S, h)
+        kw = dict(sigma_P=0.0, c_pmj=0.0)
+    else:
+        nodes, el, act = small_tree(S, h)
+        kw = dict(sigma_P=0.45, c_pmj=0.0)
+    kw.update(voxel_size=h, dx_p=0.2, sigma_l=0.24, sigma_t=0.08, Cm=0.01, A_M=140.0, R_P=0.015)
+    prob = OMD.Problem(nodes, el, act, S, phi, f, **kw)
+    L = prob.L_nopmj
+    rho = abs(spla.eigs(L, k=1, which="LM", return_eigenvectors=False)[0])
+    bound = 2.0 * prob.Cm / rho
+    from hrdayapy_amd.layout import BrickLayout
+    lay = BrickLayout(S.astype(bool), (4, 4, 4))
+    D6 = tissue.conductivity_D6(f, lay.vox_idx, 0.24, 0.08)
+    pc = tissue.purkinje_constants(0.015, kw["sigma_P"], 0.2, 140.0, h)
+    lim = tissue.explicit_dt_limit(D6, h, 140.0, 0.01, prob.comp_edges, pc["w_pp"], pc["Ai_P"])
+    print(f"  {kind}: Gershgorin limit {lim:.4f} ms, true bound {bound:.4f} ms")
+    assert lim <= bound * (1 + 1e-9)
+    assert lim >= 0.5 * bound
+
+
+def test_torso_grid_dict_and_stiffness(tmp_path):
+    lab = np.zeros((14, 12, 10), np.int32)
+    lab[2:12, 2:10, 2:8] = 1
+    lab[5:9, 4:8, 3:6] = 2
+    lab[3, 3, 3] = 0
+    smap = {1: 0.2, 2: 0.05}
+    g = compute_torso_grid(sigma_map=smap, label_volume=lab, dx_coarse_mm=1.5,
+                           save_path=tmp_path / "torso.npz", verbose=False)
+    for k in ("shape", "dx_mm", "origin_mm", "sigma_vol", "active_mask", "tissue_mask",
+              "active_ids", "active_id_map", "surface_flat", "surface_rows", "surface_xyz",
+              "pin_row", "use_sbm"):
+        assert k in g, k
+    sig = np.zeros(lab.shape)
+    for k, v in smap.items():
+        sig[lab == k] = v
+    A, ids, idm = OE.assemble_stiffness(sig, 1.5e-3, lab != 0)
+    assert g["pin_row"] == OE.pin_row_of(lab != 0, idm)
+    np.testing.assert_array_equal(g["active_ids"], ids)
+    D = (g["A_scipy"] - A).tocoo()
+    assert np.abs(D.data).max(initial=0.0) <= 1e-12 * np.abs(A.data).max()
+    z = np.load(tmp_path / "torso.npz")
+    assert set(z.files) >= {"shape", "dx_mm", "origin_mm", "sigma_vol", "active_mask",
+                            "tissue_mask", "active_ids", "active_id_map", "surface_flat",
+                            "surface_rows", "surface_xyz", "pin_row", "use_sbm"}
+    with pytest.raises(NotImplementedError):
+        compute_torso_grid(sigma_map=smap, label_volume=lab, use_sbm=True)
