@@ -127,9 +127,9 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
     # per-kernel breakdown (HIP events on the launching stream; advances the run)
     out["reaction_us"] = _event_us(run.launch_reaction, 20)
     if form == "mlp":  # the other inference kernel shape, for comparison
-        run.sp.variant = 1
-        out["reaction_us_32_per_wave"] = _event_us(run.launch_reaction, 20)
         run.sp.variant = 0
+        out["reaction_us_16_per_wave"] = _event_us(run.launch_reaction, 20)
+        run.sp.variant = 1
     out["substep_us"] = _event_us(run.launch_substep, 20)
     if ref_at is not None:
         both = np.isfinite(at) & np.isfinite(ref_at)
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x8x8, or auto")
     ap.add_argument("--forms", type=str, default="plain,surrogate,bigstep")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
-    ap.add_argument("--train-epochs", type=int, default=4)
+    ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)

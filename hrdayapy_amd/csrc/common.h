@@ -15,6 +15,35 @@ int check_launch(const char* what);
 inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }
 
 // ---------------------------------------------------------------------------
+// Raw buffer access: a 128-bit resource over `bytes` bytes of one array, and
+// loads/stores at a 32-bit element index.  The address is the scalar base plus
+// the lane's 32-bit offset, so a run of per-variable loads costs no 64-bit
+// vector address arithmetic; loads past `bytes` return 0 and such stores are
+// dropped by the hardware.  Word 3 is the gfx9-family raw (untyped) format.
+// ---------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t buf_t;
+__device__ __forceinline__ buf_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T buf_ld(buf_t r, uint32_t idx);
+template <> __device__ __forceinline__ float buf_ld<float>(buf_t r, uint32_t idx) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, idx * 4u, 0, 0));
+}
+template <> __device__ __forceinline__ double buf_ld<double>(buf_t r, uint32_t idx) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 8u, 0, 0));
+}
+__device__ __forceinline__ void buf_st(buf_t r, uint32_t idx, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, idx * 4u, 0, 0);
+}
+__device__ __forceinline__ void buf_st(buf_t r, uint32_t idx, double v) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, idx * 8u, 0, 0);
+}
+__device__ __forceinline__ uint32_t buf_ld_u8(buf_t r, uint32_t idx) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, idx, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
 // Arithmetic traits.  The FP32 path uses the hardware transcendental and
 // reciprocal instructions (v_exp_f32 / v_log_f32 / v_rcp_f32 / v_sqrt_f32,
 // ~1 ulp); the FP64 path uses the correctly rounded library routines so that

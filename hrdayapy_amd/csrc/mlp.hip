@@ -293,11 +293,12 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
   }
   __syncthreads();
 
-  const int64_t n_slots = g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
+  const int64_t n_items = sp.slot_list ? sp.n_list : g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
   const int col = lane & 15, grp = lane >> 4;
   const float dtm = sp.t_macro;
-  // persistent loop: one TV-voxel tile (NC 16-voxel MFMA columns) per wave
-  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile * TV < n_slots;
+  // persistent loop: one TV-voxel tile (NC 16-voxel MFMA columns) per wave,
+  // over the slots or over the listed slots
+  for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile * TV < n_items;
        tile += (int64_t)gridDim.x * WAVES) {
     const int64_t base = tile * TV;
     int64_t slot[NC];
@@ -305,8 +306,10 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     bool any = false;
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
-      slot[t] = base + 16 * t + col;
-      lab[t] = g.label[slot[t]];
+      const int64_t item = base + 16 * t + col;
+      slot[t] = sp.slot_list ? (int64_t)sp.slot_list[item] : item;
+      lab[t] = slot[t] >= 0 ? g.label[slot[t]] : (uint8_t)0xFF;
+      if (slot[t] < 0) slot[t] = 0;  // no voxel: a valid address, never written (label 0xFF)
       any |= lab[t] != 0xFF;
     }
     // skip tiles without myocardium (wave-uniform)
@@ -435,13 +438,14 @@ __global__ void __launch_bounds__(256) bigstep_kernel(hp_bricks g, const float* 
     return;
   }
   const int ct = lab & 3;
+  const uint32_t row_bytes = (uint32_t)ld * 4u, sl = (uint32_t)slot;
   float s[N_STATE];
 #pragma unroll
-  for (int k = 0; k < N_STATE; ++k) s[k] = state[(int64_t)k * ld + slot];
+  for (int k = 0; k < N_STATE; ++k) s[k] = buf_ld<float>(buf_rsrc(state + k * (size_t)ld, row_bytes), sl);
   const float I = ttp06_cell<float, false>(v, s, dtm, (float)kGKs[ct], (float)kGto[ct], ct != 2,
                                            sp.ki_base, sp.nai_base);
 #pragma unroll
-  for (int k = 0; k < N_STATE; ++k) state[(int64_t)k * ld + slot] = s[k];
+  for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), sl, s[k]);
   rate[slot] = -I;
 }
 
@@ -526,8 +530,9 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
     return -1;
   }
   const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
-  if (n_slots % 32) {
-    set_error("hp_surrogate_step: slot count must be a multiple of 32");
+  const int64_t n_items = sp->slot_list ? sp->n_list : n_slots;
+  if (n_items % 32 || n_items <= 0) {
+    set_error("hp_surrogate_step: slot (or slot_list) count must be a positive multiple of 32");
     return -1;
   }
   const size_t smem = sizeof(MlpSmem);
@@ -541,7 +546,7 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
   }
   if (n_blocks <= 0) n_blocks = 256;
   const int nc = sp->variant == 1 ? 2 : 1, waves = nc == 2 ? 8 : 12;
-  const int64_t tiles = n_slots / (16 * nc);
+  const int64_t tiles = n_items / (16 * nc);
   if ((int64_t)n_blocks * waves > tiles) n_blocks = (int)((tiles + waves - 1) / waves);
   if (nc == 1)
     hipLaunchKernelGGL((surrogate_kernel<1, 768>), dim3(n_blocks), dim3(768), smem,
@@ -554,11 +559,11 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
 
 int hp_bigstep_reaction(const hp_bricks* g, const float* V, float* V_prev, float* state,
                         int64_t ld, float* rate, const hp_surrogate_params* sp, void* stream) {
-  if (!g || !sp || g->n_bricks <= 0) {
-    set_error("hp_bigstep_reaction: bad arguments");
+  const int64_t n_slots = g ? g->n_bricks * (int64_t)(g->bx * g->by * g->bz) : 0;
+  if (!g || !sp || g->n_bricks <= 0 || ld < n_slots || (uint64_t)ld * 4 >= (1ull << 32)) {
+    set_error("hp_bigstep_reaction: bad arguments (ld must cover the slots, 32-bit offsets)");
     return -1;
   }
-  const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
   hipLaunchKernelGGL(bigstep_kernel, dim3(ceil_div(n_slots, 256)), dim3(256), 0,
                      (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *sp);
   return check_launch("hp_bigstep_reaction");

@@ -36,9 +36,10 @@ struct Tile {
   static constexpr int N = TX * TY * TZ;
   static constexpr int SX = TY * TZ, SY = TZ;
   static constexpr int BVOL = BX * BY * BZ;
-  T V[N];
-  T D[6][N];
-  uint8_t L[N];
+  // one spare cell at index N takes the writes of lanes without a tile cell
+  T V[N + 1];
+  T D[6][N + 1];
+  uint8_t L[N + 1];
   int32_t nbr[27];
 };
 
@@ -60,72 +61,100 @@ struct HaloPlan {
 };
 
 struct HaloCell {
-  int tile;        // index into the LDS tile, -1 = no cell for this lane
-  int dir;         // neighbour-table entry of the brick holding the cell
-  int local;       // slot of the cell inside that brick
-  int ncomp;       // D components to load: 6 (own brick), 3 (face layer), 1 (edge line)
-  int c0, c1, c2;  // the components (face layers and edges)
+  int tile;   // index into the LDS tile, -1 = no cell for this lane
+  int dir;    // neighbour-table entry of the brick holding the cell
+  int local;  // slot of the cell inside that brick
+  int kind;   // 0 none, 1 own brick (six D components), 2 face layer (three), 3 edge line (one)
+  int c0;     // first D component of a face layer / the component of an edge line
 };
 
 template <int BX, int BY, int BZ>
-__device__ __forceinline__ HaloCell halo_cell(int e) {
+constexpr HaloCell halo_cell(int e) {
   typedef HaloPlan<BX, BY, BZ> H;
   constexpr int TY = BY + 2, TZ = BZ + 2;
-  HaloCell h;
-  int gx, gy, gz, dx = 0, dy = 0, dz = 0, lx, ly, lz;
-  h.ncomp = 3;
-  h.c0 = h.c1 = h.c2 = 0;
+  HaloCell h{-1, 13, 0, 0, 0};
+  int gx = 0, gy = 0, gz = 0, dx = 0, dy = 0, dz = 0, lx = 0, ly = 0, lz = 0;
+  h.kind = 2;
   if (e < H::E0) {  // own brick
     lx = e / (BY * BZ); ly = (e / BZ) % BY; lz = e % BZ;
     gx = lx; gy = ly; gz = lz;
-    h.ncomp = 6;
+    h.kind = 1;
   } else if (e < H::E2) {  // x layers: D00, D01, D02
     const int q = e - H::E0, sd = q / (BY * BZ), r = q % (BY * BZ);
     ly = r / BZ; lz = r % BZ; dx = sd ? 1 : -1; lx = sd ? 0 : BX - 1;
     gx = sd ? BX : -1; gy = ly; gz = lz;
-    h.c0 = 0; h.c1 = 3; h.c2 = 4;
+    h.c0 = 0;
   } else if (e < H::E4) {  // y layers: D11, D01, D12
     const int q = e - H::E2, sd = q / (BX * BZ), r = q % (BX * BZ);
     lx = r / BZ; lz = r % BZ; dy = sd ? 1 : -1; ly = sd ? 0 : BY - 1;
     gx = lx; gy = sd ? BY : -1; gz = lz;
-    h.c0 = 1; h.c1 = 3; h.c2 = 5;
+    h.c0 = 1;
   } else if (e < H::E6) {  // z layers: D22, D02, D12
     const int q = e - H::E4, sd = q / (BX * BY), r = q % (BX * BY);
     lx = r / BY; ly = r % BY; dz = sd ? 1 : -1; lz = sd ? 0 : BZ - 1;
     gx = lx; gy = ly; gz = sd ? BZ : -1;
-    h.c0 = 2; h.c1 = 4; h.c2 = 5;
+    h.c0 = 2;
   } else if (e < H::E7) {  // z-parallel edges: D01
     const int q = e - H::E6, ed = q / BZ;
     const int sx = ed >> 1, sy = ed & 1;
     lz = q % BZ; dx = sx ? 1 : -1; dy = sy ? 1 : -1;
     lx = sx ? 0 : BX - 1; ly = sy ? 0 : BY - 1;
     gx = sx ? BX : -1; gy = sy ? BY : -1; gz = lz;
-    h.ncomp = 1; h.c0 = 3;
+    h.kind = 3; h.c0 = 3;
   } else if (e < H::E8) {  // y-parallel edges: D02
     const int q = e - H::E7, ed = q / BY;
     const int sx = ed >> 1, sz = ed & 1;
     ly = q % BY; dx = sx ? 1 : -1; dz = sz ? 1 : -1;
     lx = sx ? 0 : BX - 1; lz = sz ? 0 : BZ - 1;
     gx = sx ? BX : -1; gy = ly; gz = sz ? BZ : -1;
-    h.ncomp = 1; h.c0 = 4;
+    h.kind = 3; h.c0 = 4;
   } else if (e < H::NT) {  // x-parallel edges: D12
     const int q = e - H::E8, ed = q / BX;
     const int sy = ed >> 1, sz = ed & 1;
     lx = q % BX; dy = sy ? 1 : -1; dz = sz ? 1 : -1;
     ly = sy ? 0 : BY - 1; lz = sz ? 0 : BZ - 1;
     gx = lx; gy = sy ? BY : -1; gz = sz ? BZ : -1;
-    h.ncomp = 1; h.c0 = 5;
+    h.kind = 3; h.c0 = 5;
   } else {
-    h.tile = -1;
-    h.dir = 13;
-    h.local = 0;
-    h.ncomp = 0;
+    h.kind = 0;
     return h;
   }
   h.tile = ((gx + 1) * TY + (gy + 1)) * TZ + (gz + 1);
   h.dir = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
   h.local = (lx * BY + ly) * BZ + lz;
   return h;
+}
+
+// The staging plan of every (iteration, lane), packed as
+// tile | dir << 10 | local << 15 | kind << 24 | c0 << 26, built at compile time
+// and read from constant memory: the kernels decode one word per iteration
+// instead of evaluating the case analysis above.
+template <int BX, int BY, int BZ>
+struct PlanTable {
+  static constexpr int N = HaloPlan<BX, BY, BZ>::NIT * BX * BY * BZ;
+  uint32_t w[N];
+  constexpr PlanTable() : w() {
+    for (int e = 0; e < N; ++e) {
+      const HaloCell h = halo_cell<BX, BY, BZ>(e);
+      w[e] = h.kind == 0 ? 0u
+                         : (uint32_t)h.tile | ((uint32_t)h.dir << 10) | ((uint32_t)h.local << 15) |
+                               ((uint32_t)h.kind << 24) | ((uint32_t)h.c0 << 26);
+    }
+  }
+};
+template <int BX, int BY, int BZ>
+__constant__ PlanTable<BX, BY, BZ> kPlan = PlanTable<BX, BY, BZ>();
+
+// Element idx of a uniform base pointer, addressed as base + 32-bit byte
+// offset (the global "saddr + voffset" form: no 64-bit address arithmetic per
+// lane).  Callers keep idx * sizeof(T) below 2^32 (checked at launch).
+template <typename T>
+__device__ __forceinline__ T ld32(const T* base, uint32_t idx) {
+  return *(const T*)((const char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* base, uint32_t idx, T v) {
+  *(T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T))) = v;
 }
 
 // Stage the halo tile of brick `b` into LDS, in two phases: stage_nbr puts
@@ -143,77 +172,73 @@ __device__ __forceinline__ void stage_nbr(Tile<T, BX, BY, BZ>& t, const hp_brick
 
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ void stage_cells(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
-                                            const T* __restrict__ V, int64_t b, int64_t ld,
-                                            bool need_D, uint8_t mylab) {
+                                            const T* __restrict__ V, uint32_t b, uint32_t ld,
+                                            uint8_t mylab) {
   typedef Tile<T, BX, BY, BZ> TT;
   typedef HaloPlan<BX, BY, BZ> H;
   constexpr int NIT = H::NIT;
+  constexpr uint32_t OOB = 0x1FFFFFFFu;  // element index past every bound: loads return 0
   const int tid = threadIdx.x;
-  const T* __restrict__ D6 = (const T*)g.D6;
-  HaloCell hc[NIT];
-  int64_t sl[NIT];
+  const uint32_t n_slots = (uint32_t)g.n_bricks * TT::BVOL;
+  const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
+  const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
+  const buf_t rL = buf_rsrc(g.label, n_slots);
+  uint32_t pw[NIT], q[NIT];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    hc[it] = halo_cell<BX, BY, BZ>(it * TT::BVOL + tid);
-    sl[it] = -1;
-    if (hc[it].tile >= 0) {
-      const int nb = it == 0 ? (mylab != 0xFF ? (int)b : -1) : t.nbr[hc[it].dir];
-      if (nb >= 0) sl[it] = (int64_t)nb * TT::BVOL + hc[it].local;
-    }
+    pw[it] = kPlan<BX, BY, BZ>.w[it * TT::BVOL + tid];
+    const int kind = (pw[it] >> 24) & 3;
+    const int nb = it == 0 ? (mylab != 0xFF ? (int)b : -1)
+                           : (kind ? t.nbr[(pw[it] >> 10) & 31] : -1);
+    q[it] = nb >= 0 ? (uint32_t)nb * TT::BVOL + ((pw[it] >> 15) & 511) : OOB;
   }
-  uint8_t lab[NIT];
+  // all loads first (cells without a brick read out of range: 0, no traffic)
+  uint32_t lab[NIT];
   T v[NIT], d[NIT][3], dself[3];
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    lab[it] = 0xFF;
-    if (sl[it] >= 0) {
-      const int64_t q = sl[it];
-      lab[it] = g.label[q];
-      v[it] = V[q];
-      if (need_D) {
-        if (it == 0) {  // own brick: all six components
+    lab[it] = buf_ld_u8(rL, q[it]);
+    v[it] = buf_ld<T>(rV, q[it]);
+    if (it == 0) {  // own brick: all six components
 #pragma unroll
-          for (int k = 0; k < 3; ++k) d[0][k] = D6[k * ld + q];
+      for (int k = 0; k < 3; ++k) d[0][k] = buf_ld<T>(rD, q[0] == OOB ? OOB : k * ld + q[0]);
 #pragma unroll
-          for (int k = 0; k < 3; ++k) dself[k] = D6[(3 + k) * ld + q];
-        } else {
-          d[it][0] = D6[hc[it].c0 * ld + q];
-          if (hc[it].ncomp == 3) {
-            d[it][1] = D6[hc[it].c1 * ld + q];
-            d[it][2] = D6[hc[it].c2 * ld + q];
-          }
-        }
-      }
+      for (int k = 0; k < 3; ++k)
+        dself[k] = buf_ld<T>(rD, q[0] == OOB ? OOB : (3 + k) * ld + q[0]);
+    } else {
+      const uint32_t c0 = (pw[it] >> 26) & 7;
+      const bool face = ((pw[it] >> 24) & 3) == 2 && q[it] != OOB;
+      const uint32_t c1 = c0 == 2 ? 4u : 3u, c2 = c0 == 0 ? 4u : 5u;
+      d[it][0] = buf_ld<T>(rD, q[it] == OOB ? OOB : c0 * ld + q[it]);
+      d[it][1] = buf_ld<T>(rD, face ? c1 * ld + q[it] : OOB);  // face layers: the two
+      d[it][2] = buf_ld<T>(rD, face ? c2 * ld + q[it] : OOB);  // off-diagonals of its planes
     }
   }
+  // then the LDS writes; lanes without a tile cell write the spare cell N
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
-    const int i = hc[it].tile;
-    if (i >= 0) {
-      t.L[i] = lab[it];
-      if (lab[it] != 0xFF) {
-        t.V[i] = v[it];
-        if (need_D) {
-          if (it == 0) {
+    const int kind = (pw[it] >> 24) & 3;
+    const int i = kind ? (int)(pw[it] & 1023) : TT::N;
+    t.L[i] = q[it] == OOB ? (uint8_t)0xFF : (uint8_t)lab[it];
+    t.V[i] = v[it];
+    if (it == 0) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) t.D[k][i] = d[0][k];
+      for (int k = 0; k < 3; ++k) t.D[k][i] = d[0][k];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) t.D[3 + k][i] = dself[k];
-          } else {
-            t.D[hc[it].c0][i] = d[it][0];
-            if (hc[it].ncomp == 3) {
-              t.D[hc[it].c1][i] = d[it][1];
-              t.D[hc[it].c2][i] = d[it][2];
-            }
-          }
-        }
-      }
+      for (int k = 0; k < 3; ++k) t.D[3 + k][i] = dself[k];
+    } else {
+      const int c0 = (pw[it] >> 26) & 7;
+      const int i12 = kind == 2 ? i : TT::N;
+      t.D[c0 == 2 ? 4 : 3][i12] = d[it][1];
+      t.D[c0 == 0 ? 4 : 5][i12] = d[it][2];
+      t.D[c0][i] = d[it][0];
     }
   }
   __syncthreads();
 }
 
 // Sum over the 18 neighbours of w_ab*(V_b - V_a)/h, in units of D (mS/mm)*mV.
+// Terms of absent neighbours (label 0xFF) are skipped.
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
   typedef Tile<T, BX, BY, BZ> TT;
@@ -260,26 +285,33 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
 // kReact = true: do_reaction 1 (TTP06 every voxel) and 2 (TTP06 where rate_ext
 // is NaN); kReact = false: do_reaction 0 (no TTP06; dV/dt = rate_ext if given),
 // a lighter specialisation for the multirate substeps.
+// FP32: at least six waves per SIMD (at most 80 VGPRs), so that the state and
+// halo loads of several bricks are in flight together.
 template <typename T, int BX, int BY, int BZ, bool kReact>
-__global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
+__global__ void __launch_bounds__(BX * BY * BZ)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) myo_step_kernel(
     hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
-    int64_t ld, hp_mono_params p) {
+    int64_t ld64, hp_mono_params p) {
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t ld = (uint32_t)ld64;
   const int tid = threadIdx.x;
-  const int64_t slot = b * TT::BVOL + tid;
+  const uint32_t slot = b * TT::BVOL + tid;
   // every-voxel reaction: the 18 state loads of a tissue voxel are issued
   // before the halo staging loads, so that both are in flight together
   const bool pre = kReact && p.do_reaction == 1;
   const uint8_t mylab = g.label[slot];
   stage_nbr<T, BX, BY, BZ>(t, g, b);
   T s[N_STATE];
-  if (pre && mylab != 0xFF) {
+  const uint32_t row_bytes = ld * (uint32_t)sizeof(T);
+  if (pre) {  // padding lanes read out of range: 0, no memory traffic
+    const uint32_t sidx = mylab != 0xFF ? slot : 0x1FFFFFFFu;
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+    for (int k = 0; k < N_STATE; ++k)
+      s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
   }
-  stage_cells<T, BX, BY, BZ>(t, g, V_in, b, ld, true, mylab);
+  stage_cells<T, BX, BY, BZ>(t, g, V_in, b, ld, mylab);
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
   const uint8_t lab = t.L[c];
@@ -290,7 +322,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
   T dvdt = T(0);  // reaction part of dV/dt
   bool full = kReact && p.do_reaction == 1;
   if (kReact && p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
-    const T r = ((const T*)p.rate_ext)[slot];
+    const T r = ld32((const T*)p.rate_ext, slot);
     full = r != r;
     if (!full) dvdt = r;
   }
@@ -304,29 +336,30 @@ __global__ void __launch_bounds__(BX * BY * BZ) myo_step_kernel(
     }
     if (!pre) {
 #pragma unroll
-      for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
+      for (int k = 0; k < N_STATE; ++k)
+        s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), slot);
     }
     constexpr bool exact = sizeof(T) == 8;
     dvdt = -ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
+    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), slot, s[k]);
   }
-  if (!kReact && p.rate_ext) dvdt = ((const T*)p.rate_ext)[slot];
+  if (!kReact && p.rate_ext) dvdt = ld32((const T*)p.rate_ext, slot);
   int grp = (lab >> 2) & 31;
   if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
   Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
-  V_out[slot] = Vn;
+  st32(V_out, slot, Vn);
   // PMJ target voxels (label bit 7) are tested after the PMJ exchange instead
   if (p.at && !(lab & 0x80)) {
     const T thr = T(p.at_thr);
     if (Vc < thr && Vn >= thr) {
       T* at = (T*)p.at;
-      const T a = at[slot];
+      const T a = ld32((const T*)at, slot);
       if (a != a) {
         const int64_t step = *p.step_ctr;
-        at[slot] = (T)((double)(step - 1) * p.dt) + dt * (thr - Vc) / (Vn - Vc);
+        st32(at, slot, (T)((double)(step - 1) * p.dt) + dt * (thr - Vc) / (Vn - Vc));
       }
     }
   }
@@ -339,15 +372,15 @@ __global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
                                                                  double dscale) {
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
-  const int64_t b = xcd_remap(blockIdx.x, gridDim.x);
-  const uint8_t mylab = g.label[b * TT::BVOL + threadIdx.x];
+  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t slot = b * TT::BVOL + threadIdx.x;
+  const uint8_t mylab = g.label[slot];
   stage_nbr<T, BX, BY, BZ>(t, g, b);
-  stage_cells<T, BX, BY, BZ>(t, g, V, b, ld, true, mylab);
+  stage_cells<T, BX, BY, BZ>(t, g, V, b, (uint32_t)ld, mylab);
   const int tid = threadIdx.x;
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
-  const int64_t slot = b * TT::BVOL + tid;
-  out[slot] = t.L[c] == 0xFF ? T(0) : T(dscale) * stencil19<T, BX, BY, BZ>(t, c);
+  st32(out, slot, t.L[c] == 0xFF ? T(0) : T(dscale) * stencil19<T, BX, BY, BZ>(t, c));
 }
 
 template <typename T, template <typename, int, int, int> class K, typename... A>
@@ -357,6 +390,13 @@ static int launch_bricks(const hp_bricks* g, const char* what, hipStream_t st, A
     return -1;
   }
   if (g->n_bricks == 0) return 0;
+  // the kernels address every per-slot array (and the six D rows together)
+  // with 32-bit byte offsets
+  if ((uint64_t)g->n_bricks * g->bx * g->by * g->bz * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("%s: %lld bricks exceed the 32-bit offset range of one launch", what,
+              (long long)g->n_bricks);
+    return -1;
+  }
   const dim3 grid((unsigned)g->n_bricks);
   const int key = g->bx * 100 + g->by * 10 + g->bz;
   switch (key) {
@@ -403,14 +443,17 @@ __global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __
   T dvdt = T(0);
   if (p.do_reaction) {
     const int ct = lab & 3;
+    const uint32_t row_bytes = (uint32_t)ld * (uint32_t)sizeof(T);
     T s[N_STATE];
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + i];
+    for (int k = 0; k < N_STATE; ++k)
+      s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), (uint32_t)i);
     constexpr bool exact = sizeof(T) == 8;
     dvdt = -ttp06_cell<T, exact>(Vc, s, dt, T(kGKs[ct]), T(kGto[ct]), ct != 2, T(p.ki_base),
                                  T(p.nai_base));
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) state[k * ld + i] = s[k];
+    for (int k = 0; k < N_STATE; ++k)
+      buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), (uint32_t)i, s[k]);
   }
   const int grp = (lab >> 2) & 31;
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
@@ -458,7 +501,7 @@ __device__ __forceinline__ void pmj_junction(const hp_pmj& j, int64_t q, T* __re
 template <typename T>
 __device__ __forceinline__ void pmj_target(const hp_pmj& j, int64_t q, T* __restrict__ V_myo,
                                            const T* __restrict__ scratch, double vmin, double vmax,
-                                           const hp_pmj_at& at) {
+                                           const hp_pmj_at& at, int64_t step) {
   T acc = T(0);
   for (int k = j.tgt_ptr[q]; k < j.tgt_ptr[q + 1]; ++k) acc += scratch[j.tgt_entry[k]];
   const int s = j.tgt_slot[q];
@@ -469,10 +512,8 @@ __device__ __forceinline__ void pmj_target(const hp_pmj& j, int64_t q, T* __rest
     const T v0 = ((const T*)at.V_prev)[s];
     const T thr = T(at.at_thr);
     T* a = (T*)at.at;
-    if (v0 < thr && v >= thr && a[s] != a[s]) {
-      const int64_t step = *at.step_ctr;
+    if (v0 < thr && v >= thr && a[s] != a[s])
       a[s] = (T)((double)(step - 1) * at.dt) + T(at.dt) * (thr - v0) / (v - v0);
-    }
   }
 }
 
@@ -488,7 +529,7 @@ template <typename T>
 __global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __restrict__ scratch,
                                   double vmin, double vmax, hp_pmj_at at) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < j.n_targets) pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at);
+  if (q < j.n_targets) pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at, at.at ? *at.step_ctr : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,10 +575,12 @@ __global__ void __launch_bounds__(STIM_THREADS) stim_kernel(const double* __rest
   if (advance && threadIdx.x == 0) *step_ctr = step;
 }
 
-// One workgroup finishes the step: PMJ junctions, PMJ targets, then the
-// step counter and the stimulus amplitudes of the next step (replaces three
-// latency-bound launches; used while the junction count fits one block well).
+// One workgroup finishes the step: the PMJ junctions and, concurrently in the
+// last TAIL_STIM_THREADS lanes, the stimulus amplitudes of the next step; then
+// the PMJ targets and the step counter (replaces three latency-bound launches;
+// used while the junction count fits one block well).
 constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_STIM_THREADS = 128;
 constexpr int64_t TAIL_MAX_JUNCTIONS = 65536;
 
 template <typename T>
@@ -545,21 +588,25 @@ __global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
     hp_pmj j, int has_pmj, T* __restrict__ V_pk, T* __restrict__ V_myo, T* __restrict__ scratch,
     double dt, double vmin, double vmax, hp_pmj_at at, const double* __restrict__ sp,
     const uint32_t* __restrict__ gs, int n_groups, int64_t* step_ctr, T* __restrict__ amp) {
+  constexpr int JT = TAIL_THREADS - TAIL_STIM_THREADS;
   const int tid = threadIdx.x;
-  if (has_pmj) {
-    for (int64_t q = tid; q < j.n_junctions; q += TAIL_THREADS)
-      pmj_junction<T>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax);
-    __syncthreads();
-    for (int64_t q = tid; q < j.n_targets; q += TAIL_THREADS)
-      pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at);
-    __syncthreads();
+  // every lane reads the counter before the barrier below; lane 0 advances it
+  // only after that barrier
+  const int64_t step = *step_ctr;
+  if (tid < JT) {
+    if (has_pmj)
+      for (int64_t q = tid; q < j.n_junctions; q += JT)
+        pmj_junction<T>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax);
+  } else {
+    const double t = (double)(step + 1) * dt;
+    for (int g = tid - JT; g < n_groups; g += TAIL_STIM_THREADS)
+      amp[g] = g == 0 ? T(0) : (T)group_amplitude(g, sp, gs, t);
   }
-  const int64_t step = *step_ctr + 1;
   __syncthreads();
-  const double t = (double)step * dt;
-  for (int g = tid; g < n_groups; g += TAIL_THREADS)
-    amp[g] = g == 0 ? T(0) : (T)group_amplitude(g, sp, gs, t);
-  if (tid == 0) *step_ctr = step;
+  if (has_pmj)
+    for (int64_t q = tid; q < j.n_targets; q += TAIL_THREADS)
+      pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at, step);
+  if (tid == 0) *step_ctr = step + 1;
 }
 
 template <typename T>
@@ -580,6 +627,11 @@ static int myo_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64
     set_error("hp_myo_step: do_reaction must be 0, 1 or 2 (2 needs rate_ext)");
     return -1;
   }
+  if (!g || ld < g->n_bricks * g->bx * g->by * g->bz ||
+      (uint64_t)ld * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_myo_step: ld must cover every slot and 6*ld elements fit 32-bit offsets");
+    return -1;
+  }
   if (p->do_reaction == 0)
     return launch_bricks<T, MyoDiffK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state,
                                       ld, *p);
@@ -597,8 +649,9 @@ __global__ void __launch_bounds__(256) fallback_react_kernel(
     float* __restrict__ rate, const int32_t* __restrict__ list, const int32_t* __restrict__ count,
     hp_mono_params p) {
   const int n = *count;
+  const uint32_t row_bytes = (uint32_t)ld * 4u;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int64_t slot = list[i];
+    const uint32_t slot = (uint32_t)list[i];
     const int ct = g.label[slot] & 3;
     float gks = (float)kGKs[ct], gto = (float)kGto[ct];
     if (p.inst_params && g.inst) {
@@ -608,20 +661,22 @@ __global__ void __launch_bounds__(256) fallback_react_kernel(
     }
     float s[N_STATE];
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) s[k] = state[k * ld + slot];
-    const float I = ttp06_cell<float, false>(V[slot], s, (float)p.dt, gks, gto, ct != 2,
+    for (int k = 0; k < N_STATE; ++k)
+      s[k] = buf_ld<float>(buf_rsrc(state + k * (size_t)ld, row_bytes), slot);
+    const float I = ttp06_cell<float, false>(ld32(V, slot), s, (float)p.dt, gks, gto, ct != 2,
                                              (float)p.ki_base, (float)p.nai_base);
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) state[k * ld + slot] = s[k];
-    rate[slot] = -I;
+    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), slot, s[k]);
+    st32(rate, slot, -I);
   }
 }
 
 template <typename T>
 static int cable_step(const hp_cable* c, const T* V_in, T* V_out, T* state, int64_t ld,
                       const hp_mono_params* p, void* stream) {
-  if (!c || !p || c->n < 0 || !p->group_amp) {
-    set_error("hp_cable_step: bad arguments");
+  if (!c || !p || c->n < 0 || !p->group_amp || ld < c->n ||
+      (uint64_t)ld * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_cable_step: bad arguments (ld must cover the nodes and fit 32-bit offsets)");
     return -1;
   }
   if (c->n == 0) return 0;
@@ -714,7 +769,8 @@ int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, doubl
 int hp_fallback_react_f32(const hp_bricks* g, const float* V, float* state, int64_t ld,
                           float* rate, const int32_t* list, const int32_t* count,
                           int64_t max_count, const hp_mono_params* p, void* stream) {
-  if (!g || !p || !list || !count || !rate || max_count < 0) {
+  if (!g || !p || !list || !count || !rate || max_count < 0 ||
+      ld < g->n_bricks * g->bx * g->by * g->bz || (uint64_t)ld * 4 >= (1ull << 32)) {
     set_error("hp_fallback_react: bad arguments");
     return -1;
   }

@@ -87,7 +87,11 @@ int hp_ttp06_step_host_f64(const double* V, double* state, double* I_ion,
  *   label  uint8: 0xFF = not myocardium; else bits 0-1 phenotype code,
  *          bits 2-6 stimulus group (0 = none), bit 7 = PMJ target voxel
  *   D6     conductivity tensor D00,D11,D22,D01,D02,D12 [mS/mm], SoA [6][n_slots]
+ *          (row stride ld = the slot count passed with the state)
  *   inst   uint16 ensemble instance (or NULL: instance 0)
+ * Padding slots carry label 0xFF and D6 = 0.  Every per-slot array, and the
+ * six D6 rows together, must fit 32-bit byte offsets (6*ld*sizeof(T) < 2^32);
+ * the entry points reject larger grids.
  * nbr[b*27 + (dx+1)*9 + (dy+1)*3 + (dz+1)] is the brick at offset (dx,dy,dz),
  * or -1.
  *
@@ -435,6 +439,11 @@ typedef struct hp_surrogate_params {
    * the caller zeroes before the reaction call.  List order is unspecified. */
   int32_t* fb_list;
   int32_t* fb_count;
+  /* Optional list of the slots the inference kernel visits (NULL = every
+   * slot): n_list entries, a multiple of 32, -1 = no voxel.  Listing only the
+   * tissue slots skips the padding of partly filled bricks. */
+  const int32_t* slot_list;
+  int64_t n_list;
 } hp_surrogate_params;
 
 /* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),

@@ -86,9 +86,10 @@ def test_gradients_match_cpu_autograd(cuda):
         assert relb < 5e-2
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_inference_kernel_matches_cpu_forward(cuda, variant):
-    print(f"[case] fused surrogate inference, kernel variant {variant}", flush=True)
+@pytest.mark.parametrize("variant,listed", [(0, False), (1, False), (1, True)])
+def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
+    print(f"[case] fused surrogate inference, kernel variant {variant}, slot list {listed}",
+          flush=True)
     from hrdayapy_amd.layout import BrickLayout
     S = np.zeros((12, 10, 9), bool)
     S[1:11, 1:9, 1:8] = True
@@ -121,6 +122,12 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant):
     br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None)
     rate = torch.zeros(lay.n_slots, device=cuda)
     sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, variant)
+    if listed:  # the tissue slots in a shuffled order, padded with -1 to a multiple of 32
+        tis = rng.permutation(lay.slot_of_node.astype(np.int32))
+        lst = np.full((len(tis) + 31) // 32 * 32, -1, np.int32)
+        lst[:len(tis)] = tis
+        lst_d = torch.from_numpy(lst).to(cuda)
+        sp.slot_list, sp.n_list = lst_d.data_ptr(), len(lst)
     import ctypes
     nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
              state.data_ptr(), lay.n_slots, rate.data_ptr(), ctypes.byref(net.hp_w),
