@@ -61,13 +61,14 @@ def build_geometry(n=256, h=0.4, seed=0):
     return geo, (nodes, el, act), kw, time.perf_counter() - t0
 
 
-def build_engine(geo, tree, kw, dtype=None, brick=None):
+def build_engine(geo, tree, kw, dtype=None, brick=None, alloc="torch"):
     import torch
     from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
         build_coupled_system)
     nodes, el, act = tree
     return build_coupled_system(nodes, el, act, geo["S"], geo["phi"], geo["fibre_dir"], dt=0.02,
-                                dtype=dtype or torch.float32, device="cuda", brick=brick, **kw)
+                                dtype=dtype or torch.float32, device="cuda", brick=brick,
+                                alloc=alloc, **kw)
 
 
 def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None):
@@ -145,17 +146,9 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
 
 
 def _event_us(fn, reps):
-    """Average device time of fn() over reps calls, HIP events on the current stream."""
-    import torch
-    st = torch.cuda.current_stream()
-    fn()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for _ in range(reps):
-        fn()
-    e1.record(st)
-    e1.synchronize()
-    return round(e0.elapsed_time(e1) / reps * 1e3, 2)
+    """Average device time (us) of fn() over reps calls, replayed from a HIP graph."""
+    from hrdayapy_amd import workloads as W
+    return round(W.graph_us(lambda i: fn(), reps), 2)
 
 
 def stream_copy_bandwidth(mib, reps=20):
@@ -165,34 +158,28 @@ def stream_copy_bandwidth(mib, reps=20):
     n = mib * (1 << 20) // 4
     x = torch.ones(n, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
-    s = torch.cuda.current_stream()
     us = _event_us(lambda: nat.call("hp_stream_copy_f32", nat.ptr(x), nat.ptr(y), n,
-                                    s.cuda_stream), reps)
+                                    nat.stream_handle()), reps)
     del x, y
     torch.cuda.empty_cache()
     return dict(mib=mib, us=us, GBs=2 * n * 4 / (us * 1e-6) / 1e9)
 
 
 def time_myo_kernel(eng, reps=50):
-    """Average duration of the fused myocardium kernel, HIP events on its stream."""
-    import torch
+    """Average duration of the fused myocardium kernel: reps launches (ping-pong
+    V buffers, advancing the state) replayed from a HIP graph, HIP events on
+    the replaying stream."""
     from hrdayapy_amd import _native as nat
-    st = torch.cuda.current_stream()
-    s = st.cuda_stream
+    from hrdayapy_amd import workloads as W
     a, b = eng.p, 1 - eng.p
-    args = lambda i: (C.byref(eng.bricks), nat.ptr(eng.V[(a + i) % 2]),
-                      nat.ptr(eng.V[(b + i) % 2]), nat.ptr(eng.state), eng.lay.n_slots,
-                      C.byref(eng.p_myo), s)
     fn = f"hp_myo_step_{eng.sfx}"
-    for i in range(4):
-        nat.call(fn, *args(i))
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(st)
-    for i in range(reps):
-        nat.call(fn, *args(i))
-    e1.record(st)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps * 1e-3  # s per launch
+
+    def launch(i):
+        nat.call(fn, C.byref(eng.bricks), nat.ptr(eng.V[(a + i) % 2]),
+                 nat.ptr(eng.V[(b + i) % 2]), nat.ptr(eng.state), eng.lay.n_slots,
+                 C.byref(eng.p_myo), nat.stream_handle())
+
+    return W.graph_us(launch, reps) * 1e-6  # s per launch
 
 
 def cpu_baseline(args_geo, kw, n_steps, dtype_name="float32"):

@@ -39,7 +39,7 @@ class HpPmj(C.Structure):
     _fields_ = [("n_junctions", c_i64), ("n_entries", c_i64), ("pk", c_vp), ("ptr", c_vp),
                 ("patch", c_vp), ("n_targets", c_i64), ("tgt_slot", c_vp), ("tgt_ptr", c_vp),
                 ("tgt_entry", c_vp), ("c_pmj", c_f64), ("ai_p", c_f64), ("ai_m", c_f64),
-                ("cm", c_f64), ("coef", c_vp)]
+                ("cm", c_f64), ("coef", c_vp), ("exclusive", c_i32), ("pad_", c_i32)]
 
 
 class HpPmjAt(C.Structure):
@@ -93,6 +93,8 @@ P = C.POINTER
 _SIGS = {
     "hp_abi_version": (C.c_int, []),
     "hp_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "hp_device_alloc": (C.c_int, [C.POINTER(c_vp), c_i64, C.c_int]),
+    "hp_device_free": (C.c_int, [c_vp]),
     "hp_ttp06_step_f32": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
                                     c_i64, c_f32, c_f32, c_f32, c_vp]),
     "hp_ttp06_step_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
@@ -224,3 +226,65 @@ def sfx(dtype) -> str:
     if dtype == torch.float64:
         return "f64"
     raise ValueError(f"unsupported dtype {dtype}; the HIP kernels take float32 or float64")
+
+
+class DeviceArena:
+    """One device allocation carved into typed views, each 2 MiB aligned.
+
+    ``specs``: list of (name, shape, dtype).  With ``contiguous`` the memory
+    comes from hp_device_alloc(..., contiguous=1) (physically contiguous pages)
+    and is freed when the arena is collected -- the views must not outlive it
+    (the engine that owns the arena also owns the views); otherwise it is one
+    torch allocation.  Views are available as attributes / ``arena[name]``.
+    """
+
+    ALIGN = 2 << 20
+
+    def __init__(self, specs, device, contiguous=False):
+        import torch
+        self._ptr = None
+        offs, total = [], 0
+        for _, shape, dtype in specs:
+            total = -(-total // self.ALIGN) * self.ALIGN
+            offs.append(total)
+            n = 1
+            for d in (shape if isinstance(shape, tuple) else (shape,)):
+                n *= int(d)
+            total += n * torch.empty((), dtype=dtype).element_size()
+        total = max(total, 1)
+        self.nbytes = total
+        if contiguous:
+            load()
+            p = c_vp()
+            call("hp_device_alloc", C.byref(p), total, 1)
+            self._ptr = p.value
+
+            class _CAI:
+                __cuda_array_interface__ = {"shape": (total,), "typestr": "|u1",
+                                            "data": (p.value, False), "version": 2,
+                                            "strides": None}
+            buf = torch.as_tensor(_CAI(), device=device)
+        else:
+            buf = torch.empty(total, dtype=torch.uint8, device=device)
+        self._buf = buf
+        self._views = {}
+        for (name, shape, dtype), o in zip(specs, offs):
+            shape = shape if isinstance(shape, tuple) else (shape,)
+            n = 1
+            for d in shape:
+                n *= int(d)
+            es = torch.empty((), dtype=dtype).element_size()
+            self._views[name] = buf[o:o + n * es].view(dtype).view(shape)
+
+    def __getitem__(self, name):
+        return self._views[name]
+
+    def __del__(self):
+        if self._ptr is not None:
+            try:
+                import torch
+                torch.cuda.synchronize()
+                call("hp_device_free", self._ptr)
+            except Exception:
+                pass
+            self._ptr = None

@@ -34,7 +34,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
                          stim_regions=None, stim_len_mm=2.0, c_pmj=0.05, n_pmj=1,
                          phi_endo_max=0.35, phi_epi_min=0.65, dtype=torch.float32,
                          device="cuda", brick=None, verbose=False, sigma_scale=None,
-                         inst=None, inst_params=None, auto_substep=False):
+                         inst=None, inst_params=None, auto_substep=False, alloc="torch"):
     """Set up the tissue on the host and upload it into a MonodomainEngine.
 
     ``sigma_M`` selects the isotropic 6-point operator (simulation.compute_coupled);
@@ -75,7 +75,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
         lay, D6, ct, h=voxel_size, A_M=A_M, Cm=Cm, dt=dt / n_sub, dtype=dtype, device=device,
         cable=dict(n=Np, edges=comp_edges, w_pp=pc["w_pp"], Ai_P=pc["Ai_P"]),
         pmj=dict(ids=pmj_ids, patches=patches, c_pmj=c_pmj, Ai_P=pc["Ai_P"], Ai_M=pc["Ai_M"]),
-        stim_sites=sites, inst=inst, inst_params=inst_params)
+        stim_sites=sites, inst=inst, inst_params=inst_params, alloc=alloc)
     meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim,
                 comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,

@@ -29,6 +29,32 @@ int check_launch(const char* what) {
 
 extern "C" int hp_abi_version(void) { return HP_ABI_VERSION; }
 
+extern "C" int hp_device_alloc(void** ptr, int64_t bytes, int contiguous) {
+  if (!ptr || bytes <= 0) {
+    hp::set_error("hp_device_alloc: bad arguments");
+    return -1;
+  }
+  const hipError_t e =
+      hipExtMallocWithFlags(ptr, (size_t)bytes, contiguous ? hipDeviceMallocContiguous
+                                                           : hipDeviceMallocDefault);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    hp::set_error("hp_device_alloc: %lld bytes: %s", (long long)bytes, hipGetErrorString(e));
+    return -(int)e;
+  }
+  return 0;
+}
+
+extern "C" int hp_device_free(void* ptr) {
+  if (!ptr) return 0;
+  const hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) {
+    hp::set_error("hp_device_free: %s", hipGetErrorString(e));
+    return -(int)e;
+  }
+  return 0;
+}
+
 extern "C" int hp_last_error(char* buf, size_t size) {
   if (!buf || size == 0) return -1;
   strncpy(buf, hp::g_err, size - 1);

@@ -214,17 +214,23 @@ __device__ __forceinline__ float raw_feature(int f, const float* __restrict__ V,
   return (f - 19) == ct ? 1.f : 0.f;
 }
 
-// bias + SiLU of two C tiles (rows 4*grp..+4 of output tile m) into the B
-// fragment positions (m&1)*4..+4 of K-step m/2
-__device__ __forceinline__ void silu_pack(const floatx4& c, const float* bias, int m, int grp,
-                                          bf16x8& dst) {
+// The bias of rows 4*grp..+4 of output tile m: the accumulator's initial value
+__device__ __forceinline__ floatx4 bias4(const float* bias, int m, int grp) {
+  const float* b = bias + m * 16 + grp * 4;
+  return floatx4{b[0], b[1], b[2], b[3]};
+}
+
+// Scaled SiLU of a C tile (rows 4*grp..+4 of output tile m; bias already in
+// the accumulator) into the B fragment positions (m&1)*4..+4 of K-step m/2.
+// The weights are pre-scaled (see hp_mlp_weights): the accumulator holds
+// u = -log2(e)*z and the stored activation is g = u/(1 + 2^u) = -log2(e)*SiLU(z),
+// with the hardware exp2 and reciprocal (~1 ulp each, far below the bf16
+// rounding of g); 2^u -> inf gives g = 0 for z << 0.
+__device__ __forceinline__ void silu_pack(const floatx4& c, int m, bf16x8& dst) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const float z = c[q] + bias[m * 16 + grp * 4 + q];
-    // z * sigmoid(z) with the hardware exp2 and reciprocal (~1 ulp each, far
-    // below the bf16 rounding of the result); exp2 -> inf gives 0 for z << 0
-    dst[(m & 1) * 4 + q] =
-        f2bf(z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f)));
+    const float u = c[q];
+    dst[(m & 1) * 4 + q] = f2bf(u * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u)));
   }
 }
 
@@ -335,9 +341,9 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       const bf16x8 a = *(const bf16x8*)&S.w1[m * 16 + col][grp * 8];
-      const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+      const floatx4 z = bias4(S.b1, m, grp);
 #pragma unroll
-      for (int t = 0; t < NC; ++t) silu_pack(mfma16(a, xin[t], z), S.b1, m, grp, ha[m >> 1][t]);
+      for (int t = 0; t < NC; ++t) silu_pack(mfma16(a, xin[t], z), m, ha[m >> 1][t]);
     }
     // ---- layers 2 and 3
 #pragma unroll
@@ -350,7 +356,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
       for (int m = 0; m < 8; ++m) {
         floatx4 c[NC];
 #pragma unroll
-        for (int t = 0; t < NC; ++t) c[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NC; ++t) c[t] = bias4(bias, m, grp);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const bf16x8 a = *(const bf16x8*)&W[m * 16 + col][j * 32 + grp * 8];
@@ -358,7 +364,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
           for (int t = 0; t < NC; ++t) c[t] = mfma16(a, hin[j][t], c[t]);
         }
 #pragma unroll
-        for (int t = 0; t < NC; ++t) silu_pack(c[t], bias, m, grp, hout[m >> 1][t]);
+        for (int t = 0; t < NC; ++t) silu_pack(c[t], m, hout[m >> 1][t]);
       }
     }
     // ---- output layer (19 of 32 rows used); input is `ha` after two swaps
@@ -367,7 +373,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     for (int m = 0; m < 2; ++m) {
       floatx4 c[NC];
 #pragma unroll
-      for (int t = 0; t < NC; ++t) c[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int t = 0; t < NC; ++t) c[t] = bias4(S.b4, m, grp);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bf16x8 a = *(const bf16x8*)&S.w4[m * 16 + col][j * 32 + grp * 8];
@@ -394,7 +400,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
         for (int q = 0; q < 4; ++q) {
           const int o = m * 16 + grp * 4 + q;
           if (o >= NOUT) continue;
-          const float d = (out[m][t][q] + S.b4[o]) * S.out_std[o] + S.out_mean[o];
+          const float d = out[m][t][q] * S.out_std[o] + S.out_mean[o];
           if (o == 0) {
             rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
           } else if (!fallback) {

@@ -40,7 +40,6 @@ struct Tile {
   T V[N + 1];
   T D[6][N + 1];
   uint8_t L[N + 1];
-  int32_t nbr[27];
 };
 
 // Cells of the halo tile that the 19-point stencil reads, enumerated so that
@@ -157,83 +156,128 @@ __device__ __forceinline__ void st32(T* base, uint32_t idx, T v) {
   *(T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T))) = v;
 }
 
-// Stage the halo tile of brick `b` into LDS, in two phases: stage_nbr puts
-// the brick's neighbour table in LDS; stage_cells then loads NIT cells per
-// lane, all global loads (label, V, D components) issued before the first LDS
-// write, so a workgroup waits for one memory round trip.  `mylab` is the
-// lane's own label (loaded by the caller before stage_nbr): padding voxels of
-// the brick itself load nothing.  Padding slots of neighbour bricks are read
-// (they hold V0 and D = 0) and masked by their label.
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void stage_nbr(Tile<T, BX, BY, BZ>& t, const hp_bricks& g, int64_t b) {
-  if (threadIdx.x < 27) t.nbr[threadIdx.x] = g.nbr[b * 27 + threadIdx.x];
-  __syncthreads();
+// Halo staging, split into a load half (global -> registers) and a store half
+// (registers -> LDS) so that a persistent kernel can issue the loads of its
+// next brick before it computes the current one.  Per lane and plan
+// iteration: the plan word, the slot of the cell (OOB for cells without a
+// brick: their loads return 0 without memory traffic), label, V and up to
+// three D components.  The lane's own cell (iteration 0) loads all six D
+// components; padding voxels of the brick itself load nothing (mylab 0xFF).
+// Padding slots of neighbour bricks are read (they hold V0 and D = 0) and
+// masked by their label.
+constexpr uint32_t OOB = 0x1FFFFFFFu;  // element index past every bound
+constexpr int MAX_GROUPS = 8192;       // stimulus groups (group_amp entries) at most
+
+template <int BX, int BY, int BZ>
+struct HaloNbr {
+  static constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
+  uint32_t pw[NIT];  // plan words of this lane
+  int nb[NIT];       // brick holding each cell, -1 = none (iteration 0: the brick itself)
+  bool valid;        // false: no brick (its loads read out of range)
+};
+
+// The lane's plan words and the neighbour-table entries they name (read from
+// the brick's 27-entry row; one cache line per brick).  Nothing here consumes
+// a loaded value, so a persistent kernel can issue these loads early and use
+// them an iteration later; valid = false issues out-of-range reads only.
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ void halo_nbr(HaloNbr<BX, BY, BZ>& n, const hp_bricks& g, uint32_t b,
+                                         bool valid = true) {
+  constexpr int BVOL = BX * BY * BZ;
+  const buf_t rN = buf_rsrc(g.nbr, (uint32_t)g.n_bricks * 27u * 4u);
+  n.valid = valid;
+#pragma unroll
+  for (int it = 0; it < HaloNbr<BX, BY, BZ>::NIT; ++it) {
+    n.pw[it] = kPlan<BX, BY, BZ>.w[it * BVOL + threadIdx.x];
+    const int kind = (n.pw[it] >> 24) & 3;
+    n.nb[it] = it == 0 ? (int)b
+                       : (kind ? __builtin_bit_cast(int, buf_ld<float>(
+                                     rN, valid ? b * 27u + ((n.pw[it] >> 10) & 31) : OOB))
+                               : -1);
+  }
 }
 
+template <typename T, int BX, int BY, int BZ>
+struct HaloRegs {
+  static constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
+  uint32_t pw[NIT], q[NIT], lab[NIT];
+  T v[NIT], d[NIT][3], dself[3];
+};
+
+// mylab: the lane's own label in brick n (0xFF: padding voxel, loads nothing)
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const HaloNbr<BX, BY, BZ>& n,
+                                          const hp_bricks& g, const T* __restrict__ V,
+                                          uint32_t ld, uint8_t mylab) {
+  constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT, BVOL = BX * BY * BZ;
+  const uint32_t n_slots = (uint32_t)g.n_bricks * BVOL;
+  const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
+  const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
+  const buf_t rL = buf_rsrc(g.label, n_slots);
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    h.pw[it] = n.pw[it];
+    const bool ok = n.valid && (it == 0 ? mylab != 0xFF : n.nb[it] >= 0);
+    h.q[it] = ok ? (uint32_t)n.nb[it] * BVOL + ((n.pw[it] >> 15) & 511) : OOB;
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const uint32_t q = h.q[it];
+    h.lab[it] = buf_ld_u8(rL, q);
+    h.v[it] = buf_ld<T>(rV, q);
+    if (it == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) h.d[0][k] = buf_ld<T>(rD, q == OOB ? OOB : k * ld + q);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) h.dself[k] = buf_ld<T>(rD, q == OOB ? OOB : (3 + k) * ld + q);
+    } else {
+      const uint32_t c0 = (h.pw[it] >> 26) & 7;
+      const bool face = ((h.pw[it] >> 24) & 3) == 2 && q != OOB;
+      const uint32_t c1 = c0 == 2 ? 4u : 3u, c2 = c0 == 0 ? 4u : 5u;
+      h.d[it][0] = buf_ld<T>(rD, q == OOB ? OOB : c0 * ld + q);
+      h.d[it][1] = buf_ld<T>(rD, face ? c1 * ld + q : OOB);  // face layers: the two
+      h.d[it][2] = buf_ld<T>(rD, face ? c2 * ld + q : OOB);  // off-diagonals of its planes
+    }
+  }
+}
+
+// Registers -> LDS tile; lanes without a tile cell write the spare cell N.
+// The caller places the barriers.
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void halo_store(Tile<T, BX, BY, BZ>& t,
+                                           const HaloRegs<T, BX, BY, BZ>& h) {
+  typedef Tile<T, BX, BY, BZ> TT;
+#pragma unroll
+  for (int it = 0; it < HaloPlan<BX, BY, BZ>::NIT; ++it) {
+    const int kind = (h.pw[it] >> 24) & 3;
+    const int i = kind ? (int)(h.pw[it] & 1023) : TT::N;
+    t.L[i] = h.q[it] == OOB ? (uint8_t)0xFF : (uint8_t)h.lab[it];
+    t.V[i] = h.v[it];
+    if (it == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t.D[k][i] = h.d[0][k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t.D[3 + k][i] = h.dself[k];
+    } else {
+      const int c0 = (h.pw[it] >> 26) & 7;
+      const int i12 = kind == 2 ? i : TT::N;
+      t.D[c0 == 2 ? 4 : 3][i12] = h.d[it][1];
+      t.D[c0 == 0 ? 4 : 5][i12] = h.d[it][2];
+      t.D[c0][i] = h.d[it][0];
+    }
+  }
+}
+
+// Whole staging of brick b for the one-brick-per-workgroup kernels.
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ void stage_cells(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
                                             const T* __restrict__ V, uint32_t b, uint32_t ld,
                                             uint8_t mylab) {
-  typedef Tile<T, BX, BY, BZ> TT;
-  typedef HaloPlan<BX, BY, BZ> H;
-  constexpr int NIT = H::NIT;
-  constexpr uint32_t OOB = 0x1FFFFFFFu;  // element index past every bound: loads return 0
-  const int tid = threadIdx.x;
-  const uint32_t n_slots = (uint32_t)g.n_bricks * TT::BVOL;
-  const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
-  const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
-  const buf_t rL = buf_rsrc(g.label, n_slots);
-  uint32_t pw[NIT], q[NIT];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    pw[it] = kPlan<BX, BY, BZ>.w[it * TT::BVOL + tid];
-    const int kind = (pw[it] >> 24) & 3;
-    const int nb = it == 0 ? (mylab != 0xFF ? (int)b : -1)
-                           : (kind ? t.nbr[(pw[it] >> 10) & 31] : -1);
-    q[it] = nb >= 0 ? (uint32_t)nb * TT::BVOL + ((pw[it] >> 15) & 511) : OOB;
-  }
-  // all loads first (cells without a brick read out of range: 0, no traffic)
-  uint32_t lab[NIT];
-  T v[NIT], d[NIT][3], dself[3];
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    lab[it] = buf_ld_u8(rL, q[it]);
-    v[it] = buf_ld<T>(rV, q[it]);
-    if (it == 0) {  // own brick: all six components
-#pragma unroll
-      for (int k = 0; k < 3; ++k) d[0][k] = buf_ld<T>(rD, q[0] == OOB ? OOB : k * ld + q[0]);
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-        dself[k] = buf_ld<T>(rD, q[0] == OOB ? OOB : (3 + k) * ld + q[0]);
-    } else {
-      const uint32_t c0 = (pw[it] >> 26) & 7;
-      const bool face = ((pw[it] >> 24) & 3) == 2 && q[it] != OOB;
-      const uint32_t c1 = c0 == 2 ? 4u : 3u, c2 = c0 == 0 ? 4u : 5u;
-      d[it][0] = buf_ld<T>(rD, q[it] == OOB ? OOB : c0 * ld + q[it]);
-      d[it][1] = buf_ld<T>(rD, face ? c1 * ld + q[it] : OOB);  // face layers: the two
-      d[it][2] = buf_ld<T>(rD, face ? c2 * ld + q[it] : OOB);  // off-diagonals of its planes
-    }
-  }
-  // then the LDS writes; lanes without a tile cell write the spare cell N
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const int kind = (pw[it] >> 24) & 3;
-    const int i = kind ? (int)(pw[it] & 1023) : TT::N;
-    t.L[i] = q[it] == OOB ? (uint8_t)0xFF : (uint8_t)lab[it];
-    t.V[i] = v[it];
-    if (it == 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t.D[k][i] = d[0][k];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t.D[3 + k][i] = dself[k];
-    } else {
-      const int c0 = (pw[it] >> 26) & 7;
-      const int i12 = kind == 2 ? i : TT::N;
-      t.D[c0 == 2 ? 4 : 3][i12] = d[it][1];
-      t.D[c0 == 0 ? 4 : 5][i12] = d[it][2];
-      t.D[c0][i] = d[it][0];
-    }
-  }
+  HaloNbr<BX, BY, BZ> n;
+  halo_nbr<BX, BY, BZ>(n, g, b);
+  HaloRegs<T, BX, BY, BZ> h;
+  halo_load<T, BX, BY, BZ>(h, n, g, V, ld, mylab);
+  halo_store<T, BX, BY, BZ>(t, h);
   __syncthreads();
 }
 
@@ -282,12 +326,11 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
   return acc;
 }
 
-// kReact = true: do_reaction 1 (TTP06 every voxel) and 2 (TTP06 where rate_ext
-// is NaN); kReact = false: do_reaction 0 (no TTP06; dV/dt = rate_ext if given),
-// a lighter specialisation for the multirate substeps.
+// do_reaction 1 (TTP06 every voxel) and 2 (TTP06 where rate_ext is NaN, else
+// dV/dt = rate_ext); do_reaction 0 runs diff_step_kernel instead.
 // FP32: at least six waves per SIMD (at most 80 VGPRs), so that the state and
 // halo loads of several bricks are in flight together.
-template <typename T, int BX, int BY, int BZ, bool kReact>
+template <typename T, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX * BY * BZ)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) myo_step_kernel(
     hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
@@ -300,9 +343,8 @@ __global__ void __launch_bounds__(BX * BY * BZ)
   const uint32_t slot = b * TT::BVOL + tid;
   // every-voxel reaction: the 18 state loads of a tissue voxel are issued
   // before the halo staging loads, so that both are in flight together
-  const bool pre = kReact && p.do_reaction == 1;
+  const bool pre = p.do_reaction == 1;
   const uint8_t mylab = g.label[slot];
-  stage_nbr<T, BX, BY, BZ>(t, g, b);
   T s[N_STATE];
   const uint32_t row_bytes = ld * (uint32_t)sizeof(T);
   if (pre) {  // padding lanes read out of range: 0, no memory traffic
@@ -320,13 +362,13 @@ __global__ void __launch_bounds__(BX * BY * BZ)
   const T lv = T(p.dscale) * stencil19<T, BX, BY, BZ>(t, c);
   const T dt = T(p.dt);
   T dvdt = T(0);  // reaction part of dV/dt
-  bool full = kReact && p.do_reaction == 1;
-  if (kReact && p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
+  bool full = p.do_reaction == 1;
+  if (p.do_reaction == 2) {  // surrogate form: NaN rate marks a fallback voxel
     const T r = ld32((const T*)p.rate_ext, slot);
     full = r != r;
     if (!full) dvdt = r;
   }
-  if (kReact && full) {
+  if (full) {
     const int ct = lab & 3;
     T gks = T(kGKs[ct]), gto = T(kGto[ct]);
     if (p.inst_params && g.inst) {
@@ -344,7 +386,6 @@ __global__ void __launch_bounds__(BX * BY * BZ)
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), slot, s[k]);
   }
-  if (!kReact && p.rate_ext) dvdt = ld32((const T*)p.rate_ext, slot);
   int grp = (lab >> 2) & 31;
   if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
@@ -365,6 +406,99 @@ __global__ void __launch_bounds__(BX * BY * BZ)
   }
 }
 
+// Multirate substep (do_reaction = 0): diffusion plus a given reaction rate
+// (dV/dt = rate_ext, or 0 without it), the same update as myo_step_kernel
+// without the cell model.  Persistent over bricks: the halo loads of the next
+// brick are issued before the current brick is stored and computed, and the
+// neighbour rows of the brick after that are fetched one iteration earlier,
+// so the two dependent memory round trips of a brick overlap the work on the
+// previous one.  Each XCD (blockIdx & 7) works through a contiguous run of
+// bricks (compact in Morton order); the blocks of one XCD take its bricks
+// round robin.
+template <typename T, int BX, int BY, int BZ, bool kInst>
+__global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, int64_t ld64,
+    hp_mono_params p) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  typedef HaloNbr<BX, BY, BZ> HN;
+  typedef HaloRegs<T, BX, BY, BZ> HR;
+  __shared__ TT t;
+  const uint32_t ld = (uint32_t)ld64, NB = (uint32_t)g.n_bricks;
+  const uint32_t xcd = blockIdx.x & 7, G = gridDim.x;
+  const uint32_t m = (G - xcd + 7) >> 3;  // blocks with index = xcd (mod 8)
+  const uint32_t base = NB >> 3, rem = NB & 7;
+  const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
+  const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
+  uint32_t b = lo + (blockIdx.x >> 3);
+  if (b >= hi) return;  // whole workgroup: uniform
+  const int tid = threadIdx.x;
+  const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
+  const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
+  const uint32_t n_slots = NB * TT::BVOL;
+  const buf_t rR = buf_rsrc(p.rate_ext, p.rate_ext ? n_slots * (uint32_t)sizeof(T) : 0u);
+  const buf_t rG = buf_rsrc(p.group_amp, (uint32_t)(MAX_GROUPS * sizeof(T)));
+  const T dt = T(p.dt), inv_cm = T(p.inv_cm), dscale = T(p.dscale);
+  // prologue: brick b loaded, neighbour rows of the next brick requested.
+  // Every load of an iteration is issued unconditionally (out of range past
+  // the last brick), so the wait counts stay exact: memory returns in issue
+  // order, and a load issued after the prefetch, or a path with fewer loads,
+  // would make the compute wait for the whole prefetch.
+  const buf_t rLab = buf_rsrc(g.label, n_slots);
+  uint8_t mylab = g.label[b * TT::BVOL + tid];
+  HN n;
+  halo_nbr<BX, BY, BZ>(n, g, b);
+  HR h;
+  halo_load<T, BX, BY, BZ>(h, n, g, V_in, ld, mylab);
+  T r = buf_ld<T>(rR, mylab != 0xFF ? b * TT::BVOL + tid : OOB);
+  uint32_t bn = b + m;
+  uint8_t labn = (uint8_t)buf_ld_u8(rLab, bn < hi ? bn * TT::BVOL + tid : OOB);
+  HN nn;
+  halo_nbr<BX, BY, BZ>(nn, g, bn, bn < hi);
+  for (;;) {
+    // this brick's stimulus amplitude, before the next brick's loads
+    const uint32_t slot = b * TT::BVOL + tid;
+    const uint32_t lab0 = h.q[0] == OOB ? 0xFFu : h.lab[0];
+    int grp = lab0 != 0xFF ? (int)((lab0 >> 2) & 31) : 0;
+    if (kInst && grp) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
+    const T stim = buf_ld<T>(rG, (uint32_t)grp);
+    // next brick: halo and rate; the brick after: label and neighbour rows
+    const bool more = bn < hi;
+    HR hn;
+    halo_load<T, BX, BY, BZ>(hn, nn, g, V_in, ld, labn);
+    const T rn = buf_ld<T>(rR, more && labn != 0xFF ? bn * TT::BVOL + tid : OOB);
+    const uint32_t bnn = bn + m;
+    const bool more2 = more && bnn < hi;
+    const uint8_t labnn = (uint8_t)buf_ld_u8(rLab, more2 ? bnn * TT::BVOL + tid : OOB);
+    HN nnn;
+    halo_nbr<BX, BY, BZ>(nnn, g, bnn, more2);
+    __syncthreads();  // every lane is done with the previous tile
+    halo_store<T, BX, BY, BZ>(t, h);
+    __syncthreads();
+    if (lab0 != 0xFF) {
+      const T Vc = t.V[c];
+      const T lv = dscale * stencil19<T, BX, BY, BZ>(t, c);
+      T Vn = Vc + dt * r + dt * inv_cm * (stim + lv);
+      Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
+      st32(V_out, slot, Vn);
+      const T thr = T(p.at_thr);
+      if (p.at && !(lab0 & 0x80) && Vc < thr && Vn >= thr) {
+        T* at = (T*)p.at;
+        if (ld32((const T*)at, slot) != ld32((const T*)at, slot)) {
+          const int64_t step = *p.step_ctr;
+          st32(at, slot, (T)((double)(step - 1) * p.dt) + dt * (thr - Vc) / (Vn - Vc));
+        }
+      }
+    }
+    if (!more) break;
+    b = bn;
+    bn = bnn;
+    h = hn;
+    r = rn;
+    nn = nnn;
+    labn = labnn;
+  }
+}
+
 template <typename T, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
                                                                  const T* __restrict__ V,
@@ -375,7 +509,6 @@ __global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t slot = b * TT::BVOL + threadIdx.x;
   const uint8_t mylab = g.label[slot];
-  stage_nbr<T, BX, BY, BZ>(t, g, b);
   stage_cells<T, BX, BY, BZ>(t, g, V, b, (uint32_t)ld, mylab);
   const int tid = threadIdx.x;
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
@@ -413,12 +546,49 @@ static int launch_bricks(const hp_bricks* g, const char* what, hipStream_t st, A
 
 template <typename T, int BX, int BY, int BZ>
 struct MyoK {
-  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ, true>;
+  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ>;
 };
-template <typename T, int BX, int BY, int BZ>
-struct MyoDiffK {
-  static constexpr auto fn = myo_step_kernel<T, BX, BY, BZ, false>;
-};
+// The persistent diffusion kernel: 16 one-wave workgroups per CU.
+static int compute_units() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+template <typename T>
+static int launch_diff(const hp_bricks* g, const T* V_in, T* V_out, int64_t ld,
+                       const hp_mono_params* p, hipStream_t st) {
+  if (g->n_bricks == 0) return 0;
+  const int64_t want = (int64_t)compute_units() * 16;
+  const dim3 grid((unsigned)(g->n_bricks < want ? g->n_bricks : want));
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  switch (key) {
+#define HP_DIFF(BX, BY, BZ)                                                                  \
+  if (g->inst)                                                                               \
+    hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, true>), grid, dim3(BX * BY * BZ), 0, \
+                       st, *g, V_in, V_out, ld, *p);                                         \
+  else                                                                                       \
+    hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, false>), grid, dim3(BX * BY * BZ),   \
+                       0, st, *g, V_in, V_out, ld, *p)
+    case 444: HP_DIFF(4, 4, 4); break;
+    case 844: HP_DIFF(8, 4, 4); break;
+    case 884: HP_DIFF(8, 8, 4); break;
+    case 888: HP_DIFF(8, 8, 8); break;
+#undef HP_DIFF
+    default:
+      set_error("hp_myo_step: unsupported brick shape %dx%dx%d", g->bx, g->by, g->bz);
+      return -1;
+  }
+  return check_launch("hp_myo_step");
+}
 template <typename T, int BX, int BY, int BZ>
 struct StencilK {
   static constexpr auto fn = stencil19_kernel<T, BX, BY, BZ>;
@@ -464,10 +634,25 @@ __global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __
 // ---------------------------------------------------------------------------
 // PMJ exchange: exact integration of the junction conductances over dt.
 // ---------------------------------------------------------------------------
+// First activation of a PMJ target voxel between V^n (start of step) and the
+// final V^{n+1} of step `step`
 template <typename T>
+__device__ __forceinline__ void pmj_at_update(const hp_pmj_at& at, int s, T v, int64_t step) {
+  const T v0 = ((const T*)at.V_prev)[s];
+  const T thr = T(at.at_thr);
+  T* a = (T*)at.at;
+  if (v0 < thr && v >= thr && a[s] != a[s])
+    a[s] = (T)((double)(step - 1) * at.dt) + T(at.dt) * (thr - v0) / (v - v0);
+}
+
+// One junction's exchange.  kDirect (patches exclusive): the junction applies
+// its increments to the patch voxels itself; else it writes them to scratch
+// for pmj_target.  Both round the increment to T and add it to V_myo in T.
+template <typename T, bool kDirect = false>
 __device__ __forceinline__ void pmj_junction(const hp_pmj& j, int64_t q, T* __restrict__ V_pk,
-                                             const T* __restrict__ V_myo, T* __restrict__ scratch,
-                                             double dt, double vmin, double vmax) {
+                                             T* __restrict__ V_myo, T* __restrict__ scratch,
+                                             double dt, double vmin, double vmax,
+                                             const hp_pmj_at* at = nullptr, int64_t step = 0) {
   const int e0 = j.ptr[q], e1 = j.ptr[q + 1];
   const int n = e1 - e0;
   const int p = j.pk[q];
@@ -492,8 +677,17 @@ __device__ __forceinline__ void pmj_junction(const hp_pmj& j, int64_t q, T* __re
   const double vp_new = vbar + (Vp - vbar) * r1;
   const double mean_new = vbar + (mean - vbar) * r1;
   for (int e = e0; e < e1; ++e) {
-    const double vm = (double)V_myo[j.patch[e]];
-    scratch[e] = (T)(mean_new + (vm - mean) * r2 - vm);
+    const int s = j.patch[e];
+    const T vt = V_myo[s];
+    const double vm = (double)vt;
+    const T delta = (T)(mean_new + (vm - mean) * r2 - vm);
+    if (kDirect) {
+      const T v = M<T>::mn(M<T>::mx(vt + delta, T(vmin)), T(vmax));
+      V_myo[s] = v;
+      if (at && at->at) pmj_at_update<T>(*at, s, v, step);
+    } else {
+      scratch[e] = delta;
+    }
   }
   V_pk[p] = (T)fmin(fmax(vp_new, vmin), vmax);
 }
@@ -508,17 +702,11 @@ __device__ __forceinline__ void pmj_target(const hp_pmj& j, int64_t q, T* __rest
   T v = V_myo[s] + acc;
   v = M<T>::mn(M<T>::mx(v, T(vmin)), T(vmax));
   V_myo[s] = v;
-  if (at.at) {  // first activation between V^n (start of step) and the final V^{n+1}
-    const T v0 = ((const T*)at.V_prev)[s];
-    const T thr = T(at.at_thr);
-    T* a = (T*)at.at;
-    if (v0 < thr && v >= thr && a[s] != a[s])
-      a[s] = (T)((double)(step - 1) * at.dt) + T(at.dt) * (thr - v0) / (v - v0);
-  }
+  if (at.at) pmj_at_update<T>(at, s, v, step);
 }
 
 template <typename T>
-__global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, const T* __restrict__ V_myo,
+__global__ void pmj_junction_kernel(hp_pmj j, T* __restrict__ V_pk, T* __restrict__ V_myo,
                                     T* __restrict__ scratch, double dt, double vmin,
                                     double vmax) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -559,7 +747,6 @@ __device__ __forceinline__ double group_amplitude(int g, const double* __restric
 }
 
 constexpr int STIM_THREADS = 1024;
-constexpr int MAX_GROUPS = 8192;
 
 template <typename T>
 __global__ void __launch_bounds__(STIM_THREADS) stim_kernel(const double* __restrict__ sp,
@@ -594,7 +781,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
   // only after that barrier
   const int64_t step = *step_ctr;
   if (tid < JT) {
-    if (has_pmj)
+    if (has_pmj && j.exclusive)
+      for (int64_t q = tid; q < j.n_junctions; q += JT)
+        pmj_junction<T, true>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax, &at, step);
+    else if (has_pmj)
       for (int64_t q = tid; q < j.n_junctions; q += JT)
         pmj_junction<T>(j, q, V_pk, V_myo, scratch, dt, vmin, vmax);
   } else {
@@ -603,7 +793,7 @@ __global__ void __launch_bounds__(TAIL_THREADS) step_tail_kernel(
       amp[g] = g == 0 ? T(0) : (T)group_amplitude(g, sp, gs, t);
   }
   __syncthreads();
-  if (has_pmj)
+  if (has_pmj && !j.exclusive)
     for (int64_t q = tid; q < j.n_targets; q += TAIL_THREADS)
       pmj_target<T>(j, q, V_myo, scratch, vmin, vmax, at, step);
   if (tid == 0) *step_ctr = step + 1;
@@ -632,9 +822,7 @@ static int myo_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64
     set_error("hp_myo_step: ld must cover every slot and 6*ld elements fit 32-bit offsets");
     return -1;
   }
-  if (p->do_reaction == 0)
-    return launch_bricks<T, MyoDiffK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state,
-                                      ld, *p);
+  if (p->do_reaction == 0) return launch_diff<T>(g, V_in, V_out, ld, p, (hipStream_t)stream);
   return launch_bricks<T, MyoK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state, ld,
                                 *p);
 }

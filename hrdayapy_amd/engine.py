@@ -79,7 +79,8 @@ class MonodomainEngine:
 
     def __init__(self, layout: BrickLayout, D6, celltype_myo, *, h, A_M, Cm, dt,
                  dtype=torch.float32, device="cuda", cable=None, pmj=None, stim_sites=None,
-                 at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None):
+                 at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None,
+                 alloc="torch"):
         nat.load()
         self.lay = layout
         self.dtype = dtype
@@ -123,15 +124,31 @@ class MonodomainEngine:
         if pmj and self.Np and pmj["c_pmj"] != 0.0 and len(pmj["ids"]):
             tgt_nodes = np.unique(np.concatenate([np.asarray(p, np.int64) for p in pmj["patches"]]))
             lab[L.slot_of_node[tgt_nodes]] |= np.uint8(0x80)
-        self.label = torch.from_numpy(lab).to(dev)
-        self.D6 = torch.from_numpy(L.to_slots(np.asarray(D6, dtype=np.float64))).to(dev, dt_)
+        # the resident per-slot arrays share one allocation (2 MiB-aligned views)
+        n = L.n_slots
+        specs = [("state", (18, n), dt_), ("D6", (6, n), dt_), ("V0", n, dt_), ("V1", n, dt_),
+                 ("at", n, dt_), ("label", n, torch.uint8)]
+        if inst is not None:
+            specs.append(("inst", n, torch.int16))
+        self.arena = nat.DeviceArena(specs, dev, contiguous=(alloc == "contiguous"))
+        ar = self.arena
+        self.label = ar["label"]
+        self.label.copy_(torch.from_numpy(lab))
+        self.D6 = ar["D6"]
+        self.D6.copy_(torch.from_numpy(L.to_slots(np.asarray(D6, dtype=np.float64))).to(dt_))
         self.nbr = torch.from_numpy(L.nbr).to(dev)
         self.slot_of_node = torch.from_numpy(L.slot_of_node).to(dev)
-        self.V = [torch.full((L.n_slots,), V0, dtype=dt_, device=dev) for _ in range(2)]
-        self.state = self._init_state(L.n_slots)
-        self.at = torch.full((L.n_slots,), float("nan"), dtype=dt_, device=dev)
-        self.inst = None if inst is None else torch.from_numpy(
-            L.to_slots(np.asarray(inst, dtype=np.uint16)).view(np.int16)).to(dev)
+        self.V = [ar["V0"], ar["V1"]]
+        for v in self.V:
+            v.fill_(V0)
+        self.state = self._init_state(ar["state"])
+        self.at = ar["at"]
+        self.at.fill_(float("nan"))
+        self.inst = None
+        if inst is not None:
+            self.inst = ar["inst"]
+            self.inst.copy_(torch.from_numpy(
+                L.to_slots(np.asarray(inst, dtype=np.uint16)).view(np.int16)))
         self.inst_params = None if inst_params is None else torch.as_tensor(
             np.asarray(inst_params, dtype=np.float64).reshape(-1, 4)).to(dev, dt_).contiguous()
         self.bricks = nat.HpBricks(L.n_bricks, *L.brick, 0, nat.ptr(self.nbr), nat.ptr(self.label),
@@ -156,7 +173,7 @@ class MonodomainEngine:
             self.pk_indices = torch.from_numpy(b.astype(np.int32)).to(dev)
             self.pk_label = torch.from_numpy(lab_pk.astype(np.uint8)).to(dev)
             self.Vpk = [torch.full((n,), V0, dtype=dt_, device=dev) for _ in range(2)]
-            self.state_pk = self._init_state(n)
+            self.state_pk = self._init_state(torch.empty((18, n), dtype=dt_, device=dev))
             self.cable = nat.HpCable(n, nat.ptr(self.pk_indptr), nat.ptr(self.pk_indices),
                                      nat.ptr(self.pk_label), float(cable["w_pp"]))
             self.p_pk = self._params(1.0 / float(cable["Ai_P"]), at_thr, None)
@@ -187,7 +204,8 @@ class MonodomainEngine:
             self.pmj = nat.HpPmj(len(patches), len(entries), nat.ptr(pk_), nat.ptr(ptr_),
                                  nat.ptr(ent_), len(tgt), nat.ptr(tgt_), nat.ptr(tptr_),
                                  nat.ptr(tent_), float(pmj["c_pmj"]), float(pmj["Ai_P"]),
-                                 float(pmj["Ai_M"]), self.Cm, nat.ptr(self.pmj_coef))
+                                 float(pmj["Ai_M"]), self.Cm, nat.ptr(self.pmj_coef),
+                                 int(len(tgt) == len(entries)), 0)
             self.pmj_at = [nat.HpPmjAt(nat.ptr(self.at), nat.ptr(self.V[k]),
                                        nat.ptr(self.step_ctr), self.dt, float(at_thr))
                            for k in range(2)]
@@ -197,8 +215,8 @@ class MonodomainEngine:
         self._eval_stim()
 
     # ------------------------------------------------------------------ setup
-    def _init_state(self, n):
-        st = torch.empty((18, n), dtype=self.dtype, device=self.dev)
+    def _init_state(self, st):
+        """Resting TTP06 state into st (18, n) (Ki, Nai as offsets from their bases)."""
         for k, name in enumerate(STATE_NAMES):
             v = INIT[name]
             if name == "Ki":

@@ -1,8 +1,10 @@
 """Brick layout of the myocardium in HBM.
 
 The voxel mask S (Nx,Ny,Nz) is cut into bricks of (bx,by,bz) voxels; only
-bricks holding at least one myocardial voxel are stored, in raster order of
-the brick grid.  Every per-voxel array (V, the 18 TTP06 state variables, the
+bricks holding at least one myocardial voxel are stored, in Morton (Z-curve)
+order of the brick grid: a run of consecutive bricks is a compact block of
+space, so the neighbour bricks whose faces a workgroup stages were mostly read
+by nearby workgroups moments before (L2 reuse).  Every per-voxel array (V, the 18 TTP06 state variables, the
 six conductivity-tensor components, the label byte) is a flat array of
 n_bricks*bvol slots, slot = brick*bvol + (lx*by + ly)*bz + lz, so a wave of
 64 lanes reads 64 consecutive slots of one brick (coalesced SoA).  A
@@ -20,8 +22,22 @@ BRICK_SHAPES = ((4, 4, 4), (8, 4, 4), (8, 8, 4), (8, 8, 8))
 INACTIVE = 0xFF
 
 
+def _spread3(v):
+    """Bits of v (< 2^21) moved to every third position (Morton interleave)."""
+    v = np.asarray(v, dtype=np.uint64) & np.uint64(0x1FFFFF)
+    for sh, m in ((32, 0x1F00000000FFFF), (16, 0x1F0000FF0000FF), (8, 0x100F00F00F00F00F),
+                  (4, 0x10C30C30C30C30C3), (2, 0x1249249249249249)):
+        v = (v | (v << np.uint64(sh))) & np.uint64(m)
+    return v
+
+
+def morton3(x, y, z):
+    """Morton code with z as the fastest-varying axis."""
+    return (_spread3(x) << np.uint64(2)) | (_spread3(y) << np.uint64(1)) | _spread3(z)
+
+
 class BrickLayout:
-    def __init__(self, S: np.ndarray, brick=(8, 8, 8)):
+    def __init__(self, S: np.ndarray, brick=(8, 8, 8), order="morton"):
         brick = tuple(int(b) for b in brick)
         if brick not in BRICK_SHAPES:
             raise ValueError(f"brick shape {brick} not supported; choose one of {BRICK_SHAPES}")
@@ -41,6 +57,12 @@ class BrickLayout:
         bc = vox // np.array(brick)
         bid = (bc[:, 0] * gy + bc[:, 1]) * gz + bc[:, 2]
         active = np.unique(bid)
+        if order == "morton":
+            a3 = np.stack([active // (gy * gz), (active // gz) % gy, active % gz], axis=1)
+            active = active[np.argsort(morton3(a3[:, 0], a3[:, 1], a3[:, 2]), kind="stable")]
+        elif order != "raster":
+            raise ValueError(f"brick order {order!r}: 'morton' or 'raster'")
+        self.order = order
         self.n_bricks = len(active)
         index_of = -np.ones(gx * gy * gz, dtype=np.int64)
         index_of[active] = np.arange(self.n_bricks)

@@ -59,8 +59,21 @@ class SurrogateNet:
     # -- parameters as C structs
     def refresh(self):
         self.Wb = [_bf16(w) for w in self.W]
-        self.hp_w = nat.HpMlpWeights(*[w.data_ptr() for w in self.Wb],
-                                     *[b.data_ptr() for b in self.b])
+        self.refresh_inference()
+        self._refresh_norm()
+
+    def refresh_inference(self):
+        """The inference kernel's scaled weights (hp_mlp_weights): w1, b1..b3
+        times -log2(e), w4 times -ln 2, from the FP32 master weights.  Call
+        after the master weights change (training)."""
+        c = 1.0 / math.log(2.0)
+        self.Wi = [_bf16(self.W[0] * -c), _bf16(self.W[1]), _bf16(self.W[2]),
+                   _bf16(self.W[3] * -math.log(2.0))]
+        self.bi = [(b * -c).contiguous() for b in self.b[:3]] + [self.b[3].clone()]
+        self.hp_w = nat.HpMlpWeights(*[w.data_ptr() for w in self.Wi],
+                                     *[b.data_ptr() for b in self.bi])
+
+    def _refresh_norm(self):
         n = nat.HpMlpNorm()
         for k in range(N_IN):
             n.in_mean[k] = float(self.in_mean[k])
@@ -499,6 +512,7 @@ def train_surrogate(X, D, device="cuda", epochs=8, batch=8192, lr=2e-3, lr_decay
     Xn, Tn = tr.prepare(X, D)
     t0 = time.perf_counter()
     hist = tr.fit(Xn[:ntr], Tn[:ntr], epochs=epochs, lr_decay=lr_decay, log=log, seed=seed)
+    net.refresh_inference()
     train_s = time.perf_counter() - t0
     # held-out error of dV over the macro-step (mV)
     preds = []

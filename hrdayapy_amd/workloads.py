@@ -17,6 +17,30 @@ SPMV_BYTES_PER_CELL = 20   # p 4 + Ap 4 + 3 face conductances 12
 FUSED_BYTES_PER_VOXEL = 177
 
 
+def graph_us(fn, reps):
+    """Average device time (us) of fn(i), i = 0..reps-1, captured into one HIP
+    graph and replayed, HIP events around the replay on the capturing stream:
+    no host launch overhead between the kernels (eager launches through ctypes
+    can starve the GPU and time the host instead)."""
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            for i in range(reps):
+                fn(i)
+    torch.cuda.current_stream().wait_stream(side)
+    g.replay()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    g.replay()
+    e1.record(st)
+    e1.synchronize()
+    del g
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
 def _event_us(fn, reps):
     st = torch.cuda.current_stream()
     fn()

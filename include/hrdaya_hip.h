@@ -41,6 +41,13 @@ extern "C" {
 int hp_abi_version(void);
 /* Copies the last error message of the calling thread into buf (NUL-terminated). */
 int hp_last_error(char* buf, size_t size);
+/* Device memory for the resident per-slot arrays (no counterpart in the
+ * reference, whose tensors come from torch's allocator): one allocation,
+ * optionally physically contiguous (hipDeviceMallocContiguous), so the arrays
+ * of a long-running engine do not inherit the fragmentation of earlier frees.
+ * Free with hp_device_free after the last kernel that uses it has finished. */
+int hp_device_alloc(void** ptr, int64_t bytes, int contiguous);
+int hp_device_free(void* ptr);
 
 /* ---------------------------------------------------------------------------
  * TTP06 reaction step (ionic plugin seam).
@@ -205,6 +212,11 @@ typedef struct hp_pmj {
    * Cp/(Cp+Cq), exp(-c dt (1/Cp + 1/Cq)), exp(-(c/n) dt/(Ai_M Cm)), 1/n
    * (Cp = Ai_P Cm, Cq = n Ai_M Cm, n = patch size); NULL = computed per step */
   const double* coef;
+  /* 1 when no two entries name the same slot (every target has one entry):
+   * hp_step_tail then updates the patch voxels in the junction pass, without
+   * the scratch array and the target pass (same arithmetic, same result) */
+  int32_t exclusive;
+  int32_t pad_;
 } hp_pmj;
 
 /* Optional first-activation bookkeeping for the junction voxels: these carry
@@ -409,6 +421,12 @@ int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t
  * 22 -> 128 -> 128 -> 128 -> 19 (SiLU), BF16 weights, FP32 accumulation on
  * MFMA, predicts the increments of V and of the 18 TTP06 variables over one
  * macro-step of 10 reaction substeps.
+ *
+ * hp_surrogate_step takes the weights in a scaled form that saves one
+ * multiplication per activation: with c = log2(e), w1 and b1, b2, b3 are
+ * multiplied by -c and w4 by -1/c (= -ln 2); b4 is unchanged.  Each hidden
+ * layer then stores g = u/(1 + 2^u) = -c*SiLU(z) for its scaled
+ * pre-activation u = -c*z, which the next layer's unscaled weights consume.
  * ------------------------------------------------------------------------- */
 typedef struct hp_mlp_weights {
   const void* w1;  /* bf16 [128][22]  (out, in) */

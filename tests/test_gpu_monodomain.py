@@ -24,8 +24,10 @@ def _case(kind, h=0.25):
     else:
         nodes, el, act = small_tree(S, h)
         # root pulse into the cable plus a later myocardial corner pulse: the
-        # PMJs then carry current in both directions
-        kw = dict(c_pmj=2.0, n_pmj=3, sigma_P=0.45, stim_len_mm=0.6,
+        # PMJs then carry current in both directions; "overlap": patches of 300
+        # voxels that share voxels (the exchange's scatter pass), else 3
+        kw = dict(c_pmj=2.0, n_pmj=300 if kind == "overlap" else 3, sigma_P=0.45,
+                  stim_len_mm=0.6,
                   ectopic_region=corner_region(S),
                   stim_protocol=[dict(name="root", target="root", amp=1.0, dur=2.0, onset=0.0),
                                  dict(name="corner", target="ectopic", amp=0.52, dur=2.0,
@@ -42,7 +44,7 @@ def _oracle(args, kw, T, scheme, record_every):
     return prob, OMD.run(prob, T, 0.02, scheme=scheme, record_every=record_every)
 
 
-@pytest.mark.parametrize("kind", ["stub", "tree"])
+@pytest.mark.parametrize("kind", ["stub", "tree", "overlap"])
 @pytest.mark.parametrize("brick", [(4, 4, 4), (8, 8, 4)])
 def test_fp64_engine_matches_explicit_oracle(cuda, kind, brick):
     print(f"[case] fp64 engine vs explicit oracle kind={kind} brick={brick}", flush=True)
@@ -53,6 +55,8 @@ def test_fp64_engine_matches_explicit_oracle(cuda, kind, brick):
     eng, meta = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, dtype=torch.float64,
                                      device=cuda, brick=brick, **kw)
     assert meta["Np"] == prob.Np and meta["N_myo"] == prob.N_myo
+    if eng.pmj is not None:  # which exchange pass runs
+        assert eng.pmj.exclusive == (0 if kind == "overlap" else 1)
     eng.run(int(T / 0.02))
     Vg = eng.V_global().cpu().numpy()
     np.testing.assert_allclose(Vg, ref["V"], rtol=0, atol=1e-8)
@@ -143,3 +147,40 @@ def test_stencil_apply_matches_oracle_laplacian(cuda):
             got = out.double().cpu().numpy()[lay.slot_of_node]
             scale = np.abs(ref).max()
             assert np.max(np.abs(got - ref)) / scale < tol, (brick, dtype)
+
+
+@pytest.mark.parametrize("brick", [(4, 4, 4), (8, 4, 4), (8, 8, 4), (8, 8, 8)])
+def test_diffusion_substep_equals_fused_rate_form(cuda, brick):
+    """The persistent diffusion-only kernel (do_reaction 0, dV/dt = rate_ext)
+    against the fused kernel's rate form (do_reaction 2, no NaN rates) over 30
+    steps of a running slab, activation times included: the same update, so
+    equal up to the compiler's choice of fused multiply-adds."""
+    print(f"[case] diffusion substep kernel vs fused rate form, brick {brick}", flush=True)
+    import ctypes
+    args, kw = _case("stub")
+    nodes, el, act, S, phi, f = args
+    engs = [build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, device=cuda, brick=brick,
+                                 **kw)[0] for _ in range(2)]
+    rng = np.random.default_rng(11)
+    for e in engs:
+        e.run(150)  # a wave under way
+    torch.cuda.synchronize()
+    assert torch.equal(engs[0].V[engs[0].p], engs[1].V[engs[1].p])
+    n = engs[0].lay.n_slots
+    for step in range(30):
+        rate = torch.from_numpy(rng.uniform(-5, 5, n).astype(np.float32)).to(cuda)
+        for e, mode in zip(engs, (0, 2)):
+            e.p_myo.do_reaction = mode
+            e.p_myo.rate_ext = rate.data_ptr()
+            a, b = e.p, 1 - e.p
+            nat.call("hp_myo_step_f32", ctypes.byref(e.bricks), nat.ptr(e.V[a]), nat.ptr(e.V[b]),
+                     nat.ptr(e.state), n, ctypes.byref(e.p_myo), nat.stream_handle())
+            e.p = b
+        torch.cuda.synchronize()
+    dv = float((engs[0].V[engs[0].p] - engs[1].V[engs[1].p]).abs().max())
+    assert torch.equal(engs[0].at.isnan(), engs[1].at.isnan())
+    ok = ~engs[0].at.isnan()
+    dat = float((engs[0].at[ok] - engs[1].at[ok]).abs().max())
+    print(f"  max |dV| {dv:.2e} mV, max |dAT| {dat:.2e} ms over {int(ok.sum())} activated")
+    assert ok.sum() > 0
+    assert dv < 1e-3 and dat < 1e-4

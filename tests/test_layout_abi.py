@@ -12,11 +12,12 @@ from tests._problems import slab
 ROOT = Path(__file__).resolve().parents[1]
 
 
+@pytest.mark.parametrize("order", ["morton", "raster"])
 @pytest.mark.parametrize("brick", BRICK_SHAPES)
-def test_layout_slots_and_neighbours(brick):
+def test_layout_slots_and_neighbours(brick, order):
     S, _, _ = slab(13, 11, 7)
     S[4:7, 3:9, 2:5] = 0
-    lay = BrickLayout(S, brick)
+    lay = BrickLayout(S, brick, order=order)
     vox = lay.vox_idx
     assert lay.n_nodes == S.sum()
     assert len(np.unique(lay.slot_of_node)) == lay.n_nodes
@@ -34,6 +35,20 @@ def test_layout_slots_and_neighbours(brick):
         ok = lay.nbr[:, k] >= 0
         assert np.array_equal(lay.brick_coords[lay.nbr[ok, k]],
                               lay.brick_coords[ok] + np.array([dx, dy, dz]))
+    # absent entries are bricks without tissue
+    stored = {tuple(c) for c in lay.brick_coords}
+    for k, (dx, dy, dz) in enumerate([(a, c, d) for a in (-1, 0, 1) for c in (-1, 0, 1)
+                                      for d in (-1, 0, 1)]):
+        for i in np.nonzero(lay.nbr[:, k] < 0)[0]:
+            assert tuple(lay.brick_coords[i] + np.array([dx, dy, dz])) not in stored
+    # brick order: Morton codes (bits interleaved x, y, z from the top) ascending
+    c = lay.brick_coords.astype(np.int64)
+    key = np.zeros(len(c), np.int64)
+    for bit in range(6):
+        for ax in range(3):
+            key |= ((c[:, ax] >> bit) & 1) << (3 * bit + 2 - ax)
+    lin = (c[:, 0] * lay.grid[1] + c[:, 1]) * lay.grid[2] + c[:, 2]
+    assert np.all(np.diff(key if order == "morton" else lin) > 0)
     lab = lay.labels(np.full(lay.n_nodes, 2, np.uint8), np.ones(lay.n_nodes, np.int64))
     assert (lab != 0xFF).sum() == lay.n_nodes
     assert np.all(lab[lay.slot_of_node] == (2 | (1 << 2)))
