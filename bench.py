@@ -138,10 +138,12 @@ def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None
         out["at_err_vs_plain_mean_ms"] = float(np.mean(np.abs(at[both] - ref_at[both]))) if both.any() else None
         out["activated_mismatch"] = int(np.isfinite(at).sum() - np.isfinite(ref_at).sum())
     if phi is not None and phi_ref is not None and len(phi) == len(phi_ref):
-        r, rel = W.lead_agreement(W.twelve_leads(ecg.lf.names, phi),
-                                  W.twelve_leads(ecg.lf.names, phi_ref))
+        r, rel, per = W.lead_agreement(W.twelve_leads(ecg.lf.names, phi),
+                                       W.twelve_leads(ecg.lf.names, phi_ref), per_lead=True)
         out["ecg_r_min_vs_plain"] = r
         out["ecg_rel_l2_max_vs_plain"] = rel
+        from hrdayapy_amd.ecg.functions.forward import LEAD_NAMES
+        out["ecg_per_lead_r_rel"] = dict(zip(LEAD_NAMES, per))
     return out
 
 
@@ -242,7 +244,7 @@ def main():
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x8x8, or auto")
     ap.add_argument("--forms", type=str, default="plain,surrogate,bigstep")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
-    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=14)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)

@@ -95,6 +95,8 @@ _SIGS = {
     "hp_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
     "hp_device_alloc": (C.c_int, [C.POINTER(c_vp), c_i64, C.c_int]),
     "hp_device_free": (C.c_int, [c_vp]),
+    "hp_crossing_events_f32": (C.c_int, [c_vp, c_vp, c_i64, c_i64, c_f64, C.c_int, c_f64, c_f64,
+                                         c_vp, c_vp, c_i32, c_vp]),
     "hp_ttp06_step_f32": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,
                                     c_i64, c_f32, c_f32, c_f32, c_vp]),
     "hp_ttp06_step_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, C.c_int, c_vp, c_vp, c_i64, c_i64,

@@ -137,8 +137,18 @@ def purkinje_myocardium_anisotropic_solver(
     run_vm = vm_save_dt is not None and out_vm_npz is not None
     vm_every = max(1, int(round(vm_save_dt / dt))) if run_vm else 0
     bmaps = [np.asarray(b, dtype=np.int64) for b in meta["branch_map"]]
-    frames_surf, branch_frames, times = [], [[] for _ in bmaps], []
-    vm_frames, vm_times = [], []
+    # Output frames stay in HBM while the simulation runs (one device gather per
+    # frame, no host synchronisation) and travel to the host once at the end.
+    n_anim = (n_steps - 1) // save_every + 1 if n_steps else 0
+    n_vm = (n_steps - 1) // vm_every + 1 if run_vm and n_steps else 0
+    f32 = dict(dtype=torch.float32, device=dev)
+    surf_t = torch.from_numpy(surf_ids.astype(np.int64)).to(dev)
+    bcat = np.concatenate(bmaps) if bmaps else np.zeros(0, np.int64)
+    bcat_t = torch.from_numpy(bcat.astype(np.int64)).to(dev)
+    surf_buf = torch.empty((n_anim, len(surf_ids)), **f32)
+    branch_buf = torch.empty((n_anim, len(bcat)), **f32)
+    vm_buf = torch.empty((n_vm, N_myo), **f32)
+    times, vm_times = [], []
     # the loop advances to the next output step with a captured graph chunk
     cadence = [c for c in (save_every, vm_every) if c]
     t0 = time.perf_counter()
@@ -150,28 +160,34 @@ def purkinje_myocardium_anisotropic_solver(
         k = (nxt - step + 1) * n_sub
         eng.run(k, graph_chunk=graph_chunk if k >= graph_chunk else 0)
         t = nxt * dt
-        if nxt % save_every == 0:
-            Vg = eng.V_global().float().cpu().numpy()
-            frames_surf.append(Vg[Np:][surf_ids])
-            for b, bm in enumerate(bmaps):
-                branch_frames[b].append(Vg[:Np][bm])
-            times.append(t)
-        if run_vm and nxt % vm_every == 0:
-            vm_frames.append(eng.V_myo_nodes().float().cpu().numpy())
-            vm_times.append(t)
+        want_anim, want_vm = nxt % save_every == 0, run_vm and nxt % vm_every == 0
+        if want_anim or want_vm:
+            vm_now = eng.V_myo_nodes()
+            if want_anim:
+                surf_buf[len(times)] = vm_now[surf_t]
+                if len(bcat):
+                    branch_buf[len(times)] = eng.Vpk[eng.p][bcat_t]
+                times.append(t)
+            if want_vm:
+                vm_buf[len(vm_times)] = vm_now
+                vm_times.append(t)
         step = nxt + 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     print(f"[hrdaya] {n_steps * n_sub:,} steps x {N_myo:,} voxels in {wall:.2f} s "
           f"({n_steps * n_sub * N_myo / max(wall, 1e-9):.3e} voxel-steps/s)")
-    if run_vm and vm_frames:
+    if run_vm and vm_times:
         _save_vm_snapshots(out_vm_npz, np.arange(N_myo, dtype=np.int32) + Np, meta["nodes_mm"],
-                           vox, np.stack(vm_frames), np.array(vm_times, np.float32), dt,
-                           vm_every * dt)
+                           vox, vm_buf[:len(vm_times)].cpu().numpy(),
+                           np.array(vm_times, np.float32), dt, vm_every * dt)
+    frames_surf = surf_buf[:len(times)].cpu().numpy()
+    bf = branch_buf[:len(times)].cpu().numpy()
+    offs = np.cumsum([0] + [len(b) for b in bmaps])
+    branch_frames = [bf[:, offs[i]:offs[i + 1]] for i in range(len(bmaps))]
     results = dict(
         comp_nodes=meta["comp_nodes"], comp_edges=meta["comp_edges"],
         branch_map=meta["branch_map"],
-        branch_frames_p=[np.asarray(f, dtype=np.float32).reshape(len(f), -1) for f in branch_frames],
+        branch_frames_p=[np.ascontiguousarray(f, dtype=np.float32) for f in branch_frames],
         surf_verts=np.zeros((0, 3), np.float32), surf_faces=np.zeros((0, 3), np.int32),
         vert_to_surf_node=np.zeros(0, np.int32),
         frames_surf=np.asarray(frames_surf, dtype=np.float32), times=times,

@@ -68,8 +68,10 @@ __global__ void __launch_bounds__(PBLOCK) residual_f64_kernel(hp_torso t, const 
   block_reduce_last<2>(v, part, counter, out);
 }
 
-// PCG init for A d = r (FP32), d = 0: z = r/diag, p = z, rz, rr0 = ||r||^2.
-__global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, const float* __restrict__ r,
+// PCG init for A d = r (FP32), d = 0: z = r/diag (stored over r: the solve
+// carries z = M^-1 r instead of r, so the product kernel reads two arrays per
+// neighbour and divides nothing), p = z, rz, rr0 = ||r||^2.
+__global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, float* __restrict__ r,
                                                           float* __restrict__ d,
                                                           float* __restrict__ p, double* part,
                                                           int* counter, double* sc, double tol,
@@ -78,6 +80,7 @@ __global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, const floa
   for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float rc = r[c];
     const float z = rc / t.diag[c];
+    r[c] = z;
     d[c] = 0.0f;
     p[c] = z;
     v[0] += (double)rc * (double)z;
@@ -102,8 +105,9 @@ __global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, const floa
 }
 
 // p = z + beta p (skipped on the first iteration: beta = 0 and p = z already),
-// Ap = A p, pAp.  z of a neighbour is recomputed as r/diag on the fly.
-__global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const float* __restrict__ r,
+// Ap = A p, pAp.  z (held in the r array) and the old p of a neighbour give
+// its new p on the fly.
+__global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const float* __restrict__ z,
                                                           float* __restrict__ p,
                                                           float* __restrict__ Ap, double* part,
                                                           int* counter, double* sc) {
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const floa
   const float beta = (float)sc[S_BETA];
   const bool first = sc[S_ITER] == 0.0;
   double v[1] = {0.0};
-  auto pnew = [&](int n) -> float { return first ? p[n] : r[n] / t.diag[n] + beta * p[n]; };
+  auto pnew = [&](int n) -> float { return first ? p[n] : z[n] + beta * p[n]; };
   for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
     const float pc = pnew(c);
     const float acc = row_f32(t, c, k, m, pc, pnew);
@@ -128,9 +132,10 @@ __global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const floa
   }
 }
 
-// p <- z + beta p (materialised), d += alpha p, r -= alpha Ap, rz, rr; finish:
-// beta, stopping test, iteration count.
-__global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* __restrict__ r,
+// p <- z + beta p (materialised), d += alpha p, z -= alpha Ap/diag (z = M^-1 r
+// held in the r array), rz, rr with r = diag z; finish: beta, stopping test,
+// iteration count.
+__global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* __restrict__ z,
                                                             float* __restrict__ d,
                                                             float* __restrict__ p,
                                                             const float* __restrict__ Ap,
@@ -142,16 +147,17 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
   const bool first = sc[S_ITER] == 0.0;
   double v[3] = {0.0, 0.0, 0.0};
   for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const float inv = 1.0f / t.diag[c];
-    const float pc = first ? p[c] : r[c] * inv + beta * p[c];
+    const float zc = z[c];
+    const float pc = first ? p[c] : zc + beta * p[c];
     p[c] = pc;
     d[c] += alpha * pc;
-    const float rc = r[c] - alpha * Ap[c];
-    r[c] = rc;
-    const double z = (double)(rc * inv);
-    v[0] += (double)rc * z;
-    v[1] += (double)rc * (double)rc;
-    v[2] += z * z;
+    const float dg = t.diag[c];
+    const float zn = zc - alpha * (Ap[c] / dg);
+    z[c] = zn;
+    const double rn = (double)dg * (double)zn;
+    v[0] += rn * (double)zn;
+    v[1] += rn * rn;
+    v[2] += (double)zn * (double)zn;
   });
   double red[3];
   if (block_reduce_last<3>(v, part, counter, red) && threadIdx.x == 0) {
@@ -349,7 +355,7 @@ int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x,
   return check_launch("hp_poisson_residual_f64");
 }
 
-int hp_pcg_begin_f32(const hp_torso* t, const float* r, float* d, float* p, hp_pcg_ws* ws,
+int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
                      double tol, int max_iter, int mode, void* stream) {
   if (!torso_ok(t, "hp_pcg_begin_f32") || !ws) return -1;
   hipLaunchKernelGGL(pcg_init_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,

@@ -245,35 +245,77 @@ class ActivationMaps:
 
 
 def _as_snapshots(v):
+    """VmSnapshots from a VmSnapshots, a dict or an npz; Vm may be a GPU tensor."""
     if isinstance(v, VmSnapshots):
         return v
-    n = np.asarray(v["Vm"]).shape[1]
+    Vm = v["Vm"]
+    if not (hasattr(Vm, "is_cuda") and Vm.is_cuda):
+        Vm = np.asarray(Vm)
+    n = Vm.shape[1]
 
     def get(k, default):
         return v[k] if k in v else default
 
     return VmSnapshots(np.asarray(get("node_ids", np.arange(n, dtype=np.int32))),
-                       np.asarray(v["coords_mm"]), np.asarray(v["vox_idx"]), np.asarray(v["Vm"]),
+                       np.asarray(v["coords_mm"]), np.asarray(v["vox_idx"]), Vm,
                        np.asarray(v["time_vm"]), float(get("dt_solver", np.nan)),
                        float(get("vm_save_dt", np.nan)))
 
 
+def _device_table(Vm, t, threshold, rising, min_vel, min_gap):
+    """The same event table computed by hp_crossing_events_f32 on the device:
+    Vm (n_frames, N) FP32 torch tensor on the GPU, t (n_frames,) host times."""
+    import torch
+    from ... import _native as nat
+    nf, N = Vm.shape
+    dev = Vm.device
+    td = torch.as_tensor(np.asarray(t, dtype=np.float64), device=dev)
+    mv = -1.0 if min_vel is None else float(min_vel)
+    mg = -1.0 if min_gap is None else float(min_gap)
+    counts = torch.zeros(N, dtype=torch.int32, device=dev)
+    s = nat.stream_handle(dev)
+    nat.call("hp_crossing_events_f32", Vm.data_ptr(), td.data_ptr(), nf, N, float(threshold),
+             int(rising), mv, mg, counts.data_ptr(), None, 0, s)
+    m = int(counts.max()) if N else 0
+    table = torch.empty((N, m), dtype=torch.float32, device=dev)
+    if m and N:
+        nat.call("hp_crossing_events_f32", Vm.data_ptr(), td.data_ptr(), nf, N, float(threshold),
+                 int(rising), mv, mg, None, table.data_ptr(), m, s)
+    return table.cpu().numpy()
+
+
 def compute_activation_maps(vm_snapshots, S=None, *, act_threshold=-40.0, deact_threshold=-40.0,
                             min_event_gap_ms=None, min_upstroke_vel=None, min_downstroke_vel=None,
-                            save_path=None) -> ActivationMaps:
-    """Activation/deactivation tables of a VmSnapshots (or a dict / npz with its keys)."""
+                            save_path=None, device=None) -> ActivationMaps:
+    """Activation/deactivation tables of a VmSnapshots (or a dict / npz with its keys).
+
+    With ``device`` (e.g. "cuda"), or when Vm is already a GPU tensor, the tables
+    are computed on the device by hp_crossing_events_f32 (identical results for
+    FP32 snapshots); only the tables travel to the host."""
     snap = _as_snapshots(vm_snapshots)
-    Vm, t = np.asarray(snap.Vm), np.asarray(snap.time_vm)
+    Vm, t = snap.Vm, np.asarray(snap.time_vm)
+    on_dev = hasattr(Vm, "is_cuda") and Vm.is_cuda
+    if on_dev and device is None:
+        device = Vm.device
     if Vm.shape[0] < 2:
         raise ValueError("vm_snapshots has fewer than 2 frames -- cannot detect threshold "
                          "crossings. Re-run the simulation with a smaller vm_save_dt / longer T.")
     N = Vm.shape[1]
-    up = np.diff((Vm >= act_threshold).astype(np.int8), axis=0) == 1
-    act = _table(*_gate(*_crossings(Vm, t, up, act_threshold), min_upstroke_vel), N)
-    act = _debounce(act, min_event_gap_ms)
-    down = np.diff((Vm >= deact_threshold).astype(np.int8), axis=0) == -1
-    deact = _table(*_gate(*_crossings(Vm, t, down, deact_threshold), min_downstroke_vel), N)
-    deact = _debounce(deact, min_event_gap_ms)
+    if device is not None:
+        import torch
+        Vd = Vm if on_dev else torch.as_tensor(np.asarray(Vm, dtype=np.float32), device=device)
+        Vd = Vd.to(torch.float32).contiguous()
+        act = _device_table(Vd, t, act_threshold, True, min_upstroke_vel, min_event_gap_ms)
+        deact = _device_table(Vd, t, deact_threshold, False, min_downstroke_vel,
+                              min_event_gap_ms)
+    else:
+        Vm = np.asarray(Vm)
+        up = np.diff((Vm >= act_threshold).astype(np.int8), axis=0) == 1
+        act = _table(*_gate(*_crossings(Vm, t, up, act_threshold), min_upstroke_vel), N)
+        act = _debounce(act, min_event_gap_ms)
+        down = np.diff((Vm >= deact_threshold).astype(np.int8), axis=0) == -1
+        deact = _table(*_gate(*_crossings(Vm, t, down, deact_threshold), min_downstroke_vel), N)
+        deact = _debounce(deact, min_event_gap_ms)
     vox = np.asarray(snap.vox_idx)
     if S is not None:
         S = np.asarray(S)

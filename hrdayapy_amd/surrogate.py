@@ -29,6 +29,8 @@ FEATURE_NAMES = ["V"] + STATE_NAMES + ["is_endo", "is_mid", "is_epi"]
 IN_LOG = np.array([1 if n in LOG_FEATURES else 0 for n in FEATURE_NAMES], dtype=np.int32)
 KI_BASE, NAI_BASE = INIT["Ki"], INIT["Nai"]
 T_MACRO, N_SUB = 0.2, 10
+# output loss weights of train_surrogate (see SurrogateNet.fit_normalisation)
+LOSS_WEIGHTS = {"V": 4.0, "h": 3.0, "j": 3.0, "m": 2.0}
 
 
 def _bf16(t):
@@ -114,10 +116,12 @@ class SurrogateNet:
         X[:, lg] = torch.log(X[:, lg].clamp_min(1e-30))
         return X
 
-    def fit_normalisation(self, Xt, Dt, dv_weight=1.0):
-        """Per-feature mean/std of the inputs and outputs.  The dV output's scale
-        is divided by dv_weight, which weights its error dv_weight^2 times more
-        in the MSE on normalised outputs (V is what the tissue and the ECG see)."""
+    def fit_normalisation(self, Xt, Dt, weights=None):
+        """Per-feature mean/std of the inputs and outputs.  ``weights`` maps output
+        names (FEATURE_NAMES[:19]: "V" and the state variables) to loss weights w:
+        the output's scale is divided by w, so its error counts w^2 times more in
+        the MSE on normalised outputs (V, and the Na gates h and j that set the
+        conduction velocity, are what the tissue and the ECG see)."""
         self.in_mean = Xt.mean(0).cpu().numpy().astype(np.float32)
         sd = Xt.std(0).cpu().numpy().astype(np.float32)
         sd[19:] = 1.0
@@ -126,7 +130,8 @@ class SurrogateNet:
         self.out_mean = Dt.mean(0).cpu().numpy().astype(np.float32)
         so = Dt.std(0).cpu().numpy().astype(np.float32)
         self.out_std = np.where(so > 1e-20, so, 1e-20).astype(np.float32)
-        self.out_std[0] /= dv_weight
+        for name, w in (weights or {}).items():
+            self.out_std[FEATURE_NAMES.index(name)] /= float(w)
         self.refresh()
 
 
@@ -500,14 +505,15 @@ def build_training_set(eng, n_macro, per_snapshot, seed=0):
     return SurrogateNet.features(V, St, ct), D
 
 
-def train_surrogate(X, D, device="cuda", epochs=8, batch=8192, lr=2e-3, lr_decay=0.65,
-                    holdout=0.05, log=None, seed=0, dv_weight=4.0):
+def train_surrogate(X, D, device="cuda", epochs=14, batch=8192, lr=2e-3, lr_decay=0.75,
+                    holdout=0.05, log=None, seed=0, weights=None):
     """Fit a SurrogateNet with the HIP training kernels; returns (net, trainer, stats)."""
     n = X.shape[0]
     nte = max(batch, int(holdout * n))
     ntr = n - nte
     net = SurrogateNet(device, seed=seed)
-    net.fit_normalisation(SurrogateNet.transform(X[:ntr]), D[:ntr], dv_weight)
+    net.fit_normalisation(SurrogateNet.transform(X[:ntr]), D[:ntr],
+                          LOSS_WEIGHTS if weights is None else weights)
     tr = Trainer(net, batch=batch, lr=lr)
     Xn, Tn = tr.prepare(X, D)
     t0 = time.perf_counter()

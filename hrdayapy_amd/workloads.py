@@ -259,13 +259,22 @@ def twelve_leads(names, phi):
     return np.stack([L[k] for k in LEAD_NAMES], axis=1)
 
 
-def lead_agreement(a, b):
-    """min over the 12 leads of the correlation of a with b, and the max relative L2 error."""
+def lead_agreement(a, b, per_lead=False):
+    """min over the 12 leads of the correlation of a with b, and the max relative
+    L2 error; with per_lead, also the list of (r, relative L2) of every lead."""
     rs, rel = [], []
     for i in range(a.shape[1]):
         x, y = a[:, i], b[:, i]
         if np.std(x) == 0 or np.std(y) == 0:
+            rs.append(None)
+            rel.append(None)
             continue
         rs.append(float(np.corrcoef(x, y)[0, 1]))
         rel.append(float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)))
-    return (min(rs) if rs else None), (max(rel) if rel else None)
+    r_ok = [r for r in rs if r is not None]
+    e_ok = [e for e in rel if e is not None]
+    out = (min(r_ok) if r_ok else None), (max(e_ok) if e_ok else None)
+    if per_lead:
+        return out + ([(None if r is None else round(r, 6), None if e is None else round(e, 4))
+                       for r, e in zip(rs, rel)],)
+    return out

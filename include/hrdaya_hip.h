@@ -330,8 +330,9 @@ int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x,
  * ||r|| <= tol*||r0||; mode 1 at rms(M^-1 r) <= tol (the reference's 'rms'
  * metric, compute_bspm.py:222-256).  ws->scalars then holds rz, pAp, alpha,
  * beta, ||r0||^2, ||r||^2, tol^2, done, iterations, max_iter, ||M^-1 r||^2.
- * r is overwritten. */
-int hp_pcg_begin_f32(const hp_torso* t, const float* r, float* d, float* p, hp_pcg_ws* ws,
+ * r is overwritten: begin replaces it by z = M^-1 r, and the iterations carry
+ * z (not r) in that array. */
+int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
                      double tol, int max_iter, int mode, void* stream);
 int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap,
                        hp_pcg_ws* ws, int n_iter, void* stream);
@@ -415,6 +416,22 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
  * No counterpart in the reference (it outputs raw surface potentials only). */
 int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
                             const double* w, double* out, int64_t n_electrodes, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Threshold crossings of Vm snapshots (replaces the host tables of
+ * compute_activation_maps, activation_maps.py:63-413, with the dV/dt gate of
+ * anisotropic_simulation/functions/activation_maps.py:105-119).
+ * Vm: [n_frames][n] FP32 mV on the device, t: [n_frames] FP64 ms.  rising = 1
+ * counts upward crossings of `threshold` (activations), 0 downward ones.  A
+ * crossing whose |dV/dt| over its frame interval is below min_vel (mV/ms; <= 0:
+ * no gate) is dropped, then one closer than min_gap (ms; <= 0: none) to the
+ * previous kept event of the node.  counts[i] (optional) receives the number
+ * of kept events of node i; table (optional) [n][max_events] their times in
+ * order, NaN padded (events beyond max_events are counted, not stored).
+ * ------------------------------------------------------------------------- */
+int hp_crossing_events_f32(const float* Vm, const double* t, int64_t n_frames, int64_t n,
+                           double threshold, int rising, double min_vel, double min_gap,
+                           int32_t* counts, float* table, int32_t max_events, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Learned ionic surrogate (no counterpart in the reference).  An MLP

@@ -249,3 +249,40 @@ def test_bspm_dropin_matches_direct_solve(cuda, tmp_path):
     b = OE.source(Vm[2], K, proj, 0.2625, 0.4e-3, 1.5, A.shape[0], pr)
     x = grid["solve_fn"](b.copy())
     assert np.abs(x - ref[:, 2]).max() <= 1e-6 * np.abs(ref[:, 2]).max()
+
+
+def _same_table(a, b):
+    assert a.shape == b.shape, (a.shape, b.shape)
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    assert np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
+
+
+def test_activation_tables_on_device_equal_host(cuda):
+    """hp_crossing_events_f32 against the host tables (pinned on hand-worked
+    crossings in tests/test_dropin_host.py): identical bits, for chatter-rich
+    random traces with both velocity gates and the debounce."""
+    print("[case] activation tables device vs host", flush=True)
+    from tests.test_dropin_host import _snapshots
+    rng = np.random.default_rng(4)
+    n_frames, n = 300, 5000
+    t = np.cumsum(rng.uniform(0.5, 1.5, n_frames)).astype(np.float32)
+    base = np.sin(np.outer(t, rng.uniform(0.02, 0.2, n)) + rng.uniform(0, 6, n))
+    Vm = (-40.0 + 60.0 * base + rng.normal(0, 3.0, (n_frames, n))).astype(np.float32)
+    Vm[:, :7] = -40.0  # exactly at the threshold: above, never crossing upward
+    vox = np.stack([np.arange(n) % 50, np.arange(n) // 50, np.zeros(n, int)], 1).astype(np.int32)
+    snap = dict(Vm=Vm, time_vm=t, coords_mm=vox.astype(np.float32), vox_idx=vox)
+    cases = [dict(), dict(min_event_gap_ms=4.0), dict(min_upstroke_vel=15.0),
+             dict(min_downstroke_vel=20.0, min_event_gap_ms=2.5, act_threshold=-20.0)]
+    for kw in cases:
+        h = compute_activation_maps(snap, **kw)
+        d = compute_activation_maps(snap, device=cuda, **kw)
+        _same_table(h.activation_times, d.activation_times)
+        _same_table(h.deactivation_times, d.deactivation_times)
+        assert np.array_equal(h.n_activations, d.n_activations)
+        print(f"  {kw}: {h.activation_times.shape[1]} / {h.deactivation_times.shape[1]} events")
+    # a GPU tensor as Vm, and the hand-worked case
+    dd = compute_activation_maps(dict(snap, Vm=torch.from_numpy(Vm).to(cuda)))
+    _same_table(compute_activation_maps(snap).activation_times, dd.activation_times)
+    s = _snapshots()
+    _same_table(compute_activation_maps(s).activation_times,
+                compute_activation_maps(s, device=cuda).activation_times)
