@@ -14,7 +14,7 @@ from pathlib import Path
 _LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
 _lib = None
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
 
@@ -81,8 +81,8 @@ class HpMg(C.Structure):
 class HpTorso(C.Structure):
     _fields_ = [("ni", c_i32), ("nj", c_i32), ("nk", c_i32), ("pad_", c_i32), ("n_cells", c_i64),
                 ("col_k0", c_vp), ("col_len", c_vp), ("col_off", c_vp), ("gx", c_vp), ("gy", c_vp),
-                ("gz", c_vp), ("diag", c_vp), ("pin", c_i64), ("col_list", c_vp),
-                ("n_col_list", c_i64)]
+                ("gz", c_vp), ("diag", c_vp), ("pin", c_i64), ("chunks", c_vp),
+                ("n_chunks", c_i64)]
 
 
 class HpPcgWs(C.Structure):
@@ -126,6 +126,7 @@ _SIGS = {
                                      c_f64, C.c_int, c_vp]),
     "hp_mgpcg_iterate_f32": (C.c_int, [P(HpTorso), P(HpMg), c_vp, c_vp, c_vp, c_vp, c_vp,
                                        P(HpPcgWs), C.c_int, c_vp]),
+    "hp_torso_chunks": (C.c_int, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_poisson_spmv_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp]),
     "hp_poisson_residual_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_vp]),
     "hp_pcg_begin_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64, C.c_int,
@@ -209,6 +210,20 @@ def call(name: str, *args) -> None:
 
 def exported_symbols() -> list[str]:
     return list(_SIGS)
+
+
+def torso_chunks(shape, col_k0, col_len, col_off):
+    """The chunk table of a column-run torso grid (hp_torso_chunks): int32
+    array [n_chunks, 16], built on the host."""
+    import numpy as np
+    ni, nj, nk = (int(v) for v in shape)
+    a = [np.ascontiguousarray(v, dtype=np.int32) for v in (col_k0, col_len, col_off)]
+    n = C.c_int64(0)
+    call("hp_torso_chunks", ni, nj, nk, *(v.ctypes.data for v in a), None, 0, C.byref(n))
+    tab = np.zeros((max(n.value, 1), 16), dtype=np.int32)
+    call("hp_torso_chunks", ni, nj, nk, *(v.ctypes.data for v in a), tab.ctypes.data,
+         n.value, C.byref(n))
+    return tab[:n.value]
 
 
 def ptr(t) -> int:

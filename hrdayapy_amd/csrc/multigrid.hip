@@ -132,14 +132,32 @@ __global__ void __launch_bounds__(PBLOCK) mg_smooth0_kernel(hp_torso t, const fl
                                                            const float* __restrict__ b, float omega,
                                                            int zero, const double* sc) {
   if (mg_done(sc)) return;
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const float inv = inv0[c];
-    float v = 0.f;
-    if (inv != 0.f)
-      v = zero ? omega * inv * b[c]
-               : xin[c] + omega * inv * (b[c] - apply_row_f32(t, xin, c, i, j, k, m));
-    xout[c] = v;
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  // with zero = 1 there is no input iterate (xin may be null): an empty
+  // resource makes its loads read 0 without touching memory
+  const buf_t X = buf_rsrc(xin, zero ? 0u : vb), XO = buf_rsrc(xout, vb), B = buf_rsrc(b, vb),
+              INV = buf_rsrc(inv0, vb);
+  struct L {
+    float g[6], x[6], xc, b, inv;
+  };
+  for_cells_u<2>(
+      t,
+      [&](uint32_t c, int k, const Chunk& ch) {
+        L l;
+        const RowIdx ix = row_idx(c, k, ch);
+        load_g(t, ix, l.g);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) l.x[f] = buf_ld<float>(X, ix.n[f]);
+        l.xc = buf_ld<float>(X, c);
+        l.b = buf_ld<float>(B, c);
+        l.inv = buf_ld<float>(INV, c);
+        return l;
+      },
+      [&](uint32_t c, int, const Chunk&, const L& l) {
+        const float v = zero ? omega * l.inv * l.b
+                             : l.xc + omega * l.inv * (l.b - row_sum(t, c, l.xc, l.g, l.x));
+        buf_st(XO, c, l.inv != 0.f ? v : 0.f);
+      });
 }
 
 // level 0 -> level 1: b1[C] = sum over the stored children of (r - A x)
@@ -162,7 +180,7 @@ __global__ void __launch_bounds__(256) mg_restrict0_kernel(hp_torso t, const flo
         for (int dk = 0; dk < 2; ++dk) {
           const int k = 2 * K + dk;
           const int c = cell_at(m.c, k);
-          if (c >= 0 && inv0[c] != 0.f) s += r[c] - apply_row_f32(t, x, c, i, j, k, m);
+          if (c >= 0 && inv0[c] != 0.f) s += r[c] - apply_row_f32(t, x, c, k, ColLookup{m});
         }
       }
     }
@@ -175,8 +193,8 @@ __global__ void __launch_bounds__(PBLOCK) mg_prolong0_kernel(hp_torso t, const f
                                                             float* __restrict__ z, hp_mg_level Cl,
                                                             const double* sc) {
   if (mg_done(sc)) return;
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr&) {
-    if (inv0[c] != 0.f) z[c] += Cl.x[((int64_t)(i >> 1) * Cl.nj + (j >> 1)) * Cl.nk + (k >> 1)];
+  for_cells(t, [&](int c, int k, const Chunk& ch) {
+    if (inv0[c] != 0.f) z[c] += Cl.x[((int64_t)(ch.i >> 1) * Cl.nj + (ch.j >> 1)) * Cl.nk + (k >> 1)];
   });
 }
 
@@ -244,25 +262,30 @@ static void vcycle(const hp_torso* t, const hp_mg* mg, const float* r, float* z,
 __global__ void __launch_bounds__(PBLOCK) mgpcg_init_kernel(hp_torso t, const float* __restrict__ r,
                                                            const float* __restrict__ z,
                                                            float* __restrict__ d,
-                                                           float* __restrict__ p, double* part,
-                                                           int* counter, double* sc, double tol,
-                                                           int maxit) {
+                                                           float* __restrict__ p, double* part) {
   double v[2] = {0.0, 0.0};
-  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
+  for_cells(t, [&](int c, int, const Chunk&) {
     const float rc = r[c], zc = z[c];
     d[c] = 0.f;
     p[c] = zc;
     v[0] += (double)rc * (double)zc;
     v[1] += (double)rc * (double)rc;
   });
+  block_partials<2>(v, part);
+}
+
+__global__ void __launch_bounds__(PBLOCK) mgpcg_init_finish_kernel(const double* __restrict__ part,
+                                                                   int nb, double* sc, double tol,
+                                                                   int maxit, int64_t n_cells) {
   double red[2];
-  if (block_reduce_last<2>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<2>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_RZ] = red[0];
     sc[S_RR0] = red[1];
     sc[S_RR] = red[1];
     sc[S_ZZ] = 0.0;
     sc[S_MODE] = 0.0;
-    sc[S_NCELL] = (double)t.n_cells;
+    sc[S_NCELL] = (double)n_cells;
     sc[S_TOL2] = tol * tol;
     sc[S_ITER] = 0.0;
     sc[S_MAXIT] = (double)maxit;
@@ -271,28 +294,25 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_init_kernel(hp_torso t, const fl
   }
 }
 
-// Ap = A (z + beta p), pAp, alpha   (p itself on the first iteration)
+// Ap = A (z + beta p), pAp, alpha   (beta = 0 and p = z on the first iteration)
 __global__ void __launch_bounds__(PBLOCK) mgpcg_spmv_kernel(hp_torso t, const float* __restrict__ z,
                                                            const float* __restrict__ p,
                                                            float* __restrict__ Ap, double* part,
-                                                           int* counter, double* sc) {
+                                                           const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   const float beta = (float)sc[S_BETA];
-  const bool first = sc[S_ITER] == 0.0;
-  auto pnew = [&](int n) -> float { return first ? p[n] : z[n] + beta * p[n]; };
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t Z = buf_rsrc(z, vb), P = buf_rsrc(p, vb), AP = buf_rsrc(Ap, vb);
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const float pc = pnew(c);
-    const float acc = row_f32(t, c, k, m, pc, pnew);
-    Ap[c] = acc;
-    v[0] += (double)pc * (double)acc;
-  });
-  double red[1];
-  if (block_reduce_last<1>(v, part, counter, red) && threadIdx.x == 0) {
-    sc[S_PAP] = red[0];
-    if (!(fabs(red[0]) > 1e-300)) sc[S_DONE] = 2.0;
-    else sc[S_ALPHA] = sc[S_RZ] / red[0];
-  }
+  for_cells_u<2>(
+      t, [&](uint32_t c, int k, const Chunk& ch) { return dir_loads(t, Z, P, c, k, ch); },
+      [&](uint32_t c, int, const Chunk&, const DirLoads& l) {
+        float pc;
+        const float acc = dir_product(t, c, beta, l, pc);
+        buf_st(AP, c, acc);
+        v[0] = c != T_OOB ? v[0] + (double)pc * (double)acc : v[0];
+      });
+  block_partials<1>(v, part);
 }
 
 // p = z + beta p (materialised), d += alpha p, r -= alpha Ap, ||r||^2, stopping test
@@ -301,23 +321,41 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_update_kernel(hp_torso t, const 
                                                              float* __restrict__ d,
                                                              float* __restrict__ p,
                                                              const float* __restrict__ Ap,
-                                                             double* part, int* counter,
-                                                             double* sc) {
+                                                             double* part,
+                                                             const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   const float alpha = (float)sc[S_ALPHA];
   const float beta = (float)sc[S_BETA];
-  const bool first = sc[S_ITER] == 0.0;
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
-    const float pc = first ? p[c] : z[c] + beta * p[c];
-    p[c] = pc;
-    d[c] += alpha * pc;
-    const float rc = r[c] - alpha * Ap[c];
-    r[c] = rc;
-    v[0] += (double)rc * (double)rc;
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t Z = buf_rsrc(z, vb), R = buf_rsrc(r, vb), D = buf_rsrc(d, vb), P = buf_rsrc(p, vb),
+              AP = buf_rsrc(Ap, vb);
+  struct L {
+    float z, p, d, r, ap;
+  };
+  for_cells_u<4>(
+      t,
+      [&](uint32_t c, int, const Chunk&) {
+        return L{buf_ld<float>(Z, c), buf_ld<float>(P, c), buf_ld<float>(D, c),
+                 buf_ld<float>(R, c), buf_ld<float>(AP, c)};
+      },
+      [&](uint32_t c, int, const Chunk&, const L& l) {
+        const float pc = l.z + beta * l.p;
+        buf_st(P, c, pc);
+        buf_st(D, c, l.d + alpha * pc);
+        const float rc = l.r - alpha * l.ap;
+        buf_st(R, c, rc);
+        v[0] = c != T_OOB ? v[0] + (double)rc * (double)rc : v[0];
+      });
+  block_partials<1>(v, part);
+}
+
+__global__ void __launch_bounds__(PBLOCK) mgpcg_update_finish_kernel(const double* __restrict__ part,
+                                                                     int nb, double* sc) {
+  if (sc[S_DONE] != 0.0) return;
   double red[1];
-  if (block_reduce_last<1>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<1>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_RR] = red[0];
     sc[S_ITER] += 1.0;
     if (red[0] <= sc[S_TOL2] * sc[S_RR0] || sc[S_ITER] >= sc[S_MAXIT]) sc[S_DONE] = 1.0;
@@ -327,14 +365,26 @@ __global__ void __launch_bounds__(PBLOCK) mgpcg_update_kernel(hp_torso t, const 
 // rz = r.z (new z), beta = rz / rz_old
 __global__ void __launch_bounds__(PBLOCK) mgpcg_rz_kernel(hp_torso t, const float* __restrict__ r,
                                                          const float* __restrict__ z, double* part,
-                                                         int* counter, double* sc) {
+                                                         const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   double v[1] = {0.0};
-  for_cells(t, [&](int c, int, int, int, const ColNbr&) {
-    v[0] += (double)r[c] * (double)z[c];
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t R = buf_rsrc(r, vb), Z = buf_rsrc(z, vb);
+  struct L {
+    float r, z;
+  };
+  for_cells_u<4>(
+      t, [&](uint32_t c, int, const Chunk&) { return L{buf_ld<float>(R, c), buf_ld<float>(Z, c)}; },
+      [&](uint32_t, int, const Chunk&, const L& l) { v[0] += (double)l.r * (double)l.z; });
+  block_partials<1>(v, part);
+}
+
+__global__ void __launch_bounds__(PBLOCK) mgpcg_rz_finish_kernel(const double* __restrict__ part,
+                                                                 int nb, double* sc) {
+  if (sc[S_DONE] != 0.0) return;
   double red[1];
-  if (block_reduce_last<1>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<1>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_BETA] = red[0] / sc[S_RZ];
     sc[S_RZ] = red[0];
   }
@@ -388,8 +438,10 @@ int hp_mgpcg_begin_f32(const hp_torso* t, const hp_mg* mg, const float* r, float
   if (!mg_ok(t, mg, "hp_mgpcg_begin_f32") || !ws) return -1;
   hipStream_t st = (hipStream_t)stream;
   vcycle(t, mg, r, z, nullptr, st);
-  hipLaunchKernelGGL(mgpcg_init_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, st, *t, r, z, d, p,
-                     ws->part, ws->counter, ws->scalars, tol, max_iter);
+  const int nb = grid_blocks(t);
+  hipLaunchKernelGGL(mgpcg_init_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, z, d, p, ws->part);
+  hipLaunchKernelGGL(mgpcg_init_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
+                     ws->scalars, tol, max_iter, t->n_cells);
   return check_launch("hp_mgpcg_begin_f32");
 }
 
@@ -400,12 +452,17 @@ int hp_mgpcg_iterate_f32(const hp_torso* t, const hp_mg* mg, float* r, float* d,
   hipStream_t st = (hipStream_t)stream;
   for (int it = 0; it < n_iter; ++it) {
     hipLaunchKernelGGL(mgpcg_spmv_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, z, p, Ap, ws->part,
-                       ws->counter, ws->scalars);
+                       ws->scalars);
+    launch_pap_finish(ws->part, nb, ws->scalars, st);
     hipLaunchKernelGGL(mgpcg_update_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, z, r, d, p, Ap,
-                       ws->part, ws->counter, ws->scalars);
+                       ws->part, ws->scalars);
+    hipLaunchKernelGGL(mgpcg_update_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
+                       ws->scalars);
     vcycle(t, mg, r, z, ws->scalars, st);
     hipLaunchKernelGGL(mgpcg_rz_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, z, ws->part,
-                       ws->counter, ws->scalars);
+                       ws->scalars);
+    hipLaunchKernelGGL(mgpcg_rz_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
+                       ws->scalars);
   }
   return check_launch("hp_mgpcg_iterate_f32");
 }

@@ -19,53 +19,85 @@
 //
 // The PCG loop runs without host round trips: alpha/beta, the stopping test
 // and an iteration counter live in device memory, every kernel returns at
-// once when the `done` flag is set, and reductions are finished by the last
-// block to arrive (fixed summation order: deterministic results).
+// once when the `done` flag is set.  A reducing kernel writes one partial sum
+// per block and a one-block finish kernel after it sums them in block order
+// and updates the scalars (fixed order: deterministic results; no fences).
 #include "poisson_common.h"
 
 namespace hp {
 
 __global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const float* __restrict__ x,
                                                           float* __restrict__ y) {
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    y[c] = apply_row_f32(t, x, c, i, j, k, m);
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t X = buf_rsrc(x, vb), Y = buf_rsrc(y, vb);
+  struct L {
+    float g[6], x[6], xc;
+  };
+  for_cells_u<2>(
+      t,
+      [&](uint32_t c, int k, const Chunk& ch) {
+        L l;
+        const RowIdx ix = row_idx(c, k, ch);
+        load_g(t, ix, l.g);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) l.x[f] = buf_ld<float>(X, ix.n[f]);
+        l.xc = buf_ld<float>(X, c);
+        return l;
+      },
+      [&](uint32_t c, int, const Chunk&, const L& l) { buf_st(Y, c, row_sum(t, c, l.xc, l.g, l.x)); });
 }
 
-// FP64 row of the operator in zero-row-sum form: sum_f g_f (x_c - x_nb); a
-// cell without any conducting face (hole, isolated voxel) and the pinned cell
-// are identity rows.  The diagonal is the exact FP64 sum of the FP32 face
-// conductances, so every interior row sums to zero exactly.
+// FP64 row of the operator in zero-row-sum form: sum_f g_f (x_c - x_nb), faces
+// in the order +z, -z, +x, -x, +y, -y; a cell without any conducting face
+// (hole, isolated voxel) and the pinned cell are identity rows.  The diagonal
+// is the exact FP64 sum of the FP32 face conductances, so every interior row
+// sums to zero exactly.  Absent neighbours read g = 0 and x = 0 (T_OOB).
 __device__ __forceinline__ double apply_row_f64(const hp_torso& t, const double* __restrict__ x,
-                                                int c, int i, int j, int k, const ColNbr& m) {
-  const double xc = x[c];
-  if (c == t.pin) return xc;
+                                                uint32_t c, int k, const Chunk& nb) {
+  const uint32_t gb = (uint32_t)t.n_cells * 4u;
+  const buf_t Gx = buf_rsrc(t.gx, gb), Gy = buf_rsrc(t.gy, gb), Gz = buf_rsrc(t.gz, gb);
+  const buf_t X = buf_rsrc(x, (uint32_t)t.n_cells * 8u);
+  const uint32_t n0 = nb(0, k + 1), n1 = nb(0, k - 1), n2 = nb(1, k), n3 = nb(2, k),
+                 n4 = nb(3, k), n5 = nb(4, k);
+  const double g0 = buf_ld<float>(Gz, n0 == T_OOB ? T_OOB : c), g1 = buf_ld<float>(Gz, n1),
+               g2 = buf_ld<float>(Gx, n2 == T_OOB ? T_OOB : c), g3 = buf_ld<float>(Gx, n3),
+               g4 = buf_ld<float>(Gy, n4 == T_OOB ? T_OOB : c), g5 = buf_ld<float>(Gy, n5);
+  const double xc = buf_ld<double>(X, c);
+  const double x0 = buf_ld<double>(X, n0), x1 = buf_ld<double>(X, n1), x2 = buf_ld<double>(X, n2),
+               x3 = buf_ld<double>(X, n3), x4 = buf_ld<double>(X, n4), x5 = buf_ld<double>(X, n5);
   double acc = 0.0, gs = 0.0;
-  double g;
-  if (k + 1 < m.c.k0 + m.c.len && (g = t.gz[c]) != 0.0) { acc += g * (xc - x[c + 1]); gs += g; }
-  if (k - 1 >= m.c.k0 && (g = t.gz[c - 1]) != 0.0) { acc += g * (xc - x[c - 1]); gs += g; }
-  int n;
-  if ((n = cell_at(m.xp, k)) >= 0 && (g = t.gx[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(m.xm, k)) >= 0 && (g = t.gx[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(m.yp, k)) >= 0 && (g = t.gy[c]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  if ((n = cell_at(m.ym, k)) >= 0 && (g = t.gy[n]) != 0.0) { acc += g * (xc - x[n]); gs += g; }
-  return gs == 0.0 ? xc : acc;
+  acc += g0 * (xc - x0); gs += g0;
+  acc += g1 * (xc - x1); gs += g1;
+  acc += g2 * (xc - x2); gs += g2;
+  acc += g3 * (xc - x3); gs += g3;
+  acc += g4 * (xc - x4); gs += g4;
+  acc += g5 * (xc - x5); gs += g5;
+  return ((int64_t)c == t.pin || gs == 0.0) ? xc : acc;
 }
 
 // r = b - A x in FP64 and ||r||^2, ||b||^2 -> out[0], out[1]
 __global__ void __launch_bounds__(PBLOCK) residual_f64_kernel(hp_torso t, const double* __restrict__ b,
                                                               const double* __restrict__ x,
                                                               double* __restrict__ r,
-                                                              double* part, int* counter,
-                                                              double* out) {
+                                                              double* part) {
   double v[2] = {0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const double rc = b[c] - apply_row_f64(t, x, c, i, j, k, m);
+  for_cells(t, [&](int c, int k, const Chunk& ch) {
+    const double rc = b[c] - apply_row_f64(t, x, c, k, ch);
     r[c] = rc;
     v[0] += rc * rc;
     v[1] += b[c] * b[c];
   });
-  block_reduce_last<2>(v, part, counter, out);
+  block_partials<2>(v, part);
+}
+
+// out[0..NV) = sums of the nb block partials of the kernel before
+template <int NV>
+__global__ void __launch_bounds__(PBLOCK) finish_sums_kernel(const double* __restrict__ part, int nb,
+                                                             double* __restrict__ out) {
+  double red[NV];
+  sum_partials<NV>(part, nb, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NV; ++q) out[q] = red[q];
 }
 
 // PCG init for A d = r (FP32), d = 0: z = r/diag (stored over r: the solve
@@ -73,11 +105,9 @@ __global__ void __launch_bounds__(PBLOCK) residual_f64_kernel(hp_torso t, const 
 // neighbour and divides nothing), p = z, rz, rr0 = ||r||^2.
 __global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, float* __restrict__ r,
                                                           float* __restrict__ d,
-                                                          float* __restrict__ p, double* part,
-                                                          int* counter, double* sc, double tol,
-                                                          int maxit, int mode) {
+                                                          float* __restrict__ p, double* part) {
   double v[3] = {0.0, 0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
+  for_cells(t, [&](int c, int, const Chunk&) {
     const float rc = r[c];
     const float z = rc / t.diag[c];
     r[c] = z;
@@ -87,49 +117,73 @@ __global__ void __launch_bounds__(PBLOCK) pcg_init_kernel(hp_torso t, float* __r
     v[1] += (double)rc * (double)rc;
     v[2] += (double)z * (double)z;
   });
+  block_partials<3>(v, part);
+}
+
+__global__ void __launch_bounds__(PBLOCK) pcg_init_finish_kernel(const double* __restrict__ part,
+                                                                 int nb, double* sc, double tol,
+                                                                 int maxit, int mode,
+                                                                 int64_t n_cells) {
   double red[3];
-  if (block_reduce_last<3>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<3>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_RZ] = red[0];
     sc[S_RR0] = red[1];
     sc[S_RR] = red[1];
     sc[S_ZZ] = red[2];
     sc[S_MODE] = (double)mode;
-    sc[S_NCELL] = (double)t.n_cells;
+    sc[S_NCELL] = (double)n_cells;
     sc[S_TOL2] = tol * tol;
     sc[S_ITER] = 0.0;
     sc[S_MAXIT] = (double)maxit;
     sc[S_BETA] = 0.0;
-    const bool conv = mode == 0 ? red[1] == 0.0 : red[2] <= tol * tol * (double)t.n_cells;
+    const bool conv = mode == 0 ? red[1] == 0.0 : red[2] <= tol * tol * (double)n_cells;
     sc[S_DONE] = (conv || maxit <= 0) ? 1.0 : 0.0;
   }
 }
 
-// p = z + beta p (skipped on the first iteration: beta = 0 and p = z already),
+// p = z + beta p (on the first iteration beta = 0 and p = z: p stays z),
 // Ap = A p, pAp.  z (held in the r array) and the old p of a neighbour give
 // its new p on the fly.
 __global__ void __launch_bounds__(PBLOCK) pcg_spmv_kernel(hp_torso t, const float* __restrict__ z,
                                                           float* __restrict__ p,
                                                           float* __restrict__ Ap, double* part,
-                                                          int* counter, double* sc) {
+                                                          const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   const float beta = (float)sc[S_BETA];
-  const bool first = sc[S_ITER] == 0.0;
   double v[1] = {0.0};
-  auto pnew = [&](int n) -> float { return first ? p[n] : z[n] + beta * p[n]; };
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const float pc = pnew(c);
-    const float acc = row_f32(t, c, k, m, pc, pnew);
-    Ap[c] = acc;
-    v[0] += (double)pc * (double)acc;
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t Z = buf_rsrc(z, vb), P = buf_rsrc(p, vb), AP = buf_rsrc(Ap, vb);
+  // on the first iteration beta = 0 and p = z, so the direction is z (exactly)
+  for_cells_u<2>(
+      t, [&](uint32_t c, int k, const Chunk& ch) { return dir_loads(t, Z, P, c, k, ch); },
+      [&](uint32_t c, int, const Chunk&, const DirLoads& l) {
+        float pc;
+        const float acc = dir_product(t, c, beta, l, pc);
+        buf_st(AP, c, acc);
+        v[0] = c != T_OOB ? v[0] + (double)pc * (double)acc : v[0];
+      });
   // p itself is not written here (other blocks still read the old p of their
   // neighbours); pcg_update materialises the same z + beta*p next.
+  block_partials<1>(v, part);
+}
+
+// pAp from the partials of the product kernel; alpha = rz / pAp (a vanishing
+// pAp ends the solve: done = 2).  Shared by both PCG forms.
+__global__ void __launch_bounds__(PBLOCK) pap_finish_kernel(const double* __restrict__ part, int nb,
+                                                            double* sc) {
+  if (sc[S_DONE] != 0.0) return;
   double red[1];
-  if (block_reduce_last<1>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<1>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_PAP] = red[0];
     if (!(fabs(red[0]) > 1e-300)) sc[S_DONE] = 2.0;
     else sc[S_ALPHA] = sc[S_RZ] / red[0];
   }
+}
+
+void launch_pap_finish(double* part, int nb, double* sc, hipStream_t st) {
+  hipLaunchKernelGGL(pap_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, part, nb, sc);
 }
 
 // p <- z + beta p (materialised), d += alpha p, z -= alpha Ap/diag (z = M^-1 r
@@ -139,28 +193,46 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
                                                             float* __restrict__ d,
                                                             float* __restrict__ p,
                                                             const float* __restrict__ Ap,
-                                                            double* part, int* counter,
-                                                            double* sc) {
+                                                            double* part, const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   const float alpha = (float)sc[S_ALPHA];
   const float beta = (float)sc[S_BETA];
-  const bool first = sc[S_ITER] == 0.0;
   double v[3] = {0.0, 0.0, 0.0};
-  for_cells(t, [&](int c, int i, int j, int k, const ColNbr& m) {
-    const float zc = z[c];
-    const float pc = first ? p[c] : zc + beta * p[c];
-    p[c] = pc;
-    d[c] += alpha * pc;
-    const float dg = t.diag[c];
-    const float zn = zc - alpha * (Ap[c] / dg);
-    z[c] = zn;
-    const double rn = (double)dg * (double)zn;
-    v[0] += rn * (double)zn;
-    v[1] += rn * rn;
-    v[2] += (double)zn * (double)zn;
-  });
+  const uint32_t vb = (uint32_t)t.n_cells * 4u;
+  const buf_t Z = buf_rsrc(z, vb), D = buf_rsrc(d, vb), P = buf_rsrc(p, vb),
+              AP = buf_rsrc(Ap, vb), DG = buf_rsrc(t.diag, vb);
+  struct L {
+    float z, p, d, ap, dg;
+  };
+  for_cells_u<4>(
+      t,
+      [&](uint32_t c, int, const Chunk&) {
+        return L{buf_ld<float>(Z, c), buf_ld<float>(P, c), buf_ld<float>(D, c),
+                 buf_ld<float>(AP, c), buf_ld<float>(DG, c)};
+      },
+      [&](uint32_t c, int, const Chunk&, const L& l) {
+        const float pc = l.z + beta * l.p;
+        buf_st(P, c, pc);
+        buf_st(D, c, l.d + alpha * pc);
+        const float zn = l.z - alpha * (l.ap / l.dg);
+        buf_st(Z, c, zn);
+        const bool in = c != T_OOB;
+        const double rn = (double)l.dg * (double)zn;
+        v[0] = in ? v[0] + rn * (double)zn : v[0];
+        v[1] = in ? v[1] + rn * rn : v[1];
+        v[2] = in ? v[2] + (double)zn * (double)zn : v[2];
+      });
+  block_partials<3>(v, part);
+}
+
+// rz, ||r||^2, ||z||^2 from the update's partials: beta, stopping test,
+// iteration count
+__global__ void __launch_bounds__(PBLOCK) pcg_update_finish_kernel(const double* __restrict__ part,
+                                                                   int nb, double* sc) {
+  if (sc[S_DONE] != 0.0) return;
   double red[3];
-  if (block_reduce_last<3>(v, part, counter, red) && threadIdx.x == 0) {
+  sum_partials<3>(part, nb, red);
+  if (threadIdx.x == 0) {
     sc[S_BETA] = red[0] / sc[S_RZ];
     sc[S_RZ] = red[0];
     sc[S_RR] = red[1];
@@ -176,13 +248,13 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
 // x64 += scale * d
 __global__ void axpy_f64_kernel(hp_torso t, double* __restrict__ x, const float* __restrict__ d,
                                 double scale) {
-  for_cells(t, [&](int c, int, int, int, const ColNbr&) { x[c] += scale * (double)d[c]; });
+  for_cells(t, [&](int c, int, const Chunk&) { x[c] += scale * (double)d[c]; });
 }
 
 // r32 = r64 * scale
 __global__ void cast_scale_kernel(hp_torso t, const double* __restrict__ r64, float* __restrict__ r32,
                                   double scale) {
-  for_cells(t, [&](int c, int, int, int, const ColNbr&) { r32[c] = (float)(r64[c] * scale); });
+  for_cells(t, [&](int c, int, const Chunk&) { r32[c] = (float)(r64[c] * scale); });
 }
 
 // ---------------------------------------------------------------------------
@@ -228,12 +300,11 @@ __global__ void __launch_bounds__(BX * BY * BZ) heart_lap6_kernel(hp_bricks g, c
 // mean over the valid fine voxels of lap -> out[0]
 __global__ void __launch_bounds__(PBLOCK) masked_sum_kernel(const double* __restrict__ lap,
                                                             const int32_t* __restrict__ slots,
-                                                            int64_t n, double* part, int* counter,
-                                                            double* out) {
+                                                            int64_t n, double* part) {
   double v[1] = {0.0};
   for (int64_t q = (int64_t)blockIdx.x * PBLOCK + threadIdx.x; q < n; q += (int64_t)gridDim.x * PBLOCK)
     v[0] += lap[slots[q]];
-  block_reduce_last<1>(v, part, counter, out);
+  block_partials<1>(v, part);
 }
 
 // b[row] = coeff*(sum_{fine in row} lap - cnt*mean)/n_expected; b[pin] = 0
@@ -340,6 +411,59 @@ using namespace hp;
 
 extern "C" {
 
+int hp_torso_chunks(int32_t ni, int32_t nj, int32_t nk, const int32_t* col_k0,
+                    const int32_t* col_len, const int32_t* col_off, int32_t* chunks,
+                    int64_t capacity, int64_t* n_chunks) {
+  if (ni <= 0 || nj <= 0 || nk <= 0 || nk > 32767 || !col_k0 || !col_len || !col_off ||
+      !n_chunks || (chunks && capacity < 0)) {
+    set_error("hp_torso_chunks: bad arguments (nk <= 32767)");
+    return -1;
+  }
+  // column (i, j): base = off - k0, span = k0 | (k0 + len) << 16
+  auto col = [&](int i, int j, int32_t& base, int32_t& span) {
+    base = 0;
+    span = 0;
+    if (i < 0 || i >= ni || j < 0 || j >= nj) return;
+    const int64_t c = (int64_t)i * nj + j;
+    if (col_len[c] <= 0) return;
+    base = col_off[c] - col_k0[c];
+    span = col_k0[c] | ((col_k0[c] + col_len[c]) << 16);
+  };
+  int64_t n = 0;
+  for (int i = 0; i < ni; ++i)
+    for (int j = 0; j < nj; ++j) {
+      const int64_t c = (int64_t)i * nj + j;
+      const int k0 = col_k0[c], len = col_len[c];
+      if (len <= 0) continue;
+      if (k0 < 0 || k0 + len > nk) {
+        set_error("hp_torso_chunks: column (%d, %d) run [%d, %d) outside [0, %d)", i, j, k0,
+                  k0 + len, nk);
+        return -1;
+      }
+      for (int dk = 0; dk < len; dk += 64, ++n) {
+        if (!chunks) continue;
+        if (n >= capacity) {
+          set_error("hp_torso_chunks: capacity %lld too small", (long long)capacity);
+          return -1;
+        }
+        int32_t* r = chunks + n * HP_TORSO_CHUNK_INTS;
+        r[0] = k0 + dk;
+        r[1] = len - dk < 64 ? len - dk : 64;
+        r[2] = i;
+        r[3] = j;
+        col(i, j, r[4], r[9]);
+        col(i + 1, j, r[5], r[10]);
+        col(i - 1, j, r[6], r[11]);
+        col(i, j + 1, r[7], r[12]);
+        col(i, j - 1, r[8], r[13]);
+        r[14] = 0;
+        r[15] = 0;
+      }
+    }
+  *n_chunks = n;
+  return 0;
+}
+
 int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* stream) {
   if (!torso_ok(t, "hp_poisson_spmv_f32")) return -1;
   hipLaunchKernelGGL(spmv_f32_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,
@@ -350,16 +474,21 @@ int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* strea
 int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x, double* r,
                             hp_pcg_ws* ws, void* stream) {
   if (!torso_ok(t, "hp_poisson_residual_f64") || !ws) return -1;
-  hipLaunchKernelGGL(residual_f64_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0,
-                     (hipStream_t)stream, *t, b, x, r, ws->part, ws->counter, ws->out);
+  const int nb = grid_blocks(t);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(residual_f64_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, b, x, r, ws->part);
+  hipLaunchKernelGGL(finish_sums_kernel<2>, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb, ws->out);
   return check_launch("hp_poisson_residual_f64");
 }
 
 int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
                      double tol, int max_iter, int mode, void* stream) {
   if (!torso_ok(t, "hp_pcg_begin_f32") || !ws) return -1;
-  hipLaunchKernelGGL(pcg_init_kernel, dim3(grid_blocks(t)), dim3(PBLOCK), 0, (hipStream_t)stream,
-                     *t, r, d, p, ws->part, ws->counter, ws->scalars, tol, max_iter, mode);
+  const int nb = grid_blocks(t);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(pcg_init_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, d, p, ws->part);
+  hipLaunchKernelGGL(pcg_init_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
+                     ws->scalars, tol, max_iter, mode, t->n_cells);
   return check_launch("hp_pcg_begin_f32");
 }
 
@@ -370,9 +499,12 @@ int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* A
   hipStream_t st = (hipStream_t)stream;
   for (int it = 0; it < n_iter; ++it) {
     hipLaunchKernelGGL(pcg_spmv_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, p, Ap, ws->part,
-                       ws->counter, ws->scalars);
+                       ws->scalars);
+    launch_pap_finish(ws->part, nb, ws->scalars, st);
     hipLaunchKernelGGL(pcg_update_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, d, p, Ap,
-                       ws->part, ws->counter, ws->scalars);
+                       ws->part, ws->scalars);
+    hipLaunchKernelGGL(pcg_update_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
+                       ws->scalars);
   }
   return check_launch("hp_pcg_iterate_f32");
 }
@@ -442,7 +574,8 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
   int nb = (int)((n_valid + PBLOCK - 1) / PBLOCK);
   if (nb > 1024) nb = 1024;
   hipLaunchKernelGGL(masked_sum_kernel, dim3(nb), dim3(PBLOCK), 0, st, lap, valid_slots, n_valid,
-                     ws->part, ws->counter, ws->out);
+                     ws->part);
+  hipLaunchKernelGGL(finish_sums_kernel<1>, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb, ws->out);
   int rc = check_launch("hp_bspm_source_f64/sum");
   if (rc) return rc;
   if (n_rows_hit > 0)

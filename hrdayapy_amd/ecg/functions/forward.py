@@ -147,18 +147,14 @@ class PoissonSolver:
         self.n = n = int(L["n_cells"])
         self._t = {k: torch.from_numpy(np.ascontiguousarray(L[k])).to(dev)
                    for k in ("col_k0", "col_len", "col_off", "gx", "gy", "gz", "diag")}
-        # non-empty columns, longest first (the grid-stride loop then ends on
-        # short columns: less tail imbalance)
-        lens = np.asarray(L["col_len"])
-        cols = np.nonzero(lens > 0)[0]
-        cols = cols[np.argsort(-lens[cols], kind="stable")].astype(np.int32)
-        self._t["col_list"] = torch.from_numpy(cols).to(dev)
         ni, nj, nk = grid["shape"]
+        tab = nat.torso_chunks((ni, nj, nk), L["col_k0"], L["col_len"], L["col_off"])
+        self._t["chunks"] = torch.from_numpy(tab).to(dev)
         t = self._t
         self.torso = nat.HpTorso(ni, nj, nk, 0, n, t["col_k0"].data_ptr(), t["col_len"].data_ptr(),
                                  t["col_off"].data_ptr(), t["gx"].data_ptr(), t["gy"].data_ptr(),
                                  t["gz"].data_ptr(), t["diag"].data_ptr(), int(L["pin"]),
-                                 t["col_list"].data_ptr(), len(cols))
+                                 t["chunks"].data_ptr(), len(tab))
         self.part = torch.zeros(4 * 4096, dtype=torch.float64, device=dev)
         self.counter = torch.zeros(4, dtype=torch.int32, device=dev)
         self.out = torch.zeros(4, dtype=torch.float64, device=dev)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 1
+#define HP_ABI_VERSION 2
 
 /* ---------------------------------------------------------------------------
  * Library
@@ -305,12 +305,28 @@ typedef struct hp_torso {
   const float* gz;
   const float* diag;
   int64_t pin;             /* stored index of the pinned cell, or -1 */
-  const int32_t* col_list; /* the columns with a non-empty run (kernels visit these) */
-  int64_t n_col_list;
+  const int32_t* chunks;   /* chunk table (device memory), from hp_torso_chunks */
+  int64_t n_chunks;
 } hp_torso;
 
-/* Reduction workspace: part >= 3*4096 doubles, counter = one int (zero at
- * creation; every kernel leaves it at zero), out >= 2 doubles, scalars >= 16
+/* The kernels walk the stored cells in chunks of up to 64 consecutive cells
+ * of one column run, one wave per chunk.  Each chunk is a record of
+ * HP_TORSO_CHUNK_INTS int32: k of its first cell, its cell count, i, j, then
+ * for the own column and the +x, -x, +y, -y neighbour columns the index base
+ * (stored index of cell k = base + k; 5 values) and the stored k-range
+ * lo | hi << 16 (5 values; 0 for a column outside the box or without cells),
+ * then two zeros.  Chunks follow the stored order.
+ * hp_torso_chunks fills the table on the host from the column arrays (host
+ * memory); with chunks == NULL it only counts.  Requires nk <= 32767.
+ * Returns 0, or -1 if the capacity is too small or an argument is bad.
+ * No counterpart in the reference (a layout helper of this library). */
+#define HP_TORSO_CHUNK_INTS 16
+int hp_torso_chunks(int32_t ni, int32_t nj, int32_t nk, const int32_t* col_k0,
+                    const int32_t* col_len, const int32_t* col_off, int32_t* chunks,
+                    int64_t capacity, int64_t* n_chunks);
+
+/* Reduction workspace: part >= 3*4096 doubles (block partial sums), counter:
+ * not used (kept for the layout; may be NULL), out >= 2 doubles, scalars >= 16
  * doubles (PCG state: rz, pAp, alpha, beta, rr0, rr, tol^2, done, iter, maxit). */
 typedef struct hp_pcg_ws {
   double* part;

@@ -33,19 +33,27 @@ def _to_cells(g, v_rows, dev):
     return torch.from_numpy(out).to(dev)
 
 
-def test_operator_matches_oracle(cuda):
-    print("[case] torso operator", flush=True)
-    lab, g, Ap = _setup()
+@pytest.mark.parametrize("n", [(30, 26, 22), (72, 64, 160)])
+def test_operator_matches_oracle(cuda, n):
+    """FP64 residual and FP32 product against the oracle stiffness; the larger
+    grid has runs of several 64-cell chunks and a full grid split between
+    the XCDs."""
+    print(f"[case] torso operator {n}", flush=True)
+    lab, g, Ap = _setup(n)
     ps = PoissonSolver(g, cuda)
+    if n[2] > 64:
+        assert ps.torso.n_chunks >= 8 * 4 * 8 and int(g.layout["col_len"].max()) > 64
     x = np.random.default_rng(0).standard_normal(Ap.shape[0])
     x[g["pin_row"]] = 0.0
     b = np.zeros_like(x)
     xc, bc = _to_cells(g, x, cuda), _to_cells(g, b, cuda)
     r = torch.empty_like(xc)
-    ps.residual(bc, xc, r)
+    rn, bn = ps.residual(bc, xc, r)
     ax = -r.cpu().numpy()[g.cell_of_row]
     ref = Ap @ x
     assert np.max(np.abs(ax - ref)) < 1e-6 * np.abs(ref).max()
+    # the block-partial reduction of ||r||^2 (last block to arrive)
+    assert bn == 0.0 and abs(rn - np.linalg.norm(ref)) < 1e-6 * np.linalg.norm(ref)
     x32 = xc.float()
     y32 = torch.empty_like(x32)
     ps.spmv32(x32, y32)

@@ -1,6 +1,7 @@
 """Host-side construction of the torso multigrid hierarchy: coarse operators,
 pinned cell, dense coarsest inverse."""
 import numpy as np
+import pytest
 
 from hrdayapy_amd.ecg.functions.forward import (Multigrid, _face_sum, coarsen_faces,
                                                 dense_operator)
@@ -83,4 +84,42 @@ def test_solver_setup_on_host_memory():
     L = g.layout
     faces = L["gx"][ps.pin_nbr.numpy()]  # neighbours exist and conduct
     assert len(ps.pin_nbr) >= 1 and ps.pin_G > 0 and np.all(np.isfinite(faces))
-    assert int(ps.torso.n_col_list) == int((L["col_len"] > 0).sum())
+    assert int(ps.torso.n_chunks) == int(((L["col_len"] + 63) // 64).sum())
+
+
+def test_torso_chunk_table():
+    """hp_torso_chunks: the chunks cover every stored cell exactly once, in
+    stored order, and each record's neighbour base/span give the stored index
+    of the +x/-x/+y/-y cell at the same k (or nothing) as the column arrays do."""
+    from hrdayapy_amd import _native as nat
+    lab, smap = _torso(n=(30, 26, 150))   # runs longer than 64 cells
+    g = TorsoGrid.from_labels(lab, smap, 1.5)
+    L = g.layout
+    ni, nj, nk = g["shape"]
+    k0, ln, off = (np.asarray(L[k]) for k in ("col_k0", "col_len", "col_off"))
+    tab = nat.torso_chunks((ni, nj, nk), k0, ln, off)
+    assert tab.shape[1] == 16 and np.all(tab[:, 14:] == 0)
+    assert np.all((tab[:, 1] >= 1) & (tab[:, 1] <= 64)) and ln.max() > 64
+    cells = np.concatenate([r[4] + r[0] + np.arange(r[1]) for r in tab])
+    assert np.array_equal(cells, np.arange(int(L["n_cells"])))
+
+    def stored(i, j, k):
+        if not (0 <= i < ni and 0 <= j < nj):
+            return -1
+        c = i * nj + j
+        return int(off[c] + k - k0[c]) if ln[c] > 0 and k0[c] <= k < k0[c] + ln[c] else -1
+
+    rng = np.random.default_rng(3)
+    for r in tab[rng.choice(len(tab), 400, replace=False)]:
+        i, j = int(r[2]), int(r[3])
+        for k in range(r[0] - 1, r[0] + r[1] + 1):
+            for q, (di, dj) in enumerate(((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1))):
+                lo, hi = r[9 + q] & 0xFFFF, r[9 + q] >> 16
+                got = int(r[4 + q] + k) if lo <= k < hi else -1
+                assert got == stored(i + di, j + dj, k), (i, j, k, q)
+    import ctypes
+    a = [np.ascontiguousarray(v, dtype=np.int32) for v in (k0, ln, off)]
+    small, cnt = np.zeros((2, 16), np.int32), ctypes.c_int64(0)
+    with pytest.raises(RuntimeError, match="capacity"):
+        nat.call("hp_torso_chunks", ni, nj, nk, *(v.ctypes.data for v in a), small.ctypes.data, 1,
+                 ctypes.byref(cnt))
