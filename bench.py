@@ -178,7 +178,7 @@ def time_myo_kernel(eng, reps=50):
 
     def launch(i):
         nat.call(fn, C.byref(eng.bricks), nat.ptr(eng.V[(a + i) % 2]),
-                 nat.ptr(eng.V[(b + i) % 2]), nat.ptr(eng.state), eng.lay.n_slots,
+                 nat.ptr(eng.V[(b + i) % 2]), nat.ptr(eng.state), eng.state_ld,
                  C.byref(eng.p_myo), nat.stream_handle())
 
     return W.graph_us(launch, reps) * 1e-6  # s per launch

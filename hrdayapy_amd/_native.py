@@ -14,14 +14,14 @@ from pathlib import Path
 _LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
 _lib = None
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
 
 
 class HpBricks(C.Structure):
     _fields_ = [("n_bricks", c_i64), ("bx", c_i32), ("by", c_i32), ("bz", c_i32), ("pad_", c_i32),
-                ("nbr", c_vp), ("label", c_vp), ("D6", c_vp), ("inst", c_vp)]
+                ("nbr", c_vp), ("label", c_vp), ("D6", c_vp), ("inst", c_vp), ("groups", c_vp)]
 
 
 class HpMonoParams(C.Structure):
@@ -107,6 +107,12 @@ _SIGS = {
                                          c_i64, c_i64, c_f64, c_f64, c_f64]),
     "hp_myo_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
     "hp_myo_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), c_vp]),
+    "hp_myo_cable_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                         P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                         c_vp]),
+    "hp_myo_cable_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                         P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                         c_vp]),
     "hp_fallback_react_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                                         P(HpMonoParams), c_vp]),
     "hp_stencil19_apply_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),

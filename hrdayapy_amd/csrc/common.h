@@ -44,6 +44,32 @@ __device__ __forceinline__ uint32_t buf_ld_u8(buf_t r, uint32_t idx) {
 }
 
 // ---------------------------------------------------------------------------
+// Compact cell-state index of a brick-layout slot.  groups[g] = {myocardial
+// slots before slot 64*g, 0, myocardial mask bits 0-31, bits 32-63}; the
+// state entry of slot s is base + popcount(mask below s & 63), or the
+// out-of-range index 0x1FFFFFFF for a slot without myocardium.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t state_index(const int32_t* __restrict__ groups, uint32_t s) {
+  const int4 r = reinterpret_cast<const int4*>(groups)[s >> 6];
+  const uint64_t m = (uint64_t)(uint32_t)r.z | ((uint64_t)(uint32_t)r.w << 32);
+  const uint32_t b = s & 63u;
+  const uint64_t below = m & ((1ull << b) - 1ull);
+  return ((m >> b) & 1ull) ? (uint32_t)r.x + (uint32_t)__popcll(below) : 0x1FFFFFFFu;
+}
+
+// The same for a wave whose 64 lanes hold the 64 slots of group gidx in lane
+// order (gidx wave-uniform: one scalar load, a bit test and mbcnt).
+__device__ __forceinline__ uint32_t state_index_wave(const int32_t* __restrict__ groups,
+                                                     uint32_t gidx) {
+  const int4 r = reinterpret_cast<const int4*>(groups)[gidx];
+  const uint32_t lo = (uint32_t)r.z, hi = (uint32_t)r.w;
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+  const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+  const bool in = ((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1u) != 0;
+  return in ? (uint32_t)r.x + below : 0x1FFFFFFFu;
+}
+
+// ---------------------------------------------------------------------------
 // Arithmetic traits.  The FP32 path uses the hardware transcendental and
 // reciprocal instructions (v_exp_f32 / v_log_f32 / v_rcp_f32 / v_sqrt_f32,
 // ~1 ulp); the FP64 path uses the correctly rounded library routines so that

@@ -201,14 +201,15 @@ struct MlpSmem {
   int in_log[32];
 };
 
-// Raw input feature f (0..21) of a slot: V, the 18 state variables (absolute),
-// phenotype one-hot.
+// Raw input feature f (0..21) of a slot: V, the 18 state variables (absolute;
+// entry si of the compact state), phenotype one-hot.
 __device__ __forceinline__ float raw_feature(int f, const float* __restrict__ V,
                                              const float* __restrict__ st, int64_t ld,
-                                             int64_t slot, int ct, float ki_base, float nai_base) {
+                                             int64_t slot, int64_t si, int ct, float ki_base,
+                                             float nai_base) {
   if (f == 0) return V[slot];
   if (f <= 18) {
-    const float x = st[(int64_t)(f - 1) * ld + slot];
+    const float x = st[(int64_t)(f - 1) * ld + si];
     return f == 1 ? x + ki_base : (f == 2 ? x + nai_base : x);
   }
   return (f - 19) == ct ? 1.f : 0.f;
@@ -307,7 +308,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
   for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile * TV < n_items;
        tile += (int64_t)gridDim.x * WAVES) {
     const int64_t base = tile * TV;
-    int64_t slot[NC];
+    int64_t slot[NC], si[NC];
     uint8_t lab[NC];
     bool any = false;
 #pragma unroll
@@ -316,6 +317,8 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
       slot[t] = sp.slot_list ? (int64_t)sp.slot_list[item] : item;
       lab[t] = slot[t] >= 0 ? g.label[slot[t]] : (uint8_t)0xFF;
       if (slot[t] < 0) slot[t] = 0;  // no voxel: a valid address, never written (label 0xFF)
+      // compact state entry (read and written only where lab != 0xFF)
+      si[t] = lab[t] != 0xFF ? (int64_t)state_index(g.groups, (uint32_t)slot[t]) : 0;
       any |= lab[t] != 0xFF;
     }
     // skip tiles without myocardium (wave-uniform)
@@ -329,7 +332,8 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
         const int f = grp * 8 + q;
         float x = 0.f;
         if (lab[t] != 0xFF && f < NIN) {
-          x = raw_feature(f, V, state, ld, slot[t], lab[t] & 3, sp.ki_base, sp.nai_base);
+          x = raw_feature(f, V, state, ld, slot[t], si[t], lab[t] & 3, sp.ki_base,
+                          sp.nai_base);
           if (S.in_log[f]) x = __logf(fmaxf(x, 1e-30f));
           x = (x - S.in_mean[f]) * S.in_rstd[f];
         }
@@ -404,7 +408,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
           if (o == 0) {
             rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
           } else if (!fallback) {
-            float* p = state + (int64_t)(o - 1) * ld + sl;
+            float* p = state + (int64_t)(o - 1) * ld + si[t];
             const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
             const float x0 = *p + base_v;
             const float x1 = S.in_log[o] ? __expf(__logf(fmaxf(x0, 1e-30f)) + d) : x0 + d;
@@ -444,7 +448,7 @@ __global__ void __launch_bounds__(256) bigstep_kernel(hp_bricks g, const float* 
     return;
   }
   const int ct = lab & 3;
-  const uint32_t row_bytes = (uint32_t)ld * 4u, sl = (uint32_t)slot;
+  const uint32_t row_bytes = (uint32_t)ld * 4u, sl = state_index(g.groups, (uint32_t)slot);
   float s[N_STATE];
 #pragma unroll
   for (int k = 0; k < N_STATE; ++k) s[k] = buf_ld<float>(buf_rsrc(state + k * (size_t)ld, row_bytes), sl);
@@ -531,8 +535,8 @@ int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, f
 int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* state, int64_t ld,
                       float* rate, const hp_mlp_weights* w, const hp_mlp_norm* n,
                       const hp_surrogate_params* sp, int n_blocks, void* stream) {
-  if (!g || !w || !n || !sp || g->n_bricks <= 0) {
-    set_error("hp_surrogate_step: bad arguments");
+  if (!g || !g->groups || !w || !n || !sp || g->n_bricks <= 0 || ld < 0) {
+    set_error("hp_surrogate_step: bad arguments (groups required)");
     return -1;
   }
   const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
@@ -566,8 +570,8 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
 int hp_bigstep_reaction(const hp_bricks* g, const float* V, float* V_prev, float* state,
                         int64_t ld, float* rate, const hp_surrogate_params* sp, void* stream) {
   const int64_t n_slots = g ? g->n_bricks * (int64_t)(g->bx * g->by * g->bz) : 0;
-  if (!g || !sp || g->n_bricks <= 0 || ld < n_slots || (uint64_t)ld * 4 >= (1ull << 32)) {
-    set_error("hp_bigstep_reaction: bad arguments (ld must cover the slots, 32-bit offsets)");
+  if (!g || !g->groups || !sp || g->n_bricks <= 0 || ld < 0 || (uint64_t)ld * 4 >= (1ull << 32)) {
+    set_error("hp_bigstep_reaction: bad arguments (groups required, ld*4 < 2^32)");
     return -1;
   }
   hipLaunchKernelGGL(bigstep_kernel, dim3(ceil_div(n_slots, 256)), dim3(256), 0,

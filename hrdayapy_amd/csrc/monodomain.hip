@@ -208,9 +208,9 @@ struct HaloRegs {
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const HaloNbr<BX, BY, BZ>& n,
                                           const hp_bricks& g, const T* __restrict__ V,
-                                          uint32_t ld, uint8_t mylab) {
+                                          uint8_t mylab) {
   constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT, BVOL = BX * BY * BZ;
-  const uint32_t n_slots = (uint32_t)g.n_bricks * BVOL;
+  const uint32_t n_slots = (uint32_t)g.n_bricks * BVOL, ld = n_slots;  // D6 row stride
   const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
   const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
   const buf_t rL = buf_rsrc(g.label, n_slots);
@@ -271,12 +271,11 @@ __device__ __forceinline__ void halo_store(Tile<T, BX, BY, BZ>& t,
 // Whole staging of brick b for the one-brick-per-workgroup kernels.
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ void stage_cells(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
-                                            const T* __restrict__ V, uint32_t b, uint32_t ld,
-                                            uint8_t mylab) {
+                                            const T* __restrict__ V, uint32_t b, uint8_t mylab) {
   HaloNbr<BX, BY, BZ> n;
   halo_nbr<BX, BY, BZ>(n, g, b);
   HaloRegs<T, BX, BY, BZ> h;
-  halo_load<T, BX, BY, BZ>(h, n, g, V, ld, mylab);
+  halo_load<T, BX, BY, BZ>(h, n, g, V, mylab);
   halo_store<T, BX, BY, BZ>(t, h);
   __syncthreads();
 }
@@ -331,13 +330,11 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
 // FP32: at least six waves per SIMD (at most 80 VGPRs), so that the state and
 // halo loads of several bricks are in flight together.
 template <typename T, int BX, int BY, int BZ>
-__global__ void __launch_bounds__(BX * BY * BZ)
-    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) myo_step_kernel(
-    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
-    int64_t ld64, hp_mono_params p) {
+__device__ __forceinline__ void myo_brick(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
+                                          const T* __restrict__ V_in, T* __restrict__ V_out,
+                                          T* __restrict__ state, int64_t ld64,
+                                          const hp_mono_params& p, uint32_t b) {
   typedef Tile<T, BX, BY, BZ> TT;
-  __shared__ TT t;
-  const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t ld = (uint32_t)ld64;
   const int tid = threadIdx.x;
   const uint32_t slot = b * TT::BVOL + tid;
@@ -345,15 +342,18 @@ __global__ void __launch_bounds__(BX * BY * BZ)
   // before the halo staging loads, so that both are in flight together
   const bool pre = p.do_reaction == 1;
   const uint8_t mylab = g.label[slot];
+  // compact state entry of this lane (padding lanes: out of range, so their
+  // loads read 0 without memory traffic and their stores are dropped)
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t sidx = state_index_wave(g.groups, (b * TT::BVOL >> 6) + wave);
   T s[N_STATE];
   const uint32_t row_bytes = ld * (uint32_t)sizeof(T);
-  if (pre) {  // padding lanes read out of range: 0, no memory traffic
-    const uint32_t sidx = mylab != 0xFF ? slot : 0x1FFFFFFFu;
+  if (pre) {
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k)
       s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
   }
-  stage_cells<T, BX, BY, BZ>(t, g, V_in, b, ld, mylab);
+  stage_cells<T, BX, BY, BZ>(t, g, V_in, b, mylab);
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
   const uint8_t lab = t.L[c];
@@ -379,12 +379,12 @@ __global__ void __launch_bounds__(BX * BY * BZ)
     if (!pre) {
 #pragma unroll
       for (int k = 0; k < N_STATE; ++k)
-        s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), slot);
+        s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
     }
     constexpr bool exact = sizeof(T) == 8;
     dvdt = -ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), slot, s[k]);
+    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx, s[k]);
   }
   int grp = (lab >> 2) & 31;
   if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
@@ -406,6 +406,15 @@ __global__ void __launch_bounds__(BX * BY * BZ)
   }
 }
 
+template <typename T, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) myo_step_kernel(
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
+    int64_t ld64, hp_mono_params p) {
+  __shared__ Tile<T, BX, BY, BZ> t;
+  myo_brick<T, BX, BY, BZ>(t, g, V_in, V_out, state, ld64, p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // Multirate substep (do_reaction = 0): diffusion plus a given reaction rate
 // (dV/dt = rate_ext, or 0 without it), the same update as myo_step_kernel
 // without the cell model.  Persistent over bricks: the halo loads of the next
@@ -417,13 +426,12 @@ __global__ void __launch_bounds__(BX * BY * BZ)
 // round robin.
 template <typename T, int BX, int BY, int BZ, bool kInst>
 __global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
-    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, int64_t ld64,
-    hp_mono_params p) {
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, hp_mono_params p) {
   typedef Tile<T, BX, BY, BZ> TT;
   typedef HaloNbr<BX, BY, BZ> HN;
   typedef HaloRegs<T, BX, BY, BZ> HR;
   __shared__ TT t;
-  const uint32_t ld = (uint32_t)ld64, NB = (uint32_t)g.n_bricks;
+  const uint32_t NB = (uint32_t)g.n_bricks;
   const uint32_t xcd = blockIdx.x & 7, G = gridDim.x;
   const uint32_t m = (G - xcd + 7) >> 3;  // blocks with index = xcd (mod 8)
   const uint32_t base = NB >> 3, rem = NB & 7;
@@ -448,7 +456,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
   HN n;
   halo_nbr<BX, BY, BZ>(n, g, b);
   HR h;
-  halo_load<T, BX, BY, BZ>(h, n, g, V_in, ld, mylab);
+  halo_load<T, BX, BY, BZ>(h, n, g, V_in, mylab);
   T r = buf_ld<T>(rR, mylab != 0xFF ? b * TT::BVOL + tid : OOB);
   uint32_t bn = b + m;
   uint8_t labn = (uint8_t)buf_ld_u8(rLab, bn < hi ? bn * TT::BVOL + tid : OOB);
@@ -464,7 +472,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
     // next brick: halo and rate; the brick after: label and neighbour rows
     const bool more = bn < hi;
     HR hn;
-    halo_load<T, BX, BY, BZ>(hn, nn, g, V_in, ld, labn);
+    halo_load<T, BX, BY, BZ>(hn, nn, g, V_in, labn);
     const T rn = buf_ld<T>(rR, more && labn != 0xFF ? bn * TT::BVOL + tid : OOB);
     const uint32_t bnn = bn + m;
     const bool more2 = more && bnn < hi;
@@ -502,14 +510,14 @@ __global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
 template <typename T, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(BX * BY * BZ) stencil19_kernel(hp_bricks g,
                                                                  const T* __restrict__ V,
-                                                                 T* __restrict__ out, int64_t ld,
+                                                                 T* __restrict__ out,
                                                                  double dscale) {
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
   const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t slot = b * TT::BVOL + threadIdx.x;
   const uint8_t mylab = g.label[slot];
-  stage_cells<T, BX, BY, BZ>(t, g, V, b, (uint32_t)ld, mylab);
+  stage_cells<T, BX, BY, BZ>(t, g, V, b, mylab);
   const int tid = threadIdx.x;
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
   const int c = ((lx + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
@@ -564,8 +572,8 @@ static int compute_units() {
 }
 
 template <typename T>
-static int launch_diff(const hp_bricks* g, const T* V_in, T* V_out, int64_t ld,
-                       const hp_mono_params* p, hipStream_t st) {
+static int launch_diff(const hp_bricks* g, const T* V_in, T* V_out, const hp_mono_params* p,
+                       hipStream_t st) {
   if (g->n_bricks == 0) return 0;
   const int64_t want = (int64_t)compute_units() * 16;
   const dim3 grid((unsigned)(g->n_bricks < want ? g->n_bricks : want));
@@ -574,10 +582,10 @@ static int launch_diff(const hp_bricks* g, const T* V_in, T* V_out, int64_t ld,
 #define HP_DIFF(BX, BY, BZ)                                                                  \
   if (g->inst)                                                                               \
     hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, true>), grid, dim3(BX * BY * BZ), 0, \
-                       st, *g, V_in, V_out, ld, *p);                                         \
+                       st, *g, V_in, V_out, *p);                                             \
   else                                                                                       \
     hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, false>), grid, dim3(BX * BY * BZ),   \
-                       0, st, *g, V_in, V_out, ld, *p)
+                       0, st, *g, V_in, V_out, *p)
     case 444: HP_DIFF(4, 4, 4); break;
     case 844: HP_DIFF(8, 4, 4); break;
     case 884: HP_DIFF(8, 8, 4); break;
@@ -598,12 +606,9 @@ struct StencilK {
 // Purkinje cable
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __restrict__ V_in,
-                                                         T* __restrict__ V_out,
-                                                         T* __restrict__ state, int64_t ld,
-                                                         hp_mono_params p) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.n) return;
+__device__ __forceinline__ void cable_node(const hp_cable& c, const T* __restrict__ V_in,
+                                           T* __restrict__ V_out, T* __restrict__ state,
+                                           int64_t ld, const hp_mono_params& p, int64_t i) {
   const T Vc = V_in[i];
   T acc = T(0);
   for (int e = c.indptr[i]; e < c.indptr[i + 1]; ++e) acc += V_in[c.indices[e]] - Vc;
@@ -629,6 +634,38 @@ __global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
   V_out[i] = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) cable_step_kernel(hp_cable c, const T* __restrict__ V_in,
+                                                         T* __restrict__ V_out,
+                                                         T* __restrict__ state, int64_t ld,
+                                                         hp_mono_params p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < c.n) cable_node<T>(c, V_in, V_out, state, ld, p, i);
+}
+
+// The coupled step in one launch: the first ncb workgroups (a multiple of 8,
+// so the XCD of every brick workgroup is unchanged) update the cable nodes,
+// the others the bricks.  Within a step the two read V^n and write V^{n+1} of
+// their own domains only; the PMJ exchange that couples them follows in the
+// tail kernel.  Running the cable here removes a kernel and two of the three
+// kernel boundaries of a step.
+template <typename T, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(BX * BY * BZ)
+    __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) myo_cable_kernel(
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, T* __restrict__ state,
+    int64_t ld64, hp_mono_params p, hp_cable c, const T* __restrict__ Vpk_in,
+    T* __restrict__ Vpk_out, T* __restrict__ state_pk, int64_t ld_pk, hp_mono_params p_pk,
+    uint32_t ncb) {
+  __shared__ Tile<T, BX, BY, BZ> t;
+  if (blockIdx.x < ncb) {
+    const int64_t i = (int64_t)blockIdx.x * (BX * BY * BZ) + threadIdx.x;
+    if (i < c.n) cable_node<T>(c, Vpk_in, Vpk_out, state_pk, ld_pk, p_pk, i);
+    return;
+  }
+  myo_brick<T, BX, BY, BZ>(t, g, V_in, V_out, state, ld64, p,
+                           xcd_remap(blockIdx.x - ncb, gridDim.x - ncb));
 }
 
 // ---------------------------------------------------------------------------
@@ -817,14 +854,75 @@ static int myo_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64
     set_error("hp_myo_step: do_reaction must be 0, 1 or 2 (2 needs rate_ext)");
     return -1;
   }
-  if (!g || ld < g->n_bricks * g->bx * g->by * g->bz ||
-      (uint64_t)ld * 6 * sizeof(T) >= (1ull << 32)) {
-    set_error("hp_myo_step: ld must cover every slot and 6*ld elements fit 32-bit offsets");
+  if (!g) {
+    set_error("hp_myo_step: brick grid required");
     return -1;
   }
-  if (p->do_reaction == 0) return launch_diff<T>(g, V_in, V_out, ld, p, (hipStream_t)stream);
+  if (p->do_reaction != 0 &&
+      (!g->groups || !state || ld < 0 || (uint64_t)ld * sizeof(T) >= (1ull << 32))) {
+    set_error("hp_myo_step: the reaction needs groups and state (ld*sizeof(T) < 2^32)");
+    return -1;
+  }
+  if (p->do_reaction == 0) {
+    // launch_bricks checks the grid for the other kernels
+    if (g->n_bricks < 0 || g->n_bricks > 0x7fffffff ||
+        (uint64_t)g->n_bricks * g->bx * g->by * g->bz * 6 * sizeof(T) >= (1ull << 32)) {
+      set_error("hp_myo_step: %lld bricks exceed the 32-bit offset range of one launch",
+                (long long)g->n_bricks);
+      return -1;
+    }
+    return launch_diff<T>(g, V_in, V_out, p, (hipStream_t)stream);
+  }
   return launch_bricks<T, MyoK>(g, "hp_myo_step", (hipStream_t)stream, V_in, V_out, state, ld,
                                 *p);
+}
+
+template <typename T>
+static int myo_cable_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_t ld,
+                          const hp_mono_params* p, const hp_cable* c, const T* Vpk_in,
+                          T* Vpk_out, T* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
+                          void* stream) {
+  if (!g || !p || !p->step_ctr || !p->group_amp || (p->do_reaction != 1 && p->do_reaction != 2) ||
+      (p->do_reaction == 2 && !p->rate_ext) || !g->groups || !state || ld < 0 ||
+      (uint64_t)ld * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_myo_cable_step: myocardium arguments as hp_myo_step, do_reaction 1 or 2");
+    return -1;
+  }
+  if (!c || !p_pk || c->n < 0 || !p_pk->group_amp || ld_pk < c->n ||
+      (uint64_t)ld_pk * sizeof(T) >= (1ull << 32) || (c->n > 0 && (!Vpk_in || !Vpk_out || !state_pk))) {
+    set_error("hp_myo_cable_step: cable arguments as hp_cable_step");
+    return -1;
+  }
+  const int64_t bvol = (int64_t)g->bx * g->by * g->bz;
+  if (g->n_bricks < 0 || (uint64_t)g->n_bricks * bvol * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_myo_cable_step: %lld bricks exceed the 32-bit offset range of one launch",
+              (long long)g->n_bricks);
+    return -1;
+  }
+  const int64_t ncb = (((c->n + bvol - 1) / bvol) + 7) / 8 * 8;
+  if (g->n_bricks + ncb == 0) return 0;
+  if (g->n_bricks + ncb > 0x7fffffff) {
+    set_error("hp_myo_cable_step: grid too large");
+    return -1;
+  }
+  const dim3 grid((unsigned)(g->n_bricks + ncb));
+  hipStream_t st = (hipStream_t)stream;
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  switch (key) {
+#define HP_MC(BX, BY, BZ)                                                                      \
+  hipLaunchKernelGGL((myo_cable_kernel<T, BX, BY, BZ>), grid, dim3(BX * BY * BZ), 0, st, *g, \
+                     V_in, V_out, state, ld, *p, *c, Vpk_in, Vpk_out, state_pk, ld_pk, *p_pk, \
+                     (uint32_t)ncb)
+    case 444: HP_MC(4, 4, 4); break;
+    case 844: HP_MC(8, 4, 4); break;
+    case 884: HP_MC(8, 8, 4); break;
+    case 888: HP_MC(8, 8, 8); break;
+#undef HP_MC
+    default:
+      set_error("hp_myo_cable_step: unsupported brick shape %dx%dx%d", g->bx, g->by, g->bz);
+      return -1;
+  }
+  return check_launch("hp_myo_cable_step");
 }
 
 // ---------------------------------------------------------------------------
@@ -840,6 +938,7 @@ __global__ void __launch_bounds__(256) fallback_react_kernel(
   const uint32_t row_bytes = (uint32_t)ld * 4u;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t slot = (uint32_t)list[i];
+    const uint32_t si = state_index(g.groups, slot);
     const int ct = g.label[slot] & 3;
     float gks = (float)kGKs[ct], gto = (float)kGto[ct];
     if (p.inst_params && g.inst) {
@@ -850,11 +949,11 @@ __global__ void __launch_bounds__(256) fallback_react_kernel(
     float s[N_STATE];
 #pragma unroll
     for (int k = 0; k < N_STATE; ++k)
-      s[k] = buf_ld<float>(buf_rsrc(state + k * (size_t)ld, row_bytes), slot);
+      s[k] = buf_ld<float>(buf_rsrc(state + k * (size_t)ld, row_bytes), si);
     const float I = ttp06_cell<float, false>(ld32(V, slot), s, (float)p.dt, gks, gto, ct != 2,
                                              (float)p.ki_base, (float)p.nai_base);
 #pragma unroll
-    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), slot, s[k]);
+    for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), si, s[k]);
     st32(rate, slot, -I);
   }
 }
@@ -954,12 +1053,26 @@ int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, doubl
                     int64_t ld, const hp_mono_params* p, void* stream) {
   return myo_step<double>(g, V_in, V_out, state, ld, p, stream);
 }
+int hp_myo_cable_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state,
+                          int64_t ld, const hp_mono_params* p, const hp_cable* c,
+                          const float* Vpk_in, float* Vpk_out, float* state_pk, int64_t ld_pk,
+                          const hp_mono_params* p_pk, void* stream) {
+  return myo_cable_step<float>(g, V_in, V_out, state, ld, p, c, Vpk_in, Vpk_out, state_pk, ld_pk,
+                               p_pk, stream);
+}
+int hp_myo_cable_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                          int64_t ld, const hp_mono_params* p, const hp_cable* c,
+                          const double* Vpk_in, double* Vpk_out, double* state_pk, int64_t ld_pk,
+                          const hp_mono_params* p_pk, void* stream) {
+  return myo_cable_step<double>(g, V_in, V_out, state, ld, p, c, Vpk_in, Vpk_out, state_pk,
+                                ld_pk, p_pk, stream);
+}
 int hp_fallback_react_f32(const hp_bricks* g, const float* V, float* state, int64_t ld,
                           float* rate, const int32_t* list, const int32_t* count,
                           int64_t max_count, const hp_mono_params* p, void* stream) {
-  if (!g || !p || !list || !count || !rate || max_count < 0 ||
-      ld < g->n_bricks * g->bx * g->by * g->bz || (uint64_t)ld * 4 >= (1ull << 32)) {
-    set_error("hp_fallback_react: bad arguments");
+  if (!g || !g->groups || !p || !list || !count || !rate || max_count < 0 || ld < 0 ||
+      (uint64_t)ld * 4 >= (1ull << 32)) {
+    set_error("hp_fallback_react: bad arguments (groups required, ld*4 < 2^32)");
     return -1;
   }
   if (max_count == 0) return 0;
@@ -970,15 +1083,13 @@ int hp_fallback_react_f32(const hp_bricks* g, const float* V, float* state, int6
 }
 int hp_stencil19_apply_f32(const hp_bricks* g, const float* V, float* out, double dscale,
                            void* stream) {
-  const int64_t ld = g ? g->n_bricks * g->bx * g->by * g->bz : 0;
-  return launch_bricks<float, StencilK>(g, "hp_stencil19_apply", (hipStream_t)stream, V, out, ld,
+  return launch_bricks<float, StencilK>(g, "hp_stencil19_apply", (hipStream_t)stream, V, out,
                                         dscale);
 }
 int hp_stencil19_apply_f64(const hp_bricks* g, const double* V, double* out, double dscale,
                            void* stream) {
-  const int64_t ld = g ? g->n_bricks * g->bx * g->by * g->bz : 0;
   return launch_bricks<double, StencilK>(g, "hp_stencil19_apply", (hipStream_t)stream, V, out,
-                                         ld, dscale);
+                                         dscale);
 }
 int hp_cable_step_f32(const hp_cable* c, const float* V_in, float* V_out, float* state,
                       int64_t ld, const hp_mono_params* p, void* stream) {

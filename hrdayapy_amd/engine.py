@@ -124,10 +124,14 @@ class MonodomainEngine:
         if pmj and self.Np and pmj["c_pmj"] != 0.0 and len(pmj["ids"]):
             tgt_nodes = np.unique(np.concatenate([np.asarray(p, np.int64) for p in pmj["patches"]]))
             lab[L.slot_of_node[tgt_nodes]] |= np.uint8(0x80)
-        # the resident per-slot arrays share one allocation (2 MiB-aligned views)
+        # the resident arrays share one allocation (2 MiB-aligned views); the
+        # cell state holds the myocardial slots only (no brick padding)
         n = L.n_slots
-        specs = [("state", (18, n), dt_), ("D6", (6, n), dt_), ("V0", n, dt_), ("V1", n, dt_),
-                 ("at", n, dt_), ("label", n, torch.uint8)]
+        groups, state_of_node = L.state_groups()
+        self.state_ld = -(-L.n_nodes // 64) * 64
+        specs = [("state", (18, self.state_ld), dt_), ("D6", (6, n), dt_), ("V0", n, dt_),
+                 ("V1", n, dt_), ("at", n, dt_), ("label", n, torch.uint8),
+                 ("groups", (len(groups), 4), torch.int32)]
         if inst is not None:
             specs.append(("inst", n, torch.int16))
         self.arena = nat.DeviceArena(specs, dev, contiguous=(alloc == "contiguous"))
@@ -138,6 +142,9 @@ class MonodomainEngine:
         self.D6.copy_(torch.from_numpy(L.to_slots(np.asarray(D6, dtype=np.float64))).to(dt_))
         self.nbr = torch.from_numpy(L.nbr).to(dev)
         self.slot_of_node = torch.from_numpy(L.slot_of_node).to(dev)
+        self.groups = ar["groups"]
+        self.groups.copy_(torch.from_numpy(groups))
+        self.state_of_node = torch.from_numpy(state_of_node).to(dev)
         self.V = [ar["V0"], ar["V1"]]
         for v in self.V:
             v.fill_(V0)
@@ -152,7 +159,7 @@ class MonodomainEngine:
         self.inst_params = None if inst_params is None else torch.as_tensor(
             np.asarray(inst_params, dtype=np.float64).reshape(-1, 4)).to(dev, dt_).contiguous()
         self.bricks = nat.HpBricks(L.n_bricks, *L.brick, 0, nat.ptr(self.nbr), nat.ptr(self.label),
-                                   nat.ptr(self.D6), nat.ptr(self.inst))
+                                   nat.ptr(self.D6), nat.ptr(self.inst), nat.ptr(self.groups))
         self.p_myo = self._params(1.0 / (A_M * h * h), at_thr, self.at)
 
         # Purkinje cable
@@ -211,6 +218,7 @@ class MonodomainEngine:
                            for k in range(2)]
         self.p = 0
         self.overlap_cable = True
+        self.fuse_cable = True
         self._side = torch.cuda.Stream(device=dev) if self.Np else None
         self._eval_stim()
 
@@ -240,23 +248,35 @@ class MonodomainEngine:
 
     # ------------------------------------------------------------------- step
     def _launch_step(self):
-        """One time step: the myocardium kernel and (on a forked stream) the
-        cable kernel run concurrently; one tail kernel then does the PMJ
-        exchange and advances the step counter / stimulus amplitudes."""
+        """One time step: the myocardium and the cable in one launch
+        (hp_myo_cable_step; with the diffusion-only form, or fuse_cable off,
+        the cable kernel on a forked stream or after the myocardium kernel);
+        one tail kernel then does the PMJ exchange and advances the step
+        counter / stimulus amplitudes."""
         cur = torch.cuda.current_stream(self.dev)
         s = cur.cuda_stream
         a, b = self.p, 1 - self.p
+        if self.cable is not None and self.fuse_cable and self.p_myo.do_reaction != 0:
+            nat.call(f"hp_myo_cable_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
+                     nat.ptr(self.V[b]), nat.ptr(self.state), self.state_ld, C.byref(self.p_myo),
+                     C.byref(self.cable), nat.ptr(self.Vpk[a]), nat.ptr(self.Vpk[b]),
+                     nat.ptr(self.state_pk), self.Np, C.byref(self.p_pk), s)
+            self._launch_tail(a, b, s)
+            return
         fork = self.cable is not None and self.overlap_cable
         if fork:
             self._side.wait_stream(cur)
             self._launch_cable(a, b, self._side.cuda_stream)
         nat.call(f"hp_myo_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
-                 nat.ptr(self.V[b]), nat.ptr(self.state), self.lay.n_slots, C.byref(self.p_myo), s)
+                 nat.ptr(self.V[b]), nat.ptr(self.state), self.state_ld, C.byref(self.p_myo), s)
         if self.cable is not None:
             if fork:
                 cur.wait_stream(self._side)
             else:
                 self._launch_cable(a, b, s)
+        self._launch_tail(a, b, s)
+
+    def _launch_tail(self, a, b, s):
         nat.call(f"hp_step_tail_{self.sfx}", None if self.pmj is None else C.byref(self.pmj),
                  nat.ptr(self.Vpk[b]) if self.Np else None, nat.ptr(self.V[b]),
                  nat.ptr(self.pmj_scratch) if self.pmj is not None else None, self.dt, self.vmin,
@@ -330,7 +350,7 @@ class MonodomainEngine:
 
     def state_nodes(self):
         """(N_global, 18) state in the reference's column order, absolute Ki/Nai."""
-        sm = self.state[:, self.slot_of_node.long()]
+        sm = self.state[:, self.state_of_node.long()]
         if self.Np:
             sm = torch.cat([self.state_pk, sm], dim=1)
         sm = sm.t().contiguous()

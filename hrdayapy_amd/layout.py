@@ -137,6 +137,28 @@ class BrickLayout:
             out[:, self.slot_of_node] = values
         return out
 
+    def state_groups(self):
+        """Compact cell-state index of the slots: (groups, state_of_node).
+        groups[g] (int32 x4, one row per 64 consecutive slots) = (myocardial
+        slots before slot 64*g, 0, myocardial mask bits 0-31, bits 32-63);
+        state_of_node[i] = the state entry of node i = the rank of its slot
+        among the myocardial slots."""
+        if self.n_slots % 64:
+            raise ValueError("the slot count must be a multiple of 64")
+        tissue = np.zeros(self.n_slots, dtype=bool)
+        tissue[self.slot_of_node] = True
+        t = tissue.reshape(-1, 64)
+        cnt = t.sum(axis=1)
+        groups = np.zeros((len(t), 4), dtype=np.int64)
+        groups[:, 0] = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+        w = (np.uint64(1) << np.arange(64, dtype=np.uint64))
+        mask = (t.astype(np.uint64) * w).sum(axis=1, dtype=np.uint64)
+        groups[:, 2] = (mask & np.uint64(0xFFFFFFFF)).astype(np.int64)
+        groups[:, 3] = (mask >> np.uint64(32)).astype(np.int64)
+        groups = groups.astype(np.uint32).view(np.int32).reshape(-1, 4)
+        rank = np.cumsum(tissue) - 1
+        return groups, rank[self.slot_of_node].astype(np.int32)
+
     def labels(self, celltype: np.ndarray, group: np.ndarray | None = None) -> np.ndarray:
         """Label byte per slot: phenotype in bits 0-1, stimulus group in bits 2-6."""
         lab = celltype.astype(np.uint8) & 3

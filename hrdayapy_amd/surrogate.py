@@ -191,7 +191,7 @@ def sample_from_engine(eng, n_macro, per_snapshot, seed=0, only_surrogate=True):
             if cand.numel():
                 pick = cand[torch.randint(cand.numel(), (min(per_snapshot, cand.numel()),),
                                           generator=g, device=eng.dev)]
-                st = eng.state[:, sl[pick]].t().float()
+                st = eng.state[:, eng.state_of_node[pick].long()].t().float()
                 st[:, 0] += eng.ki_base
                 st[:, 1] += eng.nai_base
                 Vs.append(V[pick])
@@ -383,12 +383,12 @@ class SurrogateRunner:
             self.fb_count.zero_()  # a fill kernel (safe inside a captured graph)
         if self.mode == "mlp":
             nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
-                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
+                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.state_ld,
                      nat.ptr(self.rate), C.byref(self.net.hp_w), C.byref(self.net.hp_norm),
                      C.byref(self.sp), self.n_blocks, nat.stream_handle())
         else:
             nat.call("hp_bigstep_reaction", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
-                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.lay.n_slots,
+                     nat.ptr(self.V_prev), nat.ptr(eng.state), eng.state_ld,
                      nat.ptr(self.rate), C.byref(self.sp), nat.stream_handle())
 
     def launch_substep(self):
@@ -396,7 +396,7 @@ class SurrogateRunner:
         eng = self.eng
         if self.compact:
             nat.call("hp_fallback_react_f32", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
-                     nat.ptr(eng.state), eng.lay.n_slots, nat.ptr(self.rate),
+                     nat.ptr(eng.state), eng.state_ld, nat.ptr(self.rate),
                      nat.ptr(self.fb_list), nat.ptr(self.fb_count), eng.lay.n_slots,
                      C.byref(eng.p_myo), nat.stream_handle())
         eng.p_myo.do_reaction = 0 if self.compact else 2

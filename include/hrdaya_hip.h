@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 2
+#define HP_ABI_VERSION 3
 
 /* ---------------------------------------------------------------------------
  * Library
@@ -94,11 +94,21 @@ int hp_ttp06_step_host_f64(const double* V, double* state, double* I_ion,
  *   label  uint8: 0xFF = not myocardium; else bits 0-1 phenotype code,
  *          bits 2-6 stimulus group (0 = none), bit 7 = PMJ target voxel
  *   D6     conductivity tensor D00,D11,D22,D01,D02,D12 [mS/mm], SoA [6][n_slots]
- *          (row stride ld = the slot count passed with the state)
+ *          (row stride n_slots = n_bricks*bvol)
  *   inst   uint16 ensemble instance (or NULL: instance 0)
  * Padding slots carry label 0xFF and D6 = 0.  Every per-slot array, and the
- * six D6 rows together, must fit 32-bit byte offsets (6*ld*sizeof(T) < 2^32);
- * the entry points reject larger grids.
+ * six D6 rows together, must fit 32-bit byte offsets (6*n_slots*sizeof(T) <
+ * 2^32); the entry points reject larger grids.
+ *
+ * Cell state is stored for the myocardial slots only (no padding): SoA
+ * [18][ld], entry q of a row = the q-th myocardial slot in slot order, so a
+ * wave's loads cover only cache lines that hold tissue.  groups[g] (4 int32
+ * per 64 consecutive slots, g = slot >> 6) = {number of myocardial slots
+ * before slot 64*g, 0, myocardial-slot mask bits 0-31, bits 32-63}: the
+ * state entry of slot s is groups[s>>6].base + popcount(mask below s&63).
+ * The mask must agree with label != 0xFF.  The kernels that touch the state
+ * (hp_myo_step, hp_fallback_react, hp_surrogate_step, hp_bigstep_reaction)
+ * require groups; ld >= the myocardial slot count and ld*sizeof(T) < 2^32.
  * nbr[b*27 + (dx+1)*9 + (dy+1)*3 + (dz+1)] is the brick at offset (dx,dy,dz),
  * or -1.
  *
@@ -115,6 +125,7 @@ typedef struct hp_bricks {
   const uint8_t* label;  /* [n_slots] */
   const void* D6;        /* T[6][n_slots] */
   const uint16_t* inst;  /* [n_slots] or NULL */
+  const int32_t* groups; /* [n_slots/64][4] state index of each 64-slot group (see above) */
 } hp_bricks;
 
 /* Per-step scalars of the monodomain update.
@@ -140,7 +151,7 @@ typedef struct hp_mono_params {
 
 /* One fused explicit step of the myocardium: TTP06 + 19-point diffusion +
  * stimulus + clamp + first-activation detection.  Reads V_in, writes V_out,
- * updates state (SoA, ld = n_slots) in place. */
+ * updates state (compact SoA [18][ld], see groups above) in place. */
 int hp_myo_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state,
                     int64_t ld, const hp_mono_params* p, void* stream);
 int hp_myo_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
@@ -183,6 +194,22 @@ int hp_cable_step_f32(const hp_cable* c, const float* V_in, float* V_out, float*
                       int64_t ld, const hp_mono_params* p, void* stream);
 int hp_cable_step_f64(const hp_cable* c, const double* V_in, double* V_out, double* state,
                       int64_t ld, const hp_mono_params* p, void* stream);
+
+/* hp_myo_step and hp_cable_step of one time step in a single launch (the
+ * cable nodes are extra workgroups of the fused myocardium kernel; the two
+ * domains share no data within a step, the PMJ exchange follows in
+ * hp_step_tail).  Same arguments and results as the two calls; do_reaction
+ * must be 1 or 2.  No counterpart in the reference (which assembles both
+ * domains into one sparse system, purkinje_myocardium_anisotropic_solver.py:
+ * 1787-1791): it saves one kernel boundary per step. */
+int hp_myo_cable_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state,
+                          int64_t ld, const hp_mono_params* p, const hp_cable* c,
+                          const float* Vpk_in, float* Vpk_out, float* state_pk, int64_t ld_pk,
+                          const hp_mono_params* p_pk, void* stream);
+int hp_myo_cable_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                          int64_t ld, const hp_mono_params* p, const hp_cable* c,
+                          const double* Vpk_in, double* Vpk_out, double* state_pk, int64_t ld_pk,
+                          const hp_mono_params* p_pk, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Purkinje-myocardium junctions (replaces the PMJ entries of L_global,

@@ -117,6 +117,31 @@ def test_graph_replay_equals_eager(cuda):
     assert a.step_index == b.step_index == 300
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_one_launch_cable_step_equals_separate_kernels(cuda, dtype):
+    """hp_myo_cable_step (cable nodes as extra workgroups of the fused
+    myocardium kernel) against hp_myo_step + hp_cable_step on a forked stream:
+    the same arithmetic, so bit-identical V, cable V, state and activation
+    times over 400 coupled steps with PMJs, eager and graph-replayed."""
+    print(f"[case] one-launch myocardium + cable step, {dtype}", flush=True)
+    args, kw = _case("tree")
+    nodes, el, act, S, phi, f = args
+    a, _ = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, device=cuda, dtype=dtype, **kw)
+    b, _ = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, device=cuda, dtype=dtype, **kw)
+    assert a.Np > 0 and a.pmj is not None
+    b.fuse_cable = False
+    for chunk in (0, 50):
+        a.run(200, graph_chunk=chunk)
+        b.run(200, graph_chunk=chunk)
+        torch.cuda.synchronize()
+        assert torch.equal(a.V_global(), b.V_global()), chunk
+        assert torch.equal(a.state, b.state) and torch.equal(a.state_pk, b.state_pk)
+        assert torch.equal(a.at.isnan(), b.at.isnan())
+        ok = ~a.at.isnan()
+        assert torch.equal(a.at[ok], b.at[ok])
+    assert int((~a.at.isnan()).sum()) > 0
+
+
 def test_stencil_apply_matches_oracle_laplacian(cuda):
     print("[case] stencil19 apply", flush=True)
     from hrdayapy_amd.layout import BrickLayout
@@ -174,7 +199,7 @@ def test_diffusion_substep_equals_fused_rate_form(cuda, brick):
             e.p_myo.rate_ext = rate.data_ptr()
             a, b = e.p, 1 - e.p
             nat.call("hp_myo_step_f32", ctypes.byref(e.bricks), nat.ptr(e.V[a]), nat.ptr(e.V[b]),
-                     nat.ptr(e.state), n, ctypes.byref(e.p_myo), nat.stream_handle())
+                     nat.ptr(e.state), e.state_ld, ctypes.byref(e.p_myo), nat.stream_handle())
             e.p = b
         torch.cuda.synchronize()
     dv = float((engs[0].V[engs[0].p] - engs[1].V[engs[1].p]).abs().max())

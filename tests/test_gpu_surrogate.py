@@ -115,11 +115,18 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     stc = st.clone()
     stc[:, 0] -= SG.KI_BASE
     stc[:, 1] -= SG.NAI_BASE
-    state = torch.from_numpy(lay.to_slots(stc.t().numpy())).to(cuda).contiguous()
+    # compact state: entry state_of_node[i] holds node i
+    groups, son = lay.state_groups()
+    son_t = torch.from_numpy(son.astype(np.int64))
+    state = torch.zeros((18, n + 64), dtype=torch.float32)
+    state[:, son_t] = stc.t()
+    state = state.to(cuda).contiguous()
     state0 = state.clone()
     lab = torch.from_numpy(lay.labels(ct.numpy().astype(np.uint8))).to(cuda)
     nbr = torch.from_numpy(lay.nbr).to(cuda)
-    br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None)
+    grp = torch.from_numpy(groups).to(cuda)
+    br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None,
+                      grp.data_ptr())
     rate = torch.zeros(lay.n_slots, device=cuda)
     sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, variant)
     if listed:  # the tissue slots in a shuffled order, padded with -1 to a multiple of 32
@@ -130,7 +137,7 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
         sp.slot_list, sp.n_list = lst_d.data_ptr(), len(lst)
     import ctypes
     nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
-             state.data_ptr(), lay.n_slots, rate.data_ptr(), ctypes.byref(net.hp_w),
+             state.data_ptr(), n + 64, rate.data_ptr(), ctypes.byref(net.hp_w),
              ctypes.byref(net.hp_norm), ctypes.byref(sp), 4, nat.stream_handle())
     sl = torch.from_numpy(lay.slot_of_node.astype(np.int64))
     r = rate.cpu()[sl]
@@ -145,8 +152,8 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     Dc = Yc * torch.from_numpy(net.out_std) + torch.from_numpy(net.out_mean)
     ok = ~fb
     np.testing.assert_allclose(r.numpy()[ok], (Dc[:, 0] / 0.2).numpy()[ok], rtol=5e-2, atol=2e-3)
-    new = state.cpu()[:, sl].t()
-    old = state0.cpu()[:, sl].t()
+    new = state.cpu()[:, son_t].t()
+    old = state0.cpu()[:, son_t].t()
     # untouched on fallback voxels
     assert torch.equal(new[torch.from_numpy(fb)], old[torch.from_numpy(fb)])
     # updated as f^-1(f(x) + d) elsewhere

@@ -87,6 +87,33 @@ def test_concat_layout_keeps_instances_apart():
         BrickLayout.concat([parts[0], BrickLayout(slab(8, 8, 8)[0], (8, 8, 8))])
 
 
+@pytest.mark.parametrize("brick", BRICK_SHAPES)
+def test_state_groups_index_the_compact_state(brick):
+    """state_groups: the compact state entry of each node is the rank of its
+    slot among the myocardial slots (a permutation of 0..N-1 in slot order),
+    and groups[s >> 6] gives it as base + popcount(mask below s & 63), with
+    the mask bit set exactly on myocardial slots; concatenations included."""
+    S, _, _ = slab(13, 11, 7)
+    S[4:7, 3:9, 2:5] = 0
+    for lay in (BrickLayout(S, brick),
+                BrickLayout.concat([BrickLayout(S, brick), BrickLayout(slab(9, 5, 6)[0], brick)])):
+        groups, son = lay.state_groups()
+        assert groups.shape == (lay.n_slots // 64, 4) and groups.dtype == np.int32
+        order = np.argsort(lay.slot_of_node)
+        assert np.array_equal(son[order], np.arange(lay.n_nodes))
+        g = groups.astype(np.int64) & 0xFFFFFFFF
+        mask = g[:, 2] | (g[:, 3] << 32)
+        tissue = np.zeros(lay.n_slots, bool)
+        tissue[lay.slot_of_node] = True
+        bits = (mask[:, None] >> np.arange(64)[None, :]) & 1
+        assert np.array_equal(bits.ravel().astype(bool), tissue)
+        s = lay.slot_of_node.astype(np.int64)
+        below = mask[s >> 6] & ((np.int64(1) << (s & 63)) - 1)
+        pc = np.array([bin(int(v) & (2**64 - 1)).count("1") for v in below])
+        assert np.array_equal(groups[s >> 6, 0] + pc, son)
+        assert np.all(groups[:, 1] == 0)
+
+
 def test_ensemble_stimulus_groups():
     """Per-instance groups: (inst << 3) | local site combination, with the group
     table naming each instance's sites by (base, mask)."""
