@@ -32,8 +32,11 @@ sys.path.insert(0, HERE)
 METRIC = "voxel-steps/s, 2M-voxel BiV: surrogate vs plain FP32 at AT err≤1ms, ECG r≥.999"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL_STEP = 177  # 19 FP32 state r/w + 6 FP32 tensor + 1 label byte
+# cable node: 18 FP32 state r/w (144), V read and written (8), label (1), CSR row
+# pointer (4) and two column indices (8, a tree's mean degree)
+BYTES_PER_CABLE_NODE = 165
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "myo_step_biv256_traffic.json")
+                            "myo_cable_biv256_traffic.json")
 
 
 def measured_traffic():
@@ -167,21 +170,18 @@ def stream_copy_bandwidth(mib, reps=20):
     return dict(mib=mib, us=us, GBs=2 * n * 4 / (us * 1e-6) / 1e9)
 
 
-def time_myo_kernel(eng, reps=50):
-    """Average duration of the fused myocardium kernel: reps launches (ping-pong
-    V buffers, advancing the state) replayed from a HIP graph, HIP events on
-    the replaying stream."""
+def time_step_kernels(eng, reps=50):
+    """Device time per launch (s) of the coupled step's dominant kernel as the
+    simulation runs it.  reps steps (hp_myo_cable_step -- hp_myo_step without a
+    cable -- then hp_step_tail) and reps tail kernels alone are each captured
+    into a HIP graph and replayed between HIP events on the replaying stream;
+    the dominant kernel's share is the step minus the tail.  Returns
+    (dominant_s, tail_s, step_s)."""
     from hrdayapy_amd import _native as nat
     from hrdayapy_amd import workloads as W
-    a, b = eng.p, 1 - eng.p
-    fn = f"hp_myo_step_{eng.sfx}"
-
-    def launch(i):
-        nat.call(fn, C.byref(eng.bricks), nat.ptr(eng.V[(a + i) % 2]),
-                 nat.ptr(eng.V[(b + i) % 2]), nat.ptr(eng.state), eng.state_ld,
-                 C.byref(eng.p_myo), nat.stream_handle())
-
-    return W.graph_us(launch, reps) * 1e-6  # s per launch
+    step_us = W.graph_us(lambda i: eng._launch_step(), reps)
+    tail_us = W.graph_us(lambda i: eng._launch_tail(eng.p, 1 - eng.p, nat.stream_handle()), reps)
+    return (step_us - tail_us) * 1e-6, tail_us * 1e-6, step_us * 1e-6
 
 
 def cpu_baseline(args_geo, kw, n_steps, dtype_name="float32"):
@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=14)
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
                     help="size of the streaming-copy calibration (0 = off)")
     ap.add_argument("--workload", type=str, default="biv", choices=["biv", "torso", "ensemble"],
@@ -315,10 +315,13 @@ def main():
           flush=True)
     at_plain = eng.at_nodes().double().cpu().numpy()
     activated = int(np.isfinite(at_plain).sum())
-    k_s = time_myo_kernel(eng, a.kernel_reps)
-    achieved = BYTES_PER_VOXEL_STEP * n_myo / k_s / 1e9
-    print(f"[bench] fused myo kernel {k_s * 1e6:.1f} us/launch -> {achieved:.0f} GB/s algorithmic; "
-          f"activated {activated}/{n_myo} at t={(a.warmup + a.steps) * 0.02:.1f} ms", flush=True)
+    k_s, tail_s, step_s = time_step_kernels(eng, a.kernel_reps)
+    alg_bytes = BYTES_PER_VOXEL_STEP * n_myo + BYTES_PER_CABLE_NODE * meta["Np"]
+    achieved = alg_bytes / k_s / 1e9
+    print(f"[bench] fused myocardium + cable kernel {k_s * 1e6:.1f} us/launch -> {achieved:.0f} GB/s "
+          f"algorithmic ({alg_bytes / 1e6:.1f} MB); tail {tail_s * 1e6:.1f} us; replayed step "
+          f"{step_s * 1e6:.1f} us; activated {activated}/{n_myo} at "
+          f"t={(a.warmup + a.steps) * 0.02:.1f} ms", flush=True)
     ecg_info = None
     if ecg:
         fw_s, fw_stats = ecg.forward_solve_time(eng)
@@ -389,8 +392,11 @@ def main():
         "ecg": ecg_info,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "myo_step_kernel<float> (plain form)", "kernel_us": k_s * 1e6,
-                     "bytes_per_voxel": BYTES_PER_VOXEL_STEP},
+                     "kernel": "myo_cable_kernel<float> (plain form: fused myocardium step "
+                               "+ Purkinje cable)", "kernel_us": k_s * 1e6,
+                     "tail_us": tail_s * 1e6, "bytes_per_launch": alg_bytes,
+                     "bytes_per_voxel": BYTES_PER_VOXEL_STEP,
+                     "bytes_per_cable_node": BYTES_PER_CABLE_NODE},
         "cpu_baseline": cpu,
         "stream_copy": copy,
     }

@@ -391,7 +391,7 @@ __device__ __forceinline__ void myo_brick(Tile<T, BX, BY, BZ>& t, const hp_brick
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
   T Vn = Vc + dt * dvdt + dt * T(p.inv_cm) * (stim + lv);
   Vn = M<T>::mn(M<T>::mx(Vn, T(p.vmin)), T(p.vmax));
-  st32(V_out, slot, Vn);
+  buf_st(buf_rsrc(V_out, (uint32_t)g.n_bricks * TT::BVOL * (uint32_t)sizeof(T)), slot, Vn);
   // PMJ target voxels (label bit 7) are tested after the PMJ exchange instead
   if (p.at && !(lab & 0x80)) {
     const T thr = T(p.at_thr);

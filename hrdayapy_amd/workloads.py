@@ -17,9 +17,10 @@ SPMV_BYTES_PER_CELL = 20   # p 4 + Ap 4 + 3 face conductances 12
 FUSED_BYTES_PER_VOXEL = 177
 
 
-def graph_us(fn, reps):
+def graph_us(fn, reps, warm=2, timed=3):
     """Average device time (us) of fn(i), i = 0..reps-1, captured into one HIP
-    graph and replayed, HIP events around the replay on the capturing stream:
+    graph; after `warm` replays, `timed` replays between HIP events on the
+    replaying stream:
     no host launch overhead between the kernels (eager launches through ctypes
     can starve the GPU and time the host instead)."""
     side = torch.cuda.Stream()
@@ -30,15 +31,17 @@ def graph_us(fn, reps):
             for i in range(reps):
                 fn(i)
     torch.cuda.current_stream().wait_stream(side)
-    g.replay()
+    for _ in range(warm):
+        g.replay()
     st = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(st)
-    g.replay()
+    for _ in range(timed):
+        g.replay()
     e1.record(st)
     e1.synchronize()
     del g
-    return e0.elapsed_time(e1) / reps * 1e3
+    return e0.elapsed_time(e1) / (reps * timed) * 1e3
 
 
 def _event_us(fn, reps):
