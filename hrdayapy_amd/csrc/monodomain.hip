@@ -424,20 +424,36 @@ __global__ void __launch_bounds__(BX * BY * BZ)
 // previous one.  Each XCD (blockIdx & 7) works through a contiguous run of
 // bricks (compact in Morton order); the blocks of one XCD take its bricks
 // round robin.
+template <typename T>
+__device__ __forceinline__ void cable_node(const hp_cable& c, const T* __restrict__ V_in,
+                                           T* __restrict__ V_out, T* __restrict__ state,
+                                           int64_t ld, const hp_mono_params& p, int64_t i);
+
+// The first ncb workgroups (a multiple of 8, so the XCD of every brick
+// workgroup is unchanged) step the Purkinje cable instead (full TTP06 with
+// its own parameters), as in myo_cable_kernel.
 template <typename T, int BX, int BY, int BZ, bool kInst>
 __global__ void __launch_bounds__(BX * BY * BZ) diff_step_kernel(
-    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, hp_mono_params p) {
+    hp_bricks g, const T* __restrict__ V_in, T* __restrict__ V_out, hp_mono_params p,
+    hp_cable cb, const T* __restrict__ Vpk_in, T* __restrict__ Vpk_out, T* __restrict__ state_pk,
+    int64_t ld_pk, hp_mono_params p_pk, uint32_t ncb) {
   typedef Tile<T, BX, BY, BZ> TT;
   typedef HaloNbr<BX, BY, BZ> HN;
   typedef HaloRegs<T, BX, BY, BZ> HR;
   __shared__ TT t;
+  if (blockIdx.x < ncb) {
+    const int64_t i = (int64_t)blockIdx.x * (BX * BY * BZ) + threadIdx.x;
+    if (i < cb.n) cable_node<T>(cb, Vpk_in, Vpk_out, state_pk, ld_pk, p_pk, i);
+    return;
+  }
+  const uint32_t bid = blockIdx.x - ncb;
   const uint32_t NB = (uint32_t)g.n_bricks;
-  const uint32_t xcd = blockIdx.x & 7, G = gridDim.x;
+  const uint32_t xcd = bid & 7, G = gridDim.x - ncb;
   const uint32_t m = (G - xcd + 7) >> 3;  // blocks with index = xcd (mod 8)
   const uint32_t base = NB >> 3, rem = NB & 7;
   const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
   const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
-  uint32_t b = lo + (blockIdx.x >> 3);
+  uint32_t b = lo + (bid >> 3);
   if (b >= hi) return;  // whole workgroup: uniform
   const int tid = threadIdx.x;
   const int lz = tid % BZ, ly = (tid / BZ) % BY, lx = tid / (BY * BZ);
@@ -573,19 +589,29 @@ static int compute_units() {
 
 template <typename T>
 static int launch_diff(const hp_bricks* g, const T* V_in, T* V_out, const hp_mono_params* p,
-                       hipStream_t st) {
-  if (g->n_bricks == 0) return 0;
+                       hipStream_t st, const hp_cable* c = nullptr, const T* Vpk_in = nullptr,
+                       T* Vpk_out = nullptr, T* state_pk = nullptr, int64_t ld_pk = 0,
+                       const hp_mono_params* p_pk = nullptr) {
+  const hp_cable c0{};
+  const hp_mono_params p0{};
+  if (!c) c = &c0;
+  if (!p_pk) p_pk = &p0;
+  const int64_t bvol = (int64_t)g->bx * g->by * g->bz;
+  const int64_t ncb = (((c->n + bvol - 1) / bvol) + 7) / 8 * 8;
+  if (g->n_bricks == 0 && ncb == 0) return 0;
   const int64_t want = (int64_t)compute_units() * 16;
-  const dim3 grid((unsigned)(g->n_bricks < want ? g->n_bricks : want));
+  const dim3 grid((unsigned)((g->n_bricks < want ? g->n_bricks : want) + ncb));
   const int key = g->bx * 100 + g->by * 10 + g->bz;
   switch (key) {
 #define HP_DIFF(BX, BY, BZ)                                                                  \
   if (g->inst)                                                                               \
     hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, true>), grid, dim3(BX * BY * BZ), 0, \
-                       st, *g, V_in, V_out, *p);                                             \
+                       st, *g, V_in, V_out, *p, *c, Vpk_in, Vpk_out, state_pk, ld_pk, *p_pk,   \
+                       (uint32_t)ncb);                                                       \
   else                                                                                       \
     hipLaunchKernelGGL((diff_step_kernel<T, BX, BY, BZ, false>), grid, dim3(BX * BY * BZ),   \
-                       0, st, *g, V_in, V_out, *p)
+                       0, st, *g, V_in, V_out, *p, *c, Vpk_in, Vpk_out, state_pk, ld_pk,      \
+                       *p_pk, (uint32_t)ncb)
     case 444: HP_DIFF(4, 4, 4); break;
     case 844: HP_DIFF(8, 4, 4); break;
     case 884: HP_DIFF(8, 8, 4); break;
@@ -882,10 +908,11 @@ static int myo_cable_step(const hp_bricks* g, const T* V_in, T* V_out, T* state,
                           const hp_mono_params* p, const hp_cable* c, const T* Vpk_in,
                           T* Vpk_out, T* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
                           void* stream) {
-  if (!g || !p || !p->step_ctr || !p->group_amp || (p->do_reaction != 1 && p->do_reaction != 2) ||
-      (p->do_reaction == 2 && !p->rate_ext) || !g->groups || !state || ld < 0 ||
-      (uint64_t)ld * sizeof(T) >= (1ull << 32)) {
-    set_error("hp_myo_cable_step: myocardium arguments as hp_myo_step, do_reaction 1 or 2");
+  if (!g || !p || !p->step_ctr || !p->group_amp || p->do_reaction < 0 || p->do_reaction > 2 ||
+      (p->do_reaction == 2 && !p->rate_ext) ||
+      (p->do_reaction != 0 &&
+       (!g->groups || !state || ld < 0 || (uint64_t)ld * sizeof(T) >= (1ull << 32)))) {
+    set_error("hp_myo_cable_step: myocardium arguments as hp_myo_step");
     return -1;
   }
   if (!c || !p_pk || c->n < 0 || !p_pk->group_amp || ld_pk < c->n ||
@@ -899,6 +926,10 @@ static int myo_cable_step(const hp_bricks* g, const T* V_in, T* V_out, T* state,
               (long long)g->n_bricks);
     return -1;
   }
+  // the diffusion-only substep: the persistent kernel, with the cable blocks
+  if (p->do_reaction == 0)
+    return launch_diff<T>(g, V_in, V_out, p, (hipStream_t)stream, c, Vpk_in, Vpk_out, state_pk,
+                          ld_pk, p_pk);
   const int64_t ncb = (((c->n + bvol - 1) / bvol) + 7) / 8 * 8;
   if (g->n_bricks + ncb == 0) return 0;
   if (g->n_bricks + ncb > 0x7fffffff) {

@@ -249,14 +249,14 @@ class MonodomainEngine:
     # ------------------------------------------------------------------- step
     def _launch_step(self):
         """One time step: the myocardium and the cable in one launch
-        (hp_myo_cable_step; with the diffusion-only form, or fuse_cable off,
-        the cable kernel on a forked stream or after the myocardium kernel);
+        (hp_myo_cable_step; with fuse_cable off, the cable kernel on a forked
+        stream or after the myocardium kernel);
         one tail kernel then does the PMJ exchange and advances the step
         counter / stimulus amplitudes."""
         cur = torch.cuda.current_stream(self.dev)
         s = cur.cuda_stream
         a, b = self.p, 1 - self.p
-        if self.cable is not None and self.fuse_cable and self.p_myo.do_reaction != 0:
+        if self.cable is not None and self.fuse_cable:
             nat.call(f"hp_myo_cable_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
                      nat.ptr(self.V[b]), nat.ptr(self.state), self.state_ld, C.byref(self.p_myo),
                      C.byref(self.cable), nat.ptr(self.Vpk[a]), nat.ptr(self.Vpk[b]),

@@ -198,8 +198,9 @@ int hp_cable_step_f64(const hp_cable* c, const double* V_in, double* V_out, doub
 /* hp_myo_step and hp_cable_step of one time step in a single launch (the
  * cable nodes are extra workgroups of the fused myocardium kernel; the two
  * domains share no data within a step, the PMJ exchange follows in
- * hp_step_tail).  Same arguments and results as the two calls; do_reaction
- * must be 1 or 2.  No counterpart in the reference (which assembles both
+ * hp_step_tail).  Same arguments and results as the two calls, for every
+ * do_reaction (0: the cable blocks join the persistent diffusion kernel).
+ * No counterpart in the reference (which assembles both
  * domains into one sparse system, purkinje_myocardium_anisotropic_solver.py:
  * 1787-1791): it saves one kernel boundary per step. */
 int hp_myo_cable_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state,

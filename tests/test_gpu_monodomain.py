@@ -140,6 +140,16 @@ def test_one_launch_cable_step_equals_separate_kernels(cuda, dtype):
         ok = ~a.at.isnan()
         assert torch.equal(a.at[ok], b.at[ok])
     assert int((~a.at.isnan()).sum()) > 0
+    # the diffusion-only substep (do_reaction 0, given rate): the cable blocks
+    # then run inside the persistent diffusion kernel
+    rate = torch.from_numpy(np.random.default_rng(4).uniform(-2, 2, a.lay.n_slots)).to(cuda, dtype)
+    for e in (a, b):
+        e.p_myo.do_reaction = 0
+        e.p_myo.rate_ext = rate.data_ptr()
+        e.run(40, graph_chunk=0)
+    torch.cuda.synchronize()
+    assert torch.equal(a.V_global(), b.V_global())
+    assert torch.equal(a.state_pk, b.state_pk)
 
 
 def test_stencil_apply_matches_oracle_laplacian(cuda):
