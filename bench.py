@@ -206,7 +206,7 @@ def cpu_baseline(args_geo, kw, n_steps, dtype_name="float32"):
 
 
 def other_workload(a):
-    """Torso Poisson or ensemble measurement: one JSON line of its own."""
+    """Torso Poisson, ensemble or ionic-kernel measurement: one JSON line of its own."""
     from hrdayapy_amd import workloads as W
     log = lambda m: print(m, flush=True)
     if a.workload == "torso":
@@ -221,6 +221,19 @@ def other_workload(a):
                "result": r,
                "roofline": {"bound": "hbm", "achieved": r["pcg_iter_GBs"], "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": r["pcg_roofline_frac"], "traffic": None}}
+    elif a.workload == "ionic":
+        r = W.ionic_run(reps=a.kernel_reps)
+        log(f"[ionic] {r}")
+        out = {"metric": "TTP06 cell-steps/s, ionic kernel alone", "value": r["cell_steps_per_s"],
+               "unit": "cell-steps/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+               "data": "synthetic",
+               "config": {"workload": f"{r['cells']:,} cells, seeded phenotype and Vm, resting "
+                                      "state, dt 0.02 ms, SoA state"},
+               "result": r,
+               "roofline": {"bound": "valu", "achieved": r["valu_per_s"] / 1e12,
+                            "peak": W.VALU_PEAK_PER_S / 1e12, "unit": "T VALU lane-ops/s",
+                            "frac": r["valu_frac"], "traffic": None,
+                            "transcendental_frac": r["trans_frac"]}}
     else:
         brick = None if a.brick == "auto" else tuple(int(x) for x in a.brick.split("x"))
         r = W.ensemble_run(a.ensemble_size, 0, a.steps, a.warmup, a.graph_chunk, brick, log=log)
@@ -250,7 +263,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
                     help="size of the streaming-copy calibration (0 = off)")
-    ap.add_argument("--workload", type=str, default="biv", choices=["biv", "torso", "ensemble"],
+    ap.add_argument("--workload", type=str, default="biv", choices=["biv", "torso", "ensemble", "ionic"],
                     help="biv: the coupled 2M-voxel BiV step (default); torso: the 384x384x256 "
                          "Poisson solve; ensemble: 64 instances in one brick list")
     ap.add_argument("--ensemble-size", type=int, default=64)

@@ -141,6 +141,49 @@ def torso_solve(eng, meta, log=print, shape=(384, 384, 256), dx_mm=1.5, warm_ste
     return out
 
 
+# Static instruction counts of hp::ttp06_strided_kernel<float> in the gfx950
+# code object (per cell, straight-line path): 543 VALU instructions, of which
+# 100 transcendental (48 v_exp, 47 v_rcp, 4 v_log, 1 v_rsq).
+IONIC_VALU_PER_CELL = 543
+IONIC_TRANS_PER_CELL = 100
+IONIC_BYTES_PER_CELL = 152      # 18 state r+w (144) + V read + I_ion write
+# MI355X vector issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz one FP32 VALU
+# operation per lane and cycle; transcendentals at a quarter of that rate.
+VALU_PEAK_PER_S = 256 * 4 * 16 * 2.4e9
+TRANS_PEAK_PER_S = VALU_PEAK_PER_S / 4
+
+
+def ionic_run(n=2_006_132, reps=200, seed=0):
+    """The TTP06 cell step alone (hp_ttp06_step_f32) over n cells of seeded
+    random phenotype and membrane potential, SoA state (stride_var = n),
+    resting initial state: device time per launch from a graph of reps
+    launches, and the VALU / transcendental / HBM rates it implies."""
+    import ctypes as C
+    from .ttp06 import INIT, STATE_NAMES
+    KI_BASE, NAI_BASE = INIT["Ki"], INIT["Nai"]
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(seed)
+    V = torch.from_numpy(rng.uniform(-90.0, 30.0, n).astype(np.float32)).to(dev)
+    ct = torch.from_numpy(rng.integers(0, 3, n).astype(np.uint8)).to(dev)
+    st = torch.empty((18, n), dtype=torch.float32, device=dev)
+    for k, name in enumerate(STATE_NAMES):
+        st[k].fill_(INIT[name] - (KI_BASE if name == "Ki" else NAI_BASE if name == "Nai" else 0.0))
+    I = torch.empty(n, dtype=torch.float32, device=dev)
+
+    def launch(i):
+        nat.call("hp_ttp06_step_f32", nat.ptr(V), nat.ptr(st), nat.ptr(I), nat.ptr(ct), 0, None,
+                 None, n, n, 1, 0.02, KI_BASE, NAI_BASE, nat.stream_handle())
+
+    us = graph_us(launch, reps)
+    cps = n / (us * 1e-6)
+    return dict(cells=n, us_per_launch=us, cell_steps_per_s=cps,
+                valu_per_s=cps * IONIC_VALU_PER_CELL, valu_frac=cps * IONIC_VALU_PER_CELL / VALU_PEAK_PER_S,
+                trans_per_s=cps * IONIC_TRANS_PER_CELL,
+                trans_frac=cps * IONIC_TRANS_PER_CELL / TRANS_PEAK_PER_S,
+                hbm_GBs=cps * IONIC_BYTES_PER_CELL / 1e9,
+                finite=bool(torch.isfinite(I).all()))
+
+
 def ensemble_run(n_inst=64, seed=0, steps=1000, warmup=100, graph_chunk=50, brick=None,
                  size_range=(200_000, 2_000_000), log=print):
     """The seeded 64-instance ensemble (slabs and BiV shells of 0.2M-2M voxels,
