@@ -206,7 +206,7 @@ def cpu_baseline(args_geo, kw, n_steps, dtype_name="float32"):
 
 
 def other_workload(a):
-    """Torso Poisson, ensemble or ionic-kernel measurement: one JSON line of its own."""
+    """Torso Poisson, ensemble, ionic-kernel or stencil measurement: one JSON line of its own."""
     from hrdayapy_amd import workloads as W
     log = lambda m: print(m, flush=True)
     if a.workload == "torso":
@@ -221,6 +221,22 @@ def other_workload(a):
                "result": r,
                "roofline": {"bound": "hbm", "achieved": r["pcg_iter_GBs"], "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": r["pcg_roofline_frac"], "traffic": None}}
+    elif a.workload == "stencil":
+        geo, tree, kw, _ = build_geometry(n=a.grid, h=0.4 * 256 / a.grid)
+        brick = (4, 4, 4) if a.brick == "auto" else tuple(int(x) for x in a.brick.split("x"))
+        eng, meta = build_engine(geo, tree, kw, brick=brick)
+        eng.run(500, graph_chunk=a.graph_chunk)  # a wave under way
+        r = W.stencil_run(eng, reps=a.kernel_reps)
+        log(f"[stencil] {r}")
+        gbs = r["hbm_GBs"]
+        out = {"metric": "19-point stencil sweep bandwidth (L.V alone)", "value": gbs, "unit": "GB/s",
+               "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": f"synthetic BiV 256^3 @0.4 mm, {r['voxels']:,} voxels in "
+                                      f"{r['bricks']:,} bricks {tuple(r['brick'])}"},
+               "result": r,
+               "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                            "bytes_per_voxel": W.STENCIL_BYTES_PER_VOXEL}}
     elif a.workload == "ionic":
         r = W.ionic_run(reps=a.kernel_reps)
         log(f"[ionic] {r}")
@@ -263,7 +279,7 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
                     help="size of the streaming-copy calibration (0 = off)")
-    ap.add_argument("--workload", type=str, default="biv", choices=["biv", "torso", "ensemble", "ionic"],
+    ap.add_argument("--workload", type=str, default="biv", choices=["biv", "torso", "ensemble", "ionic", "stencil"],
                     help="biv: the coupled 2M-voxel BiV step (default); torso: the 384x384x256 "
                          "Poisson solve; ensemble: 64 instances in one brick list")
     ap.add_argument("--ensemble-size", type=int, default=64)

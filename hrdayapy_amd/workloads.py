@@ -184,6 +184,30 @@ def ionic_run(n=2_006_132, reps=200, seed=0):
                 finite=bool(torch.isfinite(I).all()))
 
 
+STENCIL_BYTES_PER_VOXEL = 33   # V read, L.V written, six FP32 D, one label byte
+
+
+def stencil_run(eng, reps=200):
+    """The 19-point operator alone (hp_stencil19_apply_f32: out = L V on the
+    brick layout) over the engine's bricks: device time per launch from a
+    graph of reps launches, and the algorithmic HBM rate."""
+    import ctypes as C
+    out = torch.empty_like(eng.V[0])
+    V = eng.V[eng.p]
+    dscale = eng.p_myo.dscale
+
+    def launch(i):
+        nat.call("hp_stencil19_apply_f32", C.byref(eng.bricks), nat.ptr(V), nat.ptr(out), dscale,
+                 nat.stream_handle())
+
+    us = graph_us(launch, reps)
+    n = eng.N_myo
+    return dict(voxels=n, bricks=eng.lay.n_bricks, brick=list(eng.lay.brick), us_per_launch=us,
+                voxels_per_s=n / (us * 1e-6),
+                hbm_GBs=n * STENCIL_BYTES_PER_VOXEL / (us * 1e-6) / 1e9,
+                finite=bool(torch.isfinite(out).all()))
+
+
 def ensemble_run(n_inst=64, seed=0, steps=1000, warmup=100, graph_chunk=50, brick=None,
                  size_range=(200_000, 2_000_000), log=print):
     """The seeded 64-instance ensemble (slabs and BiV shells of 0.2M-2M voxels,
