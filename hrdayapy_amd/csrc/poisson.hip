@@ -342,17 +342,33 @@ __global__ void __launch_bounds__(1024) leadfield_partial_kernel(
   for (int64_t c0 = r0; c0 < r1; c0 += LF_ROWS) {
     const int nr = (int)(r1 - c0 < LF_ROWS ? r1 - c0 : LF_ROWS);
     __syncthreads();
-    for (int q = threadIdx.x; q < nr * n_frames; q += blockDim.x) {
+    // frames past n_frames are staged as 0, so the sums below run without
+    // a per-frame branch (their accumulators are never written out)
+    for (int q = threadIdx.x; q < nr * LF_MAXF; q += blockDim.x) {
       const int f = q / nr, rr = q % nr;
-      vs[f][rr] = (double)V[(int64_t)f * ldv + row_slot[c0 + rr]];
+      vs[f][rr] = f < n_frames ? (double)V[(int64_t)f * ldv + row_slot[c0 + rr]] : 0.0;
     }
     __syncthreads();
     if (e < ne) {
-      for (int q = 0; q < nr; ++q) {
+      // eight rows of W in flight per thread (one load per row and thread
+      // would leave the loop waiting on each load in turn); the rows are
+      // still summed in order
+      int q = 0;
+      for (; q + 8 <= nr; q += 8) {
+        float w8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w8[u] = W[(c0 + q + u) * ne + e];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const double w = (double)w8[u];
+#pragma unroll
+          for (int f = 0; f < LF_MAXF; ++f) acc[f] = fma(w, vs[f][q + u], acc[f]);
+        }
+      }
+      for (; q < nr; ++q) {
         const double w = (double)W[(c0 + q) * ne + e];
 #pragma unroll
-        for (int f = 0; f < LF_MAXF; ++f)
-          if (f < n_frames) acc[f] = fma(w, vs[f][q], acc[f]);
+        for (int f = 0; f < LF_MAXF; ++f) acc[f] = fma(w, vs[f][q], acc[f]);
       }
     }
   }
