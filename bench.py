@@ -80,28 +80,6 @@ def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None):
     return geo, tree, kw, eng, meta, t_geo
 
 
-def autotune_brick(geo, tree, kw, steps=100):
-    """Pick the brick shape with the fastest fused step on this geometry."""
-    import torch
-    from hrdayapy_amd.layout import BRICK_SHAPES
-    best = None
-    res = {}
-    for b in BRICK_SHAPES:
-        eng, meta = build_engine(geo, tree, kw, brick=b)
-        eng.run(50, graph_chunk=50)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        eng.run(steps, graph_chunk=50)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / steps * 1e3
-        res["x".join(map(str, b))] = round(ms * 1e3, 1)
-        if best is None or ms < best[1]:
-            best = (b, ms)
-        del eng
-        torch.cuda.empty_cache()
-    return best[0], res
-
-
 def run_surrogate_form(geo, tree, kw, form, n_macro, warm_macro, brick, net=None, ref_at=None,
                        ecg=None, ecg_every_macro=5, phi_ref=None):
     """Time the multirate form (mlp or bigstep), with the ECG recorded every
@@ -252,7 +230,8 @@ def other_workload(a):
                             "transcendental_frac": r["trans_frac"]}}
     else:
         brick = None if a.brick == "auto" else tuple(int(x) for x in a.brick.split("x"))
-        r = W.ensemble_run(a.ensemble_size, 0, a.steps, a.warmup, a.graph_chunk, brick, log=log)
+        r = W.ensemble_run(a.ensemble_size, 0, a.steps, a.warmup,
+                           0 if a.graph_chunk_auto else a.graph_chunk, brick, log=log)
         out = {"metric": "voxel-steps/s, batched ensemble", "value": r["value"],
                "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                "ms_per_step": r["ms_per_step"], "higher_is_better": True, "dtype": "f32",
@@ -268,7 +247,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--graph-chunk", type=int, default=50)
+    ap.add_argument("--graph-chunk", type=int, default=0,
+                    help="steps per HIP graph launch; 0 = chosen by the autotuner (else 50)")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x8x8, or auto")
     ap.add_argument("--forms", type=str, default="plain,surrogate,bigstep")
@@ -287,6 +267,9 @@ def main():
     ap.add_argument("--ecg-every-ms", type=float, default=1.0)
     ap.add_argument("--bspm", type=int, default=256, help="BSPM electrodes besides the 12-lead set")
     a = ap.parse_args()
+    a.graph_chunk_auto = a.graph_chunk == 0
+    if a.graph_chunk_auto:
+        a.graph_chunk = 50  # unless the autotuner picks another
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not an implemented path)",
@@ -305,8 +288,13 @@ def main():
     geo, tree, kw, t_geo = build_geometry(n=a.grid, h=0.4 * 256 / a.grid)
     tune = None
     if a.brick == "auto":
-        brick, tune = autotune_brick(geo, tree, kw)
-        print(f"[bench] autotuned brick {brick}: us/step {tune}", flush=True)
+        from hrdayapy_amd.autotune import tune as autotune
+        brick, chunk_t, tune = autotune(lambda b: build_engine(geo, tree, kw, brick=b)[0],
+                                        chunks=(50, 10), steps=100)
+        if a.graph_chunk_auto:
+            a.graph_chunk = chunk_t
+        print(f"[bench] autotuned brick {brick}, {chunk_t} steps per graph launch: {tune}",
+              flush=True)
     else:
         brick = tuple(int(x) for x in a.brick.split("x"))
     eng, meta = build_engine(geo, tree, kw, brick=brick)

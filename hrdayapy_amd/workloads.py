@@ -213,9 +213,26 @@ def ensemble_run(n_inst=64, seed=0, steps=1000, warmup=100, graph_chunk=50, bric
     """The seeded 64-instance ensemble (slabs and BiV shells of 0.2M-2M voxels,
     sigma x U(0.7,1.3), G_Ks/G_to x U(0.5,1.5), 1-3 pacing sites), FP32,
     one fused launch per step over all instances' bricks."""
-    from .ensemble import Ensemble, draw_instances
+    from .ensemble import Ensemble, draw_instances, instance_geometries
     t0 = time.perf_counter()
     specs = draw_instances(n_inst, seed, size_range)
+    tuned = None
+    if brick is None:
+        # brick shape and steps per launch by measurement on the first 8 instances
+        from .autotune import tune
+        sample = specs[:8]
+        geoms = instance_geometries(sample)
+        brick, chunk_t, tuned = tune(
+            lambda b: Ensemble(sample, geoms=geoms, dtype=torch.float32, device="cuda",
+                               brick=b).eng, chunks=(50, 100, 200), steps=200, log=log)
+        if graph_chunk <= 0:
+            graph_chunk = chunk_t
+        log(f"[ensemble] autotuned brick {brick}, {graph_chunk} steps per graph launch: {tuned}")
+    if graph_chunk <= 0:
+        graph_chunk = 50
+    if warmup % graph_chunk or steps % graph_chunk:
+        warmup += -warmup % graph_chunk
+        steps += -steps % graph_chunk
     ens = Ensemble(specs, dtype=torch.float32, device="cuda", brick=brick)
     eng = ens.eng
     log(f"[ensemble] {n_inst} instances, {ens.n_nodes:,} voxels in {ens.layout.n_bricks:,} "
@@ -234,7 +251,8 @@ def ensemble_run(n_inst=64, seed=0, steps=1000, warmup=100, graph_chunk=50, bric
                 kinds={k: sum(s.kind == k for s in specs) for k in ("slab", "biv")},
                 brick=list(ens.layout.brick), steps=steps, wall_s=wall,
                 value=ens.n_nodes * steps / wall, ms_per_step=wall / steps * 1e3,
-                activated_fraction_mean=float(np.mean(act)), graph_chunk=graph_chunk)
+                activated_fraction_mean=float(np.mean(act)), graph_chunk=graph_chunk,
+                autotune=tuned)
 
 
 class EcgSetup:
