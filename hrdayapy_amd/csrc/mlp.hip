@@ -190,6 +190,10 @@ __device__ __forceinline__ int perm_k(int p) {
   const int g = p >> 3, q = p & 7;
   return q < 4 ? 4 * g + q : 16 + 4 * g + (q - 4);
 }
+// its inverse: neuron n (0..31) -> position
+__device__ __forceinline__ int inv_perm_k(int n) {
+  return n < 16 ? 8 * (n >> 2) + (n & 3) : 8 * ((n - 16) >> 2) + 4 + ((n - 16) & 3);
+}
 
 struct MlpSmem {
   __bf16 w1[NHID][32];          // [out][in], natural input order (padded 22 -> 32)
@@ -274,16 +278,25 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     const int o = e / 32, i = e % 32;
     S.w1[o][i] = i < NIN ? W1[o * NIN + i] : f2bf(0.f);
   }
-  for (int e = tid; e < NHID * NHID; e += THREADS) {
-    const int o = e / NHID, p = e % NHID;
-    const int kin = (p & ~31) + perm_k(p & 31);
-    S.w2[o][p] = W2[o * NHID + kin];
-    S.w3[o][p] = W3[o * NHID + kin];
+  // layers 2-4: 16-byte loads of 8 natural inputs n0..n0+7 (n0 a multiple of
+  // 8); under perm_k they land as two runs of 4 consecutive LDS positions
+  // (inputs n0..n0+3 at inv(n0), n0+4..n0+7 at inv(n0+4))
+  for (int e = tid; e < NHID * NHID / 8; e += THREADS) {
+    const int o = e / (NHID / 8), n0 = (e % (NHID / 8)) * 8;
+    const int pa = (n0 & ~31) + inv_perm_k(n0 & 31), pb = (n0 & ~31) + inv_perm_k((n0 & 31) + 4);
+    const uint4 a = *reinterpret_cast<const uint4*>(W2 + o * NHID + n0);
+    const uint4 b = *reinterpret_cast<const uint4*>(W3 + o * NHID + n0);
+    *reinterpret_cast<uint2*>(&S.w2[o][pa]) = uint2{a.x, a.y};
+    *reinterpret_cast<uint2*>(&S.w2[o][pb]) = uint2{a.z, a.w};
+    *reinterpret_cast<uint2*>(&S.w3[o][pa]) = uint2{b.x, b.y};
+    *reinterpret_cast<uint2*>(&S.w3[o][pb]) = uint2{b.z, b.w};
   }
-  for (int e = tid; e < 32 * NHID; e += THREADS) {
-    const int o = e / NHID, p = e % NHID;
-    const int kin = (p & ~31) + perm_k(p & 31);
-    S.w4[o][p] = o < NOUT ? W4[o * NHID + kin] : f2bf(0.f);
+  for (int e = tid; e < 32 * NHID / 8; e += THREADS) {
+    const int o = e / (NHID / 8), n0 = (e % (NHID / 8)) * 8;
+    const int pa = (n0 & ~31) + inv_perm_k(n0 & 31), pb = (n0 & ~31) + inv_perm_k((n0 & 31) + 4);
+    const uint4 a = o < NOUT ? *reinterpret_cast<const uint4*>(W4 + o * NHID + n0) : uint4{0, 0, 0, 0};
+    *reinterpret_cast<uint2*>(&S.w4[o][pa]) = uint2{a.x, a.y};
+    *reinterpret_cast<uint2*>(&S.w4[o][pb]) = uint2{a.z, a.w};
   }
   for (int e = tid; e < NHID; e += THREADS) {
     S.b1[e] = wts.b1[e];
