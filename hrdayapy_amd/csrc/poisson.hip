@@ -327,63 +327,84 @@ __global__ void source_gather_kernel(const double* __restrict__ lap, const int32
 // its row range into part[q][f][e]; pass 2 adds the block partials in block
 // order (deterministic).  Threads of a block take consecutive electrodes, so
 // a row of W is read coalesced; the block's V values are staged in LDS.
-constexpr int LF_ROWS = 256, LF_MAXF = 16;
-__global__ void __launch_bounds__(1024) leadfield_partial_kernel(
+constexpr int LF_MAXF = 16;  // frames per call (the M = 16 of the MFMA tile)
+// FP64 MFMA form: one wave per tile of 16 electrodes, v_mfma_f64_16x16x4f64
+// over 4 rows per instruction (A = V^T: 16 frames x 4 rows, B = W: 4 rows x
+// 16 electrodes, C = the 16 x 16 FP64 partial sums in 4 registers per lane).
+// Operand layout of the instruction: lane l holds A[l%16][l/16] and
+// B[l/16][l%16]; C[4*i + l/16][l%16] in its i-th register (the FP64 form; the
+// FP32 16x16x4 form keeps rows 4*(l/16)+i instead).  Eight K-steps
+// of loads are issued before their MFMAs.  Frames past n_frames, electrodes
+// past ne and rows past the block's range enter as 0.
+constexpr int LF_WAVES = 8, LF_UNROLL = 8;
+__global__ void __launch_bounds__(LF_WAVES * 64) leadfield_partial_kernel(
     const float* __restrict__ W, const int32_t* __restrict__ row_slot, int64_t n_rows, int ne,
     const float* __restrict__ V, int64_t ldv, int n_frames, int64_t rows_per_block,
     double* __restrict__ part) {
-  __shared__ double vs[LF_MAXF][LF_ROWS];
-  const int e = blockIdx.y * blockDim.x + threadIdx.x;
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e0 = (blockIdx.y * LF_WAVES + wave) * 16;
+  if (e0 >= ne) return;  // whole wave
+  const int mi = lane & 15, kq = lane >> 4;  // A row (frame) / B column (electrode); K index
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < n_rows ? r0 + rows_per_block : n_rows;
-  double acc[LF_MAXF];
+  const bool frame_ok = mi < n_frames, elec_ok = e0 + mi < ne;
+  // every load reads an in-range element (clamped row, frame, electrode) and
+  // is zeroed afterwards, so the loads issue without branches
+  const int mc = frame_ok ? mi : 0, ec = elec_ok ? e0 + mi : ne - 1;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  if (r0 >= r1) return;
+  for (int64_t r = r0; r < r1; r += 4 * LF_UNROLL) {
+    int32_t sl[LF_UNROLL];
+    float wv[LF_UNROLL];
 #pragma unroll
-  for (int f = 0; f < LF_MAXF; ++f) acc[f] = 0.0;
-  for (int64_t c0 = r0; c0 < r1; c0 += LF_ROWS) {
-    const int nr = (int)(r1 - c0 < LF_ROWS ? r1 - c0 : LF_ROWS);
-    __syncthreads();
-    // frames past n_frames are staged as 0, so the sums below run without
-    // a per-frame branch (their accumulators are never written out)
-    for (int q = threadIdx.x; q < nr * LF_MAXF; q += blockDim.x) {
-      const int f = q / nr, rr = q % nr;
-      vs[f][rr] = f < n_frames ? (double)V[(int64_t)f * ldv + row_slot[c0 + rr]] : 0.0;
+    for (int u = 0; u < LF_UNROLL; ++u) {
+      const int64_t row = r + 4 * u + kq, rc = row < r1 ? row : r1 - 1;
+      sl[u] = row_slot[rc];
+      wv[u] = W[rc * ne + ec];
+      if (!(row < r1 && elec_ok)) wv[u] = 0.f;
     }
-    __syncthreads();
-    if (e < ne) {
-      // eight rows of W in flight per thread (one load per row and thread
-      // would leave the loop waiting on each load in turn); the rows are
-      // still summed in order
-      int q = 0;
-      for (; q + 8 <= nr; q += 8) {
-        float w8[8];
+    float av[LF_UNROLL];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) w8[u] = W[(c0 + q + u) * ne + e];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const double w = (double)w8[u];
-#pragma unroll
-          for (int f = 0; f < LF_MAXF; ++f) acc[f] = fma(w, vs[f][q + u], acc[f]);
-        }
-      }
-      for (; q < nr; ++q) {
-        const double w = (double)W[(c0 + q) * ne + e];
-#pragma unroll
-        for (int f = 0; f < LF_MAXF; ++f) acc[f] = fma(w, vs[f][q], acc[f]);
-      }
+    for (int u = 0; u < LF_UNROLL; ++u) {
+      av[u] = V[(int64_t)mc * ldv + sl[u]];
+      if (!(r + 4 * u + kq < r1 && frame_ok)) av[u] = 0.f;
     }
+#pragma unroll
+    for (int u = 0; u < LF_UNROLL; ++u)
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], (double)wv[u], acc, 0, 0, 0);
   }
+  const int e = e0 + mi;
   if (e < ne)
-    for (int f = 0; f < n_frames; ++f)
-      part[((int64_t)blockIdx.x * n_frames + f) * ne + e] = acc[f];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = 4 * i + kq;
+      if (f < n_frames) part[((int64_t)blockIdx.x * n_frames + f) * ne + e] = acc[i];
+    }
 }
 
-__global__ void leadfield_reduce_kernel(const double* __restrict__ part, int n_blocks, int ne,
-                                        int n_frames, double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;  // f*ne + e
-  if (q >= ne * n_frames) return;
+// out[q] (q = f*ne + e) = sum over the row blocks of part: one wave per
+// output, lane l adding blocks l, l+64, ... in order (eight loads in flight),
+// then a fixed shuffle tree: a fixed summation order.
+__global__ void __launch_bounds__(256) leadfield_reduce_kernel(const double* __restrict__ part,
+                                                               int n_blocks, int ne, int n_frames,
+                                                               double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= ne * n_frames) return;  // whole wave
+  const int64_t stride = (int64_t)ne * n_frames;
   double s = 0.0;
-  for (int b = 0; b < n_blocks; ++b) s += part[(int64_t)b * ne * n_frames + q];
-  out[q] = s;
+  int b = lane;
+  for (; b + 7 * 64 < n_blocks; b += 8 * 64) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)(b + 64 * u) * stride + q];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += x[u];
+  }
+  for (; b < n_blocks; b += 64) s += part[(int64_t)b * stride + q];
+  s = wave_sum(s);
+  if (lane == 0) out[q] = s;
 }
 
 // out[e] = sum_k w[k] * phi[idx[k]] over k in [ptr[e], ptr[e+1])  (FP64)
@@ -558,7 +579,13 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
     set_error("hp_leadfield_apply_f32: bad arguments (1..%d frames)", LF_MAXF);
     return -1;
   }
-  int nb = 1024;
+  // row blocks: as many as the partial buffer holds, up to 4096
+  int64_t nbl = part_len / ((int64_t)n_frames * ne);
+  int nb = (int)(nbl > 4096 ? 4096 : nbl);
+  if (nb < 1) {
+    set_error("hp_leadfield_apply_f32: partial buffer needs %lld doubles", (long long)n_frames * ne);
+    return -1;
+  }
   const int64_t rpb = (n_rows + nb - 1) / nb;
   nb = (int)((n_rows + rpb - 1) / rpb);
   if ((int64_t)nb * n_frames * ne > part_len) {
@@ -567,13 +594,14 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
     return -1;
   }
   hipStream_t st = (hipStream_t)stream;
-  // one thread per electrode (up to 1024 per block), whole waves
-  const int bt = ne <= 1024 ? (ne + 63) / 64 * 64 : 1024;
-  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (ne + bt - 1) / bt), dim3(bt), 0, st, W,
-                     row_slot, n_rows, ne, V, ldv, n_frames, rpb, part);
+  // one wave per 16-electrode tile, LF_WAVES tiles per workgroup
+  const int tiles = (ne + 15) / 16;
+  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (tiles + LF_WAVES - 1) / LF_WAVES),
+                     dim3(LF_WAVES * 64), 0, st, W, row_slot, n_rows, ne, V, ldv, n_frames, rpb,
+                     part);
   const int rc = check_launch("hp_leadfield_apply_f32/partial");
   if (rc) return rc;
-  hipLaunchKernelGGL(leadfield_reduce_kernel, dim3(ceil_div((int64_t)ne * n_frames, 256)),
+  hipLaunchKernelGGL(leadfield_reduce_kernel, dim3(ceil_div((int64_t)ne * n_frames, 4)),
                      dim3(256), 0, st, part, nb, ne, n_frames, out);
   return check_launch("hp_leadfield_apply_f32/reduce");
 }

@@ -398,7 +398,7 @@ class LeadField:
                 log(f"[leadfield] {e + 1}/{ne} electrodes, {time.perf_counter() - t0:.1f} s")
         self.W = W
         self.setup_s = time.perf_counter() - t0
-        self.part = torch.zeros(1024 * self.batch * ne, dtype=torch.float64, device=dev)
+        self.part = torch.zeros(4096 * self.batch * ne, dtype=torch.float64, device=dev)
         self.out = torch.zeros((self.batch, ne), dtype=torch.float64, device=dev)
 
     def apply(self, V_frames, n_frames=None, ldv=None):

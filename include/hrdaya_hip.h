@@ -447,8 +447,10 @@ int hp_heart_lap6_f64(const hp_bricks* g, const double* V, double* lap, double v
  * every frame): the electrode potentials are a fixed linear functional of Vm,
  * phi[e] = sum_r W[r][e] Vm[slot[r]] (reciprocity with the pinned operator),
  * so out[f*ne + e] = sum_r W[r*ne + e] * V[f*ldv + row_slot[r]] for n_frames
- * (<= 16) frames, FP32 products summed in FP64 in a fixed order.  part holds
- * part_len >= 1024*n_frames*ne doubles. */
+ * (<= 16) frames, FP32 products summed in FP64 in a fixed order (FP64 MFMA
+ * over row blocks, then the blocks).  part holds part_len >= n_frames*ne
+ * doubles per row block; the row count is split into min(4096,
+ * part_len/(n_frames*ne)) blocks. */
 int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_rows, int ne,
                            const float* V, int64_t ldv, int n_frames, double* part,
                            int64_t part_len, double* out, void* stream);
