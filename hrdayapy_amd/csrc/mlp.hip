@@ -205,20 +205,6 @@ struct MlpSmem {
   int in_log[32];
 };
 
-// Raw input feature f (0..21) of a slot: V, the 18 state variables (absolute;
-// entry si of the compact state), phenotype one-hot.
-__device__ __forceinline__ float raw_feature(int f, const float* __restrict__ V,
-                                             const float* __restrict__ st, int64_t ld,
-                                             int64_t slot, int64_t si, int ct, float ki_base,
-                                             float nai_base) {
-  if (f == 0) return V[slot];
-  if (f <= 18) {
-    const float x = st[(int64_t)(f - 1) * ld + si];
-    return f == 1 ? x + ki_base : (f == 2 ? x + nai_base : x);
-  }
-  return (f - 19) == ct ? 1.f : 0.f;
-}
-
 // The bias of rows 4*grp..+4 of output tile m: the accumulator's initial value
 __device__ __forceinline__ floatx4 bias4(const float* bias, int m, int grp) {
   const float* b = bias + m * 16 + grp * 4;
@@ -336,7 +322,18 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     }
     // skip tiles without myocardium (wave-uniform)
     if (!__any(any)) continue;
-    // ---- inputs: B fragments (32 features x 16 voxels) of the columns
+    // ---- inputs: B fragments (32 features x 16 voxels) of the columns.
+    // The raw values are loaded first, unconditionally, from valid addresses
+    // (V or a state row; the one-hot features and lanes without a voxel
+    // are replaced afterwards), so the loads of a tile are in flight together.
+    float raw[NC][8];
+#pragma unroll
+    for (int t = 0; t < NC; ++t)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int f = grp * 8 + q, fc = f < 19 ? f : 0;
+        raw[t][q] = fc == 0 ? V[slot[t]] : state[(int64_t)(fc - 1) * ld + si[t]];
+      }
     bf16x8 xin[NC];
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
@@ -345,8 +342,8 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
         const int f = grp * 8 + q;
         float x = 0.f;
         if (lab[t] != 0xFF && f < NIN) {
-          x = raw_feature(f, V, state, ld, slot[t], si[t], lab[t] & 3, sp.ki_base,
-                          sp.nai_base);
+          x = f >= 19 ? ((f - 19) == (lab[t] & 3) ? 1.f : 0.f)
+                      : raw[t][q] + (f == 1 ? sp.ki_base : (f == 2 ? sp.nai_base : 0.f));
           if (S.in_log[f]) x = __logf(fmaxf(x, 1e-30f));
           x = (x - S.in_mean[f]) * S.in_rstd[f];
         }
@@ -400,7 +397,19 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
 #pragma unroll
       for (int t = 0; t < NC; ++t) out[m][t] = c[t];
     }
-    // ---- mode decision and update, per voxel (this lane's 4+4 outputs)
+    // ---- mode decision and update, per voxel (this lane's 4+4 outputs).
+    // The state values this lane updates are loaded first, unconditionally
+    // (valid addresses; used only for non-fallback voxels).
+    float sx[NC][2][4];
+#pragma unroll
+    for (int t = 0; t < NC; ++t)
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int o = m * 16 + grp * 4 + q, oc = o >= 1 && o < NOUT ? o : 1;
+          sx[t][m][q] = state[(int64_t)(oc - 1) * ld + si[t]];
+        }
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
       const int64_t sl = slot[t];
@@ -423,7 +432,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
           } else if (!fallback) {
             float* p = state + (int64_t)(o - 1) * ld + si[t];
             const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
-            const float x0 = *p + base_v;
+            const float x0 = sx[t][m][q] + base_v;
             const float x1 = S.in_log[o] ? __expf(__logf(fmaxf(x0, 1e-30f)) + d) : x0 + d;
             *p = x1 - base_v;
           }
