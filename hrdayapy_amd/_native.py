@@ -60,7 +60,8 @@ class HpMlpNorm(C.Structure):
 class HpSurrogateParams(C.Structure):
     _fields_ = [("t_macro", c_f32), ("dvdt_max", c_f32), ("v_lo", c_f32), ("v_hi", c_f32),
                 ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("variant", c_i32),
-                ("fb_list", c_vp), ("fb_count", c_vp), ("slot_list", c_vp), ("n_list", c_i64)]
+                ("fb_list", c_vp), ("fb_count", c_vp), ("slot_list", c_vp), ("n_list", c_i64),
+                ("state_list", c_vp)]
 
 
 class HpMgLevel(C.Structure):

@@ -317,7 +317,10 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
       lab[t] = slot[t] >= 0 ? g.label[slot[t]] : (uint8_t)0xFF;
       if (slot[t] < 0) slot[t] = 0;  // no voxel: a valid address, never written (label 0xFF)
       // compact state entry (read and written only where lab != 0xFF)
-      si[t] = lab[t] != 0xFF ? (int64_t)state_index(g.groups, (uint32_t)slot[t]) : 0;
+      si[t] = lab[t] == 0xFF ? 0
+              : (sp.slot_list && sp.state_list)
+                  ? (int64_t)sp.state_list[item]
+                  : (int64_t)state_index(g.groups, (uint32_t)slot[t]);
       any |= lab[t] != 0xFF;
     }
     // skip tiles without myocardium (wave-uniform)

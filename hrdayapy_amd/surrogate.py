@@ -366,12 +366,17 @@ class SurrogateRunner:
         n_list = (tissue.numel() + 31) // 32 * 32
         self.slot_list = torch.full((max(32, n_list),), -1, dtype=torch.int32, device=eng.dev)
         self.slot_list[:tissue.numel()] = tissue
+        # tissue slots in slot order: their compact state indices are 0, 1, ...
+        self.state_list = torch.zeros_like(self.slot_list)
+        self.state_list[:tissue.numel()] = torch.arange(tissue.numel(), dtype=torch.int32,
+                                                        device=eng.dev)
         # inference kernel shape: variant 1 (two 16-voxel MFMA columns per wave,
         # 512 threads) measured 17 % faster than variant 0 on the 2 M-voxel BiV
         self.sp = nat.HpSurrogateParams(T_MACRO, dvdt_max, v_lo, v_hi, eng.ki_base, eng.nai_base,
                                         0, int(variant), nat.ptr(self.fb_list) if compact else None,
                                         nat.ptr(self.fb_count) if compact else None,
-                                        nat.ptr(self.slot_list), self.slot_list.numel())
+                                        nat.ptr(self.slot_list), self.slot_list.numel(),
+                                        nat.ptr(self.state_list))
         self.n_blocks = n_blocks
         self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
         self._graph = None

@@ -525,6 +525,10 @@ typedef struct hp_surrogate_params {
    * tissue slots skips the padding of partly filled bricks. */
   const int32_t* slot_list;
   int64_t n_list;
+  /* Optional, with slot_list: the compact state index of each listed slot
+   * (any value at -1 entries), which spares the kernel the group-table
+   * lookup; NULL = looked up in g->groups. */
+  const int32_t* state_list;
 } hp_surrogate_params;
 
 /* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),

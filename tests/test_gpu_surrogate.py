@@ -86,7 +86,7 @@ def test_gradients_match_cpu_autograd(cuda):
         assert relb < 5e-2
 
 
-@pytest.mark.parametrize("variant,listed", [(0, False), (1, False), (1, True)])
+@pytest.mark.parametrize("variant,listed", [(0, False), (1, False), (1, True), (1, "state")])
 def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     print(f"[case] fused surrogate inference, kernel variant {variant}, slot list {listed}",
           flush=True)
@@ -130,11 +130,17 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     rate = torch.zeros(lay.n_slots, device=cuda)
     sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, variant)
     if listed:  # the tissue slots in a shuffled order, padded with -1 to a multiple of 32
-        tis = rng.permutation(lay.slot_of_node.astype(np.int32))
+        perm = rng.permutation(lay.n_nodes)
+        tis = lay.slot_of_node[perm].astype(np.int32)
         lst = np.full((len(tis) + 31) // 32 * 32, -1, np.int32)
         lst[:len(tis)] = tis
         lst_d = torch.from_numpy(lst).to(cuda)
         sp.slot_list, sp.n_list = lst_d.data_ptr(), len(lst)
+        if listed == "state":  # with the state index of each listed slot
+            stl = np.zeros_like(lst)
+            stl[:len(tis)] = son[perm]
+            stl_d = torch.from_numpy(stl).to(cuda)
+            sp.state_list = stl_d.data_ptr()
     import ctypes
     nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
              state.data_ptr(), n + 64, rate.data_ptr(), ctypes.byref(net.hp_w),
