@@ -37,9 +37,10 @@ template <typename T>
 static int ttp06_launch(const T* V, T* state, T* I_ion, const uint8_t* celltype,
                         int fixed_celltype, const T* gks, const T* gto, int64_t n, int64_t sv,
                         int64_t sn, T dt, T ki_base, T nai_base, void* stream) {
-  if (n < 0 || (!celltype && (fixed_celltype < 0 || fixed_celltype > 2))) {
-    set_error("hp_ttp06_step: bad arguments (n=%lld, fixed_celltype=%d)", (long long)n,
-              fixed_celltype);
+  if (n < 0 || (!celltype && (fixed_celltype < 0 || fixed_celltype > 2)) ||
+      (n > 0 && (!V || !state))) {
+    set_error("hp_ttp06_step: bad arguments (n=%lld, fixed_celltype=%d; V and state required)",
+              (long long)n, fixed_celltype);
     return -1;
   }
   if (n == 0) return 0;

@@ -493,8 +493,8 @@ extern "C" {
 int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
                  int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
                  const float* bias, int ksplit, void* stream) {
-  if (M <= 0 || N <= 0 || K <= 0 || ksplit < 1 || ksplit > 1024) {
-    set_error("hp_gemm_bf16: bad shape %d x %d x %d (ksplit %d)", M, N, K, ksplit);
+  if (M <= 0 || N <= 0 || K <= 0 || ksplit < 1 || ksplit > 1024 || !A || !B || !C) {
+    set_error("hp_gemm_bf16: bad shape %d x %d x %d (ksplit %d) or null operand", M, N, K, ksplit);
     return -1;
   }
   if (ksplit > 1 && (beta != 0.f || bias)) {

@@ -1065,7 +1065,11 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const float4* __restri
 
 template <typename T>
 static int gather(const T* src, const int32_t* idx, T* out, int64_t n, void* stream) {
-  if (n <= 0) return 0;
+  if (n < 0 || (n > 0 && (!src || !idx || !out))) {
+    set_error("hp_gather: bad arguments (n=%lld)", (long long)n);
+    return -1;
+  }
+  if (n == 0) return 0;
   hipLaunchKernelGGL(gather_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
                      src, idx, out, n);
   return check_launch("hp_gather");

@@ -1,0 +1,127 @@
+"""CPU checks of the C-ABI's argument validation: bad arguments return -1 with a
+message naming the entry point, and empty inputs return 0; every case here is
+decided on the host before any HIP call (no GPU needed)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hrdayapy_amd import _native as nat
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return nat.load()
+
+
+def _rejects(rc, needle):
+    assert rc == -1
+    assert needle in nat.last_error()
+
+
+def test_library_level_errors(lib):
+    p = C.c_void_p()
+    _rejects(lib.hp_device_alloc(C.byref(p), 0, 0), "hp_device_alloc")
+    assert p.value is None
+    assert lib.hp_last_error(C.create_string_buffer(4), 0) == -1
+
+
+def test_ttp06_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    # no celltype array and an out-of-range fixed type
+    _rejects(lib.hp_ttp06_step_f32(ptr, ptr, ptr, None, 3, None, None, 1, 1, 1, 0.02, 138.0, 7.0,
+                                   None), "hp_ttp06_step")
+    _rejects(lib.hp_ttp06_step_f32(ptr, ptr, ptr, None, 0, None, None, -1, 1, 1, 0.02, 138.0, 7.0,
+                                   None), "hp_ttp06_step")
+    # cells but no membrane potential or state
+    _rejects(lib.hp_ttp06_step_f32(None, ptr, ptr, None, 0, None, None, 4, 1, 1, 0.02, 138.0, 7.0,
+                                   None), "V and state required")
+    _rejects(lib.hp_ttp06_step_f32(ptr, None, ptr, None, 0, None, None, 4, 1, 1, 0.02, 138.0, 7.0,
+                                   None), "V and state required")
+    # no cells: nothing to do
+    assert lib.hp_ttp06_step_f32(None, None, None, None, 0, None, None, 0, 1, 1, 0.02, 138.0, 7.0,
+                                 None) == 0
+
+
+def test_brick_grid_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    g = nat.HpBricks(n_bricks=-1, bx=4, by=4, bz=4)
+    _rejects(lib.hp_stencil19_apply_f32(C.byref(g), ptr, ptr, 1.0, None), "bad brick grid")
+    _rejects(lib.hp_stencil19_apply_f32(None, ptr, ptr, 1.0, None), "bad brick grid")
+    # an empty grid launches nothing
+    g0 = nat.HpBricks(n_bricks=0, bx=4, by=4, bz=4)
+    assert lib.hp_stencil19_apply_f32(C.byref(g0), None, None, 1.0, None) == 0
+    # a grid whose six D rows overflow 32-bit byte offsets
+    big = nat.HpBricks(n_bricks=(1 << 31) // 64, bx=4, by=4, bz=4)
+    _rejects(lib.hp_stencil19_apply_f32(C.byref(big), ptr, ptr, 1.0, None), "32-bit offset")
+    # the step needs its parameters (step counter and stimulus amplitudes)
+    _rejects(lib.hp_myo_step_f32(C.byref(g0), ptr, ptr, ptr, 0, None, None),
+             "step_ctr and group_amp are required")
+    p = nat.HpMonoParams(do_reaction=3, step_ctr=ptr, group_amp=ptr)
+    _rejects(lib.hp_myo_step_f32(C.byref(g0), ptr, ptr, ptr, 0, C.byref(p), None),
+             "do_reaction must be 0, 1 or 2")
+    # the reaction needs the compact-state group table
+    p1 = nat.HpMonoParams(do_reaction=1, step_ctr=ptr, group_amp=ptr)
+    g1 = nat.HpBricks(n_bricks=1, bx=4, by=4, bz=4)
+    _rejects(lib.hp_myo_step_f32(C.byref(g1), ptr, ptr, ptr, 64, C.byref(p1), None),
+             "needs groups and state")
+
+
+def test_torso_and_leadfield_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    _rejects(lib.hp_poisson_spmv_f32(None, ptr, ptr, None), "bad torso grid")
+    t = nat.HpTorso(ni=4, nj=4, nk=4, n_cells=64)  # no chunk table
+    _rejects(lib.hp_poisson_spmv_f32(C.byref(t), ptr, ptr, None), "chunk table required")
+    t.chunks, t.nk = ptr, 40000  # columns longer than the 15-bit span fields
+    _rejects(lib.hp_poisson_spmv_f32(C.byref(t), ptr, ptr, None), "nk <= 32767")
+    # lead field: at most 16 frames per call, and a partial buffer that holds one block
+    part = np.zeros(16, np.float64)
+    args = (ptr, ptr, 4, 2, ptr, 4)
+    _rejects(lib.hp_leadfield_apply_f32(*args, 17, part.ctypes.data, 16, part.ctypes.data, None),
+             "1..16 frames")
+    _rejects(lib.hp_leadfield_apply_f32(*args, 0, part.ctypes.data, 16, part.ctypes.data, None),
+             "1..16 frames")
+    _rejects(lib.hp_leadfield_apply_f32(*args, 16, part.ctypes.data, 16, part.ctypes.data, None),
+             "partial buffer needs 32 doubles")
+
+
+def test_mlp_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    _rejects(lib.hp_gemm_bf16(0, 32, 32, ptr, 32, 1, ptr, 32, 1, ptr, 32, 1.0, 0.0, None, 1, None),
+             "bad shape")
+    _rejects(lib.hp_gemm_bf16(32, 32, 32, None, 32, 1, ptr, 32, 1, ptr, 32, 1.0, 0.0, None, 1,
+                              None), "null operand")
+    _rejects(lib.hp_gemm_bf16(32, 32, 32, ptr, 32, 1, ptr, 32, 1, ptr, 32, 1.0, 0.0, ptr, 2, None),
+             "split-K")
+    w, n, sp = nat.HpMlpWeights(), nat.HpMlpNorm(), nat.HpSurrogateParams()
+    g = nat.HpBricks(n_bricks=1, bx=4, by=4, bz=4)  # no group table
+    _rejects(lib.hp_surrogate_step(C.byref(g), ptr, ptr, ptr, 64, ptr, C.byref(w), C.byref(n),
+                                   C.byref(sp), 0, None), "groups required")
+    g.groups = ptr
+    sp.slot_list, sp.n_list = ptr, 33  # a listed count that is not a multiple of 32
+    _rejects(lib.hp_surrogate_step(C.byref(g), ptr, ptr, ptr, 64, ptr, C.byref(w), C.byref(n),
+                                   C.byref(sp), 0, None), "multiple of 32")
+
+
+def test_small_kernels_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    t = np.zeros(4, np.float64)
+    ptr = buf.ctypes.data
+    # events: a table or a count array is required; empty node sets do nothing
+    _rejects(lib.hp_crossing_events_f32(None, t.ctypes.data, 4, 16, -60.0, 1, 0.0, 0.0, ptr, None,
+                                        0, None), "hp_crossing_events_f32")
+    _rejects(lib.hp_crossing_events_f32(ptr, t.ctypes.data, 4, 16, -60.0, 1, 0.0, 0.0, None,
+                                        None, 0, None), "hp_crossing_events_f32")
+    assert lib.hp_crossing_events_f32(ptr, t.ctypes.data, 4, 0, -60.0, 1, 0.0, 0.0, ptr, None, 0,
+                                      None) == 0
+    # copy: whole float4 vectors only; empty copies do nothing
+    _rejects(lib.hp_stream_copy_f32(ptr, ptr, 3, None), "multiple of 4")
+    assert lib.hp_stream_copy_f32(None, None, 0, None) == 0
+    # gather: pointers required for a non-empty gather
+    _rejects(lib.hp_gather_f32(None, ptr, ptr, 8, None), "hp_gather")
+    _rejects(lib.hp_gather_f32(ptr, ptr, ptr, -1, None), "hp_gather")
+    assert lib.hp_gather_f32(None, None, None, 0, None) == 0
