@@ -512,6 +512,10 @@ int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int
                  void* stream) {
   const int64_t n = rows * cols;
   if (n <= 0) return 0;
+  if (!x || !y) {
+    set_error("hp_silu_bf16: bad arguments (null operand)");
+    return -1;
+  }
   hipLaunchKernelGGL(silu_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, x,
                      (__bf16*)y, n, ld_in, ld_out, cols);
   return check_launch("hp_silu_bf16");
@@ -520,6 +524,10 @@ int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int
 int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, float* dxf, int64_t n,
                      void* stream) {
   if (n <= 0) return 0;
+  if (!x || !dy || !dx) {
+    set_error("hp_silu_bwd_bf16: bad arguments (null operand)");
+    return -1;
+  }
   hipLaunchKernelGGL(silu_bwd_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
                      x, dy, (__bf16*)dx, dxf, n);
   return check_launch("hp_silu_bwd_bf16");
@@ -527,6 +535,10 @@ int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, float* dxf, int6
 
 int hp_cast_bf16(const float* x, void* y, int64_t n, void* stream) {
   if (n <= 0) return 0;
+  if (!x || !y) {
+    set_error("hp_cast_bf16: bad arguments (null operand)");
+    return -1;
+  }
   hipLaunchKernelGGL(cast_bf16_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream,
                      x, (__bf16*)y, n);
   return check_launch("hp_cast_bf16");
@@ -536,6 +548,10 @@ int hp_mse_grad(const float* Y, const float* T, void* dY, float* dYf, int rows, 
                 float* loss, void* stream) {
   const int64_t n = (int64_t)rows * cols;
   if (n <= 0) return 0;
+  if (!Y || !T || !dY || !loss || used < 1 || used > cols) {
+    set_error("hp_mse_grad: bad arguments (null operand, or used outside 1..cols)");
+    return -1;
+  }
   hipLaunchKernelGGL(mse_grad_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, Y,
                      T, (__bf16*)dY, dYf, rows, cols, used, loss);
   return check_launch("hp_mse_grad");
@@ -543,6 +559,10 @@ int hp_mse_grad(const float* Y, const float* T, void* dY, float* dYf, int rows, 
 
 int hp_colsum(const float* X, int rows, int cols, int ld, float* out, void* stream) {
   if (rows <= 0 || cols <= 0) return 0;
+  if (!X || !out || ld < cols) {
+    set_error("hp_colsum: bad arguments (null operand, or ld < cols)");
+    return -1;
+  }
   hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, (hipStream_t)stream, X, rows, cols,
                      ld, out);
   return check_launch("hp_colsum");
@@ -551,6 +571,10 @@ int hp_colsum(const float* X, int rows, int cols, int ld, float* out, void* stre
 int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
             float eps, int step, float wd, void* stream) {
   if (n <= 0 || step <= 0) return 0;
+  if (!w || !g || !m || !v) {
+    set_error("hp_adam: bad arguments (null operand)");
+    return -1;
+  }
   const float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
   hipLaunchKernelGGL(adam_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, (hipStream_t)stream, w, g, m,
                      v, n, lr, b1, b2, eps, bc1, bc2, wd);

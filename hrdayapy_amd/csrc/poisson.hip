@@ -632,6 +632,10 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
 int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
                             const double* w, double* out, int64_t n_electrodes, void* stream) {
   if (n_electrodes <= 0) return 0;
+  if (!phi || !ptr || !idx || !w || !out) {
+    set_error("hp_leads_accumulate_f64: bad arguments (null operand)");
+    return -1;
+  }
   hipLaunchKernelGGL(leads_kernel, dim3(ceil_div(n_electrodes, 256)), dim3(256), 0,
                      (hipStream_t)stream, phi, ptr, idx, w, out, n_electrodes);
   return check_launch("hp_leads_accumulate_f64");

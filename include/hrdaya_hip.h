@@ -538,6 +538,9 @@ typedef struct hp_surrogate_params {
 int hp_gemm_bf16(int M, int N, int K, const void* A, int64_t sam, int64_t sak, const void* B,
                  int64_t sbk, int64_t sbn, float* C, int64_t ldc, float alpha, float beta,
                  const float* bias, int ksplit, void* stream);
+/* In the helpers below the FP32 gradient copies dxf and dYf are optional (NULL:
+ * not written); every other pointer is required when the size is non-zero
+ * (NULL returns -1). */
 int hp_silu_bf16(const float* x, void* y, int64_t rows, int cols, int ld_in, int ld_out,
                  void* stream);
 int hp_silu_bwd_bf16(const float* x, const float* dy, void* dx, float* dxf, int64_t n,

@@ -125,3 +125,23 @@ def test_small_kernels_arguments(lib):
     _rejects(lib.hp_gather_f32(None, ptr, ptr, 8, None), "hp_gather")
     _rejects(lib.hp_gather_f32(ptr, ptr, ptr, -1, None), "hp_gather")
     assert lib.hp_gather_f32(None, None, None, 0, None) == 0
+
+
+def test_training_and_leads_helpers_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    _rejects(lib.hp_silu_bf16(None, ptr, 4, 4, 4, 4, None), "hp_silu_bf16")
+    _rejects(lib.hp_silu_bwd_bf16(ptr, ptr, None, None, 8, None), "hp_silu_bwd_bf16")
+    _rejects(lib.hp_cast_bf16(ptr, None, 8, None), "hp_cast_bf16")
+    _rejects(lib.hp_mse_grad(ptr, ptr, ptr, None, 4, 4, 4, None, None), "hp_mse_grad")
+    _rejects(lib.hp_mse_grad(ptr, ptr, ptr, None, 4, 4, 5, ptr, None), "used outside 1..cols")
+    _rejects(lib.hp_colsum(ptr, 4, 4, 3, ptr, None), "ld < cols")
+    _rejects(lib.hp_adam(ptr, None, ptr, ptr, 8, 1e-3, 0.9, 0.999, 1e-8, 1, 0.0, None), "hp_adam")
+    _rejects(lib.hp_leads_accumulate_f64(None, ptr, ptr, ptr, ptr, 2, None),
+             "hp_leads_accumulate_f64")
+    # empty sizes do nothing, whatever the pointers
+    assert lib.hp_silu_bf16(None, None, 0, 4, 4, 4, None) == 0
+    assert lib.hp_cast_bf16(None, None, 0, None) == 0
+    assert lib.hp_colsum(None, 0, 4, 4, None, None) == 0
+    assert lib.hp_adam(None, None, None, None, 0, 1e-3, 0.9, 0.999, 1e-8, 1, 0.0, None) == 0
+    assert lib.hp_leads_accumulate_f64(None, None, None, None, None, 0, None) == 0
