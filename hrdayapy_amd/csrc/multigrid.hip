@@ -435,7 +435,8 @@ int hp_mg_vcycle_f32(const hp_torso* t, const hp_mg* mg, const float* r, float* 
 
 int hp_mgpcg_begin_f32(const hp_torso* t, const hp_mg* mg, const float* r, float* d, float* z,
                        float* p, hp_pcg_ws* ws, double tol, int max_iter, void* stream) {
-  if (!mg_ok(t, mg, "hp_mgpcg_begin_f32") || !ws) return -1;
+  if (!mg_ok(t, mg, "hp_mgpcg_begin_f32") ||
+      !ws_ok(ws, 0, true, "hp_mgpcg_begin_f32")) return -1;
   hipStream_t st = (hipStream_t)stream;
   vcycle(t, mg, r, z, nullptr, st);
   const int nb = grid_blocks(t);
@@ -447,7 +448,8 @@ int hp_mgpcg_begin_f32(const hp_torso* t, const hp_mg* mg, const float* r, float
 
 int hp_mgpcg_iterate_f32(const hp_torso* t, const hp_mg* mg, float* r, float* d, float* z,
                          float* p, float* Ap, hp_pcg_ws* ws, int n_iter, void* stream) {
-  if (!mg_ok(t, mg, "hp_mgpcg_iterate_f32") || !ws || n_iter < 0) return -1;
+  if (!mg_ok(t, mg, "hp_mgpcg_iterate_f32") ||
+      !ws_ok(ws, n_iter, true, "hp_mgpcg_iterate_f32")) return -1;
   const int nb = grid_blocks(t);
   hipStream_t st = (hipStream_t)stream;
   for (int it = 0; it < n_iter; ++it) {

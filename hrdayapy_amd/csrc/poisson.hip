@@ -510,7 +510,8 @@ int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* strea
 
 int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x, double* r,
                             hp_pcg_ws* ws, void* stream) {
-  if (!torso_ok(t, "hp_poisson_residual_f64") || !ws) return -1;
+  if (!torso_ok(t, "hp_poisson_residual_f64") ||
+      !ws_ok(ws, 0, false, "hp_poisson_residual_f64")) return -1;
   const int nb = grid_blocks(t);
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(residual_f64_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, b, x, r, ws->part);
@@ -520,7 +521,8 @@ int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x,
 
 int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
                      double tol, int max_iter, int mode, void* stream) {
-  if (!torso_ok(t, "hp_pcg_begin_f32") || !ws) return -1;
+  if (!torso_ok(t, "hp_pcg_begin_f32") ||
+      !ws_ok(ws, 0, true, "hp_pcg_begin_f32")) return -1;
   const int nb = grid_blocks(t);
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(pcg_init_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, d, p, ws->part);
@@ -531,7 +533,8 @@ int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws*
 
 int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap, hp_pcg_ws* ws,
                        int n_iter, void* stream) {
-  if (!torso_ok(t, "hp_pcg_iterate_f32") || !ws || n_iter < 0) return -1;
+  if (!torso_ok(t, "hp_pcg_iterate_f32") ||
+      !ws_ok(ws, n_iter, true, "hp_pcg_iterate_f32")) return -1;
   const int nb = grid_blocks(t);
   hipStream_t st = (hipStream_t)stream;
   for (int it = 0; it < n_iter; ++it) {
@@ -610,7 +613,7 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
                        const int32_t* rows, const int32_t* ptr, const int32_t* fine,
                        int64_t n_rows_hit, double coeff, double n_expected, double* b,
                        int64_t pin, hp_pcg_ws* ws, void* stream) {
-  if (!ws || n_valid <= 0 || n_expected <= 0.0) {
+  if (!ws || !ws->part || !ws->out || n_valid <= 0 || n_expected <= 0.0) {
     set_error("hp_bspm_source_f64: bad arguments");
     return -1;
   }

@@ -355,6 +355,16 @@ inline int grid_blocks(const hp_torso* t) {
 // pAp from the partials of a product kernel, alpha = rz / pAp (poisson.hip)
 void launch_pap_finish(double* part, int nb, double* sc, hipStream_t st);
 
+// the workspace: partials plus the PCG scalar slots (pcg) or the two sums
+// (residual); a non-negative iteration count
+inline bool ws_ok(const hp_pcg_ws* ws, int64_t n_iter, bool pcg, const char* what) {
+  if (!ws || !ws->part || !(pcg ? (void*)ws->scalars : (void*)ws->out) || n_iter < 0) {
+    set_error("%s: workspace (part and %s) required, n_iter >= 0", what, pcg ? "scalars" : "out");
+    return false;
+  }
+  return true;
+}
+
 inline bool torso_ok(const hp_torso* t, const char* what) {
   if (!t || t->ni <= 0 || t->nj <= 0 || t->nk <= 0 || t->nk > 32767 || !t->chunks ||
       t->n_chunks < 0 || t->n_cells < 0 || t->n_cells > ((int64_t)1 << 28)) {

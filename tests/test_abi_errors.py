@@ -145,3 +145,17 @@ def test_training_and_leads_helpers_arguments(lib):
     assert lib.hp_colsum(None, 0, 4, 4, None, None) == 0
     assert lib.hp_adam(None, None, None, None, 0, 1e-3, 0.9, 0.999, 1e-8, 1, 0.0, None) == 0
     assert lib.hp_leads_accumulate_f64(None, None, None, None, None, 0, None) == 0
+
+
+def test_pcg_workspace_arguments(lib):
+    buf = np.zeros(64, np.float32)
+    ptr = buf.ctypes.data
+    t = nat.HpTorso(ni=4, nj=4, nk=4, n_cells=64, chunks=ptr, n_chunks=1)
+    _rejects(lib.hp_poisson_residual_f64(C.byref(t), ptr, ptr, ptr, None, None),
+             "workspace (part and out) required")
+    ws = nat.HpPcgWs(part=ptr, out=ptr)  # no PCG scalar slots
+    _rejects(lib.hp_pcg_begin_f32(C.byref(t), ptr, ptr, ptr, C.byref(ws), 1e-8, 10, 0, None),
+             "workspace (part and scalars) required")
+    ws.scalars = ptr
+    _rejects(lib.hp_pcg_iterate_f32(C.byref(t), ptr, ptr, ptr, ptr, C.byref(ws), -1, None),
+             "n_iter >= 0")
