@@ -136,6 +136,7 @@ _SIGS = {
     "hp_torso_chunks": (C.c_int, [c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_poisson_spmv_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp]),
     "hp_poisson_residual_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_vp]),
+    "hp_poisson_residual_f64_noalloc": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, c_vp, c_vp]),
     "hp_pcg_begin_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64, C.c_int,
                                    C.c_int, c_vp]),
     "hp_pcg_iterate_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, c_vp, P(HpPcgWs), C.c_int,

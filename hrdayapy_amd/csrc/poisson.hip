@@ -22,9 +22,44 @@
 // once when the `done` flag is set.  A reducing kernel writes one partial sum
 // per block and a one-block finish kernel after it sums them in block order
 // and updates the scalars (fixed order: deterministic results; no fences).
+#include <mutex>
+
 #include "poisson_common.h"
 
 namespace hp {
+
+// Scratch of the allocation-free residual (hp_poisson_residual_f64_noalloc):
+// the block partial sums of residual_f64_kernel, two per block for at most
+// PMAX_BLOCKS blocks (64 KiB on every GPU the library runs on, held until the
+// process ends).  Host code never passes `scratch` itself to a kernel: on the
+// host the name is only a stand-in for the symbol; the device address comes
+// from hipGetSymbolAddress, once per device (scratch_address below).
+static __device__ __align__(256) double scratch[2 * PMAX_BLOCKS];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on each device, with that device current); nullptr with the error set.
+static double* scratch_address(const char* what) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+    set_error("%s: no current HIP device (or device index >= %d)", what, SCRATCH_MAX_DEVICES);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("%s: hipGetSymbolAddress(scratch) on device %d: %s", what, dev,
+                hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_dev[dev] = (double*)p;
+  }
+  return g_scratch_dev[dev];
+}
 
 __global__ void __launch_bounds__(PBLOCK) spmv_f32_kernel(hp_torso t, const float* __restrict__ x,
                                                           float* __restrict__ y) {
@@ -517,6 +552,29 @@ int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x,
   hipLaunchKernelGGL(residual_f64_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, b, x, r, ws->part);
   hipLaunchKernelGGL(finish_sums_kernel<2>, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb, ws->out);
   return check_launch("hp_poisson_residual_f64");
+}
+
+int hp_poisson_residual_f64_noalloc(const hp_torso* t, const double* b, const double* x,
+                                    double* r, double* out, void* stream) {
+  if (!torso_ok(t, "hp_poisson_residual_f64_noalloc")) return -1;
+  if (!b || !x || !r || !out) {
+    set_error("hp_poisson_residual_f64_noalloc: b, x, r and out are required");
+    return -1;
+  }
+  const int nb = grid_blocks(t);
+  if (nb > PMAX_BLOCKS) {  // grid_blocks caps the grid; the scratch holds 2 partials per block
+    set_error("hp_poisson_residual_f64_noalloc: %d blocks exceed the scratch of %d", nb,
+              PMAX_BLOCKS);
+    return -1;
+  }
+  double* part = scratch_address("hp_poisson_residual_f64_noalloc");
+  if (!part) return -1;
+  // every block writes both of its partials before the finish kernel reads
+  // them, so nothing left in the scratch by an earlier call is read
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(residual_f64_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, b, x, r, part);
+  hipLaunchKernelGGL(finish_sums_kernel<2>, dim3(1), dim3(PBLOCK), 0, st, part, nb, out);
+  return check_launch("hp_poisson_residual_f64_noalloc");
 }
 
 int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,

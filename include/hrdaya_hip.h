@@ -368,6 +368,17 @@ int hp_poisson_spmv_f32(const hp_torso* t, const float* x, float* y, void* strea
 /* r = b - A x in FP64; ws->out[0] = ||r||^2, ws->out[1] = ||b||^2. */
 int hp_poisson_residual_f64(const hp_torso* t, const double* b, const double* x, double* r,
                             hp_pcg_ws* ws, void* stream);
+/* hp_poisson_residual_f64 without a workspace argument and without any
+ * allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  out >= 2 doubles (device): ||r||^2,
+ * ||b||^2.  The block partial sums go to a scratch array inside the library,
+ * 2*4096 doubles (64 KiB) per GPU, held until the process ends; every call
+ * writes the partials it reads, so nothing needs clearing between calls.
+ * Calls that overlap on one GPU (two streams, two host threads) share that
+ * array and race: serialise them.  Same results as hp_poisson_residual_f64. */
+int hp_poisson_residual_f64_noalloc(const hp_torso* t, const double* b, const double* x,
+                                    double* r, double* out, void* stream);
 /* Jacobi-PCG for A d = r (FP32 vectors, FP64 inner products), d = 0 initially.
  * begin: sets p = M^-1 r and the device-side state; iterate: n_iter iterations,
  * each a no-op once converged or max_iter was reached.  mode 0 stops at
