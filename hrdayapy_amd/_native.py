@@ -47,6 +47,23 @@ class HpPmjAt(C.Structure):
                 ("at_thr", c_f64)]
 
 
+class HpCnPmj(C.Structure):
+    _fields_ = [("n_targets", c_i64), ("tgt_slot", c_vp), ("tgt_ptr", c_vp), ("tgt_node", c_vp),
+                ("tgt_w", c_vp), ("node_ptr", c_vp), ("node_slot", c_vp), ("node_w", c_vp),
+                ("n_jnodes", c_i64), ("jnode", c_vp)]
+
+
+class HpCnParams(C.Structure):
+    _fields_ = [("cm_dt", c_f64), ("theta", c_f64), ("dscale", c_f64), ("dscale_pk", c_f64),
+                ("tol", c_f64), ("max_sweeps", c_i32), ("pad_", c_i32), ("scalars", c_vp),
+                ("part", c_vp)]
+
+
+class HpCnWork(C.Structure):
+    _fields_ = [("Vs", c_vp), ("b", c_vp), ("x", c_vp), ("Vs_pk", c_vp), ("b_pk", c_vp),
+                ("x_pk", c_vp)]
+
+
 class HpMlpWeights(C.Structure):
     _fields_ = [("w1", c_vp), ("w2", c_vp), ("w3", c_vp), ("w4", c_vp), ("b1", c_vp), ("b2", c_vp),
                 ("b3", c_vp), ("b4", c_vp)]
@@ -114,6 +131,12 @@ _SIGS = {
     "hp_myo_cable_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                          P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                          c_vp]),
+    "hp_cn_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                 P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
+                                 P(HpCnParams), P(HpCnWork), c_vp]),
+    "hp_cn_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
+                                 P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
+                                 P(HpCnParams), P(HpCnWork), c_vp]),
     "hp_fallback_react_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
                                         P(HpMonoParams), c_vp]),
     "hp_stencil19_apply_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),

@@ -6,17 +6,27 @@ purkinje_myocardium_anisotropic_solver.py:1308-1881): same arguments, same
 returned dict keys and the same ``vm_snapshots.npz`` schema (:1923-1993).
 
 What differs is the machinery: the state lives in HBM in a brick layout
-(hrdayapy_amd.layout), and every 0.02 ms step is one fused HIP kernel for
-the myocardium (TTP06 + 19-point stencil + stimulus + clamp + activation
-detection), one for the Purkinje cable and two for the PMJ exchange, replayed
-from a captured HIP graph.  Diffusion is explicit (forward Euler on V^n); the
-reference's Crank-Nicolson/BiCGSTAB arguments (theta, cg_tol, cg_max_iter)
-are accepted for signature compatibility and ignored.
+(hrdayapy_amd.layout) and the steps are HIP kernels replayed from captured
+HIP graphs.  Two schemes:
+
+* ``scheme="cn"`` (default): the reference's step -- Lie splitting, then
+  Crank-Nicolson diffusion with ``theta`` on the whole coupled system (PMJ
+  couplings included), solved to ``||b - A x|| <= cg_tol*||b||`` (:1746-1791).
+  The reference's BiCGSTAB is replaced by matrix-free Jacobi sweeps with the
+  same stopping rule (hp_cn_step); at most min(cg_max_iter, 12) sweeps per
+  step, and a warning names the steps that stopped at that bound.
+* ``scheme="explicit"``: the fast form -- one fused kernel per step (TTP06 +
+  forward-Euler 19-point diffusion on V^n + stimulus + clamp + activation),
+  the cable in the same launch, the PMJ exchange integrated exactly; ``dt``
+  above the stencil's stability limit is split into substeps.  It does not
+  use theta/cg_tol/cg_max_iter (a warning says so when they are changed) and
+  differs from the CN scheme by the schemes' truncation error.
 """
 from __future__ import annotations
 
 import math
 import time
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -27,6 +37,11 @@ from ...engine import MonodomainEngine
 from ...layout import BrickLayout, choose_brick
 
 
+# sweeps of the CN solve per step captured in the step's graph (each step
+# launches them all; the ones after convergence return at once)
+CN_MAX_SWEEPS = 12
+
+
 def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=None, *,
                          voxel_size=0.4, dx_p=0.2, sigma_P=0.45, sigma_l=3 * 1.31e-4,
                          sigma_t=1.31e-4, sigma_M=None, Cm=0.01, A_M=140.0, R_P=0.015, A_P=None,
@@ -34,14 +49,16 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
                          stim_regions=None, stim_len_mm=2.0, c_pmj=0.05, n_pmj=1,
                          phi_endo_max=0.35, phi_epi_min=0.65, dtype=torch.float32,
                          device="cuda", brick=None, verbose=False, sigma_scale=None,
-                         inst=None, inst_params=None, auto_substep=False, alloc="torch"):
+                         inst=None, inst_params=None, auto_substep=False, alloc="torch",
+                         scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
     """Set up the tissue on the host and upload it into a MonodomainEngine.
 
     ``sigma_M`` selects the isotropic 6-point operator (simulation.compute_coupled);
     otherwise sigma_l/sigma_t and fibre_dir give the 19-point anisotropic one.
-    Diffusion is explicit, so ``dt`` above the forward-Euler limit of the
-    stencil or the cable raises ValueError; with ``auto_substep`` the engine
-    instead steps dt/n_sub, n_sub = meta["n_sub"] engine steps per dt.
+    ``scheme="explicit"``: ``dt`` above the forward-Euler limit of the stencil
+    or the cable raises ValueError; with ``auto_substep`` the engine instead
+    steps dt/n_sub, n_sub = meta["n_sub"] engine steps per dt.
+    ``scheme="cn"``: the reference's Crank-Nicolson step (any dt).
     Returns (engine, meta) where meta holds the reference-order bookkeeping.
     """
     t0 = time.perf_counter()
@@ -66,7 +83,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
     sites = tissue.stim_sites(stim_protocol, root, ectopic_region, vox, Np, stim_regions)
     dt_lim = tissue.explicit_dt_limit(D6, voxel_size, A_M, Cm, comp_edges, pc["w_pp"], pc["Ai_P"])
     n_sub = 1
-    if dt > dt_lim:
+    if scheme == "explicit" and dt > dt_lim:
         if not auto_substep:
             raise ValueError(f"dt={dt} ms exceeds the explicit diffusion limit "
                              f"{dt_lim:.4g} ms of this tissue (pass a smaller dt)")
@@ -75,8 +92,9 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
         lay, D6, ct, h=voxel_size, A_M=A_M, Cm=Cm, dt=dt / n_sub, dtype=dtype, device=device,
         cable=dict(n=Np, edges=comp_edges, w_pp=pc["w_pp"], Ai_P=pc["Ai_P"]),
         pmj=dict(ids=pmj_ids, patches=patches, c_pmj=c_pmj, Ai_P=pc["Ai_P"], Ai_M=pc["Ai_M"]),
-        stim_sites=sites, inst=inst, inst_params=inst_params, alloc=alloc)
-    meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim,
+        stim_sites=sites, inst=inst, inst_params=inst_params, alloc=alloc, scheme=scheme,
+        theta=theta, cg_tol=cg_tol, max_sweeps=max_sweeps)
+    meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim, scheme=scheme,
                 comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,
                 pmj_ids=pmj_ids, pmj_patches=patches, sites=sites, layout=lay,
@@ -107,8 +125,15 @@ def purkinje_myocardium_anisotropic_solver(
         stim_regions=None, stim_len_mm=2.0, c_pmj=0.05, n_pmj=1, cg_tol=1e-5, cg_max_iter=100,
         n_frames=100, frame_save_dt=None, anim_tmp_dir=None, device=None, dtype=torch.float32,
         out_npz=None, phi_endo_max=0.35, phi_epi_min=0.65, vm_save_dt=None, out_vm_npz=None,
-        vm_tmp_dir=None, graph_chunk=50, _sigma_M=None):
+        vm_tmp_dir=None, graph_chunk=50, scheme="cn", _sigma_M=None):
     """Run the coupled simulation; see the module docstring for what differs from the reference."""
+    if scheme not in ("cn", "explicit"):
+        raise ValueError(f"scheme must be 'cn' or 'explicit' (got {scheme!r})")
+    if scheme == "explicit" and (theta != 0.5 or cg_tol != 1e-5):
+        warnings.warn("scheme='explicit' integrates diffusion by forward Euler: theta and cg_tol "
+                      "apply to scheme='cn' only and are not used", stacklevel=2)
+    if scheme == "cn" and (not 0.0 <= theta <= 1.0 or cg_tol < 0 or int(cg_max_iter) < 1):
+        raise ValueError("scheme='cn' needs 0 <= theta <= 1, cg_tol >= 0 and cg_max_iter >= 1")
     if A_P is not None and _sigma_M is None:
         raise TypeError(
             "A_P is no longer a settable parameter -- it's mechanically determined by R_P "
@@ -124,7 +149,9 @@ def purkinje_myocardium_anisotropic_solver(
         R_P=R_P, A_P=A_P if _sigma_M is not None else None, dt=dt, T=T,
         stim_protocol=stim_protocol, ectopic_region=ectopic_region, stim_regions=stim_regions,
         stim_len_mm=stim_len_mm, c_pmj=c_pmj, n_pmj=n_pmj, phi_endo_max=phi_endo_max,
-        phi_epi_min=phi_epi_min, dtype=dtype, device=dev, verbose=True, auto_substep=True)
+        phi_epi_min=phi_epi_min, dtype=dtype, device=dev, verbose=True, auto_substep=True,
+        scheme=scheme, theta=theta, cg_tol=cg_tol,
+        max_sweeps=min(int(cg_max_iter), CN_MAX_SWEEPS))
     Np, N_myo, n_sub = meta["Np"], meta["N_myo"], meta["n_sub"]
     if n_sub > 1:
         print(f"[hrdaya] dt={dt} ms is above the explicit diffusion limit "
@@ -176,6 +203,15 @@ def purkinje_myocardium_anisotropic_solver(
     wall = time.perf_counter() - t0
     print(f"[hrdaya] {n_steps * n_sub:,} steps x {N_myo:,} voxels in {wall:.2f} s "
           f"({n_steps * n_sub * N_myo / max(wall, 1e-9):.3e} voxel-steps/s)")
+    cn = eng.cn_stats()
+    if cn is not None:
+        print(f"[hrdaya] CN: {cn['mean_sweeps']:.2f} Jacobi sweeps per step on average "
+              f"(tolerance {cn['tol']:g}, at most {cn['max_sweeps']})")
+        if cn["not_converged"]:
+            warnings.warn(f"CN solve: {cn['not_converged']} of {cn['steps']} steps stopped at "
+                          f"{cn['max_sweeps']} sweeps with relative residual up to "
+                          f"{cn['worst_rel_res']:.2e} > cg_tol={cn['tol']:g}", RuntimeWarning,
+                          stacklevel=2)
     if run_vm and vm_times:
         _save_vm_snapshots(out_vm_npz, np.arange(N_myo, dtype=np.int32) + Np, meta["nodes_mm"],
                            vox, vm_buf[:len(vm_times)].cpu().numpy(),
@@ -192,7 +228,7 @@ def purkinje_myocardium_anisotropic_solver(
         vert_to_surf_node=np.zeros(0, np.int32),
         frames_surf=np.asarray(frames_surf, dtype=np.float32), times=times,
         surface_node_ids=surf_ids, activation_times=eng.at_nodes().float().cpu().numpy(),
-        wall_s=wall)
+        wall_s=wall, cn_stats=cn)
     if out_npz is not None:
         _save_npz(results, out_npz)
     return results

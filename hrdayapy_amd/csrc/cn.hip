@@ -1,0 +1,525 @@
+// Crank-Nicolson diffusion of the coupled Purkinje + myocardium monodomain:
+// the reference's time discretisation (purkinje_myocardium_anisotropic_solver.py
+// :757-849 and the loop :1746-1791), matrix-free on the brick layout.
+//
+// One time step (Lie splitting, as the reference):
+//   V*  = V^n - dt*I_ion(V^n, s^n) + (dt/Cm)*I_stim            cn_react_kernel
+//   b   = (Cm/dt) V* + (1-theta) L V*
+//   A x = b,  A = (Cm/dt) I - theta L,  from x0 = V*            cn_sweep_kernel
+//   V^{n+1} = clamp(x, -150, 80)                               cn_final_kernel
+// L = D^-1 K of the whole system: the 19-point anisotropic stencil on the
+// myocardial rows (weights computed from D in-kernel), the cable edges w_pp on
+// the Purkinje rows, and the PMJ couplings c_pmj/|patch| on both (:606-746).
+//
+// The linear solve replaces the reference's Jacobi-preconditioned BiCGSTAB
+// (:786-849) by Jacobi sweeps x <- x + (b - A x)/diag(A) with the same stopping
+// rule, ||b - A x|| <= tol*||b||, bounded by a sweep count.  A is strongly
+// diagonally dominant at the reference's step sizes (the Jacobi contraction
+// is ~0.05-0.2 on the tissue and ~0.4 per sweep on a junction pair), and each
+// sweep is one pass over V and D with no reduction other than ||r||: 2-4
+// sweeps meet tol = 1e-5 where BiCGSTAB needs 1-2 iterations of two products
+// and five reductions each.  Both are approximations of the same CN solution
+// within the tolerance.
+//
+// Each sweep evaluates the residual of its input iterate and writes the next
+// iterate; FP64 block partials of ||r||^2 (and of ||b||^2 on the first sweep)
+// are summed in block order by a one-block finish kernel, which sets the
+// device-side `done` flag; later sweeps of the step return at once.  The
+// residual of sweep k belongs to x_{k-1}, so the iterate returned is the one
+// after the sweep whose input met the rule (one sweep past it).
+//
+// Junction rows: the PMJ conductance is orders of magnitude above the cable's
+// own (c_pmj/Ai_P ~ 100/ms against ~0.1/ms), so a plain Jacobi sweep moves a
+// junction node and its patch against each other with a contraction of ~0.5
+// per sweep.  The finish kernel therefore re-solves each junction node's row
+// after the sweep from the patch voxels' new values (and its cable
+// neighbours' old ones): a Gauss-Seidel step on the stiff coupling, which
+// halves the sweeps a step needs on trees with junctions, for no extra
+// launch.
+#include "brick_stencil.h"
+#include "ttp06_cell.h"
+
+namespace hp {
+
+constexpr int CN_MAX_BLOCKS = 8192;   // block partials: part >= 2*CN_MAX_BLOCKS doubles
+constexpr int CN_MAX_CABLE_BLOCKS = 512;
+constexpr int CN_FIN_THREADS = 256;
+
+// scalars[]: 0 ||b||^2, 1 ||r||^2 of the last sweep, 2 done, 3 sweeps this step,
+// 4 steps that stopped at max_sweeps, 5 largest ||r||/||b|| of those, 6 total
+// sweeps, 7 steps solved
+enum { SC_BB = 0, SC_RR, SC_DONE, SC_SWEEPS, SC_NCONV, SC_WORST, SC_TOTAL, SC_STEPS };
+
+__device__ __forceinline__ double cn_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sums of (a, b) -> part[blockIdx], part[G + blockIdx] (NW waves per block).
+template <int NW>
+__device__ __forceinline__ void cn_block_partials(double a, double b, double* part) {
+  __shared__ double sh[2][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  a = cn_wave_sum(a);
+  b = cn_wave_sum(b);
+  if (lane == 0) {
+    sh[0][wave] = a;
+    sh[1][wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      sa += sh[0][w];
+      sb += sh[1][w];
+    }
+    part[blockIdx.x] = sa;
+    part[gridDim.x + blockIdx.x] = sb;
+  }
+}
+
+// g_Ks and g_to of a myocardial slot (phenotype, instance scales)
+template <typename T>
+__device__ __forceinline__ void cell_conductances(const hp_bricks& g, const hp_mono_params& p,
+                                                  uint32_t slot, int ct, T& gks, T& gto) {
+  gks = T(kGKs[ct]);
+  gto = T(kGto[ct]);
+  if (p.inst_params && g.inst) {
+    const T* ip = (const T*)p.inst_params + 4 * (int)g.inst[slot];
+    gks *= ip[0];
+    gto *= ip[1];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reaction: V* for every myocardial slot (one lane per slot, 256 per block, so
+// each wave holds the 64 slots of one state group) and, in the first ncb
+// blocks, for every cable node.  No clamp: the reference clamps after the
+// diffusion solve only (:1791).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) cn_react_kernel(
+    hp_bricks g, const T* __restrict__ V, T* __restrict__ Vs, T* __restrict__ state, int64_t ld,
+    hp_mono_params p, hp_cable c, const T* __restrict__ Vpk, T* __restrict__ Vspk,
+    T* __restrict__ state_pk, int64_t ld_pk, hp_mono_params p_pk, uint32_t ncb) {
+  constexpr bool exact = sizeof(T) == 8;
+  if (blockIdx.x < ncb) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= c.n) return;
+    const uint8_t lab = c.label[i];
+    const int ct = lab & 3;
+    const uint32_t row_bytes = (uint32_t)ld_pk * (uint32_t)sizeof(T);
+    T s[N_STATE];
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k)
+      s[k] = buf_ld<T>(buf_rsrc(state_pk + k * (size_t)ld_pk, row_bytes), (uint32_t)i);
+    const T Vc = Vpk[i];
+    const T dt = T(p_pk.dt);
+    const T I = ttp06_cell<T, exact>(Vc, s, dt, T(kGKs[ct]), T(kGto[ct]), ct != 2,
+                                     T(p_pk.ki_base), T(p_pk.nai_base));
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k)
+      buf_st(buf_rsrc(state_pk + k * (size_t)ld_pk, row_bytes), (uint32_t)i, s[k]);
+    const int grp = (lab >> 2) & 31;
+    const T stim = grp ? ((const T*)p_pk.group_amp)[grp] : T(0);
+    Vspk[i] = Vc - dt * I + dt * T(p_pk.inv_cm) * stim;
+    return;
+  }
+  const uint32_t slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
+  const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
+  if (slot >= n_slots) return;
+  const uint32_t sidx = state_index_wave(g.groups, slot >> 6);
+  const uint8_t lab = g.label[slot];
+  if (lab == 0xFF) return;
+  const uint32_t row_bytes = (uint32_t)ld * (uint32_t)sizeof(T);
+  T s[N_STATE];
+#pragma unroll
+  for (int k = 0; k < N_STATE; ++k)
+    s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
+  const int ct = lab & 3;
+  T gks, gto;
+  cell_conductances<T>(g, p, slot, ct, gks, gto);
+  const T Vc = ld32(V, slot);
+  const T dt = T(p.dt);
+  const T I = ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
+#pragma unroll
+  for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx, s[k]);
+  int grp = (lab >> 2) & 31;
+  if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
+  const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
+  st32(Vs, slot, Vc - dt * I + dt * T(p.inv_cm) * stim);
+}
+
+// Index of `slot` in the ascending target list (the slot is known to be there).
+__device__ __forceinline__ int cn_find_target(const int32_t* __restrict__ tgt, int n, int slot) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (tgt[mid] < slot) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// One Jacobi update of a row: r = b - A x (first sweep: b is formed, and
+// r = L x exactly), x_new = x + r / (Cm/dt - theta*L_ii).  lx = (L x)_i,
+// ld = -L_ii.  Accumulates r^2 (and b^2) into rr (bb).
+template <typename T, bool kFirst>
+__device__ __forceinline__ T cn_row(T xc, T lx, T ld, T& bv, const hp_cn_params& cp, double& rr,
+                                    double& bb) {
+  const T cm_dt = T(cp.cm_dt), th = T(cp.theta);
+  T r;
+  if (kFirst) {
+    bv = cm_dt * xc + (T(1) - th) * lx;
+    r = lx;
+    bb += (double)bv * (double)bv;
+  } else {
+    r = bv - cm_dt * xc + th * lx;
+  }
+  rr += (double)r * (double)r;
+  return xc + r / (cm_dt + th * ld);
+}
+
+// One sweep over every myocardial row (bricks, persistent: each XCD takes a
+// contiguous run of bricks, its blocks round robin) and, in the first ncb
+// blocks, every cable row.
+template <typename T, int BX, int BY, int BZ, bool kFirst>
+__global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
+    hp_bricks g, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ bvec,
+    hp_cable c, const T* __restrict__ xpk_in, T* __restrict__ xpk_out, T* __restrict__ bpk,
+    hp_cn_pmj j, hp_cn_params cp, uint32_t ncb) {
+  constexpr int BVOL = BX * BY * BZ;
+  typedef Tile<T, BX, BY, BZ> TT;
+  __shared__ TT t;
+  if (!kFirst && cp.scalars[SC_DONE] != 0.0) return;  // block-uniform
+  double rr = 0.0, bb = 0.0;
+  const int tid = threadIdx.x;
+  if (blockIdx.x < ncb) {
+    const T dsw = T(cp.dscale_pk * c.w);
+    for (int64_t i = (int64_t)blockIdx.x * BVOL + tid; i < c.n; i += (int64_t)ncb * BVOL) {
+      const T xc = xpk_in[i];
+      T acc = T(0);
+      const int e0 = c.indptr[i], e1 = c.indptr[i + 1];
+      for (int e = e0; e < e1; ++e) acc += xpk_in[c.indices[e]] - xc;
+      T lx = dsw * acc, ldg = dsw * T(e1 - e0);
+      if (j.node_ptr) {
+        for (int e = j.node_ptr[i]; e < j.node_ptr[i + 1]; ++e) {
+          const T w = T(j.node_w[e]);
+          lx += w * (ld32(x_in, (uint32_t)j.node_slot[e]) - xc);
+          ldg += w;
+        }
+      }
+      T bv = kFirst ? T(0) : bpk[i];
+      xpk_out[i] = cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb);
+      if (kFirst) bpk[i] = bv;
+    }
+  } else {
+    const uint32_t bid = blockIdx.x - ncb;
+    const uint32_t NB = (uint32_t)g.n_bricks;
+    const uint32_t xcd = bid & 7, G = gridDim.x - ncb;
+    const uint32_t m = (G - xcd + 7) >> 3;  // blocks with index = xcd (mod 8)
+    const uint32_t base = NB >> 3, rem = NB & 7;
+    const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
+    const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
+    const int lz = tid % BZ, ly = (tid / BZ) % BY, lx_ = tid / (BY * BZ);
+    const int cc = ((lx_ + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
+    const T dscale = T(cp.dscale);
+    for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
+      const uint32_t slot = b * BVOL + tid;
+      const uint8_t mylab = g.label[slot];
+      __syncthreads();  // every lane is done with the previous tile
+      stage_cells<T, BX, BY, BZ>(t, g, x_in, b, mylab);
+      if (mylab == 0xFF) continue;
+      const T xc = t.V[cc];
+      T ws;
+      const T acc = stencil19_wsum<T, BX, BY, BZ>(t, cc, ws);
+      T lx = dscale * acc, ldg = dscale * ws;
+      if ((mylab & 0x80) && j.n_targets > 0) {  // PMJ target voxel: its junction couplings
+        const int q = cn_find_target(j.tgt_slot, (int)j.n_targets, (int)slot);
+        const int e1 = j.tgt_slot[q] == (int)slot ? j.tgt_ptr[q + 1] : 0;  // listed slots only
+        for (int e = j.tgt_ptr[q]; e < e1; ++e) {
+          const T w = T(j.tgt_w[e]);
+          lx += w * (xpk_in[j.tgt_node[e]] - xc);
+          ldg += w;
+        }
+      }
+      T bv = kFirst ? T(0) : ld32((const T*)bvec, slot);
+      st32(x_out, slot, cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb));
+      if (kFirst) st32(bvec, slot, bv);
+    }
+  }
+  cn_block_partials<(BVOL + 63) / 64>(rr, bb, cp.part);
+}
+
+// Sums the partials of the sweep before it (fixed block order) and applies
+// the stopping rule; re-solves the junction nodes' rows (see the top of the
+// file): x_p = (b_p + theta*(w_pp/Ai_P * sum of its cable neighbours' input
+// values + sum_m w_pm * x_m new)) / (Cm/dt + theta * (-L_pp)).  Returns at
+// once when the step's solve was already done before the sweep.
+template <typename T, bool kFirst>
+__global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
+    hp_cn_params cp, int nb, hp_cable c, hp_cn_pmj j, const T* __restrict__ xpk_in,
+    T* __restrict__ xpk_out, const T* __restrict__ x_out, const T* __restrict__ bpk) {
+  double* sc = cp.scalars;
+  if (!kFirst && sc[SC_DONE] != 0.0) return;
+  if (j.n_jnodes > 0) {
+    const T cm_dt = T(cp.cm_dt), th = T(cp.theta), dsw = T(cp.dscale_pk * c.w);
+    for (int64_t q = threadIdx.x; q < j.n_jnodes; q += CN_FIN_THREADS) {
+      const int pn = j.jnode[q];
+      T nb_sum = T(0);
+      const int e0 = c.indptr[pn], e1 = c.indptr[pn + 1];
+      for (int e = e0; e < e1; ++e) nb_sum += xpk_in[c.indices[e]];
+      T m_sum = T(0), wsum = T(0);
+      for (int e = j.node_ptr[pn]; e < j.node_ptr[pn + 1]; ++e) {
+        const T w = T(j.node_w[e]);
+        m_sum += w * ld32(x_out, (uint32_t)j.node_slot[e]);
+        wsum += w;
+      }
+      const T diag = cm_dt + th * (dsw * T(e1 - e0) + wsum);
+      xpk_out[pn] = (bpk[pn] + th * (dsw * nb_sum + m_sum)) / diag;
+    }
+  }
+  __shared__ double sh[2][CN_FIN_THREADS / 64];
+  double rr = 0.0, bb = 0.0;
+  for (int b = threadIdx.x; b < nb; b += CN_FIN_THREADS) {
+    rr += cp.part[b];
+    bb += cp.part[nb + b];
+  }
+  // per-thread strided sums, then a fixed-order combination: deterministic
+  rr = cn_wave_sum(rr);
+  bb = cn_wave_sum(bb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    sh[0][wave] = rr;
+    sh[1][wave] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  rr = 0.0;
+  bb = 0.0;
+  for (int w = 0; w < CN_FIN_THREADS / 64; ++w) {
+    rr += sh[0][w];
+    bb += sh[1][w];
+  }
+  if (kFirst) sc[SC_BB] = bb;
+  else bb = sc[SC_BB];
+  const double sweeps = kFirst ? 1.0 : sc[SC_SWEEPS] + 1.0;
+  sc[SC_RR] = rr;
+  sc[SC_SWEEPS] = sweeps;
+  const bool conv = rr <= cp.tol * cp.tol * bb;  // also when ||b|| = 0 (then r = 0)
+  const bool done = conv || sweeps >= (double)cp.max_sweeps;
+  sc[SC_DONE] = done ? 1.0 : 0.0;
+  if (done) {
+    sc[SC_TOTAL] += sweeps;
+    sc[SC_STEPS] += 1.0;
+    if (!conv) {
+      sc[SC_NCONV] += 1.0;
+      const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+      if (rel > sc[SC_WORST]) sc[SC_WORST] = rel;
+    }
+  }
+}
+
+// V^{n+1} = clamp(x) from the buffer of the last sweep (odd sweep count: x,
+// even: V_out itself), first activation between V^n and V^{n+1} (every
+// myocardial slot, junction voxels included: the PMJ coupling is part of the
+// solve); cable nodes in the first ncb blocks.
+template <typename T>
+__global__ void __launch_bounds__(256) cn_final_kernel(
+    hp_bricks g, const T* __restrict__ V_in, const T* __restrict__ x, T* __restrict__ V_out,
+    const T* __restrict__ xpk, T* __restrict__ Vpk_out, int64_t n_cable, hp_mono_params p,
+    const double* __restrict__ sc, uint32_t ncb) {
+  const bool odd = ((int64_t)sc[SC_SWEEPS] & 1) != 0;
+  const T vmin = T(p.vmin), vmax = T(p.vmax);
+  if (blockIdx.x < ncb) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n_cable) {
+      const T v = odd ? xpk[i] : Vpk_out[i];
+      Vpk_out[i] = M<T>::mn(M<T>::mx(v, vmin), vmax);
+    }
+    return;
+  }
+  const uint32_t slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
+  const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
+  if (slot >= n_slots || g.label[slot] == 0xFF) return;
+  const T v = odd ? ld32(x, slot) : ld32((const T*)V_out, slot);
+  const T Vn = M<T>::mn(M<T>::mx(v, vmin), vmax);
+  st32(V_out, slot, Vn);
+  if (p.at) {
+    const T Vc = ld32(V_in, slot), thr = T(p.at_thr);
+    if (Vc < thr && Vn >= thr) {
+      T* at = (T*)p.at;
+      if (ld32((const T*)at, slot) != ld32((const T*)at, slot)) {
+        const int64_t step = *p.step_ctr;
+        st32(at, slot, (T)((double)(step - 1) * p.dt) + T(p.dt) * (thr - Vc) / (Vn - Vc));
+      }
+    }
+  }
+}
+
+static int cn_compute_units() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+template <typename T, int BX, int BY, int BZ>
+static void launch_sweep(bool first, dim3 grid, hipStream_t st, const hp_bricks& g, const T* xi,
+                         T* xo, T* bvec, const hp_cable& c, const T* xpi, T* xpo, T* bpk,
+                         const hp_cn_pmj& j, const hp_cn_params& cp, uint32_t ncb) {
+  if (first)
+    hipLaunchKernelGGL((cn_sweep_kernel<T, BX, BY, BZ, true>), grid, dim3(BX * BY * BZ), 0, st, g,
+                       xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb);
+  else
+    hipLaunchKernelGGL((cn_sweep_kernel<T, BX, BY, BZ, false>), grid, dim3(BX * BY * BZ), 0, st,
+                       g, xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb);
+}
+
+template <typename T>
+static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_t ld,
+                   const hp_mono_params* p, const hp_cable* c, const T* Vpk_in, T* Vpk_out,
+                   T* state_pk, int64_t ld_pk, const hp_mono_params* p_pk, const hp_cn_pmj* j,
+                   const hp_cn_params* cp, const hp_cn_work* w, void* stream) {
+  const char* what = "hp_cn_step";
+  if (!g || !p || !cp || !w || !p->step_ctr || !p->group_amp || p->do_reaction != 1 ||
+      !g->groups || !g->label || !g->D6 || !g->nbr || !state || !V_in || !V_out || ld < 0 ||
+      (uint64_t)ld * sizeof(T) >= (1ull << 32)) {
+    set_error("%s: brick grid (groups, label, D6, nbr), V_in, V_out, state, params (step_ctr, "
+              "group_amp, do_reaction = 1), CN params and work vectors are required",
+              what);
+    return -1;
+  }
+  if (!cp->scalars || !cp->part || !w->Vs || !w->b || !w->x || cp->max_sweeps < 1 ||
+      !(cp->cm_dt > 0.0) || !(cp->theta >= 0.0 && cp->theta <= 1.0) || !(cp->tol >= 0.0)) {
+    set_error("%s: CN params need scalars, part, Cm/dt > 0, 0 <= theta <= 1, tol >= 0, "
+              "max_sweeps >= 1, and the work vectors Vs, b, x",
+              what);
+    return -1;
+  }
+  const int64_t ncab = c ? c->n : 0;
+  if (ncab < 0 || (ncab > 0 && (!p_pk || !p_pk->group_amp || !Vpk_in || !Vpk_out || !state_pk ||
+                                ld_pk < ncab || (uint64_t)ld_pk * sizeof(T) >= (1ull << 32) ||
+                                !c->indptr || !c->indices || !c->label || !w->Vs_pk ||
+                                !w->b_pk || !w->x_pk))) {
+    set_error("%s: a cable needs its params, V, state (ld_pk >= n), CSR, labels and the work "
+              "vectors Vs_pk, b_pk, x_pk",
+              what);
+    return -1;
+  }
+  const hp_cn_pmj j0{};
+  if (!j) j = &j0;
+  if (j->n_targets < 0 || (j->n_targets > 0 && (!j->tgt_slot || !j->tgt_ptr || !j->tgt_node ||
+                                                !j->tgt_w || ncab == 0)) ||
+      (j->node_ptr && (!j->node_slot || !j->node_w)) || j->n_jnodes < 0 ||
+      (j->n_jnodes > 0 && (!j->jnode || !j->node_ptr))) {
+    set_error("%s: PMJ rows need tgt_slot/tgt_ptr/tgt_node/tgt_w (and a cable), node_slot and "
+              "node_w with node_ptr, and node_ptr with the junction node list",
+              what);
+    return -1;
+  }
+  const int64_t bvol = (int64_t)g->bx * g->by * g->bz;
+  if (g->n_bricks < 0 || (uint64_t)g->n_bricks * bvol * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("%s: %lld bricks exceed the 32-bit offset range of one launch", what,
+              (long long)g->n_bricks);
+    return -1;
+  }
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  if (key != 444 && key != 844 && key != 884 && key != 888) {
+    set_error("%s: unsupported brick shape %dx%dx%d", what, g->bx, g->by, g->bz);
+    return -1;
+  }
+  const hp_cable c0{};
+  const hp_mono_params p0{};
+  if (!c) c = &c0;
+  if (!p_pk) p_pk = &p0;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n_slots = g->n_bricks * bvol;
+  T* Vs = (T*)w->Vs;
+  T* bvec = (T*)w->b;
+  T* x = (T*)w->x;
+  T* Vspk = (T*)w->Vs_pk;
+  T* bpk = (T*)w->b_pk;
+  T* xpk = (T*)w->x_pk;
+
+  // reaction
+  const uint32_t ncb_r = (uint32_t)ceil_div(ncab, 256);
+  const int64_t nblk_r = ncb_r + ceil_div(n_slots, 256);
+  if (nblk_r > 0) {
+    hipLaunchKernelGGL(cn_react_kernel<T>, dim3((unsigned)nblk_r), dim3(256), 0, st, *g, V_in, Vs,
+                       state, ld, *p, *c, Vpk_in, Vspk, state_pk, ld_pk, *p_pk, ncb_r);
+    if (int rc = check_launch("hp_cn_step/react")) return rc;
+  }
+  // sweeps: x_0 = V*, sweep k writes x (k odd) or V_out (k even)
+  int64_t ncb = ceil_div(ncab, bvol);
+  if (ncb > CN_MAX_CABLE_BLOCKS) ncb = CN_MAX_CABLE_BLOCKS;
+  ncb = (ncb + 7) / 8 * 8;
+  int64_t nbb = (int64_t)cn_compute_units() * 16 * 64 / bvol;
+  if (nbb > g->n_bricks) nbb = g->n_bricks;
+  const int64_t G = ncb + nbb;
+  if (G > CN_MAX_BLOCKS) {
+    set_error("%s: %lld blocks exceed the partial buffer", what, (long long)G);
+    return -1;
+  }
+  for (int k = 1; k <= cp->max_sweeps; ++k) {
+    const T* xi = k == 1 ? Vs : (k % 2 == 0 ? x : V_out);
+    T* xo = k % 2 == 1 ? x : V_out;
+    const T* xpi = k == 1 ? Vspk : (k % 2 == 0 ? xpk : Vpk_out);
+    T* xpo = k % 2 == 1 ? xpk : Vpk_out;
+    if (G > 0) {
+      switch (key) {
+        case 444: launch_sweep<T, 4, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
+        case 844: launch_sweep<T, 8, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
+        case 884: launch_sweep<T, 8, 8, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
+        default:  launch_sweep<T, 8, 8, 8>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
+      }
+      if (int rc = check_launch("hp_cn_step/sweep")) return rc;
+    }
+    if (k == 1)
+      hipLaunchKernelGGL((cn_finish_kernel<T, true>), dim3(1), dim3(CN_FIN_THREADS), 0, st, *cp,
+                         (int)G, *c, *j, xpi, xpo, xo, bpk);
+    else
+      hipLaunchKernelGGL((cn_finish_kernel<T, false>), dim3(1), dim3(CN_FIN_THREADS), 0, st, *cp,
+                         (int)G, *c, *j, xpi, xpo, xo, bpk);
+    if (int rc = check_launch("hp_cn_step/finish")) return rc;
+  }
+  // clamp + activation times
+  const uint32_t ncb_f = (uint32_t)ceil_div(ncab, 256);
+  const int64_t nblk_f = ncb_f + ceil_div(n_slots, 256);
+  if (nblk_f > 0)
+    hipLaunchKernelGGL(cn_final_kernel<T>, dim3((unsigned)nblk_f), dim3(256), 0, st, *g, V_in, x,
+                       V_out, xpk, Vpk_out, ncab, *p, cp->scalars, ncb_f);
+  return check_launch("hp_cn_step/final");
+}
+
+}  // namespace hp
+
+using namespace hp;
+
+extern "C" {
+int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state, int64_t ld,
+                   const hp_mono_params* p, const hp_cable* c, const float* Vpk_in, float* Vpk_out,
+                   float* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
+                   const hp_cn_pmj* j, const hp_cn_params* cp, const hp_cn_work* w,
+                   void* stream) {
+  return cn_step<float>(g, V_in, V_out, state, ld, p, c, Vpk_in, Vpk_out, state_pk, ld_pk, p_pk,
+                        j, cp, w, stream);
+}
+int hp_cn_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                   int64_t ld, const hp_mono_params* p, const hp_cable* c, const double* Vpk_in,
+                   double* Vpk_out, double* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
+                   const hp_cn_pmj* j, const hp_cn_params* cp, const hp_cn_work* w,
+                   void* stream) {
+  return cn_step<double>(g, V_in, V_out, state, ld, p, c, Vpk_in, Vpk_out, state_pk, ld_pk, p_pk,
+                         j, cp, w, stream);
+}
+}

@@ -5,325 +5,21 @@
 // Scheme (one call per 0.02 ms step, all terms evaluated at V^n):
 //   V^{n+1} = clamp(V^n - dt*I_ion(V^n, s^n) + (dt/Cm)*(I_stim + (L V^n)), -150, 80)
 //   s^{n+1} = TTP06 Rush-Larsen / forward-Euler update from (V^n, s^n)
-// followed by the exact PMJ exchange on V^{n+1}.  The reference instead
-// solves the diffusion part with Crank-Nicolson + BiCGSTAB
-// (purkinje_myocardium_anisotropic_solver.py:1787-1791); at dt = 0.02 ms the
-// explicit operator is stable (dt*|diag L|/Cm < 0.2 for the shipped
-// conductivities) and the two schemes agree to within the activation-time
-// and Vm tolerances that the tests check.
+// followed by the exact PMJ exchange on V^{n+1}.  This is the fast form; the
+// reference's own scheme (Crank-Nicolson diffusion after the reaction,
+// purkinje_myocardium_anisotropic_solver.py:1746-1791) is hp_cn_step in
+// cn.hip.  At dt = 0.02 ms the explicit operator is stable (dt*|diag L|/Cm <
+// 0.2 for the shipped conductivities); it differs from the CN scheme by the
+// schemes' truncation error (upstroke timing).
 //
 // Memory plan per workgroup (one brick of BX*BY*BZ voxels, one lane each):
 // the (BX+2)(BY+2)(BZ+2) halo tile of V, label and the six D components is
 // staged in LDS once; each lane then reads its 18 neighbours from LDS and
 // streams its own 18 state variables from HBM (coalesced SoA).
-#include "common.h"
-#include "hrdaya_hip.h"
+#include "brick_stencil.h"
 #include "ttp06_cell.h"
 
 namespace hp {
-
-__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-  // contiguous run of bricks so that halo re-reads stay in its own L2.
-  const int xcd = bid & 7, k = bid >> 3;
-  const int base = nblocks >> 3, rem = nblocks & 7;
-  return xcd * base + (xcd < rem ? xcd : rem) + k;
-}
-
-template <typename T, int BX, int BY, int BZ>
-struct Tile {
-  static constexpr int TX = BX + 2, TY = BY + 2, TZ = BZ + 2;
-  static constexpr int N = TX * TY * TZ;
-  static constexpr int SX = TY * TZ, SY = TZ;
-  static constexpr int BVOL = BX * BY * BZ;
-  // one spare cell at index N takes the writes of lanes without a tile cell
-  T V[N + 1];
-  T D[6][N + 1];
-  uint8_t L[N + 1];
-};
-
-// Cells of the halo tile that the 19-point stencil reads, enumerated so that
-// consecutive lanes read consecutive slots of one neighbour brick:
-//   [0, NI)           the brick itself (lane = own slot), all six D components;
-//   then the -x/+x face layers (BY*BZ cells each, contiguous slots), the -y/+y
-//   layers (runs of BZ), the -z/+z layers (stride BZ): three D components each
-//   (the face-axis diagonal and the two off-diagonals of the planes through the
-//   face); then the four z-, y- and x-parallel edge lines of the tile: one
-//   off-diagonal each.  Tile corners are never read.
-template <int BX, int BY, int BZ>
-struct HaloPlan {
-  static constexpr int NI = BX * BY * BZ;
-  static constexpr int E0 = NI, E1 = E0 + BY * BZ, E2 = E1 + BY * BZ, E3 = E2 + BX * BZ,
-                       E4 = E3 + BX * BZ, E5 = E4 + BX * BY, E6 = E5 + BX * BY,
-                       E7 = E6 + 4 * BZ, E8 = E7 + 4 * BY, NT = E8 + 4 * BX;
-  static constexpr int NIT = (NT + NI - 1) / NI;
-};
-
-struct HaloCell {
-  int tile;   // index into the LDS tile, -1 = no cell for this lane
-  int dir;    // neighbour-table entry of the brick holding the cell
-  int local;  // slot of the cell inside that brick
-  int kind;   // 0 none, 1 own brick (six D components), 2 face layer (three), 3 edge line (one)
-  int c0;     // first D component of a face layer / the component of an edge line
-};
-
-template <int BX, int BY, int BZ>
-constexpr HaloCell halo_cell(int e) {
-  typedef HaloPlan<BX, BY, BZ> H;
-  constexpr int TY = BY + 2, TZ = BZ + 2;
-  HaloCell h{-1, 13, 0, 0, 0};
-  int gx = 0, gy = 0, gz = 0, dx = 0, dy = 0, dz = 0, lx = 0, ly = 0, lz = 0;
-  h.kind = 2;
-  if (e < H::E0) {  // own brick
-    lx = e / (BY * BZ); ly = (e / BZ) % BY; lz = e % BZ;
-    gx = lx; gy = ly; gz = lz;
-    h.kind = 1;
-  } else if (e < H::E2) {  // x layers: D00, D01, D02
-    const int q = e - H::E0, sd = q / (BY * BZ), r = q % (BY * BZ);
-    ly = r / BZ; lz = r % BZ; dx = sd ? 1 : -1; lx = sd ? 0 : BX - 1;
-    gx = sd ? BX : -1; gy = ly; gz = lz;
-    h.c0 = 0;
-  } else if (e < H::E4) {  // y layers: D11, D01, D12
-    const int q = e - H::E2, sd = q / (BX * BZ), r = q % (BX * BZ);
-    lx = r / BZ; lz = r % BZ; dy = sd ? 1 : -1; ly = sd ? 0 : BY - 1;
-    gx = lx; gy = sd ? BY : -1; gz = lz;
-    h.c0 = 1;
-  } else if (e < H::E6) {  // z layers: D22, D02, D12
-    const int q = e - H::E4, sd = q / (BX * BY), r = q % (BX * BY);
-    lx = r / BY; ly = r % BY; dz = sd ? 1 : -1; lz = sd ? 0 : BZ - 1;
-    gx = lx; gy = ly; gz = sd ? BZ : -1;
-    h.c0 = 2;
-  } else if (e < H::E7) {  // z-parallel edges: D01
-    const int q = e - H::E6, ed = q / BZ;
-    const int sx = ed >> 1, sy = ed & 1;
-    lz = q % BZ; dx = sx ? 1 : -1; dy = sy ? 1 : -1;
-    lx = sx ? 0 : BX - 1; ly = sy ? 0 : BY - 1;
-    gx = sx ? BX : -1; gy = sy ? BY : -1; gz = lz;
-    h.kind = 3; h.c0 = 3;
-  } else if (e < H::E8) {  // y-parallel edges: D02
-    const int q = e - H::E7, ed = q / BY;
-    const int sx = ed >> 1, sz = ed & 1;
-    ly = q % BY; dx = sx ? 1 : -1; dz = sz ? 1 : -1;
-    lx = sx ? 0 : BX - 1; lz = sz ? 0 : BZ - 1;
-    gx = sx ? BX : -1; gy = ly; gz = sz ? BZ : -1;
-    h.kind = 3; h.c0 = 4;
-  } else if (e < H::NT) {  // x-parallel edges: D12
-    const int q = e - H::E8, ed = q / BX;
-    const int sy = ed >> 1, sz = ed & 1;
-    lx = q % BX; dy = sy ? 1 : -1; dz = sz ? 1 : -1;
-    ly = sy ? 0 : BY - 1; lz = sz ? 0 : BZ - 1;
-    gx = lx; gy = sy ? BY : -1; gz = sz ? BZ : -1;
-    h.kind = 3; h.c0 = 5;
-  } else {
-    h.kind = 0;
-    return h;
-  }
-  h.tile = ((gx + 1) * TY + (gy + 1)) * TZ + (gz + 1);
-  h.dir = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
-  h.local = (lx * BY + ly) * BZ + lz;
-  return h;
-}
-
-// The staging plan of every (iteration, lane), packed as
-// tile | dir << 10 | local << 15 | kind << 24 | c0 << 26, built at compile time
-// and read from constant memory: the kernels decode one word per iteration
-// instead of evaluating the case analysis above.
-template <int BX, int BY, int BZ>
-struct PlanTable {
-  static constexpr int N = HaloPlan<BX, BY, BZ>::NIT * BX * BY * BZ;
-  uint32_t w[N];
-  constexpr PlanTable() : w() {
-    for (int e = 0; e < N; ++e) {
-      const HaloCell h = halo_cell<BX, BY, BZ>(e);
-      w[e] = h.kind == 0 ? 0u
-                         : (uint32_t)h.tile | ((uint32_t)h.dir << 10) | ((uint32_t)h.local << 15) |
-                               ((uint32_t)h.kind << 24) | ((uint32_t)h.c0 << 26);
-    }
-  }
-};
-template <int BX, int BY, int BZ>
-__constant__ PlanTable<BX, BY, BZ> kPlan = PlanTable<BX, BY, BZ>();
-
-// Element idx of a uniform base pointer, addressed as base + 32-bit byte
-// offset (the global "saddr + voffset" form: no 64-bit address arithmetic per
-// lane).  Callers keep idx * sizeof(T) below 2^32 (checked at launch).
-template <typename T>
-__device__ __forceinline__ T ld32(const T* base, uint32_t idx) {
-  return *(const T*)((const char*)base + (uint32_t)(idx * (uint32_t)sizeof(T)));
-}
-template <typename T>
-__device__ __forceinline__ void st32(T* base, uint32_t idx, T v) {
-  *(T*)((char*)base + (uint32_t)(idx * (uint32_t)sizeof(T))) = v;
-}
-
-// Halo staging, split into a load half (global -> registers) and a store half
-// (registers -> LDS) so that a persistent kernel can issue the loads of its
-// next brick before it computes the current one.  Per lane and plan
-// iteration: the plan word, the slot of the cell (OOB for cells without a
-// brick: their loads return 0 without memory traffic), label, V and up to
-// three D components.  The lane's own cell (iteration 0) loads all six D
-// components; padding voxels of the brick itself load nothing (mylab 0xFF).
-// Padding slots of neighbour bricks are read (they hold V0 and D = 0) and
-// masked by their label.
-constexpr uint32_t OOB = 0x1FFFFFFFu;  // element index past every bound
-constexpr int MAX_GROUPS = 8192;       // stimulus groups (group_amp entries) at most
-
-template <int BX, int BY, int BZ>
-struct HaloNbr {
-  static constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
-  uint32_t pw[NIT];  // plan words of this lane
-  int nb[NIT];       // brick holding each cell, -1 = none (iteration 0: the brick itself)
-  bool valid;        // false: no brick (its loads read out of range)
-};
-
-// The lane's plan words and the neighbour-table entries they name (read from
-// the brick's 27-entry row; one cache line per brick).  Nothing here consumes
-// a loaded value, so a persistent kernel can issue these loads early and use
-// them an iteration later; valid = false issues out-of-range reads only.
-template <int BX, int BY, int BZ>
-__device__ __forceinline__ void halo_nbr(HaloNbr<BX, BY, BZ>& n, const hp_bricks& g, uint32_t b,
-                                         bool valid = true) {
-  constexpr int BVOL = BX * BY * BZ;
-  const buf_t rN = buf_rsrc(g.nbr, (uint32_t)g.n_bricks * 27u * 4u);
-  n.valid = valid;
-#pragma unroll
-  for (int it = 0; it < HaloNbr<BX, BY, BZ>::NIT; ++it) {
-    n.pw[it] = kPlan<BX, BY, BZ>.w[it * BVOL + threadIdx.x];
-    const int kind = (n.pw[it] >> 24) & 3;
-    n.nb[it] = it == 0 ? (int)b
-                       : (kind ? __builtin_bit_cast(int, buf_ld<float>(
-                                     rN, valid ? b * 27u + ((n.pw[it] >> 10) & 31) : OOB))
-                               : -1);
-  }
-}
-
-template <typename T, int BX, int BY, int BZ>
-struct HaloRegs {
-  static constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
-  uint32_t pw[NIT], q[NIT], lab[NIT];
-  T v[NIT], d[NIT][3], dself[3];
-};
-
-// mylab: the lane's own label in brick n (0xFF: padding voxel, loads nothing)
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const HaloNbr<BX, BY, BZ>& n,
-                                          const hp_bricks& g, const T* __restrict__ V,
-                                          uint8_t mylab) {
-  constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT, BVOL = BX * BY * BZ;
-  const uint32_t n_slots = (uint32_t)g.n_bricks * BVOL, ld = n_slots;  // D6 row stride
-  const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
-  const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
-  const buf_t rL = buf_rsrc(g.label, n_slots);
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    h.pw[it] = n.pw[it];
-    const bool ok = n.valid && (it == 0 ? mylab != 0xFF : n.nb[it] >= 0);
-    h.q[it] = ok ? (uint32_t)n.nb[it] * BVOL + ((n.pw[it] >> 15) & 511) : OOB;
-  }
-#pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    const uint32_t q = h.q[it];
-    h.lab[it] = buf_ld_u8(rL, q);
-    h.v[it] = buf_ld<T>(rV, q);
-    if (it == 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) h.d[0][k] = buf_ld<T>(rD, q == OOB ? OOB : k * ld + q);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) h.dself[k] = buf_ld<T>(rD, q == OOB ? OOB : (3 + k) * ld + q);
-    } else {
-      const uint32_t c0 = (h.pw[it] >> 26) & 7;
-      const bool face = ((h.pw[it] >> 24) & 3) == 2 && q != OOB;
-      const uint32_t c1 = c0 == 2 ? 4u : 3u, c2 = c0 == 0 ? 4u : 5u;
-      h.d[it][0] = buf_ld<T>(rD, q == OOB ? OOB : c0 * ld + q);
-      h.d[it][1] = buf_ld<T>(rD, face ? c1 * ld + q : OOB);  // face layers: the two
-      h.d[it][2] = buf_ld<T>(rD, face ? c2 * ld + q : OOB);  // off-diagonals of its planes
-    }
-  }
-}
-
-// Registers -> LDS tile; lanes without a tile cell write the spare cell N.
-// The caller places the barriers.
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void halo_store(Tile<T, BX, BY, BZ>& t,
-                                           const HaloRegs<T, BX, BY, BZ>& h) {
-  typedef Tile<T, BX, BY, BZ> TT;
-#pragma unroll
-  for (int it = 0; it < HaloPlan<BX, BY, BZ>::NIT; ++it) {
-    const int kind = (h.pw[it] >> 24) & 3;
-    const int i = kind ? (int)(h.pw[it] & 1023) : TT::N;
-    t.L[i] = h.q[it] == OOB ? (uint8_t)0xFF : (uint8_t)h.lab[it];
-    t.V[i] = h.v[it];
-    if (it == 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t.D[k][i] = h.d[0][k];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) t.D[3 + k][i] = h.dself[k];
-    } else {
-      const int c0 = (h.pw[it] >> 26) & 7;
-      const int i12 = kind == 2 ? i : TT::N;
-      t.D[c0 == 2 ? 4 : 3][i12] = h.d[it][1];
-      t.D[c0 == 0 ? 4 : 5][i12] = h.d[it][2];
-      t.D[c0][i] = h.d[it][0];
-    }
-  }
-}
-
-// Whole staging of brick b for the one-brick-per-workgroup kernels.
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void stage_cells(Tile<T, BX, BY, BZ>& t, const hp_bricks& g,
-                                            const T* __restrict__ V, uint32_t b, uint8_t mylab) {
-  HaloNbr<BX, BY, BZ> n;
-  halo_nbr<BX, BY, BZ>(n, g, b);
-  HaloRegs<T, BX, BY, BZ> h;
-  halo_load<T, BX, BY, BZ>(h, n, g, V, mylab);
-  halo_store<T, BX, BY, BZ>(t, h);
-  __syncthreads();
-}
-
-// Sum over the 18 neighbours of w_ab*(V_b - V_a)/h, in units of D (mS/mm)*mV.
-// Terms of absent neighbours (label 0xFF) are skipped.
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
-  typedef Tile<T, BX, BY, BZ> TT;
-  const T Vc = t.V[c];
-  T acc = T(0);
-  // face neighbours: harmonic mean of the axis component (aniso.py:996-1004)
-  const int fo[3] = {TT::SX, TT::SY, 1};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const T Da = t.D[a][c];
-#pragma unroll
-    for (int sgn = -1; sgn <= 1; sgn += 2) {
-      const int nb = c + sgn * fo[a];
-      if (t.L[nb] != 0xFF) {
-        const T Db = t.D[a][nb];
-        const T s = Da + Db;
-        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
-        acc += dh * (t.V[nb] - Vc);
-      }
-    }
-  }
-  // edge-diagonal neighbours: +-0.5*mean(D_ij) (aniso.py:1006-1029); in
-  // units of D*mV/h this is +-0.25*(D_ij,a + D_ij,b)*(V_b - V_a)
-  const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    const T Da = t.D[3 + q][c];
-    const int o1 = po[q][0], o2 = po[q][1];
-    const int co[2] = {o1 + o2, -(o1 + o2)};
-    const int ct[2] = {o1 - o2, -(o1 - o2)};
-    T sum = T(0);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int nb = c + co[k];
-      if (t.L[nb] != 0xFF) sum += (Da + t.D[3 + q][nb]) * (t.V[nb] - Vc);
-      const int nm = c + ct[k];
-      if (t.L[nm] != 0xFF) sum -= (Da + t.D[3 + q][nm]) * (t.V[nm] - Vc);
-    }
-    acc += T(0.25) * sum;
-  }
-  return acc;
-}
 
 // do_reaction 1 (TTP06 every voxel) and 2 (TTP06 where rate_ext is NaN, else
 // dV/dt = rate_ext); do_reaction 0 runs diff_step_kernel instead.
@@ -786,11 +482,31 @@ __global__ void pmj_target_kernel(hp_pmj j, T* __restrict__ V_myo, const T* __re
 // ---------------------------------------------------------------------------
 // Stimulus schedule and step counter
 // ---------------------------------------------------------------------------
+// Floor division of doubles as Python's float `//` defines it (the reference
+// computes the pulse number with it): the remainder from fmod, the quotient of
+// the remainder-free dividend, rounded to the nearest integer when it lies
+// more than 0.5 above its floor.  Plain floor(a/b) can differ from it where
+// a/b rounds up to an integer.
+__device__ __forceinline__ double py_floordiv(double a, double b) {
+#pragma clang fp contract(off)
+  double mod = fmod(a, b);
+  double div = (a - mod) / b;
+  if (mod != 0.0 && ((b < 0.0) != (mod < 0.0))) div -= 1.0;
+  if (div == 0.0) return copysign(0.0, a / b);
+  double fl = floor(div);
+  if (div - fl > 0.5) fl += 1.0;
+  return fl;
+}
+
+// _stim_active (purkinje_myocardium_anisotropic_solver.py:1290-1301), each
+// operation rounded on its own as Python evaluates it (no fused multiply-add:
+// a phase equal to the pulse duration must compare as it does there)
 __device__ __forceinline__ bool site_active(const double* sp, double t) {
+#pragma clang fp contract(off)
   const double onset = sp[0], dur = sp[1], period = sp[2], npulse = sp[3];
   if (t < onset) return false;
   if (period <= 0.0) return t < onset + dur;
-  const double k = floor((t - onset) / period);
+  const double k = py_floordiv(t - onset, period);
   if (npulse >= 0.0 && k >= npulse) return false;
   return (t - onset) - k * period < dur;
 }

@@ -75,13 +75,22 @@ class MonodomainEngine:
     phenotype codes 0/1/2; ``cable`` dict(n, edges, w_pp, Ai_P, celltype);
     ``pmj`` dict(ids, patches, c_pmj) with patches in myocardial node ids;
     ``stim_sites`` list of dict(node_ids (global), amp, dur, onset, period, n_pulses).
+
+    ``scheme``: "explicit" (forward-Euler diffusion fused with the reaction,
+    exact PMJ exchange; stable below ``tissue.explicit_dt_limit``) or "cn"
+    (the reference's Crank-Nicolson step with ``theta``, solved by Jacobi
+    sweeps to ``cg_tol`` relative residual, at most ``max_sweeps`` per step;
+    purkinje_myocardium_anisotropic_solver.py:757-849, 1746-1791).
     """
 
     def __init__(self, layout: BrickLayout, D6, celltype_myo, *, h, A_M, Cm, dt,
                  dtype=torch.float32, device="cuda", cable=None, pmj=None, stim_sites=None,
                  at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None,
-                 alloc="torch"):
+                 alloc="torch", scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
         nat.load()
+        if scheme not in ("explicit", "cn"):
+            raise ValueError(f"unknown scheme {scheme!r} (explicit or cn)")
+        self.scheme = scheme
         self.lay = layout
         self.dtype = dtype
         self.dev = torch.device(device)
@@ -216,6 +225,10 @@ class MonodomainEngine:
             self.pmj_at = [nat.HpPmjAt(nat.ptr(self.at), nat.ptr(self.V[k]),
                                        nat.ptr(self.step_ctr), self.dt, float(at_thr))
                            for k in range(2)]
+        self.cn = None
+        if scheme == "cn":
+            self._setup_cn(L, pmj, float(h), float(A_M), float(theta), float(cg_tol),
+                           int(max_sweeps), cable)
         self.p = 0
         self.overlap_cable = True
         self.fuse_cable = True
@@ -223,6 +236,61 @@ class MonodomainEngine:
         self._eval_stim()
 
     # ------------------------------------------------------------------ setup
+    def _setup_cn(self, L, pmj, h, A_M, theta, tol, max_sweeps, cable):
+        """Work vectors, PMJ rows and device scalars of the Crank-Nicolson step."""
+        if not 0.0 <= theta <= 1.0:
+            raise ValueError(f"theta must lie in [0, 1] (got {theta})")
+        if not tol >= 0.0 or max_sweeps < 1:
+            raise ValueError("cg_tol must be >= 0 and max_sweeps >= 1")
+        dev, dt_ = self.dev, self.dtype
+        n, Np = L.n_slots, self.Np
+        z = lambda m: torch.zeros(max(m, 1), dtype=dt_, device=dev)
+        self.cn_work_t = [z(n), z(n), z(n), z(Np), z(Np), z(Np)]
+        self.cn_scalars = torch.zeros(16, dtype=torch.float64, device=dev)
+        self.cn_part = torch.zeros(16384, dtype=torch.float64, device=dev)
+        Ai_P = float(cable["Ai_P"]) if Np else 1.0
+        Ai_M = A_M * h ** 3
+        self.cn_pmj = None
+        if pmj and Np and pmj["c_pmj"] != 0.0 and len(pmj["ids"]):
+            # every (junction node, patch voxel) pair: conductance c_pmj/|patch|
+            # (purkinje_myocardium_anisotropic_solver.py:684-692)
+            node, slot, w = [], [], []
+            for q, patch in zip(pmj["ids"], pmj["patches"]):
+                patch = np.asarray(patch, dtype=np.int64)
+                node.append(np.full(len(patch), int(q), np.int64))
+                slot.append(L.slot_of_node[patch].astype(np.int64))
+                w.append(np.full(len(patch), float(pmj["c_pmj"]) / len(patch)))
+            node, slot, w = np.concatenate(node), np.concatenate(slot), np.concatenate(w)
+            o = np.lexsort((node, slot))  # myocardial side: by target slot, then node
+            tgt, first = np.unique(slot[o], return_index=True)
+            tptr = np.append(first, len(o)).astype(np.int32)
+            o2 = np.lexsort((slot, node))  # cable side: by node, then slot
+            nptr = np.zeros(Np + 1, dtype=np.int64)
+            np.add.at(nptr, node[o2] + 1, 1)
+            nptr = np.cumsum(nptr).astype(np.int32)
+            jn = np.unique(node).astype(np.int32)
+            tens = [tgt.astype(np.int32), tptr, node[o].astype(np.int32), w[o] / Ai_M, nptr,
+                    slot[o2].astype(np.int32), w[o2] / Ai_P, jn]
+            self.cn_pmj_t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in tens]
+            t = self.cn_pmj_t
+            self.cn_pmj = nat.HpCnPmj(len(tgt), *(nat.ptr(a) for a in t[:7]), len(jn),
+                                      nat.ptr(t[7]))
+        self.cn_work = nat.HpCnWork(*(nat.ptr(a) for a in self.cn_work_t))
+        self.cn = nat.HpCnParams(self.Cm / self.dt, theta, 1.0 / (A_M * h * h), 1.0 / Ai_P, tol,
+                                 max_sweeps, 0, nat.ptr(self.cn_scalars), nat.ptr(self.cn_part))
+
+    def cn_stats(self):
+        """Solver statistics of the CN steps so far: steps, mean sweeps, steps that
+        stopped at max_sweeps without meeting the tolerance and their worst
+        relative residual (one host synchronisation)."""
+        if self.cn is None:
+            return None
+        sc = self.cn_scalars.cpu().numpy()
+        steps = int(sc[7])
+        return dict(steps=steps, mean_sweeps=float(sc[6] / steps) if steps else 0.0,
+                    not_converged=int(sc[4]), worst_rel_res=float(sc[5]),
+                    max_sweeps=int(self.cn.max_sweeps), tol=float(self.cn.tol))
+
     def _init_state(self, st):
         """Resting TTP06 state into st (18, n) (Ki, Nai as offsets from their bases)."""
         for k, name in enumerate(STATE_NAMES):
@@ -256,6 +324,22 @@ class MonodomainEngine:
         cur = torch.cuda.current_stream(self.dev)
         s = cur.cuda_stream
         a, b = self.p, 1 - self.p
+        if self.cn is not None:
+            cab = self.cable is not None
+            nat.call(f"hp_cn_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
+                     nat.ptr(self.V[b]), nat.ptr(self.state), self.state_ld, C.byref(self.p_myo),
+                     C.byref(self.cable) if cab else None,
+                     nat.ptr(self.Vpk[a]) if cab else None, nat.ptr(self.Vpk[b]) if cab else None,
+                     nat.ptr(self.state_pk) if cab else None, self.Np,
+                     C.byref(self.p_pk) if cab else None,
+                     C.byref(self.cn_pmj) if self.cn_pmj is not None else None,
+                     C.byref(self.cn), C.byref(self.cn_work), s)
+            nat.call(f"hp_step_tail_{self.sfx}", None, nat.ptr(self.Vpk[b]) if cab else None,
+                     nat.ptr(self.V[b]), None, self.dt, self.vmin, self.vmax, None,
+                     nat.ptr(self.site_params), self.n_sites, nat.ptr(self.group_sites),
+                     self.n_groups, nat.ptr(self.step_ctr), nat.ptr(self.group_amp), s)
+            self.p = b
+            return
         if self.cable is not None and self.fuse_cable:
             nat.call(f"hp_myo_cable_step_{self.sfx}", C.byref(self.bricks), nat.ptr(self.V[a]),
                      nat.ptr(self.V[b]), nat.ptr(self.state), self.state_ld, C.byref(self.p_myo),

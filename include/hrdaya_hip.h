@@ -263,6 +263,89 @@ int hp_pmj_couple_f64(const hp_pmj* j, double* V_pk, double* V_myo, double* scra
                       double vmin, double vmax, const hp_pmj_at* at, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Crank-Nicolson step of the coupled system: the reference's scheme
+ * (purkinje_myocardium_anisotropic_solver.py:757-849, loop :1746-1791):
+ *   V*  = V^n - dt*I_ion + (dt/Cm)*I_stim                (Lie splitting)
+ *   ((Cm/dt) I - theta L) V^{n+1} = (Cm/dt) V* + (1-theta) L V*
+ *   V^{n+1} = clamp(V^{n+1}, vmin, vmax)
+ * with L = D^-1 K of the whole system (19-point stencil on the myocardial
+ * rows, cable edges and PMJ couplings, :606-746).  The linear system is solved
+ * matrix-free by Jacobi sweeps from x0 = V* (the junction nodes' rows
+ * re-solved after each sweep from their patches' new values), stopping when ||b - A x|| <=
+ * tol*||b|| (the rule of the reference's _bicgstab, :786-849, which this
+ * replaces) or after max_sweeps sweeps; the iterate returned is the one after
+ * the sweep whose input met the rule.  All decisions stay on the device (no
+ * host synchronisation), so a step can be captured in a HIP graph.
+ * hp_cn_step replaces the reaction + diffusion part of one step (the
+ * explicit path's hp_myo_cable_step); hp_step_tail (without junctions, j =
+ * NULL) then advances the step counter and the stimulus, as for that path.
+ * ------------------------------------------------------------------------- */
+/* PMJ couplings as rows of L (purkinje_myocardium_anisotropic_solver.py:684-692,
+ * 721-724): each (junction node p, patch voxel m) pair has conductance
+ * w = c_pmj/|patch|.  Myocardial side, per target slot (label bit 7, the slots
+ * listed in ascending order): the cable node of each coupling and w/Ai_M.
+ * Cable side (node_ptr may be NULL: no coupling), per cable node: the coupled
+ * slots and w/Ai_P.  jnode lists the cable nodes that have couplings (their
+ * rows are re-solved after each sweep from the patch's new values). */
+typedef struct hp_cn_pmj {
+  int64_t n_targets;
+  const int32_t* tgt_slot;   /* [n_targets], ascending */
+  const int32_t* tgt_ptr;    /* [n_targets+1] */
+  const int32_t* tgt_node;   /* cable node of each coupling */
+  const double* tgt_w;       /* w/Ai_M of each coupling */
+  const int32_t* node_ptr;   /* [n_cable+1] or NULL */
+  const int32_t* node_slot;  /* myocardial slot of each coupling */
+  const double* node_w;      /* w/Ai_P of each coupling */
+  int64_t n_jnodes;
+  const int32_t* jnode;      /* [n_jnodes] distinct cable nodes with couplings */
+} hp_cn_pmj;
+
+/* scalars (device, >= 8 doubles): 0 ||b||^2, 1 ||r||^2 of the last sweep,
+ * 2 done flag, 3 sweeps of the current step, 4 steps that stopped at
+ * max_sweeps without meeting the rule, 5 the largest ||r||/||b|| of those,
+ * 6 total sweeps, 7 steps solved (4-7 accumulate over calls; the caller
+ * zeroes them).  part: device, >= 16384 doubles (block partial sums). */
+typedef struct hp_cn_params {
+  double cm_dt;        /* Cm/dt */
+  double theta;        /* 0 <= theta <= 1 (reference default 0.5) */
+  double dscale;       /* 1/(A_M h^2): L of the myocardial rows = dscale * stencil sum */
+  double dscale_pk;    /* 1/Ai_P: L of the cable rows = dscale_pk * w_pp * sum */
+  double tol;          /* relative residual of the stopping rule (reference cg_tol) */
+  int32_t max_sweeps;  /* >= 1, sweeps per step (launched every step; idle after done) */
+  int32_t pad_;
+  double* scalars;
+  double* part;
+} hp_cn_params;
+
+/* Work vectors (device, the arithmetic type of the call): Vs, b, x of n_slots
+ * entries; Vs_pk, b_pk, x_pk of n_cable entries (may be NULL without a cable).
+ * V_out serves as the second iterate buffer. */
+typedef struct hp_cn_work {
+  void* Vs;
+  void* b;
+  void* x;
+  void* Vs_pk;
+  void* b_pk;
+  void* x_pk;
+} hp_cn_work;
+
+/* p / p_pk: dt, inv_cm, ki/nai bases, instance scales, group_amp (stimulus of
+ * this step), step_ctr, vmin/vmax, at (first activation times, optional) and
+ * at_thr, as for hp_myo_step; do_reaction must be 1.  c may be NULL (no
+ * cable); j may be NULL (no junctions).  Reads V_in / Vpk_in (V^n), writes
+ * V_out / Vpk_out (V^{n+1}), updates the cell states in place. */
+int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state, int64_t ld,
+                   const hp_mono_params* p, const hp_cable* c, const float* Vpk_in,
+                   float* Vpk_out, float* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
+                   const hp_cn_pmj* j, const hp_cn_params* cp, const hp_cn_work* w,
+                   void* stream);
+int hp_cn_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
+                   int64_t ld, const hp_mono_params* p, const hp_cable* c, const double* Vpk_in,
+                   double* Vpk_out, double* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
+                   const hp_cn_pmj* j, const hp_cn_params* cp, const hp_cn_work* w,
+                   void* stream);
+
+/* ---------------------------------------------------------------------------
  * Stimulus schedule on the device (replaces _stim_active, :1290-1301, and the
  * host loop :1754-1759).  site_params[s*5 + 0..4] = onset, dur, period (0 =
  * single pulse), n_pulses (< 0 = unlimited), amp [uA/mm^2].  Stimulus group

@@ -11,8 +11,16 @@ Reference: purkinje_myocardium_anisotropic_solver.py (``aniso.py``)
       (Cm/dt - theta L) V^{n+1} = (Cm/dt) V* + (1-theta) L V*   (BiCGSTAB)
       V^{n+1}        = clamp(V^{n+1}, -150, 80)
 
-Two integrators are provided:
+Three integrators are provided:
   * ``run(..., scheme="cn")``: the reference's scheme above;
+  * ``run(..., scheme="cn_jacobi")``: the same CN step with the linear system
+    solved as the HIP kernels solve it -- Jacobi sweeps x <- x + (b - A x)/diag A
+    from x0 = V*, the first residual taken as L V* (= b - A V*), after each
+    sweep the rows of the junction nodes re-solved from the new patch values
+    and the old cable-neighbour values, stopping after the sweep whose
+    residual meets ||r|| <= cg_tol*||b|| or after ``max_sweeps``; the GPU's
+    FP64 CN path is compared with this one at round-off level, and this one
+    with ``cn`` through the solver tolerance;
   * ``run(..., scheme="explicit")``: the scheme of the HIP kernels -- the
     same reaction step with forward-Euler diffusion on V^n (PMJ entries
     excluded from L), clamp, then the exact two-compartment PMJ exchange.
@@ -117,9 +125,36 @@ class Problem:
         return V
 
 
+def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0):
+    """The HIP path's CN solve (see the module docstring); returns (x, sweeps).
+
+    ``junctions``: the cable nodes with PMJ couplings (global ids < Np); after
+    each sweep row p is solved for x_p with its myocardial entries at the new
+    iterate and its cable entries at the sweep's input."""
+    bb = float(b @ b)
+    x = Vs.copy()
+    r = L @ Vs
+    Ad = A.diagonal()
+    rows = [A.getrow(p) for p in junctions]
+    for k in range(1, max_sweeps + 1):
+        if k > 1:
+            r = b - A @ x
+        xn = x + r * M_inv
+        for p, row in zip(junctions, rows):
+            idx, val = row.indices, row.data
+            off = idx != p
+            cab = off & (idx < Np)
+            myo = off & (idx >= Np)
+            xn[p] = (b[p] - val[cab] @ x[idx[cab]] - val[myo] @ xn[idx[myo]]) / Ad[p]
+        x = xn
+        if float(r @ r) <= tol * tol * bb:
+            return x, k
+    return x, max_sweeps
+
+
 def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtype=torch.float64,
         record_every=0, record_nodes=None, at_thr=-40.0, max_steps=None, state_init=None,
-        V_init=None, g_ks_scale=None, g_to_scale=None):
+        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10):
     """Integrate the problem; returns dict(V, state, at, trace_t, trace_V, steps, wall_s).
 
     ``at`` is (N,) first activation time (NaN if none), on every node.
@@ -137,7 +172,8 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
     ct = torch.from_numpy(prob.celltype)
     gks_s = None if g_ks_scale is None else torch.as_tensor(np.asarray(g_ks_scale), dtype=dtype)
     gto_s = None if g_to_scale is None else torch.as_tensor(np.asarray(g_to_scale), dtype=dtype)
-    if scheme == "cn":
+    sweeps = []
+    if scheme in ("cn", "cn_jacobi"):
         A, M_inv = M.cn_matrix(prob.L, prob.diag_L, Cm / dt, prob.theta)
     at = np.full(N, np.nan)
     tr_t, tr_V = [], []
@@ -151,6 +187,13 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
             Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)
             b = (Cm / dt) * Vs + (1 - prob.theta) * (prob.L @ Vs)
             Vn = M.bicgstab(A, M_inv, b, Vs, cg_tol, cg_max_iter)
+            Vn = np.clip(Vn, -150.0, 80.0)
+        elif scheme == "cn_jacobi":
+            Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)
+            b = (Cm / dt) * Vs + (1 - prob.theta) * (prob.L @ Vs)
+            jn = sorted(set(int(q) for q in prob.pmj_ids)) if prob.c_pmj != 0.0 else []
+            Vn, k = cn_jacobi_solve(A, M_inv, prob.L, Vs, b, cg_tol, max_sweeps, jn, prob.Np)
+            sweeps.append(k)
             Vn = np.clip(Vn, -150.0, 80.0)
         elif scheme == "explicit":
             Vn = V - dt * Iion + (dt / Cm) * (I_stim + prob.L_nopmj @ V)
@@ -167,4 +210,4 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
             tr_V.append(V.copy() if record_nodes is None else V[record_nodes].copy())
     return dict(V=V, state=st, at=at, trace_t=np.array(tr_t),
                 trace_V=np.array(tr_V) if tr_V else None, steps=n_steps,
-                wall_s=time.perf_counter() - t0)
+                wall_s=time.perf_counter() - t0, sweeps=np.array(sweeps, dtype=np.int64))

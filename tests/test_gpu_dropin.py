@@ -1,6 +1,6 @@
 """The reference's call surface on the GPU: compute_anisotropic_coupled /
-compute_coupled (result dict, animation and vm_snapshots files, substepping
-above the explicit limit), the ionic plugin classes and HeteroIonicSolver,
+compute_coupled (result dict, animation and vm_snapshots files, the default
+Crank-Nicolson scheme, substepping of the explicit form above its limit), the ionic plugin classes and HeteroIonicSolver,
 and compute_torso_grid / compute_bspm, each against the oracle or the
 reference's golden vectors."""
 import os
@@ -61,10 +61,13 @@ def test_anisotropic_coupled_files_and_oracle(cuda, tmp_path):
     assert SNAP_KEYS <= set(z.files) and z["Vm"].dtype == np.float32
     assert snap.dt_solver == pytest.approx(dt) and snap.vm_save_dt == pytest.approx(1.0)
     assert snap.n_frames == 16 and snap.N_myo == S.sum()
-    # the oracle's explicit scheme, recorded every 50 steps with the same stamps
+    # the drop-in's default scheme is the reference's Crank-Nicolson step: the
+    # oracle's CN with the kernels' Jacobi solve (min(cg_max_iter, 12) sweeps),
+    # recorded every 50 steps with the same stamps
     prob = OMD.Problem(nodes, el, act, S, phi, f, T=T, **aniso, **kw)
-    ref = OMD.run(prob, T, dt, scheme="explicit", record_every=50,
+    ref = OMD.run(prob, T, dt, scheme="cn_jacobi", record_every=50, max_sweeps=12,
                   record_nodes=np.arange(prob.Np, prob.N))
+    assert res["cn_stats"]["not_converged"] == 0
     np.testing.assert_allclose(snap.time_vm, ref["trace_t"], rtol=0, atol=1e-5)
     # snapshots are stored in FP32: compare at FP32 resolution
     np.testing.assert_allclose(snap.Vm, ref["trace_V"], rtol=0, atol=1e-4)
@@ -95,7 +98,8 @@ def test_step_above_explicit_limit_substeps(cuda, tmp_path):
     assert 0.05 <= lim < 0.1  # so the drop-in takes exactly two substeps
     compute_anisotropic_coupled(nodes, el, act, S, S, phi, f, dt=0.1, T=T, n_frames=4,
                                 vm_save_dt=1.0, vm_save_path=tmp_path / "vm",
-                                dtype=torch.float64, device=cuda, **aniso, **kw)
+                                dtype=torch.float64, device=cuda, scheme="explicit", **aniso,
+                                **kw)
     snap = load_vm_snapshots(tmp_path / "vm")
     prob = OMD.Problem(nodes, el, act, S, phi, f, T=T, **aniso, **kw)
     ref = OMD.run(prob, T, 0.05, scheme="explicit", record_every=1,
@@ -118,7 +122,7 @@ def test_isotropic_coupled_matches_oracle(cuda, tmp_path):
     snap = load_vm_snapshots(tmp_path / "vm")
     prob = OMD.Problem(nodes, el, act, S, phi, None, T=T, A_P=133.33, isotropic_sigma=0.13,
                        **kw)
-    ref = OMD.run(prob, T, 0.02, scheme="explicit", record_every=100,
+    ref = OMD.run(prob, T, 0.02, scheme="cn_jacobi", record_every=100, max_sweeps=12,
                   record_nodes=np.arange(prob.Np, prob.N))
     np.testing.assert_allclose(snap.Vm, ref["trace_V"], rtol=0, atol=1e-4)
     at = res["activation_times"]

@@ -219,3 +219,38 @@ def test_diffusion_substep_equals_fused_rate_form(cuda, brick):
     print(f"  max |dV| {dv:.2e} mV, max |dAT| {dat:.2e} ms over {int(ok.sum())} activated")
     assert ok.sum() > 0
     assert dv < 1e-3 and dat < 1e-4
+
+
+def test_periodic_stimulus_schedule(cuda):
+    """Periodic sites (period, n_pulses) on the device against the reference's
+    _stim_active rule (oracle.mesh.stim_active): the amplitude of every step,
+    including steps whose time lies on a pulse boundary, then V after the run
+    against the explicit oracle."""
+    print("[case] periodic stimulus schedule", flush=True)
+    args, kw = _case("stub")
+    nodes, el, act, S, phi, f = args
+    # pulses of 1 ms every 6 ms from 0.5 ms, three of them; a second site of
+    # 0.3 ms pulses every 0.9 ms (period and duration not multiples of dt)
+    kw = {**kw, "stim_protocol": [
+        dict(name="a", target="ectopic", amp=0.52, dur=1.0, onset=0.5, period=6.0, n_pulses=3),
+        dict(name="b", target="ectopic", amp=0.11, dur=0.3, onset=0.1, period=0.9, n_pulses=7)]}
+    eng, meta = build_coupled_system(nodes, el, act, S, phi, f, dt=0.02, dtype=torch.float64,
+                                     device=cuda, **kw)
+    sites = meta["sites"]
+    n = 1000
+    dt = 0.02
+    got = np.empty(n)
+    want = np.empty(n)
+    gid = int(eng.label[eng.slot_of_node[0].long()].item() >> 2) & 31  # corner group
+    for i in range(n):
+        t = i * dt
+        want[i] = sum(s["amp"] for s in sites
+                      if OM.stim_active(t, s["onset"], s["dur"], s["period"], s["n_pulses"]))
+        got[i] = float(eng.group_amp[gid].item())
+        eng.step(1)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, f"steps {bad[:10]}: device {got[bad[:10]]} vs rule {want[bad[:10]]}"
+    assert (want > 0).sum() > 100 and want[-1] == 0.0  # pulses happened, and ended
+    prob = OMD.Problem(nodes, el, act, S, phi, f, **kw)
+    ref = OMD.run(prob, n * dt, dt, scheme="explicit")
+    np.testing.assert_allclose(eng.V_global().cpu().numpy(), ref["V"], rtol=0, atol=1e-8)
