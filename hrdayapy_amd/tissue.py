@@ -89,8 +89,12 @@ def discretise_tree(nodes_mm, elements, dx):
         m = int(nseg[e])
         ids = [int(elements[e, 0])] + list(range(nxt, nxt + m - 1)) + [int(elements[e, 1])]
         if m > 1:
-            t = (np.arange(1, m) / m)[:, None]
-            extra[k:k + m - 1] = (1 - t) * p0[e] + t * p1[e]
+            # the reference interpolates in FP32: both weights rounded to FP32,
+            # FP32 products and sum (purkinje_myocardium_anisotropic_solver.py:124-125)
+            t = np.arange(1, m) / m
+            w0 = (1.0 - t).astype(np.float32)[:, None]
+            w1 = t.astype(np.float32)[:, None]
+            extra[k:k + m - 1] = w0 * p0[e] + w1 * p1[e]
             k += m - 1
             nxt += m - 1
         edges[ke:ke + m, 0] = ids[:-1]

@@ -1,0 +1,220 @@
+"""The oracle (and the product's host-side set-up that mirrors it) against
+golden vectors written by the reference's own functions
+(tests/golden/make_mesh_vectors.py -> tests/golden/mesh_golden.npz): tree
+discretisation, terminals, the 18-neighbour mesh and its weights, PMJ
+patches, the global Laplacian with its Purkinje and PMJ rows, the CN matrix,
+BiCGSTAB, stimulus sites and schedule, the torso stiffness, surface and pin,
+the heart Laplacian, the projection map and whole frame solves
+(source + Jacobi PCG, the rms / p99 / linf stopping metrics, warm start).
+The inline PMJ-site and root-zone rules of the solver are pinned on
+hand-worked trees."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from hrdayapy_amd import tissue
+from hrdayapy_amd.ecg.functions.forward import projection_rows
+from oracle import ecg as E
+from oracle import mesh as M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def g():
+    return np.load(os.path.join(HERE, "golden", "mesh_golden.npz"))
+
+
+def _branch_map(g):
+    p, c = g["dt_branch_ptr"], g["dt_branch_cat"]
+    return [c[p[i]:p[i + 1]].tolist() for i in range(len(p) - 1)]
+
+
+def _tree_mm(g):
+    return g["tree_nodes"].astype(np.float32) * 0.25
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_discretise_tree(g, impl):
+    fn = M.discretise_tree if impl == "oracle" else tissue.discretise_tree
+    cn, ce, bm = fn(_tree_mm(g), g["tree_el"], 0.2)
+    np.testing.assert_array_equal(np.asarray(cn, np.float32), g["dt_comp_nodes"])
+    np.testing.assert_array_equal(np.asarray(ce), g["dt_comp_edges"])
+    assert [list(map(int, b)) for b in bm] == _branch_map(g)
+
+
+def test_terminals(g):
+    assert M.find_terminals(g["tree_el"], len(g["tree_nodes"])) == g["terminals"].tolist()
+
+
+def test_myo_mesh_edges_and_weights(g):
+    myo = M.build_myo_mesh(g["slab_S"], 0.25, g["slab_fibre"], 0.24, 0.08)
+    np.testing.assert_array_equal(myo["vox_idx"], g["myo_vox_idx"])
+    np.testing.assert_array_equal(myo["edges"], g["myo_edges"])
+    np.testing.assert_allclose(myo["edge_w"], g["myo_edge_w"], rtol=1e-12, atol=0)
+    # nodes in the swapped (col, row, depth) convention of the PMJ search
+    np.testing.assert_array_equal(myo["nodes_mm"], g["myo_nodes_mm"])
+
+
+@pytest.mark.parametrize("n_pmj", [1, 3])
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_pmj_patches(g, n_pmj, impl):
+    ids = g["pmj_ids"].tolist()
+    cn = g["dt_comp_nodes"]
+    if impl == "oracle":
+        got = M.map_pmj(cn, ids, g["myo_nodes_mm"], n_pmj)
+    else:
+        got = tissue.pmj_patches(cn, ids, g["myo_nodes_mm"], n_pmj)
+    np.testing.assert_array_equal(np.asarray(got), g[f"pmj_patches_{n_pmj}"])
+    # the site ids of the solver's inline rule on this tree
+    bm = _branch_map(g)
+    n0 = len(g["tree_nodes"])
+    assert M.pmj_comp_ids(g["tree_el"], bm, n0, len(cn)) == ids
+    assert list(tissue.pmj_sites(g["tree_el"], bm, n0, len(cn))) == ids
+
+
+def _laplacian(g):
+    w_pp, Ai_P, Ai_M, c_pmj = g["lap_consts"]
+    Np = len(g["dt_comp_nodes"])
+    return M.global_laplacian(Np, int(g["myo_N"]), g["dt_comp_edges"], w_pp, Ai_P,
+                              g["myo_edges"], g["myo_edge_w"], Ai_M, g["pmj_ids"].tolist(),
+                              g["pmj_patches_3"].tolist(), c_pmj)
+
+
+def test_global_laplacian_with_pmj_rows(g):
+    L, diag = _laplacian(g)
+    ref = g["lap_LX"]
+    np.testing.assert_allclose(L @ g["lap_X"], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(diag, g["lap_diag"], rtol=1e-13)
+    # the rows of the junction nodes carry the coupling c_pmj/|patch| to each patch voxel
+    Np = len(g["dt_comp_nodes"])
+    assert np.abs(L[g["pmj_ids"], Np:]).sum() > 0
+
+
+def test_cn_matrix_and_bicgstab(g):
+    L, diag = _laplacian(g)
+    A, Minv = M.cn_matrix(L, diag, 0.01 / 0.02, 0.5)
+    ref = g["cn_AX"]
+    np.testing.assert_allclose(A @ g["lap_X"], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(Minv, g["cn_Minv"], rtol=1e-14)
+    for tol in (1e-5, 1e-10):
+        x = M.bicgstab(A, Minv, g["bicg_b"], g["bicg_x0"], tol, 500)
+        xr = g[f"bicg_x_{tol:g}"]
+        # the same iterates (numpy against torch rounding): agreement far
+        # below the solver tolerance
+        np.testing.assert_allclose(x, xr, rtol=0, atol=1e-8 * np.abs(xr).max())
+
+
+@pytest.mark.parametrize("impl", ["oracle", "product"])
+def test_stimulus_sites(g, impl):
+    Np = len(g["dt_comp_nodes"])
+    N_myo = int(g["myo_N"])
+    proto = [dict(name="av", target="root", amp=12.0, dur=2.0, onset=0.0),
+             dict(name="e", target="ectopic", amp=0.52, dur=1.0, onset=3.0, period=6.0,
+                  n_pulses=3),
+             dict(name="r", target="region", region="apex", amp=0.3, dur=0.5, onset=1.0),
+             dict(name="c", target="custom", node_ids=[Np + 3, Np + 7], amp=0.2, dur=1.0)]
+    reg = {"apex": g["stim_reg"]}
+    if impl == "oracle":
+        sites = M.resolve_stim_sites(proto, g["stim_root"], g["stim_ect"], g["myo_vox_idx"], Np,
+                                     N_myo, reg)
+    else:
+        sites = tissue.stim_sites(proto, g["stim_root"], g["stim_ect"], g["myo_vox_idx"], Np,
+                                  reg)
+    amp = np.zeros((len(sites), Np + N_myo))
+    for k, s in enumerate(sites):
+        amp[k, np.asarray(s["node_ids"], dtype=np.int64)] = s["amp"]
+    np.testing.assert_array_equal(amp, g["stim_amp"])
+    assert [s["period"] for s in sites] == [None, 6.0, None, None]
+
+
+def test_stimulus_schedule(g):
+    args = ((3.0, 1.0, 6.0, 3), (0.1, 0.3, 0.9, 7), (0.0, 2.0, None, None),
+            (0.5, 0.25, 0.75, None))
+    got = np.array([[M.stim_active(float(t), *a) for t in g["stim_t"]] for a in args])
+    np.testing.assert_array_equal(got, g["stim_active"])
+
+
+def _torso(g):
+    lab, sig = g["torso_lab"], g["torso_sig"]
+    A, ids, idm = E.assemble_stiffness(sig, 1.5e-3, lab > 0)
+    return lab, A, ids, idm
+
+
+def test_torso_stiffness_surface_pin(g):
+    lab, A, ids, idm = _torso(g)
+    ref = g["torso_AX"]
+    np.testing.assert_allclose(A @ g["torso_X"], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(A.diagonal(), g["torso_diag"], rtol=1e-13)
+    np.testing.assert_array_equal(E.surface_flat(lab, lab > 0), g["torso_surf_flat"])
+    assert E.pin_row_of(lab > 0, idm) == int(g["torso_pin"])
+
+
+def test_heart_laplacian_and_projection(g):
+    K = E.heart_laplacian(g["heart_vox"])
+    np.testing.assert_allclose(K @ g["heart_V"], g["heart_KV"], rtol=0, atol=1e-12)
+    lab = g["torso_lab"]
+    _, _, _, idm = _torso(g)
+    coords = g["heart_vox"][:, [1, 0, 2]].astype(np.float64) * 0.5
+    a = (coords, g["proj_R"], g["proj_t"], float(g["proj_scale"]), g["proj_origin"], 1.5, idm,
+         lab.shape)
+    np.testing.assert_array_equal(E.projection(*a), g["proj_rows"])
+    np.testing.assert_array_equal(projection_rows(*a), g["proj_rows"])
+    assert np.array_equal(g["proj_rows"] >= 0, g["proj_valid"])
+
+
+@pytest.mark.parametrize("metric", ["rms", "p99", "linf"])
+def test_frame_solves(g, metric):
+    """Source and Jacobi PCG of two frames (the second warm-started), FP32 as
+    the reference runs them: the same iteration counts and the same surface
+    and full-field potentials."""
+    lab, A, ids, idm = _torso(g)
+    pin = int(g["torso_pin"])
+    Ap = E.pin(A, pin).astype(np.float32)
+    d = Ap.diagonal()
+    Minv = (1.0 / np.where(np.abs(d) > 1e-30, d, 1.0)).astype(np.float32)
+    K = E.heart_laplacian(g["heart_vox"])
+    proj = g["proj_rows"]
+    x = None
+    for k in range(2):
+        b = E.source(g["frame_Vm"][k], K, proj, 0.2625, 0.5e-3, 1.5, A.shape[0], pin)
+        x, it, res = E.pcg(Ap, b, Minv, x0=x, tol_abs_mV=0.01, max_iter=500, metric=metric,
+                           dtype=np.float32)
+        full = g[f"frame_{metric}_phi_full"][k]
+        assert abs(it - int(g[f"frame_{metric}_iters"][k])) <= 1
+        np.testing.assert_allclose(x, full, rtol=0, atol=2e-3 * np.abs(full).max())
+        sr = g["torso_surf_rows"]
+        np.testing.assert_allclose(x[sr], g[f"frame_{metric}_phi_surf"][k], rtol=0,
+                                   atol=2e-3 * np.abs(full).max())
+
+
+def test_pmj_site_rule_hand_worked():
+    """Elements (0,1),(1,2),(1,3),(3,4),(5,1): nodes 2, 4 and 5 have degree 1,
+    but node 5 is an element's first node, so the sites are the last
+    computational nodes of elements (1,2) and (3,4): original nodes 2 and 4."""
+    el = np.array([[0, 1], [1, 2], [1, 3], [3, 4], [5, 1]])
+    nodes = np.array([[0, 0, 0], [0, 0, 1.0], [0, 1, 2], [0, -1, 2], [0, -1, 3.5], [1, 0, 0.0]],
+                     np.float32)
+    cn, ce, bm = M.discretise_tree(nodes, el, 0.4)
+    assert M.find_terminals(el, 6) == [2, 4, 5]
+    assert M.pmj_comp_ids(el, bm, 6, len(cn)) == [2, 4]
+    assert list(tissue.pmj_sites(el, bm, 6, len(cn))) == [2, 4]
+    # a tree without terminals: the last computational node
+    ring = np.array([[0, 1], [1, 2], [2, 0]])
+    cn2, _, bm2 = M.discretise_tree(nodes[:3], ring, 10.0)
+    assert M.pmj_comp_ids(ring, bm2, 3, len(cn2)) == [2]
+
+
+def test_root_zone_hand_worked():
+    """Earliest activation at nodes 0 and 2 (tie within 1e-6): the zone is
+    centred on their mean (0, 0, 1) mm; NaN times count as T; with a radius
+    below every distance the nearest node alone is stimulated."""
+    nodes = np.array([[0, 0, 0], [0, 0, 1], [0, 0, 2], [0, 3, 0]], np.float32)
+    act = np.array([0.5, 1.0, 0.5 + 1e-7, np.nan])
+    cn = np.array([[0, 0, 0], [0, 0, 1], [0, 0, 2], [0, 3, 0], [0, 0, 1.4]], np.float32)
+    for fn in (M.root_stim_mask, tissue.root_zone):
+        np.testing.assert_array_equal(fn(nodes, act, cn, 100.0, 1.0), [1, 1, 1, 0, 1])
+        np.testing.assert_array_equal(fn(nodes, act, cn, 100.0, 0.1), [0, 1, 0, 0, 0])
+        np.testing.assert_array_equal(fn(nodes, act, cn, 100.0, 0.45), [0, 1, 0, 0, 1])
