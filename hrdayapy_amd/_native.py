@@ -56,7 +56,7 @@ class HpCnPmj(C.Structure):
 class HpCnParams(C.Structure):
     _fields_ = [("cm_dt", c_f64), ("theta", c_f64), ("dscale", c_f64), ("dscale_pk", c_f64),
                 ("tol", c_f64), ("max_sweeps", c_i32), ("pad_", c_i32), ("scalars", c_vp),
-                ("part", c_vp)]
+                ("part", c_vp), ("part_len", c_i64)]
 
 
 class HpCnWork(C.Structure):
@@ -134,6 +134,7 @@ _SIGS = {
     "hp_cn_step_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                  P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
                                  P(HpCnParams), P(HpCnWork), c_vp]),
+    "hp_cn_sweep_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, P(HpCnParams), c_vp]),
     "hp_cn_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                  P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
                                  P(HpCnParams), P(HpCnWork), c_vp]),
@@ -162,6 +163,8 @@ _SIGS = {
     "hp_poisson_residual_f64_noalloc": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, c_vp, c_vp]),
     "hp_pcg_begin_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64, C.c_int,
                                    C.c_int, c_vp]),
+    "hp_pcg_begin_metric_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, P(HpPcgWs), c_f64,
+                                          C.c_int, C.c_int, c_i64, c_vp]),
     "hp_pcg_iterate_f32": (C.c_int, [P(HpTorso), c_vp, c_vp, c_vp, c_vp, P(HpPcgWs), C.c_int,
                                      c_vp]),
     "hp_refine_axpy_f64": (C.c_int, [P(HpTorso), c_vp, c_vp, c_f64, c_vp]),

@@ -41,9 +41,9 @@
 
 namespace hp {
 
-constexpr int CN_MAX_BLOCKS = 8192;   // block partials: part >= 2*CN_MAX_BLOCKS doubles
 constexpr int CN_MAX_CABLE_BLOCKS = 512;
-constexpr int CN_FIN_THREADS = 256;
+constexpr int64_t CN_SWEEP_BLOCKS = 8192;  // brick workgroups of a sweep at most
+constexpr int CN_FIN_THREADS = 1024;
 
 // scalars[]: 0 ||b||^2, 1 ||r||^2 of the last sweep, 2 done, 3 sweeps this step,
 // 4 steps that stopped at max_sweeps, 5 largest ||r||/||b|| of those, 6 total
@@ -182,9 +182,11 @@ __device__ __forceinline__ T cn_row(T xc, T lx, T ld, T& bv, const hp_cn_params&
   return xc + r / (cm_dt + th * ld);
 }
 
-// One sweep over every myocardial row (bricks, persistent: each XCD takes a
-// contiguous run of bricks, its blocks round robin) and, in the first ncb
-// blocks, every cable row.
+// One sweep: bricks in XCD-aware order (each XCD takes a contiguous run of
+// bricks, so halo re-reads stay in its L2), every myocardial row of a brick
+// per workgroup iteration, and, in the first ncb workgroups, the cable rows
+// (grid-stride).  Each workgroup writes its partial sums of r^2 (and
+// b^2) to part[blockIdx] and part[gridDim + blockIdx].
 template <typename T, int BX, int BY, int BZ, bool kFirst>
 __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
     hp_bricks g, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ bvec,
@@ -216,22 +218,25 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
       if (kFirst) bpk[i] = bv;
     }
   } else {
-    const uint32_t bid = blockIdx.x - ncb;
-    const uint32_t NB = (uint32_t)g.n_bricks;
-    const uint32_t xcd = bid & 7, G = gridDim.x - ncb;
-    const uint32_t m = (G - xcd + 7) >> 3;  // blocks with index = xcd (mod 8)
-    const uint32_t base = NB >> 3, rem = NB & 7;
-    const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
-    const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
-    const int lz = tid % BZ, ly = (tid / BZ) % BY, lx_ = tid / (BY * BZ);
-    const int cc = ((lx_ + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
-    const T dscale = T(cp.dscale);
-    for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
-      const uint32_t slot = b * BVOL + tid;
-      const uint8_t mylab = g.label[slot];
-      __syncthreads();  // every lane is done with the previous tile
-      stage_cells<T, BX, BY, BZ>(t, g, x_in, b, mylab);
-      if (mylab == 0xFF) continue;
+   // each XCD (blockIdx & 7) takes a contiguous run of bricks, its
+   // workgroups in turn; with at most CN_SWEEP_BLOCKS workgroups the sweeps
+   // launched after convergence cost little
+   const uint32_t bid = blockIdx.x - ncb;
+   const uint32_t NB = (uint32_t)g.n_bricks;
+   const uint32_t xcd = bid & 7, G = gridDim.x - ncb;
+   const uint32_t m = (G - xcd + 7) >> 3;  // workgroups with index = xcd (mod 8)
+   const uint32_t base = NB >> 3, rem = NB & 7;
+   const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
+   const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
+   for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
+    const uint32_t slot = b * BVOL + tid;
+    const uint8_t mylab = g.label[slot];
+    __syncthreads();  // every lane is done with the previous tile
+    stage_cells<T, BX, BY, BZ>(t, g, x_in, b, mylab);
+    if (mylab != 0xFF) {
+      const int lz = tid % BZ, ly = (tid / BZ) % BY, lx_ = tid / (BY * BZ);
+      const int cc = ((lx_ + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
+      const T dscale = T(cp.dscale);
       const T xc = t.V[cc];
       T ws;
       const T acc = stencil19_wsum<T, BX, BY, BZ>(t, cc, ws);
@@ -249,6 +254,7 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
       st32(x_out, slot, cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb));
       if (kFirst) st32(bvec, slot, bv);
     }
+   }
   }
   cn_block_partials<(BVOL + 63) / 64>(rr, bb, cp.part);
 }
@@ -283,9 +289,20 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
   }
   __shared__ double sh[2][CN_FIN_THREADS / 64];
   double rr = 0.0, bb = 0.0;
-  for (int b = threadIdx.x; b < nb; b += CN_FIN_THREADS) {
-    rr += cp.part[b];
-    bb += cp.part[nb + b];
+  // eight independent loads in flight per array before the adds
+  for (int b0 = 0; b0 < nb; b0 += 8 * CN_FIN_THREADS) {
+    double xr[8], xb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * CN_FIN_THREADS + (int)threadIdx.x;
+      xr[u] = b < nb ? cp.part[b] : 0.0;
+      xb[u] = (kFirst && b < nb) ? cp.part[nb + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      rr += xr[u];
+      bb += xb[u];
+    }
   }
   // per-thread strided sums, then a fixed-order combination: deterministic
   rr = cn_wave_sum(rr);
@@ -357,20 +374,6 @@ __global__ void __launch_bounds__(256) cn_final_kernel(
       }
     }
   }
-}
-
-static int cn_compute_units() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev]) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        n <= 0)
-      n = 256;
-    cus[dev] = n;
-  }
-  return cus[dev];
 }
 
 template <typename T, int BX, int BY, int BZ>
@@ -463,11 +466,10 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
   int64_t ncb = ceil_div(ncab, bvol);
   if (ncb > CN_MAX_CABLE_BLOCKS) ncb = CN_MAX_CABLE_BLOCKS;
   ncb = (ncb + 7) / 8 * 8;
-  int64_t nbb = (int64_t)cn_compute_units() * 16 * 64 / bvol;
-  if (nbb > g->n_bricks) nbb = g->n_bricks;
-  const int64_t G = ncb + nbb;
-  if (G > CN_MAX_BLOCKS) {
-    set_error("%s: %lld blocks exceed the partial buffer", what, (long long)G);
+  const int64_t G = ncb + (g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS);
+  if (G > 0x7fffffff || 2 * G > cp->part_len) {
+    set_error("%s: the sweeps' %lld workgroups need part_len >= %lld doubles (got %lld)", what,
+              (long long)G, (long long)(2 * G), (long long)cp->part_len);
     return -1;
   }
   for (int k = 1; k <= cp->max_sweeps; ++k) {
@@ -501,11 +503,62 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
   return check_launch("hp_cn_step/final");
 }
 
+// One sweep outside a step (the solve's dominant kernel alone, for
+// measurement): x_out from x_in and b as the step's sweeps after the first;
+// the partial sums land in cp->part, the done flag is not read or written.
+template <typename T>
+static int cn_sweep_only(const hp_bricks* g, const T* x_in, T* x_out, const T* bvec,
+                         const hp_cn_params* cp, void* stream) {
+  if (!g || !cp || !cp->part || !x_in || !x_out || !bvec || !g->label || !g->D6 || !g->nbr ||
+      g->n_bricks < 0) {
+    set_error("hp_cn_sweep: brick grid, x_in, x_out, b and CN params (part) are required");
+    return -1;
+  }
+  const int64_t bvol = (int64_t)g->bx * g->by * g->bz;
+  if ((uint64_t)g->n_bricks * bvol * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_cn_sweep: %lld bricks exceed the 32-bit offset range", (long long)g->n_bricks);
+    return -1;
+  }
+  const int64_t G = g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS;
+  if (G == 0) return 0;
+  if (2 * G > cp->part_len) {
+    set_error("hp_cn_sweep: part_len >= %lld doubles required", (long long)(2 * G));
+    return -1;
+  }
+  // a private copy of the scalars' done flag: a sweep alone always runs
+  hp_cn_params c2 = *cp;
+  static __device__ double zero_flag[8];
+  double* zf = nullptr;
+  if (hipGetSymbolAddress((void**)&zf, HIP_SYMBOL(zero_flag)) != hipSuccess || !zf) {
+    set_error("hp_cn_sweep: hipGetSymbolAddress failed");
+    return -1;
+  }
+  c2.scalars = zf;
+  const hp_cable c0{};
+  const hp_cn_pmj j0{};
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  hipStream_t st = (hipStream_t)stream;
+  switch (key) {
+    case 444: launch_sweep<T, 4, 4, 4>(false, dim3(G), st, *g, x_in, x_out, (T*)bvec, c0, nullptr, nullptr, nullptr, j0, c2, 0u); break;
+    case 844: launch_sweep<T, 8, 4, 4>(false, dim3(G), st, *g, x_in, x_out, (T*)bvec, c0, nullptr, nullptr, nullptr, j0, c2, 0u); break;
+    case 884: launch_sweep<T, 8, 8, 4>(false, dim3(G), st, *g, x_in, x_out, (T*)bvec, c0, nullptr, nullptr, nullptr, j0, c2, 0u); break;
+    case 888: launch_sweep<T, 8, 8, 8>(false, dim3(G), st, *g, x_in, x_out, (T*)bvec, c0, nullptr, nullptr, nullptr, j0, c2, 0u); break;
+    default:
+      set_error("hp_cn_sweep: unsupported brick shape %dx%dx%d", g->bx, g->by, g->bz);
+      return -1;
+  }
+  return check_launch("hp_cn_sweep");
+}
+
 }  // namespace hp
 
 using namespace hp;
 
 extern "C" {
+int hp_cn_sweep_f32(const hp_bricks* g, const float* x_in, float* x_out, const float* b,
+                    const hp_cn_params* cp, void* stream) {
+  return cn_sweep_only<float>(g, x_in, x_out, b, cp, stream);
+}
 int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state, int64_t ld,
                    const hp_mono_params* p, const hp_cable* c, const float* Vpk_in, float* Vpk_out,
                    float* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,

@@ -172,7 +172,10 @@ __global__ void __launch_bounds__(PBLOCK) pcg_init_finish_kernel(const double* _
     sc[S_ITER] = 0.0;
     sc[S_MAXIT] = (double)maxit;
     sc[S_BETA] = 0.0;
-    const bool conv = mode == 0 ? red[1] == 0.0 : red[2] <= tol * tol * (double)n_cells;
+    sc[S_ZSTAT] = 0.0;
+    // the reference tests its metric only after an iteration (compute_bspm.py:
+    // 240-252); a zero residual ends a relative-residual solve at once
+    const bool conv = mode == 0 && red[1] == 0.0;
     sc[S_DONE] = (conv || maxit <= 0) ? 1.0 : 0.0;
   }
 }
@@ -224,6 +227,38 @@ void launch_pap_finish(double* part, int nb, double* sc, hipStream_t st) {
 // p <- z + beta p (materialised), d += alpha p, z -= alpha Ap/diag (z = M^-1 r
 // held in the r array), rz, rr with r = diag z; finish: beta, stopping test,
 // iteration count.
+// Extreme values of |z| for the 'linf' and 'p99' stopping metrics
+// (compute_bspm.py:222-237): per block, the largest |z| below tolz, the
+// smallest |z| at or above it, and the count at or above it (linf uses the
+// largest |z| overall, kept in the first slot with tolz = +inf).
+__device__ __forceinline__ void zstat_partials(double below_max, double above_min, double count,
+                                               double* part) {
+  __shared__ double sh[3][PWAVES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    below_max = fmax(below_max, __shfl_xor(below_max, o, 64));
+    above_min = fmin(above_min, __shfl_xor(above_min, o, 64));
+    count += __shfl_xor(count, o, 64);
+  }
+  if (lane == 0) {
+    sh[0][wave] = below_max;
+    sh[1][wave] = above_min;
+    sh[2][wave] = count;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < PWAVES; ++w) {
+      below_max = fmax(below_max, sh[0][w]);
+      above_min = fmin(above_min, sh[1][w]);
+      count += sh[2][w];
+    }
+    part[3 * gridDim.x + blockIdx.x] = below_max;
+    part[4 * gridDim.x + blockIdx.x] = above_min;
+    part[5 * gridDim.x + blockIdx.x] = count;
+  }
+}
+
 __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* __restrict__ z,
                                                             float* __restrict__ d,
                                                             float* __restrict__ p,
@@ -232,6 +267,9 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
   if (sc[S_DONE] != 0.0) return;
   const float alpha = (float)sc[S_ALPHA];
   const float beta = (float)sc[S_BETA];
+  const int mode = (int)sc[S_MODE];
+  const double tolz = mode == 3 ? sqrt(sc[S_TOL2]) : HUGE_VAL;
+  double bmax = 0.0, amin = HUGE_VAL, cnt = 0.0;
   double v[3] = {0.0, 0.0, 0.0};
   const uint32_t vb = (uint32_t)t.n_cells * 4u;
   const buf_t Z = buf_rsrc(z, vb), D = buf_rsrc(d, vb), P = buf_rsrc(p, vb),
@@ -256,8 +294,17 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_kernel(hp_torso t, float* _
         v[0] = in ? v[0] + rn * (double)zn : v[0];
         v[1] = in ? v[1] + rn * rn : v[1];
         v[2] = in ? v[2] + (double)zn * (double)zn : v[2];
+        if (mode >= 2 && in) {
+          const double a = fabs((double)zn);
+          if (a < tolz) bmax = fmax(bmax, a);
+          else {
+            amin = fmin(amin, a);
+            cnt += 1.0;
+          }
+        }
       });
   block_partials<3>(v, part);
+  if (mode >= 2) zstat_partials(bmax, amin, cnt, part);
 }
 
 // rz, ||r||^2, ||z||^2 from the update's partials: beta, stopping test,
@@ -267,15 +314,59 @@ __global__ void __launch_bounds__(PBLOCK) pcg_update_finish_kernel(const double*
   if (sc[S_DONE] != 0.0) return;
   double red[3];
   sum_partials<3>(part, nb, red);
+  const int mode = (int)sc[S_MODE];
+  double bmax = 0.0, amin = HUGE_VAL, cnt = 0.0;
+  if (mode >= 2) {  // the |z| extremes over the blocks
+    __shared__ double sh[3][PBLOCK / 64];
+    for (int b = threadIdx.x; b < nb; b += PBLOCK) {
+      bmax = fmax(bmax, part[3 * nb + b]);
+      amin = fmin(amin, part[4 * nb + b]);
+      cnt += part[5 * nb + b];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      bmax = fmax(bmax, __shfl_xor(bmax, o, 64));
+      amin = fmin(amin, __shfl_xor(amin, o, 64));
+      cnt += __shfl_xor(cnt, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      sh[0][threadIdx.x >> 6] = bmax;
+      sh[1][threadIdx.x >> 6] = amin;
+      sh[2][threadIdx.x >> 6] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w < PBLOCK / 64; ++w) {
+        bmax = fmax(bmax, sh[0][w]);
+        amin = fmin(amin, sh[1][w]);
+        cnt += sh[2][w];
+      }
+  }
   if (threadIdx.x == 0) {
     sc[S_BETA] = red[0] / sc[S_RZ];
     sc[S_RZ] = red[0];
     sc[S_RR] = red[1];
     sc[S_ZZ] = red[2];
     sc[S_ITER] += 1.0;
-    // mode 0: relative residual; mode 1: rms of z = M^-1 r (compute_bspm.py:222-256)
-    const bool conv = sc[S_MODE] == 0.0 ? red[1] <= sc[S_TOL2] * sc[S_RR0]
-                                        : red[2] <= sc[S_TOL2] * sc[S_NCELL];
+    const double tol2 = sc[S_TOL2], n = sc[S_NCELL];
+    bool conv;
+    if (mode == 0) {  // relative residual
+      conv = red[1] <= tol2 * sc[S_RR0];
+    } else if (mode == 1) {  // rms(z) < tol (compute_bspm.py:222-223)
+      conv = red[2] < tol2 * n;
+      sc[S_ZSTAT] = sqrt(red[2] / n);
+    } else if (mode == 2) {  // max |z| < tol
+      sc[S_ZSTAT] = bmax;
+      conv = bmax < sqrt(tol2);
+    } else {  // 99th percentile of |z| (linear interpolation) < tol
+      // sorted |z| of the n rows: the cnt largest are >= tol; position
+      // q = 0.99 (n-1) = k + frac lies below them, straddles them, or not
+      const double tol = sqrt(tol2), q = 0.99 * (n - 1.0), k = floor(q), frac = q - k;
+      const double first_above = n - cnt;  // index of the smallest value >= tol
+      if (k + 1.0 < first_above) conv = true;  // both neighbours of q below tol
+      else if (k >= first_above) conv = false;
+      else conv = bmax + frac * (amin - bmax) < tol;  // s[k] < tol <= s[k+1]
+    }
     if (conv || sc[S_ITER] >= sc[S_MAXIT]) sc[S_DONE] = 1.0;
   }
 }
@@ -577,15 +668,36 @@ int hp_poisson_residual_f64_noalloc(const hp_torso* t, const double* b, const do
   return check_launch("hp_poisson_residual_f64_noalloc");
 }
 
+static int pcg_begin(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws, double tol,
+                     int max_iter, int mode, int64_t n_rows, void* stream);
+
 int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
                      double tol, int max_iter, int mode, void* stream) {
+  if (mode < 0 || mode > 1) {
+    set_error("hp_pcg_begin_f32: mode 0 or 1 (hp_pcg_begin_metric_f32 has the others)");
+    return -1;
+  }
+  return pcg_begin(t, r, d, p, ws, tol, max_iter, mode, t ? t->n_cells : 0, stream);
+}
+
+int hp_pcg_begin_metric_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
+                            double tol, int max_iter, int mode, int64_t n_rows, void* stream) {
+  if (mode < 0 || mode > 3 || n_rows < 1 || (t && n_rows > t->n_cells)) {
+    set_error("hp_pcg_begin_metric_f32: mode 0..3 and 1 <= n_rows <= n_cells required");
+    return -1;
+  }
+  return pcg_begin(t, r, d, p, ws, tol, max_iter, mode, n_rows, stream);
+}
+
+static int pcg_begin(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws, double tol,
+                     int max_iter, int mode, int64_t n_rows, void* stream) {
   if (!torso_ok(t, "hp_pcg_begin_f32") ||
       !ws_ok(ws, 0, true, "hp_pcg_begin_f32")) return -1;
   const int nb = grid_blocks(t);
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(pcg_init_kernel, dim3(nb), dim3(PBLOCK), 0, st, *t, r, d, p, ws->part);
   hipLaunchKernelGGL(pcg_init_finish_kernel, dim3(1), dim3(PBLOCK), 0, st, ws->part, nb,
-                     ws->scalars, tol, max_iter, mode, t->n_cells);
+                     ws->scalars, tol, max_iter, mode, n_rows);
   return check_launch("hp_pcg_begin_f32");
 }
 

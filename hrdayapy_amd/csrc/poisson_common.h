@@ -336,7 +336,7 @@ __device__ __forceinline__ float dir_product(const hp_torso& t, uint32_t c, floa
 // PCG state in device memory (FP64 scalars)
 // ---------------------------------------------------------------------------
 enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR0, S_RR, S_TOL2, S_DONE, S_ITER, S_MAXIT, S_ZZ,
-       S_MODE, S_NCELL, S_NSC };
+       S_MODE, S_NCELL, S_ZSTAT, S_NSC };
 
 // Blocks of PBLOCK threads over the chunk table: at most 4096 (16 per CU;
 // measured 6 % faster for the SpMV than 2048 on the 18.6 M-cell torso), a

@@ -5,9 +5,10 @@ Drop-in for hrdayapy/ecg/bspm.py:33-145 and run_bspm_loop
 (N_surf, n_frames) mV result and the same bspm.npz keys.  Per frame the Vm
 snapshot is uploaded into the heart's brick layout, the source is built on
 the device and the torso potential is solved by warm-started Jacobi PCG with
-the reference's 'rms' stopping rule (cg_tol_mV), or -- with
-``refine_tol`` -- by FP32 PCG inside FP64 iterative refinement to a relative
-residual.  Only the 'rms' metric is implemented on the device.
+the reference's stopping rule (cg_metric 'rms', 'p99' or 'linf' of
+z = M^-1 r below cg_tol_mV, tested on the device after every iteration), or
+-- with ``refine_tol`` -- by FP32 PCG inside FP64 iterative refinement to a
+relative residual.
 """
 from __future__ import annotations
 
@@ -32,8 +33,8 @@ def compute_bspm(vm_snapshots_path, grid, registration, *, anatomy=None, sigma_i
     if save_path is None:
         raise ValueError("compute_bspm requires save_path (run_bspm_loop writes results "
                          "incrementally as it solves).")
-    if cg_metric != "rms":
-        raise NotImplementedError("only cg_metric='rms' is implemented on the device")
+    if cg_metric not in ("rms", "p99", "linf"):
+        raise ValueError(f"metric must be one of 'rms', 'p99', 'linf', got {cg_metric!r}")
     h_fine_mm = resolve_voxel_size(h_fine_mm, anatomy, param_name="h_fine_mm")
     dev = torch.device(device)
     snap = np.load(str(Path(vm_snapshots_path)))
@@ -82,7 +83,8 @@ def compute_bspm(vm_snapshots_path, grid, registration, *, anatomy=None, sigma_i
         if refine_tol is not None:
             ps.solve_refined(b, x, tol=refine_tol)
         else:
-            ps.solve_rms(b, x, tol_V=cg_tol_mV * 1e-3, max_iter=cg_max_iter)
+            ps.solve_metric(b, x, tol_V=cg_tol_mV * 1e-3, max_iter=cg_max_iter,
+                            metric=cg_metric)
         iters.append(ps.last_stats.get("inner", 0))
         bspm[:, f] = (x[surf_cells] * 1e3).float().cpu().numpy()
         if f in ff:

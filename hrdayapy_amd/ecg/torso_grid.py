@@ -17,8 +17,8 @@ from .functions.torso import TorsoGrid
 
 
 def compute_torso_grid(torso_nrrd_path=None, sigma_map=None, *, dx_coarse_mm=2.0, verbose=True,
-                       save_path=None, use_sbm=False, label_volume=None, origin_mm=(0.0, 0.0, 0.0),
-                       **_ignored):
+                       save_path=None, use_sbm=False, sbm_interface_width_mm=3.0,
+                       sbm_psi_cutoff=1e-3, label_volume=None, origin_mm=(0.0, 0.0, 0.0)):
     if use_sbm:
         raise NotImplementedError("the Smoothed Boundary Method is not part of the GPU path")
     if sigma_map is None:

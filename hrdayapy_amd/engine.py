@@ -247,7 +247,8 @@ class MonodomainEngine:
         z = lambda m: torch.zeros(max(m, 1), dtype=dt_, device=dev)
         self.cn_work_t = [z(n), z(n), z(n), z(Np), z(Np), z(Np)]
         self.cn_scalars = torch.zeros(16, dtype=torch.float64, device=dev)
-        self.cn_part = torch.zeros(16384, dtype=torch.float64, device=dev)
+        part_len = 2 * (min(L.n_bricks, 8192) + 1024)  # per sweep workgroup (<= 520 cable)
+        self.cn_part = torch.zeros(part_len, dtype=torch.float64, device=dev)
         Ai_P = float(cable["Ai_P"]) if Np else 1.0
         Ai_M = A_M * h ** 3
         self.cn_pmj = None
@@ -277,7 +278,8 @@ class MonodomainEngine:
                                       nat.ptr(t[7]))
         self.cn_work = nat.HpCnWork(*(nat.ptr(a) for a in self.cn_work_t))
         self.cn = nat.HpCnParams(self.Cm / self.dt, theta, 1.0 / (A_M * h * h), 1.0 / Ai_P, tol,
-                                 max_sweeps, 0, nat.ptr(self.cn_scalars), nat.ptr(self.cn_part))
+                                 max_sweeps, 0, nat.ptr(self.cn_scalars), nat.ptr(self.cn_part),
+                                 part_len)
 
     def cn_stats(self):
         """Solver statistics of the CN steps so far: steps, mean sweeps, steps that

@@ -304,7 +304,9 @@ typedef struct hp_cn_pmj {
  * 2 done flag, 3 sweeps of the current step, 4 steps that stopped at
  * max_sweeps without meeting the rule, 5 the largest ||r||/||b|| of those,
  * 6 total sweeps, 7 steps solved (4-7 accumulate over calls; the caller
- * zeroes them).  part: device, >= 16384 doubles (block partial sums). */
+ * zeroes them).  part: device, part_len >= 2*(min(n_bricks, 8192) + ncb)
+ * doubles (the workgroup partial sums of a sweep: up to 8192 brick workgroups
+ * plus ncb <= 520 for the cable rows). */
 typedef struct hp_cn_params {
   double cm_dt;        /* Cm/dt */
   double theta;        /* 0 <= theta <= 1 (reference default 0.5) */
@@ -315,6 +317,7 @@ typedef struct hp_cn_params {
   int32_t pad_;
   double* scalars;
   double* part;
+  int64_t part_len;    /* doubles in part */
 } hp_cn_params;
 
 /* Work vectors (device, the arithmetic type of the call): Vs, b, x of n_slots
@@ -339,6 +342,12 @@ int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* s
                    float* Vpk_out, float* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
                    const hp_cn_pmj* j, const hp_cn_params* cp, const hp_cn_work* w,
                    void* stream);
+/* One Jacobi sweep of the myocardial rows alone (no cable, no junctions):
+ * x_out = x_in + (b - A x_in)/diag(A) with the step's kernel, the residual's
+ * block partial sums into cp->part; cp->scalars is not used.  No counterpart in
+ * the reference: the solve's dominant kernel by itself, for measurement. */
+int hp_cn_sweep_f32(const hp_bricks* g, const float* x_in, float* x_out, const float* b,
+                    const hp_cn_params* cp, void* stream);
 int hp_cn_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
                    int64_t ld, const hp_mono_params* p, const hp_cable* c, const double* Vpk_in,
                    double* Vpk_out, double* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,
@@ -436,9 +445,11 @@ int hp_torso_chunks(int32_t ni, int32_t nj, int32_t nk, const int32_t* col_k0,
                     const int32_t* col_len, const int32_t* col_off, int32_t* chunks,
                     int64_t capacity, int64_t* n_chunks);
 
-/* Reduction workspace: part >= 3*4096 doubles (block partial sums), counter:
- * not used (kept for the layout; may be NULL), out >= 2 doubles, scalars >= 16
- * doubles (PCG state: rz, pAp, alpha, beta, rr0, rr, tol^2, done, iter, maxit). */
+/* Reduction workspace: part >= 3*4096 doubles (block partial sums; 6*4096 for
+ * the linf and p99 metrics of hp_pcg_begin_metric_f32), counter: not used (kept
+ * for the layout; may be NULL), out >= 2 doubles, scalars >= 16 doubles (PCG
+ * state: rz, pAp, alpha, beta, rr0, rr, tol^2, done, iter, maxit, ||z||^2,
+ * mode, rows, the metric's last value). */
 typedef struct hp_pcg_ws {
   double* part;
   int* counter;
@@ -474,6 +485,17 @@ int hp_pcg_begin_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws*
                      double tol, int max_iter, int mode, void* stream);
 int hp_pcg_iterate_f32(const hp_torso* t, float* r, float* d, float* p, float* Ap,
                        hp_pcg_ws* ws, int n_iter, void* stream);
+/* hp_pcg_begin_f32 with every stopping metric of _pcg_gpu (compute_bspm.py:
+ * 164-263) on z = M^-1 r over the n_rows rows of the reference system (the
+ * stored cells minus the holes of the column runs): mode 1 rms(z) < tol,
+ * 2 max|z| < tol, 3 the 99th percentile of |z| (linear interpolation, as
+ * torch.quantile) < tol, each tested after every iteration and never before
+ * the first; mode 0 as in hp_pcg_begin_f32.  scalars[13] holds the last value
+ * of the rms / linf statistic (p99: evaluate the quantile from z afterwards).
+ * The p99 test needs no sort: from the count of |z| >= tol and the values
+ * next to tol on either side it decides exactly where the quantile falls. */
+int hp_pcg_begin_metric_f32(const hp_torso* t, float* r, float* d, float* p, hp_pcg_ws* ws,
+                            double tol, int max_iter, int mode, int64_t n_rows, void* stream);
 /* ---------------------------------------------------------------------------
  * Geometric multigrid preconditioner for the torso operator.  No counterpart
  * in the reference, whose _pcg_gpu preconditions with the diagonal

@@ -186,6 +186,58 @@ def test_hetero_solver_matches_oracle(cuda):
                                st.numpy()[hs._groups["mid"]], rtol=1e-9, atol=1e-12)
 
 
+class _Leak:
+    """A user model outside the HIP phenotypes: I = 0.5 (V + 80)."""
+    name = "Leak"
+
+    def state_names(self):
+        return []
+
+    def init_state_tensor(self, N, device, dtype):
+        return torch.empty((N, 0), dtype=dtype, device=device)
+
+    def step(self, V, state, dt, I_stim):
+        return 0.5 * (V + 80.0), state
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_node_dispatcher_matches_oracle(cuda, mixed):
+    """NodeIonicDispatcher (dispatcher.py:69-209) with the three TTP06
+    phenotypes in one launch, plus (mixed) a silent region and a user model:
+    currents and states against the oracle over 20 steps; get/set_state."""
+    from hrdayapy_amd.simulation.functions.ionic_models import NodeIonicDispatcher, SilentCell
+    print(f"[case] NodeIonicDispatcher vs oracle mixed={mixed}", flush=True)
+    rng = np.random.default_rng(11)
+    n = 600
+    labels = rng.integers(1, 4 if not mixed else 6, n)  # 1 endo, 2 mid, 3 epi, 4 leak, 5 silent
+    reg = {1: EndoTTP06(), 2: MidTTP06(), 3: EpiTTP06()}
+    if mixed:
+        reg[4] = _Leak()
+    disp = NodeIonicDispatcher(labels, reg, cuda, torch.float64)
+    assert isinstance(disp.model_for_region(5 if mixed else 1), SilentCell if mixed else EndoTTP06)
+    nat_mask = labels <= 3
+    code = torch.from_numpy(np.where(labels == 1, 0, np.where(labels == 2, 1, 2))[nat_mask])
+    V = torch.from_numpy(rng.uniform(-90, 30, n))
+    st = O.init_state(int(nat_mask.sum()))
+    for k in range(20):
+        I = disp.step(V.to(cuda), 0.02).cpu()
+        I_ref, st = O.step_celltype(V[nat_mask], st, 0.02, code)
+        np.testing.assert_allclose(I.numpy()[nat_mask], I_ref.numpy(), rtol=1e-9, atol=1e-10)
+        if mixed:
+            np.testing.assert_allclose(I.numpy()[labels == 4], 0.5 * (V.numpy()[labels == 4] + 80))
+            assert np.all(I.numpy()[labels == 5] == 0.0)
+        V = V - 0.02 * I
+    for lab in (1, 2, 3):
+        idx = np.nonzero(labels == lab)[0]
+        pos = np.searchsorted(np.nonzero(nat_mask)[0], idx)
+        np.testing.assert_allclose(disp.get_state(lab).cpu().numpy(), st.numpy()[pos], rtol=1e-9,
+                                   atol=1e-12)
+    # set_state broadcasts a pre-paced state over a region (the reference's use)
+    s0 = O.init_state(int((labels == 2).sum())).to(cuda)
+    disp.set_state(2, s0)
+    assert torch.equal(disp.get_state(2), s0)
+
+
 def _heart_snapshots(tmp_path, h=0.4):
     """A synthetic vm_snapshots.npz: a 14x12x6 voxel slab crossed by a planar
     front (smooth tanh upstroke) at four times."""
@@ -290,3 +342,36 @@ def test_activation_tables_on_device_equal_host(cuda):
     s = _snapshots()
     _same_table(compute_activation_maps(s).activation_times,
                 compute_activation_maps(s, device=cuda).activation_times)
+
+
+@pytest.mark.parametrize("metric", ["rms", "p99", "linf"])
+def test_bspm_metrics_match_reference_frames(cuda, tmp_path, metric):
+    """compute_torso_grid + compute_bspm on the torso, heart, registration and
+    Vm frames for which the reference's own make_gpu_solver wrote its frame
+    solves (tests/golden/mesh_golden.npz: two frames, warm start, each of the
+    rms / p99 / linf stopping metrics at cg_tol_mV = 0.01): the same pin and
+    surface rows, and surface potentials within the stopping tolerance's
+    reach (2e-3 of the largest potential)."""
+    print(f"[case] compute_bspm metric={metric} vs reference frames", flush=True)
+    g = np.load(os.path.join(HERE, "golden", "mesh_golden.npz"))
+    lab = g["torso_lab"]
+    smap = {1: 0.2, 3: 0.05, 4: 0.7}
+    grid = compute_torso_grid(label_volume=lab, sigma_map=smap, dx_coarse_mm=1.5,
+                              origin_mm=tuple(g["proj_origin"]), verbose=False)
+    assert int(grid["pin_row"]) == int(g["torso_pin"])
+    np.testing.assert_array_equal(np.asarray(grid["surface_flat"]), g["torso_surf_flat"])
+    hv = g["heart_vox"]
+    Vm = g["frame_Vm"]
+    np.savez(tmp_path / "vm_snapshots.npz", node_ids=np.arange(len(hv), dtype=np.int32),
+             coords_mm=(hv[:, [1, 0, 2]] * 0.5).astype(np.float32), vox_idx=hv.astype(np.int32),
+             Vm=Vm.astype(np.float32), time_vm=np.array([1.0, 2.0], np.float32),
+             dt_solver=np.float32(0.02), vm_save_dt=np.float32(1.0))
+    reg = dict(R=g["proj_R"], t=g["proj_t"], scale=float(g["proj_scale"]))
+    out = compute_bspm(tmp_path / "vm_snapshots.npz", grid, reg, h_fine_mm=0.5, cg_tol_mV=0.01,
+                       cg_max_iter=500, cg_metric=metric, save_path=tmp_path / "bspm.npz",
+                       verbose=False)
+    ref = g[f"frame_{metric}_phi_surf"].T * 1e3  # V -> mV, (N_surf, n_frames)
+    scale = np.abs(g[f"frame_{metric}_phi_full"]).max() * 1e3
+    err = np.abs(out - ref).max()
+    print(f"  max |dphi| {err:.3e} mV of {scale:.3e} mV")
+    assert out.shape == ref.shape and err < 2e-3 * scale

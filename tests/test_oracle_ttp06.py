@@ -64,3 +64,13 @@ def test_trace_matches_reference_single_cell_ap():
     assert 25.0 < tr.max() < 45.0
     m = O.ap_metrics(np.arange(len(tr)) * 0.2, tr)
     assert 240.0 < m["APD90_ms"] < 320.0
+
+
+def test_pace_single_cell_matches_reference_trace():
+    """The oracle's single-cell pacing loop (experiment 1's protocol) against
+    the reference's own trace, first 40 ms (upstroke, notch, early plateau)."""
+    g = _gold()
+    t, V = O.pace_single_cell("epi", dt=0.02, bcl=40.0)
+    ref = g["trace_epi"][:len(V[::10])]
+    np.testing.assert_allclose(V[::10], ref, rtol=0, atol=1e-9)
+    assert ref.max() > 20.0
