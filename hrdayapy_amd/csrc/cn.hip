@@ -195,7 +195,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
   constexpr int BVOL = BX * BY * BZ;
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
-  if (!kFirst && cp.scalars[SC_DONE] != 0.0) return;  // block-uniform
+  // block-uniform; a sweep outside a step (hp_cn_sweep) has no scalars
+  if (!kFirst && cp.scalars && cp.scalars[SC_DONE] != 0.0) return;
   double rr = 0.0, bb = 0.0;
   const int tid = threadIdx.x;
   if (blockIdx.x < ncb) {
@@ -525,15 +526,9 @@ static int cn_sweep_only(const hp_bricks* g, const T* x_in, T* x_out, const T* b
     set_error("hp_cn_sweep: part_len >= %lld doubles required", (long long)(2 * G));
     return -1;
   }
-  // a private copy of the scalars' done flag: a sweep alone always runs
+  // without the scalars the sweep reads no done flag: it always runs
   hp_cn_params c2 = *cp;
-  static __device__ double zero_flag[8];
-  double* zf = nullptr;
-  if (hipGetSymbolAddress((void**)&zf, HIP_SYMBOL(zero_flag)) != hipSuccess || !zf) {
-    set_error("hp_cn_sweep: hipGetSymbolAddress failed");
-    return -1;
-  }
-  c2.scalars = zf;
+  c2.scalars = nullptr;
   const hp_cable c0{};
   const hp_cn_pmj j0{};
   const int key = g->bx * 100 + g->by * 10 + g->bz;

@@ -347,6 +347,8 @@ class SurrogateRunner:
                  v_hi=-40.0, n_blocks=0, compact=True, graph_macros=2, variant=1):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
+        if eng.scheme != "explicit":
+            raise ValueError("the multirate forms substep the explicit engine (scheme='explicit')")
         if mode not in ("mlp", "bigstep"):
             raise ValueError(mode)
         if mode == "mlp" and net is None:
