@@ -99,10 +99,22 @@ def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None, scheme="cn"):
     return geo, tree, kw, eng, meta, t_geo
 
 
-FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule)
-    "ref64": ("cn", "float64", None), "plain": ("cn", "float32", None),
-    "explicit": ("explicit", "float32", None), "bigstep": ("explicit", "float32", "bigstep"),
-    "surrogate": ("explicit", "float32", "mlp")}
+FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
+    "ref64": ("cn", "float64", None, {}), "plain": ("cn", "float32", None, {}),
+    "explicit": ("explicit", "float32", None, {}),
+    # multirate forms: one reaction per 0.2 ms macro-step (exact big step /
+    # network), ten substeps of the reference's CN diffusion driven by the rate
+    # (fallback voxels run the full cell model every substep)
+    "bigstep": ("cn", "float32", "bigstep", {}),
+    "surrogate": ("cn", "float32", "mlp", {}),
+    # the same rules on explicit substeps (the earlier multirate form)
+    "bigstep_explicit": ("explicit", "float32", "bigstep", {}),
+    "surrogate_explicit": ("explicit", "float32", "mlp", {}),
+    # the CN step with theta = 0 and one sweep: V^{n+1} = V* + (dt/Cm) L V*
+    # exactly (forward Euler after the reaction, the reference's Lie order)
+    "lie0": ("cn", "float32", None, dict(theta=0.0, max_sweeps=1)),
+    # the reference's CN with at most two sweeps per step
+    "cn2": ("cn", "float32", None, dict(max_sweeps=2))}
 
 
 def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, sample_slots,
@@ -114,9 +126,10 @@ def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, s
     import torch
     from hrdayapy_amd import surrogate as SG
     from hrdayapy_amd import workloads as W
-    scheme, dname, rule = FORM_SETUP[form]
-    eng, meta = build_engine(geo, tree, kw, dtype=getattr(torch, dname), brick=brick,
-                             scheme=scheme)
+    scheme, dname, rule, opts = FORM_SETUP[form]
+    kwf = dict(kw, **{k: v for k, v in opts.items() if k != "max_sweeps"})
+    eng, meta = build_engine(geo, tree, kwf, dtype=getattr(torch, dname), brick=brick,
+                             scheme=scheme, max_sweeps=opts.get("max_sweeps", 5))
     run = SG.SurrogateRunner(eng, net, mode=rule, graph_macros=5) if rule else None
     advance = (lambda: run.run(STEPS_PER_MS // SG.N_SUB)) if run else (
         lambda: eng.run(STEPS_PER_MS, graph_chunk=STEPS_PER_MS))
@@ -467,7 +480,7 @@ def main():
     del eng0
     torch.cuda.empty_cache()
     net, net_info = None, None
-    if "surrogate" in forms:
+    if any(f.startswith("surrogate") for f in forms):
         from hrdayapy_amd import surrogate as SG
         t1 = time.perf_counter()
         X, D = SG.training_runs("cuda", a.train_samples)
@@ -485,20 +498,22 @@ def main():
     for form in ["ref64"] + forms:
         r, at, phi, vm, eng, m = run_form_beat(form, geo, tree, kw, brick, ecg, a.beat_ms,
                                                a.warmup, a.steps, sample_slots,
-                                               net=net if form == "surrogate" else None, log=log)
+                                               net=net if form.startswith("surrogate") else None,
+                                               log=log)
         if form == "ref64":
             ref = (at, phi, vm)
             r["activated"] = int(np.isfinite(at).sum())
         else:
             r["vs_ref64"] = gates(at, phi, vm, ref, names)
-        if form == "surrogate":
+        if form.startswith("surrogate"):
             r["network"] = net_info
         results[form] = r
         log(f"[bench] {form}: {json.dumps(r)}")
         del eng, vm
         torch.cuda.empty_cache()
     plain_v = results["plain"]["value"] if "plain" in results else None
-    for f in ("explicit", "bigstep", "surrogate"):
+    for f in ("explicit", "bigstep", "surrogate", "bigstep_explicit", "surrogate_explicit",
+              "cn2"):
         if f in results and plain_v:
             results[f]["speedup_vs_plain"] = results[f]["value"] / plain_v
             results[f]["beat_speedup_vs_plain"] = (results["plain"]["beat_s"]

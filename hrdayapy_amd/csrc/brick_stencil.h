@@ -312,6 +312,110 @@ __device__ __forceinline__ T stencil19(const Tile<T, BX, BY, BZ>& t, int c) {
   return acc;
 }
 
+// halo_store with every cell that holds no tissue (label 0xFF, or no brick)
+// marked by V = NaN, so that a stencil can test presence on the value it
+// reads anyway instead of reading the label (stencil19_wsum_nan).
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void halo_store_nan(Tile<T, BX, BY, BZ>& t,
+                                               const HaloRegs<T, BX, BY, BZ>& h) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  const T qnan = __builtin_nan("");
+#pragma unroll
+  for (int it = 0; it < HaloPlan<BX, BY, BZ>::NIT; ++it) {
+    const int kind = (h.pw[it] >> 24) & 3;
+    const int i = kind ? (int)(h.pw[it] & 1023) : TT::N;
+    const bool none = h.q[it] == OOB || (uint8_t)h.lab[it] == 0xFF;
+    t.L[i] = none ? (uint8_t)0xFF : (uint8_t)h.lab[it];
+    t.V[i] = none ? qnan : h.v[it];
+    if (it == 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t.D[k][i] = h.d[0][k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) t.D[3 + k][i] = h.dself[k];
+    } else {
+      const int c0 = (h.pw[it] >> 26) & 7;
+      const int i12 = kind == 2 ? i : TT::N;
+      t.D[c0 == 2 ? 4 : 3][i12] = h.d[it][1];
+      t.D[c0 == 0 ? 4 : 5][i12] = h.d[it][2];
+      t.D[c0][i] = h.d[it][0];
+    }
+  }
+}
+
+// stencil19_wsum on a tile staged by halo_store_nan: an absent neighbour is a
+// NaN value (the comparison v == v is false for it), no label reads.
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ T stencil19_wsum_nan(const Tile<T, BX, BY, BZ>& t, int c, T& wsum) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  const T Vc = t.V[c];
+  T acc = T(0), ws = T(0);
+  const int fo[3] = {TT::SX, TT::SY, 1};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T Da = t.D[a][c];
+#pragma unroll
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      const int nb = c + sgn * fo[a];
+      const T vn = t.V[nb];
+      if (vn == vn) {
+        const T Db = t.D[a][nb];
+        const T s = Da + Db;
+        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
+        acc += dh * (vn - Vc);
+        ws += dh;
+      }
+    }
+  }
+  const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T Da = t.D[3 + q][c];
+    const int o1 = po[q][0], o2 = po[q][1];
+    const int co[2] = {o1 + o2, -(o1 + o2)};
+    const int ct[2] = {o1 - o2, -(o1 - o2)};
+    T sum = T(0), wq = T(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int nb = c + co[k];
+      const T vb = t.V[nb];
+      if (vb == vb) {
+        const T w = Da + t.D[3 + q][nb];
+        sum += w * (vb - Vc);
+        wq += w;
+      }
+      const int nm = c + ct[k];
+      const T vm = t.V[nm];
+      if (vm == vm) {
+        const T w = Da + t.D[3 + q][nm];
+        sum -= w * (vm - Vc);
+        wq -= w;
+      }
+    }
+    acc += T(0.25) * sum;
+    ws += T(0.25) * wq;
+  }
+  wsum = ws;
+  return acc;
+}
+
+// Voxel of a brick that lane `tid` computes after staging.  Default: the
+// slot order (z fastest).  8x4x4 bricks: each half-wave takes one y-row
+// (all x, all z): with the tile's x-stride of 36 = 4 (mod 32) its 32 LDS
+// addresses fall into 32 different banks, where the slot order leaves six
+// banks with two lanes each.
+template <int BX, int BY, int BZ>
+__device__ __forceinline__ void compute_voxel(int tid, int& lx, int& ly, int& lz) {
+  if (BX == 8 && BY == 4 && BZ == 4) {
+    ly = tid >> 5;
+    lx = (tid >> 2) & 7;
+    lz = tid & 3;
+  } else {
+    lz = tid % BZ;
+    ly = (tid / BZ) % BY;
+    lx = tid / (BY * BZ);
+  }
+}
+
 // stencil19 together with the sum of the 18 weights that enter it (the
 // negated diagonal of the operator row, in the same units): the reference's
 // diagonal is minus the sum of the signed weights of the node's edges

@@ -133,19 +133,28 @@ __global__ void __launch_bounds__(256) cn_react_kernel(
   const uint32_t sidx = state_index_wave(g.groups, slot >> 6);
   const uint8_t lab = g.label[slot];
   if (lab == 0xFF) return;
-  const uint32_t row_bytes = (uint32_t)ld * (uint32_t)sizeof(T);
-  T s[N_STATE];
-#pragma unroll
-  for (int k = 0; k < N_STATE; ++k)
-    s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
-  const int ct = lab & 3;
-  T gks, gto;
-  cell_conductances<T>(g, p, slot, ct, gks, gto);
   const T Vc = ld32(V, slot);
   const T dt = T(p.dt);
-  const T I = ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
+  T I;
+  // do_reaction 2 (multirate forms): dV/dt = rate_ext where it is a number,
+  // the full cell step where it is NaN (a fallback voxel)
+  const T rate = p.do_reaction == 2 ? ld32((const T*)p.rate_ext, slot) : T(0);
+  if (p.do_reaction == 2 && rate == rate) {
+    I = -rate;
+  } else {
+    const uint32_t row_bytes = (uint32_t)ld * (uint32_t)sizeof(T);
+    T s[N_STATE];
 #pragma unroll
-  for (int k = 0; k < N_STATE; ++k) buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx, s[k]);
+    for (int k = 0; k < N_STATE; ++k)
+      s[k] = buf_ld<T>(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx);
+    const int ct = lab & 3;
+    T gks, gto;
+    cell_conductances<T>(g, p, slot, ct, gks, gto);
+    I = ttp06_cell<T, exact>(Vc, s, dt, gks, gto, ct != 2, T(p.ki_base), T(p.nai_base));
+#pragma unroll
+    for (int k = 0; k < N_STATE; ++k)
+      buf_st(buf_rsrc(state + k * (size_t)ld, row_bytes), sidx, s[k]);
+  }
   int grp = (lab >> 2) & 31;
   if (grp && g.inst) grp = ((int)g.inst[slot] << 3) | (grp & 7);  // per-instance groups
   const T stim = grp ? ((const T*)p.group_amp)[grp] : T(0);
@@ -229,18 +238,27 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
    const uint32_t base = NB >> 3, rem = NB & 7;
    const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
    const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
+   int vx, vy, vz;  // the voxel this lane computes (staging uses the slot order)
+   compute_voxel<BX, BY, BZ>(tid, vx, vy, vz);
+   const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
+   const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
    for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
-    const uint32_t slot = b * BVOL + tid;
-    const uint8_t mylab = g.label[slot];
     __syncthreads();  // every lane is done with the previous tile
-    stage_cells<T, BX, BY, BZ>(t, g, x_in, b, mylab);
+    {
+      HaloNbr<BX, BY, BZ> hn;
+      halo_nbr<BX, BY, BZ>(hn, g, b);
+      HaloRegs<T, BX, BY, BZ> hr;
+      halo_load<T, BX, BY, BZ>(hr, hn, g, x_in, g.label[b * BVOL + tid]);
+      halo_store_nan<T, BX, BY, BZ>(t, hr);
+      __syncthreads();
+    }
+    const uint32_t slot = b * BVOL + vloc;
+    const uint8_t mylab = t.L[cc];
     if (mylab != 0xFF) {
-      const int lz = tid % BZ, ly = (tid / BZ) % BY, lx_ = tid / (BY * BZ);
-      const int cc = ((lx_ + 1) * TT::TY + (ly + 1)) * TT::TZ + (lz + 1);
       const T dscale = T(cp.dscale);
       const T xc = t.V[cc];
       T ws;
-      const T acc = stencil19_wsum<T, BX, BY, BZ>(t, cc, ws);
+      const T acc = stencil19_wsum_nan<T, BX, BY, BZ>(t, cc, ws);
       T lx = dscale * acc, ldg = dscale * ws;
       if ((mylab & 0x80) && j.n_targets > 0) {  // PMJ target voxel: its junction couplings
         const int q = cn_find_target(j.tgt_slot, (int)j.n_targets, (int)slot);
@@ -395,11 +413,13 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
                    T* state_pk, int64_t ld_pk, const hp_mono_params* p_pk, const hp_cn_pmj* j,
                    const hp_cn_params* cp, const hp_cn_work* w, void* stream) {
   const char* what = "hp_cn_step";
-  if (!g || !p || !cp || !w || !p->step_ctr || !p->group_amp || p->do_reaction != 1 ||
+  if (!g || !p || !cp || !w || !p->step_ctr || !p->group_amp ||
+      !(p->do_reaction == 1 || (p->do_reaction == 2 && p->rate_ext)) ||
       !g->groups || !g->label || !g->D6 || !g->nbr || !state || !V_in || !V_out || ld < 0 ||
       (uint64_t)ld * sizeof(T) >= (1ull << 32)) {
     set_error("%s: brick grid (groups, label, D6, nbr), V_in, V_out, state, params (step_ctr, "
-              "group_amp, do_reaction = 1), CN params and work vectors are required",
+              "group_amp, do_reaction 1, or 2 with rate_ext), CN params and work vectors are "
+              "required",
               what);
     return -1;
   }

@@ -347,8 +347,10 @@ class SurrogateRunner:
                  v_hi=-40.0, n_blocks=0, compact=True, graph_macros=2, variant=1):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
-        if eng.scheme != "explicit":
-            raise ValueError("the multirate forms substep the explicit engine (scheme='explicit')")
+        # substeps: the engine's own diffusion scheme, the reaction from the rate
+        # array (CN: the reaction pass runs the cell model for the NaN-rate
+        # fallback voxels itself; explicit: compacted fallback list + the
+        # diffusion-only fused kernel)
         if mode not in ("mlp", "bigstep"):
             raise ValueError(mode)
         if mode == "mlp" and net is None:
@@ -358,8 +360,8 @@ class SurrogateRunner:
                                device=eng.dev)
         self.V_prev = torch.full((eng.lay.n_slots,), float("nan"), dtype=torch.float32,
                                  device=eng.dev)
-        self.compact = bool(compact)
-        self.fb_list = torch.zeros(eng.lay.n_slots if compact else 1, dtype=torch.int32,
+        self.compact = bool(compact) and eng.scheme == "explicit"
+        self.fb_list = torch.zeros(eng.lay.n_slots if self.compact else 1, dtype=torch.int32,
                                    device=eng.dev)
         self.fb_count = torch.zeros(1, dtype=torch.int32, device=eng.dev)
         # the inference kernel visits the tissue slots only (padding of partly
@@ -375,8 +377,9 @@ class SurrogateRunner:
         # inference kernel shape: variant 1 (two 16-voxel MFMA columns per wave,
         # 512 threads) measured 17 % faster than variant 0 on the 2 M-voxel BiV
         self.sp = nat.HpSurrogateParams(T_MACRO, dvdt_max, v_lo, v_hi, eng.ki_base, eng.nai_base,
-                                        0, int(variant), nat.ptr(self.fb_list) if compact else None,
-                                        nat.ptr(self.fb_count) if compact else None,
+                                        0, int(variant),
+                                        nat.ptr(self.fb_list) if self.compact else None,
+                                        nat.ptr(self.fb_count) if self.compact else None,
                                         nat.ptr(self.slot_list), self.slot_list.numel(),
                                         nat.ptr(self.state_list))
         self.n_blocks = n_blocks
@@ -406,7 +409,7 @@ class SurrogateRunner:
                      nat.ptr(eng.state), eng.state_ld, nat.ptr(self.rate),
                      nat.ptr(self.fb_list), nat.ptr(self.fb_count), eng.lay.n_slots,
                      C.byref(eng.p_myo), nat.stream_handle())
-        eng.p_myo.do_reaction = 0 if self.compact else 2
+        eng.p_myo.do_reaction = 0 if self.compact else 2  # CN engines: always 2
         eng.p_myo.rate_ext = nat.ptr(self.rate)
         try:
             eng._launch_step()

@@ -334,8 +334,10 @@ typedef struct hp_cn_work {
 
 /* p / p_pk: dt, inv_cm, ki/nai bases, instance scales, group_amp (stimulus of
  * this step), step_ctr, vmin/vmax, at (first activation times, optional) and
- * at_thr, as for hp_myo_step; do_reaction must be 1.  c may be NULL (no
- * cable); j may be NULL (no junctions).  Reads V_in / Vpk_in (V^n), writes
+ * at_thr, as for hp_myo_step; do_reaction 1 (the cell model everywhere) or 2
+ * (the multirate forms: dV/dt = rate_ext where it is a number, the full cell
+ * step where it is NaN; the cable always runs the cell model).  c may be NULL
+ * (no cable); j may be NULL (no junctions).  Reads V_in / Vpk_in (V^n), writes
  * V_out / Vpk_out (V^{n+1}), updates the cell states in place. */
 int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* state, int64_t ld,
                    const hp_mono_params* p, const hp_cable* c, const float* Vpk_in,
