@@ -15,6 +15,7 @@ Einthoven/Goldberger/Wilson definitions.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 import time
 
 import numpy as np
@@ -251,7 +252,10 @@ class PoissonSolver:
     def solve_refined(self, b, x=None, tol=1e-8, inner_tol=1e-4, max_outer=30, max_inner=20000,
                       check_every=32):
         """Solve A x = b to ||b - A x|| <= tol ||b|| (FP64 residual) by FP32 PCG
-        corrections.  b, x: FP64 device vectors in cell order; x is updated in place."""
+        corrections.  b, x: FP64 device vectors in cell order; x is updated in place.
+        Each outer step solves A d = r/||r|| to inner_tol relative to that
+        (current) residual.  last_stats["converged"] says whether tol was met;
+        a RuntimeWarning says so when max_outer ends the loop first."""
         x = torch.zeros(self.n, dtype=torch.float64, device=self.dev) if x is None else x
         inner_total, outer = 0, 0
         rn, bn = self.residual(b, x)
@@ -264,7 +268,13 @@ class PoissonSolver:
             inner_total += it
             outer += 1
             rn, bn = self.residual(b, x)
-        self.last_stats = dict(outer=outer, inner=inner_total, rel_res=rn / bn if bn else 0.0)
+        conv = not (bn > 0 and rn > tol * bn)
+        self.last_stats = dict(outer=outer, inner=inner_total, rel_res=rn / bn if bn else 0.0,
+                               converged=conv)
+        if not conv:
+            warnings.warn(f"solve_refined: relative residual {rn / bn:.3e} > tol {tol:g} after "
+                          f"{outer} outer steps ({inner_total} inner iterations)", RuntimeWarning,
+                          stacklevel=2)
         return x
 
     METRICS = {"rms": 1, "linf": 2, "p99": 3}

@@ -124,7 +124,15 @@ class Ensemble:
     """
 
     def __init__(self, specs, *, geoms=None, dt=0.02, dtype=torch.float32, device="cuda",
-                 brick=None, A_M=140.0, Cm=0.01, phi_endo_max=1 / 3, phi_epi_min=2 / 3):
+                 brick=None, A_M=140.0, Cm=0.01, phi_endo_max=1 / 3, phi_epi_min=2 / 3,
+                 scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
+        """scheme "cn": the reference's Crank-Nicolson step for every instance.
+        The reference solves one instance per call, each to ||r|| <= cg_tol
+        ||b|| of its own system; the batch is stopped on its whole residual, so
+        the batch tolerance is cg_tol * sqrt(smallest instance / all voxels),
+        which bounds every instance's own relative residual by about cg_tol
+        (its ||b|| scales with the square root of its voxel count at the
+        voltages of a beat)."""
         t0 = time.perf_counter()
         self.specs = list(specs)
         geoms = geoms if geoms is not None else instance_geometries(self.specs)
@@ -156,10 +164,13 @@ class Ensemble:
         inst_params[:, 0] = [s.gks_scale for s in self.specs]
         inst_params[:, 1] = [s.gto_scale for s in self.specs]
         self.sites = sites
+        sizes = np.diff(self.node_offset)
+        self.batch_tol = float(cg_tol * np.sqrt(sizes.min() / max(1, sizes.sum())))
         self.eng = MonodomainEngine(
             self.layout, np.concatenate(D6s, axis=1), np.concatenate(cts), h=H_REF, A_M=A_M,
             Cm=Cm, dt=dt, dtype=dtype, device=device, stim_sites=sites,
-            inst=np.concatenate(insts), inst_params=inst_params)
+            inst=np.concatenate(insts), inst_params=inst_params, scheme=scheme, theta=theta,
+            cg_tol=self.batch_tol, max_sweeps=max_sweeps)
         self.t_setup = time.perf_counter() - t0
 
     @property

@@ -34,8 +34,8 @@ def _instances():
     return specs, geoms
 
 
-def _oracle_instance(spec, geom, site_ids, T):
-    """The instance alone through the oracle (explicit scheme, FP64)."""
+def _oracle_instance(spec, geom, site_ids, T, scheme="explicit"):
+    """The instance alone through the oracle (FP64)."""
     S, phi, f, h = geom
     nodes, el, act = stub_tree(S, h)
     vox = np.argwhere(np.asarray(S).astype(bool))
@@ -52,7 +52,7 @@ def _oracle_instance(spec, geom, site_ids, T):
                       phi_epi_min=2 / 3)
     gks = np.concatenate([np.ones(prob.Np), np.full(prob.N_myo, spec.gks_scale)])
     gto = np.concatenate([np.ones(prob.Np), np.full(prob.N_myo, spec.gto_scale)])
-    r = OM.run(prob, T, 0.02, scheme="explicit", g_ks_scale=gks, g_to_scale=gto)
+    r = OM.run(prob, T, 0.02, scheme=scheme, g_ks_scale=gks, g_to_scale=gto)
     return r["V"][prob.Np:], r["at"][prob.Np:]
 
 
@@ -78,6 +78,36 @@ def test_ensemble_f64_matches_oracle(cuda):
         assert np.array_equal(np.isfinite(ato), np.isfinite(ats[i]))
         assert both.sum() > 0
         assert err_v < 1e-7 and err_at < 1e-7
+
+
+def test_ensemble_cn_matches_oracle_per_instance(cuda):
+    """scheme "cn": the batch, stopped on its whole residual at the batch
+    tolerance, against each instance alone through the oracle's CN step
+    (BiCGSTAB to 1e-5 of that instance's own system), FP32 and FP64."""
+    specs, geoms = _instances()
+    T = 24.0
+    ref = None
+    for dtype in (torch.float64, torch.float32):
+        print(f"[case] ensemble cn {dtype} vs oracle", flush=True)
+        ens = E.Ensemble(specs, geoms=geoms, dtype=dtype, device=cuda, brick=(4, 4, 4),
+                         scheme="cn")
+        ens.run(int(round(T / 0.02)), graph_chunk=50)
+        st = ens.eng.cn_stats()
+        print(f"  batch tol {ens.batch_tol:.2e}: {st}")
+        assert st["not_converged"] == 0
+        Vs, ats = ens.V_instances(), ens.at_instances()
+        if ref is None:
+            ref = [_oracle_instance(spec, geom, _site_ids(ens, i), T, scheme="cn")
+                   for i, (spec, geom) in enumerate(zip(specs, geoms))]
+        for i, (Vo, ato) in enumerate(ref):
+            both = np.isfinite(ato) & np.isfinite(ats[i])
+            rms = float(np.sqrt(np.mean((Vs[i] - Vo) ** 2)))
+            err_at = float(np.max(np.abs(ats[i][both] - ato[both])))
+            print(f"  instance {i} ({specs[i].kind}, {len(Vo)} voxels): Vm RMS {rms:.2e} mV, "
+                  f"max |dAT| {err_at:.2e} ms, activated {both.sum()}")
+            assert np.array_equal(np.isfinite(ato), np.isfinite(ats[i]))
+            assert both.sum() > 0
+            assert rms < 0.5 and err_at < 0.5
 
 
 def test_ensemble_f32_equals_instances_alone(cuda):
