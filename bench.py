@@ -14,22 +14,24 @@ whole coupled system (myocardium + Purkinje + PMJ + stimulus) and one ECG
 frame; --steps K times exactly K of them.  Forms, each run from t = 0 over the
 full 600 ms beat (the timed window is part of that run):
   plain     the reference's scheme in FP32: TTP06 reaction, then Crank-Nicolson
-            diffusion of the coupled system solved to cg_tol = 1e-5 (hp_cn_step)
+            diffusion of the coupled system solved by the reference's
+            Jacobi-preconditioned BiCGSTAB to cg_tol = 1e-5 (hp_cn_step)
+  jacobi    the same CN system solved by Jacobi sweeps to the same residual
   explicit  the fast form: forward-Euler diffusion fused with the reaction
   bigstep / surrogate  the multirate forms (one 0.2 ms reaction per
             macro-step by the exact model / the learned network, plain
             integration where the upstroke indicator fires)
-  ref64     the reference scheme in FP64 (the oracle's algorithm, equal to it to
-            ~1e-12 mV on the test problems): the gates' reference
+  ref64     the reference scheme in FP64 (the oracle's algorithm, equal to it at
+            round-off level on the test problems): the gates' reference
 Every form's activation times and 12 leads over the full beat are compared
 with ref64 (AT error <= 1 ms, lead correlation >= 0.999).  The headline value
 is the surrogate form's throughput if it meets both gates, else the plain
 form's.
 
 Prints one JSON line with the throughput, a roofline object for the dominant
-kernel of the plain form (the CN Jacobi sweep: 19-point stencil, measured
-alone with HIP events on its stream) and a CPU baseline (the oracle, CN +
-BiCGSTAB, timed on a bounded number of steps of the same workload).
+kernel of the plain form (the BiCGSTAB product pass: 19-point stencil,
+measured alone with HIP events on its stream) and a CPU baseline (the oracle,
+CN + BiCGSTAB, timed on a bounded number of steps of the same workload).
 """
 from __future__ import annotations
 
@@ -47,14 +49,18 @@ sys.path.insert(0, HERE)
 
 METRIC = "voxel-steps/s, 2M-voxel BiV: surrogate vs plain FP32 at AT err≤1ms, ECG r≥.999"
 HBM_PEAK_GBS = 8000.0
-# CN Jacobi sweep, per myocardial voxel: x read 4, D tensor 24, label 1, b read 4,
-# x written 4 (the first sweep of a step writes b instead of reading it)
-SWEEP_BYTES_PER_VOXEL = 37
+# BiCGSTAB product pass t = A M^-1 (r - alpha v), per myocardial voxel: r, v,
+# M^-1 read 12, D tensor 24, label 1, t written 4
+APPLY_BYTES_PER_VOXEL = 41
 # CN reaction pass, per voxel: 18 state variables read and written 144, V read 4,
 # V* written 4, label 1
 REACT_BYTES_PER_VOXEL = 153
 STEPS_PER_MS = 50
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "cn_sweep_biv256_traffic.json")
+# CN solve per step captured in the step's graph: BiCGSTAB iterations (the
+# reference's solver) / Jacobi sweeps; the ones after the solve ended return
+# at once
+CN_MAX_ITERS = {"bicgstab": 6, "jacobi": 5}
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "cn_apply_biv256_traffic.json")
 
 
 def measured_traffic():
@@ -83,14 +89,15 @@ def build_geometry(n=256, h=0.4, seed=0):
 
 
 def build_engine(geo, tree, kw, dtype=None, brick=None, alloc="torch", scheme="cn",
-                 max_sweeps=5):
+                 max_sweeps=None, cn_solver="bicgstab"):
     import torch
     from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
         build_coupled_system)
     nodes, el, act = tree
     return build_coupled_system(nodes, el, act, geo["S"], geo["phi"], geo["fibre_dir"], dt=0.02,
                                 dtype=dtype or torch.float32, device="cuda", brick=brick,
-                                alloc=alloc, scheme=scheme, max_sweeps=max_sweeps, **kw)
+                                alloc=alloc, scheme=scheme, cn_solver=cn_solver,
+                                max_sweeps=max_sweeps or CN_MAX_ITERS[cn_solver], **kw)
 
 
 def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None, scheme="cn"):
@@ -100,7 +107,12 @@ def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None, scheme="cn"):
 
 
 FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
+    # the reference's step: CN with its BiCGSTAB, in FP64 (the gates'
+    # reference) and FP32 (the plain form)
     "ref64": ("cn", "float64", None, {}), "plain": ("cn", "float32", None, {}),
+    # the same CN system solved by Jacobi sweeps (the exact CN solution to
+    # well within the tolerance)
+    "jacobi": ("cn", "float32", None, dict(cn_solver="jacobi")),
     "explicit": ("explicit", "float32", None, {}),
     # multirate forms: one reaction per 0.2 ms macro-step (exact big step /
     # network), ten substeps of the reference's CN diffusion driven by the rate
@@ -112,9 +124,9 @@ FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
     "surrogate_explicit": ("explicit", "float32", "mlp", {}),
     # the CN step with theta = 0 and one sweep: V^{n+1} = V* + (dt/Cm) L V*
     # exactly (forward Euler after the reaction, the reference's Lie order)
-    "lie0": ("cn", "float32", None, dict(theta=0.0, max_sweeps=1)),
-    # the reference's CN with at most two sweeps per step
-    "cn2": ("cn", "float32", None, dict(max_sweeps=2))}
+    "lie0": ("cn", "float32", None, dict(theta=0.0, max_sweeps=1, cn_solver="jacobi")),
+    # Jacobi sweeps, at most two per step
+    "cn2": ("cn", "float32", None, dict(max_sweeps=2, cn_solver="jacobi"))}
 
 
 def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, sample_slots,
@@ -127,9 +139,10 @@ def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, s
     from hrdayapy_amd import surrogate as SG
     from hrdayapy_amd import workloads as W
     scheme, dname, rule, opts = FORM_SETUP[form]
-    kwf = dict(kw, **{k: v for k, v in opts.items() if k != "max_sweeps"})
+    kwf = dict(kw, **{k: v for k, v in opts.items() if k not in ("max_sweeps", "cn_solver")})
     eng, meta = build_engine(geo, tree, kwf, dtype=getattr(torch, dname), brick=brick,
-                             scheme=scheme, max_sweeps=opts.get("max_sweeps", 5))
+                             scheme=scheme, max_sweeps=opts.get("max_sweeps"),
+                             cn_solver=opts.get("cn_solver", "bicgstab"))
     run = SG.SurrogateRunner(eng, net, mode=rule, graph_macros=5) if rule else None
     advance = (lambda: run.run(STEPS_PER_MS // SG.N_SUB)) if run else (
         lambda: eng.run(STEPS_PER_MS, graph_chunk=STEPS_PER_MS))
@@ -206,21 +219,21 @@ def stream_copy_bandwidth(mib, reps=20):
 
 
 def time_cn_kernels(eng, reps=100):
-    """Device time (us) of the CN step's dominant kernel -- one Jacobi sweep of
-    the myocardial rows (hp_cn_sweep_f32, the 19-point stencil) -- and of the
-    whole step, each replayed from a HIP graph between HIP events on the
-    replaying stream."""
+    """Device time (us) of the CN step's dominant kernel -- one BiCGSTAB product
+    pass t = A M^-1 (r - alpha v) over the myocardial rows (hp_cn_apply_f32,
+    the 19-point stencil) -- and of the whole step, each replayed from a HIP
+    graph between HIP events on the replaying stream."""
+    import torch
     from hrdayapy_amd import _native as nat
     from hrdayapy_amd import workloads as W
-    Vs, bvec, x = eng.cn_work_t[:3]
-
-    def sweep(i):
-        src, dst = (Vs, x) if i % 2 == 0 else (x, Vs)
-        nat.call("hp_cn_sweep_f32", C.byref(eng.bricks), nat.ptr(src), nat.ptr(dst),
-                 nat.ptr(bvec), C.byref(eng.cn), nat.stream_handle())
-    sweep_us = W.graph_us(sweep, reps)
+    sc = torch.zeros(16, dtype=torch.float64, device=eng.dev)  # done = 0, alpha = 0
+    cp = nat.HpCnParams.from_buffer_copy(eng.cn)
+    cp.scalars = nat.ptr(sc)
+    apply_us = W.graph_us(lambda i: nat.call("hp_cn_apply_f32", C.byref(eng.bricks),
+                                             C.byref(cp), C.byref(eng.cn_work),
+                                             nat.stream_handle()), reps)
     step_us = W.graph_us(lambda i: eng._launch_step(), reps)
-    return sweep_us, step_us
+    return apply_us, step_us
 
 
 def cpu_info():
@@ -392,7 +405,7 @@ def main():
                     help="length of every form's run (the gates compare the whole run)")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x4x4, or auto")
-    ap.add_argument("--forms", type=str, default="plain,explicit,bigstep,surrogate")
+    ap.add_argument("--forms", type=str, default="plain,jacobi,explicit,bigstep,surrogate")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
     ap.add_argument("--train-epochs", type=int, default=14)
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -450,11 +463,11 @@ def main():
         f"geo {t_geo:.1f}s setup {meta['setup_s']:.1f}s")
     ecg = W.EcgSetup(eng0, meta, n_bspm=a.bspm, log=log) if a.ecg == "leadfield" else None
     # dominant kernel of the plain form and its whole step (device time)
-    sweep_us, step_us = time_cn_kernels(eng0, a.kernel_reps)
-    sweep_bytes = SWEEP_BYTES_PER_VOXEL * n_myo
-    achieved = sweep_bytes / (sweep_us * 1e-6) / 1e9
-    log(f"[bench] CN Jacobi sweep {sweep_us:.1f} us/launch -> {achieved:.0f} GB/s algorithmic "
-        f"({sweep_bytes / 1e6:.1f} MB); CN step replayed {step_us:.1f} us")
+    apply_us, step_us = time_cn_kernels(eng0, a.kernel_reps)
+    apply_bytes = APPLY_BYTES_PER_VOXEL * n_myo
+    achieved = apply_bytes / (apply_us * 1e-6) / 1e9
+    log(f"[bench] CN BiCGSTAB product pass {apply_us:.1f} us/launch -> {achieved:.0f} GB/s "
+        f"algorithmic ({apply_bytes / 1e6:.1f} MB); CN step replayed {step_us:.1f} us")
     ecg_info = None
     if ecg:
         rec0 = W.EcgRecorder(ecg.lf, eng0)
@@ -550,10 +563,11 @@ def main():
         "end_to_end": e2e,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "cn_sweep_kernel<float> (plain form: one Jacobi sweep of the CN "
-                               "solve, the 19-point stencil; 3 per solver step)",
-                     "kernel_us": sweep_us, "step_us": step_us, "bytes_per_launch": sweep_bytes,
-                     "bytes_per_voxel": SWEEP_BYTES_PER_VOXEL},
+                     "kernel": "cn_bi_pass_kernel<float,8,4,4,S> (plain form: the BiCGSTAB "
+                               "product pass t = A M^-1 s of the CN solve, the 19-point "
+                               "stencil; with the A pass, two per iteration)",
+                     "kernel_us": apply_us, "step_us": step_us, "bytes_per_launch": apply_bytes,
+                     "bytes_per_voxel": APPLY_BYTES_PER_VOXEL},
         "cpu_baseline": cpu,
         "stream_copy": copy,
         "bench_wall_s": round(time.perf_counter() - t_start, 1),

@@ -14,7 +14,7 @@ from pathlib import Path
 _LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
 _lib = None
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
 
@@ -55,13 +55,13 @@ class HpCnPmj(C.Structure):
 
 class HpCnParams(C.Structure):
     _fields_ = [("cm_dt", c_f64), ("theta", c_f64), ("dscale", c_f64), ("dscale_pk", c_f64),
-                ("tol", c_f64), ("max_sweeps", c_i32), ("pad_", c_i32), ("scalars", c_vp),
+                ("tol", c_f64), ("max_sweeps", c_i32), ("solver", c_i32), ("scalars", c_vp),
                 ("part", c_vp), ("part_len", c_i64)]
 
 
 class HpCnWork(C.Structure):
     _fields_ = [("Vs", c_vp), ("b", c_vp), ("x", c_vp), ("Vs_pk", c_vp), ("b_pk", c_vp),
-                ("x_pk", c_vp)]
+                ("x_pk", c_vp), ("kr", c_vp * 8), ("kr_pk", c_vp * 8)]
 
 
 class HpMlpWeights(C.Structure):
@@ -135,6 +135,7 @@ _SIGS = {
                                  P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
                                  P(HpCnParams), P(HpCnWork), c_vp]),
     "hp_cn_sweep_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, P(HpCnParams), c_vp]),
+    "hp_cn_apply_f32": (C.c_int, [P(HpBricks), P(HpCnParams), P(HpCnWork), c_vp]),
     "hp_cn_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                  P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
                                  P(HpCnParams), P(HpCnWork), c_vp]),

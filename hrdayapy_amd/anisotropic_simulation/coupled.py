@@ -25,7 +25,7 @@ def compute_anisotropic_coupled(nodes, elements, act_times, S, Z, phi, fibre_dir
                                 anim_tmp_dir=None, device=None, phi_endo_max=0.35,
                                 phi_epi_min=0.65, save_path=None, vm_save_dt=None,
                                 vm_save_path=None, vm_tmp_dir=None, dtype=torch.float32,
-                                scheme="cn"):
+                                scheme="cn", cn_solver="bicgstab"):
     voxel_size = resolve_voxel_size(voxel_size, anatomy, param_name="voxel_size")
     return purkinje_myocardium_anisotropic_solver(
         nodes=nodes, elements=elements, activation_times=act_times, S=S, Z=Z, phi=phi,
@@ -37,7 +37,7 @@ def compute_anisotropic_coupled(nodes, elements, act_times, S, Z, phi, fibre_dir
         anim_tmp_dir=anim_tmp_dir, device=device, out_npz=save_path,
         phi_endo_max=phi_endo_max, phi_epi_min=phi_epi_min, vm_save_dt=vm_save_dt,
         out_vm_npz=vm_save_path, vm_tmp_dir=vm_tmp_dir, dtype=dtype,
-        scheme=scheme)
+        scheme=scheme, cn_solver=cn_solver)
 
 
 def load_anisotropic_coupled(path):

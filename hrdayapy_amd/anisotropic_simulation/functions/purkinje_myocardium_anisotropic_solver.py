@@ -11,10 +11,15 @@ HIP graphs.  Two schemes:
 
 * ``scheme="cn"`` (default): the reference's step -- Lie splitting, then
   Crank-Nicolson diffusion with ``theta`` on the whole coupled system (PMJ
-  couplings included), solved to ``||b - A x|| <= cg_tol*||b||`` (:1746-1791).
-  The reference's BiCGSTAB is replaced by matrix-free Jacobi sweeps with the
-  same stopping rule (hp_cn_step); at most min(cg_max_iter, 12) sweeps per
-  step, and a warning names the steps that stopped at that bound.
+  couplings included), solved by the reference's Jacobi-preconditioned
+  BiCGSTAB with its stopping rules at ``cg_tol`` (:786-849, :1746-1791),
+  matrix-free on the device (hp_cn_step, solver 1); the iterations launched
+  per step are bounded by cg_max_iter and fitted to the largest count any step
+  has needed (plus two) as the run goes, and a warning names the steps that
+  stopped at the bound.  ``cn_solver="jacobi"`` solves the same system by
+  Jacobi sweeps to the same relative residual (at most min(cg_max_iter, 12)):
+  the exact CN solution to well within the tolerance, which differs from the
+  reference's by the error its BiCGSTAB leaves at the fronts.
 * ``scheme="explicit"``: the fast form -- one fused kernel per step (TTP06 +
   forward-Euler 19-point diffusion on V^n + stimulus + clamp + activation),
   the cable in the same launch, the PMJ exchange integrated exactly; ``dt``
@@ -37,9 +42,25 @@ from ...engine import MonodomainEngine
 from ...layout import BrickLayout, choose_brick
 
 
-# sweeps of the CN solve per step captured in the step's graph (each step
-# launches them all; the ones after convergence return at once)
-CN_MAX_SWEEPS = 12
+# BiCGSTAB iterations / Jacobi sweeps of the CN solve per step captured in the
+# step's graph at the start (each step launches them all; the ones after the
+# solve ended return at once).  After every output chunk the bound is fitted to
+# the largest count any step has needed plus CN_ITER_HEADROOM, and doubled
+# when a step reached it without meeting the tolerance; never above
+# cg_max_iter.
+CN_MAX_ITERS = {"bicgstab": 16, "jacobi": 12}
+CN_ITER_HEADROOM = 2
+
+
+def _fit_cn_bound(eng, cap, seen):
+    """Refit the engine's per-step iteration bound from its solver statistics
+    (one host synchronisation); seen holds the non-converged count so far."""
+    st = eng.cn_stats()
+    if st["not_converged"] > seen[0]:
+        seen[0] = st["not_converged"]
+        eng.set_cn_max_iters(min(cap, 2 * st["max_sweeps"]))
+    else:
+        eng.set_cn_max_iters(min(cap, max(st["max_used"] + CN_ITER_HEADROOM, 2)))
 
 
 def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=None, *,
@@ -50,7 +71,8 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
                          phi_endo_max=0.35, phi_epi_min=0.65, dtype=torch.float32,
                          device="cuda", brick=None, verbose=False, sigma_scale=None,
                          inst=None, inst_params=None, auto_substep=False, alloc="torch",
-                         scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
+                         scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10,
+                         cn_solver="bicgstab"):
     """Set up the tissue on the host and upload it into a MonodomainEngine.
 
     ``sigma_M`` selects the isotropic 6-point operator (simulation.compute_coupled);
@@ -93,7 +115,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
         cable=dict(n=Np, edges=comp_edges, w_pp=pc["w_pp"], Ai_P=pc["Ai_P"]),
         pmj=dict(ids=pmj_ids, patches=patches, c_pmj=c_pmj, Ai_P=pc["Ai_P"], Ai_M=pc["Ai_M"]),
         stim_sites=sites, inst=inst, inst_params=inst_params, alloc=alloc, scheme=scheme,
-        theta=theta, cg_tol=cg_tol, max_sweeps=max_sweeps)
+        theta=theta, cg_tol=cg_tol, max_sweeps=max_sweeps, cn_solver=cn_solver)
     meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim, scheme=scheme,
                 comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,
@@ -125,15 +147,17 @@ def purkinje_myocardium_anisotropic_solver(
         stim_regions=None, stim_len_mm=2.0, c_pmj=0.05, n_pmj=1, cg_tol=1e-5, cg_max_iter=100,
         n_frames=100, frame_save_dt=None, anim_tmp_dir=None, device=None, dtype=torch.float32,
         out_npz=None, phi_endo_max=0.35, phi_epi_min=0.65, vm_save_dt=None, out_vm_npz=None,
-        vm_tmp_dir=None, graph_chunk=50, scheme="cn", _sigma_M=None):
+        vm_tmp_dir=None, graph_chunk=50, scheme="cn", cn_solver="bicgstab", _sigma_M=None):
     """Run the coupled simulation; see the module docstring for what differs from the reference."""
     if scheme not in ("cn", "explicit"):
         raise ValueError(f"scheme must be 'cn' or 'explicit' (got {scheme!r})")
     if scheme == "explicit" and (theta != 0.5 or cg_tol != 1e-5):
         warnings.warn("scheme='explicit' integrates diffusion by forward Euler: theta and cg_tol "
                       "apply to scheme='cn' only and are not used", stacklevel=2)
-    if scheme == "cn" and (not 0.0 <= theta <= 1.0 or cg_tol < 0 or int(cg_max_iter) < 1):
-        raise ValueError("scheme='cn' needs 0 <= theta <= 1, cg_tol >= 0 and cg_max_iter >= 1")
+    if scheme == "cn" and (not 0.0 <= theta <= 1.0 or cg_tol < 0 or int(cg_max_iter) < 1
+                           or cn_solver not in CN_MAX_ITERS):
+        raise ValueError("scheme='cn' needs 0 <= theta <= 1, cg_tol >= 0, cg_max_iter >= 1 and "
+                         "cn_solver 'bicgstab' or 'jacobi'")
     if A_P is not None and _sigma_M is None:
         raise TypeError(
             "A_P is no longer a settable parameter -- it's mechanically determined by R_P "
@@ -150,8 +174,8 @@ def purkinje_myocardium_anisotropic_solver(
         stim_protocol=stim_protocol, ectopic_region=ectopic_region, stim_regions=stim_regions,
         stim_len_mm=stim_len_mm, c_pmj=c_pmj, n_pmj=n_pmj, phi_endo_max=phi_endo_max,
         phi_epi_min=phi_epi_min, dtype=dtype, device=dev, verbose=True, auto_substep=True,
-        scheme=scheme, theta=theta, cg_tol=cg_tol,
-        max_sweeps=min(int(cg_max_iter), CN_MAX_SWEEPS))
+        scheme=scheme, theta=theta, cg_tol=cg_tol, cn_solver=cn_solver,
+        max_sweeps=min(int(cg_max_iter), CN_MAX_ITERS.get(cn_solver, 1)))
     Np, N_myo, n_sub = meta["Np"], meta["N_myo"], meta["n_sub"]
     if n_sub > 1:
         print(f"[hrdaya] dt={dt} ms is above the explicit diffusion limit "
@@ -180,12 +204,15 @@ def purkinje_myocardium_anisotropic_solver(
     cadence = [c for c in (save_every, vm_every) if c]
     t0 = time.perf_counter()
     step = 0
+    cn_seen = [0]
     while step < n_steps:
         # output after step `step` is due when step % every == 0 (frame stamped step*dt)
         nxt = min([((step // c) + (1 if step % c else 0)) * c for c in cadence] + [n_steps - 1])
         nxt = max(nxt, step)
         k = (nxt - step + 1) * n_sub
         eng.run(k, graph_chunk=graph_chunk if k >= graph_chunk else 0)
+        if scheme == "cn":
+            _fit_cn_bound(eng, int(cg_max_iter), cn_seen)
         t = nxt * dt
         want_anim, want_vm = nxt % save_every == 0, run_vm and nxt % vm_every == 0
         if want_anim or want_vm:
@@ -205,11 +232,12 @@ def purkinje_myocardium_anisotropic_solver(
           f"({n_steps * n_sub * N_myo / max(wall, 1e-9):.3e} voxel-steps/s)")
     cn = eng.cn_stats()
     if cn is not None:
-        print(f"[hrdaya] CN: {cn['mean_sweeps']:.2f} Jacobi sweeps per step on average "
-              f"(tolerance {cn['tol']:g}, at most {cn['max_sweeps']})")
+        what = "BiCGSTAB iterations" if cn["solver"] == "bicgstab" else "Jacobi sweeps"
+        print(f"[hrdaya] CN: {cn['mean_sweeps']:.2f} {what} per step on average, "
+              f"{cn['max_used']} at most (tolerance {cn['tol']:g}, bound {cn['max_sweeps']})")
         if cn["not_converged"]:
             warnings.warn(f"CN solve: {cn['not_converged']} of {cn['steps']} steps stopped at "
-                          f"{cn['max_sweeps']} sweeps with relative residual up to "
+                          f"the bound on {what} with relative residual up to "
                           f"{cn['worst_rel_res']:.2e} > cg_tol={cn['tol']:g}", RuntimeWarning,
                           stacklevel=2)
     if run_vm and vm_times:

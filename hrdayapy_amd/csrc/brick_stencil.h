@@ -191,14 +191,15 @@ struct HaloRegs {
   T v[NIT], d[NIT][3], dself[3];
 };
 
-// mylab: the lane's own label in brick n (0xFF: padding voxel, loads nothing)
-template <typename T, int BX, int BY, int BZ>
-__device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const HaloNbr<BX, BY, BZ>& n,
-                                          const hp_bricks& g, const T* __restrict__ V,
-                                          uint8_t mylab) {
+// mylab: the lane's own label in brick n (0xFF: padding voxel, loads nothing).
+// value(q) gives the tile value of slot q (q = OOB for cells without a
+// brick: it must then read out of range only); halo_load stages V itself.
+template <typename T, int BX, int BY, int BZ, typename F>
+__device__ __forceinline__ void halo_load_fn(HaloRegs<T, BX, BY, BZ>& h,
+                                             const HaloNbr<BX, BY, BZ>& n, const hp_bricks& g,
+                                             uint8_t mylab, F&& value) {
   constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT, BVOL = BX * BY * BZ;
   const uint32_t n_slots = (uint32_t)g.n_bricks * BVOL, ld = n_slots;  // D6 row stride
-  const buf_t rV = buf_rsrc(V, n_slots * (uint32_t)sizeof(T));
   const buf_t rD = buf_rsrc(g.D6, 6u * ld * (uint32_t)sizeof(T));
   const buf_t rL = buf_rsrc(g.label, n_slots);
 #pragma unroll
@@ -211,7 +212,7 @@ __device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const Halo
   for (int it = 0; it < NIT; ++it) {
     const uint32_t q = h.q[it];
     h.lab[it] = buf_ld_u8(rL, q);
-    h.v[it] = buf_ld<T>(rV, q);
+    h.v[it] = value(q);
     if (it == 0) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) h.d[0][k] = buf_ld<T>(rD, q == OOB ? OOB : k * ld + q);
@@ -226,6 +227,14 @@ __device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const Halo
       h.d[it][2] = buf_ld<T>(rD, face ? c2 * ld + q : OOB);  // off-diagonals of its planes
     }
   }
+}
+
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ void halo_load(HaloRegs<T, BX, BY, BZ>& h, const HaloNbr<BX, BY, BZ>& n,
+                                          const hp_bricks& g, const T* __restrict__ V,
+                                          uint8_t mylab) {
+  const buf_t rV = buf_rsrc(V, (uint32_t)g.n_bricks * (BX * BY * BZ) * (uint32_t)sizeof(T));
+  halo_load_fn<T, BX, BY, BZ>(h, n, g, mylab, [&](uint32_t q) { return buf_ld<T>(rV, q); });
 }
 
 // Registers -> LDS tile; lanes without a tile cell write the spare cell N.

@@ -47,8 +47,9 @@ constexpr int CN_FIN_THREADS = 1024;
 
 // scalars[]: 0 ||b||^2, 1 ||r||^2 of the last sweep, 2 done, 3 sweeps this step,
 // 4 steps that stopped at max_sweeps, 5 largest ||r||/||b|| of those, 6 total
-// sweeps, 7 steps solved
-enum { SC_BB = 0, SC_RR, SC_DONE, SC_SWEEPS, SC_NCONV, SC_WORST, SC_TOTAL, SC_STEPS };
+// sweeps, 7 steps solved, 16 the largest sweep count of any step
+enum { SC_BB = 0, SC_RR, SC_DONE, SC_SWEEPS, SC_NCONV, SC_WORST, SC_TOTAL, SC_STEPS,
+       SC_MAXIT = 16 };
 
 __device__ __forceinline__ double cn_wave_sum(double v) {
 #pragma unroll
@@ -350,6 +351,7 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
   if (done) {
     sc[SC_TOTAL] += sweeps;
     sc[SC_STEPS] += 1.0;
+    if (sweeps > sc[SC_MAXIT]) sc[SC_MAXIT] = sweeps;
     if (!conv) {
       sc[SC_NCONV] += 1.0;
       const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
@@ -359,15 +361,15 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
 }
 
 // V^{n+1} = clamp(x) from the buffer of the last sweep (odd sweep count: x,
-// even: V_out itself), first activation between V^n and V^{n+1} (every
+// even: V_out itself; BiCGSTAB: always x), first activation between V^n and V^{n+1} (every
 // myocardial slot, junction voxels included: the PMJ coupling is part of the
 // solve); cable nodes in the first ncb blocks.
 template <typename T>
 __global__ void __launch_bounds__(256) cn_final_kernel(
     hp_bricks g, const T* __restrict__ V_in, const T* __restrict__ x, T* __restrict__ V_out,
     const T* __restrict__ xpk, T* __restrict__ Vpk_out, int64_t n_cable, hp_mono_params p,
-    const double* __restrict__ sc, uint32_t ncb) {
-  const bool odd = ((int64_t)sc[SC_SWEEPS] & 1) != 0;
+    const double* __restrict__ sc, uint32_t ncb, bool by_parity) {
+  const bool odd = !by_parity || ((int64_t)sc[SC_SWEEPS] & 1) != 0;
   const T vmin = T(p.vmin), vmax = T(p.vmax);
   if (blockIdx.x < ncb) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -392,6 +394,462 @@ __global__ void __launch_bounds__(256) cn_final_kernel(
         st32(at, slot, (T)((double)(step - 1) * p.dt) + T(p.dt) * (thr - Vc) / (Vn - Vc));
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB (solver 1): the reference's Jacobi-preconditioned BiCGSTAB
+// (:786-849) from x0 = V*, with its stopping rules and breakdown guards, on
+// the same matrix-free rows.  Iteration k is three passes, each followed by a
+// one-block finish kernel that sums the passes' FP64 block partials in block
+// order and takes the iteration's scalar decisions on the device:
+//   A(k): p = r + beta (p - omega v) (k = 0: p = r), v = A M^-1 p,  r^.v
+//   S(k): s = r - alpha v, t = A M^-1 s,  s.s, t.t, t.s
+//   X(k): x += alpha M^-1 p (+ omega M^-1 s), r = s - omega t,  r.r, r^.r
+// The stencil passes form M^-1 p (M^-1 s) of every tile cell from r, p, v
+// and M^-1 as they stage the halo, so no preconditioned vector is stored; p
+// and v alternate between two buffers (a pass reads its neighbours' old
+// values while writing its own new ones).  The elementwise updates are
+// evaluated as the reference writes them, without fused multiply-adds.
+// The first pass (INIT) forms r = b - A x0 = L V*, r^ = r, ||b|| and M^-1.
+// Passes after the step's solve ended return at once.
+// ---------------------------------------------------------------------------
+enum { SC_RHO0 = 8, SC_RHO1, SC_ALPHA, SC_OMEGA, SC_BETA, SC_RESET, SC_SEXIT, SC_CONV };
+enum { BI_INIT = 0, BI_P0, BI_P, BI_S };
+
+// One side of the solve (myocardial slots or cable nodes): n entries each.
+template <typename T>
+struct BiSide {
+  T* x;        // the iterate, in place of V*
+  T* r;
+  T* rh;       // r^ (shadow residual)
+  T* minv;     // M^-1 = 1/(Cm/dt - theta L_ii)
+  T* t;
+  const T* pin;
+  const T* vin;
+  T* pout;
+  T* vout;
+  uint32_t n;
+};
+
+template <typename T>
+__device__ __forceinline__ T bi_pdir(T r, T p, T v, T beta, T omega) {
+#pragma clang fp contract(off)
+  return r + beta * (p - omega * v);
+}
+template <typename T>
+__device__ __forceinline__ T bi_sdir(T r, T v, T alpha) {
+#pragma clang fp contract(off)
+  return r - alpha * v;
+}
+
+// The vector the pass applies A to, at entry q of one side (q may be OOB:
+// every load then reads out of range and returns 0).
+template <typename T, int MODE>
+struct BiZ {
+  buf_t x, r, p, v, mi;
+  T alpha, beta, omega;
+  __device__ __forceinline__ BiZ(const BiSide<T>& s, T a, T b, T w) : alpha(a), beta(b), omega(w) {
+    const uint32_t bytes = s.n * (uint32_t)sizeof(T);
+    x = buf_rsrc(s.x, MODE == BI_INIT ? bytes : 0u);
+    r = buf_rsrc(s.r, MODE == BI_INIT ? 0u : bytes);
+    p = buf_rsrc(s.pin, MODE == BI_P ? bytes : 0u);
+    v = buf_rsrc(s.vin, MODE >= BI_P ? bytes : 0u);
+    mi = buf_rsrc(s.minv, MODE == BI_INIT ? 0u : bytes);
+  }
+  __device__ __forceinline__ T operator()(uint32_t q) const {
+#pragma clang fp contract(off)
+    if (MODE == BI_INIT) return buf_ld<T>(x, q);
+    const T mq = buf_ld<T>(mi, q), rq = buf_ld<T>(r, q);
+    if (MODE == BI_P0) return mq * rq;
+    if (MODE == BI_P) return mq * bi_pdir(rq, buf_ld<T>(p, q), buf_ld<T>(v, q), beta, omega);
+    return mq * bi_sdir(rq, buf_ld<T>(v, q), alpha);
+  }
+};
+
+// Outputs of one row i of a pass given z_i, (L z)_i and -L_ii; accumulates
+// the pass's inner products in FP64.
+template <typename T, int MODE>
+__device__ __forceinline__ void bi_row(const BiSide<T>& s, uint32_t i, T zc, T lz, T ldg,
+                                       const hp_cn_params& cp, T alpha, T beta, T omega,
+                                       bool reset, double (&q)[3]) {
+#pragma clang fp contract(off)
+  const T cm_dt = T(cp.cm_dt), th = T(cp.theta);
+  if (MODE == BI_INIT) {
+    const T bv = cm_dt * zc + (T(1) - th) * lz;
+    st32(s.r, i, lz);
+    st32(s.rh, i, lz);
+    st32(s.minv, i, T(1) / (cm_dt + th * ldg));
+    q[0] += (double)bv * (double)bv;
+    q[1] += (double)lz * (double)lz;
+    return;
+  }
+  const T az = cm_dt * zc - th * lz;
+  const T r = ld32((const T*)s.r, i);
+  if (MODE == BI_S) {
+    const T sv = bi_sdir(r, ld32(s.vin, i), alpha);
+    st32(s.t, i, az);
+    q[0] += (double)sv * (double)sv;
+    q[1] += (double)az * (double)az;
+    q[2] += (double)az * (double)sv;
+    return;
+  }
+  const T pn = MODE == BI_P0 ? r : bi_pdir(r, ld32(s.pin, i), ld32(s.vin, i), beta, omega);
+  st32(s.pout, i, pn);
+  st32(s.vout, i, az);
+  T rh;
+  if (reset) {
+    rh = r;
+    st32(s.rh, i, r);
+  } else {
+    rh = ld32((const T*)s.rh, i);
+  }
+  q[0] += (double)rh * (double)az;
+}
+
+// Block sums of q[0..NV) -> part[v * gridDim.x + blockIdx.x] (NW waves per block).
+template <int NW, int NV>
+__device__ __forceinline__ void bi_block_partials(const double (&q)[3], double* part) {
+  __shared__ double sh[NV][NW];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double w = cn_wave_sum(q[v]);
+    if (lane == 0) sh[v][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double a = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) a += sh[v][w];
+      part[v * gridDim.x + blockIdx.x] = a;
+    }
+  }
+}
+
+// The XCD-contiguous brick range of a capped grid (as the sweeps).
+struct BrickRange {
+  uint32_t first, end, stride;
+};
+__device__ __forceinline__ BrickRange brick_range(uint32_t bid, uint32_t G, uint32_t NB) {
+  const uint32_t xcd = bid & 7;
+  const uint32_t m = (G - xcd + 7) >> 3;
+  const uint32_t base = NB >> 3, rem = NB & 7;
+  const uint32_t lo = xcd * base + (xcd < rem ? xcd : rem);
+  const uint32_t hi = lo + base + (xcd < rem ? 1u : 0u);
+  return BrickRange{lo + (bid >> 3), hi, m};
+}
+
+// Stencil pass of the solve (INIT, P0/P = A(k), S = S(k)): cable rows in the
+// first ncb workgroups (grid-stride), bricks after them.
+template <typename T, int BX, int BY, int BZ, int MODE>
+__global__ void __launch_bounds__(BX * BY * BZ) cn_bi_pass_kernel(
+    hp_bricks g, BiSide<T> m, hp_cable c, BiSide<T> cv, hp_cn_pmj j, hp_cn_params cp,
+    double* part, uint32_t ncb) {
+  constexpr int BVOL = BX * BY * BZ;
+  constexpr int NV = MODE == BI_INIT ? 2 : MODE == BI_S ? 3 : 1;
+  typedef Tile<T, BX, BY, BZ> TT;
+  __shared__ TT t;
+  const double* sc = cp.scalars;
+  if ((MODE == BI_P || MODE == BI_S) && sc[SC_DONE] != 0.0) return;
+  const T alpha = MODE == BI_S ? T(sc[SC_ALPHA]) : T(0);
+  const T beta = MODE == BI_P ? T(sc[SC_BETA]) : T(0);
+  const T omega = MODE == BI_P ? T(sc[SC_OMEGA]) : T(0);
+  const bool reset = MODE == BI_P && sc[SC_RESET] != 0.0;
+  const BiZ<T, MODE> zm(m, alpha, beta, omega), zp(cv, alpha, beta, omega);
+  double q[3] = {0.0, 0.0, 0.0};
+  const int tid = threadIdx.x;
+  if (blockIdx.x < ncb) {
+    const T dsw = T(cp.dscale_pk * c.w);
+    for (int64_t i = (int64_t)blockIdx.x * BVOL + tid; i < c.n; i += (int64_t)ncb * BVOL) {
+      const T zc = zp((uint32_t)i);
+      T acc = T(0);
+      const int e0 = c.indptr[i], e1 = c.indptr[i + 1];
+      for (int e = e0; e < e1; ++e) acc += zp((uint32_t)c.indices[e]) - zc;
+      T lz = dsw * acc, ldg = dsw * T(e1 - e0);
+      if (j.node_ptr) {
+        for (int e = j.node_ptr[i]; e < j.node_ptr[i + 1]; ++e) {
+          const T w = T(j.node_w[e]);
+          lz += w * (zm((uint32_t)j.node_slot[e]) - zc);
+          ldg += w;
+        }
+      }
+      bi_row<T, MODE>(cv, (uint32_t)i, zc, lz, ldg, cp, alpha, beta, omega, reset, q);
+    }
+  } else {
+    const BrickRange br = brick_range(blockIdx.x - ncb, gridDim.x - ncb, (uint32_t)g.n_bricks);
+    int vx, vy, vz;
+    compute_voxel<BX, BY, BZ>(tid, vx, vy, vz);
+    const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
+    const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
+    const T dscale = T(cp.dscale);
+    for (uint32_t b = br.first; b < br.end; b += br.stride) {
+      __syncthreads();  // every lane is done with the previous tile
+      {
+        HaloNbr<BX, BY, BZ> hn;
+        halo_nbr<BX, BY, BZ>(hn, g, b);
+        HaloRegs<T, BX, BY, BZ> hr;
+        halo_load_fn<T, BX, BY, BZ>(hr, hn, g, g.label[b * BVOL + tid], zm);
+        halo_store_nan<T, BX, BY, BZ>(t, hr);
+        __syncthreads();
+      }
+      const uint32_t slot = b * BVOL + vloc;
+      const uint8_t mylab = t.L[cc];
+      if (mylab == 0xFF) continue;
+      const T zc = t.V[cc];
+      T ws;
+      const T acc = stencil19_wsum_nan<T, BX, BY, BZ>(t, cc, ws);
+      T lz = dscale * acc, ldg = dscale * ws;
+      if ((mylab & 0x80) && j.n_targets > 0) {  // PMJ target voxel: its junction couplings
+        const int qi = cn_find_target(j.tgt_slot, (int)j.n_targets, (int)slot);
+        const int e1 = j.tgt_slot[qi] == (int)slot ? j.tgt_ptr[qi + 1] : 0;  // listed slots only
+        for (int e = j.tgt_ptr[qi]; e < e1; ++e) {
+          const T w = T(j.tgt_w[e]);
+          lz += w * (zp((uint32_t)j.tgt_node[e]) - zc);
+          ldg += w;
+        }
+      }
+      bi_row<T, MODE>(m, slot, zc, lz, ldg, cp, alpha, beta, omega, reset, q);
+    }
+  }
+  bi_block_partials<(BVOL + 63) / 64, NV>(q, part);
+}
+
+template <typename T>
+__device__ __forceinline__ void bi_update(const BiSide<T>& s, uint32_t i, T alpha, T omega,
+                                          bool sexit, double (&q)[3]) {
+#pragma clang fp contract(off)
+  const T ph = ld32((const T*)s.minv, i) * ld32(s.pin, i);
+  const T x = ld32((const T*)s.x, i);
+  if (sexit) {
+    st32(s.x, i, x + alpha * ph);
+    return;
+  }
+  const T sv = bi_sdir(ld32((const T*)s.r, i), ld32(s.vin, i), alpha);
+  const T sh = ld32((const T*)s.minv, i) * sv;
+  st32(s.x, i, (x + alpha * ph) + omega * sh);
+  const T rn = sv - omega * ld32((const T*)s.t, i);
+  st32(s.r, i, rn);
+  q[0] += (double)rn * (double)rn;
+  q[1] += (double)ld32((const T*)s.rh, i) * (double)rn;
+}
+
+// X(k): the iterate and residual update (pointwise), same grid as the passes.
+template <typename T, int BVOL>
+__global__ void __launch_bounds__(BVOL) cn_bi_update_kernel(hp_bricks g, BiSide<T> m, hp_cable c,
+                                                            BiSide<T> cv, hp_cn_params cp,
+                                                            double* part, uint32_t ncb) {
+  const double* sc = cp.scalars;
+  if (sc[SC_DONE] != 0.0) return;
+  const T alpha = T(sc[SC_ALPHA]), omega = T(sc[SC_OMEGA]);
+  const bool sexit = sc[SC_SEXIT] != 0.0;
+  double q[3] = {0.0, 0.0, 0.0};
+  if (blockIdx.x < ncb) {
+    for (int64_t i = (int64_t)blockIdx.x * BVOL + threadIdx.x; i < c.n; i += (int64_t)ncb * BVOL)
+      bi_update<T>(cv, (uint32_t)i, alpha, omega, sexit, q);
+  } else {
+    const BrickRange br = brick_range(blockIdx.x - ncb, gridDim.x - ncb, (uint32_t)g.n_bricks);
+    for (uint32_t b = br.first; b < br.end; b += br.stride) {
+      const uint32_t slot = b * BVOL + threadIdx.x;
+      if (g.label[slot] != 0xFF) bi_update<T>(m, slot, alpha, omega, sexit, q);
+    }
+  }
+  if (!sexit) bi_block_partials<(BVOL + 63) / 64, 2>(q, part);
+}
+
+// Sums of NV partial columns of nb workgroups (fixed order), in thread 0.
+template <int NV>
+__device__ __forceinline__ void bi_sum_cols(const double* __restrict__ part, int nb,
+                                            double (&out)[NV]) {
+  __shared__ double sh[NV][CN_FIN_THREADS / 64];
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int b0 = 0; b0 < nb; b0 += 4 * CN_FIN_THREADS) {
+    double x[NV][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + u * CN_FIN_THREADS + (int)threadIdx.x;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) x[v][u] = b < nb ? part[v * nb + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += x[v][u];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double w = cn_wave_sum(acc[v]);
+    if (lane == 0) sh[v][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double a = 0.0;
+      for (int w = 0; w < CN_FIN_THREADS / 64; ++w) a += sh[v][w];
+      out[v] = a;
+    }
+  }
+}
+
+// The step's solve ends after `iters` iterations begun; statistics as for
+// the sweeps (the iteration count in place of the sweep count).
+__device__ __forceinline__ void bi_done(double* sc, bool conv, int iters, double rel) {
+  sc[SC_DONE] = 1.0;
+  sc[SC_CONV] = conv ? 1.0 : 0.0;
+  sc[SC_SWEEPS] = (double)iters;
+  sc[SC_TOTAL] += (double)iters;
+  sc[SC_STEPS] += 1.0;
+  if ((double)iters > sc[SC_MAXIT]) sc[SC_MAXIT] = (double)iters;
+  if (!conv) {
+    sc[SC_NCONV] += 1.0;
+    if (rel > sc[SC_WORST]) sc[SC_WORST] = rel;
+  }
+}
+
+enum { BI_FIN_A0 = 0, BI_FIN_A, BI_FIN_S, BI_FIN_X };
+
+// The scalar part of iteration k (reference :786-849, in its order):
+//   A0: ||b|| (0: x0 returned), rho = r^.r = r.r, da = r^.v (|da| < 1e-300:
+//       stop), alpha = rho/da;   A(k >= 1): the same from da and the stored rho
+//   S:  ||s|| < tol ||b||: the update takes x += alpha p^ and the solve ends;
+//       omega = t.s/t.t (0 when |t.t| <= 1e-300)
+//   X:  ||r||/||b|| < tol: converged; |omega| < 1e-300 or the last iteration:
+//       stop; rho' = r^.r (|rho'| < 1e-300: r^ := r, rho' = r.r),
+//       beta = (rho'/rho)(alpha/omega)
+// part points at the stage's columns: A0 (bb, rr, da), A (da), S (ss, tt, ts),
+// X (rr, r^.r); nb = workgroups of the passes.
+template <int STAGE>
+__global__ void __launch_bounds__(CN_FIN_THREADS) cn_bi_finish_kernel(hp_cn_params cp,
+                                                                      const double* part, int nb,
+                                                                      int k) {
+  double* sc = cp.scalars;
+  if (STAGE != BI_FIN_A0 && sc[SC_DONE] != 0.0) return;
+  constexpr int NV = STAGE == BI_FIN_A ? 1 : STAGE == BI_FIN_X ? 2 : 3;
+  double v[NV];
+  bi_sum_cols<NV>(part, nb, v);
+  if (threadIdx.x != 0) return;
+  const double tol = cp.tol;
+  if (STAGE == BI_FIN_A0) {
+    const double bb = v[0], rr = v[1], da = v[2 % NV];
+    sc[SC_BB] = bb;
+    sc[SC_RR] = rr;
+    sc[SC_SEXIT] = 0.0;
+    sc[SC_RESET] = 0.0;
+    sc[SC_DONE] = 0.0;
+    const double bn = sqrt(bb);
+    if (bn == 0.0) {
+      bi_done(sc, true, 0, 0.0);
+      return;
+    }
+    sc[SC_RHO0] = rr;
+    if (fabs(da) < 1e-300) {
+      const double rel = sqrt(rr) / bn;
+      bi_done(sc, rel < tol, 1, rel);
+      return;
+    }
+    sc[SC_ALPHA] = rr / da;
+    return;
+  }
+  const double bn = sqrt(sc[SC_BB]);
+  if (STAGE == BI_FIN_A) {
+    const double da = v[0];
+    if (fabs(da) < 1e-300) {
+      const double rel = sqrt(sc[SC_RR]) / bn;
+      bi_done(sc, rel < tol, k + 1, rel);
+      return;
+    }
+    sc[SC_ALPHA] = sc[SC_RHO0 + (k & 1)] / da;
+    return;
+  }
+  if (STAGE == BI_FIN_S) {
+    const double ss = v[0], tt = v[1 % NV], ts = v[2 % NV];
+    if (sqrt(ss) < tol * bn) {
+      sc[SC_SEXIT] = 1.0;
+      return;
+    }
+    sc[SC_OMEGA] = fabs(tt) > 1e-300 ? ts / tt : 0.0;
+    return;
+  }
+  // BI_FIN_X
+  if (sc[SC_SEXIT] != 0.0) {
+    sc[SC_SEXIT] = 0.0;
+    bi_done(sc, true, k + 1, 0.0);
+    return;
+  }
+  const double rr = v[0], rhr = v[1 % NV];
+  sc[SC_RR] = rr;
+  const double rel = sqrt(rr) / bn;
+  if (rel < tol) {
+    bi_done(sc, true, k + 1, rel);
+    return;
+  }
+  const double omega = sc[SC_OMEGA];
+  if (fabs(omega) < 1e-300 || k + 1 >= cp.max_sweeps) {
+    bi_done(sc, false, k + 1, rel);
+    return;
+  }
+  double rho = rhr;
+  const bool reset = fabs(rho) < 1e-300;
+  if (reset) rho = rr;
+  sc[SC_RESET] = reset ? 1.0 : 0.0;
+  sc[SC_BETA] = (rho / sc[SC_RHO0 + (k & 1)]) * (sc[SC_ALPHA] / omega);
+  sc[SC_RHO0 + ((k + 1) & 1)] = rho;
+}
+
+template <typename T, int BX, int BY, int BZ>
+static void launch_bi_pass(int mode, dim3 grid, hipStream_t st, const hp_bricks& g,
+                           const BiSide<T>& m, const hp_cable& c, const BiSide<T>& cv,
+                           const hp_cn_pmj& j, const hp_cn_params& cp, double* part,
+                           uint32_t ncb) {
+  const dim3 blk(BX * BY * BZ);
+  switch (mode) {
+    case BI_INIT:
+      hipLaunchKernelGGL((cn_bi_pass_kernel<T, BX, BY, BZ, BI_INIT>), grid, blk, 0, st, g, m, c,
+                         cv, j, cp, part, ncb);
+      break;
+    case BI_P0:
+      hipLaunchKernelGGL((cn_bi_pass_kernel<T, BX, BY, BZ, BI_P0>), grid, blk, 0, st, g, m, c, cv,
+                         j, cp, part, ncb);
+      break;
+    case BI_P:
+      hipLaunchKernelGGL((cn_bi_pass_kernel<T, BX, BY, BZ, BI_P>), grid, blk, 0, st, g, m, c, cv,
+                         j, cp, part, ncb);
+      break;
+    default:
+      hipLaunchKernelGGL((cn_bi_pass_kernel<T, BX, BY, BZ, BI_S>), grid, blk, 0, st, g, m, c, cv,
+                         j, cp, part, ncb);
+      break;
+  }
+}
+
+template <typename T>
+static void bi_pass(int key, int mode, dim3 grid, hipStream_t st, const hp_bricks& g,
+                    const BiSide<T>& m, const hp_cable& c, const BiSide<T>& cv,
+                    const hp_cn_pmj& j, const hp_cn_params& cp, double* part, uint32_t ncb) {
+  switch (key) {
+    case 444: launch_bi_pass<T, 4, 4, 4>(mode, grid, st, g, m, c, cv, j, cp, part, ncb); break;
+    case 844: launch_bi_pass<T, 8, 4, 4>(mode, grid, st, g, m, c, cv, j, cp, part, ncb); break;
+    case 884: launch_bi_pass<T, 8, 8, 4>(mode, grid, st, g, m, c, cv, j, cp, part, ncb); break;
+    default:  launch_bi_pass<T, 8, 8, 8>(mode, grid, st, g, m, c, cv, j, cp, part, ncb); break;
+  }
+}
+
+template <typename T>
+static void bi_update_launch(int key, dim3 grid, hipStream_t st, const hp_bricks& g,
+                             const BiSide<T>& m, const hp_cable& c, const BiSide<T>& cv,
+                             const hp_cn_params& cp, double* part, uint32_t ncb) {
+  const int bvol = key == 444 ? 64 : key == 844 ? 128 : key == 884 ? 256 : 512;
+  switch (bvol) {
+    case 64: hipLaunchKernelGGL((cn_bi_update_kernel<T, 64>), grid, dim3(64), 0, st, g, m, c, cv, cp, part, ncb); break;
+    case 128: hipLaunchKernelGGL((cn_bi_update_kernel<T, 128>), grid, dim3(128), 0, st, g, m, c, cv, cp, part, ncb); break;
+    case 256: hipLaunchKernelGGL((cn_bi_update_kernel<T, 256>), grid, dim3(256), 0, st, g, m, c, cv, cp, part, ncb); break;
+    default: hipLaunchKernelGGL((cn_bi_update_kernel<T, 512>), grid, dim3(512), 0, st, g, m, c, cv, cp, part, ncb); break;
   }
 }
 
@@ -423,10 +881,14 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
               what);
     return -1;
   }
-  if (!cp->scalars || !cp->part || !w->Vs || !w->b || !w->x || cp->max_sweeps < 1 ||
+  const bool bicg = cp->solver == 1;
+  bool kr_ok = true;
+  for (int q = 0; q < 8; ++q) kr_ok = kr_ok && w->kr[q];
+  if (!cp->scalars || !cp->part || !w->Vs || cp->max_sweeps < 1 ||
+      !(cp->solver == 0 || cp->solver == 1) || (!bicg && (!w->b || !w->x)) || (bicg && !kr_ok) ||
       !(cp->cm_dt > 0.0) || !(cp->theta >= 0.0 && cp->theta <= 1.0) || !(cp->tol >= 0.0)) {
     set_error("%s: CN params need scalars, part, Cm/dt > 0, 0 <= theta <= 1, tol >= 0, "
-              "max_sweeps >= 1, and the work vectors Vs, b, x",
+              "max_sweeps >= 1, solver 0 (work vectors Vs, b, x) or 1 (Vs and kr[0..8))",
               what);
     return -1;
   }
@@ -434,9 +896,12 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
   if (ncab < 0 || (ncab > 0 && (!p_pk || !p_pk->group_amp || !Vpk_in || !Vpk_out || !state_pk ||
                                 ld_pk < ncab || (uint64_t)ld_pk * sizeof(T) >= (1ull << 32) ||
                                 !c->indptr || !c->indices || !c->label || !w->Vs_pk ||
-                                !w->b_pk || !w->x_pk))) {
+                                (!bicg && (!w->b_pk || !w->x_pk)) ||
+                                (bicg && !(w->kr_pk[0] && w->kr_pk[1] && w->kr_pk[2] &&
+                                           w->kr_pk[3] && w->kr_pk[4] && w->kr_pk[5] &&
+                                           w->kr_pk[6] && w->kr_pk[7]))))) {
     set_error("%s: a cable needs its params, V, state (ld_pk >= n), CSR, labels and the work "
-              "vectors Vs_pk, b_pk, x_pk",
+              "vectors Vs_pk with b_pk, x_pk (solver 0) or kr_pk[0..8) (solver 1)",
               what);
     return -1;
   }
@@ -483,16 +948,77 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
                        state, ld, *p, *c, Vpk_in, Vspk, state_pk, ld_pk, *p_pk, ncb_r);
     if (int rc = check_launch("hp_cn_step/react")) return rc;
   }
-  // sweeps: x_0 = V*, sweep k writes x (k odd) or V_out (k even)
   int64_t ncb = ceil_div(ncab, bvol);
   if (ncb > CN_MAX_CABLE_BLOCKS) ncb = CN_MAX_CABLE_BLOCKS;
   ncb = (ncb + 7) / 8 * 8;
   const int64_t G = ncb + (g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS);
-  if (G > 0x7fffffff || 2 * G > cp->part_len) {
-    set_error("%s: the sweeps' %lld workgroups need part_len >= %lld doubles (got %lld)", what,
-              (long long)G, (long long)(2 * G), (long long)cp->part_len);
+  const int64_t cols = cp->solver == 1 ? 8 : 2;
+  if (G > 0x7fffffff || cols * G > cp->part_len) {
+    set_error("%s: the passes' %lld workgroups need part_len >= %lld doubles (got %lld)", what,
+              (long long)G, (long long)(cols * G), (long long)cp->part_len);
     return -1;
   }
+  const uint32_t ncb_f = (uint32_t)ceil_div(ncab, 256);
+  const int64_t nblk_f = ncb_f + ceil_div(n_slots, 256);
+  if (cp->solver == 1) {
+    // BiCGSTAB from x0 = V* (the iterate overwrites Vs)
+    BiSide<T> m{}, cv{};
+    T* const* kr = (T* const*)w->kr;
+    T* const* kp = (T* const*)w->kr_pk;
+    m = BiSide<T>{Vs, kr[0], kr[1], kr[2], kr[3], nullptr, nullptr, nullptr, nullptr,
+                  (uint32_t)n_slots};
+    if (ncab > 0)
+      cv = BiSide<T>{Vspk, kp[0], kp[1], kp[2], kp[3], nullptr, nullptr, nullptr, nullptr,
+                     (uint32_t)ncab};
+    double* part = cp->part;
+    const dim3 grid((unsigned)G);
+    if (G > 0) {
+      bi_pass<T>(key, BI_INIT, grid, st, *g, m, *c, cv, *j, *cp, part, (uint32_t)ncb);
+      if (int rc = check_launch("hp_cn_step/bicgstab init")) return rc;
+    }
+    for (int k = 0; k < cp->max_sweeps; ++k) {
+      const int a = k & 1, b = a ^ 1;
+      m.pin = kr[4 + a];
+      m.vin = kr[6 + a];
+      m.pout = kr[4 + b];
+      m.vout = kr[6 + b];
+      if (ncab > 0) {
+        cv.pin = kp[4 + a];
+        cv.vin = kp[6 + a];
+        cv.pout = kp[4 + b];
+        cv.vout = kp[6 + b];
+      }
+      if (G > 0) bi_pass<T>(key, k == 0 ? BI_P0 : BI_P, grid, st, *g, m, *c, cv, *j, *cp,
+                            part + 2 * G, (uint32_t)ncb);
+      if (k == 0)
+        hipLaunchKernelGGL((cn_bi_finish_kernel<BI_FIN_A0>), dim3(1), dim3(CN_FIN_THREADS), 0, st,
+                           *cp, part, (int)G, k);
+      else
+        hipLaunchKernelGGL((cn_bi_finish_kernel<BI_FIN_A>), dim3(1), dim3(CN_FIN_THREADS), 0, st,
+                           *cp, part + 2 * G, (int)G, k);
+      if (int rc = check_launch("hp_cn_step/bicgstab A")) return rc;
+      // S(k) and X(k) read the new direction and product
+      m.pin = m.pout;
+      m.vin = m.vout;
+      cv.pin = cv.pout;
+      cv.vin = cv.vout;
+      if (G > 0) bi_pass<T>(key, BI_S, grid, st, *g, m, *c, cv, *j, *cp, part + 3 * G,
+                            (uint32_t)ncb);
+      hipLaunchKernelGGL((cn_bi_finish_kernel<BI_FIN_S>), dim3(1), dim3(CN_FIN_THREADS), 0, st,
+                         *cp, part + 3 * G, (int)G, k);
+      if (int rc = check_launch("hp_cn_step/bicgstab S")) return rc;
+      if (G > 0) bi_update_launch<T>(key, grid, st, *g, m, *c, cv, *cp, part + 6 * G,
+                                     (uint32_t)ncb);
+      hipLaunchKernelGGL((cn_bi_finish_kernel<BI_FIN_X>), dim3(1), dim3(CN_FIN_THREADS), 0, st,
+                         *cp, part + 6 * G, (int)G, k);
+      if (int rc = check_launch("hp_cn_step/bicgstab X")) return rc;
+    }
+    if (nblk_f > 0)
+      hipLaunchKernelGGL(cn_final_kernel<T>, dim3((unsigned)nblk_f), dim3(256), 0, st, *g, V_in,
+                         Vs, V_out, Vspk, Vpk_out, ncab, *p, cp->scalars, ncb_f, false);
+    return check_launch("hp_cn_step/final");
+  }
+  // sweeps: x_0 = V*, sweep k writes x (k odd) or V_out (k even)
   for (int k = 1; k <= cp->max_sweeps; ++k) {
     const T* xi = k == 1 ? Vs : (k % 2 == 0 ? x : V_out);
     T* xo = k % 2 == 1 ? x : V_out;
@@ -516,11 +1042,9 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
     if (int rc = check_launch("hp_cn_step/finish")) return rc;
   }
   // clamp + activation times
-  const uint32_t ncb_f = (uint32_t)ceil_div(ncab, 256);
-  const int64_t nblk_f = ncb_f + ceil_div(n_slots, 256);
   if (nblk_f > 0)
     hipLaunchKernelGGL(cn_final_kernel<T>, dim3((unsigned)nblk_f), dim3(256), 0, st, *g, V_in, x,
-                       V_out, xpk, Vpk_out, ncab, *p, cp->scalars, ncb_f);
+                       V_out, xpk, Vpk_out, ncab, *p, cp->scalars, ncb_f, true);
   return check_launch("hp_cn_step/final");
 }
 
@@ -565,11 +1089,54 @@ static int cn_sweep_only(const hp_bricks* g, const T* x_in, T* x_out, const T* b
   return check_launch("hp_cn_sweep");
 }
 
+// One BiCGSTAB product pass outside a step (the solve's stencil kernel alone,
+// for measurement): t = A M^-1 (r - alpha v) over the myocardial rows, with the
+// inner products' partials in cp->part.
+template <typename T>
+static int cn_apply_only(const hp_bricks* g, const hp_cn_params* cp, const hp_cn_work* w,
+                         void* stream) {
+  if (!g || !cp || !w || !cp->part || !cp->scalars || !g->label || !g->D6 || !g->nbr ||
+      g->n_bricks < 0 || !w->kr[0] || !w->kr[2] || !w->kr[3] || !w->kr[6]) {
+    set_error("hp_cn_apply: brick grid, CN params (scalars, part) and the work vectors r, M^-1, "
+              "t, v are required");
+    return -1;
+  }
+  const int64_t bvol = (int64_t)g->bx * g->by * g->bz;
+  if ((uint64_t)g->n_bricks * bvol * 6 * sizeof(T) >= (1ull << 32)) {
+    set_error("hp_cn_apply: %lld bricks exceed the 32-bit offset range", (long long)g->n_bricks);
+    return -1;
+  }
+  const int64_t G = g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS;
+  if (G == 0) return 0;
+  if (3 * G > cp->part_len) {
+    set_error("hp_cn_apply: part_len >= %lld doubles required", (long long)(3 * G));
+    return -1;
+  }
+  const int key = g->bx * 100 + g->by * 10 + g->bz;
+  if (key != 444 && key != 844 && key != 884 && key != 888) {
+    set_error("hp_cn_apply: unsupported brick shape %dx%dx%d", g->bx, g->by, g->bz);
+    return -1;
+  }
+  T* const* kr = (T* const*)w->kr;
+  const BiSide<T> m{(T*)w->Vs, kr[0], kr[1], kr[2], kr[3], kr[4], kr[6], nullptr, nullptr,
+                    (uint32_t)(g->n_bricks * bvol)};
+  const BiSide<T> cv{};
+  const hp_cable c0{};
+  const hp_cn_pmj j0{};
+  bi_pass<T>(key, BI_S, dim3((unsigned)G), (hipStream_t)stream, *g, m, c0, cv, j0, *cp, cp->part,
+             0u);
+  return check_launch("hp_cn_apply");
+}
+
 }  // namespace hp
 
 using namespace hp;
 
 extern "C" {
+int hp_cn_apply_f32(const hp_bricks* g, const hp_cn_params* cp, const hp_cn_work* w,
+                    void* stream) {
+  return cn_apply_only<float>(g, cp, w, stream);
+}
 int hp_cn_sweep_f32(const hp_bricks* g, const float* x_in, float* x_out, const float* b,
                     const hp_cn_params* cp, void* stream) {
   return cn_sweep_only<float>(g, x_in, x_out, b, cp, stream);

@@ -78,15 +78,21 @@ class MonodomainEngine:
 
     ``scheme``: "explicit" (forward-Euler diffusion fused with the reaction,
     exact PMJ exchange; stable below ``tissue.explicit_dt_limit``) or "cn"
-    (the reference's Crank-Nicolson step with ``theta``, solved by Jacobi
-    sweeps to ``cg_tol`` relative residual, at most ``max_sweeps`` per step;
-    purkinje_myocardium_anisotropic_solver.py:757-849, 1746-1791).
+    (the reference's Crank-Nicolson step with ``theta``;
+    purkinje_myocardium_anisotropic_solver.py:757-849, 1746-1791).  The CN
+    system is solved by ``cn_solver`` "bicgstab" (the reference's
+    Jacobi-preconditioned BiCGSTAB with its stopping rules at ``cg_tol``, at
+    most ``max_sweeps`` iterations per step: the reference's iterates) or
+    "jacobi" (Jacobi sweeps to the same relative residual, at most
+    ``max_sweeps``: the exact CN solution to well within the tolerance, where
+    the reference's BiCGSTAB stops with an error of its own at the fronts).
     """
 
     def __init__(self, layout: BrickLayout, D6, celltype_myo, *, h, A_M, Cm, dt,
                  dtype=torch.float32, device="cuda", cable=None, pmj=None, stim_sites=None,
                  at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None,
-                 alloc="torch", scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
+                 alloc="torch", scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10,
+                 cn_solver="bicgstab"):
         nat.load()
         if scheme not in ("explicit", "cn"):
             raise ValueError(f"unknown scheme {scheme!r} (explicit or cn)")
@@ -228,7 +234,7 @@ class MonodomainEngine:
         self.cn = None
         if scheme == "cn":
             self._setup_cn(L, pmj, float(h), float(A_M), float(theta), float(cg_tol),
-                           int(max_sweeps), cable)
+                           int(max_sweeps), cable, cn_solver)
         self.p = 0
         self.overlap_cable = True
         self.fuse_cable = True
@@ -236,18 +242,27 @@ class MonodomainEngine:
         self._eval_stim()
 
     # ------------------------------------------------------------------ setup
-    def _setup_cn(self, L, pmj, h, A_M, theta, tol, max_sweeps, cable):
+    def _setup_cn(self, L, pmj, h, A_M, theta, tol, max_sweeps, cable, solver):
         """Work vectors, PMJ rows and device scalars of the Crank-Nicolson step."""
         if not 0.0 <= theta <= 1.0:
             raise ValueError(f"theta must lie in [0, 1] (got {theta})")
         if not tol >= 0.0 or max_sweeps < 1:
             raise ValueError("cg_tol must be >= 0 and max_sweeps >= 1")
+        if solver not in ("bicgstab", "jacobi"):
+            raise ValueError(f"cn_solver must be 'bicgstab' or 'jacobi' (got {solver!r})")
+        self.cn_solver = solver
+        bicg = solver == "bicgstab"
         dev, dt_ = self.dev, self.dtype
         n, Np = L.n_slots, self.Np
         z = lambda m: torch.zeros(max(m, 1), dtype=dt_, device=dev)
-        self.cn_work_t = [z(n), z(n), z(n), z(Np), z(Np), z(Np)]
-        self.cn_scalars = torch.zeros(16, dtype=torch.float64, device=dev)
-        part_len = 2 * (min(L.n_bricks, 8192) + 1024)  # per sweep workgroup (<= 520 cable)
+        self.cn_work_t = ([z(n), None, None, z(Np), None, None] if bicg else
+                          [z(n), z(n), z(n), z(Np), z(Np), z(Np)])
+        # BiCGSTAB: r, r^, M^-1, t, p (2), v (2) per side
+        self.cn_kr_t = [z(n) for _ in range(8)] if bicg else []
+        self.cn_kr_pk_t = [z(Np) for _ in range(8)] if bicg and Np else []
+        self.cn_scalars = torch.zeros(24, dtype=torch.float64, device=dev)
+        # per pass workgroup (<= 520 cable): 2 partial columns (sweeps), 8 (BiCGSTAB)
+        part_len = (8 if bicg else 2) * (min(L.n_bricks, 8192) + 1024)
         self.cn_part = torch.zeros(part_len, dtype=torch.float64, device=dev)
         Ai_P = float(cable["Ai_P"]) if Np else 1.0
         Ai_M = A_M * h ** 3
@@ -276,10 +291,14 @@ class MonodomainEngine:
             t = self.cn_pmj_t
             self.cn_pmj = nat.HpCnPmj(len(tgt), *(nat.ptr(a) for a in t[:7]), len(jn),
                                       nat.ptr(t[7]))
-        self.cn_work = nat.HpCnWork(*(nat.ptr(a) for a in self.cn_work_t))
+        kr = (C.c_void_p * 8)(*(nat.ptr(a) for a in self.cn_kr_t)) if bicg else (C.c_void_p * 8)()
+        kr_pk = ((C.c_void_p * 8)(*(nat.ptr(a) for a in self.cn_kr_pk_t)) if self.cn_kr_pk_t
+                 else (C.c_void_p * 8)())
+        self.cn_work = nat.HpCnWork(*(nat.ptr(a) if a is not None else None
+                                      for a in self.cn_work_t), kr, kr_pk)
         self.cn = nat.HpCnParams(self.Cm / self.dt, theta, 1.0 / (A_M * h * h), 1.0 / Ai_P, tol,
-                                 max_sweeps, 0, nat.ptr(self.cn_scalars), nat.ptr(self.cn_part),
-                                 part_len)
+                                 max_sweeps, 1 if bicg else 0, nat.ptr(self.cn_scalars),
+                                 nat.ptr(self.cn_part), part_len)
 
     def cn_stats(self):
         """Solver statistics of the CN steps so far: steps, mean sweeps, steps that
@@ -290,8 +309,18 @@ class MonodomainEngine:
         sc = self.cn_scalars.cpu().numpy()
         steps = int(sc[7])
         return dict(steps=steps, mean_sweeps=float(sc[6] / steps) if steps else 0.0,
-                    not_converged=int(sc[4]), worst_rel_res=float(sc[5]),
-                    max_sweeps=int(self.cn.max_sweeps), tol=float(self.cn.tol))
+                    max_used=int(sc[16]), not_converged=int(sc[4]),
+                    worst_rel_res=float(sc[5]), max_sweeps=int(self.cn.max_sweeps),
+                    tol=float(self.cn.tol), solver=self.cn_solver)
+
+    def set_cn_max_iters(self, n: int):
+        """Bound the CN solve at n sweeps (BiCGSTAB iterations) per step from the
+        next step on; graphs captured with the old bound are dropped."""
+        if self.cn is None or n < 1:
+            raise ValueError("set_cn_max_iters needs a CN engine and n >= 1")
+        if int(n) != self.cn.max_sweeps:
+            self.cn.max_sweeps = int(n)
+            self._graphs = {}
 
     def _init_state(self, st):
         """Resting TTP06 state into st (18, n) (Ki, Nai as offsets from their bases)."""
@@ -396,9 +425,14 @@ class MonodomainEngine:
         return g
 
     def run(self, n_steps: int, graph_chunk: int = 0):
-        """Advance n_steps; with graph_chunk > 0, replay a captured graph of that many steps."""
+        """Advance n_steps; with graph_chunk > 0, replay a captured graph of that many steps.
+
+        A graph holds the buffer pointers of the V/V_pk ping-pong as they stood
+        when it was captured, so graphs are kept per parity of the current
+        buffer (and per cable launch mode): a replay after an odd number of
+        eager steps uses the graph captured from that parity."""
         if graph_chunk and n_steps >= graph_chunk:
-            key = ("graph", graph_chunk)
+            key = ("graph", graph_chunk, self.p, self.fuse_cable, self.overlap_cable)
             if not hasattr(self, "_graphs"):
                 self._graphs = {}
             if key not in self._graphs:

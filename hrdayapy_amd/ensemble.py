@@ -126,7 +126,9 @@ class Ensemble:
     def __init__(self, specs, *, geoms=None, dt=0.02, dtype=torch.float32, device="cuda",
                  brick=None, A_M=140.0, Cm=0.01, phi_endo_max=1 / 3, phi_epi_min=2 / 3,
                  scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10):
-        """scheme "cn": the reference's Crank-Nicolson step for every instance.
+        """scheme "cn": the reference's Crank-Nicolson step for every instance,
+        solved by Jacobi sweeps (a Krylov solve of the batch would couple the
+        instances through its inner products; the sweeps never do).
         The reference solves one instance per call, each to ||r|| <= cg_tol
         ||b|| of its own system; the batch is stopped on its whole residual, so
         the batch tolerance is cg_tol * sqrt(smallest instance / all voxels),
@@ -170,7 +172,7 @@ class Ensemble:
             self.layout, np.concatenate(D6s, axis=1), np.concatenate(cts), h=H_REF, A_M=A_M,
             Cm=Cm, dt=dt, dtype=dtype, device=device, stim_sites=sites,
             inst=np.concatenate(insts), inst_params=inst_params, scheme=scheme, theta=theta,
-            cg_tol=self.batch_tol, max_sweeps=max_sweeps)
+            cg_tol=self.batch_tol, max_sweeps=max_sweeps, cn_solver="jacobi")
         self.t_setup = time.perf_counter() - t0
 
     @property

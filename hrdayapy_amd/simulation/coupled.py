@@ -24,7 +24,7 @@ def compute_coupled(nodes, elements, act_times, S, Z, phi, *, anatomy=None, stim
                     stim_len_mm=2.0, c_pmj=0.05, n_pmj=1, cg_tol=1e-5, cg_max_iter=500,
                     n_frames=100, device=None, phi_endo_max=0.35, phi_epi_min=0.65,
                     save_path=None, vm_save_dt=None, vm_save_path=None, vm_tmp_dir=None,
-                    dtype=torch.float32, scheme="cn"):
+                    dtype=torch.float32, scheme="cn", cn_solver="bicgstab"):
     voxel_size = resolve_voxel_size(voxel_size, anatomy, param_name="voxel_size")
     return purkinje_myocardium_anisotropic_solver(
         nodes=nodes, elements=elements, activation_times=act_times, S=S, Z=Z, phi=phi,
@@ -34,7 +34,7 @@ def compute_coupled(nodes, elements, act_times, S, Z, phi, *, anatomy=None, stim
         c_pmj=c_pmj, n_pmj=n_pmj, cg_tol=cg_tol, cg_max_iter=cg_max_iter, n_frames=n_frames,
         device=device, out_npz=save_path, phi_endo_max=phi_endo_max, phi_epi_min=phi_epi_min,
         vm_save_dt=vm_save_dt, out_vm_npz=vm_save_path, vm_tmp_dir=vm_tmp_dir, dtype=dtype,
-        scheme=scheme, _sigma_M=sigma_M)
+        scheme=scheme, cn_solver=cn_solver, _sigma_M=sigma_M)
 
 
 def load_coupled(path):

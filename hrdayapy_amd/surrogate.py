@@ -384,7 +384,7 @@ class SurrogateRunner:
                                         nat.ptr(self.state_list))
         self.n_blocks = n_blocks
         self._orig = (eng.p_myo.do_reaction, eng.p_myo.rate_ext)
-        self._graph = None
+        self._graphs = {}
         self.graph_macros = int(graph_macros)
 
     def launch_reaction(self):
@@ -426,8 +426,9 @@ class SurrogateRunner:
             for _ in range(n_macro):
                 self._launch_macro()
             return
-        if self._graph is None:
-            eng = self.eng
+        eng = self.eng
+        key = (eng.p, eng.fuse_cable, eng.overlap_cable)  # buffers as captured
+        if self._graphs.get(key) is None:
             p0 = eng.p
             g = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream(device=eng.dev)
@@ -441,10 +442,10 @@ class SurrogateRunner:
                         self._launch_macro()
             torch.cuda.current_stream(eng.dev).wait_stream(side)
             eng.p = p0
-            self._graph = g
+            self._graphs[key] = g
         k = self.graph_macros
         for _ in range(n_macro // k):
-            self._graph.replay()
+            self._graphs[key].replay()
         for _ in range(n_macro % k):
             self._launch_macro()
 

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 3
+#define HP_ABI_VERSION 4
 
 /* ---------------------------------------------------------------------------
  * Library
@@ -300,21 +300,29 @@ typedef struct hp_cn_pmj {
   const int32_t* jnode;      /* [n_jnodes] distinct cable nodes with couplings */
 } hp_cn_pmj;
 
-/* scalars (device, >= 8 doubles): 0 ||b||^2, 1 ||r||^2 of the last sweep,
- * 2 done flag, 3 sweeps of the current step, 4 steps that stopped at
- * max_sweeps without meeting the rule, 5 the largest ||r||/||b|| of those,
- * 6 total sweeps, 7 steps solved (4-7 accumulate over calls; the caller
- * zeroes them).  part: device, part_len >= 2*(min(n_bricks, 8192) + ncb)
- * doubles (the workgroup partial sums of a sweep: up to 8192 brick workgroups
- * plus ncb <= 520 for the cable rows). */
+/* solver 0: Jacobi sweeps x <- x + (b - A x)/diag(A) to the stopping rule
+ * ||b - A x|| <= tol ||b|| (an approximation of the exact CN solution well
+ * inside the tolerance); solver 1: the reference's Jacobi-preconditioned
+ * BiCGSTAB (:786-849) with its stopping rules and breakdown guards, which
+ * reproduces the reference's iterates (its error against the exact solution
+ * included).  max_sweeps bounds the sweeps (BiCGSTAB iterations) per step.
+ * scalars (device, >= 24 doubles): 0 ||b||^2, 1 ||r||^2 last evaluated,
+ * 2 done flag, 3 sweeps (iterations) of the current step, 4 steps that stopped
+ * at max_sweeps (BiCGSTAB: or at a breakdown) without meeting the rule, 5 the
+ * largest ||r||/||b|| of those, 6 total sweeps (iterations), 7 steps solved,
+ * 16 the largest sweep (iteration) count of any step (4-7 and 16 accumulate
+ * over calls; the caller zeroes them); 8-15 BiCGSTAB state.
+ * part: device, doubles, part_len >= 2*G (solver 0) or 8*G (solver 1) with
+ * G = min(n_bricks, 8192) + ncb the workgroups of a pass (ncb <= 520 for the
+ * cable rows). */
 typedef struct hp_cn_params {
   double cm_dt;        /* Cm/dt */
   double theta;        /* 0 <= theta <= 1 (reference default 0.5) */
   double dscale;       /* 1/(A_M h^2): L of the myocardial rows = dscale * stencil sum */
   double dscale_pk;    /* 1/Ai_P: L of the cable rows = dscale_pk * w_pp * sum */
   double tol;          /* relative residual of the stopping rule (reference cg_tol) */
-  int32_t max_sweeps;  /* >= 1, sweeps per step (launched every step; idle after done) */
-  int32_t pad_;
+  int32_t max_sweeps;  /* >= 1, sweeps (iterations) per step (launched every step; idle after done) */
+  int32_t solver;      /* 0 Jacobi sweeps, 1 BiCGSTAB */
   double* scalars;
   double* part;
   int64_t part_len;    /* doubles in part */
@@ -322,7 +330,9 @@ typedef struct hp_cn_params {
 
 /* Work vectors (device, the arithmetic type of the call): Vs, b, x of n_slots
  * entries; Vs_pk, b_pk, x_pk of n_cable entries (may be NULL without a cable).
- * V_out serves as the second iterate buffer. */
+ * Solver 0 uses Vs, b, x and V_out as the second iterate buffer; solver 1
+ * uses Vs (the iterate) and kr[8] = r, r^, M^-1, t, p (two buffers), v (two
+ * buffers), n_slots entries each, kr_pk[8] likewise over the cable nodes. */
 typedef struct hp_cn_work {
   void* Vs;
   void* b;
@@ -330,6 +340,8 @@ typedef struct hp_cn_work {
   void* Vs_pk;
   void* b_pk;
   void* x_pk;
+  void* kr[8];
+  void* kr_pk[8];
 } hp_cn_work;
 
 /* p / p_pk: dt, inv_cm, ki/nai bases, instance scales, group_amp (stimulus of
@@ -350,6 +362,15 @@ int hp_cn_step_f32(const hp_bricks* g, const float* V_in, float* V_out, float* s
  * the reference: the solve's dominant kernel by itself, for measurement. */
 int hp_cn_sweep_f32(const hp_bricks* g, const float* x_in, float* x_out, const float* b,
                     const hp_cn_params* cp, void* stream);
+/* One BiCGSTAB product pass of the myocardial rows alone (no cable, no
+ * junctions): t = A M^-1 (r - alpha v) with the step's kernel, from the work
+ * vectors r = kr[0], M^-1 = kr[2], v = kr[6] into t = kr[3], the partial sums
+ * of s.s, t.t, t.s into cp->part (>= 3*min(n_bricks, 8192) doubles);
+ * cp->scalars: 16 doubles with entry 2 (done) 0 and entry 10 = alpha.  No
+ * counterpart in the reference: the BiCGSTAB solve's stencil kernel by itself,
+ * for measurement. */
+int hp_cn_apply_f32(const hp_bricks* g, const hp_cn_params* cp, const hp_cn_work* w,
+                    void* stream);
 int hp_cn_step_f64(const hp_bricks* g, const double* V_in, double* V_out, double* state,
                    int64_t ld, const hp_mono_params* p, const hp_cable* c, const double* Vpk_in,
                    double* Vpk_out, double* state_pk, int64_t ld_pk, const hp_mono_params* p_pk,

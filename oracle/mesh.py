@@ -220,22 +220,38 @@ def cn_matrix(L, diag_L, scale, theta):
     return A, 1.0 / np.maximum(scale - theta * diag_L, 1e-30)
 
 
-def bicgstab(A, M_inv, b, x0, tol=1e-5, maxiter=500):
-    """Jacobi-preconditioned BiCGSTAB with the reference's stopping rules (aniso.py:786-849)."""
+def bicgstab(A, M_inv, b, x0, tol=1e-5, maxiter=500, return_iters=False, dot=None):
+    """Jacobi-preconditioned BiCGSTAB with the reference's stopping rules (aniso.py:786-849).
+    With return_iters, also the number of iterations begun (0 when ||b|| = 0).
+    ``dot`` replaces the inner product (and the norms, as sqrt(dot(a, a))):
+    tests use it to measure how far the iterates move when the sums are taken
+    in another order."""
+    if dot is not None:
+        vdot = dot
+
+        class _LA:
+            @staticmethod
+            def norm(a):
+                return float(np.sqrt(vdot(a, a)))
+    else:
+        def vdot(a, c):
+            return a @ c
+        _LA = np.linalg
     x = x0.copy()
     r = b - A @ x
     r_hat = r.copy()
     rho_prev = alpha = omega = 1.0
     v = np.zeros_like(b)
     p = np.zeros_like(b)
-    b_norm = float(np.linalg.norm(b))
+    b_norm = float(_LA.norm(b))
     if b_norm == 0.0:
-        return x
+        return (x, 0) if return_iters else x
+    it = -1
     for it in range(maxiter):
-        rho = float(r_hat @ r)
+        rho = float(vdot(r_hat, r))
         if abs(rho) < 1e-300:
             r_hat = r.copy()
-            rho = float(r_hat @ r)
+            rho = float(vdot(r_hat, r))
         if it == 0:
             p = r.copy()
         else:
@@ -243,26 +259,26 @@ def bicgstab(A, M_inv, b, x0, tol=1e-5, maxiter=500):
             p = r + beta * (p - omega * v)
         p_hat = M_inv * p
         v = A @ p_hat
-        da = float(r_hat @ v)
+        da = float(vdot(r_hat, v))
         if abs(da) < 1e-300:
             break
         alpha = rho / da
         s = r - alpha * v
-        if np.linalg.norm(s) < tol * b_norm:
+        if _LA.norm(s) < tol * b_norm:
             x = x + alpha * p_hat
             break
         s_hat = M_inv * s
         t = A @ s_hat
-        dw = float(t @ t)
-        omega = float(t @ s) / dw if abs(dw) > 1e-300 else 0.0
+        dw = float(vdot(t, t))
+        omega = float(vdot(t, s)) / dw if abs(dw) > 1e-300 else 0.0
         x = x + alpha * p_hat + omega * s_hat
         r = s - omega * t
         rho_prev = rho
-        if np.linalg.norm(r) / b_norm < tol:
+        if _LA.norm(r) / b_norm < tol:
             break
         if abs(omega) < 1e-300:
             break
-    return x
+    return (x, it + 1) if return_iters else x
 
 
 # ---------------------------------------------------------------------------

@@ -12,7 +12,11 @@ Reference: purkinje_myocardium_anisotropic_solver.py (``aniso.py``)
       V^{n+1}        = clamp(V^{n+1}, -150, 80)
 
 Three integrators are provided:
-  * ``run(..., scheme="cn")``: the reference's scheme above;
+  * ``run(..., scheme="cn")``: the reference's scheme above (its BiCGSTAB,
+    mesh.bicgstab); ``sweeps`` then holds the BiCGSTAB iterations of every
+    step.  The GPU's FP64 CN path with its BiCGSTAB solver is compared with
+    this one at round-off level (the inner products are summed in another
+    order, so the iterates agree to round-off, not bit for bit);
   * ``run(..., scheme="cn_jacobi")``: the same CN step with the linear system
     solved as the HIP kernels solve it -- Jacobi sweeps x <- x + (b - A x)/diag A
     from x0 = V*, the first residual taken as L V* (= b - A V*), after each
@@ -154,7 +158,7 @@ def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0):
 
 def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtype=torch.float64,
         record_every=0, record_nodes=None, at_thr=-40.0, max_steps=None, state_init=None,
-        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10):
+        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10, dot=None):
     """Integrate the problem; returns dict(V, state, at, trace_t, trace_V, steps, wall_s).
 
     ``at`` is (N,) first activation time (NaN if none), on every node.
@@ -186,7 +190,8 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
         if scheme == "cn":
             Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)
             b = (Cm / dt) * Vs + (1 - prob.theta) * (prob.L @ Vs)
-            Vn = M.bicgstab(A, M_inv, b, Vs, cg_tol, cg_max_iter)
+            Vn, k = M.bicgstab(A, M_inv, b, Vs, cg_tol, cg_max_iter, return_iters=True, dot=dot)
+            sweeps.append(k)
             Vn = np.clip(Vn, -150.0, 80.0)
         elif scheme == "cn_jacobi":
             Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)

@@ -2,11 +2,14 @@
 (purkinje_myocardium_anisotropic_solver.py:757-849, 1746-1791) on the brick
 layout, with the PMJ couplings inside the implicit solve.
 
-* FP64 against the oracle's ``cn_jacobi`` integrator (the same CN system and
-  the same Jacobi iteration and stopping rule): round-off agreement of V, the
+* FP64, BiCGSTAB solver, against the oracle's ``cn`` integrator (the
+  reference's BiCGSTAB and stopping rules): round-off agreement of V, the
+  activation times and the iteration counts of every step.
+* FP64, Jacobi solver, against the oracle's ``cn_jacobi`` integrator (the
+  same Jacobi iteration and stopping rule): round-off agreement of V, the
   activation times and the sweep counts.
-* FP32 against the oracle's ``cn`` integrator (the reference's BiCGSTAB solve):
-  the north-star bar, Vm RMS < 0.5 mV and AT < 0.5 ms.
+* FP32 (BiCGSTAB) against the oracle's ``cn`` integrator: the north-star bar,
+  Vm RMS < 0.5 mV and AT < 0.5 ms.
 * Graph replay equals eager launches; theta = 1 (backward Euler) and the
   drop-in's defaults (dt = 0.1 ms, one CN step per dt) against the oracle.
 """
@@ -28,40 +31,67 @@ def _engine(args, kw, cuda, dtype, **extra):
                                 scheme="cn", **{"dt": 0.02, **kw, **extra})
 
 
+ORACLE_SCHEME = {"bicgstab": "cn", "jacobi": "cn_jacobi"}
+# FP64 agreement bars (V mV, AT ms, state rtol).  The Jacobi sweeps agree with
+# the oracle to round-off.  A BiCGSTAB iterate depends on its inner products,
+# which the device sums in another order than numpy; the excitable dynamics
+# then grow round-off differences through the upstrokes: on the "tree" case
+# the oracle itself, with its inner products summed in reverse order, moves by
+# 6.4e-6 mV over these 20 ms, and by 5.6e-4 mV at dt = 0.1 ms, where the solve
+# takes ~6 iterations (tests/test_oracle_mesh.py pins both bounds); the bars
+# are 1e-4 and 5e-3 mV.
+BARS = {"jacobi": (1e-8, 1e-8, 1e-9), "bicgstab": (1e-4, 1e-5, 1e-5)}
+BAR_V_DT01 = 5e-3
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "jacobi"])
 @pytest.mark.parametrize("kind", ["stub", "tree", "overlap"])
 @pytest.mark.parametrize("brick", [(4, 4, 4), (8, 8, 4)])
-def test_fp64_cn_matches_oracle(cuda, kind, brick):
-    print(f"[case] fp64 CN engine vs cn_jacobi oracle kind={kind} brick={brick}", flush=True)
+def test_fp64_cn_matches_oracle(cuda, kind, brick, solver):
+    print(f"[case] fp64 CN engine ({solver}) vs {ORACLE_SCHEME[solver]} oracle kind={kind} "
+          f"brick={brick}", flush=True)
     args, kw = _case(kind)
     T = 20.0
     nodes, el, act, S, phi, f = args
     prob = OMD.Problem(nodes, el, act, S, phi, f, **kw)
-    ref = OMD.run(prob, T, 0.02, scheme="cn_jacobi", max_sweeps=10)
-    eng, meta = _engine(args, kw, cuda, torch.float64, brick=brick, max_sweeps=10)
+    ref = OMD.run(prob, T, 0.02, scheme=ORACLE_SCHEME[solver], max_sweeps=10)
+    eng, meta = _engine(args, kw, cuda, torch.float64, brick=brick, max_sweeps=10,
+                        cn_solver=solver)
     assert meta["Np"] == prob.Np and meta["N_myo"] == prob.N_myo
     if kind != "stub":
         assert eng.cn_pmj is not None
     eng.run(int(T / 0.02), graph_chunk=50)
     Vg = eng.V_global().cpu().numpy()
     err = float(np.max(np.abs(Vg - ref["V"])))
-    print(f"  max |dV| {err:.2e} mV, sweeps/step {ref['sweeps'].mean():.2f}")
-    np.testing.assert_allclose(Vg, ref["V"], rtol=0, atol=1e-8)
+    print(f"  max |dV| {err:.2e} mV, iterations per step {ref['sweeps'].mean():.2f} "
+          f"(max {ref['sweeps'].max()})")
+    bar_v, bar_at, bar_st = BARS[solver]
+    np.testing.assert_allclose(Vg, ref["V"], rtol=0, atol=bar_v)
     at = eng.at_nodes().cpu().numpy()
     ra = ref["at"][prob.Np:]
     assert np.array_equal(np.isnan(at), np.isnan(ra))
-    np.testing.assert_allclose(at[~np.isnan(at)], ra[~np.isnan(ra)], atol=1e-8)
+    np.testing.assert_allclose(at[~np.isnan(at)], ra[~np.isnan(ra)], atol=bar_at)
     st = eng.state_nodes().cpu().numpy()
-    np.testing.assert_allclose(st, ref["state"].numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(st, ref["state"].numpy(), rtol=bar_st, atol=1e-12)
     cs = eng.cn_stats()
     assert cs["steps"] == int(T / 0.02) and cs["not_converged"] == 0
-    # the device's stopping decisions are the oracle's, step for step in total
-    assert cs["mean_sweeps"] * cs["steps"] == pytest.approx(ref["sweeps"].sum())
+    # the device's stopping decisions are the oracle's: every step for the
+    # sweeps; for BiCGSTAB all but a rare step whose iterate sits that close to
+    # the rule (on "tree", step 710 of 1000 stops after 3 iterations at
+    # ||r||/||b|| = 7.4e-6 on the device where the oracle's third iterate has
+    # 1.2e-5: an erratic iteration amplifies the round-off differences)
+    total = cs["mean_sweeps"] * cs["steps"]
+    if solver == "jacobi":
+        assert total == pytest.approx(ref["sweeps"].sum())
+    else:
+        assert abs(total - ref["sweeps"].sum()) <= 2
+    assert ref["sweeps"].max() <= 10
 
 
 @pytest.mark.parametrize("kind", ["stub", "tree"])
 def test_fp32_cn_vs_reference_scheme(cuda, kind):
-    """The production arithmetic (FP32) against the reference's own scheme
-    (FP64, CN + Jacobi-BiCGSTAB to cg_tol = 1e-5): Vm RMS < 0.5 mV and
+    """The production arithmetic (FP32, BiCGSTAB) against the reference's own
+    scheme (FP64, CN + Jacobi-BiCGSTAB to cg_tol = 1e-5): Vm RMS < 0.5 mV and
     activation times within 0.5 ms, over 60 ms (wave through the slab)."""
     print(f"[case] fp32 CN engine vs fp64 CN/BiCGSTAB oracle kind={kind}", flush=True)
     args, kw = _case(kind)
@@ -109,6 +139,27 @@ def test_cn_graph_replay_equals_eager(cuda):
     assert a.cn_stats() == b.cn_stats()
 
 
+@pytest.mark.parametrize("scheme", ["cn", "explicit"])
+def test_graph_replay_after_odd_eager_steps(cuda, scheme):
+    """Graphs hold the V buffers of the ping-pong as captured: a replay after an
+    odd number of eager steps must use the graph of that parity (24-step
+    graphs with one eager step between them equal 49 eager steps)."""
+    print(f"[case] {scheme}: 24-step graphs around an eager step vs eager", flush=True)
+    args, kw = _case("tree")
+    nodes, el, act, S, phi, f = args
+    mk = lambda: build_coupled_system(nodes, el, act, S, phi, f, device=cuda,
+                                      dtype=torch.float32, scheme=scheme, dt=0.02, **kw)[0]
+    a, b = mk(), mk()
+    for _ in range(4):
+        a.run(24, graph_chunk=24)
+        a.run(1)
+    b.run(100)
+    torch.cuda.synchronize()
+    assert a.step_index == b.step_index == 100
+    assert torch.equal(a.V_global(), b.V_global())
+    assert torch.equal(a.at.nan_to_num(-1.0), b.at.nan_to_num(-1.0))
+
+
 def test_cn_backward_euler_and_large_step(cuda):
     """theta = 1 at dt = 0.02 and the reference's default dt = 0.1 ms (above the
     explicit limit: one CN step per dt, no substeps) against the oracle."""
@@ -118,10 +169,16 @@ def test_cn_backward_euler_and_large_step(cuda):
         print(f"[case] fp64 CN theta={theta} dt={dt}", flush=True)
         kwt = {**kw, "theta": theta}
         prob = OMD.Problem(nodes, el, act, S, phi, f, **kwt)
-        ref = OMD.run(prob, T, dt, scheme="cn_jacobi", max_sweeps=10)
+        ref = OMD.run(prob, T, dt, scheme="cn", max_sweeps=10)
         eng, meta = _engine(args, {**kw, "dt": dt}, cuda, torch.float64, theta=theta,
                             max_sweeps=10)
         assert meta["n_sub"] == 1
         eng.run(int(round(T / dt)))
-        np.testing.assert_allclose(eng.V_global().cpu().numpy(), ref["V"], rtol=0, atol=1e-8)
+        print(f"  BiCGSTAB iterations per step {ref['sweeps'].mean():.2f} "
+              f"(max {ref['sweeps'].max()})")
+        assert ref["sweeps"].max() <= 10
+        Vg = eng.V_global().cpu().numpy()
+        print(f"  max |dV| {np.abs(Vg - ref['V']).max():.2e} mV")
+        np.testing.assert_allclose(Vg, ref["V"], rtol=0,
+                                   atol=BARS["bicgstab"][0] if dt == 0.02 else BAR_V_DT01)
         assert ref["V"].max() > 0.0 or dt == 0.02  # a wave under way at dt = 0.1

@@ -66,7 +66,8 @@ def test_config1_slab_500ms(cuda):
         tr = [eng.V_myo_nodes()[probe].double().cpu().numpy()]
         at50 = None
         for k in range(1, 1000):
-            eng.run(25, graph_chunk=25)
+            eng.run(24, graph_chunk=24)  # graphs hold an even number of steps
+            eng.run(1)
             tr.append(eng.V_myo_nodes()[probe].double().cpu().numpy())
             if k == n_ref:
                 # step 2501: every voxel activated well before the fixture's
@@ -81,9 +82,11 @@ def test_config1_slab_500ms(cuda):
         err = np.abs(r["at50"] - fx["at"]).max()
         rms = np.sqrt(np.mean((r["tr"][:n_ref] - fx["probe_V"]) ** 2))
         print(f"  vs the oracle over 50 ms: AT max err {err:.2e} ms, probe Vm RMS {rms:.2e} mV; "
-              f"{r['cn']['mean_sweeps']:.2f} sweeps per step", flush=True)
+              f"{r['cn']['mean_sweeps']:.2f} BiCGSTAB iterations per step", flush=True)
         assert np.isfinite(r["at50"]).all()
-        assert err < 0.5 and rms < 0.5
+        # FP64: the reference's iterates to round-off; FP32
+        lim = 1e-6 if dtype == torch.float64 else 0.5
+        assert err < lim and rms < lim
         assert np.isfinite(V).all() and V.max() < -80.0 and V.min() > -90.0
     d, f = runs[torch.float64], runs[torch.float32]
     at_d = np.abs(d["at"] - f["at"]).max()
@@ -103,7 +106,7 @@ def test_config2_biv_beat(cuda):
         print(f"[case] configuration 2, BiV 600 ms CN {dtype}", flush=True)
         eng, meta = build_coupled_system(nodes, el, act, geo["S"], geo["phi"], geo["fibre_dir"],
                                          dt=0.02, dtype=dtype, device=cuda, scheme="cn",
-                                         brick=(8, 4, 4), max_sweeps=5, **kw)
+                                         brick=(8, 4, 4), max_sweeps=8, **kw)
         if sample is None:
             sample = torch.from_numpy(np.sort(rng.choice(meta["N_myo"], 20000, replace=False))
                                       ).to(cuda)
@@ -119,7 +122,7 @@ def test_config2_biv_beat(cuda):
         runs[dtype] = r
         print(f"  {r['n']:,} voxels: activated {np.isfinite(at).sum():,}, last AT "
               f"{np.nanmax(at):.1f} ms, V at 600 ms in [{V.min():.1f}, {V.max():.1f}] mV, "
-              f"{r['cn']['mean_sweeps']:.2f} sweeps per step", flush=True)
+              f"{r['cn']['mean_sweeps']:.2f} BiCGSTAB iterations per step", flush=True)
         assert r["n"] > 2_000_000 and np.isfinite(at).all()
         assert np.isfinite(V).all() and V.max() < -70.0
         assert np.isfinite(r["Vpk"]).all() and r["Vpk"].max() < -70.0

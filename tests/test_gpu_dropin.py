@@ -40,7 +40,7 @@ def _slab_case(h=0.25):
 
 
 def test_anisotropic_coupled_files_and_oracle(cuda, tmp_path):
-    print("[case] compute_anisotropic_coupled fp64 vs explicit oracle", flush=True)
+    print("[case] compute_anisotropic_coupled fp64 vs CN oracle", flush=True)
     (nodes, el, act, S, phi, f), kw = _slab_case()
     aniso = dict(sigma_l=0.24, sigma_t=0.08)
     T, dt = 16.0, 0.02
@@ -61,12 +61,13 @@ def test_anisotropic_coupled_files_and_oracle(cuda, tmp_path):
     assert SNAP_KEYS <= set(z.files) and z["Vm"].dtype == np.float32
     assert snap.dt_solver == pytest.approx(dt) and snap.vm_save_dt == pytest.approx(1.0)
     assert snap.n_frames == 16 and snap.N_myo == S.sum()
-    # the drop-in's default scheme is the reference's Crank-Nicolson step: the
-    # oracle's CN with the kernels' Jacobi solve (min(cg_max_iter, 12) sweeps),
-    # recorded every 50 steps with the same stamps
+    # the drop-in's default scheme is the reference's Crank-Nicolson step with
+    # its BiCGSTAB solve: the oracle's CN, recorded every 50 steps with the
+    # same stamps
     prob = OMD.Problem(nodes, el, act, S, phi, f, T=T, **aniso, **kw)
-    ref = OMD.run(prob, T, dt, scheme="cn_jacobi", record_every=50, max_sweeps=12,
+    ref = OMD.run(prob, T, dt, scheme="cn", record_every=50,
                   record_nodes=np.arange(prob.Np, prob.N))
+    assert ref["sweeps"].max() <= 8  # within the iterations the drop-in captures
     assert res["cn_stats"]["not_converged"] == 0
     np.testing.assert_allclose(snap.time_vm, ref["trace_t"], rtol=0, atol=1e-5)
     # snapshots are stored in FP32: compare at FP32 resolution
@@ -122,7 +123,7 @@ def test_isotropic_coupled_matches_oracle(cuda, tmp_path):
     snap = load_vm_snapshots(tmp_path / "vm")
     prob = OMD.Problem(nodes, el, act, S, phi, None, T=T, A_P=133.33, isotropic_sigma=0.13,
                        **kw)
-    ref = OMD.run(prob, T, 0.02, scheme="cn_jacobi", record_every=100, max_sweeps=12,
+    ref = OMD.run(prob, T, 0.02, scheme="cn", record_every=100,
                   record_nodes=np.arange(prob.Np, prob.N))
     np.testing.assert_allclose(snap.Vm, ref["trace_V"], rtol=0, atol=1e-4)
     at = res["activation_times"]
