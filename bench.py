@@ -13,25 +13,29 @@ One bench step is 1 ms of simulated time: 50 solver steps of 0.02 ms of the
 whole coupled system (myocardium + Purkinje + PMJ + stimulus) and one ECG
 frame; --steps K times exactly K of them.  Forms, each run from t = 0 over the
 full 600 ms beat (the timed window is part of that run):
-  plain     the reference's scheme in FP32: TTP06 reaction, then Crank-Nicolson
-            diffusion of the coupled system solved by the reference's
-            Jacobi-preconditioned BiCGSTAB to cg_tol = 1e-5 (hp_cn_step)
-  jacobi    the same CN system solved by Jacobi sweeps to the same residual
+  plain     the reference's scheme in FP32: plain Rush-Larsen TTP06 reaction,
+            then Crank-Nicolson diffusion of the coupled system solved to the
+            reference's relative residual cg_tol = 1e-5 by Jacobi sweeps
+            (hp_cn_step, solver 0)
+  bicgstab  the same with the reference's own solver, Jacobi-preconditioned
+            BiCGSTAB with its stopping rules (solver 1, the drop-in default)
   explicit  the fast form: forward-Euler diffusion fused with the reaction
   bigstep / surrogate  the multirate forms (one 0.2 ms reaction per
             macro-step by the exact model / the learned network, plain
             integration where the upstroke indicator fires)
-  ref64     the reference scheme in FP64 (the oracle's algorithm, equal to it at
-            round-off level on the test problems): the gates' reference
+  ref64     the reference scheme with its BiCGSTAB in FP64 (the oracle's
+            algorithm, equal to it at round-off level on the test problems):
+            the gates' reference
 Every form's activation times and 12 leads over the full beat are compared
 with ref64 (AT error <= 1 ms, lead correlation >= 0.999).  The headline value
 is the surrogate form's throughput if it meets both gates, else the plain
 form's.
 
 Prints one JSON line with the throughput, a roofline object for the dominant
-kernel of the plain form (the BiCGSTAB product pass: 19-point stencil,
-measured alone with HIP events on its stream) and a CPU baseline (the oracle,
-CN + BiCGSTAB, timed on a bounded number of steps of the same workload).
+kernel of the plain form (the CN Jacobi sweep: 19-point stencil, measured
+alone with HIP events on its stream; the BiCGSTAB form's product pass beside
+it) and a CPU baseline (the oracle, CN + BiCGSTAB, timed on a bounded number
+of steps of the same workload).
 """
 from __future__ import annotations
 
@@ -49,6 +53,9 @@ sys.path.insert(0, HERE)
 
 METRIC = "voxel-steps/s, 2M-voxel BiV: surrogate vs plain FP32 at AT err≤1ms, ECG r≥.999"
 HBM_PEAK_GBS = 8000.0
+# CN Jacobi sweep, per myocardial voxel: x read 4, D tensor 24, label 1, b read 4,
+# x written 4 (the first sweep of a step writes b instead of reading it)
+SWEEP_BYTES_PER_VOXEL = 37
 # BiCGSTAB product pass t = A M^-1 (r - alpha v), per myocardial voxel: r, v,
 # M^-1 read 12, D tensor 24, label 1, t written 4
 APPLY_BYTES_PER_VOXEL = 41
@@ -60,7 +67,7 @@ STEPS_PER_MS = 50
 # reference's solver) / Jacobi sweeps; the ones after the solve ended return
 # at once
 CN_MAX_ITERS = {"bicgstab": 6, "jacobi": 5}
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "cn_apply_biv256_traffic.json")
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "cn_sweep_biv256_traffic.json")
 
 
 def measured_traffic():
@@ -107,18 +114,20 @@ def build_workload(n=256, h=0.4, seed=0, dtype=None, brick=None, scheme="cn"):
 
 
 FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
-    # the reference's step: CN with its BiCGSTAB, in FP64 (the gates'
-    # reference) and FP32 (the plain form)
-    "ref64": ("cn", "float64", None, {}), "plain": ("cn", "float32", None, {}),
-    # the same CN system solved by Jacobi sweeps (the exact CN solution to
-    # well within the tolerance)
-    "jacobi": ("cn", "float32", None, dict(cn_solver="jacobi")),
+    # the reference's step with its own solver, CN + Jacobi-preconditioned
+    # BiCGSTAB, in FP64: the gates' reference
+    "ref64": ("cn", "float64", None, dict(cn_solver="bicgstab")),
+    # plain Rush-Larsen reaction + CN diffusion in FP32, the CN system solved
+    # by Jacobi sweeps to the reference's relative residual
+    "plain": ("cn", "float32", None, dict(cn_solver="jacobi")),
+    # the same in FP32 with the reference's BiCGSTAB (the drop-in's default)
+    "bicgstab": ("cn", "float32", None, dict(cn_solver="bicgstab")),
     "explicit": ("explicit", "float32", None, {}),
     # multirate forms: one reaction per 0.2 ms macro-step (exact big step /
-    # network), ten substeps of the reference's CN diffusion driven by the rate
-    # (fallback voxels run the full cell model every substep)
-    "bigstep": ("cn", "float32", "bigstep", {}),
-    "surrogate": ("cn", "float32", "mlp", {}),
+    # network), ten substeps of the plain form's CN diffusion driven by the
+    # rate (fallback voxels run the full cell model every substep)
+    "bigstep": ("cn", "float32", "bigstep", dict(cn_solver="jacobi")),
+    "surrogate": ("cn", "float32", "mlp", dict(cn_solver="jacobi")),
     # the same rules on explicit substeps (the earlier multirate form)
     "bigstep_explicit": ("explicit", "float32", "bigstep", {}),
     "surrogate_explicit": ("explicit", "float32", "mlp", {}),
@@ -218,22 +227,34 @@ def stream_copy_bandwidth(mib, reps=20):
     return dict(mib=mib, us=round(us, 2), GBs=2 * n * 4 / (us * 1e-6) / 1e9)
 
 
-def time_cn_kernels(eng, reps=100):
-    """Device time (us) of the CN step's dominant kernel -- one BiCGSTAB product
-    pass t = A M^-1 (r - alpha v) over the myocardial rows (hp_cn_apply_f32,
-    the 19-point stencil) -- and of the whole step, each replayed from a HIP
-    graph between HIP events on the replaying stream."""
+def time_cn_kernels(eng, eng_b, reps=100, steps=True):
+    """Device times (us), each replayed from a HIP graph between HIP events on
+    the replaying stream: one Jacobi sweep of the myocardial rows
+    (hp_cn_sweep_f32, the plain form's dominant kernel, the 19-point stencil)
+    and the plain form's whole step (eng, solver 0); one BiCGSTAB product pass
+    t = A M^-1 (r - alpha v) (hp_cn_apply_f32) and the whole step of the
+    BiCGSTAB form (eng_b, solver 1)."""
     import torch
     from hrdayapy_amd import _native as nat
     from hrdayapy_amd import workloads as W
-    sc = torch.zeros(16, dtype=torch.float64, device=eng.dev)  # done = 0, alpha = 0
-    cp = nat.HpCnParams.from_buffer_copy(eng.cn)
+    Vs, bvec, x = eng.cn_work_t[:3]
+
+    def sweep(i):
+        src, dst = (Vs, x) if i % 2 == 0 else (x, Vs)
+        nat.call("hp_cn_sweep_f32", C.byref(eng.bricks), nat.ptr(src), nat.ptr(dst),
+                 nat.ptr(bvec), C.byref(eng.cn), nat.stream_handle())
+    out = dict(sweep_us=W.graph_us(sweep, reps))
+    if steps:
+        out["step_us"] = W.graph_us(lambda i: eng._launch_step(), reps)
+    sc = torch.zeros(16, dtype=torch.float64, device=eng_b.dev)  # done = 0, alpha = 0
+    cp = nat.HpCnParams.from_buffer_copy(eng_b.cn)
     cp.scalars = nat.ptr(sc)
-    apply_us = W.graph_us(lambda i: nat.call("hp_cn_apply_f32", C.byref(eng.bricks),
-                                             C.byref(cp), C.byref(eng.cn_work),
-                                             nat.stream_handle()), reps)
-    step_us = W.graph_us(lambda i: eng._launch_step(), reps)
-    return apply_us, step_us
+    out["apply_us"] = W.graph_us(lambda i: nat.call("hp_cn_apply_f32", C.byref(eng_b.bricks),
+                                                    C.byref(cp), C.byref(eng_b.cn_work),
+                                                    nat.stream_handle()), reps)
+    if steps:
+        out["bicgstab_step_us"] = W.graph_us(lambda i: eng_b._launch_step(), reps)
+    return out
 
 
 def cpu_info():
@@ -405,12 +426,14 @@ def main():
                     help="length of every form's run (the gates compare the whole run)")
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x4x4, or auto")
-    ap.add_argument("--forms", type=str, default="plain,jacobi,explicit,bigstep,surrogate")
+    ap.add_argument("--forms", type=str, default="plain,bicgstab,explicit,bigstep,surrogate")
     ap.add_argument("--train-samples", type=int, default=20_000_000)
     ap.add_argument("--train-epochs", type=int, default=14)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=100)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="time the CN Jacobi sweep and BiCGSTAB product pass alone, then exit")
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
                     help="size of the streaming-copy calibration (0 = off)")
     ap.add_argument("--workload", type=str, default="biv",
@@ -450,24 +473,37 @@ def main():
     tune = None
     if a.brick == "auto":
         from hrdayapy_amd.autotune import tune as autotune
-        brick, _, tune = autotune(lambda b: build_engine(geo, tree, kw, brick=b)[0],
+        brick, _, tune = autotune(lambda b: build_engine(geo, tree, kw, brick=b,
+                                                         cn_solver="jacobi")[0],
                                   shapes=((4, 4, 4), (8, 4, 4), (8, 8, 4)), chunks=(50,),
                                   steps=100, log=log)
         log(f"[bench] autotuned brick {brick} for the plain (CN) form: {tune}")
     else:
         brick = tuple(int(x) for x in a.brick.split("x"))
-    eng0, meta = build_engine(geo, tree, kw, brick=brick)
+    eng0, meta = build_engine(geo, tree, kw, brick=brick, cn_solver="jacobi")
     n_myo = meta["N_myo"]
     log(f"[bench] N_myo={n_myo:,} Np={meta['Np']:,} bricks={eng0.lay.n_bricks:,} "
         f"{eng0.lay.brick} occ={eng0.lay.occupancy:.3f} PMJ={len(meta['pmj_ids'])} "
         f"geo {t_geo:.1f}s setup {meta['setup_s']:.1f}s")
     ecg = W.EcgSetup(eng0, meta, n_bspm=a.bspm, log=log) if a.ecg == "leadfield" else None
-    # dominant kernel of the plain form and its whole step (device time)
-    apply_us, step_us = time_cn_kernels(eng0, a.kernel_reps)
-    apply_bytes = APPLY_BYTES_PER_VOXEL * n_myo
-    achieved = apply_bytes / (apply_us * 1e-6) / 1e9
-    log(f"[bench] CN BiCGSTAB product pass {apply_us:.1f} us/launch -> {achieved:.0f} GB/s "
-        f"algorithmic ({apply_bytes / 1e6:.1f} MB); CN step replayed {step_us:.1f} us")
+    # dominant kernel of the plain form and its whole step (device time); the
+    # BiCGSTAB form's product pass and step
+    eng_b, _ = build_engine(geo, tree, kw, brick=brick, cn_solver="bicgstab")
+    if a.kernels_only:  # the two solver kernels alone (for a profiler run)
+        kt = time_cn_kernels(eng0, eng_b, a.kernel_reps, steps=False)
+        kt["copy"] = stream_copy_bandwidth(a.calib_copy_mb) if a.calib_copy_mb else None
+        log(f"[bench] kernels only: {json.dumps(kt)}")
+        return
+    kt = time_cn_kernels(eng0, eng_b, a.kernel_reps)
+    del eng_b
+    sweep_us, step_us = kt["sweep_us"], kt["step_us"]
+    sweep_bytes = SWEEP_BYTES_PER_VOXEL * n_myo
+    achieved = sweep_bytes / (sweep_us * 1e-6) / 1e9
+    apply_gbs = APPLY_BYTES_PER_VOXEL * n_myo / (kt["apply_us"] * 1e-6) / 1e9
+    log(f"[bench] CN Jacobi sweep {sweep_us:.1f} us/launch -> {achieved:.0f} GB/s algorithmic "
+        f"({sweep_bytes / 1e6:.1f} MB); plain CN step replayed {step_us:.1f} us; BiCGSTAB "
+        f"product pass {kt['apply_us']:.1f} us ({apply_gbs:.0f} GB/s), BiCGSTAB step "
+        f"{kt['bicgstab_step_us']:.1f} us")
     ecg_info = None
     if ecg:
         rec0 = W.EcgRecorder(ecg.lf, eng0)
@@ -563,11 +599,16 @@ def main():
         "end_to_end": e2e,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "cn_bi_pass_kernel<float,8,4,4,S> (plain form: the BiCGSTAB "
-                               "product pass t = A M^-1 s of the CN solve, the 19-point "
-                               "stencil; with the A pass, two per iteration)",
-                     "kernel_us": apply_us, "step_us": step_us, "bytes_per_launch": apply_bytes,
-                     "bytes_per_voxel": APPLY_BYTES_PER_VOXEL},
+                     "kernel": "cn_sweep_kernel<float,8,4,4> (plain form: one Jacobi sweep "
+                               "of the CN solve, the 19-point stencil; ~3 per solver step)",
+                     "kernel_us": sweep_us, "step_us": step_us, "bytes_per_launch": sweep_bytes,
+                     "bytes_per_voxel": SWEEP_BYTES_PER_VOXEL,
+                     "bicgstab_apply": {"kernel": "cn_bi_pass_kernel<float,8,4,4,S> (the "
+                                        "BiCGSTAB product pass t = A M^-1 s; two stencil "
+                                        "passes per iteration)",
+                                        "kernel_us": kt["apply_us"], "achieved": apply_gbs,
+                                        "bytes_per_voxel": APPLY_BYTES_PER_VOXEL,
+                                        "step_us": kt["bicgstab_step_us"]}},
         "cpu_baseline": cpu,
         "stream_copy": copy,
         "bench_wall_s": round(time.perf_counter() - t_start, 1),

@@ -56,7 +56,7 @@ class HpCnPmj(C.Structure):
 class HpCnParams(C.Structure):
     _fields_ = [("cm_dt", c_f64), ("theta", c_f64), ("dscale", c_f64), ("dscale_pk", c_f64),
                 ("tol", c_f64), ("max_sweeps", c_i32), ("solver", c_i32), ("scalars", c_vp),
-                ("part", c_vp), ("part_len", c_i64)]
+                ("part", c_vp), ("part_len", c_i64), ("grid_cap", c_i64)]
 
 
 class HpCnWork(C.Structure):

@@ -467,12 +467,32 @@ struct BiZ {
   }
 };
 
-// Outputs of one row i of a pass given z_i, (L z)_i and -L_ii; accumulates
-// the pass's inner products in FP64.
+// The row's own entries a pass reads besides z: r (all but INIT), p, v, r^
+// (P), v (S).
+template <typename T>
+struct BiOwn {
+  T r, p, v, rh;
+};
+template <typename T, int MODE>
+__device__ __forceinline__ BiOwn<T> bi_own(const BiSide<T>& s, uint32_t i) {
+  BiOwn<T> o{T(0), T(0), T(0), T(0)};
+  if (MODE == BI_INIT) return o;
+  o.r = ld32((const T*)s.r, i);
+  if (MODE == BI_P || MODE == BI_S) o.v = ld32(s.vin, i);
+  if (MODE == BI_P) {
+    o.p = ld32(s.pin, i);
+    o.rh = ld32((const T*)s.rh, i);
+  }
+  if (MODE == BI_P0) o.rh = ld32((const T*)s.rh, i);
+  return o;
+}
+
+// Outputs of one row i of a pass given z_i, (L z)_i, -L_ii and the row's own
+// entries; accumulates the pass's inner products in FP64.
 template <typename T, int MODE>
 __device__ __forceinline__ void bi_row(const BiSide<T>& s, uint32_t i, T zc, T lz, T ldg,
-                                       const hp_cn_params& cp, T alpha, T beta, T omega,
-                                       bool reset, double (&q)[3]) {
+                                       const BiOwn<T>& o, const hp_cn_params& cp, T alpha,
+                                       T beta, T omega, bool reset, double (&q)[3]) {
 #pragma clang fp contract(off)
   const T cm_dt = T(cp.cm_dt), th = T(cp.theta);
   if (MODE == BI_INIT) {
@@ -485,24 +505,21 @@ __device__ __forceinline__ void bi_row(const BiSide<T>& s, uint32_t i, T zc, T l
     return;
   }
   const T az = cm_dt * zc - th * lz;
-  const T r = ld32((const T*)s.r, i);
   if (MODE == BI_S) {
-    const T sv = bi_sdir(r, ld32(s.vin, i), alpha);
+    const T sv = bi_sdir(o.r, o.v, alpha);
     st32(s.t, i, az);
     q[0] += (double)sv * (double)sv;
     q[1] += (double)az * (double)az;
     q[2] += (double)az * (double)sv;
     return;
   }
-  const T pn = MODE == BI_P0 ? r : bi_pdir(r, ld32(s.pin, i), ld32(s.vin, i), beta, omega);
+  const T pn = MODE == BI_P0 ? o.r : bi_pdir(o.r, o.p, o.v, beta, omega);
   st32(s.pout, i, pn);
   st32(s.vout, i, az);
-  T rh;
+  T rh = o.rh;
   if (reset) {
-    rh = r;
-    st32(s.rh, i, r);
-  } else {
-    rh = ld32((const T*)s.rh, i);
+    rh = o.r;
+    st32(s.rh, i, o.r);
   }
   q[0] += (double)rh * (double)az;
 }
@@ -576,7 +593,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_bi_pass_kernel(
           ldg += w;
         }
       }
-      bi_row<T, MODE>(cv, (uint32_t)i, zc, lz, ldg, cp, alpha, beta, omega, reset, q);
+      bi_row<T, MODE>(cv, (uint32_t)i, zc, lz, ldg, bi_own<T, MODE>(cv, (uint32_t)i), cp, alpha,
+                      beta, omega, reset, q);
     }
   } else {
     const BrickRange br = brick_range(blockIdx.x - ncb, gridDim.x - ncb, (uint32_t)g.n_bricks);
@@ -585,6 +603,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_bi_pass_kernel(
     const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
     const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
     const T dscale = T(cp.dscale);
+    // one brick per workgroup iteration (a workgroup takes ~2 bricks at the
+    // default grid; prefetching the next brick's halo measured slower)
     for (uint32_t b = br.first; b < br.end; b += br.stride) {
       __syncthreads();  // every lane is done with the previous tile
       {
@@ -611,7 +631,8 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_bi_pass_kernel(
           ldg += w;
         }
       }
-      bi_row<T, MODE>(m, slot, zc, lz, ldg, cp, alpha, beta, omega, reset, q);
+      bi_row<T, MODE>(m, slot, zc, lz, ldg, bi_own<T, MODE>(m, slot), cp, alpha, beta, omega,
+                      reset, q);
     }
   }
   bi_block_partials<(BVOL + 63) / 64, NV>(q, part);
@@ -951,7 +972,8 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
   int64_t ncb = ceil_div(ncab, bvol);
   if (ncb > CN_MAX_CABLE_BLOCKS) ncb = CN_MAX_CABLE_BLOCKS;
   ncb = (ncb + 7) / 8 * 8;
-  const int64_t G = ncb + (g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS);
+  const int64_t cap = cp->grid_cap > 0 ? cp->grid_cap : CN_SWEEP_BLOCKS;
+  const int64_t G = ncb + (g->n_bricks < cap ? g->n_bricks : cap);
   const int64_t cols = cp->solver == 1 ? 8 : 2;
   if (G > 0x7fffffff || cols * G > cp->part_len) {
     set_error("%s: the passes' %lld workgroups need part_len >= %lld doubles (got %lld)", what,
@@ -1106,7 +1128,8 @@ static int cn_apply_only(const hp_bricks* g, const hp_cn_params* cp, const hp_cn
     set_error("hp_cn_apply: %lld bricks exceed the 32-bit offset range", (long long)g->n_bricks);
     return -1;
   }
-  const int64_t G = g->n_bricks < CN_SWEEP_BLOCKS ? g->n_bricks : CN_SWEEP_BLOCKS;
+  const int64_t cap = cp->grid_cap > 0 ? cp->grid_cap : CN_SWEEP_BLOCKS;
+  const int64_t G = g->n_bricks < cap ? g->n_bricks : cap;
   if (G == 0) return 0;
   if (3 * G > cp->part_len) {
     set_error("hp_cn_apply: part_len >= %lld doubles required", (long long)(3 * G));

@@ -313,8 +313,8 @@ typedef struct hp_cn_pmj {
  * 16 the largest sweep (iteration) count of any step (4-7 and 16 accumulate
  * over calls; the caller zeroes them); 8-15 BiCGSTAB state.
  * part: device, doubles, part_len >= 2*G (solver 0) or 8*G (solver 1) with
- * G = min(n_bricks, 8192) + ncb the workgroups of a pass (ncb <= 520 for the
- * cable rows). */
+ * G = min(n_bricks, grid_cap) + ncb the workgroups of a pass (ncb <= 520 for
+ * the cable rows). */
 typedef struct hp_cn_params {
   double cm_dt;        /* Cm/dt */
   double theta;        /* 0 <= theta <= 1 (reference default 0.5) */
@@ -326,6 +326,7 @@ typedef struct hp_cn_params {
   double* scalars;
   double* part;
   int64_t part_len;    /* doubles in part */
+  int64_t grid_cap;    /* brick workgroups of a pass at most (0: 8192) */
 } hp_cn_params;
 
 /* Work vectors (device, the arithmetic type of the call): Vs, b, x of n_slots
