@@ -519,6 +519,8 @@ def standard_electrodes(grid, heart_centre_mm, n_bspm=256, seed=0):
     pos["LL"] = ray([0.3, 0, -1], bottom)
     for i, ang in enumerate(np.deg2rad([-20, 0, 20, 40, 60, 80])):
         pos[f"V{i + 1}"] = ray([np.sin(ang), -np.cos(ang), 0.0], hc)
+    if n_bspm <= 0:
+        return pos
     rng = np.random.default_rng(seed)
     nz = int(np.sqrt(n_bspm))
     na = n_bspm // nz

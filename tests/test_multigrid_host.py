@@ -123,3 +123,20 @@ def test_torso_chunk_table():
     with pytest.raises(RuntimeError, match="capacity"):
         nat.call("hp_torso_chunks", ni, nj, nk, *(v.ctypes.data for v in a), small.ctypes.data, 1,
                  ctypes.byref(cnt))
+
+
+def test_standard_electrodes_with_and_without_bspm():
+    """The 12-lead set alone (n_bspm = 0) and with a BSPM grid: limb and
+    precordial electrodes on the body surface, the BSPM sites added after them."""
+    from hrdayapy_amd import geometry as G
+    from hrdayapy_amd.ecg.functions.forward import standard_electrodes
+    from hrdayapy_amd.ecg.functions.torso import TorsoGrid
+    lab, origin, hc = G.torso((64, 64, 48), 6.0)
+    grid = TorsoGrid.from_labels(lab, G.TORSO_SIGMA, 6.0, origin)
+    twelve = ["RA", "LA", "LL"] + [f"V{i}" for i in range(1, 7)]
+    pos = standard_electrodes(grid, hc, n_bspm=0)
+    assert list(pos) == twelve
+    surf = {tuple(p) for p in grid["surface_xyz"].astype(float)}
+    assert all(tuple(np.asarray(p, float)) in surf for p in pos.values())
+    pos16 = standard_electrodes(grid, hc, n_bspm=16)
+    assert list(pos16)[:9] == twelve and len(pos16) == 25
