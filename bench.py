@@ -2,8 +2,9 @@
 """Benchmark: voxel-steps/s of the monodomain hot path on one MI355X.
 
 Workload (default): the 2M-voxel synthetic biventricular shell -- two
-truncated confocal ellipsoids voxelised at 0.4 mm in a 256^3 grid, rule-based
-helix fibres (+60 -> -60 deg), sigma_l = 0.24, sigma_t = 0.08 S/m, endo/mid/epi
+truncated confocal ellipsoids voxelised at 0.4 mm in a 256^3 grid, the
+Laplace transmural coordinate and the rule-based helix fibres (+60 -> -60
+deg) computed on the GPU, sigma_l = 0.24, sigma_t = 0.08 S/m, endo/mid/epi
 TTP06 by transmural thirds, a seeded Purkinje tree with 800 PMJs coupled at
 c_pmj = 2 mS (500 Ohm), FP32, dt = 0.02 ms, plain Rush-Larsen; stimulus at the
 Purkinje root and at an LV endocardial apical site (2 ms, 52 pA/pF), ECG (12
@@ -84,7 +85,7 @@ def measured_traffic():
 def build_geometry(n=256, h=0.4, seed=0):
     from hrdayapy_amd import geometry as G
     t0 = time.perf_counter()
-    geo = G.biv_shell(n, h, seed=seed)
+    geo = G.biv_shell(n, h, seed=seed, coords="laplace")
     nodes, el, act = G.purkinje_tree(geo, 800, seed=seed)
     kw = dict(voxel_size=h, dx_p=0.2, sigma_P=0.45, sigma_l=0.24, sigma_t=0.08, Cm=0.01,
               A_M=140.0, R_P=0.015, c_pmj=2.0, n_pmj=1, stim_len_mm=2.0, T=600.0,

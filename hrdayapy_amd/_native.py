@@ -59,6 +59,11 @@ class HpCnParams(C.Structure):
                 ("part", c_vp), ("part_len", c_i64), ("grid_cap", c_i64)]
 
 
+class HpLapCg(C.Structure):
+    _fields_ = [("x", c_vp), ("r", c_vp), ("p", c_vp * 2), ("ap", c_vp), ("scalars", c_vp),
+                ("part", c_vp), ("part_len", c_i64), ("tol", c_f64), ("launched", c_i64)]
+
+
 class HpCnWork(C.Structure):
     _fields_ = [("Vs", c_vp), ("b", c_vp), ("x", c_vp), ("Vs_pk", c_vp), ("b_pk", c_vp),
                 ("x_pk", c_vp), ("kr", c_vp * 8), ("kr_pk", c_vp * 8)]
@@ -136,6 +141,10 @@ _SIGS = {
                                  P(HpCnParams), P(HpCnWork), c_vp]),
     "hp_cn_sweep_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, P(HpCnParams), c_vp]),
     "hp_cn_apply_f32": (C.c_int, [P(HpBricks), P(HpCnParams), P(HpCnWork), c_vp]),
+    "hp_laplace_cg_begin_f32": (C.c_int, [P(HpBricks), P(HpLapCg), c_vp]),
+    "hp_laplace_cg_iterate_f32": (C.c_int, [P(HpBricks), P(HpLapCg), c_i32, c_i64, c_vp]),
+    "hp_fibre_rule_f32": (C.c_int, [c_vp, c_vp, c_i32, c_i32, c_i32, c_f64, c_f64, c_f64, c_f64,
+                                    c_f64, c_vp, c_vp]),
     "hp_cn_step_f64": (C.c_int, [P(HpBricks), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams),
                                  P(HpCable), c_vp, c_vp, c_vp, c_i64, P(HpMonoParams), P(HpCnPmj),
                                  P(HpCnParams), P(HpCnWork), c_vp]),

@@ -34,7 +34,8 @@ def _ellipsoid(X, Y, Z, c, ax):
     return ((X - c[0]) / ax[0]) ** 2 + ((Y - c[1]) / ax[1]) ** 2 + ((Z - c[2]) / ax[2]) ** 2 <= 1.0
 
 
-def biv_shell(n=256, h=0.4, seed=0, wall_mm=(11.5, 5.0)):
+def biv_shell(n=256, h=0.4, seed=0, wall_mm=(11.5, 5.0), coords="distance",
+              device="cuda"):
     """Biventricular shell on an n^3 grid of spacing h (mm).
 
     LV: confocal prolate shell (cavity semi-axes a_i, c_i; epicardium with the
@@ -42,7 +43,17 @@ def biv_shell(n=256, h=0.4, seed=0, wall_mm=(11.5, 5.0)):
     wider confocal shell shifted along x, wrapping the LV, free wall
     wall_mm[1]; both truncated by a base plane.  Returns dict(S, phi,
     fibre_dir, cavity_lv, cavity_rv, h).
+
+    coords="distance": phi is the distance ratio d_endo/(d_endo + d_epi) and
+    the fibres follow `rule_based_fibres` (host only).  coords="laplace": phi
+    is the reference's Laplace transmural coordinate and the fibres its
+    rule-based field from phi and an apicobasal coordinate psi (a Laplace
+    solve between the apex and base regions of `biv_apex_base_masks`), all on
+    the GPU (`hrdayapy_amd.coordinates`, `hrdayapy_amd.fibres`); the result
+    then also holds psi.
     """
+    if coords not in ("distance", "laplace"):
+        raise ValueError(f"coords must be 'distance' or 'laplace' (got {coords!r})")
     L = n * h
     ax = (np.arange(n) + 0.5) * h - L / 2
     X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
@@ -73,9 +84,60 @@ def biv_shell(n=256, h=0.4, seed=0, wall_mm=(11.5, 5.0)):
     d_epi = ndi.distance_transform_edt(~outside)
     phi = np.where(S, d_endo / np.maximum(d_endo + d_epi, 1e-6), np.nan).astype(np.float32)
     phi = np.clip(phi, 0.0, 1.0)
-    fibre = rule_based_fibres(S, phi)
-    return dict(S=S.astype(np.uint8), phi=phi, fibre_dir=fibre, cavity_lv=lv_cav & base,
-                cavity_rv=rv_cav & base, h=h)
+    geo = dict(S=S.astype(np.uint8), phi=phi, cavity_lv=lv_cav & base, cavity_rv=rv_cav & base,
+               h=h)
+    if coords == "distance":
+        geo["fibre_dir"] = rule_based_fibres(S, phi)
+        return geo
+    from .coordinates import LaplaceSolver, generate_transmural_coordinate
+    from .fibres import generate_fibre_direction
+    geo["phi"] = generate_transmural_coordinate(geo["S"], biv_surface_labels(geo), device=device,
+                                                verbose=False)
+    apex, basal = biv_apex_base_masks(geo)
+    vox = np.argwhere(S)
+    dirich = (apex | basal)[tuple(vox.T)]
+    z = vox[:, 2].astype(np.float32)
+    x0 = np.where(dirich, basal[tuple(vox.T)].astype(np.float32),
+                  (z - z.min()) / max(1.0, float(z.max() - z.min()))).astype(np.float32)
+    solver = LaplaceSolver(S, device)
+    psi = np.full(S.shape, np.nan, np.float32)
+    psi[tuple(vox.T)] = np.clip(solver.solve(dirich, x0, tol=1e-6, maxiter=20_000), 0.0, 1.0)
+    geo["psi"] = psi
+    geo["fibre_dir"] = generate_fibre_direction(S, geo["phi"], psi, (h, h, h), verbose=False,
+                                                device=device)
+    return geo
+
+
+def biv_surface_labels(geo):
+    """Transmural boundary labels of a BiV shell in the reference's convention
+    (+1 epicardium, -1 endocardium, 0 elsewhere): wall voxels face-adjacent to
+    a cavity are endocardial, wall voxels face-adjacent to the outside below
+    the base plane epicardial; the cut base itself is neither (Neumann)."""
+    S = geo["S"].astype(bool)
+    cav = geo["cavity_lv"] | geo["cavity_rv"]
+    k = S.shape[2]
+    rows = np.nonzero(S.any(axis=(0, 1)))[0]
+    top = int(rows.max()) if len(rows) else k - 1
+    below = np.zeros(S.shape, bool)
+    below[:, :, :top + 1] = True          # the base plane is the wall's top layer
+    outside = ~(S | cav) & below
+    st = ndi.generate_binary_structure(3, 1)
+    lab = np.zeros(S.shape, np.int8)
+    lab[S & ndi.binary_dilation(outside, st)] = 1
+    lab[S & ndi.binary_dilation(cav, st)] = -1
+    return lab
+
+
+def biv_apex_base_masks(geo, apex_frac=0.02):
+    """Dirichlet regions of the apicobasal coordinate: the wall voxels of the
+    lowest apex_frac of the wall's height (apex) and of its top layer (base)."""
+    S = geo["S"].astype(bool)
+    z = np.nonzero(S.any(axis=(0, 1)))[0]
+    z0, z1 = int(z.min()), int(z.max())
+    kk = np.arange(S.shape[2])[None, None, :]
+    apex = S & (kk <= z0 + max(1, int(round(apex_frac * (z1 - z0)))))
+    base = S & (kk == z1)
+    return apex, base
 
 
 def rule_based_fibres(S, phi, alpha_endo=60.0, alpha_epi=-60.0):

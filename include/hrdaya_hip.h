@@ -709,6 +709,49 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
 int hp_bigstep_reaction(const hp_bricks* g, const float* V, float* V_prev, float* state,
                         int64_t ld, float* rate, const hp_surrogate_params* sp, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Geometry set-up: the transmural coordinate and the
+ * rule-based fibre field.
+ *
+ * Laplace solve of generate_transmural_coordinate.py:35-186 on a brick grid
+ * of the myocardium: label bit 0 set = a Dirichlet row (identity; its value
+ * is already in x), otherwise the 6-neighbour graph Laplacian (Neumann at the
+ * mask boundary); label 0xFF = padding; D6 rows 0-2 = 1 and rows 3-5 = 0 on
+ * the tissue slots.  CG in FP32 with FP64 inner products and the reference's
+ * rule ||r|| / ||r0|| < tol, its decisions on the device: begin forms r and p
+ * from x; iterate launches n_iter iterations (4 kernels each; those after
+ * the stop return at once) and advances `launched`.  scalars (device, >= 8
+ * doubles): 0 r.r, 1 ||r0||, 2 alpha, 3 beta, 4 done (1 converged, 2 stopped
+ * at max_iter), 5 iterations, 6 last ||r||/||r0||.  part: device, >= n_bricks
+ * doubles.  Replaces _conjugate_gradient and the torch sparse Laplacian of
+ * generate_transmural_coordinate.py:35-78, 85-183.
+ * ------------------------------------------------------------------------- */
+typedef struct hp_lap_cg {
+  float* x;          /* n_slots: the iterate */
+  float* r;          /* n_slots */
+  float* p[2];       /* n_slots each: the direction, alternating per iteration */
+  float* ap;         /* n_slots */
+  double* scalars;
+  double* part;
+  int64_t part_len;
+  double tol;
+  int64_t launched;  /* iterations launched since begin (host side) */
+} hp_lap_cg;
+int hp_laplace_cg_begin_f32(const hp_bricks* g, hp_lap_cg* cg, void* stream);
+int hp_laplace_cg_iterate_f32(const hp_bricks* g, hp_lap_cg* cg, int32_t n_iter,
+                              int64_t max_iter, void* stream);
+
+/* Rule-based fibre frame (generate_fibre_direction.py:135-281) on a dense
+ * nx*ny*nz box (C order) of the filled phi and psi fields (the nearest
+ * in-mask value in a 2-voxel halo, 0 beyond: _fill_nan_nearest, :90-111):
+ * gradients by np.gradient's rule at spacings (hx, hy, hz), e_t, e_l, e_c,
+ * the helix angle alpha_endo + (alpha_epi - alpha_endo) phi (radians) and
+ * f = cos(a) e_c + sin(a) e_l, normalised, into f[3*idx + c], in FP32 in the
+ * reference's order of operations.  Every side >= 2. */
+int hp_fibre_rule_f32(const float* phi_filled, const float* psi_filled, int32_t nx, int32_t ny,
+                      int32_t nz, double hx, double hy, double hz, double alpha_endo,
+                      double alpha_epi, float* f, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
