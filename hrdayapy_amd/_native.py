@@ -14,7 +14,7 @@ from pathlib import Path
 _LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
 _lib = None
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
 
@@ -184,7 +184,7 @@ _SIGS = {
                                          c_i64, c_vp, c_vp]),
     "hp_heart_lap6_f32": (C.c_int, [P(HpBricks), c_vp, c_vp, c_f64, c_vp]),
     "hp_bspm_source_f64": (C.c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_f64, c_f64,
-                                     c_vp, c_i64, P(HpPcgWs), c_vp]),
+                                     c_vp, c_vp, c_i64, P(HpPcgWs), c_vp]),
     "hp_leads_accumulate_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gemm_bf16": (C.c_int, [C.c_int, C.c_int, C.c_int, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64,
                                c_vp, c_i64, c_f32, c_f32, c_vp, C.c_int, c_vp]),
@@ -206,6 +206,27 @@ _SIGS = {
     "hp_gather_f32": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_gather_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "hp_stream_copy_f32": (C.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "hp_register_dice_f64": (C.c_int, [c_vp, c_i64, c_vp, c_i64, c_vp, c_i32, c_vp, c_vp, c_vp,
+                                       c_vp, c_f64, c_vp, c_i64, c_vp, c_vp]),
+    "hp_project_rows_f64": (C.c_int, [c_vp, c_i64, c_vp, c_f64, c_vp, c_f64, c_vp, c_i32, c_i32,
+                                      c_i32, c_vp, c_vp]),
+    "hp_feature_transform": (C.c_int, [c_vp, C.c_uint8, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp,
+                                       c_vp, c_i64, c_vp]),
+    "hp_sbm_psi_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_f64, c_vp, c_vp]),
+    "hp_zoom_linear_f64": (C.c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_f64, c_vp, c_vp,
+                                     c_vp]),
+    "hp_zoom_nearest_i32": (C.c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp]),
+    "hp_label_sigma_f64": (C.c_int, [c_vp, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+    "hp_extend_sigma_f64": (C.c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "hp_column_stats": (C.c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "hp_scan_i32": (C.c_int, [c_vp, c_i64, c_vp, c_vp]),
+    "hp_column_emit": (C.c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                 c_vp]),
+    "hp_surface_flags": (C.c_int, [c_vp, c_i32, c_i32, c_i32, c_vp, c_vp]),
+    "hp_select_rank": (C.c_int, [c_vp, c_vp, c_i64, c_i32, c_i64, c_vp, c_vp]),
+    "hp_torso_faces_f64": (C.c_int, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_f64, c_vp, c_vp,
+                                     c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp,
+                                     c_vp]),
 }
 
 

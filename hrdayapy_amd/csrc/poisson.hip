@@ -438,14 +438,17 @@ __global__ void source_gather_kernel(const double* __restrict__ lap, const int32
                                      const int32_t* __restrict__ ptr, const int32_t* __restrict__ fine,
                                      int64_t n_rows_hit, const double* __restrict__ sum_n,
                                      double inv_nvalid, double coeff, double inv_nexp,
-                                     double* __restrict__ b, int64_t pin) {
+                                     const double* __restrict__ psi, double* __restrict__ b,
+                                     int64_t pin) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_rows_hit) return;
   const double mean = sum_n[0] * inv_nvalid;
   double s = 0.0;
   for (int e = ptr[q]; e < ptr[q + 1]; ++e) s += lap[fine[e]] - mean;
   const int row = rows[q];
-  b[row] = row == pin ? 0.0 : coeff * s * inv_nexp;
+  double v = coeff * s * inv_nexp;
+  if (psi) v *= psi[row];  // smoothed boundary: the row's psi scales its source
+  b[row] = row == pin ? 0.0 : v;
 }
 
 // Lead-field ECG: out[f][e] = sum_r W[r][e] * V[f][slot[r]] over the rows r
@@ -781,8 +784,8 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
 
 int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,
                        const int32_t* rows, const int32_t* ptr, const int32_t* fine,
-                       int64_t n_rows_hit, double coeff, double n_expected, double* b,
-                       int64_t pin, hp_pcg_ws* ws, void* stream) {
+                       int64_t n_rows_hit, double coeff, double n_expected, const double* psi,
+                       double* b, int64_t pin, hp_pcg_ws* ws, void* stream) {
   if (!ws || !ws->part || !ws->out || n_valid <= 0 || n_expected <= 0.0) {
     set_error("hp_bspm_source_f64: bad arguments");
     return -1;
@@ -798,7 +801,7 @@ int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_
   if (n_rows_hit > 0)
     hipLaunchKernelGGL(source_gather_kernel, dim3(ceil_div(n_rows_hit, 256)), dim3(256), 0, st,
                        lap, rows, ptr, fine, n_rows_hit, ws->out, 1.0 / (double)n_valid, coeff,
-                       1.0 / n_expected, b, pin);
+                       1.0 / n_expected, psi, b, pin);
   return check_launch("hp_bspm_source_f64/gather");
 }
 

@@ -23,6 +23,7 @@ import torch
 
 from ... import _native as nat
 from ...layout import BrickLayout
+from .registration import projection_rows
 
 
 def _face_sum(Gx, Gy, Gz):
@@ -313,19 +314,6 @@ class PoissonSolver:
         return self.solve_metric(b, x, tol_V, max_iter, "rms", check_every)
 
 
-def projection_rows(coords_mm, R, t, scale, origin_mm, dx_mm, id_map, shape):
-    """Heart node -> torso row (-1 if outside), as build_projection_map (:126-157)."""
-    torso = (np.asarray(R) @ (np.asarray(coords_mm, dtype=np.float64) * scale).T).T + np.asarray(t)
-    ijk = np.floor((torso - np.asarray(origin_mm)) / dx_mm).astype(np.int64)
-    ni, nj, nk = shape
-    inb = ((ijk[:, 0] >= 0) & (ijk[:, 0] < ni) & (ijk[:, 1] >= 0) & (ijk[:, 1] < nj)
-           & (ijk[:, 2] >= 0) & (ijk[:, 2] < nk))
-    proj = np.full(len(torso), -1, dtype=np.int64)
-    flat = (ijk[inb, 0] * nj + ijk[inb, 1]) * nk + ijk[inb, 2]
-    proj[np.nonzero(inb)[0]] = id_map[flat]
-    return proj
-
-
 class HeartSource:
     """Per-frame torso right-hand side from the heart's Vm on the brick layout."""
 
@@ -335,7 +323,7 @@ class HeartSource:
         self.lay = layout
         R, t, sc = registration["R"], registration["t"], float(registration["scale"])
         proj = projection_rows(coords_mm, R, t, sc, grid["origin_mm"], grid["dx_mm"],
-                               grid["active_id_map"], grid["shape"])
+                               grid["active_id_map"], grid["shape"], dev)
         valid = proj >= 0
         self.n_valid = int(valid.sum())
         if self.n_valid == 0:
@@ -353,6 +341,10 @@ class HeartSource:
         self.coeff = sigma_i / h_m ** 2
         self.n_expected = (grid["dx_mm"] / h_fine_mm) ** 3
         self.pin = int(grid.layout["pin"])
+        # smoothed boundary: each torso row's source is scaled by its psi
+        # (compute_bspm.py:413-414); None for the sharp boundary
+        psi = getattr(grid, "psi_cell", None)
+        self.psi = None if psi is None else torch.as_tensor(psi, dtype=torch.float64).to(dev)
         self.lap = torch.zeros(layout.n_slots, dtype=torch.float64, device=dev)
         if bricks is None:
             self._label = torch.from_numpy(layout.labels(np.zeros(layout.n_nodes, np.uint8))).to(dev)
@@ -369,8 +361,8 @@ class HeartSource:
                  self.lap.data_ptr(), 1e-3, s)
         nat.call("hp_bspm_source_f64", self.lap.data_ptr(), self.valid_slots.data_ptr(),
                  self.n_valid, self.rows.data_ptr(), self.ptr.data_ptr(), self.fine.data_ptr(),
-                 int(self.rows.numel()), self.coeff, self.n_expected, b64.data_ptr(), self.pin,
-                 C.byref(ws), s)
+                 int(self.rows.numel()), self.coeff, self.n_expected, nat.ptr(self.psi),
+                 b64.data_ptr(), self.pin, C.byref(ws), s)
 
 
 class LeadField:
@@ -422,6 +414,8 @@ class LeadField:
             ps.solve_refined(rhs, y, tol=tol, max_outer=8, max_inner=2000)
             self.solve_stats.append(dict(ps.last_stats))
             g = y[src.valid_cells]
+            if src.psi is not None:  # the adjoint of the row scale psi
+                g = g * src.psi[src.valid_cells]
             h.zero_()
             h[vslots] = g - g.mean()
             nat.call("hp_heart_lap6_f64", C.byref(src.bricks), h.data_ptr(), kh.data_ptr(), 1e-3,

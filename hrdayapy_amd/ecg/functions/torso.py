@@ -3,46 +3,18 @@ operator, the pinned reference cell and the body-surface cells.
 
 Reference: hrdayapy/ecg/functions/build_torso_grid.py
   * sigma volume / tissue mask        :554-576 (active = label != 0)
-  * 7-point harmonic-mean stiffness   :276-397 (sharp boundary)
+  * 7-point harmonic-mean stiffness   :276-397 (sharp and smoothed boundary)
   * pinned cell                       :616-623 (middle tissue voxel in flat order)
   * surface cells                     :469-488 (tissue voxels face-adjacent to air)
   * returned dict keys                :672-691
-The operator coefficients are computed vectorised on the host once and held
-in HBM in the column-run layout consumed by the HIP kernels (no CSR matrix is
-formed); ``stiffness_csr`` builds the equivalent scipy matrix on request.
+The grid is built by HIP kernels (ecg/functions/torso_build.py) and held in
+HBM in the column-run layout consumed by the solver kernels (no CSR matrix is
+formed); ``stiffness_csr`` builds the equivalent scipy matrix on request from
+the device's FP64 face conductances.
 """
 from __future__ import annotations
 
 import numpy as np
-from scipy import ndimage as ndi
-
-
-def _harm(a, b):
-    s = a + b
-    return np.where(s > 1e-30, 2.0 * a * b / np.where(s > 1e-30, s, 1.0), 0.0)
-
-
-def _face_conductances(sigma, tissue, flat_of_cell, dx_mm):
-    """FP64 conductance of the +x/+y/+z face of every stored cell:
-    harm(sigma_c, sigma_nb)/h^2 between two active cells, 0 otherwise
-    (build_torso_grid.py:325-390, sharp boundary)."""
-    ni, nj, nk = sigma.shape
-    sflat, act = sigma.ravel(), tissue.ravel()
-    inv_h2 = 1.0 / (dx_mm * 1e-3) ** 2
-    n_cells = len(flat_of_cell)
-    ci, cj, kk = flat_of_cell // (nj * nk), (flat_of_cell // nk) % nj, flat_of_cell % nk
-
-    def face(off_flat, valid):
-        g = np.zeros(n_cells, dtype=np.float64)
-        f = flat_of_cell[valid]
-        nb = f + off_flat
-        both = act[f] & act[nb]
-        gv = np.zeros(len(f))
-        gv[both] = _harm(sflat[f[both]], sflat[nb[both]]) * inv_h2
-        g[valid] = gv
-        return g
-
-    return face(nj * nk, ci + 1 < ni), face(nk, cj + 1 < nj), face(1, kk + 1 < nk)
 
 
 class TorsoGrid(dict):
@@ -84,80 +56,90 @@ class TorsoGrid(dict):
         return solve_fn
 
     @classmethod
-    def from_labels(cls, label_vol, sigma_map, dx_mm, origin_mm=(0.0, 0.0, 0.0), verbose=False):
-        lab = np.asarray(label_vol)
-        shape = lab.shape
+    def from_labels(cls, label_vol, sigma_map, dx_mm, origin_mm=(0.0, 0.0, 0.0), verbose=False,
+                    *, psi_vol=None, device="cuda"):
+        """Build the grid of a coarse label volume on the device (the kernels
+        of ecg/functions/torso_build.py).  psi_vol (coarse, FP64; optional)
+        selects the smoothed-boundary system: active = psi > 0, sigma extended
+        through the band, conductances scaled by the harmonic mean of psi."""
+        import torch
+        from . import torso_build as TB
+        dev = torch.device(device)
+        if isinstance(label_vol, torch.Tensor):
+            lab = label_vol.to(dev, torch.int32).contiguous()
+        else:
+            lab = torch.as_tensor(np.ascontiguousarray(label_vol, dtype=np.int32)).to(dev)
+        shape = tuple(int(x) for x in lab.shape)
         ni, nj, nk = shape
-        sigma = np.zeros(shape, dtype=np.float64)
-        for k, v in sigma_map.items():
-            sigma[lab == k] = v
-        tissue = lab != 0
-        active_flat = np.flatnonzero(tissue.ravel()).astype(np.int64)
-        n_act = len(active_flat)
-        id_map = np.full(lab.size, -1, dtype=np.int32)
-        id_map[active_flat] = np.arange(n_act, dtype=np.int32)
-        # column runs over the active cells
-        any_col = tissue.any(axis=2)
-        k_first = np.where(any_col, tissue.argmax(axis=2), 0)
-        k_last = np.where(any_col, nk - 1 - tissue[:, :, ::-1].argmax(axis=2), -1)
-        col_len = np.where(any_col, k_last - k_first + 1, 0).astype(np.int64)
-        col_off = np.zeros(ni * nj, dtype=np.int64)
-        col_off[1:] = np.cumsum(col_len.ravel())[:-1]
-        n_cells = int(col_len.sum())
-        if n_cells >= 2 ** 31:
-            raise ValueError("torso grid too large for 32-bit cell indices")
-        # stored cell -> flat index
-        cols = np.repeat(np.arange(ni * nj), col_len.ravel())
-        dk = np.arange(n_cells) - col_off[cols]
-        kk = k_first.ravel()[cols] + dk
-        flat_of_cell = cols * nk + kk
-        gx, gy, gz = _face_conductances(sigma, tissue, flat_of_cell, dx_mm)
-        act = tissue.ravel()
-        ci, cj = cols // nj, cols % nj
-        # diag = sum of the six faces: + faces of this cell and + faces of the - neighbours
-        cell_of_flat = np.full(lab.size, -1, dtype=np.int64)
-        cell_of_flat[flat_of_cell] = np.arange(n_cells)
-        diag = gx + gy + gz
-        for g, off, valid in ((gx, nj * nk, ci > 0), (gy, nk, cj > 0), (gz, 1, kk > 0)):
-            nb = cell_of_flat[flat_of_cell[valid] - off]
-            ok = nb >= 0
-            idx = np.nonzero(valid)[0][ok]
-            diag[idx] += g[nb[ok]]
-        hole = ~act[flat_of_cell]
-        zero_row = (diag <= 0.0) & ~hole
-        diag[hole | zero_row] = 1.0
-        # pinned reference cell, build_torso_grid.py:621-623
-        interior = np.flatnonzero(tissue.ravel())
-        pin_flat = int(interior[len(interior) // 2])
+        sigma, tissue = TB.label_sigma(lab, sigma_map, dev)
+        psi = None
+        if psi_vol is not None:
+            psi = torch.as_tensor(psi_vol, dtype=torch.float64).to(dev).contiguous()
+            if tuple(psi.shape) != shape:
+                raise ValueError(f"psi_vol shape {tuple(psi.shape)} != label shape {shape}")
+            active = (psi > 0.0).to(torch.uint8)
+            sigma = TB.extend_conductivity(sigma, tissue, active, dev)
+        else:
+            active = tissue
+        out = TB.assemble(sigma, tissue, active, dx_mm, psi, dev)
+        h = lambda t: t.cpu().numpy()  # noqa: E731
+        n_cells = out["n_cells"]
+        col_k0, col_len, col_off = h(out["k0"]), h(out["run"]), h(out["col_off"])
+        cols = np.repeat(np.arange(ni * nj, dtype=np.int64), col_len.astype(np.int64))
+        flat_of_cell = cols * nk + col_k0[cols] + (np.arange(n_cells) - col_off[cols])
+        active_ids = h(out["active_ids"])
+        id_map = h(out["id_map"])
+        pin_flat = out["pin_flat"]
         pin_row = int(id_map[pin_flat])
-        pin_cell = int(cell_of_flat[pin_flat])
-        diag[pin_cell] = 1.0
-        # body surface, build_torso_grid.py:469-488
-        air = ~tissue
-        surf = tissue & ndi.binary_dilation(air, structure=ndi.generate_binary_structure(3, 1))
-        surface_flat = np.flatnonzero(surf.ravel()).astype(np.int32)
+        c = pin_flat // nk
+        pin_cell = int(col_off[c] + (pin_flat % nk) - col_k0[c])
+        surface_flat = h(out["surface_flat"])
         surface_rows = id_map[surface_flat]
         sidx = np.stack(np.unravel_index(surface_flat, shape), axis=1)
         surface_xyz = (np.asarray(origin_mm) + sidx * dx_mm).astype(np.float32)
+        tissue_h = h(tissue).astype(bool)
         g = cls(shape=shape, dx_mm=float(dx_mm), origin_mm=np.asarray(origin_mm, dtype=np.float64),
-                sigma_vol=sigma, active_mask=tissue, tissue_mask=tissue,
-                active_ids=active_flat.astype(np.int32), active_id_map=id_map,
-                surface_flat=surface_flat, surface_rows=surface_rows,
-                surface_xyz=surface_xyz, pin_row=pin_row, use_sbm=False)
-        g.layout = dict(col_k0=k_first.ravel().astype(np.int32),
-                        col_len=col_len.ravel().astype(np.int32),
-                        col_off=col_off.astype(np.int32), gx=gx.astype(np.float32),
-                        gy=gy.astype(np.float32), gz=gz.astype(np.float32),
-                        diag=diag.astype(np.float32), pin=pin_cell, n_cells=n_cells)
-        g.cell_of_row = cell_of_flat[active_flat].astype(np.int64)
+                sigma_vol=h(sigma), active_mask=h(active).astype(bool), tissue_mask=tissue_h,
+                active_ids=active_ids, active_id_map=id_map, surface_flat=surface_flat,
+                surface_rows=surface_rows, surface_xyz=surface_xyz, pin_row=pin_row,
+                use_sbm=psi is not None)
+        if psi is not None:
+            g["psi_vol"] = h(psi)
+        g.layout = dict(col_k0=col_k0.astype(np.int32), col_len=col_len.astype(np.int32),
+                        col_off=col_off.astype(np.int32), gx=h(out["gx"]), gy=h(out["gy"]),
+                        gz=h(out["gz"]), diag=h(out["diag"]), pin=pin_cell, n_cells=n_cells)
+        g.g64 = out["g64"]
+        g.cell_of_row = h(out["cell_of_row"]).astype(np.int64)
         g.flat_of_cell = flat_of_cell
+        # psi of every stored cell (the source's row scale), 0 on cells outside the system
+        g.psi_cell = None if psi is None else psi.reshape(-1)[
+            torch.from_numpy(flat_of_cell).to(dev)].contiguous()
         if verbose:
-            print(f"  torso {shape} @ {dx_mm} mm: {n_act:,} active, {n_cells:,} stored cells, "
+            kind = "SBM" if psi is not None else "sharp"
+            print(f"  torso {shape} @ {dx_mm} mm ({kind}, built on the device): "
+                  f"{len(active_ids):,} active, {n_cells:,} stored cells, "
                   f"{len(surface_flat):,} surface, pin row {pin_row}")
         return g
 
+    @classmethod
+    def from_segmentation(cls, fine_labels, spacing, sigma_map, dx_mm, origin_mm=(0.0, 0.0, 0.0),
+                          *, use_sbm=False, sbm_interface_width_mm=3.0, sbm_psi_cutoff=1e-3,
+                          verbose=False, device="cuda"):
+        """Resample a fine segmentation to dx_mm (nearest neighbour) and build
+        its grid; with use_sbm the domain function comes from the fine
+        segmentation (build_torso_grid.py:549-596)."""
+        from . import torso_build as TB
+        coarse = TB.resample_labels(fine_labels, spacing, dx_mm, device)
+        psi = None
+        if use_sbm:
+            psi, _ = TB.smoothed_domain(fine_labels, spacing, dx_mm, tuple(coarse.shape),
+                                        sbm_interface_width_mm, sbm_psi_cutoff, device)
+        return cls.from_labels(coarse, sigma_map, dx_mm, origin_mm, verbose, psi_vol=psi,
+                               device=device)
+
     def stiffness_csr(self):
-        """The (unpinned) FP64 stiffness in row numbering, for small grids."""
+        """The (unpinned) FP64 stiffness in row numbering (off-diagonals from the
+        FP64 face conductances, diagonal from the kernel's six-face sum)."""
         import scipy.sparse as sp
         L = self.layout
         cor = self.cell_of_row
@@ -168,17 +150,16 @@ class TorsoGrid(dict):
         flat = self.flat_of_cell
         cell_of_flat = np.full(ni * nj * nk, -1, dtype=np.int64)
         cell_of_flat[flat] = np.arange(L["n_cells"])
-        rows, cols, vals = [], [], []
-        g64 = _face_conductances(np.asarray(self["sigma_vol"], dtype=np.float64),
-                                 np.asarray(self["tissue_mask"]), flat, self["dx_mm"])
-        for g, off in zip(g64, (nj * nk, nk, 1)):
+        g64 = self.g64.cpu().numpy() if hasattr(self.g64, "cpu") else np.asarray(self.g64)
+        rows, cols, vals = [np.arange(n)], [np.arange(n)], [g64[3][cor]]
+        for g, off in zip(g64[:3], (nj * nk, nk, 1)):
             c = np.nonzero(g > 0)[0]
             nb = cell_of_flat[flat[c] + off]
             a, b = row_of_cell[c], row_of_cell[nb]
             w = g[c]
-            rows += [a, b, a, b]
-            cols += [b, a, a, b]
-            vals += [-w, -w, w, w]
+            rows += [a, b]
+            cols += [b, a]
+            vals += [-w, -w]
         A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                           shape=(n, n)).tocsr()
         return A

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 4
+#define HP_ABI_VERSION 5
 
 /* ---------------------------------------------------------------------------
  * Library
@@ -577,7 +577,9 @@ int hp_cast_scale_f32(const hp_torso* t, const double* r64, float* r32, double s
  * compute_bspm.py:88-119, 381-419): lap = vscale * (6-neighbour unit-weight
  * graph Laplacian of V) on the brick layout (FP64 out); then
  * b[rows[q]] = coeff * sum_{e in ptr[q]..} (lap[fine[e]] - mean) / n_expected
- * with mean over valid_slots, and b[pin] = 0.  Rows not listed are untouched. */
+ * with mean over valid_slots, times psi[rows[q]] when psi (the smoothed-
+ * boundary domain function per stored cell, :413-414) is given, and
+ * b[pin] = 0.  Rows not listed are untouched. */
 int hp_heart_lap6_f32(const hp_bricks* g, const float* V, double* lap, double vscale,
                       void* stream);
 int hp_heart_lap6_f64(const hp_bricks* g, const double* V, double* lap, double vscale,
@@ -596,8 +598,8 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
                            int64_t part_len, double* out, void* stream);
 int hp_bspm_source_f64(const double* lap, const int32_t* valid_slots, int64_t n_valid,
                        const int32_t* rows, const int32_t* ptr, const int32_t* fine,
-                       int64_t n_rows_hit, double coeff, double n_expected, double* b,
-                       int64_t pin, hp_pcg_ws* ws, void* stream);
+                       int64_t n_rows_hit, double coeff, double n_expected, const double* psi,
+                       double* b, int64_t pin, hp_pcg_ws* ws, void* stream);
 /* Electrode potentials, FP64: out[e] = sum_k w[k]*phi[idx[k]], k in [ptr[e], ptr[e+1]).
  * No counterpart in the reference (it outputs raw surface potentials only). */
 int hp_leads_accumulate_f64(const double* phi, const int32_t* ptr, const int32_t* idx,
@@ -751,6 +753,112 @@ int hp_laplace_cg_iterate_f32(const hp_bricks* g, hp_lap_cg* cg, int32_t n_iter,
 int hp_fibre_rule_f32(const float* phi_filled, const float* psi_filled, int32_t nx, int32_t ny,
                       int32_t nz, double hx, double hy, double hz, double alpha_endo,
                       double alpha_epi, float* f, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Heart-to-torso registration: the voxel Dice of the 48 signed permutations
+ * of the rotation search (register_heart_to_torso.py:146-224, its Dice
+ * _voxel_dice :109-138).  Candidate r is s_i * src_c[:, perm_i] + c_tgt[i]
+ * with perm_sign[6r..6r+5] = {perm_0, perm_1, perm_2, s_0, s_1, s_2}; it and
+ * the target tgt (n_tgt x 3) are voxelised at floor((x - mn[3r+a]) / vox_mm)
+ * on a dims[3r..3r+2] lattice whose bits live at bits[word_off[r]] (candidate,
+ * then target, (word_off[r+1] - word_off[r]) / 2 words each; word_off[n_perm]
+ * <= n_words).  counts (device, 3 n_perm + 1 uint64): per permutation the
+ * occupied voxels of the candidate, of the target and of both; the last
+ * entry counts points outside their lattice (0 for correct mn / dims, and
+ * such points are never written).  FP64 arithmetic rounded as numpy's.
+ * ------------------------------------------------------------------------- */
+int hp_register_dice_f64(const double* src_c, int64_t n_src, const double* tgt, int64_t n_tgt,
+                         const int32_t* perm_sign, int32_t n_perm, const double* c_tgt,
+                         const double* mn, const int32_t* dims, const int64_t* word_off,
+                         double vox_mm, uint32_t* bits, int64_t n_words,
+                         unsigned long long* counts, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Coarse torso grid on the device (build_torso_grid.py).  Dense volumes are
+ * C order (n0, n1, n2); flags are uint8 0/1.
+ * ------------------------------------------------------------------------- */
+/* Feature transform: for every voxel the coordinates of its nearest voxel
+ * with mask == feature_value (Euclidean, voxel spacing `spacing`, a host
+ * array of 3), into ft[3][N] int32 (-1 where there is none), by lower
+ * envelopes along axis 0, then 1, then 2 (Maurer, Qi, Raghavan 2003).  The
+ * nearest-feature search of scipy.ndimage.distance_transform_edt, which
+ * compute_smoothed_domain (:169-171) and extend_conductivity (:202) call.
+ * ft_tmp: 3N int32; stack: stack_len int32 (>= the longest side; lines are
+ * processed in batches of stack_len / side). */
+int hp_feature_transform(const uint8_t* mask, uint8_t feature_value, int32_t n0, int32_t n1,
+                         int32_t n2, const double* spacing, int32_t* ft, int32_t* ft_tmp,
+                         int32_t* stack, int64_t stack_len, void* stream);
+/* psi = (1 + tanh(3 d / width_mm)) / 2 of the signed distance d (+ inside,
+ * mm) from the two feature transforms (nearest outside voxel, nearest inside
+ * voxel): compute_smoothed_domain :169-173. */
+int hp_sbm_psi_f64(const uint8_t* inside, const int32_t* ft_nearest_out,
+                   const int32_t* ft_nearest_in, int32_t n0, int32_t n1, int32_t n2,
+                   const double* spacing, double width_mm, double* psi, void* stream);
+/* Linear zoom of a dense FP64 volume to oshape (host int32[3], corner to
+ * corner, 0 beyond the input's edges), cropped / padded with 0 to mshape
+ * (host int32[3]), values below cutoff set to 0 (compute_smoothed_domain
+ * :175-186, scipy.ndimage.zoom order 1); positive (optional) = out > 0. */
+int hp_zoom_linear_f64(const double* in, int32_t n0, int32_t n1, int32_t n2, const int32_t* oshape,
+                       const int32_t* mshape, double cutoff, double* out, uint8_t* positive,
+                       void* stream);
+/* Nearest-neighbour zoom of a label volume to oshape (host int32[3]):
+ * _resample_labels :260-269 (scipy.ndimage.zoom order 0). */
+int hp_zoom_nearest_i32(const int32_t* in, int32_t n0, int32_t n1, int32_t n2,
+                        const int32_t* oshape, int32_t* out, void* stream);
+/* sigma[v] = vals[q] for keys[q] == labels[v] (0 without an entry), tissue =
+ * label != 0: build_torso_grid :554-576. */
+int hp_label_sigma_f64(const int32_t* labels, int64_t n, const int32_t* keys, const double* vals,
+                       int32_t n_keys, double* sigma, uint8_t* tissue, void* stream);
+/* sigma_out = sigma_in, with the band cells (active, not tissue) taking the
+ * sigma of their nearest tissue cell from ft_tissue: extend_conductivity
+ * :189-206. */
+int hp_extend_sigma_f64(const int32_t* ft_tissue, const uint8_t* tissue, const uint8_t* active,
+                        const double* sigma_in, int32_t n0, int32_t n1, int32_t n2,
+                        double* sigma_out, void* stream);
+/* Per column (i, j) of a flag volume: first set k (k0), run length to the last
+ * set k (run) and number of set flags (cnt); each output optional. */
+int hp_column_stats(const uint8_t* flag, int32_t ni, int32_t nj, int32_t nk, int32_t* k0,
+                    int32_t* run, int32_t* cnt, void* stream);
+/* out[0..n]: exclusive prefix sums of in, out[n] = total (one workgroup). */
+int hp_scan_i32(const int32_t* in, int64_t n, int64_t* out, void* stream);
+/* Walk each column's set flags in k order from row base[col]: list[row] =
+ * flat index (optional), id_map[flat] = row or -1 (optional, every cell),
+ * cell_of_row[row] = col_off[col] + k - k0[col] (optional). */
+int hp_column_emit(const uint8_t* flag, int32_t ni, int32_t nj, int32_t nk, const int64_t* base,
+                   int32_t* list, int32_t* id_map, const int32_t* k0, const int64_t* col_off,
+                   int64_t* cell_of_row, void* stream);
+/* surface = tissue cells with an in-box face neighbour outside the tissue:
+ * _find_surface_ids :469-488 (6-connected dilation of the air). */
+int hp_surface_flags(const uint8_t* tissue, int32_t ni, int32_t nj, int32_t nk, uint8_t* surf,
+                     void* stream);
+/* out[0] = flat index of the set flag of the given rank in flat order (scan:
+ * the exclusive sums of the per-column counts, n_cols + 1 entries): the
+ * pinned cell interior[len // 2] of :616-623. */
+int hp_select_rank(const uint8_t* flag, const int64_t* scan, int64_t n_cols, int32_t nk,
+                   int64_t rank, int64_t* out, void* stream);
+/* Column-run stiffness of _assemble_stiffness (:276-397): per stored cell of
+ * the runs (col_k0, col_len, col_off), the +i/+j/+k face conductances
+ * harm(sigma)/h^2 (times harm(psi) when psi is given; 0 unless both cells
+ * are active) and the diagonal summed over the six faces in the reference's
+ * order (1 for inactive cells and rows without conductance), FP32 into gx,
+ * gy, gz, diag (diag 1 on pin_flat: the pinned row) and, when g64 is given,
+ * FP64 into g64[4][n_cells] (the unpinned matrix).
+ * bad (device uint64) counts nonzero conductances to inactive cells. */
+int hp_torso_faces_f64(const double* sigma, const double* psi, const uint8_t* active, int32_t ni,
+                       int32_t nj, int32_t nk, double h_m, const int32_t* col_k0,
+                       const int32_t* col_len, const int64_t* col_off, int64_t pin_flat,
+                       float* gx, float* gy, float* gz, float* diag, double* g64,
+                       int64_t n_cells, unsigned long long* bad, void* stream);
+
+/* Projection of heart nodes into the coarse torso rows
+ * (compute_bspm.py:126-157, build_projection_map): xyz (n x 3, mm), Rt =
+ * {R row-major (9), t (3)}, proj[i] = id_map[flat(floor((R (x scale) + t -
+ * origin) / dx_mm))] or -1 outside the grid (id_map: -1 for cells outside
+ * the system).  R is applied as three products summed in order; exact for
+ * the signed permutations the registration returns. */
+int hp_project_rows_f64(const double* xyz, int64_t n, const double* Rt, double scale,
+                        const double* origin, double dx_mm, const int32_t* id_map, int32_t ni,
+                        int32_t nj, int32_t nk, int64_t* proj, void* stream);
 
 #ifdef __cplusplus
 }

@@ -2,7 +2,10 @@
 imported only by tests, smoke() and bench.py's CPU baseline).
 
 Reference: hrdayapy/ecg/functions/
-  * _assemble_stiffness (sharp boundary)   build_torso_grid.py:276-397
+  * _assemble_stiffness (sharp and SBM)    build_torso_grid.py:276-397
+  * compute_smoothed_domain, extend_conductivity (SBM psi and sigma)
+                                            build_torso_grid.py:130-206
+  * _resample_labels (nearest-neighbour zoom) build_torso_grid.py:260-269
   * pin row/column, Jacobi inverse          compute_bspm.py:327-336
   * pinned cell, surface cells              build_torso_grid.py:616-623, 469-488
   * build_heart_laplacian                   compute_bspm.py:88-119
@@ -23,9 +26,11 @@ import scipy.sparse as sp
 from scipy import ndimage as ndi
 
 
-def assemble_stiffness(sigma_vol, h_m, active_mask):
+def assemble_stiffness(sigma_vol, h_m, active_mask, psi_vol=None):
     """7-point FDM stiffness with harmonic-mean face conductivity, Neumann at the
-    box edge and at air, in the reference's row order (active voxels in flat order)."""
+    box edge and at air, in the reference's row order (active voxels in flat order).
+    With psi_vol (SBM) each face conductance is scaled by the harmonic mean of
+    psi on its two cells; faces whose cells both have psi <= 0 are skipped."""
     ni, nj, nk = sigma_vol.shape
     act = active_mask.ravel()
     active_ids = np.flatnonzero(act)
@@ -33,6 +38,7 @@ def assemble_stiffness(sigma_vol, h_m, active_mask):
     id_map = np.full(act.size, -1, dtype=np.int64)
     id_map[active_ids] = np.arange(N)
     s = sigma_vol.ravel().astype(np.float64)
+    psi = None if psi_vol is None else np.asarray(psi_vol, dtype=np.float64).ravel()
     i, j, k = np.unravel_index(active_ids, (ni, nj, nk))
     diag = np.zeros(N)
     rows, cols, vals = [], [], []
@@ -45,8 +51,17 @@ def assemble_stiffness(sigma_vol, h_m, active_mask):
         nb = f + stride
         sk, sn = s[f], s[nb]
         pair = sk + sn >= 1e-30
+        if psi_vol is not None:
+            pk, pn = psi[f], psi[nb]
+            pair &= ~((pk <= 0.0) & (pn <= 0.0))
+            pk, pn = pk[pair], pn[pair]
         r, nb, sk, sn = r[pair], nb[pair], sk[pair], sn[pair]
-        cond = (2.0 * sk * sn) / (sk + sn) / h_m ** 2
+        if psi_vol is None:
+            cond = (2.0 * sk * sn) / (sk + sn) / h_m ** 2
+        else:
+            ps = pk + pn
+            pf = np.where(ps < 1e-30, 0.0, (2.0 * pk * pn) / np.where(ps < 1e-30, 1.0, ps))
+            cond = pf * ((2.0 * sk * sn) / (sk + sn)) / h_m ** 2
         np.add.at(diag, r, cond)
         inside = act[nb]
         rows.append(r[inside])
@@ -58,6 +73,43 @@ def assemble_stiffness(sigma_vol, h_m, active_mask):
     A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(N, N))
     return A, active_ids, id_map
+
+
+def resample_labels(volume, spacing, dx_coarse_mm):
+    """Nearest-neighbour resample of a label volume to dx_coarse_mm."""
+    zoom = np.asarray(spacing, dtype=np.float64) / dx_coarse_mm
+    return ndi.zoom(np.asarray(volume).astype(np.float32), zoom, order=0).astype(np.int32)
+
+
+def smoothed_domain(volume_fine, spacing_fine, dx_coarse_mm, coarse_shape,
+                    interface_width_mm=3.0, psi_cutoff=1e-3):
+    """SBM domain function on the coarse grid: psi_fine = (1 + tanh(3 d / w)) / 2
+    of the signed distance d (mm, + inside) of the fine segmentation, linearly
+    zoomed to the coarse spacing, cropped/padded with 0 to coarse_shape, and 0
+    where it falls below psi_cutoff."""
+    m = (np.asarray(volume_fine) != 0).astype(np.float64)
+    sp_f = np.asarray(spacing_fine, dtype=np.float64)
+    d = (ndi.distance_transform_edt(m, sampling=sp_f)
+         - ndi.distance_transform_edt(1.0 - m, sampling=sp_f))
+    psi_f = 0.5 * (1.0 + np.tanh(3.0 * d / interface_width_mm))
+    psi_c = ndi.zoom(psi_f, sp_f / dx_coarse_mm, order=1)
+    out = np.zeros(coarse_shape, dtype=np.float64)
+    sl = tuple(slice(0, min(a, b)) for a, b in zip(psi_c.shape, coarse_shape))
+    out[sl] = psi_c[sl]
+    out[out < psi_cutoff] = 0.0
+    return out
+
+
+def extend_conductivity(sigma_vol, tissue_mask, sbm_mask):
+    """sigma of the nearest tissue voxel (Euclidean, scipy's feature transform)
+    on the band cells (sbm_mask and not tissue)."""
+    band = sbm_mask & ~tissue_mask
+    out = np.array(sigma_vol, dtype=np.float64, copy=True)
+    if not band.any():
+        return out
+    _, idx = ndi.distance_transform_edt(~tissue_mask, return_indices=True)
+    out[band] = sigma_vol[tuple(ix[band] for ix in idx)]
+    return out
 
 
 def pin_row_of(tissue_mask, id_map):
@@ -112,8 +164,9 @@ def projection(coords_mm, R, t, scale, origin, dx, id_map, shape):
     return proj
 
 
-def source(Vm_mV, K, proj, sigma_i, h_fine_m, dx_coarse_mm, n_rows, pin_row):
-    """RHS of one frame (compute_bspm.py:381-419), FP64."""
+def source(Vm_mV, K, proj, sigma_i, h_fine_m, dx_coarse_mm, n_rows, pin_row, psi_row=None):
+    """RHS of one frame (compute_bspm.py:381-419), FP64; with psi_row (SBM) the
+    RHS is scaled by psi row-wise before the pinned row is cleared."""
     valid = proj >= 0
     lap = K @ (np.asarray(Vm_mV, dtype=np.float64) * 1e-3)
     Iv = lap[valid] * (sigma_i / h_fine_m ** 2)
@@ -121,6 +174,8 @@ def source(Vm_mV, K, proj, sigma_i, h_fine_m, dx_coarse_mm, n_rows, pin_row):
     b = np.zeros(n_rows)
     np.add.at(b, proj[valid], src)
     b /= (dx_coarse_mm / (h_fine_m * 1e3)) ** 3
+    if psi_row is not None:
+        b *= np.asarray(psi_row, dtype=np.float64)
     b[pin_row] = 0.0
     return b
 

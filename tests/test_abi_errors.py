@@ -159,3 +159,41 @@ def test_pcg_workspace_arguments(lib):
     ws.scalars = ptr
     _rejects(lib.hp_pcg_iterate_f32(C.byref(t), ptr, ptr, ptr, ptr, C.byref(ws), -1, None),
              "n_iter >= 0")
+
+
+def test_torso_build_and_registration_arguments(lib):
+    """The device torso build, registration and projection reject null operands,
+    empty or non-positive sizes and spacings on the host."""
+    buf = np.zeros(64, np.float64)
+    ptr = buf.ctypes.data
+    sp3 = (C.c_double * 3)(0.75, 0.75, 1.0)
+    bad3 = (C.c_double * 3)(0.75, 0.0, 1.0)
+    shp = (C.c_int32 * 3)(4, 4, 4)
+    _rejects(lib.hp_feature_transform(None, 0, 4, 4, 4, sp3, ptr, ptr, ptr, 64, None),
+             "hp_feature_transform")
+    _rejects(lib.hp_feature_transform(ptr, 0, 4, 4, 4, bad3, ptr, ptr, ptr, 64, None),
+             "spacing must be positive")
+    _rejects(lib.hp_feature_transform(ptr, 0, 4, 4, 9, sp3, ptr, ptr, ptr, 8, None),
+             "holds no line")
+    _rejects(lib.hp_sbm_psi_f64(ptr, ptr, ptr, 4, 4, 4, sp3, 0.0, ptr, None), "hp_sbm_psi_f64")
+    _rejects(lib.hp_zoom_linear_f64(ptr, 4, 4, 4, (C.c_int32 * 3)(4, 0, 4), shp, 1e-3, ptr, None,
+                                    None), "empty output shape")
+    _rejects(lib.hp_zoom_nearest_i32(None, 4, 4, 4, shp, ptr, None), "hp_zoom_nearest_i32")
+    _rejects(lib.hp_label_sigma_f64(ptr, 8, None, None, 2, ptr, ptr, None), "hp_label_sigma_f64")
+    assert lib.hp_label_sigma_f64(ptr, 0, None, None, 0, ptr, ptr, None) == 0
+    _rejects(lib.hp_extend_sigma_f64(ptr, ptr, ptr, ptr, 4, 4, 4, ptr, None), "hp_extend_sigma")
+    _rejects(lib.hp_column_stats(ptr, 4, 4, 4, None, None, None, None), "hp_column_stats")
+    _rejects(lib.hp_scan_i32(ptr, 0, ptr, None), "hp_scan_i32")
+    _rejects(lib.hp_column_emit(ptr, 4, 4, 4, ptr, None, None, None, None, ptr, None),
+             "hp_column_emit")
+    _rejects(lib.hp_surface_flags(None, 4, 4, 4, ptr, None), "hp_surface_flags")
+    _rejects(lib.hp_select_rank(ptr, ptr, 16, 4, -1, ptr, None), "hp_select_rank")
+    _rejects(lib.hp_torso_faces_f64(ptr, None, ptr, 4, 4, 4, 0.0, ptr, ptr, ptr, 0, ptr, ptr,
+                                    ptr, ptr, None, 8, ptr, None), "hp_torso_faces_f64")
+    _rejects(lib.hp_register_dice_f64(ptr, 8, ptr, 8, ptr, 48, ptr, ptr, ptr, ptr, 0.0, ptr, 16,
+                                      ptr, None), "hp_register_dice_f64")
+    _rejects(lib.hp_register_dice_f64(ptr, 0, ptr, 8, ptr, 48, ptr, ptr, ptr, ptr, 3.0, ptr, 16,
+                                      ptr, None), "hp_register_dice_f64")
+    _rejects(lib.hp_project_rows_f64(ptr, 8, ptr, 1.0, ptr, 0.0, ptr, 4, 4, 4, ptr, None),
+             "hp_project_rows_f64")
+    assert lib.hp_project_rows_f64(ptr, 0, ptr, 1.0, ptr, 1.5, ptr, 4, 4, 4, ptr, None) == 0

@@ -10,11 +10,9 @@ from hrdayapy_amd import tissue
 from hrdayapy_amd.anisotropic_simulation import compute_anisotropic_coupled
 from hrdayapy_amd.anisotropic_simulation.functions.purkinje_myocardium_anisotropic_solver import (
     build_coupled_system)
-from hrdayapy_amd.ecg import compute_torso_grid
 from hrdayapy_amd.simulation import compute_coupled
 from hrdayapy_amd.simulation.functions.activation_maps import (
     VmSnapshots, compute_activation_maps, load_activation_maps, load_vm_snapshots)
-from oracle import ecg as OE
 from oracle import monodomain as OMD
 from tests._problems import corner_region, slab, small_tree, stub_tree
 
@@ -158,29 +156,3 @@ def test_explicit_limit_is_below_the_true_stability_bound(kind):
     assert lim >= 0.5 * bound
 
 
-def test_torso_grid_dict_and_stiffness(tmp_path):
-    lab = np.zeros((14, 12, 10), np.int32)
-    lab[2:12, 2:10, 2:8] = 1
-    lab[5:9, 4:8, 3:6] = 2
-    lab[3, 3, 3] = 0
-    smap = {1: 0.2, 2: 0.05}
-    g = compute_torso_grid(sigma_map=smap, label_volume=lab, dx_coarse_mm=1.5,
-                           save_path=tmp_path / "torso.npz", verbose=False)
-    for k in ("shape", "dx_mm", "origin_mm", "sigma_vol", "active_mask", "tissue_mask",
-              "active_ids", "active_id_map", "surface_flat", "surface_rows", "surface_xyz",
-              "pin_row", "use_sbm"):
-        assert k in g, k
-    sig = np.zeros(lab.shape)
-    for k, v in smap.items():
-        sig[lab == k] = v
-    A, ids, idm = OE.assemble_stiffness(sig, 1.5e-3, lab != 0)
-    assert g["pin_row"] == OE.pin_row_of(lab != 0, idm)
-    np.testing.assert_array_equal(g["active_ids"], ids)
-    D = (g["A_scipy"] - A).tocoo()
-    assert np.abs(D.data).max(initial=0.0) <= 1e-12 * np.abs(A.data).max()
-    z = np.load(tmp_path / "torso.npz")
-    assert set(z.files) >= {"shape", "dx_mm", "origin_mm", "sigma_vol", "active_mask",
-                            "tissue_mask", "active_ids", "active_id_map", "surface_flat",
-                            "surface_rows", "surface_xyz", "pin_row", "use_sbm"}
-    with pytest.raises(NotImplementedError):
-        compute_torso_grid(sigma_map=smap, label_volume=lab, use_sbm=True)

@@ -5,7 +5,7 @@ import pytest
 
 from hrdayapy_amd.ecg.functions.forward import (Multigrid, _face_sum, coarsen_faces,
                                                 dense_operator)
-from hrdayapy_amd.ecg.functions.torso import TorsoGrid
+from tests._torso_host import host_grid
 from tests.test_oracle_ecg import _torso
 
 
@@ -44,7 +44,7 @@ def test_dense_operator_symmetric_zero_rowsum():
 
 def test_hierarchy_of_a_torso():
     lab, smap = _torso(n=(30, 26, 22))
-    g = TorsoGrid.from_labels(lab, smap, 1.5)
+    g = host_grid(lab, smap, 1.5)
     mg = Multigrid(g, "cpu")
     assert mg.shapes[0] == (15, 13, 11)
     assert np.prod(mg.shapes[-1]) <= 1024
@@ -77,7 +77,7 @@ def test_solver_setup_on_host_memory():
     import torch
     from hrdayapy_amd.ecg.functions.forward import PoissonSolver
     lab, smap = _torso(n=(30, 26, 22))
-    g = TorsoGrid.from_labels(lab, smap, 1.5)
+    g = host_grid(lab, smap, 1.5)
     ps = PoissonSolver(g, "cpu", precond="mg")
     assert ps.z32.numel() == ps.n == g.layout["n_cells"]
     assert abs(float(ps.pin_w.sum()) - 1.0) < 1e-12
@@ -93,7 +93,7 @@ def test_torso_chunk_table():
     of the +x/-x/+y/-y cell at the same k (or nothing) as the column arrays do."""
     from hrdayapy_amd import _native as nat
     lab, smap = _torso(n=(30, 26, 150))   # runs longer than 64 cells
-    g = TorsoGrid.from_labels(lab, smap, 1.5)
+    g = host_grid(lab, smap, 1.5)
     L = g.layout
     ni, nj, nk = g["shape"]
     k0, ln, off = (np.asarray(L[k]) for k in ("col_k0", "col_len", "col_off"))
@@ -130,9 +130,8 @@ def test_standard_electrodes_with_and_without_bspm():
     precordial electrodes on the body surface, the BSPM sites added after them."""
     from hrdayapy_amd import geometry as G
     from hrdayapy_amd.ecg.functions.forward import standard_electrodes
-    from hrdayapy_amd.ecg.functions.torso import TorsoGrid
     lab, origin, hc = G.torso((64, 64, 48), 6.0)
-    grid = TorsoGrid.from_labels(lab, G.TORSO_SIGMA, 6.0, origin)
+    grid = host_grid(lab, G.TORSO_SIGMA, 6.0, origin)
     twelve = ["RA", "LA", "LL"] + [f"V{i}" for i in range(1, 7)]
     pos = standard_electrodes(grid, hc, n_bspm=0)
     assert list(pos) == twelve

@@ -15,7 +15,6 @@ import pytest
 import scipy.sparse as sp
 
 from hrdayapy_amd import tissue
-from hrdayapy_amd.ecg.functions.forward import projection_rows
 from oracle import ecg as E
 from oracle import mesh as M
 
@@ -161,7 +160,6 @@ def test_heart_laplacian_and_projection(g):
     a = (coords, g["proj_R"], g["proj_t"], float(g["proj_scale"]), g["proj_origin"], 1.5, idm,
          lab.shape)
     np.testing.assert_array_equal(E.projection(*a), g["proj_rows"])
-    np.testing.assert_array_equal(projection_rows(*a), g["proj_rows"])
     assert np.array_equal(g["proj_rows"] >= 0, g["proj_valid"])
 
 
