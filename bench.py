@@ -154,8 +154,11 @@ def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, s
                              scheme=scheme, max_sweeps=opts.get("max_sweeps"),
                              cn_solver=opts.get("cn_solver", "bicgstab"))
     run = SG.SurrogateRunner(eng, net, mode=rule, graph_macros=5) if rule else None
-    advance = (lambda: run.run(STEPS_PER_MS // SG.N_SUB)) if run else (
-        lambda: eng.run(STEPS_PER_MS, graph_chunk=STEPS_PER_MS))
+    # Jacobi forms: each 1 ms chunk bounded by the sweeps the last one needed
+    # (replayed from its kept state with the full bound if a step stopped there)
+    jac = eng.cn is not None and eng.cn_solver == "jacobi"
+    advance = (lambda: run.run(STEPS_PER_MS // SG.N_SUB, adaptive=jac)) if run else (
+        lambda: eng.run(STEPS_PER_MS, graph_chunk=STEPS_PER_MS, adaptive=jac))
     rec = W.EcgRecorder(ecg.lf, eng) if ecg else None
     vm = torch.empty((beat_ms, len(sample_slots)), dtype=torch.float32, device=eng.dev)
     torch.cuda.synchronize()
@@ -184,6 +187,9 @@ def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, s
                setup_s=round(meta["setup_s"], 1))
     if eng.cn is not None:
         out["cn_solver"] = eng.cn_stats()
+        if jac:
+            out["cn_solver"].update(adaptive_bound=True, warm_start=bool(eng.cn_warm),
+                                    chunks_replayed=int(getattr(eng, "retries", 0)))
     if run is not None:
         out["fallback_fraction"] = round(run.fallback_fraction(), 4)
         out["reaction_us"] = round(W.graph_us(lambda i: run.launch_reaction(), 20), 2)

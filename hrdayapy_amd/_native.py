@@ -66,7 +66,7 @@ class HpLapCg(C.Structure):
 
 class HpCnWork(C.Structure):
     _fields_ = [("Vs", c_vp), ("b", c_vp), ("x", c_vp), ("Vs_pk", c_vp), ("b_pk", c_vp),
-                ("x_pk", c_vp), ("kr", c_vp * 8), ("kr_pk", c_vp * 8)]
+                ("x_pk", c_vp), ("kr", c_vp * 8), ("kr_pk", c_vp * 8), ("warm", c_vp * 4)]
 
 
 class HpMlpWeights(C.Structure):

@@ -72,7 +72,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
                          device="cuda", brick=None, verbose=False, sigma_scale=None,
                          inst=None, inst_params=None, auto_substep=False, alloc="torch",
                          scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10,
-                         cn_solver="bicgstab"):
+                         cn_solver="bicgstab", cn_warm=True):
     """Set up the tissue on the host and upload it into a MonodomainEngine.
 
     ``sigma_M`` selects the isotropic 6-point operator (simulation.compute_coupled);
@@ -115,7 +115,7 @@ def build_coupled_system(nodes, elements, activation_times, S, phi, fibre_dir=No
         cable=dict(n=Np, edges=comp_edges, w_pp=pc["w_pp"], Ai_P=pc["Ai_P"]),
         pmj=dict(ids=pmj_ids, patches=patches, c_pmj=c_pmj, Ai_P=pc["Ai_P"], Ai_M=pc["Ai_M"]),
         stim_sites=sites, inst=inst, inst_params=inst_params, alloc=alloc, scheme=scheme,
-        theta=theta, cg_tol=cg_tol, max_sweeps=max_sweeps, cn_solver=cn_solver)
+        theta=theta, cg_tol=cg_tol, max_sweeps=max_sweeps, cn_solver=cn_solver, cn_warm=cn_warm)
     meta = dict(Np=Np, N_myo=lay.n_nodes, n_sub=n_sub, dt_limit=dt_lim, scheme=scheme,
                 comp_nodes=comp_nodes, comp_edges=comp_edges,
                 branch_map=branch_map, vox_idx=vox, nodes_mm=myo_nodes_mm, celltype=ct,
@@ -210,8 +210,11 @@ def purkinje_myocardium_anisotropic_solver(
         nxt = min([((step // c) + (1 if step % c else 0)) * c for c in cadence] + [n_steps - 1])
         nxt = max(nxt, step)
         k = (nxt - step + 1) * n_sub
-        eng.run(k, graph_chunk=graph_chunk if k >= graph_chunk else 0)
-        if scheme == "cn":
+        # Jacobi sweeps: each chunk bounded by what the last one needed, replayed
+        # with the full bound if a step stopped at it (MonodomainEngine.run)
+        jac = scheme == "cn" and cn_solver == "jacobi"
+        eng.run(k, graph_chunk=graph_chunk if k >= graph_chunk else 0, adaptive=jac)
+        if scheme == "cn" and not jac:
             _fit_cn_bound(eng, int(cg_max_iter), cn_seen)
         t = nxt * dt
         want_anim, want_vm = nxt % save_every == 0, run_vm and nxt % vm_every == 0

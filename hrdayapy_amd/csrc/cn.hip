@@ -47,9 +47,25 @@ constexpr int CN_FIN_THREADS = 1024;
 
 // scalars[]: 0 ||b||^2, 1 ||r||^2 of the last sweep, 2 done, 3 sweeps this step,
 // 4 steps that stopped at max_sweeps, 5 largest ||r||/||b|| of those, 6 total
-// sweeps, 7 steps solved, 16 the largest sweep count of any step
+// sweeps, 7 steps solved, 16 the largest sweep count of any step, 17 steps
+// begun with the warm start (capped at 2), 18 the largest sweep count since
+// the caller last zeroed it (solver 0)
 enum { SC_BB = 0, SC_RR, SC_DONE, SC_SWEEPS, SC_NCONV, SC_WORST, SC_TOTAL, SC_STEPS,
-       SC_MAXIT = 16 };
+       SC_MAXIT = 16, SC_WARM = 17, SC_CMAX = 18 };
+
+// Warm start of the sweeps (solver 0): x_0 = V* + e, e the increment x - V*
+// of the last step extrapolated linearly from the last two steps (the last
+// one alone after the first step, 0 before it).  The first sweep then forms
+// b from V* as before and the residual of x_0 as
+//   r_0 = b - A x_0 = L V* + theta L e - (Cm/dt) e,
+// which is L V* exactly when e = 0.
+template <typename T>
+struct CnWarm {
+  T* d;       // [n_slots] x - V* of the last step
+  T* e;       // [n_slots] the extrapolated increment of this step
+  T* d_pk;    // cable nodes
+  T* e_pk;
+};
 
 __device__ __forceinline__ double cn_wave_sum(double v) {
 #pragma unroll
@@ -178,18 +194,82 @@ __device__ __forceinline__ int cn_find_target(const int32_t* __restrict__ tgt, i
 // ld = -L_ii.  Accumulates r^2 (and b^2) into rr (bb).
 template <typename T, bool kFirst>
 __device__ __forceinline__ T cn_row(T xc, T lx, T ld, T& bv, const hp_cn_params& cp, double& rr,
-                                    double& bb) {
+                                    double& bb, T ec = T(0), T le = T(0)) {
   const T cm_dt = T(cp.cm_dt), th = T(cp.theta);
   T r;
   if (kFirst) {
     bv = cm_dt * xc + (T(1) - th) * lx;
-    r = lx;
+    r = lx + th * le - cm_dt * ec;  // the residual of x_0 = xc + ec (lx when ec = le = 0)
     bb += (double)bv * (double)bv;
-  } else {
-    r = bv - cm_dt * xc + th * lx;
+    rr += (double)r * (double)r;
+    return (xc + ec) + r / (cm_dt + th * ld);
   }
+  r = bv - cm_dt * xc + th * lx;
   rr += (double)r * (double)r;
   return xc + r / (cm_dt + th * ld);
+}
+
+// stencil19_wsum_nan applied to V and, with the same weights, to a second
+// field E of the tile (absent neighbours: NaN in V)
+template <typename T, int BX, int BY, int BZ>
+__device__ __forceinline__ T stencil19_wsum2_nan(const Tile<T, BX, BY, BZ>& t,
+                                                 const T* __restrict__ E, int c, T& wsum,
+                                                 T& acc_e) {
+  typedef Tile<T, BX, BY, BZ> TT;
+  const T Vc = t.V[c], Ec = E[c];
+  T acc = T(0), ws = T(0), ae = T(0);
+  const int fo[3] = {TT::SX, TT::SY, 1};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T Da = t.D[a][c];
+#pragma unroll
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      const int nb = c + sgn * fo[a];
+      const T vn = t.V[nb];
+      if (vn == vn) {
+        const T Db = t.D[a][nb];
+        const T s = Da + Db;
+        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
+        acc += dh * (vn - Vc);
+        ae += dh * (E[nb] - Ec);
+        ws += dh;
+      }
+    }
+  }
+  const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const T Da = t.D[3 + q][c];
+    const int o1 = po[q][0], o2 = po[q][1];
+    const int co[2] = {o1 + o2, -(o1 + o2)};
+    const int ct[2] = {o1 - o2, -(o1 - o2)};
+    T sum = T(0), wq = T(0), se = T(0);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int nb = c + co[k];
+      const T vb = t.V[nb];
+      if (vb == vb) {
+        const T w = Da + t.D[3 + q][nb];
+        sum += w * (vb - Vc);
+        se += w * (E[nb] - Ec);
+        wq += w;
+      }
+      const int nm = c + ct[k];
+      const T vm = t.V[nm];
+      if (vm == vm) {
+        const T w = Da + t.D[3 + q][nm];
+        sum -= w * (vm - Vc);
+        se -= w * (E[nm] - Ec);
+        wq -= w;
+      }
+    }
+    acc += T(0.25) * sum;
+    ae += T(0.25) * se;
+    ws += T(0.25) * wq;
+  }
+  wsum = ws;
+  acc_e = ae;
+  return acc;
 }
 
 // One sweep: bricks in XCD-aware order (each XCD takes a contiguous run of
@@ -197,14 +277,16 @@ __device__ __forceinline__ T cn_row(T xc, T lx, T ld, T& bv, const hp_cn_params&
 // per workgroup iteration, and, in the first ncb workgroups, the cable rows
 // (grid-stride).  Each workgroup writes its partial sums of r^2 (and
 // b^2) to part[blockIdx] and part[gridDim + blockIdx].
-template <typename T, int BX, int BY, int BZ, bool kFirst>
+template <typename T, int BX, int BY, int BZ, bool kFirst, bool kWarm = false>
 __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
     hp_bricks g, const T* __restrict__ x_in, T* __restrict__ x_out, T* __restrict__ bvec,
     hp_cable c, const T* __restrict__ xpk_in, T* __restrict__ xpk_out, T* __restrict__ bpk,
-    hp_cn_pmj j, hp_cn_params cp, uint32_t ncb) {
+    hp_cn_pmj j, hp_cn_params cp, uint32_t ncb, CnWarm<T> wm) {
+  static_assert(kFirst || !kWarm, "the warm start enters the first sweep only");
   constexpr int BVOL = BX * BY * BZ;
   typedef Tile<T, BX, BY, BZ> TT;
   __shared__ TT t;
+  __shared__ T Et[kWarm ? TT::N + 1 : 1];  // the extrapolated increment e on the tile
   // block-uniform; a sweep outside a step (hp_cn_sweep) has no scalars
   if (!kFirst && cp.scalars && cp.scalars[SC_DONE] != 0.0) return;
   double rr = 0.0, bb = 0.0;
@@ -213,19 +295,24 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
     const T dsw = T(cp.dscale_pk * c.w);
     for (int64_t i = (int64_t)blockIdx.x * BVOL + tid; i < c.n; i += (int64_t)ncb * BVOL) {
       const T xc = xpk_in[i];
-      T acc = T(0);
+      const T ec = kWarm ? wm.e_pk[i] : T(0);
+      T acc = T(0), acce = T(0);
       const int e0 = c.indptr[i], e1 = c.indptr[i + 1];
-      for (int e = e0; e < e1; ++e) acc += xpk_in[c.indices[e]] - xc;
-      T lx = dsw * acc, ldg = dsw * T(e1 - e0);
+      for (int e = e0; e < e1; ++e) {
+        acc += xpk_in[c.indices[e]] - xc;
+        if (kWarm) acce += wm.e_pk[c.indices[e]] - ec;
+      }
+      T lx = dsw * acc, ldg = dsw * T(e1 - e0), le = dsw * acce;
       if (j.node_ptr) {
         for (int e = j.node_ptr[i]; e < j.node_ptr[i + 1]; ++e) {
           const T w = T(j.node_w[e]);
           lx += w * (ld32(x_in, (uint32_t)j.node_slot[e]) - xc);
+          if (kWarm) le += w * (ld32((const T*)wm.e, (uint32_t)j.node_slot[e]) - ec);
           ldg += w;
         }
       }
       T bv = kFirst ? T(0) : bpk[i];
-      xpk_out[i] = cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb);
+      xpk_out[i] = cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb, ec, le);
       if (kFirst) bpk[i] = bv;
     }
   } else {
@@ -243,13 +330,30 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
    compute_voxel<BX, BY, BZ>(tid, vx, vy, vz);
    const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
    const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
-   for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
+   // the neighbour-table row of the next brick is read while the current
+   // brick's halo is in flight (it depends on nothing loaded for this one)
+   uint32_t b = lo + (bid >> 3);
+   HaloNbr<BX, BY, BZ> hn;
+   halo_nbr<BX, BY, BZ>(hn, g, b < hi ? b : 0u, b < hi);
+   for (; b < hi; b += m) {
     __syncthreads();  // every lane is done with the previous tile
     {
-      HaloNbr<BX, BY, BZ> hn;
-      halo_nbr<BX, BY, BZ>(hn, g, b);
       HaloRegs<T, BX, BY, BZ> hr;
       halo_load<T, BX, BY, BZ>(hr, hn, g, x_in, g.label[b * BVOL + tid]);
+      const uint32_t bn = b + m;
+      halo_nbr<BX, BY, BZ>(hn, g, bn < hi ? bn : 0u, bn < hi);
+      if (kWarm) {  // e at the same tile cells (0 for cells without a brick)
+        constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
+        const buf_t rE = buf_rsrc(wm.e, NB * (uint32_t)BVOL * (uint32_t)sizeof(T));
+        T ev[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) ev[it] = buf_ld<T>(rE, hr.q[it]);
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+          const int kind = (hr.pw[it] >> 24) & 3;
+          Et[kind ? (int)(hr.pw[it] & 1023) : TT::N] = ev[it];
+        }
+      }
       halo_store_nan<T, BX, BY, BZ>(t, hr);
       __syncthreads();
     }
@@ -258,20 +362,23 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
     if (mylab != 0xFF) {
       const T dscale = T(cp.dscale);
       const T xc = t.V[cc];
-      T ws;
-      const T acc = stencil19_wsum_nan<T, BX, BY, BZ>(t, cc, ws);
-      T lx = dscale * acc, ldg = dscale * ws;
+      T ws, acce = T(0);
+      const T acc = kWarm ? stencil19_wsum2_nan<T, BX, BY, BZ>(t, Et, cc, ws, acce)
+                          : stencil19_wsum_nan<T, BX, BY, BZ>(t, cc, ws);
+      T lx = dscale * acc, ldg = dscale * ws, le = dscale * acce;
+      const T ec = kWarm ? Et[cc] : T(0);
       if ((mylab & 0x80) && j.n_targets > 0) {  // PMJ target voxel: its junction couplings
         const int q = cn_find_target(j.tgt_slot, (int)j.n_targets, (int)slot);
         const int e1 = j.tgt_slot[q] == (int)slot ? j.tgt_ptr[q + 1] : 0;  // listed slots only
         for (int e = j.tgt_ptr[q]; e < e1; ++e) {
           const T w = T(j.tgt_w[e]);
           lx += w * (xpk_in[j.tgt_node[e]] - xc);
+          if (kWarm) le += w * (wm.e_pk[j.tgt_node[e]] - ec);
           ldg += w;
         }
       }
       T bv = kFirst ? T(0) : ld32((const T*)bvec, slot);
-      st32(x_out, slot, cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb));
+      st32(x_out, slot, cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb, ec, le));
       if (kFirst) st32(bvec, slot, bv);
     }
    }
@@ -284,10 +391,13 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
 // file): x_p = (b_p + theta*(w_pp/Ai_P * sum of its cable neighbours' input
 // values + sum_m w_pm * x_m new)) / (Cm/dt + theta * (-L_pp)).  Returns at
 // once when the step's solve was already done before the sweep.
+// With the warm start (epk_in: e of the cable nodes, first sweep only) the
+// sweep's input on the cable is x_0 = V* + e.
 template <typename T, bool kFirst>
 __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
     hp_cn_params cp, int nb, hp_cable c, hp_cn_pmj j, const T* __restrict__ xpk_in,
-    T* __restrict__ xpk_out, const T* __restrict__ x_out, const T* __restrict__ bpk) {
+    T* __restrict__ xpk_out, const T* __restrict__ x_out, const T* __restrict__ bpk,
+    const T* __restrict__ epk_in, int warm) {
   double* sc = cp.scalars;
   if (!kFirst && sc[SC_DONE] != 0.0) return;
   if (j.n_jnodes > 0) {
@@ -296,7 +406,8 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
       const int pn = j.jnode[q];
       T nb_sum = T(0);
       const int e0 = c.indptr[pn], e1 = c.indptr[pn + 1];
-      for (int e = e0; e < e1; ++e) nb_sum += xpk_in[c.indices[e]];
+      for (int e = e0; e < e1; ++e)
+        nb_sum += epk_in ? xpk_in[c.indices[e]] + epk_in[c.indices[e]] : xpk_in[c.indices[e]];
       T m_sum = T(0), wsum = T(0);
       for (int e = j.node_ptr[pn]; e < j.node_ptr[pn + 1]; ++e) {
         const T w = T(j.node_w[e]);
@@ -340,8 +451,12 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
     rr += sh[0][w];
     bb += sh[1][w];
   }
-  if (kFirst) sc[SC_BB] = bb;
-  else bb = sc[SC_BB];
+  if (kFirst) {
+    sc[SC_BB] = bb;
+    if (warm && sc[SC_WARM] < 2.0) sc[SC_WARM] += 1.0;  // steps begun warm, capped at 2
+  } else {
+    bb = sc[SC_BB];
+  }
   const double sweeps = kFirst ? 1.0 : sc[SC_SWEEPS] + 1.0;
   sc[SC_RR] = rr;
   sc[SC_SWEEPS] = sweeps;
@@ -352,6 +467,7 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
     sc[SC_TOTAL] += sweeps;
     sc[SC_STEPS] += 1.0;
     if (sweeps > sc[SC_MAXIT]) sc[SC_MAXIT] = sweeps;
+    if (sweeps > sc[SC_CMAX]) sc[SC_CMAX] = sweeps;
     if (!conv) {
       sc[SC_NCONV] += 1.0;
       const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
@@ -364,18 +480,32 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
 // even: V_out itself; BiCGSTAB: always x), first activation between V^n and V^{n+1} (every
 // myocardial slot, junction voxels included: the PMJ coupling is part of the
 // solve); cable nodes in the first ncb blocks.
+// With the warm start (wm.d set): d = x - V* of this step (before the clamp)
+// and the next step's e = d + (d - d_last), or d after the first warm step.
+template <typename T>
+__device__ __forceinline__ void cn_warm_update(T* __restrict__ d, T* __restrict__ e, int64_t i,
+                                               T v, T vs, bool linear) {
+  const T dn = v - vs;
+  const T dl = d[i];
+  d[i] = dn;
+  e[i] = linear ? dn + (dn - dl) : dn;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) cn_final_kernel(
     hp_bricks g, const T* __restrict__ V_in, const T* __restrict__ x, T* __restrict__ V_out,
     const T* __restrict__ xpk, T* __restrict__ Vpk_out, int64_t n_cable, hp_mono_params p,
-    const double* __restrict__ sc, uint32_t ncb, bool by_parity) {
+    const double* __restrict__ sc, uint32_t ncb, bool by_parity, const T* __restrict__ Vs,
+    const T* __restrict__ Vspk, CnWarm<T> wm) {
   const bool odd = !by_parity || ((int64_t)sc[SC_SWEEPS] & 1) != 0;
+  const bool linear = wm.d && sc[SC_WARM] >= 2.0;
   const T vmin = T(p.vmin), vmax = T(p.vmax);
   if (blockIdx.x < ncb) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n_cable) {
       const T v = odd ? xpk[i] : Vpk_out[i];
       Vpk_out[i] = M<T>::mn(M<T>::mx(v, vmin), vmax);
+      if (wm.d) cn_warm_update<T>(wm.d_pk, wm.e_pk, i, v, Vspk[i], linear);
     }
     return;
   }
@@ -385,6 +515,7 @@ __global__ void __launch_bounds__(256) cn_final_kernel(
   const T v = odd ? ld32(x, slot) : ld32((const T*)V_out, slot);
   const T Vn = M<T>::mn(M<T>::mx(v, vmin), vmax);
   st32(V_out, slot, Vn);
+  if (wm.d) cn_warm_update<T>(wm.d, wm.e, slot, v, ld32(Vs, slot), linear);
   if (p.at) {
     const T Vc = ld32(V_in, slot), thr = T(p.at_thr);
     if (Vc < thr && Vn >= thr) {
@@ -877,13 +1008,17 @@ static void bi_update_launch(int key, dim3 grid, hipStream_t st, const hp_bricks
 template <typename T, int BX, int BY, int BZ>
 static void launch_sweep(bool first, dim3 grid, hipStream_t st, const hp_bricks& g, const T* xi,
                          T* xo, T* bvec, const hp_cable& c, const T* xpi, T* xpo, T* bpk,
-                         const hp_cn_pmj& j, const hp_cn_params& cp, uint32_t ncb) {
-  if (first)
+                         const hp_cn_pmj& j, const hp_cn_params& cp, uint32_t ncb,
+                         const CnWarm<T>& wm = CnWarm<T>{}) {
+  if (first && wm.d)
+    hipLaunchKernelGGL((cn_sweep_kernel<T, BX, BY, BZ, true, true>), grid, dim3(BX * BY * BZ), 0,
+                       st, g, xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb, wm);
+  else if (first)
     hipLaunchKernelGGL((cn_sweep_kernel<T, BX, BY, BZ, true>), grid, dim3(BX * BY * BZ), 0, st, g,
-                       xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb);
+                       xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb, wm);
   else
     hipLaunchKernelGGL((cn_sweep_kernel<T, BX, BY, BZ, false>), grid, dim3(BX * BY * BZ), 0, st,
-                       g, xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb);
+                       g, xi, xo, bvec, c, xpi, xpo, bpk, j, cp, ncb, wm);
 }
 
 template <typename T>
@@ -924,6 +1059,11 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
     set_error("%s: a cable needs its params, V, state (ld_pk >= n), CSR, labels and the work "
               "vectors Vs_pk with b_pk, x_pk (solver 0) or kr_pk[0..8) (solver 1)",
               what);
+    return -1;
+  }
+  if (w->warm[0] && (bicg || !w->warm[1] || (ncab > 0 && !(w->warm[2] && w->warm[3])))) {
+    set_error("%s: the warm start (solver 0 only) needs d and e over the slots, and d_pk and "
+              "e_pk with a cable", what);
     return -1;
   }
   const hp_cn_pmj j0{};
@@ -1037,10 +1177,13 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
     }
     if (nblk_f > 0)
       hipLaunchKernelGGL(cn_final_kernel<T>, dim3((unsigned)nblk_f), dim3(256), 0, st, *g, V_in,
-                         Vs, V_out, Vspk, Vpk_out, ncab, *p, cp->scalars, ncb_f, false);
+                         Vs, V_out, Vspk, Vpk_out, ncab, *p, cp->scalars, ncb_f, false, Vs, Vspk,
+                         CnWarm<T>{});
     return check_launch("hp_cn_step/final");
   }
-  // sweeps: x_0 = V*, sweep k writes x (k odd) or V_out (k even)
+  // sweeps: x_0 = V* (+ e with the warm start), sweep k writes x (k odd) or
+  // V_out (k even)
+  const CnWarm<T> wm{(T*)w->warm[0], (T*)w->warm[1], (T*)w->warm[2], (T*)w->warm[3]};
   for (int k = 1; k <= cp->max_sweeps; ++k) {
     const T* xi = k == 1 ? Vs : (k % 2 == 0 ? x : V_out);
     T* xo = k % 2 == 1 ? x : V_out;
@@ -1048,25 +1191,26 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
     T* xpo = k % 2 == 1 ? xpk : Vpk_out;
     if (G > 0) {
       switch (key) {
-        case 444: launch_sweep<T, 4, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
-        case 844: launch_sweep<T, 8, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
-        case 884: launch_sweep<T, 8, 8, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
-        default:  launch_sweep<T, 8, 8, 8>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb); break;
+        case 444: launch_sweep<T, 4, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb, wm); break;
+        case 844: launch_sweep<T, 8, 4, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb, wm); break;
+        case 884: launch_sweep<T, 8, 8, 4>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb, wm); break;
+        default:  launch_sweep<T, 8, 8, 8>(k == 1, dim3(G), st, *g, xi, xo, bvec, *c, xpi, xpo, bpk, *j, *cp, (uint32_t)ncb, wm); break;
       }
       if (int rc = check_launch("hp_cn_step/sweep")) return rc;
     }
     if (k == 1)
       hipLaunchKernelGGL((cn_finish_kernel<T, true>), dim3(1), dim3(CN_FIN_THREADS), 0, st, *cp,
-                         (int)G, *c, *j, xpi, xpo, xo, bpk);
+                         (int)G, *c, *j, xpi, xpo, xo, bpk, (const T*)wm.e_pk,
+                         wm.d ? 1 : 0);
     else
       hipLaunchKernelGGL((cn_finish_kernel<T, false>), dim3(1), dim3(CN_FIN_THREADS), 0, st, *cp,
-                         (int)G, *c, *j, xpi, xpo, xo, bpk);
+                         (int)G, *c, *j, xpi, xpo, xo, bpk, (const T*)nullptr, 0);
     if (int rc = check_launch("hp_cn_step/finish")) return rc;
   }
   // clamp + activation times
   if (nblk_f > 0)
     hipLaunchKernelGGL(cn_final_kernel<T>, dim3((unsigned)nblk_f), dim3(256), 0, st, *g, V_in, x,
-                       V_out, xpk, Vpk_out, ncab, *p, cp->scalars, ncb_f, true);
+                       V_out, xpk, Vpk_out, ncab, *p, cp->scalars, ncb_f, true, Vs, Vspk, wm);
   return check_launch("hp_cn_step/final");
 }
 

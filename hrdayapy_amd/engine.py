@@ -86,13 +86,17 @@ class MonodomainEngine:
     "jacobi" (Jacobi sweeps to the same relative residual, at most
     ``max_sweeps``: the exact CN solution to well within the tolerance, where
     the reference's BiCGSTAB stops with an error of its own at the fronts).
+    With ``cn_warm`` (Jacobi only) the sweeps start from V* plus the last
+    steps' increment x - V*, extrapolated linearly, instead of V*: the same
+    stopping rule is met after fewer sweeps (a step's increment changes little
+    from one step to the next).
     """
 
     def __init__(self, layout: BrickLayout, D6, celltype_myo, *, h, A_M, Cm, dt,
                  dtype=torch.float32, device="cuda", cable=None, pmj=None, stim_sites=None,
                  at_thr=-40.0, vmin=-150.0, vmax=80.0, V0=V_REST, inst=None, inst_params=None,
                  alloc="torch", scheme="explicit", theta=0.5, cg_tol=1e-5, max_sweeps=10,
-                 cn_solver="bicgstab"):
+                 cn_solver="bicgstab", cn_warm=True):
         nat.load()
         if scheme not in ("explicit", "cn"):
             raise ValueError(f"unknown scheme {scheme!r} (explicit or cn)")
@@ -234,7 +238,7 @@ class MonodomainEngine:
         self.cn = None
         if scheme == "cn":
             self._setup_cn(L, pmj, float(h), float(A_M), float(theta), float(cg_tol),
-                           int(max_sweeps), cable, cn_solver)
+                           int(max_sweeps), cable, cn_solver, bool(cn_warm))
         self.p = 0
         self.overlap_cable = True
         self.fuse_cable = True
@@ -242,7 +246,7 @@ class MonodomainEngine:
         self._eval_stim()
 
     # ------------------------------------------------------------------ setup
-    def _setup_cn(self, L, pmj, h, A_M, theta, tol, max_sweeps, cable, solver):
+    def _setup_cn(self, L, pmj, h, A_M, theta, tol, max_sweeps, cable, solver, warm=True):
         """Work vectors, PMJ rows and device scalars of the Crank-Nicolson step."""
         if not 0.0 <= theta <= 1.0:
             raise ValueError(f"theta must lie in [0, 1] (got {theta})")
@@ -294,8 +298,14 @@ class MonodomainEngine:
         kr = (C.c_void_p * 8)(*(nat.ptr(a) for a in self.cn_kr_t)) if bicg else (C.c_void_p * 8)()
         kr_pk = ((C.c_void_p * 8)(*(nat.ptr(a) for a in self.cn_kr_pk_t)) if self.cn_kr_pk_t
                  else (C.c_void_p * 8)())
+        # Jacobi sweeps start from V* plus the last steps' increment, extrapolated
+        # (d, e per slot and per cable node; zero before the first step)
+        self.cn_warm = warm and not bicg
+        self.cn_warm_t = [z(n), z(n), z(Np), z(Np)] if self.cn_warm else []
+        wv = ((C.c_void_p * 4)(*(nat.ptr(a) for a in self.cn_warm_t)) if self.cn_warm
+              else (C.c_void_p * 4)())
         self.cn_work = nat.HpCnWork(*(nat.ptr(a) if a is not None else None
-                                      for a in self.cn_work_t), kr, kr_pk)
+                                      for a in self.cn_work_t), kr, kr_pk, wv)
         self.cn = nat.HpCnParams(self.Cm / self.dt, theta, 1.0 / (A_M * h * h), 1.0 / Ai_P, tol,
                                  max_sweeps, 1 if bicg else 0, nat.ptr(self.cn_scalars),
                                  nat.ptr(self.cn_part), part_len)
@@ -320,7 +330,7 @@ class MonodomainEngine:
             raise ValueError("set_cn_max_iters needs a CN engine and n >= 1")
         if int(n) != self.cn.max_sweeps:
             self.cn.max_sweeps = int(n)
-            self._graphs = {}
+            self._b_cur = int(n)
 
     def _init_state(self, st):
         """Resting TTP06 state into st (18, n) (Ki, Nai as offsets from their bases)."""
@@ -424,24 +434,86 @@ class MonodomainEngine:
         self.p = p0
         return g
 
-    def run(self, n_steps: int, graph_chunk: int = 0):
+    def _graph(self, graph_chunk):
+        key = ("graph", graph_chunk, self.p, self.fuse_cable, self.overlap_cable,
+               self.cn.max_sweeps if self.cn is not None else 0)
+        if not hasattr(self, "_graphs"):
+            self._graphs = {}
+        if key not in self._graphs:
+            self._graphs[key] = self.capture(graph_chunk)
+        return self._graphs[key]
+
+    def run(self, n_steps: int, graph_chunk: int = 0, adaptive: bool = False):
         """Advance n_steps; with graph_chunk > 0, replay a captured graph of that many steps.
 
         A graph holds the buffer pointers of the V/V_pk ping-pong as they stood
         when it was captured, so graphs are kept per parity of the current
-        buffer (and per cable launch mode): a replay after an odd number of
-        eager steps uses the graph captured from that parity."""
-        if graph_chunk and n_steps >= graph_chunk:
-            key = ("graph", graph_chunk, self.p, self.fuse_cable, self.overlap_cable)
-            if not hasattr(self, "_graphs"):
-                self._graphs = {}
-            if key not in self._graphs:
-                self._graphs[key] = self.capture(graph_chunk)
-            g = self._graphs[key]
+        buffer (and per cable launch mode and sweep bound): a replay after an
+        odd number of eager steps uses the graph captured from that parity.
+
+        adaptive (CN with Jacobi sweeps, graph_chunk > 0): every step of a
+        graph launches as many sweeps as its bound, also after the solve has
+        converged (those return at once, but each launch costs a few
+        microseconds).  Each chunk is therefore replayed with the bound the
+        previous chunk needed; the chunk's state is kept first, and a chunk in
+        which a step stopped at that bound without meeting the tolerance is
+        replayed from it with the full bound (max_sweeps), so every step still
+        meets the stopping rule.  One host synchronisation per chunk."""
+        if adaptive and graph_chunk and self.cn is not None and self.cn_solver == "jacobi":
+            n_steps = self._run_adaptive(n_steps, graph_chunk)
+        elif graph_chunk and n_steps >= graph_chunk:
+            g = self._graph(graph_chunk)
             for _ in range(n_steps // graph_chunk):
                 g.replay()
             n_steps %= graph_chunk
         self.step(n_steps)
+
+    def _mutable(self):
+        """The device buffers a CN step writes that outlive the step."""
+        ts = [self.V[self.p], self.state, self.at, self.group_amp, self.step_ctr, self.cn_scalars]
+        if self.Np:
+            ts += [self.Vpk[self.p], self.state_pk]
+        return ts + list(self.cn_warm_t)
+
+    def _run_adaptive(self, n_steps, k):
+        self._run_chunks_adaptive(n_steps // k, lambda: self._graph(k).replay())
+        return n_steps % k
+
+    def _run_chunks_adaptive(self, n_chunks, replay, extra=()):
+        """Replay n_chunks graph chunks, each with the sweep bound the previous
+        one needed; `replay` replays one chunk captured at self.cn.max_sweeps,
+        `extra` lists further device buffers the chunk writes (kept with the
+        engine's own for a replay from the chunk's start)."""
+        b_max = self.cn.max_sweeps
+        b = min(getattr(self, "_b_cur", b_max), b_max)
+        sc = self.cn_scalars
+        n0 = float(sc[4].item()) if n_chunks > 0 else 0.0  # steps stopped at a bound so far
+        try:
+            while n_chunks > 0:
+                keep = b < b_max
+                if keep:
+                    src = self._mutable() + list(extra)
+                    if getattr(self, "_shadow", None) is None or len(self._shadow) != len(src) \
+                            or any(d.shape != t.shape for d, t in zip(self._shadow, src)):
+                        self._shadow = [torch.empty_like(t) for t in src]
+                    for d, t in zip(self._shadow, src):
+                        d.copy_(t)
+                sc[18].zero_()
+                self.cn.max_sweeps = b
+                replay()
+                n1, used = (float(v) for v in sc[[4, 18]].cpu().tolist())
+                if keep and n1 > n0:  # a step stopped at the bound: again with the full one
+                    for d, t in zip(self._shadow, self._mutable() + list(extra)):
+                        t.copy_(d)
+                    b = b_max
+                    self.retries = getattr(self, "retries", 0) + 1
+                    continue
+                b = max(1, min(b_max, int(used)))
+                n0 = n1
+                n_chunks -= 1
+        finally:
+            self.cn.max_sweeps = b_max
+            self._b_cur = b
 
     # --------------------------------------------------------------- readback
     @property

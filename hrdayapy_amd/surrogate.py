@@ -421,13 +421,37 @@ class SurrogateRunner:
         for _ in range(N_SUB):
             self.launch_substep()
 
-    def run(self, n_macro, graph=True):
+    def run(self, n_macro, graph=True, adaptive=False):
+        """Advance n_macro macro-steps (graphs of graph_macros at a time).
+
+        adaptive (CN engines with Jacobi sweeps): the substeps' sweep bound is
+        fitted per graph as MonodomainEngine.run(adaptive=True) fits it, the
+        graph replayed from its kept state with the full bound if a substep
+        stopped at the fitted one (the runner's rate, V_prev and fallback
+        lists are kept with the engine's state)."""
         if not graph:
             for _ in range(n_macro):
                 self._launch_macro()
             return
         eng = self.eng
-        key = (eng.p, eng.fuse_cable, eng.overlap_cable)  # buffers as captured
+        if adaptive and eng.cn is not None and eng.cn_solver == "jacobi":
+            k = self.graph_macros
+            extra = [self.rate, self.V_prev, self.fb_list, self.fb_count]
+            eng._run_chunks_adaptive(n_macro // k, lambda: self._graph().replay(), extra)
+            for _ in range(n_macro % k):
+                self._launch_macro()
+            return
+        g = self._graph()
+        k = self.graph_macros
+        for _ in range(n_macro // k):
+            g.replay()
+        for _ in range(n_macro % k):
+            self._launch_macro()
+
+    def _graph(self):
+        eng = self.eng
+        key = (eng.p, eng.fuse_cable, eng.overlap_cable,  # buffers as captured
+               eng.cn.max_sweeps if eng.cn is not None else 0)
         if self._graphs.get(key) is None:
             p0 = eng.p
             g = torch.cuda.CUDAGraph()
@@ -443,11 +467,7 @@ class SurrogateRunner:
             torch.cuda.current_stream(eng.dev).wait_stream(side)
             eng.p = p0
             self._graphs[key] = g
-        k = self.graph_macros
-        for _ in range(n_macro // k):
-            self._graphs[key].replay()
-        for _ in range(n_macro % k):
-            self._launch_macro()
+        return self._graphs[key]
 
     def _fallback_mask(self):
         """(n_slots,) bool: fallback voxels of the current macro-step."""

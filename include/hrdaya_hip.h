@@ -311,7 +311,11 @@ typedef struct hp_cn_pmj {
  * at max_sweeps (BiCGSTAB: or at a breakdown) without meeting the rule, 5 the
  * largest ||r||/||b|| of those, 6 total sweeps (iterations), 7 steps solved,
  * 16 the largest sweep (iteration) count of any step (4-7 and 16 accumulate
- * over calls; the caller zeroes them); 8-15 BiCGSTAB state.
+ * over calls; the caller zeroes them); 8-15 BiCGSTAB state; 17 the warm
+ * start's step count (hp_cn_work.warm); 18 (solver 0) the largest sweep count
+ * since the caller last zeroed it (for a caller that bounds the sweeps of a
+ * batch of steps by what recent steps needed and repeats the batch with a
+ * higher bound when a step stopped at it: slot 4 tells).
  * part: device, doubles, part_len >= 2*G (solver 0) or 8*G (solver 1) with
  * G = min(n_bricks, grid_cap) + ncb the workgroups of a pass (ncb <= 520 for
  * the cable rows). */
@@ -333,7 +337,13 @@ typedef struct hp_cn_params {
  * entries; Vs_pk, b_pk, x_pk of n_cable entries (may be NULL without a cable).
  * Solver 0 uses Vs, b, x and V_out as the second iterate buffer; solver 1
  * uses Vs (the iterate) and kr[8] = r, r^, M^-1, t, p (two buffers), v (two
- * buffers), n_slots entries each, kr_pk[8] likewise over the cable nodes. */
+ * buffers), n_slots entries each, kr_pk[8] likewise over the cable nodes.
+ * warm (solver 0, optional; no counterpart in the reference, whose BiCGSTAB
+ * starts from V*): d, e over the slots and d_pk, e_pk over the cable nodes,
+ * zeroed by the caller before the first step.  The sweeps then start from
+ * x_0 = V* + e, e the last step's increment x - V* (d) extrapolated linearly
+ * from the last two steps; the stopping rule is unchanged.  scalars[17]
+ * counts the steps begun warm (capped at 2; zero it with d and e). */
 typedef struct hp_cn_work {
   void* Vs;
   void* b;
@@ -343,6 +353,7 @@ typedef struct hp_cn_work {
   void* x_pk;
   void* kr[8];
   void* kr_pk[8];
+  void* warm[4];
 } hp_cn_work;
 
 /* p / p_pk: dt, inv_cm, ki/nai bases, instance scales, group_amp (stimulus of

@@ -19,12 +19,15 @@ Three integrators are provided:
     order, so the iterates agree to round-off, not bit for bit);
   * ``run(..., scheme="cn_jacobi")``: the same CN step with the linear system
     solved as the HIP kernels solve it -- Jacobi sweeps x <- x + (b - A x)/diag A
-    from x0 = V*, the first residual taken as L V* (= b - A V*), after each
-    sweep the rows of the junction nodes re-solved from the new patch values
-    and the old cable-neighbour values, stopping after the sweep whose
-    residual meets ||r|| <= cg_tol*||b|| or after ``max_sweeps``; the GPU's
-    FP64 CN path is compared with this one at round-off level, and this one
-    with ``cn`` through the solver tolerance;
+    from x0 = V* (with ``warm``, the default: x0 = V* + e, e the previous step's
+    increment x - V* extrapolated linearly from the last two steps, the first
+    residual then taken as L V* + theta L e - (Cm/dt) e), the first residual
+    otherwise L V* (= b - A V*), after each sweep the rows of the junction
+    nodes re-solved from the new patch values and the old cable-neighbour
+    values, stopping after the sweep whose residual meets
+    ||r|| <= cg_tol*||b|| or after ``max_sweeps``; the GPU's FP64 CN path is
+    compared with this one at round-off level, and this one with ``cn``
+    through the solver tolerance;
   * ``run(..., scheme="explicit")``: the scheme of the HIP kernels -- the
     same reaction step with forward-Euler diffusion on V^n (PMJ entries
     excluded from L), clamp, then the exact two-compartment PMJ exchange.
@@ -129,15 +132,22 @@ class Problem:
         return V
 
 
-def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0):
+def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0, e=None,
+                    theta=0.5, cm_dt=None):
     """The HIP path's CN solve (see the module docstring); returns (x, sweeps).
 
     ``junctions``: the cable nodes with PMJ couplings (global ids < Np); after
     each sweep row p is solved for x_p with its myocardial entries at the new
-    iterate and its cable entries at the sweep's input."""
+    iterate and its cable entries at the sweep's input.  ``e`` (optional) is the
+    warm start: x_0 = V* + e, whose residual b - A x_0 is formed as
+    L V* + theta L e - (Cm/dt) e (L V* alone when e is None; cm_dt = Cm/dt)."""
     bb = float(b @ b)
-    x = Vs.copy()
-    r = L @ Vs
+    if e is None:
+        x = Vs.copy()
+        r = L @ Vs
+    else:
+        x = Vs + e
+        r = (L @ Vs) + theta * (L @ e) - cm_dt * e
     Ad = A.diagonal()
     rows = [A.getrow(p) for p in junctions]
     for k in range(1, max_sweeps + 1):
@@ -158,7 +168,7 @@ def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0):
 
 def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtype=torch.float64,
         record_every=0, record_nodes=None, at_thr=-40.0, max_steps=None, state_init=None,
-        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10, dot=None):
+        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10, dot=None, warm=True):
     """Integrate the problem; returns dict(V, state, at, trace_t, trace_V, steps, wall_s).
 
     ``at`` is (N,) first activation time (NaN if none), on every node.
@@ -177,6 +187,7 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
     gks_s = None if g_ks_scale is None else torch.as_tensor(np.asarray(g_ks_scale), dtype=dtype)
     gto_s = None if g_to_scale is None else torch.as_tensor(np.asarray(g_to_scale), dtype=dtype)
     sweeps = []
+    d = e = None  # cn_jacobi warm start: last increment x - V*, its extrapolation
     if scheme in ("cn", "cn_jacobi"):
         A, M_inv = M.cn_matrix(prob.L, prob.diag_L, Cm / dt, prob.theta)
     at = np.full(N, np.nan)
@@ -197,8 +208,13 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
             Vs = V + (dt / Cm) * (-Iion * Cm + I_stim)
             b = (Cm / dt) * Vs + (1 - prob.theta) * (prob.L @ Vs)
             jn = sorted(set(int(q) for q in prob.pmj_ids)) if prob.c_pmj != 0.0 else []
-            Vn, k = cn_jacobi_solve(A, M_inv, prob.L, Vs, b, cg_tol, max_sweeps, jn, prob.Np)
+            Vn, k = cn_jacobi_solve(A, M_inv, prob.L, Vs, b, cg_tol, max_sweeps, jn, prob.Np,
+                                    e if warm else None, prob.theta, Cm / dt)
             sweeps.append(k)
+            if warm:  # next start: the increment, extrapolated from the last two steps
+                dn = Vn - Vs
+                e = dn if d is None else dn + (dn - d)
+                d = dn
             Vn = np.clip(Vn, -150.0, 80.0)
         elif scheme == "explicit":
             Vn = V - dt * Iion + (dt / Cm) * (I_stim + prob.L_nopmj @ V)

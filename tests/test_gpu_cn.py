@@ -139,6 +139,34 @@ def test_cn_graph_replay_equals_eager(cuda):
     assert a.cn_stats() == b.cn_stats()
 
 
+def test_adaptive_sweep_bound_equals_fixed_bound(cuda):
+    """Jacobi sweeps with the bound adapted per graph chunk (replayed from the
+    kept state with the full bound when a step stopped at the adapted one)
+    equal the run with the fixed bound bit for bit: the stopping rule, not the
+    bound, decides every step.  The adaptive run starts from bound 1, so its
+    first chunks are replayed; warm-started (the default) and cold sweeps."""
+    print("[case] adaptive sweep bound vs fixed bound", flush=True)
+    args, kw = _case("tree")
+    for warm in (True, False):
+        a, _ = _engine(args, kw, cuda, torch.float32, cn_solver="jacobi", max_sweeps=6,
+                       cn_warm=warm)
+        b, _ = _engine(args, kw, cuda, torch.float32, cn_solver="jacobi", max_sweeps=6,
+                       cn_warm=warm)
+        a._b_cur = 1
+        for rep in range(4):
+            a.run(100, graph_chunk=50, adaptive=True)
+            b.run(100, graph_chunk=50)
+            torch.cuda.synchronize()
+            assert torch.equal(a.V_global(), b.V_global()), (warm, rep)
+            assert torch.equal(a.state, b.state)
+            assert torch.equal(a.at.nan_to_num(-1.0), b.at.nan_to_num(-1.0))
+        sa, sb = a.cn_stats(), b.cn_stats()
+        assert sa == sb and sa["not_converged"] == 0, (sa, sb)
+        assert getattr(a, "retries", 0) >= 1 and a.cn.max_sweeps == 6
+        print(f"  warm={warm}: mean sweeps {sa['mean_sweeps']:.3f}, max {sa['max_used']}, "
+              f"{a.retries} chunk(s) replayed, bound now {a._b_cur}", flush=True)
+
+
 @pytest.mark.parametrize("scheme", ["cn", "explicit"])
 def test_graph_replay_after_odd_eager_steps(cuda, scheme):
     """Graphs hold the V buffers of the ping-pong as captured: a replay after an

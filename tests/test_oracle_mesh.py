@@ -137,6 +137,38 @@ def test_cn_jacobi_solve_against_direct_and_bicgstab():
     assert k5 < kj
 
 
+def test_cn_jacobi_warm_start():
+    """The warm start x0 = V* + e: with e = 0 it is the cold solve bit for bit;
+    its first residual L V* + theta L e - (Cm/dt) e equals b - A x0 (a sign or
+    factor slip in that form fails here); with a good guess the rule is met in
+    fewer sweeps and the answer still meets it."""
+    from oracle import monodomain as OMD
+    from tests.test_gpu_monodomain import _case
+    args, kw = _case("tree")
+    nodes, el, act, S, phi, f = args
+    prob = OMD.Problem(nodes, el, act, S, phi, f, **kw)
+    cm_dt = prob.Cm / 0.02
+    A, Minv = M.cn_matrix(prob.L, prob.diag_L, cm_dt, 0.5)
+    rng = np.random.default_rng(6)
+    Vs = -80.0 + 10.0 * rng.standard_normal(prob.N)
+    b = cm_dt * Vs + 0.5 * (prob.L @ Vs)
+    jn = sorted(set(int(q) for q in prob.pmj_ids))
+    x0, k0 = OMD.cn_jacobi_solve(A, Minv, prob.L, Vs, b, 1e-5, 50, jn, prob.Np)
+    xz, kz = OMD.cn_jacobi_solve(A, Minv, prob.L, Vs, b, 1e-5, 50, jn, prob.Np,
+                                 np.zeros(prob.N), 0.5, cm_dt)
+    assert kz == k0 and np.array_equal(xz, x0)
+    e = rng.standard_normal(prob.N)
+    r_form = prob.L @ Vs + 0.5 * (prob.L @ e) - cm_dt * e
+    r_true = b - A @ (Vs + e)
+    assert np.abs(r_form - r_true).max() <= 1e-12 * np.abs(r_true).max()
+    x_ex, _ = OMD.cn_jacobi_solve(A, Minv, prob.L, Vs, b, 1e-13, 200, jn, prob.Np)
+    guess = (x_ex - Vs) + 1e-4 * rng.standard_normal(prob.N)
+    xw, kw_ = OMD.cn_jacobi_solve(A, Minv, prob.L, Vs, b, 1e-5, 50, jn, prob.Np, guess, 0.5,
+                                  cm_dt)
+    assert kw_ < k0
+    assert np.linalg.norm(b - A @ xw) <= 1e-5 * np.linalg.norm(b)
+
+
 def test_cn_jacobi_trajectory_matches_reference_scheme():
     """Over 8 ms of a tree-driven activation the Jacobi-solved CN trajectory
     stays within a few hundredths of a mV of the BiCGSTAB-solved one."""
@@ -147,9 +179,12 @@ def test_cn_jacobi_trajectory_matches_reference_scheme():
     prob = OMD.Problem(nodes, el, act, S, phi, f, **kw)
     a = OMD.run(prob, 8.0, 0.02, scheme="cn", record_every=25)
     b = OMD.run(prob, 8.0, 0.02, scheme="cn_jacobi", record_every=25, max_sweeps=12)
-    d = a["trace_V"] - b["trace_V"]
-    assert np.sqrt(np.mean(d ** 2)) < 0.05 and np.abs(d).max() < 1.0
-    assert b["sweeps"].max() < 12
+    c = OMD.run(prob, 8.0, 0.02, scheme="cn_jacobi", record_every=25, max_sweeps=12, warm=False)
+    for tr in (b, c):  # warm-started (the default) and cold-started sweeps
+        d = a["trace_V"] - tr["trace_V"]
+        assert np.sqrt(np.mean(d ** 2)) < 0.05 and np.abs(d).max() < 1.0
+        assert tr["sweeps"].max() < 12
+    assert b["sweeps"].sum() < c["sweeps"].sum()  # the warm start saves sweeps
     assert a["trace_V"][:, :prob.Np].max() > 0.0  # the tree fired
 
 
