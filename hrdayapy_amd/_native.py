@@ -28,7 +28,8 @@ class HpMonoParams(C.Structure):
     _fields_ = [("dt", c_f64), ("inv_cm", c_f64), ("dscale", c_f64), ("ki_base", c_f64),
                 ("nai_base", c_f64), ("vmin", c_f64), ("vmax", c_f64), ("at_thr", c_f64),
                 ("do_reaction", c_i32), ("n_inst", c_i32), ("inst_params", c_vp),
-                ("step_ctr", c_vp), ("group_amp", c_vp), ("at", c_vp), ("rate_ext", c_vp)]
+                ("step_ctr", c_vp), ("group_amp", c_vp), ("at", c_vp), ("rate_ext", c_vp),
+                ("cell_slot", c_vp), ("n_cells", c_i64)]
 
 
 class HpCable(C.Structure):

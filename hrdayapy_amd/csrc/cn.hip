@@ -144,10 +144,17 @@ __global__ void __launch_bounds__(256) cn_react_kernel(
     Vspk[i] = Vc - dt * I + dt * T(p_pk.inv_cm) * stim;
     return;
   }
-  const uint32_t slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
-  const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
-  if (slot >= n_slots) return;
-  const uint32_t sidx = state_index_wave(g.groups, slot >> 6);
+  uint32_t slot, sidx;
+  if (p.cell_slot) {  // one lane per cell (compact state order)
+    sidx = (blockIdx.x - ncb) * 256u + threadIdx.x;
+    if (sidx >= (uint32_t)p.n_cells) return;
+    slot = (uint32_t)p.cell_slot[sidx];
+  } else {  // one lane per slot: waves of 64 slots, padding lanes idle
+    slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
+    const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
+    if (slot >= n_slots) return;
+    sidx = state_index_wave(g.groups, slot >> 6);
+  }
   const uint8_t lab = g.label[slot];
   if (lab == 0xFF) return;
   const T Vc = ld32(V, slot);
@@ -330,18 +337,13 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
    compute_voxel<BX, BY, BZ>(tid, vx, vy, vz);
    const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
    const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
-   // the neighbour-table row of the next brick is read while the current
-   // brick's halo is in flight (it depends on nothing loaded for this one)
-   uint32_t b = lo + (bid >> 3);
-   HaloNbr<BX, BY, BZ> hn;
-   halo_nbr<BX, BY, BZ>(hn, g, b < hi ? b : 0u, b < hi);
-   for (; b < hi; b += m) {
+   for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
     __syncthreads();  // every lane is done with the previous tile
     {
+      HaloNbr<BX, BY, BZ> hn;
+      halo_nbr<BX, BY, BZ>(hn, g, b);
       HaloRegs<T, BX, BY, BZ> hr;
       halo_load<T, BX, BY, BZ>(hr, hn, g, x_in, g.label[b * BVOL + tid]);
-      const uint32_t bn = b + m;
-      halo_nbr<BX, BY, BZ>(hn, g, bn < hi ? bn : 0u, bn < hi);
       if (kWarm) {  // e at the same tile cells (0 for cells without a brick)
         constexpr int NIT = HaloPlan<BX, BY, BZ>::NIT;
         const buf_t rE = buf_rsrc(wm.e, NB * (uint32_t)BVOL * (uint32_t)sizeof(T));
@@ -1103,7 +1105,11 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
 
   // reaction
   const uint32_t ncb_r = (uint32_t)ceil_div(ncab, 256);
-  const int64_t nblk_r = ncb_r + ceil_div(n_slots, 256);
+  if (p->cell_slot && (p->n_cells < 0 || p->n_cells > n_slots || p->n_cells > ld)) {
+    set_error("%s: cell_slot needs 0 <= n_cells <= min(slots, ld)", what);
+    return -1;
+  }
+  const int64_t nblk_r = ncb_r + ceil_div(p->cell_slot ? p->n_cells : n_slots, 256);
   if (nblk_r > 0) {
     hipLaunchKernelGGL(cn_react_kernel<T>, dim3((unsigned)nblk_r), dim3(256), 0, st, *g, V_in, Vs,
                        state, ld, *p, *c, Vpk_in, Vspk, state_pk, ld_pk, *p_pk, ncb_r);

@@ -180,6 +180,12 @@ class MonodomainEngine:
         self.bricks = nat.HpBricks(L.n_bricks, *L.brick, 0, nat.ptr(self.nbr), nat.ptr(self.label),
                                    nat.ptr(self.D6), nat.ptr(self.inst), nat.ptr(self.groups))
         self.p_myo = self._params(1.0 / (A_M * h * h), at_thr, self.at)
+        # the myocardial slots in compact-state order: the CN reaction pass runs
+        # one lane per cell over them (no idle padding lanes)
+        self.cell_slot = torch.from_numpy(
+            np.sort(np.asarray(L.slot_of_node)).astype(np.int32)).to(dev)
+        self.p_myo.cell_slot = nat.ptr(self.cell_slot)
+        self.p_myo.n_cells = int(self.cell_slot.numel())
 
         # Purkinje cable
         self.cable = None

@@ -147,6 +147,10 @@ typedef struct hp_mono_params {
   const void* group_amp;    /* T[n_groups] stimulus amplitude of each group this step [uA/mm^2] */
   void* at;                 /* T[n] first activation time (NaN until crossed) or NULL */
   const void* rate_ext;     /* T[n] extra dV/dt [mV/ms] (surrogate reaction) or NULL */
+  const int32_t* cell_slot; /* [n_cells] slot of each compact state entry, ascending, or NULL:
+                               the CN step's reaction pass then runs one lane per cell
+                               instead of one per slot (no idle padding lanes) */
+  int64_t n_cells;
 } hp_mono_params;
 
 /* One fused explicit step of the myocardium: TTP06 + 19-point diffusion +
