@@ -434,7 +434,7 @@ def main():
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--brick", type=str, default="auto", help="e.g. 8x4x4, or auto")
     ap.add_argument("--forms", type=str, default="plain,bicgstab,explicit,bigstep,surrogate")
-    ap.add_argument("--train-samples", type=int, default=20_000_000)
+    ap.add_argument("--train-samples", type=int, default=50_000_000)
     ap.add_argument("--train-epochs", type=int, default=14)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")

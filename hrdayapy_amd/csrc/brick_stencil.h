@@ -149,7 +149,7 @@ __device__ __forceinline__ void st32(T* base, uint32_t idx, T v) {
 // iteration: the plan word, the slot of the cell (OOB for cells without a
 // brick: their loads return 0 without memory traffic), label, V and up to
 // three D components.  The lane's own cell (iteration 0) loads all six D
-// components; padding voxels of the brick itself load nothing (mylab 0xFF).
+// components, padding voxels of the brick too (their label masks them).
 // Padding slots of neighbour bricks are read (they hold V0 and D = 0) and
 // masked by their label.
 constexpr uint32_t OOB = 0x1FFFFFFFu;  // element index past every bound
@@ -191,7 +191,8 @@ struct HaloRegs {
   T v[NIT], d[NIT][3], dself[3];
 };
 
-// mylab: the lane's own label in brick n (0xFF: padding voxel, loads nothing).
+// mylab: the lane's own label in brick n (unused: the own slot is read as a
+// neighbour slot is, its label staged with it).
 // value(q) gives the tile value of slot q (q = OOB for cells without a
 // brick: it must then read out of range only); halo_load stages V itself.
 template <typename T, int BX, int BY, int BZ, typename F>
@@ -205,7 +206,9 @@ __device__ __forceinline__ void halo_load_fn(HaloRegs<T, BX, BY, BZ>& h,
 #pragma unroll
   for (int it = 0; it < NIT; ++it) {
     h.pw[it] = n.pw[it];
-    const bool ok = n.valid && (it == 0 ? mylab != 0xFF : n.nb[it] >= 0);
+    // the brick's own slots are read unconditionally (padding slots hold V0, D = 0
+    // and label 0xFF, which the staging masks), so these loads wait for nothing
+    const bool ok = n.valid && (it == 0 || n.nb[it] >= 0);
     h.q[it] = ok ? (uint32_t)n.nb[it] * BVOL + ((n.pw[it] >> 15) & 511) : OOB;
   }
 #pragma unroll
@@ -352,7 +355,10 @@ __device__ __forceinline__ void halo_store_nan(Tile<T, BX, BY, BZ>& t,
 }
 
 // stencil19_wsum on a tile staged by halo_store_nan: an absent neighbour is a
-// NaN value (the comparison v == v is false for it), no label reads.
+// NaN value (the comparison v == v is false for it), no label reads.  Every
+// LDS value is read unconditionally and an absent neighbour's term is
+// selected away (no branch per neighbour, so the reads are issued together);
+// the present neighbours' terms are added in the same order as before.
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ T stencil19_wsum_nan(const Tile<T, BX, BY, BZ>& t, int c, T& wsum) {
   typedef Tile<T, BX, BY, BZ> TT;
@@ -366,13 +372,12 @@ __device__ __forceinline__ T stencil19_wsum_nan(const Tile<T, BX, BY, BZ>& t, in
     for (int sgn = -1; sgn <= 1; sgn += 2) {
       const int nb = c + sgn * fo[a];
       const T vn = t.V[nb];
-      if (vn == vn) {
-        const T Db = t.D[a][nb];
-        const T s = Da + Db;
-        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
-        acc += dh * (vn - Vc);
-        ws += dh;
-      }
+      const T Db = t.D[a][nb];
+      const T s = Da + Db;
+      const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
+      const bool on = vn == vn;
+      acc = on ? acc + dh * (vn - Vc) : acc;
+      ws = on ? ws + dh : ws;
     }
   }
   const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
@@ -387,18 +392,16 @@ __device__ __forceinline__ T stencil19_wsum_nan(const Tile<T, BX, BY, BZ>& t, in
     for (int k = 0; k < 2; ++k) {
       const int nb = c + co[k];
       const T vb = t.V[nb];
-      if (vb == vb) {
-        const T w = Da + t.D[3 + q][nb];
-        sum += w * (vb - Vc);
-        wq += w;
-      }
+      const T wb = Da + t.D[3 + q][nb];
+      const bool onb = vb == vb;
+      sum = onb ? sum + wb * (vb - Vc) : sum;
+      wq = onb ? wq + wb : wq;
       const int nm = c + ct[k];
       const T vm = t.V[nm];
-      if (vm == vm) {
-        const T w = Da + t.D[3 + q][nm];
-        sum -= w * (vm - Vc);
-        wq -= w;
-      }
+      const T wm = Da + t.D[3 + q][nm];
+      const bool onm = vm == vm;
+      sum = onm ? sum - wm * (vm - Vc) : sum;
+      wq = onm ? wq - wm : wq;
     }
     acc += T(0.25) * sum;
     ws += T(0.25) * wq;

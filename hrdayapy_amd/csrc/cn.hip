@@ -217,7 +217,7 @@ __device__ __forceinline__ T cn_row(T xc, T lx, T ld, T& bv, const hp_cn_params&
 }
 
 // stencil19_wsum_nan applied to V and, with the same weights, to a second
-// field E of the tile (absent neighbours: NaN in V)
+// field E of the tile (absent neighbours: NaN in V; branch-free as there)
 template <typename T, int BX, int BY, int BZ>
 __device__ __forceinline__ T stencil19_wsum2_nan(const Tile<T, BX, BY, BZ>& t,
                                                  const T* __restrict__ E, int c, T& wsum,
@@ -233,14 +233,14 @@ __device__ __forceinline__ T stencil19_wsum2_nan(const Tile<T, BX, BY, BZ>& t,
     for (int sgn = -1; sgn <= 1; sgn += 2) {
       const int nb = c + sgn * fo[a];
       const T vn = t.V[nb];
-      if (vn == vn) {
-        const T Db = t.D[a][nb];
-        const T s = Da + Db;
-        const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
-        acc += dh * (vn - Vc);
-        ae += dh * (E[nb] - Ec);
-        ws += dh;
-      }
+      const T en = E[nb];
+      const T Db = t.D[a][nb];
+      const T s = Da + Db;
+      const T dh = T(2) * Da * Db * M<T>::rcp(s > T(1e-30) ? s : T(1e-30));
+      const bool on = vn == vn;
+      acc = on ? acc + dh * (vn - Vc) : acc;
+      ae = on ? ae + dh * (en - Ec) : ae;
+      ws = on ? ws + dh : ws;
     }
   }
   const int po[3][2] = {{TT::SX, TT::SY}, {TT::SX, 1}, {TT::SY, 1}};
@@ -254,21 +254,19 @@ __device__ __forceinline__ T stencil19_wsum2_nan(const Tile<T, BX, BY, BZ>& t,
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int nb = c + co[k];
-      const T vb = t.V[nb];
-      if (vb == vb) {
-        const T w = Da + t.D[3 + q][nb];
-        sum += w * (vb - Vc);
-        se += w * (E[nb] - Ec);
-        wq += w;
-      }
+      const T vb = t.V[nb], eb = E[nb];
+      const T wb = Da + t.D[3 + q][nb];
+      const bool onb = vb == vb;
+      sum = onb ? sum + wb * (vb - Vc) : sum;
+      se = onb ? se + wb * (eb - Ec) : se;
+      wq = onb ? wq + wb : wq;
       const int nm = c + ct[k];
-      const T vm = t.V[nm];
-      if (vm == vm) {
-        const T w = Da + t.D[3 + q][nm];
-        sum -= w * (vm - Vc);
-        se -= w * (E[nm] - Ec);
-        wq -= w;
-      }
+      const T vm = t.V[nm], em = E[nm];
+      const T wm = Da + t.D[3 + q][nm];
+      const bool onm = vm == vm;
+      sum = onm ? sum - wm * (vm - Vc) : sum;
+      se = onm ? se - wm * (em - Ec) : se;
+      wq = onm ? wq - wm : wq;
     }
     acc += T(0.25) * sum;
     ae += T(0.25) * se;
@@ -338,6 +336,10 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
    const int cc = ((vx + 1) * TT::TY + (vy + 1)) * TT::TZ + (vz + 1);
    const uint32_t vloc = (uint32_t)((vx * BY + vy) * BZ + vz);
    for (uint32_t b = lo + (bid >> 3); b < hi; b += m) {
+    const uint32_t slot = b * BVOL + vloc;
+    // the right-hand side is read before the tile is staged, so its latency
+    // overlaps the halo loads (its lanes without a cell read an unused value)
+    T bv = kFirst ? T(0) : ld32((const T*)bvec, slot);
     __syncthreads();  // every lane is done with the previous tile
     {
       HaloNbr<BX, BY, BZ> hn;
@@ -359,7 +361,6 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
       halo_store_nan<T, BX, BY, BZ>(t, hr);
       __syncthreads();
     }
-    const uint32_t slot = b * BVOL + vloc;
     const uint8_t mylab = t.L[cc];
     if (mylab != 0xFF) {
       const T dscale = T(cp.dscale);
@@ -379,7 +380,6 @@ __global__ void __launch_bounds__(BX * BY * BZ) cn_sweep_kernel(
           ldg += w;
         }
       }
-      T bv = kFirst ? T(0) : ld32((const T*)bvec, slot);
       st32(x_out, slot, cn_row<T, kFirst>(xc, lx, ldg, bv, cp, rr, bb, ec, le));
       if (kFirst) st32(bvec, slot, bv);
     }
@@ -402,6 +402,15 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
     const T* __restrict__ epk_in, int warm) {
   double* sc = cp.scalars;
   if (!kFirst && sc[SC_DONE] != 0.0) return;
+  // the first batch of block partials is loaded before the junction rows'
+  // dependent loads (independent of them), so both latencies overlap
+  double xr0[8], xb0[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int b = u * CN_FIN_THREADS + (int)threadIdx.x;
+    xr0[u] = b < nb ? cp.part[b] : 0.0;
+    xb0[u] = (kFirst && b < nb) ? cp.part[nb + b] : 0.0;
+  }
   if (j.n_jnodes > 0) {
     const T cm_dt = T(cp.cm_dt), th = T(cp.theta), dsw = T(cp.dscale_pk * c.w);
     for (int64_t q = threadIdx.x; q < j.n_jnodes; q += CN_FIN_THREADS) {
@@ -422,8 +431,13 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
   }
   __shared__ double sh[2][CN_FIN_THREADS / 64];
   double rr = 0.0, bb = 0.0;
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    rr += xr0[u];
+    bb += xb0[u];
+  }
   // eight independent loads in flight per array before the adds
-  for (int b0 = 0; b0 < nb; b0 += 8 * CN_FIN_THREADS) {
+  for (int b0 = 8 * CN_FIN_THREADS; b0 < nb; b0 += 8 * CN_FIN_THREADS) {
     double xr[8], xb[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -511,9 +525,14 @@ __global__ void __launch_bounds__(256) cn_final_kernel(
     }
     return;
   }
-  const uint32_t slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
-  const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
-  if (slot >= n_slots || g.label[slot] == 0xFF) return;
+  uint32_t slot = (blockIdx.x - ncb) * 256u + threadIdx.x;
+  if (p.cell_slot) {  // one lane per cell
+    if (slot >= (uint32_t)p.n_cells) return;
+    slot = (uint32_t)p.cell_slot[slot];
+  } else {
+    const uint32_t n_slots = (uint32_t)g.n_bricks * (uint32_t)(g.bx * g.by * g.bz);
+    if (slot >= n_slots || g.label[slot] == 0xFF) return;
+  }
   const T v = odd ? ld32(x, slot) : ld32((const T*)V_out, slot);
   const T Vn = M<T>::mn(M<T>::mx(v, vmin), vmax);
   st32(V_out, slot, Vn);
@@ -1127,7 +1146,7 @@ static int cn_step(const hp_bricks* g, const T* V_in, T* V_out, T* state, int64_
     return -1;
   }
   const uint32_t ncb_f = (uint32_t)ceil_div(ncab, 256);
-  const int64_t nblk_f = ncb_f + ceil_div(n_slots, 256);
+  const int64_t nblk_f = ncb_f + ceil_div(p->cell_slot ? p->n_cells : n_slots, 256);
   if (cp->solver == 1) {
     // BiCGSTAB from x0 = V* (the iterate overwrites Vs)
     BiSide<T> m{}, cv{};

@@ -148,8 +148,9 @@ typedef struct hp_mono_params {
   void* at;                 /* T[n] first activation time (NaN until crossed) or NULL */
   const void* rate_ext;     /* T[n] extra dV/dt [mV/ms] (surrogate reaction) or NULL */
   const int32_t* cell_slot; /* [n_cells] slot of each compact state entry, ascending, or NULL:
-                               the CN step's reaction pass then runs one lane per cell
-                               instead of one per slot (no idle padding lanes) */
+                               the CN step's reaction and final (clamp, activation time)
+                               passes then run one lane per cell instead of one per slot
+                               (no idle padding lanes); the explicit kernels ignore it */
   int64_t n_cells;
 } hp_mono_params;
 
