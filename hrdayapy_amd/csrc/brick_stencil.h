@@ -177,10 +177,11 @@ __device__ __forceinline__ void halo_nbr(HaloNbr<BX, BY, BZ>& n, const hp_bricks
   for (int it = 0; it < HaloNbr<BX, BY, BZ>::NIT; ++it) {
     n.pw[it] = kPlan<BX, BY, BZ>.w[it * BVOL + threadIdx.x];
     const int kind = (n.pw[it] >> 24) & 3;
-    n.nb[it] = it == 0 ? (int)b
-                       : (kind ? __builtin_bit_cast(int, buf_ld<float>(
-                                     rN, valid ? b * 27u + ((n.pw[it] >> 10) & 31) : OOB))
-                               : -1);
+    // every lane loads (out of range when it names no neighbour): no branch
+    // around the load, so the loads that follow are issued without waiting
+    const int raw = __builtin_bit_cast(
+        int, buf_ld<float>(rN, (valid && kind) ? b * 27u + ((n.pw[it] >> 10) & 31) : OOB));
+    n.nb[it] = it == 0 ? (int)b : (kind ? raw : -1);
   }
 }
 
@@ -211,17 +212,21 @@ __device__ __forceinline__ void halo_load_fn(HaloRegs<T, BX, BY, BZ>& h,
     const bool ok = n.valid && (it == 0 || n.nb[it] >= 0);
     h.q[it] = ok ? (uint32_t)n.nb[it] * BVOL + ((n.pw[it] >> 15) & 511) : OOB;
   }
+  {  // the brick's own slots first: their addresses need no neighbour-table entry
+    const uint32_t q = h.q[0];
+    h.lab[0] = buf_ld_u8(rL, q);
+    h.v[0] = value(q);
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
+    for (int k = 0; k < 3; ++k) h.d[0][k] = buf_ld<T>(rD, q == OOB ? OOB : k * ld + q);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h.dself[k] = buf_ld<T>(rD, q == OOB ? OOB : (3 + k) * ld + q);
+  }
+#pragma unroll
+  for (int it = 1; it < NIT; ++it) {
     const uint32_t q = h.q[it];
     h.lab[it] = buf_ld_u8(rL, q);
     h.v[it] = value(q);
-    if (it == 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) h.d[0][k] = buf_ld<T>(rD, q == OOB ? OOB : k * ld + q);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) h.dself[k] = buf_ld<T>(rD, q == OOB ? OOB : (3 + k) * ld + q);
-    } else {
+    {
       const uint32_t c0 = (h.pw[it] >> 26) & 7;
       const bool face = ((h.pw[it] >> 24) & 3) == 2 && q != OOB;
       const uint32_t c1 = c0 == 2 ? 4u : 3u, c2 = c0 == 0 ? 4u : 5u;
