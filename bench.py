@@ -72,9 +72,9 @@ TRAFFIC_FILE = os.path.join(HERE, "profiles", "cn_sweep_biv256_traffic.json")
 
 
 def measured_traffic():
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the streaming-copy
-    calibration), or None."""
+    """HBM bytes per launch of the dominant kernel (one CN Jacobi sweep) from
+    the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE,
+    scaled by the streaming-copy calibration), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             return json.load(f)["bytes_per_launch"]

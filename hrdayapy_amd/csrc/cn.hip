@@ -402,6 +402,11 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
     const T* __restrict__ epk_in, int warm) {
   double* sc = cp.scalars;
   if (!kFirst && sc[SC_DONE] != 0.0) return;
+  // the scalars the stopping rule updates are read now (only this kernel's
+  // last lines write them), so their latency overlaps the work below
+  const double s_bb = sc[SC_BB], s_sweeps = sc[SC_SWEEPS], s_warm = sc[SC_WARM],
+               s_total = sc[SC_TOTAL], s_steps = sc[SC_STEPS], s_maxit = sc[SC_MAXIT],
+               s_cmax = sc[SC_CMAX], s_nconv = sc[SC_NCONV], s_worst = sc[SC_WORST];
   // the first batch of block partials is loaded before the junction rows'
   // dependent loads (independent of them), so both latencies overlap
   double xr0[8], xb0[8];
@@ -469,25 +474,25 @@ __global__ void __launch_bounds__(CN_FIN_THREADS) cn_finish_kernel(
   }
   if (kFirst) {
     sc[SC_BB] = bb;
-    if (warm && sc[SC_WARM] < 2.0) sc[SC_WARM] += 1.0;  // steps begun warm, capped at 2
+    if (warm && s_warm < 2.0) sc[SC_WARM] = s_warm + 1.0;  // steps begun warm, capped at 2
   } else {
-    bb = sc[SC_BB];
+    bb = s_bb;
   }
-  const double sweeps = kFirst ? 1.0 : sc[SC_SWEEPS] + 1.0;
+  const double sweeps = kFirst ? 1.0 : s_sweeps + 1.0;
   sc[SC_RR] = rr;
   sc[SC_SWEEPS] = sweeps;
   const bool conv = rr <= cp.tol * cp.tol * bb;  // also when ||b|| = 0 (then r = 0)
   const bool done = conv || sweeps >= (double)cp.max_sweeps;
   sc[SC_DONE] = done ? 1.0 : 0.0;
   if (done) {
-    sc[SC_TOTAL] += sweeps;
-    sc[SC_STEPS] += 1.0;
-    if (sweeps > sc[SC_MAXIT]) sc[SC_MAXIT] = sweeps;
-    if (sweeps > sc[SC_CMAX]) sc[SC_CMAX] = sweeps;
+    sc[SC_TOTAL] = s_total + sweeps;
+    sc[SC_STEPS] = s_steps + 1.0;
+    if (sweeps > s_maxit) sc[SC_MAXIT] = sweeps;
+    if (sweeps > s_cmax) sc[SC_CMAX] = sweeps;
     if (!conv) {
-      sc[SC_NCONV] += 1.0;
+      sc[SC_NCONV] = s_nconv + 1.0;
       const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
-      if (rel > sc[SC_WORST]) sc[SC_WORST] = rel;
+      if (rel > s_worst) sc[SC_WORST] = rel;
     }
   }
 }
