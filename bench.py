@@ -22,8 +22,9 @@ full 600 ms beat (the timed window is part of that run):
             BiCGSTAB with its stopping rules (solver 1, the drop-in default)
   explicit  the fast form: forward-Euler diffusion fused with the reaction
   bigstep / surrogate  the multirate forms (one 0.2 ms reaction per
-            macro-step by the exact model / the learned network, plain
-            integration where the upstroke indicator fires)
+            macro-step by the exact model / by the exact model corrected by
+            the learned network, plain integration where the upstroke
+            indicator fires)
   ref64     the reference scheme with its BiCGSTAB in FP64 (the oracle's
             algorithm, equal to it at round-off level on the test problems):
             the gates' reference
@@ -436,6 +437,9 @@ def main():
     ap.add_argument("--forms", type=str, default="plain,bicgstab,explicit,bigstep,surrogate")
     ap.add_argument("--train-samples", type=int, default=50_000_000)
     ap.add_argument("--train-epochs", type=int, default=14)
+    ap.add_argument("--train-biv", type=int, default=160,
+                    help="grid of the extra BiV training run (another seed than the bench's "
+                         "BiV; 0 = slab runs only)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=100)
@@ -539,13 +543,17 @@ def main():
     if any(f.startswith("surrogate") for f in forms):
         from hrdayapy_amd import surrogate as SG
         t1 = time.perf_counter()
-        X, D = SG.training_runs("cuda", a.train_samples)
+        X, D = SG.training_runs("cuda", a.train_samples,
+                                biv=(a.train_biv, 1) if a.train_biv else None)
         t_data = time.perf_counter() - t1
         net, tr, st = SG.train_surrogate(X, D, "cuda", epochs=a.train_epochs,
                                          log=lambda m: log("[bench] train " + m))
         net_info = dict(samples=int(X.shape[0]), data_s=round(t_data, 1),
-                        train_s=round(st["train_s"], 1), heldout_dv_mae_mV=st["heldout_dv_mae"],
-                        heldout_dv_max_mV=st["heldout_dv_max"])
+                        train_s=round(st["train_s"], 1), role="correction of the big step",
+                        heldout_dv_mae_mV=st["heldout_dv_mae"],
+                        heldout_dv_max_mV=st["heldout_dv_max"],
+                        heldout_bigstep_dv_mae_mV=st.get("heldout_bigstep_dv_mae"),
+                        heldout_bigstep_dv_max_mV=st.get("heldout_bigstep_dv_max"))
         del X, D, tr
         torch.cuda.empty_cache()
         log(f"[bench] surrogate trained: {net_info}")

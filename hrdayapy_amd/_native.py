@@ -14,7 +14,7 @@ from pathlib import Path
 _LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhrdaya_hip.so"
 _lib = None
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 c_i32, c_i64, c_f32, c_f64, c_vp = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_void_p
 
@@ -84,7 +84,8 @@ class HpSurrogateParams(C.Structure):
     _fields_ = [("t_macro", c_f32), ("dvdt_max", c_f32), ("v_lo", c_f32), ("v_hi", c_f32),
                 ("ki_base", c_f32), ("nai_base", c_f32), ("force_fallback", c_i32), ("variant", c_i32),
                 ("fb_list", c_vp), ("fb_count", c_vp), ("slot_list", c_vp), ("n_list", c_i64),
-                ("state_list", c_vp)]
+                ("state_list", c_vp), ("correct", c_i32), ("rest_v", c_f32), ("rest_rate", c_f32),
+                ("corr_max", c_f32)]
 
 
 class HpMgLevel(C.Structure):

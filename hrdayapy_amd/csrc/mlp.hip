@@ -309,6 +309,8 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     const int64_t base = tile * TV;
     int64_t slot[NC], si[NC];
     uint8_t lab[NC];
+    float r0[NC];
+    bool use[NC];  // the network's output is applied to this voxel
     bool any = false;
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
@@ -321,9 +323,17 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
               : (sp.slot_list && sp.state_list)
                   ? (int64_t)sp.state_list[item]
                   : (int64_t)state_index(g.groups, (uint32_t)slot[t]);
-      any |= lab[t] != 0xFF;
+      // correction mode: the big step has decided (rate NaN = fallback); at
+      // rest its own step stands
+      r0[t] = sp.correct ? rate[slot[t]] : 0.f;
+      use[t] = lab[t] != 0xFF;
+      if (sp.correct) {
+        const float v = V[slot[t]];
+        use[t] = use[t] && r0[t] == r0[t] && !(v < sp.rest_v && fabsf(r0[t]) < sp.rest_rate);
+      }
+      any |= use[t];
     }
-    // skip tiles without myocardium (wave-uniform)
+    // skip tiles without a voxel to update (wave-uniform)
     if (!__any(any)) continue;
     // ---- inputs: B fragments (32 features x 16 voxels) of the columns.
     // The raw values are loaded first, unconditionally, from valid addresses
@@ -416,12 +426,18 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
       const int64_t sl = slot[t];
-      const float v = V[sl], vp = V_prev[sl];
+      // correction mode: the big step has set V_prev and listed the fallback
+      // voxels; its rate is the base of ours (use[t] decided above)
+      const float v = V[sl], vp = sp.correct ? v : V_prev[sl];
       const float dvdt = (v - vp) / dtm;
-      const bool fallback = sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
-                            (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
-      if (sp.fb_list) append_fallback(lab[t] != 0xFF && grp == 0 && fallback, sl, sp.fb_list,
-                                      sp.fb_count);
+      // the V correction (output 0, held by the lane of group 0 of this
+      // column) to every lane of the column, for the trust bound
+      const float dv0 = __shfl(out[0][t][0] * S.out_std[0] + S.out_mean[0], col);
+      const bool fallback = sp.correct ? !(use[t] && fabsf(dv0) <= sp.corr_max)
+                                       : (sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
+                                          (v > sp.v_lo && v < sp.v_hi) || (vp != vp));
+      if (sp.fb_list && !sp.correct)
+        append_fallback(lab[t] != 0xFF && grp == 0 && fallback, sl, sp.fb_list, sp.fb_count);
       if (lab[t] == 0xFF) continue;
 #pragma unroll
       for (int m = 0; m < 2; ++m)
@@ -431,7 +447,8 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
           if (o >= NOUT) continue;
           const float d = out[m][t][q] * S.out_std[o] + S.out_mean[o];
           if (o == 0) {
-            rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
+            if (!sp.correct) rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
+            else if (!fallback) rate[sl] = r0[t] + d / dtm;
           } else if (!fallback) {
             float* p = state + (int64_t)(o - 1) * ld + si[t];
             const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
@@ -443,7 +460,7 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     }
     // V_prev of this tile was consumed above (the mode decision depends on it),
     // so the loads have completed before these stores are issued
-    if (grp == 0) {
+    if (grp == 0 && !sp.correct) {
 #pragma unroll
       for (int t = 0; t < NC; ++t)
         if (lab[t] != 0xFF) V_prev[slot[t]] = V[slot[t]];

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define HP_ABI_VERSION 5
+#define HP_ABI_VERSION 6
 
 /* ---------------------------------------------------------------------------
  * Library
@@ -687,6 +687,23 @@ typedef struct hp_surrogate_params {
    * (any value at -1 entries), which spares the kernel the group-table
    * lookup; NULL = looked up in g->groups. */
   const int32_t* state_list;
+  /* 0: the network's output is the macro-step increment itself.  1: it is a
+   * correction of the big step: hp_bigstep_reaction ran first in this
+   * macro-step (state advanced by one t_macro step, rate = -I_ion, V_prev
+   * updated, fallback voxels listed and given rate = NaN), and the network,
+   * fed V and the advanced state, adds its output to that state and rate.
+   * The fallback voxels are then the ones whose rate is NaN; V_prev and the
+   * fallback list are left to the big step. */
+  int32_t correct;
+  /* Correction mode only: voxels at rest, V < rest_v and |rate| < rest_rate
+   * (the big step's rate), keep the big step alone, whose error there is
+   * below the network's; a wave tile of such voxels skips the network.
+   * rest_rate = 0: the network corrects every non-fallback voxel. */
+  float rest_v, rest_rate;
+  /* Correction mode only: a voxel whose V correction exceeds corr_max mV in
+   * magnitude keeps the big step alone (a trust bound on the network; the big
+   * step's own error stays below ~0.1 mV per 0.2 ms on the plateau). */
+  float corr_max;
 } hp_surrogate_params;
 
 /* Training primitives.  C[m,n] = alpha*sum_k A(m,k)B(k,n) + beta*C[m,n] (+ bias[n]),
@@ -716,7 +733,9 @@ int hp_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, f
  * v_lo < V < v_hi, or V_prev is NaN (first call); fallback voxels get
  * rate = NaN (the fused step then runs the full TTP06 for them), the others
  * get their 18 state variables advanced by the network and rate = dV/t_macro
- * (applied by hp_myo_step with do_reaction = 2).  V_prev <- V. */
+ * (applied by hp_myo_step with do_reaction = 2).  V_prev <- V.  With
+ * sp->correct = 1 the network corrects the big step that hp_bigstep_reaction
+ * has just taken (see hp_surrogate_params.correct). */
 int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* state,
                       int64_t ld, float* rate, const hp_mlp_weights* w, const hp_mlp_norm* n,
                       const hp_surrogate_params* sp, int n_blocks, void* stream);

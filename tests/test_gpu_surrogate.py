@@ -86,10 +86,16 @@ def test_gradients_match_cpu_autograd(cuda):
         assert relb < 5e-2
 
 
-@pytest.mark.parametrize("variant,listed", [(0, False), (1, False), (1, True), (1, "state")])
-def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
-    print(f"[case] fused surrogate inference, kernel variant {variant}, slot list {listed}",
-          flush=True)
+@pytest.mark.parametrize("variant,listed,correct", [
+    (0, False, 0), (1, False, 0), (1, True, 0), (1, "state", 0), (0, False, 1), (1, "state", 1),
+    (1, "state", 2)])
+def test_inference_kernel_matches_cpu_forward(cuda, variant, listed, correct):
+    """correct = 1: the network's output is added to the state and rate a big
+    step left (fallback voxels are the ones with rate NaN, voxels at rest keep
+    the big step; V_prev untouched); correct = 2: also a trust bound at the
+    median |V correction|, above which a voxel keeps the big step."""
+    print(f"[case] fused surrogate inference, kernel variant {variant}, slot list {listed}, "
+          f"correction mode {correct}", flush=True)
     from hrdayapy_amd.layout import BrickLayout
     S = np.zeros((12, 10, 9), bool)
     S[1:11, 1:9, 1:8] = True
@@ -127,8 +133,35 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     grp = torch.from_numpy(groups).to(cuda)
     br = nat.HpBricks(lay.n_bricks, 4, 4, 4, 0, nbr.data_ptr(), lab.data_ptr(), None, None,
                       grp.data_ptr())
+    sl = torch.from_numpy(lay.slot_of_node.astype(np.int64))
+    Vn = V.numpy()
+    fb = np.zeros(n, bool)
+    fb[:10] = True
+    fb |= (Vn > -70) & (Vn < -40)
     rate = torch.zeros(lay.n_slots, device=cuda)
+    r0 = np.zeros(n, np.float32)
+    if correct:  # the big step's rate: a number, NaN on its fallback voxels
+        r0 = rng.uniform(-2.0, 2.0, n).astype(np.float32)
+        r0[fb] = np.nan
+        rate[sl.to(cuda)] = torch.from_numpy(r0).to(cuda)
+    Vprev0 = Vprev.clone()
     sp = nat.HpSurrogateParams(0.2, 1.0, -70.0, -40.0, SG.KI_BASE, SG.NAI_BASE, 0, variant)
+    sp.correct = min(correct, 1)
+    # CPU forward (bf16 inputs, as the kernel rounds them)
+    Xn = ((Xt - torch.from_numpy(net.in_mean)) / torch.from_numpy(net.in_std)).to(torch.bfloat16)
+    Yc, _ = _cpu_forward(net, Xn)
+    Dc = Yc * torch.from_numpy(net.out_std) + torch.from_numpy(net.out_mean)
+    rest = np.zeros(n, bool)
+    near = np.zeros(n, bool)  # V correction within 5 % of the trust bound: not checked
+    if correct:  # voxels at rest keep the big step's state and rate
+        sp.rest_v, sp.rest_rate, sp.corr_max = -70.0, 1.0, 1e30
+        rest = (Vn < -70.0) & (np.abs(r0) < 1.0) & ~fb
+        assert rest.sum() > 20
+    if correct == 2:  # so do voxels whose V correction exceeds the bound
+        a = np.abs(Dc[:, 0].numpy())
+        sp.corr_max = float(np.median(a[~fb & ~rest]))
+        near = np.abs(a - sp.corr_max) < 0.05 * sp.corr_max
+        rest |= (a > sp.corr_max) & ~fb & ~near
     if listed:  # the tissue slots in a shuffled order, padded with -1 to a multiple of 32
         perm = rng.permutation(lay.n_nodes)
         tis = lay.slot_of_node[perm].astype(np.int32)
@@ -145,23 +178,22 @@ def test_inference_kernel_matches_cpu_forward(cuda, variant, listed):
     nat.call("hp_surrogate_step", ctypes.byref(br), Vd.data_ptr(), Vprev.data_ptr(),
              state.data_ptr(), n + 64, rate.data_ptr(), ctypes.byref(net.hp_w),
              ctypes.byref(net.hp_norm), ctypes.byref(sp), 4, nat.stream_handle())
-    sl = torch.from_numpy(lay.slot_of_node.astype(np.int64))
     r = rate.cpu()[sl]
-    Vn = V.numpy()
-    fb = np.zeros(n, bool)
-    fb[:10] = True
-    fb |= (Vn > -70) & (Vn < -40)
     assert np.array_equal(np.isnan(r.numpy()), fb)
-    # CPU forward (bf16 inputs, as the kernel rounds them)
-    Xn = ((Xt - torch.from_numpy(net.in_mean)) / torch.from_numpy(net.in_std)).to(torch.bfloat16)
-    Yc, _ = _cpu_forward(net, Xn)
-    Dc = Yc * torch.from_numpy(net.out_std) + torch.from_numpy(net.out_mean)
-    ok = ~fb
-    np.testing.assert_allclose(r.numpy()[ok], (Dc[:, 0] / 0.2).numpy()[ok], rtol=5e-2, atol=2e-3)
+    if correct:
+        assert torch.equal(Vprev, Vprev0)  # the big step's V_prev is kept
+    else:
+        assert torch.equal(Vprev, Vd)
+    ok = ~fb & ~rest & ~near
+    np.testing.assert_allclose(r.numpy()[ok], r0[ok] + (Dc[:, 0] / 0.2).numpy()[ok], rtol=5e-2,
+                               atol=2e-3)
+    assert np.array_equal(r.numpy()[rest], r0[rest])
     new = state.cpu()[:, son_t].t()
     old = state0.cpu()[:, son_t].t()
-    # untouched on fallback voxels
-    assert torch.equal(new[torch.from_numpy(fb)], old[torch.from_numpy(fb)])
+    # untouched on fallback voxels and, in correction mode, on voxels at rest
+    # and beyond the trust bound
+    keep = torch.from_numpy(fb | rest)
+    assert torch.equal(new[keep], old[keep])
     # updated as f^-1(f(x) + d) elsewhere
     lg = torch.from_numpy(SG.IN_LOG[1:19].astype(bool))
     oa = old.clone()
@@ -183,7 +215,7 @@ def trained(cuda):
     transmural slab, as the bench does at full size."""
     eng = SG.training_slab(cuda, nx=40, ny=40, nz=12)
     X, D = SG.build_training_set(eng, 2000, 1500, seed=1)
-    net, tr, st = SG.train_surrogate(X, D, cuda, epochs=6, log=print)
+    net, tr, st = SG.train_surrogate(X, D, cuda, epochs=10, batch=2048, lr_decay=0.85, log=print)
     print(f"  {st['n_train']} samples, {st['train_s']:.1f} s, held-out |dV err| "
           f"mean {st['heldout_dv_mae']:.2e} max {st['heldout_dv_max']:.2e} mV")
     return net, st
@@ -194,6 +226,10 @@ def test_training_converges(cuda, trained):
     net, st = trained
     assert st["loss"][-1] < 0.25 * st["loss"][0]
     assert st["heldout_dv_mae"] < 0.05  # mV over a 0.2 ms macro-step
+    # the corrected macro-step is closer to the ten substeps than the big step
+    print(f"  held-out |dV err|: big step {st['heldout_bigstep_dv_mae']:.2e}, "
+          f"corrected {st['heldout_dv_mae']:.2e} mV")
+    assert st["heldout_dv_mae"] < 0.5 * st["heldout_bigstep_dv_mae"]
 
 
 def _slab_run(cuda, form, net=None, T=120.0, compact=True, graph=False):
@@ -248,6 +284,9 @@ def test_surrogate_forms_vs_plain(cuda, trained):
         assert both.sum() == np.isfinite(at0).sum()
         assert err <= 1.0
     assert res["mlp"][2] < 0.5
+    # what the network gains: its correction brings the big step's Vm closer
+    # to the plain integrator's
+    assert res["mlp"][1] < res["bigstep"][1]
 
 
 @pytest.mark.parametrize("form", ["bigstep", "mlp"])
@@ -284,7 +323,8 @@ def _config5_run(cuda, form, ecg, net=None, T=300.0):
     if ecg is None:
         ecg = W.EcgSetup(eng, meta, n_bspm=0, log=lambda m: None)
     rec = W.EcgRecorder(ecg.lf, eng)
-    run = None if form == "ref64" else SG.SurrogateRunner(eng, net, mode=form, graph_macros=5)
+    run = None if form in ("ref64", "plain") else SG.SurrogateRunner(eng, net, mode=form,
+                                                                      graph_macros=5)
     fbs = []
     for _ in range(int(T)):
         if run is None:
@@ -304,13 +344,19 @@ def test_config5_surrogate_on_slab_with_torso_ecg(cuda):
     transitions of the slab runs (the reference's protocol and a delayed-onset
     run), evaluated on the configuration-1 slab with the torso ECG (12 leads
     every 1 ms for 300 ms) against the reference's scheme in FP64 (CN +
-    BiCGSTAB), beside the network-free big step with the same fallback rule.
-    Both meet the activation-time gate (measured 0.12-0.14 and 0.04 ms against
-    1 ms); neither meets the 0.999 lead-correlation gate on this small heart
-    (measured r = 0.97-0.99 for the network over trainings whose data differ
-    by the solver's round-off, 0.998 for the big step, the T wave carrying the
-    difference), and the network is the less accurate and the costlier of the
-    two: the asserted floors sit just under the measured values."""
+    BiCGSTAB), beside the network-free big step with the same fallback rule
+    and the plain FP32 form.
+
+    What the network gains over the method without it: on held-out
+    transitions its corrected macro-step is ~27 times closer to the ten exact
+    substeps than the big step alone (measured 2.8e-5 against 7.7e-4 mV mean
+    |dV error|), and in the slab its leads are closer to the reference's than
+    the big step's.  Both multirate forms meet the activation-time gate
+    (0.04 ms against 1 ms); on this small heart both miss the 0.999 lead gate
+    in V5/V6 (r = 0.9983, the T wave), where the plain form meets it (0.99902,
+    0.03 ms): the gap
+    is the multirate splitting (the reaction held over 0.2 ms while diffusion
+    moves V), which a 27 times smaller reaction error leaves unchanged."""
     from hrdayapy_amd import workloads as W
     from hrdayapy_amd.ecg.functions.forward import LEAD_NAMES
     print("[case] configuration 5: surrogate on the slab with torso ECG", flush=True)
@@ -319,10 +365,12 @@ def test_config5_surrogate_on_slab_with_torso_ecg(cuda):
     del X, D
     torch.cuda.empty_cache()
     print(f"  trained on {st['n_train']:,} transitions in {st['train_s']:.1f} s, held-out "
-          f"|dV err| mean {st['heldout_dv_mae']:.2e} max {st['heldout_dv_max']:.2e} mV", flush=True)
+          f"|dV err| mean {st['heldout_dv_mae']:.2e} max {st['heldout_dv_max']:.2e} mV (big step "
+          f"alone: mean {st['heldout_bigstep_dv_mae']:.2e} max {st['heldout_bigstep_dv_max']:.2e})",
+          flush=True)
     at0, l0, _, ecg = _config5_run(cuda, "ref64", None)
     res = {}
-    for form in ("mlp", "bigstep"):
+    for form in ("plain", "mlp", "bigstep"):
         at, leads, fb, _ = _config5_run(cuda, form, ecg, net)
         both = np.isfinite(at) & np.isfinite(at0)
         err = float(np.max(np.abs(at[both] - at0[both])))
@@ -334,5 +382,9 @@ def test_config5_surrogate_on_slab_with_torso_ecg(cuda):
             f"{n} {a}/{b}" for n, (a, b) in zip(LEAD_NAMES, per)), flush=True)
     for form, (err, r, rel, fb, n) in res.items():
         assert n == len(at0) and err <= 1.0  # every voxel activates, AT gate met
-        assert fb < 0.5
-    assert res["mlp"][1] >= 0.96 and res["bigstep"][1] >= 0.995
+        assert form == "plain" or fb < 0.5
+    assert res["plain"][1] >= 0.999  # the plain form meets the lead gate
+    # the gain: per macro-step on held-out transitions, and in the leads
+    assert st["heldout_dv_mae"] < 0.1 * st["heldout_bigstep_dv_mae"]
+    assert res["mlp"][2] < res["bigstep"][2] and res["mlp"][1] >= res["bigstep"][1]
+    assert res["mlp"][1] >= 0.998
