@@ -195,14 +195,32 @@ __device__ __forceinline__ int inv_perm_k(int n) {
   return n < 16 ? 8 * (n >> 2) + (n & 3) : 8 * ((n - 16) >> 2) + 4 + ((n - 16) & 3);
 }
 
+// Per-feature (input f) and per-output (o) constants of the inference kernel.
+// kind: input 0 = V, 1 = state row, 2 = state row as a logarithm, 3..5 =
+// phenotype one-hot 0..2, 6 = padding; output 0 = V (the rate), 1 = state
+// row, 2 = state row as a log-ratio, 6 = padding.  row: byte offset of the
+// state row in the SoA state array, or kNoRow (past the buffer: loads read 0,
+// stores are dropped).
+struct MlpFeat {
+  float rstd, nmb, base;  // x -> (x + base) * rstd + nmb  (nmb = -mean * rstd)
+  int32_t kind;
+  uint32_t row;
+};
+struct MlpOut {
+  float std, mean, base;  // d = y * std + mean; stored value relative to base
+  int32_t kind;
+  uint32_t row;
+};
+constexpr uint32_t kNoRow = 0xF0000000u;
+
 struct MlpSmem {
   __bf16 w1[NHID][32];          // [out][in], natural input order (padded 22 -> 32)
   __bf16 w2[NHID][NHID];        // [out][perm(in)]
   __bf16 w3[NHID][NHID];
   __bf16 w4[32][NHID];          // 19 -> 32 rows
   float b1[NHID], b2[NHID], b3[NHID], b4[32];
-  float in_mean[32], in_rstd[32], out_mean[32], out_std[32];
-  int in_log[32];
+  MlpFeat fin[32];
+  MlpOut fout[32];
 };
 
 // The bias of rows 4*grp..+4 of output tile m: the accumulator's initial value
@@ -218,11 +236,15 @@ __device__ __forceinline__ floatx4 bias4(const float* bias, int m, int grp) {
 // with the hardware exp2 and reciprocal (~1 ulp each, far below the bf16
 // rounding of g); 2^u -> inf gives g = 0 for z << 0.
 __device__ __forceinline__ void silu_pack(const floatx4& c, int m, bf16x8& dst) {
+  // the four activations' steps interleaved (exp2, add, rcp, multiply), so
+  // that no instruction waits on the one before it
+  float e[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const float u = c[q];
-    dst[(m & 1) * 4 + q] = f2bf(u * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(u)));
-  }
+  for (int q = 0; q < 4; ++q) e[q] = __builtin_amdgcn_exp2f(c[q]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) e[q] = __builtin_amdgcn_rcpf(1.f + e[q]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) dst[(m & 1) * 4 + q] = f2bf(c[q] * e[q]);
 }
 
 
@@ -243,7 +265,8 @@ __device__ __forceinline__ void append_fallback(bool fb, int64_t slot, int32_t* 
 // NC MFMA columns of 16 voxels per wave, THREADS per workgroup.  NC = 1 with
 // 768 threads fits 170 registers (three waves per SIMD, the weights in LDS
 // shared by 12 waves); NC = 2 with 512 threads reuses each A fragment for two
-// columns at two waves per SIMD.
+// columns at two waves per SIMD, with 1024 threads (within 128 registers) at
+// four waves per SIMD sharing one copy of the weights.
 template <int NC, int THREADS>
 __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const float* __restrict__ V,
                                                         float* __restrict__ V_prev,
@@ -290,18 +313,33 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     S.b3[e] = wts.b3[e];
   }
   if (tid < 32) {
-    S.b4[tid] = tid < NOUT ? wts.b4[tid] : 0.f;
-    S.in_mean[tid] = tid < NIN ? nrm.in_mean[tid] : 0.f;
-    S.in_rstd[tid] = tid < NIN ? nrm.in_rstd[tid] : 0.f;
-    S.in_log[tid] = tid < NIN ? nrm.in_log[tid] : 0;
-    S.out_mean[tid] = tid < NOUT ? nrm.out_mean[tid] : 0.f;
-    S.out_std[tid] = tid < NOUT ? nrm.out_std[tid] : 0.f;
+    const int f = tid;
+    S.b4[f] = f < NOUT ? wts.b4[f] : 0.f;
+    // the FP32 engine stores Ki and Nai relative to their bases
+    const float base = f == 1 ? sp.ki_base : (f == 2 ? sp.nai_base : 0.f);
+    const bool row = f >= 1 && f < 19;
+    MlpFeat fi;
+    fi.rstd = f < NIN ? nrm.in_rstd[f] : 0.f;
+    fi.nmb = f < NIN ? -nrm.in_mean[f] * fi.rstd : 0.f;
+    fi.base = row ? base : 0.f;
+    fi.kind = f == 0 ? 0 : row ? (nrm.in_log[f] ? 2 : 1) : (f < NIN ? 3 + (f - 19) : 6);
+    fi.row = row ? (uint32_t)(f - 1) * (uint32_t)ld * 4u : kNoRow;
+    S.fin[f] = fi;
+    MlpOut fo;
+    fo.std = f < NOUT ? nrm.out_std[f] : 0.f;
+    fo.mean = f < NOUT ? nrm.out_mean[f] : 0.f;
+    fo.base = row ? base : 0.f;
+    fo.kind = f == 0 ? 0 : row ? (nrm.in_log[f] ? 2 : 1) : 6;
+    fo.row = row ? (uint32_t)(f - 1) * (uint32_t)ld * 4u : kNoRow;
+    S.fout[f] = fo;
   }
   __syncthreads();
+  // the whole SoA state array as one buffer (byte offsets; checked on the host)
+  const buf_t st_r = buf_rsrc(state, (uint32_t)(ld * N_STATE * 4));
 
   const int64_t n_items = sp.slot_list ? sp.n_list : g.n_bricks * (int64_t)(g.bx * g.by * g.bz);
   const int col = lane & 15, grp = lane >> 4;
-  const float dtm = sp.t_macro;
+  const float dtm = sp.t_macro, inv_dtm = 1.f / sp.t_macro;
   // persistent loop: one TV-voxel tile (NC 16-voxel MFMA columns) per wave,
   // over the slots or over the listed slots
   for (int64_t tile = (int64_t)blockIdx.x * WAVES + wave; tile * TV < n_items;
@@ -336,31 +374,37 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
     // skip tiles without a voxel to update (wave-uniform)
     if (!__any(any)) continue;
     // ---- inputs: B fragments (32 features x 16 voxels) of the columns.
-    // The raw values are loaded first, unconditionally, from valid addresses
-    // (V or a state row; the one-hot features and lanes without a voxel
-    // are replaced afterwards), so the loads of a tile are in flight together.
-    float raw[NC][8];
+    // Every lane loads its 8 features' state rows at once (byte offsets into
+    // the state buffer; rows that are not state variables read 0), then
+    // forms the features without branches from the per-feature table.
+    float vv[NC], raw[NC][8];
+    uint32_t sbyte[NC];
 #pragma unroll
-    for (int t = 0; t < NC; ++t)
+    for (int t = 0; t < NC; ++t) {
+      vv[t] = V[slot[t]];
+      sbyte[t] = (uint32_t)si[t] * 4u;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int f = grp * 8 + q, fc = f < 19 ? f : 0;
-        raw[t][q] = fc == 0 ? V[slot[t]] : state[(int64_t)(fc - 1) * ld + si[t]];
-      }
+      for (int q = 0; q < 8; ++q)
+        raw[t][q] = __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(st_r, S.fin[grp * 8 + q].row + sbyte[t], 0,
+                                                        0));
+    }
     bf16x8 xin[NC];
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
+      const int ct = lab[t] & 3;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int f = grp * 8 + q;
-        float x = 0.f;
-        if (lab[t] != 0xFF && f < NIN) {
-          x = f >= 19 ? ((f - 19) == (lab[t] & 3) ? 1.f : 0.f)
-                      : raw[t][q] + (f == 1 ? sp.ki_base : (f == 2 ? sp.nai_base : 0.f));
-          if (S.in_log[f]) x = __logf(fmaxf(x, 1e-30f));
-          x = (x - S.in_mean[f]) * S.in_rstd[f];
-        }
-        xin[t][q] = f2bf(x);
+        const MlpFeat c = S.fin[grp * 8 + q];
+        float x = raw[t][q] + c.base;
+        if (q == 0) x = c.kind == 0 ? vv[t] : x;  // V is feature 0 (group 0, q 0)
+        if (q >= 3 && q <= 5)                      // the one-hot is features 19-21
+          x = c.kind >= 3 ? (c.kind - 3 == ct ? 1.f : 0.f) : x;
+        // hardware log2 (the argument is clamped above the denormal range)
+        const float lx = __builtin_amdgcn_logf(fmaxf(x, 1e-30f)) * 0.69314718f;
+        x = c.kind == 2 ? lx : x;
+        x = fmaf(x, c.rstd, c.nmb);
+        xin[t][q] = f2bf(lab[t] != 0xFF ? x : 0.f);
       }
     }
     // ---- layer 1 (one K step) -> hidden activations in B-fragment form
@@ -411,52 +455,56 @@ __global__ void __launch_bounds__(THREADS) surrogate_kernel(hp_bricks g, const f
       for (int t = 0; t < NC; ++t) out[m][t] = c[t];
     }
     // ---- mode decision and update, per voxel (this lane's 4+4 outputs).
-    // The state values this lane updates are loaded first, unconditionally
-    // (valid addresses; used only for non-fallback voxels).
+    // The state values this lane updates are loaded first (rows that are not
+    // state variables read 0); stores of rows that are not updated go past
+    // the buffer and are dropped.
     float sx[NC][2][4];
 #pragma unroll
     for (int t = 0; t < NC; ++t)
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int o = m * 16 + grp * 4 + q, oc = o >= 1 && o < NOUT ? o : 1;
-          sx[t][m][q] = state[(int64_t)(oc - 1) * ld + si[t]];
-        }
+        for (int q = 0; q < 4; ++q)
+          sx[t][m][q] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(
+                         st_r, S.fout[m * 16 + grp * 4 + q].row + sbyte[t], 0, 0));
 #pragma unroll
     for (int t = 0; t < NC; ++t) {
       const int64_t sl = slot[t];
       // correction mode: the big step has set V_prev and listed the fallback
       // voxels; its rate is the base of ours (use[t] decided above)
-      const float v = V[sl], vp = sp.correct ? v : V_prev[sl];
-      const float dvdt = (v - vp) / dtm;
       // the V correction (output 0, held by the lane of group 0 of this
       // column) to every lane of the column, for the trust bound
-      const float dv0 = __shfl(out[0][t][0] * S.out_std[0] + S.out_mean[0], col);
-      const bool fallback = sp.correct ? !(use[t] && fabsf(dv0) <= sp.corr_max)
-                                       : (sp.force_fallback || fabsf(dvdt) > sp.dvdt_max ||
-                                          (v > sp.v_lo && v < sp.v_hi) || (vp != vp));
+      const float dv0 = __shfl(out[0][t][0] * S.fout[0].std + S.fout[0].mean, col);
+      bool fallback;
+      if (sp.correct) {
+        fallback = !(use[t] && fabsf(dv0) <= sp.corr_max);
+      } else {
+        const float v = vv[t], vp = V_prev[sl];
+        fallback = sp.force_fallback || fabsf((v - vp) / dtm) > sp.dvdt_max ||
+                   (v > sp.v_lo && v < sp.v_hi) || (vp != vp);
+      }
       if (sp.fb_list && !sp.correct)
         append_fallback(lab[t] != 0xFF && grp == 0 && fallback, sl, sp.fb_list, sp.fb_count);
-      if (lab[t] == 0xFF) continue;
+      const bool upd = lab[t] != 0xFF && !fallback;
 #pragma unroll
       for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int o = m * 16 + grp * 4 + q;
-          if (o >= NOUT) continue;
-          const float d = out[m][t][q] * S.out_std[o] + S.out_mean[o];
-          if (o == 0) {
-            if (!sp.correct) rate[sl] = fallback ? __builtin_nanf("") : d / dtm;
-            else if (!fallback) rate[sl] = r0[t] + d / dtm;
-          } else if (!fallback) {
-            float* p = state + (int64_t)(o - 1) * ld + si[t];
-            const float base_v = o == 1 ? sp.ki_base : (o == 2 ? sp.nai_base : 0.f);
-            const float x0 = sx[t][m][q] + base_v;
-            const float x1 = S.in_log[o] ? __expf(__logf(fmaxf(x0, 1e-30f)) + d) : x0 + d;
-            *p = x1 - base_v;
-          }
+          const MlpOut c = S.fout[m * 16 + grp * 4 + q];
+          const float d = fmaf(out[m][t][q], c.std, c.mean);
+          const float x0 = sx[t][m][q] + c.base;
+          const float x1 =
+              c.kind == 2 ? fmaxf(x0, 1e-30f) * __builtin_amdgcn_exp2f(d * 1.44269504f) : x0 + d;
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x1 - c.base), st_r,
+                                                (upd ? c.row : kNoRow) + sbyte[t], 0, 0);
         }
+      // the rate: output 0 (m = 0, q = 0) lives in the lanes of group 0
+      if (grp == 0 && lab[t] != 0xFF) {
+        const float d = fmaf(out[0][t][0], S.fout[0].std, S.fout[0].mean);
+        if (!sp.correct) rate[sl] = fallback ? __builtin_nanf("") : d * inv_dtm;
+        else if (!fallback) rate[sl] = r0[t] + d * inv_dtm;
+      }
     }
     // V_prev of this tile was consumed above (the mode decision depends on it),
     // so the loads have completed before these stores are issued
@@ -607,6 +655,14 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
   }
   const int64_t n_slots = g->n_bricks * (int64_t)(g->bx * g->by * g->bz);
   const int64_t n_items = sp->slot_list ? sp->n_list : n_slots;
+  if ((uint64_t)ld * N_STATE * 4 >= (uint64_t)kNoRow) {
+    set_error("hp_surrogate_step: the state array (18 x ld floats) must be below 3.75 GiB");
+    return -1;
+  }
+  if (sp->variant < 0 || sp->variant > 2) {
+    set_error("hp_surrogate_step: variant must be 0, 1 or 2");
+    return -1;
+  }
   if (n_items % 32 || n_items <= 0) {
     set_error("hp_surrogate_step: slot (or slot_list) count must be a positive multiple of 32");
     return -1;
@@ -618,14 +674,19 @@ int hp_surrogate_step(const hp_bricks* g, const float* V, float* V_prev, float* 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel<2, 512>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    HP_CHECK(hipFuncSetAttribute((const void*)surrogate_kernel<2, 1024>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     attr_set = true;
   }
   if (n_blocks <= 0) n_blocks = 256;
-  const int nc = sp->variant == 1 ? 2 : 1, waves = nc == 2 ? 8 : 12;
+  const int nc = sp->variant >= 1 ? 2 : 1, waves = sp->variant == 2 ? 16 : nc == 2 ? 8 : 12;
   const int64_t tiles = n_items / (16 * nc);
   if ((int64_t)n_blocks * waves > tiles) n_blocks = (int)((tiles + waves - 1) / waves);
   if (nc == 1)
     hipLaunchKernelGGL((surrogate_kernel<1, 768>), dim3(n_blocks), dim3(768), smem,
+                       (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *w, *n, *sp);
+  else if (sp->variant == 2)
+    hipLaunchKernelGGL((surrogate_kernel<2, 1024>), dim3(n_blocks), dim3(1024), smem,
                        (hipStream_t)stream, *g, V, V_prev, state, ld, rate, *w, *n, *sp);
   else
     hipLaunchKernelGGL((surrogate_kernel<2, 512>), dim3(n_blocks), dim3(512), smem,

@@ -387,7 +387,7 @@ class SurrogateRunner:
     """
 
     def __init__(self, eng, net: SurrogateNet = None, mode="mlp", dvdt_max=1.0, v_lo=-70.0,
-                 v_hi=-40.0, n_blocks=0, compact=True, graph_macros=2, variant=1,
+                 v_hi=-40.0, n_blocks=0, compact=True, graph_macros=2, variant=2,
                  rest_v=REST_V, rest_rate=REST_RATE, corr_max=CORR_MAX):
         if eng.dtype != torch.float32:
             raise ValueError("the surrogate form runs on the FP32 engine")
@@ -418,8 +418,9 @@ class SurrogateRunner:
         self.state_list = torch.zeros_like(self.slot_list)
         self.state_list[:tissue.numel()] = torch.arange(tissue.numel(), dtype=torch.int32,
                                                         device=eng.dev)
-        # inference kernel shape: variant 1 (two 16-voxel MFMA columns per wave,
-        # 512 threads) measured 17 % faster than variant 0 on the 2 M-voxel BiV
+        # inference kernel shape: two 16-voxel MFMA columns per wave; variant 2
+        # (1024 threads, four waves per SIMD) 323 us per call on the 2 M-voxel
+        # BiV against 368 us for variant 1 (512 threads)
         self.sp = nat.HpSurrogateParams(T_MACRO, dvdt_max, v_lo, v_hi, eng.ki_base, eng.nai_base,
                                         0, int(variant),
                                         nat.ptr(self.fb_list) if self.compact else None,

@@ -672,7 +672,8 @@ typedef struct hp_surrogate_params {
   float ki_base, nai_base;
   int32_t force_fallback;
   int32_t variant;      /* inference kernel: 0 = 16 voxels per wave, 768 threads;
-                           1 = 32 voxels per wave, 512 threads */
+                           1 = 32 voxels per wave, 512 threads; 2 = 32 voxels per
+                           wave, 1024 threads */
   /* Optional compaction of the fallback voxels (both NULL = off): their slots
    * are appended to fb_list (capacity n_slots) and counted in *fb_count, which
    * the caller zeroes before the reaction call.  List order is unspecified. */

@@ -105,6 +105,12 @@ def test_mlp_arguments(lib):
     sp.slot_list, sp.n_list = ptr, 33  # a listed count that is not a multiple of 32
     _rejects(lib.hp_surrogate_step(C.byref(g), ptr, ptr, ptr, 64, ptr, C.byref(w), C.byref(n),
                                    C.byref(sp), 0, None), "multiple of 32")
+    sp.n_list, sp.variant = 32, 3  # no such kernel shape
+    _rejects(lib.hp_surrogate_step(C.byref(g), ptr, ptr, ptr, 64, ptr, C.byref(w), C.byref(n),
+                                   C.byref(sp), 0, None), "variant")
+    sp.variant = 2  # a state array (18 x ld floats) beyond 32-bit byte offsets
+    _rejects(lib.hp_surrogate_step(C.byref(g), ptr, ptr, ptr, 60_000_000, ptr, C.byref(w),
+                                   C.byref(n), C.byref(sp), 0, None), "3.75 GiB")
 
 
 def test_small_kernels_arguments(lib):

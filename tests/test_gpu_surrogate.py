@@ -87,8 +87,8 @@ def test_gradients_match_cpu_autograd(cuda):
 
 
 @pytest.mark.parametrize("variant,listed,correct", [
-    (0, False, 0), (1, False, 0), (1, True, 0), (1, "state", 0), (0, False, 1), (1, "state", 1),
-    (1, "state", 2)])
+    (0, False, 0), (1, False, 0), (1, True, 0), (1, "state", 0), (2, "state", 0), (0, False, 1),
+    (1, "state", 1), (2, "state", 1), (2, "state", 2)])
 def test_inference_kernel_matches_cpu_forward(cuda, variant, listed, correct):
     """correct = 1: the network's output is added to the state and rate a big
     step left (fallback voxels are the ones with rate NaN, voxels at rest keep
