@@ -587,7 +587,8 @@ def main():
         log(f"[bench] CPU baseline ({a.cpu_steps} steps)")
         cpu = cpu_baseline((geo, tree), kw, a.cpu_steps)
     sr = results.get("surrogate")
-    head = "surrogate" if (sr and sr["vs_ref64"]["pass"]) else "plain"
+    head = "surrogate" if (sr and sr["vs_ref64"]["pass"]) else (
+        "plain" if "plain" in results else forms[0])
     hr = results[head]
     e2e = dict(geometry_s=round(t_geo, 1), engine_setup_s=hr["setup_s"],
                leadfield_build_s=ecg_info["leadfield_setup_s"] if ecg_info else 0.0,
