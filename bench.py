@@ -65,6 +65,9 @@ APPLY_BYTES_PER_VOXEL = 41
 # V* written 4, label 1
 REACT_BYTES_PER_VOXEL = 153
 STEPS_PER_MS = 50
+# the 22-128-128-128-19 network: 2 x (22*128 + 128*128 + 128*128 + 128*19) FLOP per voxel
+MLP_FLOP_PER_VOXEL = 76032
+BF16_PEAK_TFS = 2500.0
 # CN solve per step captured in the step's graph: BiCGSTAB iterations (the
 # reference's solver) / Jacobi sweeps; the ones after the solve ended return
 # at once
@@ -343,7 +346,10 @@ def config1_workload(a):
     prob = OMD.Problem(nodes, el, act, S, phi, fib, **kw)
     steps = int(round(T / 0.02)) if a.config1_cpu_steps <= 0 else a.config1_cpu_steps
     log(f"[config1] oracle on {cores} threads: {steps} steps ({model})")
-    r = OMD.run(prob, 0.02 * (steps + 0.5), 0.02, scheme="cn", max_steps=steps)
+    t_cpu = time.perf_counter()
+    r = OMD.run(prob, 0.02 * (steps + 0.5), 0.02, scheme="cn", max_steps=steps,
+                progress=lambda i, n: log(f"[config1] oracle step {i}/{n} "
+                                          f"({time.perf_counter() - t_cpu:.0f} s)"))
     cpu_vs = prob.N_myo * r["steps"] / r["wall_s"]
     res = {"metric": "voxel-steps/s, configuration 1 (64x64x16 slab, FP64, 500 ms)",
            "value": out["float64"]["voxel_steps_per_s"], "unit": "voxel-steps/s", "n_gpus": 1,
@@ -396,6 +402,51 @@ def other_workload(a):
                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": gbs / HBM_PEAK_GBS, "traffic": None,
                             "bytes_per_voxel": W.STENCIL_BYTES_PER_VOXEL}}
+    elif a.workload == "mlp":
+        # the learned correction's inference kernel with every voxel of the BiV
+        # through the network (rate set to a number: no fallback, no voxel at rest)
+        import torch
+        from hrdayapy_amd import _native as nat
+        from hrdayapy_amd import surrogate as SG
+        geo, tree, kw, _ = build_geometry(n=a.grid, h=0.4 * 256 / a.grid)
+        brick = (8, 4, 4) if a.brick == "auto" else tuple(int(x) for x in a.brick.split("x"))
+        eng, meta = build_engine(geo, tree, kw, brick=brick, cn_solver="jacobi")
+        net = SG.SurrogateNet("cuda", seed=0)
+        net.correct = True
+        # output layer zeroed: the correction is 0, so repeated calls leave the
+        # state and the rate as they are (same work per call)
+        net.W[3].zero_()
+        net.b[3].zero_()
+        net.refresh()
+        run = SG.SurrogateRunner(eng, net, mode="mlp")
+        state0 = eng.state.clone()
+        res = {}
+        for variant in (2, 1, 0):
+            sp = nat.HpSurrogateParams.from_buffer_copy(run.sp)
+            sp.variant = variant
+            run.rate.fill_(1.0)
+
+            def launch(i, sp=sp):
+                nat.call("hp_surrogate_step", C.byref(eng.bricks), nat.ptr(eng.V[eng.p]),
+                         nat.ptr(run.V_prev), nat.ptr(eng.state), eng.state_ld,
+                         nat.ptr(run.rate), C.byref(net.hp_w), C.byref(net.hp_norm),
+                         C.byref(sp), 0, nat.stream_handle())
+            res[variant] = W.graph_us(launch, a.kernel_reps)
+            eng.state.copy_(state0)
+            log(f"[mlp] variant {variant}: {res[variant]:.1f} us per call")
+        n_myo = meta["N_myo"]
+        flops = MLP_FLOP_PER_VOXEL * n_myo
+        tfs = flops / (res[2] * 1e-6) / 1e12
+        out = {"metric": "learned-correction inference, voxels/s (every voxel through the network)",
+               "value": n_myo / (res[2] * 1e-6), "unit": "voxels/s", "n_gpus": 1,
+               "higher_is_better": True, "dtype": "bf16 MFMA, fp32 accumulation",
+               "data": "synthetic (random weights, resting state)",
+               "config": {"workload": f"synthetic BiV 256^3 @0.4 mm, {n_myo:,} voxels, "
+                                      "22-128-128-128-19 SiLU MLP"},
+               "result": {"us_per_call": {str(k): round(v, 2) for k, v in res.items()}},
+               "roofline": {"bound": "mfma", "achieved": tfs, "peak": BF16_PEAK_TFS,
+                            "unit": "TFLOP/s", "frac": tfs / BF16_PEAK_TFS, "traffic": None,
+                            "flop_per_voxel": MLP_FLOP_PER_VOXEL}}
     elif a.workload == "ionic":
         r = W.ionic_run(reps=a.kernel_reps)
         log(f"[ionic] {r}")
@@ -448,7 +499,7 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=1024,
                     help="size of the streaming-copy calibration (0 = off)")
     ap.add_argument("--workload", type=str, default="biv",
-                    choices=["biv", "torso", "ensemble", "ionic", "stencil", "config1"],
+                    choices=["biv", "torso", "ensemble", "ionic", "stencil", "config1", "mlp"],
                     help="biv: the coupled 2M-voxel BiV with ECG (default); torso: the "
                          "384x384x256 Poisson solve; ensemble: 64 instances in one brick list; "
                          "config1: the 64x64x16 FP64 slab in full, GPU and CPU oracle")

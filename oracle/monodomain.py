@@ -168,7 +168,8 @@ def cn_jacobi_solve(A, M_inv, L, Vs, b, tol, max_sweeps, junctions=(), Np=0, e=N
 
 def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtype=torch.float64,
         record_every=0, record_nodes=None, at_thr=-40.0, max_steps=None, state_init=None,
-        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10, dot=None, warm=True):
+        V_init=None, g_ks_scale=None, g_to_scale=None, max_sweeps=10, dot=None, warm=True,
+        progress=None):
     """Integrate the problem; returns dict(V, state, at, trace_t, trace_V, steps, wall_s).
 
     ``at`` is (N,) first activation time (NaN if none), on every node.
@@ -176,6 +177,7 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
     ``record_every`` steps (0 = off), stamped like the reference's frames.
     ``g_ks_scale``/``g_to_scale``: optional per-node multipliers of the
     phenotype conductances (the ensemble parameter block).
+    ``progress``: optional callable(step, n_steps), called every 1000 steps.
     """
     N, Cm = prob.N, prob.Cm
     n_steps = int(T / dt)
@@ -226,6 +228,8 @@ def run(prob: Problem, T, dt, *, scheme="cn", cg_tol=1e-5, cg_max_iter=500, dtyp
         if cross.any():
             at[cross] = (i - 1) * dt + dt * (at_thr - V[cross]) / (Vn[cross] - V[cross])
         V = Vn
+        if progress is not None and (i + 1) % 1000 == 0:
+            progress(i + 1, n_steps)
         if record_every and i % record_every == 0:
             tr_t.append(t)
             tr_V.append(V.copy() if record_nodes is None else V[record_nodes].copy())
