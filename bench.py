@@ -133,6 +133,8 @@ FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
     # rate (fallback voxels run the full cell model every substep)
     "bigstep": ("cn", "float32", "bigstep", dict(cn_solver="jacobi")),
     "surrogate": ("cn", "float32", "mlp", dict(cn_solver="jacobi")),
+    # the big-step rule with cold-started sweeps (x0 = V*, no warm start)
+    "bigstep_cold": ("cn", "float32", "bigstep", dict(cn_solver="jacobi", cn_warm=False)),
     # the same rules on explicit substeps (the earlier multirate form)
     "bigstep_explicit": ("explicit", "float32", "bigstep", {}),
     "surrogate_explicit": ("explicit", "float32", "mlp", {}),
@@ -627,7 +629,8 @@ def main():
         del eng, vm
         torch.cuda.empty_cache()
     plain_v = results["plain"]["value"] if "plain" in results else None
-    for f in ("explicit", "bigstep", "surrogate", "bigstep_explicit", "surrogate_explicit",
+    for f in ("explicit", "bigstep", "surrogate", "bigstep_cold", "bigstep_explicit",
+              "surrogate_explicit",
               "cn2"):
         if f in results and plain_v:
             results[f]["speedup_vs_plain"] = results[f]["value"] / plain_v
