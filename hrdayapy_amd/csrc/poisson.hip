@@ -454,44 +454,49 @@ __global__ void source_gather_kernel(const double* __restrict__ lap, const int32
 // Lead-field ECG: out[f][e] = sum_r W[r][e] * V[f][slot[r]] over the rows r
 // (heart voxels), FP32 products accumulated in FP64.  Pass 1: block q sums
 // its row range into part[q][f][e]; pass 2 adds the block partials in block
-// order (deterministic).  Threads of a block take consecutive electrodes, so
-// a row of W is read coalesced; the block's V values are staged in LDS.
+// order (deterministic).
 constexpr int LF_MAXF = 16;  // frames per call (the M = 16 of the MFMA tile)
-// FP64 MFMA form: one wave per tile of 16 electrodes, v_mfma_f64_16x16x4f64
-// over 4 rows per instruction (A = V^T: 16 frames x 4 rows, B = W: 4 rows x
-// 16 electrodes, C = the 16 x 16 FP64 partial sums in 4 registers per lane).
+// FP64 MFMA form: v_mfma_f64_16x16x4f64 over 4 rows per instruction (A = V^T:
+// 16 frames x 4 rows, B = W: 4 rows x 16 electrodes, C = the 16 x 16 FP64
+// partial sums in 4 registers per lane).
 // Operand layout of the instruction: lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; C[4*i + l/16][l%16] in its i-th register (the FP64 form; the
 // FP32 16x16x4 form keeps rows 4*(l/16)+i instead).  Eight K-steps
 // of loads are issued before their MFMAs.  Frames past n_frames, electrodes
 // past ne and rows past the block's range enter as 0.
-constexpr int LF_WAVES = 8, LF_UNROLL = 8;
-__global__ void __launch_bounds__(LF_WAVES * 64) leadfield_partial_kernel(
+constexpr int LF_MAXW = 16, LF_UNROLL = 8;
+// One wave per 32 electrodes (two 16-electrode MFMA tiles sharing the rows'
+// slot and V loads); one workgroup of up to LF_MAXW waves covers every
+// electrode of its row range, so its waves read the same rows of W at about
+// the same time and each 128-byte line of W is fetched once.
+__global__ void __launch_bounds__(LF_MAXW * 64) leadfield_partial_kernel(
     const float* __restrict__ W, const int32_t* __restrict__ row_slot, int64_t n_rows, int ne,
     const float* __restrict__ V, int64_t ldv, int n_frames, int64_t rows_per_block,
     double* __restrict__ part) {
   typedef double d4 __attribute__((ext_vector_type(4)));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int e0 = (blockIdx.y * LF_WAVES + wave) * 16;
+  const int e0 = (blockIdx.y * (blockDim.x >> 6) + wave) * 32;
   if (e0 >= ne) return;  // whole wave
   const int mi = lane & 15, kq = lane >> 4;  // A row (frame) / B column (electrode); K index
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = r0 + rows_per_block < n_rows ? r0 + rows_per_block : n_rows;
-  const bool frame_ok = mi < n_frames, elec_ok = e0 + mi < ne;
+  const bool frame_ok = mi < n_frames, ok_a = e0 + mi < ne, ok_b = e0 + 16 + mi < ne;
   // every load reads an in-range element (clamped row, frame, electrode) and
   // is zeroed afterwards, so the loads issue without branches
-  const int mc = frame_ok ? mi : 0, ec = elec_ok ? e0 + mi : ne - 1;
-  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int mc = frame_ok ? mi : 0, ea = ok_a ? e0 + mi : ne - 1, eb = ok_b ? e0 + 16 + mi : ne - 1;
+  d4 acc_a = {0.0, 0.0, 0.0, 0.0}, acc_b = {0.0, 0.0, 0.0, 0.0};
   if (r0 >= r1) return;
   for (int64_t r = r0; r < r1; r += 4 * LF_UNROLL) {
     int32_t sl[LF_UNROLL];
-    float wv[LF_UNROLL];
+    float wa[LF_UNROLL], wb[LF_UNROLL];
 #pragma unroll
     for (int u = 0; u < LF_UNROLL; ++u) {
       const int64_t row = r + 4 * u + kq, rc = row < r1 ? row : r1 - 1;
       sl[u] = row_slot[rc];
-      wv[u] = W[rc * ne + ec];
-      if (!(row < r1 && elec_ok)) wv[u] = 0.f;
+      wa[u] = W[rc * ne + ea];
+      wb[u] = W[rc * ne + eb];
+      if (!(row < r1 && ok_a)) wa[u] = 0.f;
+      if (!(row < r1 && ok_b)) wb[u] = 0.f;
     }
     float av[LF_UNROLL];
 #pragma unroll
@@ -500,16 +505,19 @@ __global__ void __launch_bounds__(LF_WAVES * 64) leadfield_partial_kernel(
       if (!(r + 4 * u + kq < r1 && frame_ok)) av[u] = 0.f;
     }
 #pragma unroll
-    for (int u = 0; u < LF_UNROLL; ++u)
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], (double)wv[u], acc, 0, 0, 0);
-  }
-  const int e = e0 + mi;
-  if (e < ne)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int f = 4 * i + kq;
-      if (f < n_frames) part[((int64_t)blockIdx.x * n_frames + f) * ne + e] = acc[i];
+    for (int u = 0; u < LF_UNROLL; ++u) {
+      acc_a = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], (double)wa[u], acc_a, 0, 0, 0);
+      acc_b = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u], (double)wb[u], acc_b, 0, 0, 0);
     }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int f = 4 * i + kq;
+    if (f >= n_frames) continue;
+    double* dst = part + ((int64_t)blockIdx.x * n_frames + f) * ne;
+    if (ok_a) dst[e0 + mi] = acc_a[i];
+    if (ok_b) dst[e0 + 16 + mi] = acc_b[i];
+  }
 }
 
 // out[q] (q = f*ne + e) = sum over the row blocks of part: one wave per
@@ -770,10 +778,11 @@ int hp_leadfield_apply_f32(const float* W, const int32_t* row_slot, int64_t n_ro
     return -1;
   }
   hipStream_t st = (hipStream_t)stream;
-  // one wave per 16-electrode tile, LF_WAVES tiles per workgroup
-  const int tiles = (ne + 15) / 16;
-  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (tiles + LF_WAVES - 1) / LF_WAVES),
-                     dim3(LF_WAVES * 64), 0, st, W, row_slot, n_rows, ne, V, ldv, n_frames, rpb,
+  // one wave per 32 electrodes; one workgroup covers all of them (more than
+  // LF_MAXW * 32 electrodes spill into further workgroups along y)
+  const int tiles = (ne + 31) / 32, waves = tiles < LF_MAXW ? tiles : LF_MAXW;
+  hipLaunchKernelGGL(leadfield_partial_kernel, dim3(nb, (tiles + waves - 1) / waves),
+                     dim3(waves * 64), 0, st, W, row_slot, n_rows, ne, V, ldv, n_frames, rpb,
                      part);
   const int rc = check_launch("hp_leadfield_apply_f32/partial");
   if (rc) return rc;
