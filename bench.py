@@ -146,7 +146,7 @@ FORM_SETUP = {  # form -> (scheme, dtype name, multirate rule, engine options)
 
 
 def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, sample_slots,
-                  net=None, log=print):
+                  net=None, log=print, graph_macros=5):
     """One form from t = 0 over the whole beat, ECG every 1 ms (lead field) and a
     Vm sample every 1 ms; bench steps [warm_ms, warm_ms + timed_ms) are timed
     (device synchronised at both ends).  Returns the timing, the activation
@@ -159,7 +159,7 @@ def run_form_beat(form, geo, tree, kw, brick, ecg, beat_ms, warm_ms, timed_ms, s
     eng, meta = build_engine(geo, tree, kwf, dtype=getattr(torch, dname), brick=brick,
                              scheme=scheme, max_sweeps=opts.get("max_sweeps"),
                              cn_solver=opts.get("cn_solver", "bicgstab"))
-    run = SG.SurrogateRunner(eng, net, mode=rule, graph_macros=5) if rule else None
+    run = SG.SurrogateRunner(eng, net, mode=rule, graph_macros=graph_macros) if rule else None
     # Jacobi forms: each 1 ms chunk bounded by the sweeps the last one needed
     # (replayed from its kept state with the full bound if a step stopped there)
     jac = eng.cn is not None and eng.cn_solver == "jacobi"
@@ -507,6 +507,9 @@ def main():
                          "config1: the 64x64x16 FP64 slab in full, GPU and CPU oracle")
     ap.add_argument("--ensemble-size", type=int, default=64)
     ap.add_argument("--graph-chunk", type=int, default=50)
+    ap.add_argument("--graph-macros", type=int, default=5,
+                    help="multirate forms: 0.2 ms macro-steps per captured graph (and per "
+                         "adaptive sweep-bound chunk); must divide 5")
     ap.add_argument("--config1-ms", type=float, default=500.0)
     ap.add_argument("--config1-cpu-steps", type=int, default=0, help="0 = the whole run")
     ap.add_argument("--ecg", type=str, default="leadfield", choices=["leadfield", "none"])
@@ -616,7 +619,7 @@ def main():
         r, at, phi, vm, eng, m = run_form_beat(form, geo, tree, kw, brick, ecg, a.beat_ms,
                                                a.warmup, a.steps, sample_slots,
                                                net=net if form.startswith("surrogate") else None,
-                                               log=log)
+                                               log=log, graph_macros=a.graph_macros)
         if form == "ref64":
             ref = (at, phi, vm)
             r["activated"] = int(np.isfinite(at).sum())
